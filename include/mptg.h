@@ -1,0 +1,169 @@
+/* mptg — MI355X batched PRRT extension path, C ABI.
+ *
+ * The library replaces the two downcalls of the reference's PRRT/PRRT*
+ * extension loop — the nigh nearest-neighbour structure and the Scenario's
+ * valid()/link() motion validation — with batched gfx950 kernels, and adds a
+ * device-resident batched iteration that runs sample -> nearest -> steer ->
+ * valid -> link -> insert for a whole batch of samples per launch sequence.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative MPTG_E_* code; the
+ *     message of the last failure is mptg_last_error(ctx) (per context;
+ *     mptg_last_error(NULL) holds failures of mptg_create).  No exception
+ *     crosses this boundary.
+ *   - Pointers named h_* are host memory borrowed for the duration of the
+ *     call (the call is synchronous).  Pointers named d_* are device memory
+ *     on the context's device; such calls are asynchronous on the context's
+ *     stream (mptg_stream) and the caller synchronises with mptg_sync.
+ *   - States are rows of `dim` doubles.  SE(3) states are
+ *     [qx, qy, qz, qw, tx, ty, tz].
+ *   - A context owns one HIP stream and all device memory it allocates.  It is
+ *     not thread-safe: one host thread drives a context (the reference instead
+ *     runs N worker threads against one shared tree, prrt.hpp:325-346).
+ *
+ * Reference paths below are relative to the reference repository root.
+ */
+#ifndef MPTG_H
+#define MPTG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPTG_OK 0
+#define MPTG_E_ARG (-1)     /* invalid argument */
+#define MPTG_E_HIP (-2)     /* HIP runtime failure */
+#define MPTG_E_STATE (-3)   /* call out of order (no space/tree/scene yet) */
+#define MPTG_E_RANGE (-4)   /* size beyond a documented limit */
+#define MPTG_E_EMPTY (-5)   /* "there are no valid initial states" */
+
+enum mptg_metric {
+    MPTG_METRIC_L2 = 0,   /* L2Space<double, D>       src/mpt/lp_space.hpp */
+    MPTG_METRIC_L1 = 1,   /* L1Space<double, D>       demo/link_manipulator_scenario.hpp:59 */
+    MPTG_METRIC_SE3 = 2   /* SE3Space<double, w_so3>  demo/se3_rigid_body_scenario.hpp:250 */
+};
+
+enum mptg_scene_kind {
+    MPTG_SCENE_NONE = 0,
+    MPTG_SCENE_BOXES = 1, /* closed 2-D boxes, Rect::pointIsValid   demo/shape_hierarchy.hpp:177-182 */
+    MPTG_SCENE_GRID = 2,  /* occupancy grid, PNG2dScenario::valid    demo/png_2d_scenario.hpp:104-110 */
+    MPTG_SCENE_ARM = 3    /* planar arm vs disks                     demo/link_manipulator_scenario.hpp:99-116 */
+};
+
+enum mptg_link_mode {
+    MPTG_LINK_DMV = 0,    /* DiscreteMotionValidator(space, step)   src/mpt/discrete_motion_validator.hpp:71-130 */
+    MPTG_LINK_BISECT = 1  /* recursive midpoints to L-inf < tol     demo/link_manipulator_scenario.hpp:118-138 */
+};
+
+enum mptg_check_order {
+    MPTG_ORDER_REFERENCE = 0, /* the reference's waypoint order */
+    MPTG_ORDER_SURROGATE = 1  /* descending surrogate score (ties: reference order) */
+};
+
+typedef struct mptg_ctx mptg_ctx;
+
+/* ---- lifecycle --------------------------------------------------------- */
+int mptg_create(int device, mptg_ctx** out);
+void mptg_destroy(mptg_ctx* ctx);
+const char* mptg_last_error(const mptg_ctx* ctx);
+int mptg_sync(mptg_ctx* ctx);
+void* mptg_stream(mptg_ctx* ctx); /* the context's hipStream_t */
+
+/* ---- space: replaces the Space given to nigh::Nigh(space)
+ *      src/mpt/impl/prrt/prrt.hpp:122,151.  w0/w1 = SO(3)/R^3 weights for
+ *      MPTG_METRIC_SE3 (ignored otherwise). dim: 2..32 (L1/L2), 7 (SE3). */
+int mptg_space(mptg_ctx* ctx, int metric, int dim, double w0, double w1);
+
+/* ---- tree: replaces nigh::Nigh::insert / size
+ *      src/mpt/impl/prrt/prrt.hpp:177-187,447.  Node ids are insertion order. */
+int mptg_tree_reserve(mptg_ctx* ctx, int64_t capacity);
+int mptg_tree_clear(mptg_ctx* ctx);
+int mptg_tree_insert(mptg_ctx* ctx, const double* h_states, const int32_t* h_parents, int64_t n);
+int64_t mptg_tree_size(const mptg_ctx* ctx);
+int mptg_tree_read(mptg_ctx* ctx, int64_t first, int64_t n, double* h_states, int32_t* h_parents);
+
+/* ---- nearest neighbours: replaces nigh::Nigh::nearest(q)
+ *      (src/mpt/impl/prrt/prrt.hpp:406-409) and nearest(nbh, q, k, r)
+ *      (src/mpt/impl/rrg_rewire_neighbors.hpp:66,127).
+ *      For each query: the k tree nodes with the smallest distance <= radius,
+ *      ascending by (distance, node id); distances are the metric's FP64
+ *      value.  Missing slots: idx -1, dist +inf.  k in 1..32; radius +inf for
+ *      plain kNN. */
+int mptg_knn(mptg_ctx* ctx, const double* h_q, int64_t nq, int k, double radius,
+             int32_t* h_idx, double* h_dist);
+int mptg_knn_dev(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius,
+                 int32_t* d_idx, double* d_dist);
+
+/* ---- scene: the Scenario's valid() as data.  Replaces the per-worker
+ *      Scenario copy (src/mpt/impl/prrt/prrt.hpp:326,343). */
+int mptg_scene_boxes(mptg_ctx* ctx, const double* h_boxes /* n x [x0,y0,x1,y1] */, int n);
+int mptg_scene_grid(mptg_ctx* ctx, const uint8_t* h_cells /* h x w, nonzero = obstacle */,
+                    int width, int height);
+int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double robot_radius,
+                   const double* h_disks /* n x [cx,cy,r] */, int ndisks);
+/* link(): DMV step or bisection tolerance */
+int mptg_set_link(mptg_ctx* ctx, int mode, double param);
+
+/* ---- motion validation: replaces Scenario::valid / Scenario::link
+ *      (src/mpt/impl/prrt/prrt.hpp:439,454-457).  verdict[i] = link(from_i,
+ *      to_i); checks[i] = valid() calls the given order makes before the
+ *      verdict is known (reference order reproduces the reference's count).
+ *      endpoints_known_valid: endpoint checks inside link() are counted but
+ *      not evaluated (the planner has already validated both ends). */
+#define MPTG_CHECKS_OVER_LIMIT 0xffffffffu /* edge beyond the work limits (more than
+                                                 2^24 DMV waypoints or 30 bisection levels):
+                                                 not checked, verdict 0; host-pointer calls
+                                                 return MPTG_E_RANGE */
+int mptg_valid(mptg_ctx* ctx, const double* h_q, int64_t n, uint8_t* h_ok);
+int mptg_link(mptg_ctx* ctx, const double* h_from, const double* h_to, int64_t n, int order,
+              int endpoints_known_valid, uint8_t* h_verdict, uint32_t* h_checks);
+int mptg_link_dev(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                  int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks);
+
+/* ---- batched PRRT: replaces PRRT::solve / Worker::solve / addSample
+ *      (src/mpt/impl/prrt/prrt.hpp:194-201,358-452) with batch-synchronous
+ *      iterations: the samples of one iteration extend the tree as it stood
+ *      at the iteration's start and are appended in sample order.  batch = 1
+ *      is the reference's single-worker loop.  Samples come from the
+ *      project's Philox stream (seed, sample index). */
+typedef struct {
+    uint64_t seed;
+    double range;        /* setRange (+inf = no steering)     prrt.hpp:168-171 */
+    double goal_bias;    /* setGoalBias (default 0.01)        prrt.hpp:118,159-162 */
+    double goal_radius;  /* GoalState radius                  goal_state.hpp:64-69 */
+    int32_t batch;       /* samples per iteration */
+    int32_t order;       /* mptg_check_order for link() */
+} mptg_prrt_params;
+
+typedef struct {
+    int64_t iterations, samples, zero_dist, invalid_states;
+    int64_t link_calls, link_ok, valid_calls, goals, size;
+} mptg_prrt_stats;
+
+int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo, const double* h_hi,
+                   const double* h_goal);
+/* start state(s): tree must be empty before the first call (addStart, prrt.hpp:181-187) */
+int mptg_prrt_add_start(mptg_ctx* ctx, const double* h_q);
+/* Runs iterations until size >= target_size or max_iters iterations. */
+int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_prrt_stats* out);
+int mptg_prrt_stats_get(mptg_ctx* ctx, mptg_prrt_stats* out);
+/* ids of goal nodes in discovery order (foundGoal, prrt.hpp:139-146); returns count via *n */
+int mptg_prrt_goals(mptg_ctx* ctx, int32_t* h_ids, int64_t cap, int64_t* n);
+
+/* ---- diagnostics (no counterpart in the reference): evaluates the device
+ *      build of the project math used by the exact path, for bit-equality
+ *      tests against the host build. */
+enum mptg_diag_op { MPTG_DIAG_SIN = 0, MPTG_DIAG_COS = 1, MPTG_DIAG_ACOS = 2, MPTG_DIAG_SQRT = 3 };
+int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y);
+/* Work a reference-order link() batch performs on the arm scene (untimed,
+ * instrumented copy of the link kernel): h_work = [valid() calls, forward-
+ * kinematics links evaluated, segment-disk tests evaluated].  Used to price a
+ * launch in algorithmic FLOPs. */
+int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
+                   int endpoints_known_valid, uint64_t* h_work);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,321 @@
+"""mpt_amd — MI355X batched PRRT extension path (Python host mirror).
+
+The compute lives in ``libmptg.so`` (HIP kernels for gfx950 behind the C ABI
+of ``include/mptg.h``).  This module is a thin host-side mirror used by tests
+and benchmarks: :class:`Context` wraps the C ABI one-to-one and
+:class:`PRRT` mirrors the public surface of the reference planner
+(``src/mpt/impl/prrt/prrt.hpp:148-316``: ``addStart``, ``solve``,
+``solveFor``, ``solved``, ``solution``, ``size``, ``setRange``,
+``setGoalBias``, ``printStats``).  There is no CPU fallback: every call goes
+through the native library, and a missing library raises at import.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _native
+from ._native import PrrtParams, PrrtStats
+from .scenes import (LINK_BISECT, LINK_DMV, METRIC_L1, METRIC_L2, METRIC_SE3, ORDER_REFERENCE,
+                     ORDER_SURROGATE, SCENE_ARM, SCENE_BOXES, SCENE_GRID, Scene)
+
+__all__ = ["Context", "PRRT", "MptgError", "Scene", "METRIC_L1", "METRIC_L2", "METRIC_SE3", "LINK_DMV",
+           "LINK_BISECT", "ORDER_REFERENCE", "ORDER_SURROGATE"]
+
+class _Lazy:
+    """Resolves the native library on first use, so that ``mpt_amd.build``
+    can be imported before the library exists.  Any compute call without the
+    library raises ImportError."""
+
+    def __getattr__(self, name):
+        return getattr(_native.lib(), name)
+
+
+_lib = _Lazy()
+
+
+class MptgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mptg error {code}: {msg}")
+        self.code = code
+
+
+def _p(arr, ctype):
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def _f64(a, dim=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if dim is not None:
+        a = a.reshape(-1, dim)
+    return a
+
+
+class Context:
+    """One device, one HIP stream, one tree, one scene (``mptg_ctx``)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = _lib.mptg_create(device, C.byref(h))
+        if rc != 0:
+            raise MptgError(rc, _lib.mptg_last_error(None).decode())
+        self._h = h
+        self.dim = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.mptg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc: int):
+        if rc != 0:
+            raise MptgError(rc, _lib.mptg_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return int(_lib.mptg_stream(self._h) or 0)
+
+    def sync(self):
+        self._chk(_lib.mptg_sync(self._h))
+
+    # ---- space / tree
+    def space(self, metric: int, dim: int, w0: float = 1.0, w1: float = 1.0):
+        self._chk(_lib.mptg_space(self._h, metric, dim, w0, w1))
+        self.dim = dim
+
+    def tree_reserve(self, cap: int):
+        self._chk(_lib.mptg_tree_reserve(self._h, cap))
+
+    def tree_clear(self):
+        self._chk(_lib.mptg_tree_clear(self._h))
+
+    def tree_insert(self, states, parents=None):
+        s = _f64(states, self.dim)
+        par = None
+        if parents is not None:
+            par = np.ascontiguousarray(parents, dtype=np.int32)
+        self._chk(_lib.mptg_tree_insert(self._h, _p(s, C.c_double), None if par is None else _p(par, C.c_int32),
+                                        s.shape[0]))
+
+    def tree_size(self) -> int:
+        return int(_lib.mptg_tree_size(self._h))
+
+    def tree_read(self, first: int = 0, n: Optional[int] = None):
+        if n is None:
+            n = self.tree_size() - first
+        s = np.zeros((n, self.dim))
+        par = np.zeros(n, dtype=np.int32)
+        self._chk(_lib.mptg_tree_read(self._h, first, n, _p(s, C.c_double), _p(par, C.c_int32)))
+        return s, par
+
+    def knn(self, q, k: int = 1, radius: float = math.inf):
+        q = _f64(q, self.dim)
+        n = q.shape[0]
+        idx = np.zeros((n, k), dtype=np.int32)
+        dist = np.zeros((n, k))
+        self._chk(_lib.mptg_knn(self._h, _p(q, C.c_double), n, k, radius, _p(idx, C.c_int32), _p(dist, C.c_double)))
+        return idx, dist
+
+    def knn_dev(self, q_ptr: int, nq: int, k: int, radius: float, idx_ptr: int, dist_ptr: int):
+        self._chk(_lib.mptg_knn_dev(self._h, C.c_void_p(q_ptr), nq, k, radius, C.c_void_p(idx_ptr),
+                                    C.c_void_p(dist_ptr)))
+
+    # ---- scene
+    def scene(self, sc: Scene):
+        """Uploads a Scene: space, valid() model and link() form."""
+        self.space(sc.metric, sc.dim, sc.w_so3, sc.w_l2)
+        if sc.kind == SCENE_BOXES:
+            b = _f64(sc.boxes, 4)
+            self._chk(_lib.mptg_scene_boxes(self._h, _p(b, C.c_double), b.shape[0]))
+        elif sc.kind == SCENE_GRID:
+            g = np.ascontiguousarray(sc.grid, dtype=np.uint8)
+            self._chk(_lib.mptg_scene_grid(self._h, _p(g, C.c_uint8), g.shape[1], g.shape[0]))
+        elif sc.kind == SCENE_ARM:
+            ln = _f64(sc.lengths)
+            dk = _f64(sc.disks, 3)
+            self._chk(_lib.mptg_scene_arm(self._h, _p(ln, C.c_double), ln.shape[0], sc.robot_radius,
+                                          _p(dk, C.c_double), dk.shape[0]))
+        else:
+            raise ValueError(f"unknown scene kind {sc.kind}")
+        self._chk(_lib.mptg_set_link(self._h, sc.link_mode, sc.link_param))
+
+    def valid(self, q):
+        q = _f64(q, self.dim)
+        ok = np.zeros(q.shape[0], dtype=np.uint8)
+        self._chk(_lib.mptg_valid(self._h, _p(q, C.c_double), q.shape[0], _p(ok, C.c_uint8)))
+        return ok.astype(bool)
+
+    def link(self, a, b, order: int = ORDER_REFERENCE, endpoints_known_valid: bool = False):
+        a = _f64(a, self.dim)
+        b = _f64(b, self.dim)
+        n = a.shape[0]
+        v = np.zeros(n, dtype=np.uint8)
+        c = np.zeros(n, dtype=np.uint32)
+        self._chk(_lib.mptg_link(self._h, _p(a, C.c_double), _p(b, C.c_double), n, order,
+                                 1 if endpoints_known_valid else 0, _p(v, C.c_uint8), _p(c, C.c_uint32)))
+        return v.astype(bool), c
+
+    def link_dev(self, a_ptr: int, b_ptr: int, n: int, order: int, known_valid: bool, v_ptr: int, c_ptr: int):
+        self._chk(_lib.mptg_link_dev(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, order,
+                                     1 if known_valid else 0, C.c_void_p(v_ptr), C.c_void_p(c_ptr)))
+
+    def diag_math(self, op: int, x) -> np.ndarray:
+        """op: 0 sin, 1 cos, 2 acos, 3 sqrt — the device build of the project math."""
+        x = _f64(x).ravel()
+        y = np.zeros_like(x)
+        self._chk(_lib.mptg_diag_math(self._h, op, _p(x, C.c_double), x.size, _p(y, C.c_double)))
+        return y
+
+    def link_work(self, a_ptr: int, b_ptr: int, n: int, known_valid: bool = False):
+        """[valid() calls, FK links, segment-disk tests] of a reference-order
+        link() batch on device buffers (arm scene)."""
+        w = (C.c_uint64 * 3)()
+        self._chk(_lib.mptg_link_work(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, 1 if known_valid else 0, w))
+        return [int(x) for x in w]
+
+    # ---- planner
+    def prrt_init(self, sc: Scene, seed: int, batch: int, order: int = ORDER_REFERENCE, range_: Optional[float] = None,
+                  goal_bias: Optional[float] = None):
+        p = PrrtParams(seed=seed, range=sc.range if range_ is None else range_,
+                       goal_bias=sc.goal_bias if goal_bias is None else goal_bias, goal_radius=sc.goal_radius,
+                       batch=batch, order=order)
+        lo, hi, g = _f64(sc.lo), _f64(sc.hi), _f64(sc.goal)
+        self._chk(_lib.mptg_prrt_init(self._h, C.byref(p), _p(lo, C.c_double), _p(hi, C.c_double), _p(g, C.c_double)))
+
+    def prrt_add_start(self, q):
+        q = _f64(q)
+        self._chk(_lib.mptg_prrt_add_start(self._h, _p(q, C.c_double)))
+
+    def prrt_run(self, target_size: int, max_iters: int = 1 << 40) -> dict:
+        st = PrrtStats()
+        self._chk(_lib.mptg_prrt_run(self._h, target_size, max_iters, C.byref(st)))
+        return st.as_dict()
+
+    def prrt_stats(self) -> dict:
+        st = PrrtStats()
+        self._chk(_lib.mptg_prrt_stats_get(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def prrt_goals(self) -> np.ndarray:
+        n = C.c_int64()
+        buf = np.zeros(4096, dtype=np.int32)
+        self._chk(_lib.mptg_prrt_goals(self._h, _p(buf, C.c_int32), buf.shape[0], C.byref(n)))
+        return buf[: min(n.value, buf.shape[0])].copy()
+
+
+class PRRT:
+    """Host mirror of ``mpt::Planner<Scenario, PRRT<...>>`` on the batched GPU
+    path (reference surface: ``src/mpt/impl/prrt/prrt.hpp:148-316``)."""
+
+    def __init__(self, scenario: Scene, seed: int = 0, batch: int = 8192, order: int = ORDER_REFERENCE,
+                 device: int = 0, ctx: Optional[Context] = None):
+        self.scenario = scenario
+        self.ctx = ctx or Context(device)
+        self.ctx.scene(scenario)
+        self._seed, self._batch, self._order = seed, batch, order
+        self._range = scenario.range
+        self._bias = scenario.goal_bias
+        self._starts = []
+        self._init()
+
+    def _init(self):
+        self.ctx.tree_clear()
+        self.ctx.prrt_init(self.scenario, self._seed, self._batch, self._order, self._range, self._bias)
+
+    # setters mirror prrt.hpp:159-175
+    def setRange(self, r: float):
+        if not r > 0:
+            raise ValueError("range must be > 0")
+        self._range = r
+        self.ctx.prrt_init(self.scenario, self._seed, self._batch, self._order, self._range, self._bias)
+
+    def getRange(self) -> float:
+        return self._range
+
+    def setGoalBias(self, b: float):
+        if not 0 <= b <= 1:
+            raise ValueError("goal bias must be in [0,1]")
+        self._bias = b
+        self.ctx.prrt_init(self.scenario, self._seed, self._batch, self._order, self._range, self._bias)
+
+    def getGoalBias(self) -> float:
+        return self._bias
+
+    def addStart(self, q):
+        self.ctx.prrt_add_start(q)
+
+    def size(self) -> int:
+        return self.ctx.tree_size()
+
+    def solve(self, done: Callable[[], bool], iters_per_check: int = 1):
+        """Runs iterations until done() is true (checked between iterations;
+        prrt.hpp:194-201 throws on an empty tree, so does this)."""
+        if self.size() == 0:
+            raise RuntimeError("there are no valid initial states")
+        while not done():
+            self.ctx.prrt_run(1 << 31 - 1, iters_per_check)
+
+    def solveFor(self, seconds: float):
+        t_end = time.perf_counter() + seconds
+        self.solve(lambda: time.perf_counter() >= t_end)
+
+    def solveUntilSize(self, n: int, max_iters: int = 1 << 40) -> dict:
+        if self.size() == 0:
+            raise RuntimeError("there are no valid initial states")
+        return self.ctx.prrt_run(n, max_iters)
+
+    def solved(self) -> bool:
+        return self.ctx.prrt_stats()["goals"] > 0
+
+    def _path_to(self, node: int, states, parents):
+        path = []
+        while node >= 0:
+            path.append(states[node])
+            node = int(parents[node])
+        return path[::-1]
+
+    def solution(self):
+        """Best (lowest-cost) goal path as a list of states (prrt.hpp:221-274)."""
+        goals = self.ctx.prrt_goals()
+        if goals.size == 0:
+            return []
+        states, parents = self.ctx.tree_read()
+        best, best_cost = None, math.inf
+        for g in goals:
+            path = self._path_to(int(g), states, parents)
+            cost = sum(self._dist(path[i], path[i + 1]) for i in range(len(path) - 1))
+            if best is None or cost < best_cost:
+                best, best_cost = path, cost
+        return best
+
+    def _dist(self, a, b) -> float:
+        if self.scenario.metric == METRIC_L1:
+            return float(np.abs(a - b).sum())
+        return float(np.sqrt(((a - b) ** 2).sum()))
+
+    def stats(self) -> dict:
+        return self.ctx.prrt_stats()
+
+    def printStats(self):
+        s = self.stats()
+        print(f"nodes in graph: {self.size()}")
+        print(f"solutions: {s['goals']}")
+        for k, v in s.items():
+            print(f"{k}: {v}")

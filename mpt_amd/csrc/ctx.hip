@@ -1,0 +1,409 @@
+// Context lifecycle, space/tree/scene state and the host-pointer entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace mptg {
+
+thread_local std::string g_create_err;
+void prrt_free(mptg_ctx* ctx);
+
+int ensure(mptg_ctx* ctx, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return MPTG_OK;
+    if (b.p) {
+        MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        MPTG_HIP(ctx, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    MPTG_HIP(ctx, hipMalloc(&b.p, want));
+    b.bytes = want;
+    return MPTG_OK;
+}
+
+int ensure_pinned(mptg_ctx* ctx, size_t bytes) {
+    if (ctx->pinned_bytes >= bytes) return MPTG_OK;
+    if (ctx->pinned) {
+        MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        MPTG_HIP(ctx, hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 1 << 16);
+    MPTG_HIP(ctx, hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+    ctx->pinned_bytes = want;
+    return MPTG_OK;
+}
+
+int check_device_ready(mptg_ctx* ctx) {
+    if (!ctx) return fail(nullptr, MPTG_E_ARG, "null context");
+    MPTG_HIP(ctx, hipSetDevice(ctx->device));
+    return MPTG_OK;
+}
+
+static int grow_tree(mptg_ctx* ctx, int64_t cap) {
+    if (cap <= ctx->cap) return MPTG_OK;
+    int d = ctx->dim, dp = pitch32(d);
+    double* x64 = nullptr;
+    float* x32 = nullptr;
+    int32_t* par = nullptr;
+    MPTG_HIP(ctx, hipMalloc(&x64, sizeof(double) * d * cap));
+    MPTG_HIP(ctx, hipMalloc(&x32, sizeof(float) * dp * cap));
+    MPTG_HIP(ctx, hipMalloc(&par, sizeof(int32_t) * cap));
+    // zero the FP32 pad columns once so every row is fully defined
+    MPTG_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * dp * cap, ctx->stream));
+    if (ctx->size > 0) {
+        MPTG_HIP(ctx, hipMemcpyAsync(x64, ctx->x64, sizeof(double) * d * ctx->size,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+        MPTG_HIP(ctx, hipMemcpyAsync(x32, ctx->x32, sizeof(float) * dp * ctx->size,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+        MPTG_HIP(ctx, hipMemcpyAsync(par, ctx->parent, sizeof(int32_t) * ctx->size,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->x64) {
+        MPTG_HIP(ctx, hipFree(ctx->x64));
+        MPTG_HIP(ctx, hipFree(ctx->x32));
+        MPTG_HIP(ctx, hipFree(ctx->parent));
+    }
+    ctx->x64 = x64;
+    ctx->x32 = x32;
+    ctx->parent = par;
+    ctx->cap = cap;
+    return MPTG_OK;
+}
+
+int tree_reserve(mptg_ctx* ctx, int64_t cap) {
+    if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called before the tree is used");
+    if (cap < 0 || cap > (int64_t)INT32_MAX) return fail(ctx, MPTG_E_RANGE, "tree capacity must be < 2^31");
+    if (cap <= ctx->cap) return MPTG_OK;
+    int64_t c = std::max<int64_t>(cap, std::min<int64_t>((int64_t)INT32_MAX, ctx->cap * 2));
+    return grow_tree(ctx, c);
+}
+
+}  // namespace mptg
+
+using namespace mptg;
+
+extern "C" {
+
+int mptg_create(int device, mptg_ctx** out) {
+    if (!out) return fail(nullptr, MPTG_E_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(nullptr, MPTG_E_HIP, std::string("no HIP device: ") + hipGetErrorString(e));
+    if (device < 0 || device >= n) return fail(nullptr, MPTG_E_ARG, "device index out of range");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, MPTG_E_HIP, hipGetErrorString(e));
+    mptg_ctx* ctx = new mptg_ctx();
+    ctx->device = device;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, MPTG_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cus = prop.multiProcessorCount;
+    if (ctx->cus <= 0) ctx->cus = 256;
+    *out = ctx;
+    return MPTG_OK;
+}
+
+void mptg_destroy(mptg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->x64);
+    (void)hipFree(ctx->x32);
+    (void)hipFree(ctx->parent);
+    (void)hipFree(ctx->scene.boxes);
+    (void)hipFree(ctx->scene.grid);
+    (void)hipFree(ctx->scene.lengths);
+    (void)hipFree(ctx->scene.disks);
+    for (auto& b : ctx->scratch) (void)hipFree(b.p);
+    prrt_free(ctx);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* mptg_last_error(const mptg_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int mptg_sync(mptg_ctx* ctx) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+void* mptg_stream(mptg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mptg_space(mptg_ctx* ctx, int metric, int dim, double w0, double w1) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (metric != MPTG_METRIC_L2 && metric != MPTG_METRIC_L1 && metric != MPTG_METRIC_SE3)
+        return fail(ctx, MPTG_E_ARG, "unknown metric");
+    if (metric == MPTG_METRIC_SE3 && dim != 7) return fail(ctx, MPTG_E_ARG, "SE3 states have 7 coordinates");
+    if (dim < 1 || dim > kMaxDim) return fail(ctx, MPTG_E_ARG, "dimension must be in 1..32");
+    if (metric == MPTG_METRIC_SE3 && !(w0 > 0 && w1 > 0)) return fail(ctx, MPTG_E_ARG, "SE3 weights must be > 0");
+    if (ctx->size > 0 && (dim != ctx->dim || metric != ctx->metric))
+        return fail(ctx, MPTG_E_STATE, "cannot change the space of a non-empty tree");
+    if (dim != ctx->dim && ctx->x64) {
+        MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        MPTG_HIP(ctx, hipFree(ctx->x64));
+        MPTG_HIP(ctx, hipFree(ctx->x32));
+        MPTG_HIP(ctx, hipFree(ctx->parent));
+        ctx->x64 = nullptr;
+        ctx->x32 = nullptr;
+        ctx->parent = nullptr;
+        ctx->cap = 0;
+    }
+    ctx->metric = metric;
+    ctx->dim = dim;
+    ctx->w_so3 = metric == MPTG_METRIC_SE3 ? w0 : 1.0;
+    ctx->w_l2 = metric == MPTG_METRIC_SE3 ? w1 : 1.0;
+    return MPTG_OK;
+}
+
+int mptg_tree_reserve(mptg_ctx* ctx, int64_t capacity) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    return tree_reserve(ctx, capacity);
+}
+
+int mptg_tree_clear(mptg_ctx* ctx) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->size = 0;
+    ctx->coord_bound = 0;
+    ctx->prrt.iteration = 0;
+    ctx->prrt.goals = 0;
+    ctx->prrt.goal_ids.clear();
+    ctx->prrt.stats = mptg_prrt_stats{};
+    ctx->prrt.counters_valid = false;
+    return MPTG_OK;
+}
+
+int mptg_tree_insert(mptg_ctx* ctx, const double* h_states, const int32_t* h_parents, int64_t n) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !h_states)) return fail(ctx, MPTG_E_ARG, "bad states");
+    if (n == 0) return MPTG_OK;
+    rc = tree_reserve(ctx, ctx->size + n);
+    if (rc) return rc;
+    int d = ctx->dim, dp = pitch32(d);
+    std::vector<float> x32((size_t)n * dp, 0.0f);
+    double bound = ctx->coord_bound;
+    for (int64_t i = 0; i < n; ++i)
+        for (int j = 0; j < d; ++j) {
+            double v = h_states[i * d + j];
+            if (!std::isfinite(v)) return fail(ctx, MPTG_E_ARG, "non-finite state coordinate");
+            x32[i * dp + j] = (float)v;
+            bound = std::max(bound, std::fabs(v));
+        }
+    std::vector<int32_t> par((size_t)n, -1);
+    if (h_parents)
+        for (int64_t i = 0; i < n; ++i) {
+            if (h_parents[i] < -1 || h_parents[i] >= ctx->size + i)
+                return fail(ctx, MPTG_E_ARG, "parent id must be -1 or an earlier node");
+            par[i] = h_parents[i];
+        }
+    int64_t s = ctx->size;
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->x64 + s * d, h_states, sizeof(double) * d * n, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->x32 + s * dp, x32.data(), sizeof(float) * dp * n, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->parent + s, par.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->size += n;
+    ctx->coord_bound = bound;
+    return MPTG_OK;
+}
+
+int64_t mptg_tree_size(const mptg_ctx* ctx) { return ctx ? ctx->size : -1; }
+
+int mptg_tree_read(mptg_ctx* ctx, int64_t first, int64_t n, double* h_states, int32_t* h_parents) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (first < 0 || n < 0 || first + n > ctx->size) return fail(ctx, MPTG_E_RANGE, "read beyond tree size");
+    if (n == 0) return MPTG_OK;
+    int d = ctx->dim;
+    if (h_states)
+        MPTG_HIP(ctx, hipMemcpyAsync(h_states, ctx->x64 + first * d, sizeof(double) * d * n,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    if (h_parents)
+        MPTG_HIP(ctx, hipMemcpyAsync(h_parents, ctx->parent + first, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+// ---- scenes ---------------------------------------------------------------
+static int upload(mptg_ctx* ctx, void** dst, const void* src, size_t bytes) {
+    if (*dst) {
+        MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        MPTG_HIP(ctx, hipFree(*dst));
+        *dst = nullptr;
+    }
+    if (bytes == 0) return MPTG_OK;
+    MPTG_HIP(ctx, hipMalloc(dst, bytes));
+    MPTG_HIP(ctx, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_scene_boxes(mptg_ctx* ctx, const double* h_boxes, int n) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !h_boxes)) return fail(ctx, MPTG_E_ARG, "bad boxes");
+    if (n > 4096) return fail(ctx, MPTG_E_RANGE, "at most 4096 boxes");
+    rc = upload(ctx, (void**)&ctx->scene.boxes, h_boxes, sizeof(double) * 4 * n);
+    if (rc) return rc;
+    ctx->scene.kind = MPTG_SCENE_BOXES;
+    ctx->scene.nboxes = n;
+    return MPTG_OK;
+}
+
+int mptg_scene_grid(mptg_ctx* ctx, const uint8_t* h_cells, int width, int height) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (width <= 0 || height <= 0 || !h_cells) return fail(ctx, MPTG_E_ARG, "bad grid");
+    if ((int64_t)width * height > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "grid larger than 2^31 cells");
+    rc = upload(ctx, (void**)&ctx->scene.grid, h_cells, (size_t)width * height);
+    if (rc) return rc;
+    ctx->scene.kind = MPTG_SCENE_GRID;
+    ctx->scene.width = width;
+    ctx->scene.height = height;
+    return MPTG_OK;
+}
+
+int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double robot_radius,
+                   const double* h_disks, int ndisks) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (nlinks < 1 || nlinks > kMaxDim || !h_lengths) return fail(ctx, MPTG_E_ARG, "arm needs 1..32 links");
+    if (ndisks < 0 || ndisks > 65536 || (ndisks > 0 && !h_disks)) return fail(ctx, MPTG_E_ARG, "bad disks");
+    std::vector<double> dk((size_t)ndisks * 3);
+    for (int i = 0; i < ndisks; ++i) {
+        dk[3 * i + 0] = h_disks[3 * i + 0];
+        dk[3 * i + 1] = h_disks[3 * i + 1];
+        double rr = h_disks[3 * i + 2] + robot_radius;  // (radius_ + robotRadius), squared as the reference
+        dk[3 * i + 2] = rr * rr;
+    }
+    rc = upload(ctx, (void**)&ctx->scene.lengths, h_lengths, sizeof(double) * nlinks);
+    if (rc) return rc;
+    rc = upload(ctx, (void**)&ctx->scene.disks, dk.data(), sizeof(double) * 3 * ndisks);
+    if (rc) return rc;
+    ctx->scene.kind = MPTG_SCENE_ARM;
+    ctx->scene.nlinks = nlinks;
+    ctx->scene.ndisks = ndisks;
+    ctx->scene.robot_radius = robot_radius;
+    return MPTG_OK;
+}
+
+int mptg_set_link(mptg_ctx* ctx, int mode, double param) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (mode != MPTG_LINK_DMV && mode != MPTG_LINK_BISECT) return fail(ctx, MPTG_E_ARG, "unknown link mode");
+    if (!(param > 0) || !std::isfinite(param)) return fail(ctx, MPTG_E_ARG, "link parameter must be > 0");
+    ctx->link_mode = mode;
+    ctx->link_param = param;
+    return MPTG_OK;
+}
+
+// ---- host-pointer wrappers --------------------------------------------------
+int mptg_knn(mptg_ctx* ctx, const double* h_q, int64_t nq, int k, double radius, int32_t* h_idx,
+             double* h_dist) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (nq < 0 || (nq > 0 && (!h_q || !h_idx || !h_dist))) return fail(ctx, MPTG_E_ARG, "bad knn buffers");
+    if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..32");
+    if (nq == 0) return MPTG_OK;
+    int d = ctx->dim;
+    size_t qb = sizeof(double) * d * nq, ib = sizeof(int32_t) * k * nq, db = sizeof(double) * k * nq;
+    if ((rc = ensure(ctx, ctx->scratch[0], qb))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[1], ib))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[2], db))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scratch[0].p, h_q, qb, hipMemcpyHostToDevice, ctx->stream));
+    rc = launch_knn(ctx, (const double*)ctx->scratch[0].p, nq, k, radius, (int32_t*)ctx->scratch[1].p,
+                    (double*)ctx->scratch[2].p, ctx->size);
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(h_idx, ctx->scratch[1].p, ib, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(h_dist, ctx->scratch[2].p, db, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_knn_dev(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+                 double* d_dist) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (nq < 0 || (nq > 0 && (!d_q || !d_idx || !d_dist))) return fail(ctx, MPTG_E_ARG, "bad knn buffers");
+    if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..32");
+    if (nq == 0) return MPTG_OK;
+    return launch_knn(ctx, d_q, nq, k, radius, d_idx, d_dist, ctx->size);
+}
+
+int mptg_valid(mptg_ctx* ctx, const double* h_q, int64_t n, uint8_t* h_ok) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!h_q || !h_ok))) return fail(ctx, MPTG_E_ARG, "bad valid buffers");
+    if (n == 0) return MPTG_OK;
+    size_t qb = sizeof(double) * ctx->dim * n;
+    if ((rc = ensure(ctx, ctx->scratch[0], qb))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[1], n))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scratch[0].p, h_q, qb, hipMemcpyHostToDevice, ctx->stream));
+    rc = launch_valid(ctx, (const double*)ctx->scratch[0].p, n, (uint8_t*)ctx->scratch[1].p);
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(h_ok, ctx->scratch[1].p, n, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_link(mptg_ctx* ctx, const double* h_from, const double* h_to, int64_t n, int order,
+              int endpoints_known_valid, uint8_t* h_verdict, uint32_t* h_checks) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!h_from || !h_to || !h_verdict || !h_checks)))
+        return fail(ctx, MPTG_E_ARG, "bad link buffers");
+    if (n == 0) return MPTG_OK;
+    size_t qb = sizeof(double) * ctx->dim * n;
+    if ((rc = ensure(ctx, ctx->scratch[0], qb))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[1], qb))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[2], n))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[3], sizeof(uint32_t) * n))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scratch[0].p, h_from, qb, hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scratch[1].p, h_to, qb, hipMemcpyHostToDevice, ctx->stream));
+    rc = launch_link(ctx, (const double*)ctx->scratch[0].p, (const double*)ctx->scratch[1].p, n, order,
+                     endpoints_known_valid, (uint8_t*)ctx->scratch[2].p, (uint32_t*)ctx->scratch[3].p);
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(h_verdict, ctx->scratch[2].p, n, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(h_checks, ctx->scratch[3].p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < n; ++i)
+        if (h_checks[i] == MPTG_CHECKS_OVER_LIMIT)
+            return fail(ctx, MPTG_E_RANGE, "an edge exceeds the link() work limits (2^24 waypoints / 30 levels)");
+    return MPTG_OK;
+}
+
+int mptg_link_dev(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                  int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!d_from || !d_to || !d_verdict || !d_checks)))
+        return fail(ctx, MPTG_E_ARG, "bad link buffers");
+    if (n == 0) return MPTG_OK;
+    return launch_link(ctx, d_from, d_to, n, order, endpoints_known_valid, d_verdict, d_checks);
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// Diagnostic entry point: evaluates the device's project math on a batch of
+// arguments so tests can compare it bit-for-bit with the host build.
+#include "internal.hpp"
+#include "pmath.hpp"
+
+namespace mptg {
+namespace {
+
+__global__ void diag_math_kernel(int op, const double* __restrict__ x, int64_t n, double* __restrict__ y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = x[i];
+    double r;
+    switch (op) {
+        case MPTG_DIAG_SIN: r = pm::sin(v); break;
+        case MPTG_DIAG_COS: r = pm::cos(v); break;
+        case MPTG_DIAG_ACOS: r = pm::acos(v); break;
+        default: r = __builtin_sqrt(v); break;
+    }
+    y[i] = r;
+}
+
+}  // namespace
+}  // namespace mptg
+
+using namespace mptg;
+
+namespace mptg {
+int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int known_valid,
+                     unsigned long long* d_work);
+}
+
+extern "C" int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
+                              int endpoints_known_valid, uint64_t* h_work) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n < 0 || !h_work || (n > 0 && (!d_from || !d_to))) return fail(ctx, MPTG_E_ARG, "bad buffers");
+    if ((rc = ensure(ctx, ctx->scratch[3], 3 * sizeof(uint64_t)))) return rc;
+    unsigned long long* w = (unsigned long long*)ctx->scratch[3].p;
+    if ((rc = launch_link_work(ctx, d_from, d_to, n, endpoints_known_valid, w))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(h_work, w, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+extern "C" int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (op < MPTG_DIAG_SIN || op > MPTG_DIAG_SQRT) return fail(ctx, MPTG_E_ARG, "unknown diagnostic op");
+    if (n < 0 || (n > 0 && (!h_x || !h_y))) return fail(ctx, MPTG_E_ARG, "bad buffers");
+    if (n == 0) return MPTG_OK;
+    if ((rc = ensure(ctx, ctx->scratch[0], sizeof(double) * n))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[1], sizeof(double) * n))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scratch[0].p, h_x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(diag_math_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, op,
+                       (const double*)ctx->scratch[0].p, n, (double*)ctx->scratch[1].p);
+    MPTG_HIP(ctx, hipGetLastError());
+    MPTG_HIP(ctx, hipMemcpyAsync(h_y, ctx->scratch[1].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}

@@ -1,0 +1,121 @@
+// Library-internal context, error plumbing and device-buffer helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mptg.h"
+#include "hd.hpp"
+
+namespace mptg {
+
+constexpr int kMaxDim = 32;
+constexpr int kMaxK = 32;
+
+// Device buffer owned by a context.  Grows, never shrinks.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct SceneDev {
+    int kind = MPTG_SCENE_NONE;
+    // boxes
+    double* boxes = nullptr;
+    int nboxes = 0;
+    // grid
+    uint8_t* grid = nullptr;
+    int width = 0, height = 0;
+    // arm
+    double* lengths = nullptr;
+    int nlinks = 0;
+    double robot_radius = 0;
+    double* disks = nullptr;  // [cx, cy, (r+R)^2] per disk
+    int ndisks = 0;
+};
+
+struct PrrtState {
+    bool init = false;
+    bool counters_valid = false;  // device counters mirror the host state
+    mptg_prrt_stats stats_base{};  // host stats when the device counters were reset
+    DevBuf q, near, ndist, from, to, status, mask, verdict, checks, counters, goal_buf;
+    mptg_prrt_params p{};
+    double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
+    int64_t iteration = 0;       // next iteration index
+    int64_t goals = 0;           // goal nodes found so far
+    mptg_prrt_stats stats{};
+    std::vector<int32_t> goal_ids;
+};
+
+}  // namespace mptg
+
+struct mptg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cus = 0;
+
+    // space
+    int metric = -1;
+    int dim = 0;
+    double w_so3 = 1, w_l2 = 1;
+
+    // tree (node id = insertion order)
+    int64_t size = 0, cap = 0;
+    double* x64 = nullptr;   // [cap][dim] AoS, FP64 (exact coordinates)
+    float* x32 = nullptr;    // [cap][dp]  AoS, FP32 shadow for the prefilter
+    int32_t* parent = nullptr;
+    double coord_bound = 0;  // max |coordinate| over inserted nodes and bounds
+
+    // scene
+    mptg::SceneDev scene;
+    int link_mode = MPTG_LINK_DMV;
+    double link_param = 1.0;
+
+    // scratch (grow-only)
+    mptg::DevBuf scratch[8];
+    // pinned host staging
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+
+    mptg::PrrtState prrt;
+};
+
+namespace mptg {
+
+extern thread_local std::string g_create_err;
+
+inline int fail(mptg_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define MPTG_HIP(ctx, call)                                                                \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::mptg::fail((ctx), MPTG_E_HIP,                                         \
+                                std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// FP32 shadow row pitch (floats) for a dimension.
+inline int pitch32(int dim) { return dim <= 2 ? 2 : (dim <= 4 ? 4 : (dim <= 8 ? 8 : ((dim + 3) / 4) * 4)); }
+
+int ensure(mptg_ctx* ctx, DevBuf& b, size_t bytes);
+int ensure_pinned(mptg_ctx* ctx, size_t bytes);
+int check_device_ready(mptg_ctx* ctx);
+
+// kernels launched from other translation units
+int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+               double* d_dist, int64_t tree_n);
+int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                int known_valid, uint8_t* d_verdict, uint32_t* d_checks);
+int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                       int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
+int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
+
+}  // namespace mptg

@@ -1,0 +1,359 @@
+// Exact k-nearest-neighbour search over the tree (the role of nigh::Nigh::nearest,
+// src/mpt/impl/prrt/prrt.hpp:406-409 and src/mpt/impl/rrg_rewire_neighbors.hpp:66,127).
+//
+// Result contract: for each query the k tree nodes with the smallest FP64
+// metric distance (<= radius), ordered by (distance, node id).  Distances are
+// computed with the same left-to-right FP64 expression as the CPU oracle, so
+// indices and distances are bit-exact.
+//
+// Kernel shape (gfx950, wave64):
+//   * one lane owns one query; a 256-thread block holds 4 waves = 256 queries;
+//   * blockIdx.y splits the tree into chunks so that a launch has enough
+//     waves to fill 256 CUs even for a few thousand queries;
+//   * every lane of a wave visits the same tree node at the same time, so the
+//     node's FP32 row is a wave-uniform (scalar) load broadcast to all lanes;
+//   * FP32 prefilter: d32 is compared against the lane's current k-th FP64
+//     distance plus a proven rounding margin; only lanes that pass evaluate
+//     the FP64 distance and insert into a register-resident sorted list;
+//   * partial lists of the chunks are merged by a second kernel.
+//
+// Prefilter soundness.  With u = 2^-24, S = max|q| + max|p| and
+// eps = 2.01 u S, each FP32 coordinate difference satisfies
+// |t_i - (q_i - p_i)| <= eps.  For L2 (squared distances d):
+//   |d32 - d| <= 2 eps sqrt(D d) + D eps^2 + g_D (d + 2 eps sqrt(D d) + D eps^2),
+//   g_D = D u / (1 - D u)
+// and for L1: |d32 - d| <= D eps + g_D (d + D eps).  A node can enter the
+// list only if d <= T (T = current k-th distance, squared for L2), hence only
+// if d32 <= T + eta(T).  The kernels use 2 * eta plus a relative 2^-40 for
+// the sqrt rounding of list entries.
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace mptg {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ bool better(double a, int ia, double b, int ib) {
+    return a < b || (a == b && ia < ib);
+}
+
+template <int K>
+struct TopK {
+    double d[K];
+    int32_t i[K];
+
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            d[j] = __builtin_inf();
+            i[j] = INT_MAX;
+        }
+    }
+    // Inserts (nd, ni) if it beats the last entry; keeps ascending order.
+    __device__ __forceinline__ void insert(double nd, int32_t ni) {
+        if (!better(nd, ni, d[K - 1], i[K - 1])) return;
+        bool placed = false;
+#pragma unroll
+        for (int j = K - 1; j >= 1; --j) {
+            bool mv = better(nd, ni, d[j - 1], i[j - 1]);
+            double dj = mv ? d[j - 1] : (placed ? d[j] : nd);
+            int32_t ij = mv ? i[j - 1] : (placed ? i[j] : ni);
+            placed = placed || !mv;
+            d[j] = dj;
+            i[j] = ij;
+        }
+        if (!placed) {
+            d[0] = nd;
+            i[0] = ni;
+        }
+    }
+};
+
+// FP64 metric value, left-to-right, no contraction (matches the oracle).
+template <int D, int METRIC>
+__device__ __forceinline__ double exact_dist(const double* __restrict__ a, const double* __restrict__ b) {
+    double s = 0.0;
+    if constexpr (METRIC == MPTG_METRIC_L1) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) s = s + __builtin_fabs(a[i] - b[i]);
+        return s;
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double t = a[i] - b[i];
+            s = s + t * t;
+        }
+        return __builtin_sqrt(s);
+    }
+}
+
+// Prefilter threshold in the prefilter's units (squared distance for L2,
+// distance for L1) from the list's k-th distance `dk` (metric value).
+template <int D, int METRIC>
+__device__ __forceinline__ float prefilter_threshold(double dk, double S) {
+    if (!(dk < __builtin_inf())) return __builtin_inff();
+    const double u = 0x1.0p-24;
+    const double eps = 2.01 * u * S;
+    const double g = D * u / (1.0 - D * u);
+    double T, eta;
+    if constexpr (METRIC == MPTG_METRIC_L1) {
+        T = dk;
+        eta = D * eps + g * (T + D * eps);
+    } else {
+        T = dk * dk;
+        double r = 2.0 * eps * __builtin_sqrt(D * T) + D * eps * eps;
+        eta = r + g * (T + r);
+    }
+    double thr = T * (1.0 + 0x1.0p-40) + 2.0 * eta + 1e-300;
+    float f = (float)thr;
+    // round up (thr > 0) so that the FP32 comparison never undercuts the FP64 bound
+    return (double)f < thr ? float_of(bits_of(f) + 1u) : f;
+}
+
+template <int D, int METRIC, int K>
+__global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32, const double* __restrict__ x64,
+                                                   int64_t n, int64_t chunk, const double* __restrict__ q,
+                                                   int64_t nq, double radius, double pbound,
+                                                   double* __restrict__ out_d, int32_t* __restrict__ out_i) {
+    constexpr int DP = (D <= 2) ? 2 : (D <= 4 ? 4 : (D <= 8 ? 8 : ((D + 3) / 4) * 4));
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = qi < nq;
+    const int64_t qc = active ? qi : 0;
+    const int64_t p0 = (int64_t)blockIdx.y * chunk;
+    const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
+
+    float q32[D];
+    double mq = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double v = q[qc * D + i];
+        q32[i] = (float)v;
+        mq = __builtin_fmax(mq, __builtin_fabs(v));
+    }
+    const double S = mq + pbound;
+    TopK<K> top;
+    top.init();
+    float rthr = prefilter_threshold<D, METRIC>(radius, S);
+    float thr = rthr;
+    if (!active) thr = -1.0f;  // inactive lanes never take the slow path
+
+    for (int64_t p = p0; p < p1; ++p) {
+        const float* row = x32 + p * DP;
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            float t = q32[i] - row[i];
+            if constexpr (METRIC == MPTG_METRIC_L1) acc = acc + __builtin_fabsf(t);
+            else acc = __builtin_fmaf(t, t, acc);
+        }
+        if (acc <= thr) {
+            double dq = exact_dist<D, METRIC>(q + qc * D, x64 + p * D);
+            if (dq <= radius) {
+                top.insert(dq, (int32_t)p);
+                float t = prefilter_threshold<D, METRIC>(top.d[K - 1], S);
+                thr = t < rthr ? t : rthr;
+            }
+        }
+    }
+    if (!active) return;
+    const int64_t base = ((int64_t)blockIdx.y * nq + qi) * K;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        out_d[base + j] = top.d[j];
+        out_i[base + j] = top.i[j];
+    }
+}
+
+// Direct FP64 scan for metrics without a prefilter (SE(3)) and any dimension.
+__device__ __forceinline__ double se3_dist(const double* a, const double* b, double w_so3, double w_l2);
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict__ x64, int64_t n, int64_t chunk,
+                                                       const double* __restrict__ q, int64_t nq, double radius,
+                                                       double w_so3, double w_l2, double* __restrict__ out_d,
+                                                       int32_t* __restrict__ out_i);
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ pd, const int32_t* __restrict__ pi,
+                                                    int splits, int64_t nq, int k, double* __restrict__ out_d,
+                                                    int32_t* __restrict__ out_i) {
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (qi >= nq) return;
+    TopK<K> top;
+    top.init();
+    for (int s = 0; s < splits; ++s) {
+        const int64_t base = ((int64_t)s * nq + qi) * K;
+        for (int j = 0; j < K; ++j) {
+            double d = pd[base + j];
+            int32_t id = pi[base + j];
+            if (id == INT_MAX) break;
+            top.insert(d, id);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (j < k) {
+            bool have = top.i[j] != INT_MAX;
+            out_d[qi * k + j] = have ? top.d[j] : __builtin_inf();
+            out_i[qi * k + j] = have ? top.i[j] : -1;
+        }
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void knn_finish(const double* __restrict__ pd, const int32_t* __restrict__ pi,
+                                                     int64_t nq, int k, double* __restrict__ out_d,
+                                                     int32_t* __restrict__ out_i) {
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (qi >= nq) return;
+    for (int j = 0; j < k; ++j) {
+        int32_t id = pi[qi * K + j];
+        bool have = id != INT_MAX;
+        out_d[qi * k + j] = have ? pd[qi * K + j] : __builtin_inf();
+        out_i[qi * k + j] = have ? id : -1;
+    }
+}
+
+// ---- SE(3): Cartesian[Scaled(SO3, w_so3), Scaled(L2, w_l2)] ----------------
+}  // namespace
+}  // namespace mptg
+
+#include "pmath.hpp"
+
+namespace mptg {
+namespace {
+
+__device__ __forceinline__ double se3_dist(const double* a, const double* b, double w_so3, double w_l2) {
+    double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+    double ad = __builtin_fabs(dot);
+    double so3 = ad >= 1.0 ? 0.0 : pm::acos(ad);
+    double s = 0.0;
+    for (int i = 4; i < 7; ++i) {
+        double t = a[i] - b[i];
+        s = s + t * t;
+    }
+    return w_so3 * so3 + w_l2 * __builtin_sqrt(s);
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict__ x64, int64_t n, int64_t chunk,
+                                                       const double* __restrict__ q, int64_t nq, double radius,
+                                                       double w_so3, double w_l2, double* __restrict__ out_d,
+                                                       int32_t* __restrict__ out_i) {
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (qi >= nq) return;
+    const int64_t p0 = (int64_t)blockIdx.y * chunk;
+    const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
+    double qq[7];
+    for (int i = 0; i < 7; ++i) qq[i] = q[qi * 7 + i];
+    TopK<K> top;
+    top.init();
+    for (int64_t p = p0; p < p1; ++p) {
+        double dq = se3_dist(qq, x64 + p * 7, w_so3, w_l2);
+        if (dq <= radius) top.insert(dq, (int32_t)p);
+    }
+    const int64_t base = ((int64_t)blockIdx.y * nq + qi) * K;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        out_d[base + j] = top.d[j];
+        out_i[base + j] = top.i[j];
+    }
+}
+
+template <int D, int METRIC, int K>
+void launch_scan(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq, double radius,
+                 double* pd, int32_t* pi) {
+    hipLaunchKernelGGL((knn_scan<D, METRIC, K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, n, chunk,
+                       q, nq, radius, ctx->coord_bound, pd, pi);
+}
+
+template <int D, int K>
+bool dispatch_metric(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq,
+                     double radius, double* pd, int32_t* pi) {
+    if (ctx->metric == MPTG_METRIC_L1) launch_scan<D, MPTG_METRIC_L1, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+    else launch_scan<D, MPTG_METRIC_L2, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+    return true;
+}
+
+template <int K>
+bool dispatch_dim(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq, double radius,
+                  double* pd, int32_t* pi) {
+    if (ctx->metric == MPTG_METRIC_SE3) {
+        hipLaunchKernelGGL((knn_scan_se3<K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x64, n, chunk, q, nq,
+                           radius, ctx->w_so3, ctx->w_l2, pd, pi);
+        return true;
+    }
+    switch (ctx->dim) {
+        case 1: return dispatch_metric<1, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 2: return dispatch_metric<2, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 3: return dispatch_metric<3, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 4: return dispatch_metric<4, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 5: return dispatch_metric<5, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 6: return dispatch_metric<6, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 7: return dispatch_metric<7, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 8: return dispatch_metric<8, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 10: return dispatch_metric<10, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 12: return dispatch_metric<12, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 16: return dispatch_metric<16, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 20: return dispatch_metric<20, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        default: return false;
+    }
+}
+
+}  // namespace
+
+// Chooses the number of tree chunks so that a launch has ~8 waves per SIMD
+// worth of work, with at least 256 nodes per chunk.
+static int64_t choose_splits(mptg_ctx* ctx, int64_t nq, int64_t n) {
+    int64_t qblocks = (nq + kBlock - 1) / kBlock;
+    int64_t want_blocks = (int64_t)ctx->cus * 8;
+    int64_t s = (want_blocks + qblocks - 1) / qblocks;
+    int64_t max_s = (n + 255) / 256;
+    if (s > max_s) s = max_s;
+    if (s < 1) s = 1;
+    if (s > 4096) s = 4096;
+    return s;
+}
+
+int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+               double* d_dist, int64_t tree_n) {
+    if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
+    if (nq > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 queries");
+    if (!(radius >= 0)) return fail(ctx, MPTG_E_ARG, "radius must be >= 0");
+    const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
+    int64_t n = tree_n;
+    int64_t splits = n > 0 ? choose_splits(ctx, nq, n) : 1;
+    int64_t chunk = n > 0 ? (n + splits - 1) / splits : 0;
+    if (n > 0) splits = (n + chunk - 1) / chunk;
+    size_t pbytes_d = sizeof(double) * K * nq * splits, pbytes_i = sizeof(int32_t) * K * nq * splits;
+    int rc;
+    if ((rc = ensure(ctx, ctx->scratch[4], pbytes_d))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[5], pbytes_i))) return rc;
+    double* pd = (double*)ctx->scratch[4].p;
+    int32_t* pi = (int32_t*)ctx->scratch[5].p;
+    dim3 grid((unsigned)((nq + kBlock - 1) / kBlock), (unsigned)splits);
+    bool ok;
+    if (K == 1) ok = dispatch_dim<1>(ctx, grid, n, chunk, d_q, nq, radius, pd, pi);
+    else if (K == 20) ok = dispatch_dim<20>(ctx, grid, n, chunk, d_q, nq, radius, pd, pi);
+    else ok = dispatch_dim<32>(ctx, grid, n, chunk, d_q, nq, radius, pd, pi);
+    if (!ok) return fail(ctx, MPTG_E_ARG, "unsupported dimension for the nearest-neighbour kernel");
+    MPTG_HIP(ctx, hipGetLastError());
+    dim3 g1((unsigned)((nq + kBlock - 1) / kBlock));
+    if (splits > 1) {
+        if (K == 1) hipLaunchKernelGGL((knn_merge<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx);
+        else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx);
+        else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx);
+    } else {
+        if (K == 1) hipLaunchKernelGGL((knn_finish<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx);
+        else if (K == 20) hipLaunchKernelGGL((knn_finish<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx);
+        else hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx);
+    }
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+}  // namespace mptg

@@ -1,0 +1,177 @@
+// Device-side metrics, interpolation and scene validity.
+//
+// Every expression is the FP64 left-to-right form the project fixes for
+// bit-exact results (built with FP contraction off):
+//   L2   sqrt(sum_i (a_i - b_i)^2)        L1  sum_i |a_i - b_i|
+//   SE3  w_so3 * acos(|<qa,qb>|) + w_l2 * ||ta - tb||      (src/mpt/se3_space.hpp)
+//   interpolate L_p: (b_i - a_i) * t + a_i                 (src/mpt/lp_space.hpp:44-54)
+//   interpolate SO3: slerp with lerp branch and sign flip  (src/mpt/so3_space.hpp:43-83)
+#pragma once
+#include "internal.hpp"
+#include "pmath.hpp"
+
+namespace mptg {
+
+struct SpaceArgs {
+    int metric;
+    int dim;
+    double w_so3, w_l2;
+};
+
+struct SceneArgs {
+    int kind;
+    const double* boxes;
+    int nboxes;
+    const uint8_t* grid;
+    int width, height;
+    const double* lengths;
+    int nlinks;
+    const double* disks;  // [cx, cy, (r + R)^2]
+    int ndisks;
+};
+
+inline SpaceArgs space_args(const mptg_ctx* ctx) { return SpaceArgs{ctx->metric, ctx->dim, ctx->w_so3, ctx->w_l2}; }
+
+inline SceneArgs scene_args(const mptg_ctx* ctx) {
+    const SceneDev& s = ctx->scene;
+    return SceneArgs{s.kind, s.boxes, s.nboxes, s.grid, s.width, s.height, s.lengths, s.nlinks, s.disks, s.ndisks};
+}
+
+__device__ __forceinline__ double so3_dist(const double* a, const double* b) {
+    double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+    double ad = __builtin_fabs(dot);
+    return ad >= 1.0 ? 0.0 : pm::acos(ad);
+}
+
+__device__ __forceinline__ double space_dist(const SpaceArgs& sp, const double* a, const double* b) {
+    double s = 0.0;
+    if (sp.metric == MPTG_METRIC_L1) {
+        for (int i = 0; i < sp.dim; ++i) s = s + __builtin_fabs(a[i] - b[i]);
+        return s;
+    }
+    if (sp.metric == MPTG_METRIC_SE3) {
+        for (int i = 4; i < 7; ++i) {
+            double t = a[i] - b[i];
+            s = s + t * t;
+        }
+        return sp.w_so3 * so3_dist(a, b) + sp.w_l2 * __builtin_sqrt(s);
+    }
+    for (int i = 0; i < sp.dim; ++i) {
+        double t = a[i] - b[i];
+        s = s + t * t;
+    }
+    return __builtin_sqrt(s);
+}
+
+__device__ __forceinline__ void space_interp(const SpaceArgs& sp, const double* a, const double* b, double t,
+                                             double* q) {
+    int i0 = 0;
+    if (sp.metric == MPTG_METRIC_SE3) {
+        double d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+        double ad = __builtin_fabs(d);
+        double s0, s1;
+        if (d >= 1.0 - 0x1.0p-52) {
+            s0 = 1.0 - t;
+            s1 = t;
+        } else {
+            double theta = pm::acos(ad);
+            double sinTheta = pm::sin(theta);
+            s0 = pm::sin((1.0 - t) * theta) / sinTheta;
+            s1 = pm::sin(t * theta) / sinTheta;
+        }
+        if (d < 0) s1 = -s1;
+        for (int i = 0; i < 4; ++i) q[i] = s0 * a[i] + s1 * b[i];
+        i0 = 4;
+    }
+    for (int i = i0; i < sp.dim; ++i) q[i] = (b[i] - a[i]) * t + a[i];
+}
+
+// distPointSegmentSquared, demo/shape_hierarchy.hpp:259-270
+__device__ __forceinline__ double seg_dist_sq(double px, double py, double s0x, double s0y, double s1x, double s1y) {
+    double vx = s1x - s0x, vy = s1y - s0y;
+    double wx = px - s0x, wy = py - s0y;
+    double c1 = vx * wx + vy * wy;
+    if (c1 <= 0) return wx * wx + wy * wy;
+    double c2 = vx * vx + vy * vy;
+    if (c2 <= c1) {
+        double ex = px - s1x, ey = py - s1y;
+        return ex * ex + ey * ey;
+    }
+    double b = c1 / c2;
+    double ex = (s0x - px) + vx * b;
+    double ey = (s0y - py) + vy * b;
+    return ex * ex + ey * ey;
+}
+
+// LinkManipulatorScenario::valid, demo/link_manipulator_scenario.hpp:99-116
+__device__ __forceinline__ bool arm_valid(const SceneArgs& sc, const double* q) {
+    double fx = 0.0, fy = 0.0, angle = 0.0;
+    for (int i = 0; i < sc.nlinks; ++i) {
+        angle += q[i];
+        double s, c;
+        pm::sincos(angle, &s, &c);
+        double L = sc.lengths[i];
+        double tx = fx + L * c;
+        double ty = fy + L * s;
+        for (int j = 0; j < sc.ndisks; ++j) {
+            const double* dk = sc.disks + 3 * j;
+            if (!(seg_dist_sq(dk[0], dk[1], fx, fy, tx, ty) > dk[2])) return false;
+        }
+        fx = tx;
+        fy = ty;
+    }
+    return true;
+}
+
+// arm_valid with work counters: FK links evaluated and segment-disk tests
+// evaluated (same early exits); used to price a launch in FLOPs.
+__device__ __forceinline__ bool arm_valid_counted(const SceneArgs& sc, const double* q, uint64_t* links,
+                                                  uint64_t* tests) {
+    double fx = 0.0, fy = 0.0, angle = 0.0;
+    for (int i = 0; i < sc.nlinks; ++i) {
+        angle += q[i];
+        double s, c;
+        pm::sincos(angle, &s, &c);
+        *links += 1;
+        double L = sc.lengths[i];
+        double tx = fx + L * c;
+        double ty = fy + L * s;
+        for (int j = 0; j < sc.ndisks; ++j) {
+            const double* dk = sc.disks + 3 * j;
+            *tests += 1;
+            if (!(seg_dist_sq(dk[0], dk[1], fx, fy, tx, ty) > dk[2])) return false;
+        }
+        fx = tx;
+        fy = ty;
+    }
+    return true;
+}
+
+// Rect::pointIsValid over all boxes, demo/shape_hierarchy.hpp:177-182
+__device__ __forceinline__ bool boxes_valid(const SceneArgs& sc, const double* q) {
+    for (int i = 0; i < sc.nboxes; ++i) {
+        const double* b = sc.boxes + 4 * i;
+        if (q[0] >= b[0] && q[0] <= b[2] && q[1] >= b[1] && q[1] <= b[3]) return false;
+    }
+    return true;
+}
+
+// PNG2dScenario::valid, demo/png_2d_scenario.hpp:104-110, plus a bounds check.
+__device__ __forceinline__ bool grid_valid(const SceneArgs& sc, const double* q) {
+    double fx = q[0] + 0.5, fy = q[1] + 0.5;
+    if (!(fx > -1.0 && fx < (double)sc.width + 1.0 && fy > -1.0 && fy < (double)sc.height + 1.0)) return false;
+    int x = (int)fx, y = (int)fy;
+    if (x < 0 || y < 0 || x >= sc.width || y >= sc.height) return false;
+    return sc.grid[(int64_t)sc.width * y + x] == 0;
+}
+
+__device__ __forceinline__ bool state_valid(const SceneArgs& sc, const double* q) {
+    switch (sc.kind) {
+        case MPTG_SCENE_BOXES: return boxes_valid(sc, q);
+        case MPTG_SCENE_GRID: return grid_valid(sc, q);
+        case MPTG_SCENE_ARM: return arm_valid(sc, q);
+        default: return true;
+    }
+}
+
+}  // namespace mptg

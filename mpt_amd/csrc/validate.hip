@@ -1,0 +1,425 @@
+// Batched state validity and motion validation (the Scenario's valid()/link(),
+// called at src/mpt/impl/prrt/prrt.hpp:439,454-457).
+//
+// Two link() forms:
+//  * DMV — DiscreteMotionValidator, src/mpt/discrete_motion_validator.hpp:71-130:
+//    valid(to) first, steps = ceil(dist * (1/step)), waypoints i * (1/steps)
+//    for i in [1, steps-1] visited in the 256-slot BFS-bisection ring order,
+//    ranges that find the ring full are visited sequentially.  For
+//    n = steps-1 <= 255 the ring can never fill (255 disjoint non-empty
+//    ranges plus a popped range of size >= 2 need n >= 256), so the order is
+//    a pure BFS and is read from a 32 KiB table built once per context.
+//    Longer edges go to a second kernel in which one wave walks the ring in
+//    LDS and checks 64 waypoints per round.
+//  * BISECT — LinkManipulatorScenario::link/bisectLink,
+//    demo/link_manipulator_scenario.hpp:118-138: valid(a), valid(b), then the
+//    pre-order of recursive midpoints (a+b)/2 until L-inf(a-b) < tol.  The
+//    walk is stackless: the current interval is named by its left/right path
+//    from the root and recomputed from (a, b) when needed, so each midpoint
+//    has exactly the reference's bits whatever the tree's shape.
+//
+// verdict is the AND over a fixed set of valid() calls, so it does not depend
+// on the visiting order; `checks` counts the calls the given order makes
+// before the verdict is known, endpoint checks included.
+#include <algorithm>
+#include <array>
+#include <utility>
+#include <vector>
+
+#include "space_dev.hpp"
+
+namespace mptg {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTableMax = 255;  // largest n = steps-1 served by the BFS table
+// Work limits of one link() call; an edge beyond them is not checked and
+// reports verdict 0 with checks = MPTG_CHECKS_OVER_LIMIT.
+constexpr int64_t kMaxDmvWaypoints = (int64_t)1 << 24;
+constexpr int kMaxBisectDepth = 30;
+
+__global__ __launch_bounds__(kBlock) void valid_kernel(SceneArgs sc, int dim, const double* __restrict__ q,
+                                                       int64_t n, uint8_t* __restrict__ ok) {
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double s[kMaxDim];
+    for (int j = 0; j < dim; ++j) s[j] = q[i * dim + j];
+    ok[i] = state_valid(sc, s) ? 1 : 0;
+}
+
+// steps = ceil(d * inv) as the reference's size_t; returns n = steps - 1
+// (number of interior waypoints), or 0 when steps < 2.
+__device__ __forceinline__ int64_t dmv_count(const SpaceArgs& sp, const double* a, const double* b, double inv,
+                                             double* delta) {
+    double x = __builtin_ceil(space_dist(sp, a, b) * inv);
+    if (!(x >= 2.0)) return 0;
+    if (!(x < 0x1.0p62)) x = 0x1.0p62;
+    int64_t steps = (int64_t)x;
+    *delta = 1.0 / (double)steps;
+    return steps - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs sc, double inv,
+                                                         const uint8_t* __restrict__ table,
+                                                         const double* __restrict__ from,
+                                                         const double* __restrict__ to, int64_t n, int known_valid,
+                                                         const uint8_t* __restrict__ mask,
+                                                         uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks,
+                                                         int64_t* __restrict__ long_list,
+                                                         uint32_t* __restrict__ long_count) {
+    int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n || (mask && !mask[e])) return;
+    const int D = sp.dim;
+    double a[kMaxDim], b[kMaxDim], q[kMaxDim];
+    for (int j = 0; j < D; ++j) {
+        a[j] = from[e * D + j];
+        b[j] = to[e * D + j];
+    }
+    if (!known_valid && !state_valid(sc, b)) {
+        verdict[e] = 0;
+        checks[e] = 1;
+        return;
+    }
+    double delta = 0.0;
+    int64_t m = dmv_count(sp, a, b, inv, &delta);
+    if (m > kMaxDmvWaypoints) {
+        verdict[e] = 0;
+        checks[e] = MPTG_CHECKS_OVER_LIMIT;
+        return;
+    }
+    if (m > kTableMax) {
+        uint32_t slot = atomicAdd(long_count, 1u);
+        long_list[slot] = e;
+        return;
+    }
+    const uint8_t* ord = table + (m * (m - 1)) / 2;
+    for (int64_t o = 0; o < m; ++o) {
+        int64_t i = ord[o];
+        space_interp(sp, a, b, (double)i * delta, q);
+        if (!state_valid(sc, q)) {
+            verdict[e] = 0;
+            checks[e] = (uint32_t)(2 + o);
+            return;
+        }
+    }
+    verdict[e] = 1;
+    checks[e] = (uint32_t)(1 + m);
+}
+
+// One wave per long edge: lane 0 replays the reference's ring in LDS and
+// emits up to 64 waypoint indices per round; all lanes check them.
+__global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, double inv,
+                                                    const double* __restrict__ from, const double* __restrict__ to,
+                                                    const int64_t* __restrict__ long_list,
+                                                    const uint32_t* __restrict__ long_count,
+                                                    uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks) {
+    __shared__ uint64_t rmin[256], rmax[256];
+    __shared__ uint64_t emitted[64];
+    __shared__ int cnt;
+    __shared__ int fail_lane;
+    const uint32_t total = *long_count;
+    const int lane = threadIdx.x;
+    for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const int64_t e = long_list[w];
+        const int D = sp.dim;
+        double a[kMaxDim], b[kMaxDim], q[kMaxDim];
+        for (int j = 0; j < D; ++j) {
+            a[j] = from[e * D + j];
+            b[j] = to[e * D + j];
+        }
+        double delta = 0.0;
+        int64_t m = dmv_count(sp, a, b, inv, &delta);
+        // ring state lives in lane 0's registers
+        uint64_t qs = 0, qe = 1, seq_cur = 0, seq_end = 0;
+        bool seq = false;
+        if (lane == 0) {
+            rmin[0] = 1;
+            rmax[0] = (uint64_t)m;
+        }
+        int64_t done = 0;
+        bool failed = false;
+        for (;;) {
+            if (lane == 0) {
+                int c = 0;
+                while (c < 64 && (seq || qs != qe)) {
+                    if (seq) {
+                        emitted[c++] = seq_cur;
+                        if (seq_cur == seq_end) seq = false;
+                        else ++seq_cur;
+                        continue;
+                    }
+                    uint64_t mn = rmin[qs % 256], mx = rmax[qs % 256];
+                    ++qs;
+                    if (mn == mx) {
+                        emitted[c++] = mn;
+                    } else if (qe + 2 < qs + 256) {
+                        uint64_t mid = (mn + mx) / 2;
+                        emitted[c++] = mid;
+                        if (mn < mid) {
+                            rmin[qe % 256] = mn;
+                            rmax[qe % 256] = mid - 1;
+                            ++qe;
+                        }
+                        if (mid < mx) {
+                            rmin[qe % 256] = mid + 1;
+                            rmax[qe % 256] = mx;
+                            ++qe;
+                        }
+                    } else {
+                        seq = true;
+                        seq_cur = mn;
+                        seq_end = mx;
+                    }
+                }
+                cnt = c;
+                fail_lane = 64;
+            }
+            __syncthreads();
+            const int c = cnt;
+            if (c == 0) break;
+            bool bad = false;
+            if (lane < c) {
+                space_interp(sp, a, b, (double)emitted[lane] * delta, q);
+                bad = !state_valid(sc, q);
+            }
+            if (bad) atomicMin(&fail_lane, lane);
+            __syncthreads();
+            const int fl = fail_lane;
+            if (fl < 64) {
+                failed = true;
+                done += fl + 1;
+                break;
+            }
+            done += c;
+            __syncthreads();
+        }
+        if (lane == 0) {
+            verdict[e] = failed ? 0 : 1;
+            checks[e] = (uint32_t)(1 + done);
+        }
+        __syncthreads();
+    }
+}
+
+// Interval of the node at `path` (bit k = direction taken at level k,
+// 0 = left half [lo, mid], 1 = right half [mid, hi]) with `depth` levels.
+__device__ __forceinline__ void bisect_interval(int D, const double* a, const double* b, uint64_t path, int depth,
+                                                double* lo, double* hi) {
+    for (int j = 0; j < D; ++j) {
+        lo[j] = a[j];
+        hi[j] = b[j];
+    }
+    for (int k = 0; k < depth; ++k) {
+        bool right = (path >> k) & 1;
+        for (int j = 0; j < D; ++j) {
+            double mid = (lo[j] + hi[j]) / 2;
+            if (right) lo[j] = mid;
+            else hi[j] = mid;
+        }
+    }
+}
+
+__device__ __forceinline__ bool bisect_internal(int D, const double* lo, const double* hi, double tol) {
+    double m = 0.0;
+    for (int j = 0; j < D; ++j) m = __builtin_fmax(m, __builtin_fabs(lo[j] - hi[j]));
+    return !(m < tol);
+}
+
+__global__ __launch_bounds__(kBlock) void link_bisect(int D, SceneArgs sc, double tol,
+                                                      const double* __restrict__ from, const double* __restrict__ to,
+                                                      int64_t n, int known_valid, const uint8_t* __restrict__ mask,
+                                                      uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks) {
+    int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n || (mask && !mask[e])) return;
+    double a[kMaxDim], b[kMaxDim], lo[kMaxDim], hi[kMaxDim];
+    for (int j = 0; j < D; ++j) {
+        a[j] = from[e * D + j];
+        b[j] = to[e * D + j];
+    }
+    uint32_t c = 1;
+    if (!known_valid && !state_valid(sc, a)) {
+        verdict[e] = 0;
+        checks[e] = c;
+        return;
+    }
+    ++c;
+    if (!known_valid && !state_valid(sc, b)) {
+        verdict[e] = 0;
+        checks[e] = c;
+        return;
+    }
+    uint64_t path = 0;
+    int depth = 0;
+    for (;;) {
+        bisect_interval(D, a, b, path, depth, lo, hi);
+        if (bisect_internal(D, lo, hi, tol)) {
+            for (int j = 0; j < D; ++j) lo[j] = (lo[j] + hi[j]) / 2;
+            ++c;
+            if (!state_valid(sc, lo)) {
+                verdict[e] = 0;
+                checks[e] = c;
+                return;
+            }
+            if (depth >= kMaxBisectDepth) {
+                verdict[e] = 0;
+                checks[e] = MPTG_CHECKS_OVER_LIMIT;
+                return;
+            }
+            ++depth;  // descend left (bit stays 0)
+        } else {
+            while (depth > 0 && ((path >> (depth - 1)) & 1)) {
+                path &= ~(1ull << (depth - 1));
+                --depth;
+            }
+            if (depth == 0) break;
+            path |= 1ull << (depth - 1);
+        }
+    }
+    verdict[e] = 1;
+    checks[e] = c;
+}
+
+// Instrumented bisection link (not timed): same walk and verdicts, and
+// accumulates [waypoint valid() calls, FK links, segment-disk tests] for the
+// arm scene into work[0..2].
+__global__ __launch_bounds__(kBlock) void link_bisect_work(int D, SceneArgs sc, double tol,
+                                                           const double* __restrict__ from,
+                                                           const double* __restrict__ to, int64_t n, int known_valid,
+                                                           unsigned long long* __restrict__ work) {
+    int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n) return;
+    double a[kMaxDim], b[kMaxDim], lo[kMaxDim], hi[kMaxDim];
+    for (int j = 0; j < D; ++j) {
+        a[j] = from[e * D + j];
+        b[j] = to[e * D + j];
+    }
+    uint64_t calls = 0, links = 0, tests = 0;
+    bool ok = true;
+    if (!known_valid) {
+        ++calls;
+        ok = arm_valid_counted(sc, a, &links, &tests);
+        if (ok) {
+            ++calls;
+            ok = arm_valid_counted(sc, b, &links, &tests);
+        }
+    }
+    uint64_t path = 0;
+    int depth = 0;
+    while (ok) {
+        bisect_interval(D, a, b, path, depth, lo, hi);
+        if (bisect_internal(D, lo, hi, tol)) {
+            for (int j = 0; j < D; ++j) lo[j] = (lo[j] + hi[j]) / 2;
+            ++calls;
+            if (!arm_valid_counted(sc, lo, &links, &tests)) break;
+            if (depth >= kMaxBisectDepth) break;
+            ++depth;
+        } else {
+            while (depth > 0 && ((path >> (depth - 1)) & 1)) {
+                path &= ~(1ull << (depth - 1));
+                --depth;
+            }
+            if (depth == 0) break;
+            path |= 1ull << (depth - 1);
+        }
+    }
+    atomicAdd(&work[0], (unsigned long long)calls);
+    atomicAdd(&work[1], (unsigned long long)links);
+    atomicAdd(&work[2], (unsigned long long)tests);
+}
+
+// Host replay of the reference ring for n <= 255 (never overflows): the BFS
+// order of the bisection ranges.  Table row n starts at n(n-1)/2.
+std::vector<uint8_t> build_dmv_table() {
+    std::vector<uint8_t> t((size_t)kTableMax * (kTableMax + 1) / 2);
+    for (int m = 1; m <= kTableMax; ++m) {
+        size_t off = (size_t)m * (m - 1) / 2, c = 0;
+        std::vector<std::pair<int, int>> q;
+        q.reserve(m);
+        q.emplace_back(1, m);
+        for (size_t h = 0; h < q.size(); ++h) {
+            auto [mn, mx] = q[h];
+            int mid = (mn + mx) / 2;
+            t[off + c++] = (uint8_t)mid;
+            if (mn < mid) q.emplace_back(mn, mid - 1);
+            if (mid < mx) q.emplace_back(mid + 1, mx);
+        }
+    }
+    return t;
+}
+
+}  // namespace
+
+static int ensure_tables(mptg_ctx* ctx) {
+    DevBuf& tb = ctx->scratch[6];
+    if (tb.p) return MPTG_OK;
+    std::vector<uint8_t> t = build_dmv_table();
+    int rc = ensure(ctx, tb, t.size());
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(tb.p, t.data(), t.size(), hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
+    if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
+    if (ctx->scene.kind == MPTG_SCENE_NONE) return fail(ctx, MPTG_E_STATE, "no scene uploaded");
+    if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > ctx->dim)
+        return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
+    if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
+    hipLaunchKernelGGL(valid_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                       scene_args(ctx), ctx->dim, d_q, n, d_ok);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order, int known_valid,
+                uint8_t* d_verdict, uint32_t* d_checks) {
+    return launch_link_masked(ctx, d_from, d_to, n, order, known_valid, nullptr, d_verdict, d_checks);
+}
+
+int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                       int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks) {
+    if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
+    if (ctx->scene.kind == MPTG_SCENE_NONE) return fail(ctx, MPTG_E_STATE, "no scene uploaded");
+    if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > ctx->dim)
+        return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
+    if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
+    if (order != MPTG_ORDER_REFERENCE) return fail(ctx, MPTG_E_STATE, "surrogate order needs a trained surrogate");
+    if (n > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 edges per call");
+    unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    if (ctx->link_mode == MPTG_LINK_BISECT) {
+        hipLaunchKernelGGL(link_bisect, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->dim, scene_args(ctx),
+                           ctx->link_param, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks);
+        MPTG_HIP(ctx, hipGetLastError());
+        return MPTG_OK;
+    }
+    int rc = ensure_tables(ctx);
+    if (rc) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
+    uint32_t* long_count = (uint32_t*)ctx->scratch[7].p;
+    int64_t* long_list = (int64_t*)((char*)ctx->scratch[7].p + 256);
+    MPTG_HIP(ctx, hipMemsetAsync(long_count, 0, sizeof(uint32_t), ctx->stream));
+    double inv = 1.0 / ctx->link_param;
+    hipLaunchKernelGGL(link_dmv_short, dim3(blocks), dim3(kBlock), 0, ctx->stream, space_args(ctx), scene_args(ctx),
+                       inv, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks,
+                       long_list, long_count);
+    MPTG_HIP(ctx, hipGetLastError());
+    unsigned lg = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 16);
+    hipLaunchKernelGGL(link_dmv_long, dim3(lg), dim3(64), 0, ctx->stream, space_args(ctx), scene_args(ctx), inv,
+                       d_from, d_to, long_list, long_count, d_verdict, d_checks);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int known_valid,
+                     unsigned long long* d_work) {
+    if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->link_mode != MPTG_LINK_BISECT)
+        return fail(ctx, MPTG_E_STATE, "work counting is implemented for the arm scene with bisection links");
+    MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL(link_bisect_work, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                       ctx->dim, scene_args(ctx), ctx->link_param, d_from, d_to, n, known_valid, d_work);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+}  // namespace mptg

@@ -1,0 +1,191 @@
+"""Scenario descriptions and the seeded synthetic scenes of the benchmarks.
+
+A :class:`Scene` is the data form of a reference Scenario for the GPU path:
+space (metric, dimension, weights), bounds, start/goal, the ``valid()`` model
+(boxes / occupancy grid / planar arm vs disks) and the ``link()`` form
+(DiscreteMotionValidator step or recursive-bisection tolerance).
+
+The builders reproduce the scenario models of the reference demos with seeded
+synthetic obstacles:
+
+* :func:`boxes_scene` — 2-D point among closed boxes
+  (``demo/holonomic_2d_point_scenario.hpp``, ``Rect`` in ``demo/shape_hierarchy.hpp:168-211``),
+  DMV step 0.1, range 2.0.
+* :func:`grid_scene` — occupancy grid (``demo/png_2d_scenario.hpp:104-110``),
+  DMV step 1 px.
+* :func:`arm_scene` — planar link manipulator vs disks
+  (``demo/link_manipulator_scenario.hpp:54-155``), L1 metric, bisection
+  tolerance 0.02 rad, no range limit (the demo sets none).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+METRIC_L2, METRIC_L1, METRIC_SE3 = 0, 1, 2
+SCENE_NONE, SCENE_BOXES, SCENE_GRID, SCENE_ARM = 0, 1, 2, 3
+LINK_DMV, LINK_BISECT = 0, 1
+ORDER_REFERENCE, ORDER_SURROGATE = 0, 1
+
+
+@dataclass
+class Scene:
+    metric: int
+    dim: int
+    kind: int
+    link_mode: int
+    link_param: float
+    lo: np.ndarray
+    hi: np.ndarray
+    start: np.ndarray
+    goal: np.ndarray
+    goal_radius: float = 1e-6
+    range: float = math.inf
+    goal_bias: float = 0.01
+    w_so3: float = 1.0
+    w_l2: float = 1.0
+    boxes: Optional[np.ndarray] = None      # (n, 4) x0 y0 x1 y1
+    grid: Optional[np.ndarray] = None       # (h, w) uint8, nonzero = obstacle
+    lengths: Optional[np.ndarray] = None    # (links,)
+    robot_radius: float = 0.0
+    disks: Optional[np.ndarray] = None      # (n, 3) cx cy r
+    name: str = ""
+    extra: dict = field(default_factory=dict)
+
+
+# ---------------------------------------------------------------- validity
+def boxes_valid_np(boxes: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """Vectorised Rect::pointIsValid over all boxes (closed boxes)."""
+    x = q[:, 0:1]
+    y = q[:, 1:2]
+    inside = (x >= boxes[None, :, 0]) & (x <= boxes[None, :, 2]) & (y >= boxes[None, :, 1]) & (y <= boxes[None, :, 3])
+    return ~inside.any(axis=1)
+
+
+def boxes_scene(seed: int = 1, nboxes: int = 16, size: float = 100.0) -> Scene:
+    """[0,100]^2, 16 boxes with centre U[0,100]^2 and sides U[5,15]; start
+    (5,5), goal (95,95); a box covering start or goal is re-drawn."""
+    rng = np.random.default_rng(seed)
+    start = np.array([5.0, 5.0])
+    goal = np.array([95.0, 95.0])
+    boxes = np.zeros((nboxes, 4))
+    for i in range(nboxes):
+        while True:
+            c = rng.uniform(0, size, 2)
+            s = rng.uniform(5, 15, 2)
+            b = np.array([c[0] - s[0] / 2, c[1] - s[1] / 2, c[0] + s[0] / 2, c[1] + s[1] / 2])
+            if boxes_valid_np(b[None], np.stack([start, goal])).all():
+                boxes[i] = b
+                break
+    return Scene(metric=METRIC_L2, dim=2, kind=SCENE_BOXES, link_mode=LINK_DMV, link_param=0.1,
+                 lo=np.zeros(2), hi=np.full(2, size), start=start, goal=goal, range=2.0, boxes=boxes,
+                 name=f"boxes{nboxes}_seed{seed}")
+
+
+def _stamp_disk(grid: np.ndarray, cx: float, cy: float, r: float) -> None:
+    h, w = grid.shape
+    x0, x1 = max(0, int(math.floor(cx - r))), min(w - 1, int(math.ceil(cx + r)))
+    y0, y1 = max(0, int(math.floor(cy - r))), min(h - 1, int(math.ceil(cy + r)))
+    if x0 > x1 or y0 > y1:
+        return
+    xs = np.arange(x0, x1 + 1)[None, :]
+    ys = np.arange(y0, y1 + 1)[:, None]
+    m = (xs - cx) ** 2 + (ys - cy) ** 2 <= r * r
+    grid[y0:y1 + 1, x0:x1 + 1] |= m.astype(np.uint8)
+
+
+def grid_scene(seed: int = 4, size: int = 4096, occupancy: float = 0.30, rmin: float = 8, rmax: float = 64) -> Scene:
+    """size x size grid of seeded random disks (radius U[rmin,rmax]) added
+    until at least `occupancy` of the cells are obstacles."""
+    rng = np.random.default_rng(seed)
+    grid = np.zeros((size, size), dtype=np.uint8)
+    target = occupancy * size * size
+    occ = 0
+    while occ < target:
+        for _ in range(64):
+            c = rng.uniform(0, size, 2)
+            _stamp_disk(grid, c[0], c[1], rng.uniform(rmin, rmax))
+        occ = int(grid.sum())
+    free = np.argwhere(grid == 0)
+    # start near the lower-left corner, goal near the upper-right one
+    d0 = free[:, 0].astype(np.int64) ** 2 + free[:, 1].astype(np.int64) ** 2
+    d1 = (size - 1 - free[:, 0].astype(np.int64)) ** 2 + (size - 1 - free[:, 1].astype(np.int64)) ** 2
+    sy, sx = free[int(np.argmin(d0))]
+    gy, gx = free[int(np.argmin(d1))]
+    return Scene(metric=METRIC_L2, dim=2, kind=SCENE_GRID, link_mode=LINK_DMV, link_param=1.0,
+                 lo=np.zeros(2), hi=np.full(2, float(size)), start=np.array([float(sx), float(sy)]),
+                 goal=np.array([float(gx), float(gy)]), grid=grid, name=f"grid{size}_seed{seed}",
+                 extra={"occupancy": occ / float(size * size)})
+
+
+def grid_edges(scene: Scene, n: int, seed: int = 5, mean_len: float = 40.0):
+    """n edges: start U[0,size)^2, direction U[0,2pi), length ~ Exp(mean),
+    end clipped to the grid."""
+    rng = np.random.default_rng(seed)
+    size = float(scene.grid.shape[0])
+    a = rng.uniform(0, size, (n, 2))
+    th = rng.uniform(0, 2 * math.pi, n)
+    ln = rng.exponential(mean_len, n)
+    b = a + np.stack([np.cos(th), np.sin(th)], axis=1) * ln[:, None]
+    b = np.clip(b, 0.0, size - 1.0)
+    return a, b
+
+
+def arm_valid_np(scene: Scene, q: np.ndarray) -> np.ndarray:
+    """Vectorised (approximate, numpy trig) LinkManipulatorScenario::valid,
+    used only to pick start/goal states while building a scene."""
+    n = q.shape[0]
+    ok = np.ones(n, dtype=bool)
+    ang = np.cumsum(q, axis=1)
+    fx = np.zeros(n)
+    fy = np.zeros(n)
+    c = scene.disks[:, :2]
+    rr = (scene.disks[:, 2] + scene.robot_radius) ** 2
+    for i in range(q.shape[1]):
+        tx = fx + scene.lengths[i] * np.cos(ang[:, i])
+        ty = fy + scene.lengths[i] * np.sin(ang[:, i])
+        vx, vy = (tx - fx)[:, None], (ty - fy)[:, None]
+        wx, wy = c[None, :, 0] - fx[:, None], c[None, :, 1] - fy[:, None]
+        c1 = vx * wx + vy * wy
+        c2 = vx * vx + vy * vy
+        t = np.clip(np.where(c2 > 0, c1 / np.where(c2 > 0, c2, 1), 0), 0, 1)
+        ex = fx[:, None] + vx * t - c[None, :, 0]
+        ey = fy[:, None] + vy * t - c[None, :, 1]
+        ok &= ~((ex * ex + ey * ey) <= rr[None, :]).any(axis=1)
+        fx, fy = tx, ty
+    return ok
+
+
+def arm_scene(links: int = 20, ndisks: int = 200, seed_links: int = 7, seed_disks: int = 8,
+              robot_radius: float = 0.5, seed_states: int = 10) -> Scene:
+    """Planar arm: link lengths U[2,5], disks with centre U[-L,L]^2 (L = total
+    length) and radius U[1,4]; disks that would cover the base are re-drawn.
+    Start and goal are drawn uniformly in [-pi,pi]^links until valid."""
+    rl = np.random.default_rng(seed_links)
+    lengths = rl.uniform(2, 5, links)
+    total = float(lengths.sum())
+    rd = np.random.default_rng(seed_disks)
+    disks = np.zeros((ndisks, 3))
+    for i in range(ndisks):
+        while True:
+            c = rd.uniform(-total, total, 2)
+            r = rd.uniform(1, 4)
+            if math.hypot(c[0], c[1]) > r + robot_radius + lengths[0]:
+                disks[i] = (c[0], c[1], r)
+                break
+    sc = Scene(metric=METRIC_L1, dim=links, kind=SCENE_ARM, link_mode=LINK_BISECT, link_param=0.02,
+               lo=np.full(links, -math.pi), hi=np.full(links, math.pi), start=np.zeros(links),
+               goal=np.zeros(links), lengths=lengths, robot_radius=robot_radius, disks=disks,
+               name=f"arm{links}_disks{ndisks}")
+    rs = np.random.default_rng(seed_states)
+    found = []
+    while len(found) < 2:
+        q = rs.uniform(-math.pi, math.pi, (4096, links))
+        ok = arm_valid_np(sc, q)
+        found.extend(q[ok][: 2 - len(found)])
+    sc.start = np.array(found[0])
+    sc.goal = np.array(found[1])
+    return sc

@@ -1,0 +1,237 @@
+"""CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of ``oracle/liboracle.so`` (a C++ restatement of the
+reference's hot-path semantics, see ``oracle/oracle.h``).  Only the tests,
+``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` import
+this package, and only as the checker or the timed CPU baseline; the product
+library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class OrcScene(C.Structure):
+    _fields_ = [
+        ("metric", C.c_int32), ("dim", C.c_int32), ("w_so3", C.c_double), ("w_l2", C.c_double),
+        ("kind", C.c_int32), ("link_mode", C.c_int32), ("link_param", C.c_double),
+        ("boxes", _dp), ("nboxes", C.c_int32),
+        ("grid", _u8p), ("width", C.c_int32), ("height", C.c_int32),
+        ("lengths", _dp), ("nlinks", C.c_int32), ("robot_radius", C.c_double),
+        ("disks", _dp), ("ndisks", C.c_int32),
+    ]
+
+
+class OrcPrrtParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("range", C.c_double), ("goal_bias", C.c_double),
+                ("goal_radius", C.c_double), ("batch", C.c_int32), ("_pad", C.c_int32),
+                ("lo", _dp), ("hi", _dp), ("goal", _dp)]
+
+
+class OrcPrrtStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("iterations", "samples", "nearest_calls", "zero_dist",
+                                         "invalid_states", "link_calls", "link_ok", "valid_calls", "goals")]
+
+
+class OrcPoolResult(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("size", C.c_int64), ("samples", C.c_int64),
+                ("link_calls", C.c_int64), ("valid_calls", C.c_int64), ("goals", C.c_int64)]
+
+
+_SIG = [
+    ("orc_philox", None, [_u32p, _u32p, _u32p]),
+    ("orc_sincos", None, [_dp, C.c_int64, _dp, _dp]),
+    ("orc_acos", None, [_dp, C.c_int64, _dp]),
+    ("orc_uniform_at", C.c_double, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]),
+    ("orc_distance", C.c_double, [C.POINTER(OrcScene), _dp, _dp]),
+    ("orc_interpolate", None, [C.POINTER(OrcScene), _dp, _dp, C.c_double, _dp]),
+    ("orc_knn", None, [C.POINTER(OrcScene), _dp, C.c_int64, _dp, C.c_int64, C.c_int32, C.c_double, _i32p, _dp,
+                       C.c_int32]),
+    ("orc_valid", C.c_int32, [C.POINTER(OrcScene), _dp]),
+    ("orc_dmv_order", C.c_int64, [C.c_int64, _i64p, C.c_int64]),
+    ("orc_link_waypoints", C.c_int64, [C.POINTER(OrcScene), _dp, _dp, _dp, C.c_int64]),
+    ("orc_link", C.c_int32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32, _u32p]),
+    ("orc_link_batch", None, [C.POINTER(OrcScene), _dp, _dp, C.c_int64, C.c_int32, _u8p, _u32p, C.c_int32]),
+    ("orc_checks_in_score_order", C.c_uint32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32,
+                                               C.POINTER(C.c_float), C.c_int64]),
+    ("orc_prrt_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_int64, _dp,
+                                 _i32p, C.c_int64, _i32p, C.c_int64, C.POINTER(OrcPrrtStats)]),
+    ("orc_pool_run", C.c_int32, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_double,
+                                 C.c_int32, C.POINTER(OrcPoolResult)]),
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: run `python -m mpt_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for n, r, a in _SIG:
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """Oracle bound to one scene (keeps the scene's arrays alive)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self.dim = scene.dim
+        self._keep = []
+        s = OrcScene()
+        s.metric, s.dim, s.w_so3, s.w_l2 = scene.metric, scene.dim, scene.w_so3, scene.w_l2
+        s.kind, s.link_mode, s.link_param = scene.kind, scene.link_mode, scene.link_param
+        if scene.boxes is not None:
+            b = _f64(scene.boxes).reshape(-1, 4)
+            self._keep.append(b)
+            s.boxes, s.nboxes = _p(b, C.c_double), b.shape[0]
+        if scene.grid is not None:
+            g = np.ascontiguousarray(scene.grid, dtype=np.uint8)
+            self._keep.append(g)
+            s.grid, s.width, s.height = _p(g, C.c_uint8), g.shape[1], g.shape[0]
+        if scene.lengths is not None:
+            ln = _f64(scene.lengths)
+            dk = _f64(scene.disks).reshape(-1, 3)
+            self._keep += [ln, dk]
+            s.lengths, s.nlinks, s.robot_radius = _p(ln, C.c_double), ln.shape[0], scene.robot_radius
+            s.disks, s.ndisks = _p(dk, C.c_double), dk.shape[0]
+        self.s = s
+        self.L = lib()
+
+    def distance(self, a, b) -> float:
+        a, b = _f64(a), _f64(b)
+        return self.L.orc_distance(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double))
+
+    def interpolate(self, a, b, t) -> np.ndarray:
+        a, b = _f64(a), _f64(b)
+        out = np.zeros(self.dim)
+        self.L.orc_interpolate(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double), t, _p(out, C.c_double))
+        return out
+
+    def knn(self, pts, q, k=1, radius=math.inf, threads=8):
+        pts = _f64(pts).reshape(-1, self.dim)
+        q = _f64(q).reshape(-1, self.dim)
+        idx = np.zeros((q.shape[0], k), dtype=np.int32)
+        dist = np.zeros((q.shape[0], k))
+        self.L.orc_knn(C.byref(self.s), _p(pts, C.c_double), pts.shape[0], _p(q, C.c_double), q.shape[0], k, radius,
+                       _p(idx, C.c_int32), _p(dist, C.c_double), threads)
+        return idx, dist
+
+    def valid(self, q) -> np.ndarray:
+        q = _f64(q).reshape(-1, self.dim)
+        return np.array([bool(self.L.orc_valid(C.byref(self.s), _p(q[i], C.c_double))) for i in range(q.shape[0])])
+
+    def waypoints(self, a, b) -> np.ndarray:
+        a, b = _f64(a), _f64(b)
+        n = self.L.orc_link_waypoints(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double), None, 0)
+        out = np.zeros((max(n, 1), self.dim))
+        self.L.orc_link_waypoints(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double), _p(out, C.c_double), n)
+        return out[:n]
+
+    def link(self, a, b, known_valid=False, threads=8):
+        a = _f64(a).reshape(-1, self.dim)
+        b = _f64(b).reshape(-1, self.dim)
+        n = a.shape[0]
+        v = np.zeros(n, dtype=np.uint8)
+        c = np.zeros(n, dtype=np.uint32)
+        self.L.orc_link_batch(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double), n, 1 if known_valid else 0,
+                              _p(v, C.c_uint8), _p(c, C.c_uint32), threads)
+        return v.astype(bool), c
+
+    def checks_in_score_order(self, a, b, scores, known_valid=False) -> int:
+        a, b = _f64(a), _f64(b)
+        sc = np.ascontiguousarray(scores, dtype=np.float32)
+        return int(self.L.orc_checks_in_score_order(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double),
+                                                    1 if known_valid else 0, _p(sc, C.c_float), sc.shape[0]))
+
+    def _params(self, seed, batch, range_=None, goal_bias=None):
+        sc = self.scene
+        lo, hi, g = _f64(sc.lo), _f64(sc.hi), _f64(sc.goal)
+        self._keep += [lo, hi, g]
+        return OrcPrrtParams(seed=seed, range=sc.range if range_ is None else range_,
+                             goal_bias=sc.goal_bias if goal_bias is None else goal_bias,
+                             goal_radius=sc.goal_radius, batch=batch, lo=_p(lo, C.c_double),
+                             hi=_p(hi, C.c_double), goal=_p(g, C.c_double))
+
+    def prrt(self, seed, batch, target, max_iters=1 << 40, range_=None, goal_bias=None):
+        p = self._params(seed, batch, range_, goal_bias)
+        cap = target + batch + 8
+        nodes = np.zeros((cap, self.dim))
+        parents = np.zeros(cap, dtype=np.int32)
+        goals = np.zeros(4096, dtype=np.int32)
+        st = OrcPrrtStats()
+        start = _f64(self.scene.start)
+        n = self.L.orc_prrt_run(C.byref(self.s), C.byref(p), _p(start, C.c_double), target, max_iters,
+                                _p(nodes, C.c_double), _p(parents, C.c_int32), cap, _p(goals, C.c_int32), 4096,
+                                C.byref(st))
+        if n < 0:
+            raise RuntimeError("oracle capacity exceeded")
+        stats = {f: int(getattr(st, f)) for f, _ in st._fields_}
+        return nodes[:n].copy(), parents[:n].copy(), goals[: min(stats["goals"], 4096)].copy(), stats
+
+    def pool(self, seed, target, seconds, threads, range_=None, goal_bias=None):
+        p = self._params(seed, 1, range_, goal_bias)
+        r = OrcPoolResult()
+        start = _f64(self.scene.start)
+        self.L.orc_pool_run(C.byref(self.s), C.byref(p), _p(start, C.c_double), target, seconds, threads, C.byref(r))
+        return {f: getattr(r, f) for f, _ in r._fields_}
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox(_p(c, C.c_uint32), _p(k, C.c_uint32), _p(o, C.c_uint32))
+    return o
+
+
+def sincos(x):
+    x = _f64(x)
+    s = np.zeros_like(x)
+    c = np.zeros_like(x)
+    lib().orc_sincos(_p(x, C.c_double), x.size, _p(s, C.c_double), _p(c, C.c_double))
+    return s, c
+
+
+def acos(x):
+    x = _f64(x)
+    y = np.zeros_like(x)
+    lib().orc_acos(_p(x, C.c_double), x.size, _p(y, C.c_double))
+    return y
+
+
+def dmv_order(steps: int) -> np.ndarray:
+    n = max(steps - 1, 0)
+    out = np.zeros(max(n, 1), dtype=np.int64)
+    m = lib().orc_dmv_order(steps, _p(out, C.c_int64), n)
+    return out[:m]
+
+
+def uniform_at(seed, index, purpose, j) -> float:
+    return lib().orc_uniform_at(seed, index, purpose, j)
