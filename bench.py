@@ -142,8 +142,8 @@ def bench_nn(ctx, args, timer):
     n = 1_000_000
     pts = rng.uniform(-math.pi, math.pi, (n, 7))
     q = np.random.default_rng(3).uniform(-math.pi, math.pi, (65536, 7))
-    ctx.space(scenes.METRIC_L2, 7)
     ctx.tree_clear()
+    ctx.space(scenes.METRIC_L2, 7)
     ctx.tree_insert(pts)
     import torch
     dq = dev(q)
@@ -208,8 +208,9 @@ def bench_boxes(ctx, args):
 def cpu_baseline(sc, seconds):
     import oracle
     o = oracle.Oracle(sc)
-    threads = os.cpu_count() or 1
-    r = o.pool(seed=7, target=10 ** 9, seconds=seconds, threads=threads)
+    # the box's CPU share is 16 cores whatever os.cpu_count() reports
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    r = o.pool(seed=7, target=200_000, seconds=seconds, threads=threads)
     return {"value": r["link_calls"] / r["seconds"], "unit": "edge checks/s", "cores": threads, "kind": "port",
             "sample": f"std::thread worker pool (reference loop restated), {threads} threads, arm scene, "
                       f"{r['seconds']:.1f} s wall, {r['link_calls']} link() calls, tree of {r['size']} nodes",

@@ -97,6 +97,7 @@ class Context:
 
     # ---- space / tree
     def space(self, metric: int, dim: int, w0: float = 1.0, w1: float = 1.0):
+        """Sets the metric space; the tree must be empty to change it."""
         self._chk(_lib.mptg_space(self._h, metric, dim, w0, w1))
         self.dim = dim
 
@@ -139,7 +140,9 @@ class Context:
 
     # ---- scene
     def scene(self, sc: Scene):
-        """Uploads a Scene: space, valid() model and link() form."""
+        """Uploads a Scene: space, valid() model and link() form.  The tree is
+        cleared first (a scene is a new planning problem)."""
+        self.tree_clear()
         self.space(sc.metric, sc.dim, sc.w_so3, sc.w_l2)
         if sc.kind == SCENE_BOXES:
             b = _f64(sc.boxes, 4)

@@ -31,7 +31,7 @@ ARCH = "gfx950"
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-fPIC",
     "-ffp-contract=off",
     # packed FP32 VALU ops are kept out of every kernel
@@ -44,8 +44,8 @@ HIP_FLAGS = [
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall"]
 
-LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "diag.hip"]
-ORACLE_SOURCES = ["orc_core.cpp", "orc_prrt.cpp", "orc_pool.cpp"]
+LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "diag.hip", "arm.hip"]
+ORACLE_SOURCES = ["orc_core.cpp", "orc_prrt.cpp", "orc_pool.cpp", "orc_mlp.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -125,6 +125,8 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.grid);
     (void)hipFree(ctx->scene.lengths);
     (void)hipFree(ctx->scene.disks);
+    (void)hipFree(ctx->scene.arm_disks.p);
+    (void)hipFree(ctx->scene.arm_lim.p);
     for (auto& b : ctx->scratch) (void)hipFree(b.p);
     prrt_free(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -303,6 +305,24 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
     if (rc) return rc;
     rc = upload(ctx, (void**)&ctx->scene.disks, dk.data(), sizeof(double) * 3 * ndisks);
     if (rc) return rc;
+    // kernel layout: [cx, cy, (r+R)^2, r+R] and a per-link segment-length
+    // bound L (1 + 2^-30) + 1e-12 for the far-disk skip
+    std::vector<double> ad((size_t)std::max(1, ndisks) * 4, 0.0), lim((size_t)nlinks);
+    for (int i = 0; i < ndisks; ++i) {
+        double rt = h_disks[3 * i + 2] + robot_radius;
+        ad[4 * i + 0] = h_disks[3 * i + 0];
+        ad[4 * i + 1] = h_disks[3 * i + 1];
+        ad[4 * i + 2] = rt * rt;
+        ad[4 * i + 3] = rt;
+    }
+    for (int i = 0; i < nlinks; ++i) lim[i] = h_lengths[i] * (1.0 + 0x1.0p-30) + 1e-12;
+    if ((rc = ensure(ctx, ctx->scene.arm_disks, sizeof(double) * ad.size()))) return rc;
+    if ((rc = ensure(ctx, ctx->scene.arm_lim, sizeof(double) * lim.size()))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scene.arm_disks.p, ad.data(), sizeof(double) * ad.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scene.arm_lim.p, lim.data(), sizeof(double) * lim.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->scene.kind = MPTG_SCENE_ARM;
     ctx->scene.nlinks = nlinks;
     ctx->scene.ndisks = ndisks;

@@ -40,6 +40,16 @@ MPTG_HD float float_of(uint32_t u) {
     return x;
 }
 
+// BF16 <-> FP32, round to nearest even (NaN stays NaN).
+MPTG_HD uint16_t float_to_bf16(float f) {
+    uint32_t u = bits_of(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+MPTG_HD float bf16_to_float(uint16_t h) { return float_of((uint32_t)h << 16); }
+
 // ---------------------------------------------------------------------------
 // Philox-4x32-10 counter-based RNG (Salmon, Moraes, Dror, Shaw: "Parallel
 // random numbers: as easy as 1, 2, 3", SC'11).  Multipliers and Weyl key

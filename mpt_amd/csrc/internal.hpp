@@ -35,6 +35,7 @@ struct SceneDev {
     double robot_radius = 0;
     double* disks = nullptr;  // [cx, cy, (r+R)^2] per disk
     int ndisks = 0;
+    DevBuf arm_disks, arm_lim;  // kernel layout of the disks, per-link bounds
 };
 
 struct PrrtState {
@@ -82,6 +83,7 @@ struct mptg_ctx {
     size_t pinned_bytes = 0;
 
     mptg::PrrtState prrt;
+    bool force_generic = false;  // route link() to the per-edge reference kernels (testing)
 };
 
 namespace mptg {
@@ -117,5 +119,7 @@ int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t
 int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                        int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
 int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
+int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
+                    const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr);
 
 }  // namespace mptg

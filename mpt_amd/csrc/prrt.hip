@@ -51,15 +51,17 @@ __global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int D, uint64_
     }
 }
 
+template <int D>
 __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc, double range, int B,
                                                      const double* __restrict__ x64, const double* __restrict__ q,
                                                      const int32_t* __restrict__ near,
                                                      const double* __restrict__ ndist, double* __restrict__ from,
                                                      double* __restrict__ to, uint8_t* __restrict__ status,
                                                      uint8_t* __restrict__ linkmask) {
+    constexpr int DA = D > 0 ? D : kMaxDim;
+    const int dd = D > 0 ? D : sp.dim;
     int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= B) return;
-    const int D = sp.dim;
     double d = ndist[b];
     int32_t nn = near[b];
     linkmask[b] = 0;
@@ -67,19 +69,27 @@ __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc,
         status[b] = ST_ZERO;
         return;
     }
-    double a[kMaxDim], r[kMaxDim], s[kMaxDim];
-    for (int i = 0; i < D; ++i) {
-        a[i] = x64[(int64_t)nn * D + i];
-        r[i] = q[(int64_t)b * D + i];
+    double a[DA], r[DA], s[DA];
+#pragma unroll
+    for (int i = 0; i < DA; ++i)
+        if (i < dd) {
+            a[i] = x64[(int64_t)nn * dd + i];
+            r[i] = q[(int64_t)b * dd + i];
+        }
+    if (d > range) {
+        space_interp_t<D>(sp, a, r, range / d, s);
+    } else {
+#pragma unroll
+        for (int i = 0; i < DA; ++i)
+            if (i < dd) s[i] = r[i];
     }
-    if (d > range) space_interp(sp, a, r, range / d, s);
-    else
-        for (int i = 0; i < D; ++i) s[i] = r[i];
-    for (int i = 0; i < D; ++i) {
-        from[(int64_t)b * D + i] = a[i];
-        to[(int64_t)b * D + i] = s[i];
-    }
-    if (!state_valid(sc, s)) {
+#pragma unroll
+    for (int i = 0; i < DA; ++i)
+        if (i < dd) {
+            from[(int64_t)b * dd + i] = a[i];
+            to[(int64_t)b * dd + i] = s[i];
+        }
+    if (!state_valid_t<D>(sc, s)) {
         status[b] = ST_INVALID;
         return;
     }
@@ -312,10 +322,12 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
         if ((rc = launch_knn(ctx, (const double*)pb.q.p, B, 1, __builtin_inf(), (int32_t*)pb.near.p,
                              (double*)pb.ndist.p, ctx->size)))
             return rc;
-        hipLaunchKernelGGL(prrt_steer, dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx), scene_args(ctx),
-                           s.p.range, B, (const double*)ctx->x64, (const double*)pb.q.p, (const int32_t*)pb.near.p,
-                           (const double*)pb.ndist.p, (double*)pb.from.p, (double*)pb.to.p, (uint8_t*)pb.status.p,
-                           (uint8_t*)pb.mask.p);
+        dispatch_dim(D, [&]<int DD>() {
+            hipLaunchKernelGGL((prrt_steer<DD>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx),
+                               scene_args(ctx), s.p.range, B, (const double*)ctx->x64, (const double*)pb.q.p,
+                               (const int32_t*)pb.near.p, (const double*)pb.ndist.p, (double*)pb.from.p,
+                               (double*)pb.to.p, (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p);
+        });
         MPTG_HIP(ctx, hipGetLastError());
         if ((rc = launch_link_masked(ctx, (const double*)pb.from.p, (const double*)pb.to.p, B, s.p.order, 1,
                                      (const uint8_t*)pb.mask.p, (uint8_t*)pb.verdict.p, (uint32_t*)pb.checks.p)))

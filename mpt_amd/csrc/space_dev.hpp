@@ -18,6 +18,12 @@ struct SpaceArgs {
     double w_so3, w_l2;
 };
 
+// Disk record of the arm scene: centre, (r+R)^2 exactly as the reference
+// computes it, and r+R for the far-disk bound.
+struct ArmDisk {
+    double cx, cy, rr2, rt;
+};
+
 struct SceneArgs {
     int kind;
     const double* boxes;
@@ -28,13 +34,17 @@ struct SceneArgs {
     int nlinks;
     const double* disks;  // [cx, cy, (r + R)^2]
     int ndisks;
+    const ArmDisk* arm_disks;  // same disks, kernel layout (built by arm_prepare)
+    const double* arm_lim;     // per-link segment-length bound
 };
 
 inline SpaceArgs space_args(const mptg_ctx* ctx) { return SpaceArgs{ctx->metric, ctx->dim, ctx->w_so3, ctx->w_l2}; }
 
 inline SceneArgs scene_args(const mptg_ctx* ctx) {
     const SceneDev& s = ctx->scene;
-    return SceneArgs{s.kind, s.boxes, s.nboxes, s.grid, s.width, s.height, s.lengths, s.nlinks, s.disks, s.ndisks};
+    return SceneArgs{s.kind,    s.boxes,  s.nboxes, s.grid,   s.width,
+                     s.height,  s.lengths, s.nlinks, s.disks, s.ndisks,
+                     (const ArmDisk*)s.arm_disks.p, (const double*)s.arm_lim.p};
 }
 
 __device__ __forceinline__ double so3_dist(const double* a, const double* b) {
@@ -163,6 +173,130 @@ __device__ __forceinline__ bool grid_valid(const SceneArgs& sc, const double* q)
     int x = (int)fx, y = (int)fy;
     if (x < 0 || y < 0 || x >= sc.width || y >= sc.height) return false;
     return sc.grid[(int64_t)sc.width * y + x] == 0;
+}
+
+// distPointSegmentSquared(c, s0, s1) > (r+R)^2, skipping disks whose centre
+// is farther than lim + r + R from the segment start.  lim bounds the FP
+// segment length with a relative margin of 2^-30 that dominates the rounding
+// of the exact expression, so a skipped disk would have passed the exact
+// test; every other disk gets the reference expression bit-for-bit.
+__device__ __forceinline__ bool disk_clear(const ArmDisk& dk, double fx, double fy, double tx, double ty, double vx,
+                                           double vy, double lim) {
+    double wx = dk.cx - fx, wy = dk.cy - fy;
+    double w2 = wx * wx + wy * wy;
+    double far = lim + dk.rt;
+    if (w2 > far * far) return true;
+    double c1 = vx * wx + vy * wy;
+    double d;
+    if (c1 <= 0) {
+        d = w2;
+    } else {
+        double c2 = vx * vx + vy * vy;
+        if (c2 <= c1) {
+            double ex = dk.cx - tx, ey = dk.cy - ty;
+            d = ex * ex + ey * ey;
+        } else {
+            double b = c1 / c2;
+            double ex = (fx - dk.cx) + vx * b;
+            double ey = (fy - dk.cy) + vy * b;
+            d = ex * ex + ey * ey;
+        }
+    }
+    return d > dk.rr2;
+}
+
+// Arm validity for a state of D coordinates (nlinks <= D): same verdict as
+// arm_valid, registers only.  coord(i) yields the state's i-th coordinate, so
+// callers can produce coordinates on the fly instead of holding the state.
+template <int D, typename Coord>
+__device__ __forceinline__ bool arm_ok_fn(const SceneArgs& sc, Coord&& coord) {
+    double fx = 0.0, fy = 0.0, angle = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        if (i >= sc.nlinks) break;
+        angle += coord(i);
+        double s, c;
+        pm::sincos(angle, &s, &c);
+        double L = sc.lengths[i];
+        double tx = fx + L * c;
+        double ty = fy + L * s;
+        double vx = tx - fx, vy = ty - fy;
+        double li = sc.arm_lim[i];
+        for (int j = 0; j < sc.ndisks; ++j)
+            if (!disk_clear(sc.arm_disks[j], fx, fy, tx, ty, vx, vy, li)) return false;
+        fx = tx;
+        fy = ty;
+    }
+    return true;
+}
+
+template <int D>
+__device__ __forceinline__ bool arm_ok(const SceneArgs& sc, const double* q) {
+    return arm_ok_fn<D>(sc, [&](int i) { return q[i]; });
+}
+
+// Validity with the state size known at compile time (D > 0) or at run time
+// (D == 0, generic fallback).
+template <int D>
+__device__ __forceinline__ bool state_valid_t(const SceneArgs& sc, const double* q) {
+    if constexpr (D > 0) {
+        if (sc.kind == MPTG_SCENE_ARM) return arm_ok<D>(sc, q);
+    }
+    switch (sc.kind) {
+        case MPTG_SCENE_BOXES: return boxes_valid(sc, q);
+        case MPTG_SCENE_GRID: return grid_valid(sc, q);
+        case MPTG_SCENE_ARM: return arm_valid(sc, q);
+        default: return true;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ double space_dist_t(const SpaceArgs& sp, const double* a, const double* b) {
+    if constexpr (D > 0) {
+        if (sp.metric != MPTG_METRIC_SE3) {
+            double s = 0.0;
+            if (sp.metric == MPTG_METRIC_L1) {
+#pragma unroll
+                for (int i = 0; i < D; ++i) s = s + __builtin_fabs(a[i] - b[i]);
+                return s;
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                double t = a[i] - b[i];
+                s = s + t * t;
+            }
+            return __builtin_sqrt(s);
+        }
+    }
+    return space_dist(sp, a, b);
+}
+
+template <int D>
+__device__ __forceinline__ void space_interp_t(const SpaceArgs& sp, const double* a, const double* b, double t,
+                                               double* q) {
+    if constexpr (D > 0) {
+        if (sp.metric != MPTG_METRIC_SE3) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) q[i] = (b[i] - a[i]) * t + a[i];
+            return;
+        }
+    }
+    space_interp(sp, a, b, t, q);
+}
+
+// Runs f.template operator()<D>() with D the compile-time state size, or
+// D = 0 (run-time size) for dimensions without an instantiation.
+template <typename F>
+inline void dispatch_dim(int dim, F&& f) {
+    switch (dim) {
+        case 2: f.template operator()<2>(); break;
+        case 3: f.template operator()<3>(); break;
+        case 5: f.template operator()<5>(); break;
+        case 7: f.template operator()<7>(); break;
+        case 10: f.template operator()<10>(); break;
+        case 20: f.template operator()<20>(); break;
+        default: f.template operator()<0>(); break;
+    }
 }
 
 __device__ __forceinline__ bool state_valid(const SceneArgs& sc, const double* q) {

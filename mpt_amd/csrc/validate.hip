@@ -38,20 +38,26 @@ constexpr int kTableMax = 255;  // largest n = steps-1 served by the BFS table
 constexpr int64_t kMaxDmvWaypoints = (int64_t)1 << 24;
 constexpr int kMaxBisectDepth = 30;
 
+template <int D>
 __global__ __launch_bounds__(kBlock) void valid_kernel(SceneArgs sc, int dim, const double* __restrict__ q,
                                                        int64_t n, uint8_t* __restrict__ ok) {
+    constexpr int DA = D > 0 ? D : kMaxDim;
+    const int dd = D > 0 ? D : dim;
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    double s[kMaxDim];
-    for (int j = 0; j < dim; ++j) s[j] = q[i * dim + j];
-    ok[i] = state_valid(sc, s) ? 1 : 0;
+    double s[DA];
+#pragma unroll
+    for (int j = 0; j < DA; ++j)
+        if (j < dd) s[j] = q[i * dd + j];
+    ok[i] = state_valid_t<D>(sc, s) ? 1 : 0;
 }
 
 // steps = ceil(d * inv) as the reference's size_t; returns n = steps - 1
 // (number of interior waypoints), or 0 when steps < 2.
+template <int D>
 __device__ __forceinline__ int64_t dmv_count(const SpaceArgs& sp, const double* a, const double* b, double inv,
                                              double* delta) {
-    double x = __builtin_ceil(space_dist(sp, a, b) * inv);
+    double x = __builtin_ceil(space_dist_t<D>(sp, a, b) * inv);
     if (!(x >= 2.0)) return 0;
     if (!(x < 0x1.0p62)) x = 0x1.0p62;
     int64_t steps = (int64_t)x;
@@ -59,6 +65,7 @@ __device__ __forceinline__ int64_t dmv_count(const SpaceArgs& sp, const double* 
     return steps - 1;
 }
 
+template <int D>
 __global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs sc, double inv,
                                                          const uint8_t* __restrict__ table,
                                                          const double* __restrict__ from,
@@ -67,21 +74,24 @@ __global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs
                                                          uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks,
                                                          int64_t* __restrict__ long_list,
                                                          uint32_t* __restrict__ long_count) {
+    constexpr int DA = D > 0 ? D : kMaxDim;
+    const int dd = D > 0 ? D : sp.dim;
     int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (e >= n || (mask && !mask[e])) return;
-    const int D = sp.dim;
-    double a[kMaxDim], b[kMaxDim], q[kMaxDim];
-    for (int j = 0; j < D; ++j) {
-        a[j] = from[e * D + j];
-        b[j] = to[e * D + j];
-    }
-    if (!known_valid && !state_valid(sc, b)) {
+    double a[DA], b[DA], q[DA];
+#pragma unroll
+    for (int j = 0; j < DA; ++j)
+        if (j < dd) {
+            a[j] = from[e * dd + j];
+            b[j] = to[e * dd + j];
+        }
+    if (!known_valid && !state_valid_t<D>(sc, b)) {
         verdict[e] = 0;
         checks[e] = 1;
         return;
     }
     double delta = 0.0;
-    int64_t m = dmv_count(sp, a, b, inv, &delta);
+    int64_t m = dmv_count<D>(sp, a, b, inv, &delta);
     if (m > kMaxDmvWaypoints) {
         verdict[e] = 0;
         checks[e] = MPTG_CHECKS_OVER_LIMIT;
@@ -95,8 +105,8 @@ __global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs
     const uint8_t* ord = table + (m * (m - 1)) / 2;
     for (int64_t o = 0; o < m; ++o) {
         int64_t i = ord[o];
-        space_interp(sp, a, b, (double)i * delta, q);
-        if (!state_valid(sc, q)) {
+        space_interp_t<D>(sp, a, b, (double)i * delta, q);
+        if (!state_valid_t<D>(sc, q)) {
             verdict[e] = 0;
             checks[e] = (uint32_t)(2 + o);
             return;
@@ -108,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs
 
 // One wave per long edge: lane 0 replays the reference's ring in LDS and
 // emits up to 64 waypoint indices per round; all lanes check them.
+template <int D>
 __global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, double inv,
                                                     const double* __restrict__ from, const double* __restrict__ to,
                                                     const int64_t* __restrict__ long_list,
@@ -120,15 +131,18 @@ __global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, 
     const uint32_t total = *long_count;
     const int lane = threadIdx.x;
     for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+        constexpr int DA = D > 0 ? D : kMaxDim;
+        const int dd = D > 0 ? D : sp.dim;
         const int64_t e = long_list[w];
-        const int D = sp.dim;
-        double a[kMaxDim], b[kMaxDim], q[kMaxDim];
-        for (int j = 0; j < D; ++j) {
-            a[j] = from[e * D + j];
-            b[j] = to[e * D + j];
-        }
+        double a[DA], b[DA], q[DA];
+#pragma unroll
+        for (int j = 0; j < DA; ++j)
+            if (j < dd) {
+                a[j] = from[e * dd + j];
+                b[j] = to[e * dd + j];
+            }
         double delta = 0.0;
-        int64_t m = dmv_count(sp, a, b, inv, &delta);
+        int64_t m = dmv_count<D>(sp, a, b, inv, &delta);
         // ring state lives in lane 0's registers
         uint64_t qs = 0, qe = 1, seq_cur = 0, seq_end = 0;
         bool seq = false;
@@ -179,8 +193,8 @@ __global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, 
             if (c == 0) break;
             bool bad = false;
             if (lane < c) {
-                space_interp(sp, a, b, (double)emitted[lane] * delta, q);
-                bad = !state_valid(sc, q);
+                space_interp_t<D>(sp, a, b, (double)emitted[lane] * delta, q);
+                bad = !state_valid_t<D>(sc, q);
             }
             if (bad) atomicMin(&fail_lane, lane);
             __syncthreads();
@@ -225,12 +239,9 @@ __device__ __forceinline__ bool bisect_internal(int D, const double* lo, const d
     return !(m < tol);
 }
 
-__global__ __launch_bounds__(kBlock) void link_bisect(int D, SceneArgs sc, double tol,
-                                                      const double* __restrict__ from, const double* __restrict__ to,
-                                                      int64_t n, int known_valid, const uint8_t* __restrict__ mask,
-                                                      uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks) {
-    int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n || (mask && !mask[e])) return;
+__device__ void bisect_edge(int D, const SceneArgs& sc, double tol, const double* __restrict__ from,
+                            const double* __restrict__ to, int64_t e, int known_valid, uint8_t* __restrict__ verdict,
+                            uint32_t* __restrict__ checks) {
     double a[kMaxDim], b[kMaxDim], lo[kMaxDim], hi[kMaxDim];
     for (int j = 0; j < D; ++j) {
         a[j] = from[e * D + j];
@@ -277,6 +288,28 @@ __global__ __launch_bounds__(kBlock) void link_bisect(int D, SceneArgs sc, doubl
     }
     verdict[e] = 1;
     checks[e] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void link_bisect(int D, SceneArgs sc, double tol,
+                                                      const double* __restrict__ from, const double* __restrict__ to,
+                                                      int64_t n, int known_valid, const uint8_t* __restrict__ mask,
+                                                      uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks) {
+    int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n || (mask && !mask[e])) return;
+    bisect_edge(D, sc, tol, from, to, e, known_valid, verdict, checks);
+}
+
+// Exact walk of the edges the wave kernel listed (grid-stride over the list).
+__global__ __launch_bounds__(kBlock) void link_bisect_list(int D, SceneArgs sc, double tol,
+                                                           const double* __restrict__ from,
+                                                           const double* __restrict__ to, int known_valid,
+                                                           const int64_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ list_count,
+                                                           uint8_t* __restrict__ verdict,
+                                                           uint32_t* __restrict__ checks) {
+    const int64_t cnt = *list_count;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * kBlock)
+        bisect_edge(D, sc, tol, from, to, list[i], known_valid, verdict, checks);
 }
 
 // Instrumented bisection link (not timed): same walk and verdicts, and
@@ -366,8 +399,10 @@ int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
     if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > ctx->dim)
         return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
     if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
-    hipLaunchKernelGGL(valid_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
-                       scene_args(ctx), ctx->dim, d_q, n, d_ok);
+    dispatch_dim(ctx->dim, [&]<int D>() {
+        hipLaunchKernelGGL((valid_kernel<D>), dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, scene_args(ctx), ctx->dim, d_q, n, d_ok);
+    });
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }
@@ -388,6 +423,23 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     if (n > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 edges per call");
     unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     if (ctx->link_mode == MPTG_LINK_BISECT) {
+        if (ctx->scene.kind == MPTG_SCENE_ARM && !ctx->force_generic) {
+            int rc;
+            if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
+            uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
+            int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
+            MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
+            rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks, irr, nirr);
+            if (rc == MPTG_OK) {
+                // listed (irregular) edges: exact per-edge walk; usually an empty launch
+                hipLaunchKernelGGL(link_bisect_list, dim3(16), dim3(kBlock), 0, ctx->stream, ctx->dim,
+                                   scene_args(ctx), ctx->link_param, d_from, d_to, known_valid, (const int64_t*)irr,
+                                   (const uint32_t*)nirr, d_verdict, d_checks);
+                MPTG_HIP(ctx, hipGetLastError());
+                return MPTG_OK;
+            }
+            if (rc != MPTG_E_STATE) return rc;  // E_STATE: no wave kernel for this arm, use the generic one
+        }
         hipLaunchKernelGGL(link_bisect, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->dim, scene_args(ctx),
                            ctx->link_param, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks);
         MPTG_HIP(ctx, hipGetLastError());
@@ -400,13 +452,14 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     int64_t* long_list = (int64_t*)((char*)ctx->scratch[7].p + 256);
     MPTG_HIP(ctx, hipMemsetAsync(long_count, 0, sizeof(uint32_t), ctx->stream));
     double inv = 1.0 / ctx->link_param;
-    hipLaunchKernelGGL(link_dmv_short, dim3(blocks), dim3(kBlock), 0, ctx->stream, space_args(ctx), scene_args(ctx),
-                       inv, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks,
-                       long_list, long_count);
-    MPTG_HIP(ctx, hipGetLastError());
     unsigned lg = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 16);
-    hipLaunchKernelGGL(link_dmv_long, dim3(lg), dim3(64), 0, ctx->stream, space_args(ctx), scene_args(ctx), inv,
-                       d_from, d_to, long_list, long_count, d_verdict, d_checks);
+    dispatch_dim(ctx->dim, [&]<int D>() {
+        hipLaunchKernelGGL((link_dmv_short<D>), dim3(blocks), dim3(kBlock), 0, ctx->stream, space_args(ctx),
+                           scene_args(ctx), inv, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n, known_valid, d_mask,
+                           d_verdict, d_checks, long_list, long_count);
+        hipLaunchKernelGGL((link_dmv_long<D>), dim3(lg), dim3(64), 0, ctx->stream, space_args(ctx), scene_args(ctx),
+                           inv, d_from, d_to, long_list, long_count, d_verdict, d_checks);
+    });
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

@@ -171,7 +171,8 @@ def test_link_parity_grid_long_edges(ctx_factory, grid1024):
     gv, gc = ctx.link(a, b)
     ov, oc = oracle.Oracle(sc).link(a, b, threads=16)
     assert np.array_equal(gv, ov) and np.array_equal(gc, oc)
-    assert (oc > 257).sum() > 50  # the long-edge kernel was exercised
+    # edges with more than 255 interior waypoints go to the ring-replay kernel
+    assert (np.linalg.norm(b - a, axis=1) > 300).sum() > 200
 
 
 def test_link_parity_arm(ctx_factory):
@@ -183,6 +184,15 @@ def test_link_parity_arm(ctx_factory):
     a = rng.uniform(-math.pi, math.pi, (3000, 20))
     b = np.clip(a + rng.normal(0, 0.5, (3000, 20)), -math.pi, math.pi)
     b[-500:] = rng.uniform(-math.pi, math.pi, (500, 20))
+    # widths of exactly tol * 2^k (and within rounding of it) sit on the
+    # regular-tree test's boundary and take the exact per-edge walk
+    for i, k in enumerate(range(1, 8)):
+        for j, base in enumerate((0.0, 1.0, -2.5)):
+            r = 2000 + 10 * i + j
+            a[r] = np.clip(rng.normal(0, 0.2, 20), -1, 1)
+            b[r] = a[r] + rng.uniform(-0.01, 0.01, 20)
+            a[r, 3] = base
+            b[r, 3] = base + 0.02 * 2 ** k
     for kv in (False, True):
         gv, gc = ctx.link(a, b, endpoints_known_valid=kv)
         ov, oc = oracle.Oracle(sc).link(a, b, known_valid=kv, threads=16)
