@@ -31,9 +31,12 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK = 8.0e12        # B/s
 FP64_PEAK = 78.6e12      # FLOP/s (vector and matrix)
 FP32_PEAK = 157.3e12     # FLOP/s
-# algorithmic FLOP prices (see the roofline section of the design notes)
-FLOP_FK_LINK = 44        # one pm::sincos + 2 mul + 2 add
-FLOP_SEG_TEST = 20       # distPointSegmentSquared + compare
+# FP64 FLOP prices of the arm check's units of work
+FLOP_FK_LINK = 50        # angle add, pm::sincos (reduction + both kernels), 2 mul + 2 add
+FLOP_GROUP = 8           # group circle test: 2 sub, 3 mul, 2 add
+FLOP_LINK_FAR = 10       # link test settled by the far bound
+FLOP_LINK_EXACT = 24     # distPointSegmentSquared incl. the division
+FLOP_REF_TEST = 20       # reference: one segment-disk test
 
 
 def log(*a):
@@ -41,27 +44,13 @@ def log(*a):
 
 
 class Timer:
-    """HIP-event timing on the context's own stream."""
+    """HIP-event timing on the context's own stream (seconds per call)."""
 
     def __init__(self, ctx):
-        import torch
-        self.torch = torch
-        self.stream = torch.cuda.ExternalStream(ctx.stream)
+        self.ctx = ctx
 
     def time(self, fn, reps: int) -> float:
-        t = self.torch
-        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
-        e0.record(self.stream)
-        for _ in range(reps):
-            fn()
-        e1.record(self.stream)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / 1e3 / reps
-
-
-def dev(a, dtype="float64"):
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+        return self.ctx.time_ms(fn, reps) / 1e3
 
 
 def arm_edges(ctx, sc, tree_states, n, seed):
@@ -110,29 +99,31 @@ def bench_arm(ctx, args, timer):
     }
     # dominant kernel: link() over a planner-like edge batch
     states, _ = ctx.tree_read()
-    ne = 8192 if not args.quick else 2048
+    ne = 8192
     a, b = arm_edges(ctx, sc, states, ne, seed=99)
-    da, db = dev(a), dev(b)
-    import torch
-    v = torch.zeros(ne, dtype=torch.uint8, device="cuda")
-    c = torch.zeros(ne, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    v = ctx.array(ne, np.uint8)
+    c = ctx.array(ne, np.uint32)
 
     def run():
-        ctx.link_dev(da.data_ptr(), db.data_ptr(), ne, mpt_amd.ORDER_REFERENCE, True, v.data_ptr(), c.data_ptr())
+        ctx.link_dev(da.ptr, db.ptr, ne, mpt_amd.ORDER_REFERENCE, True, v.ptr, c.ptr)
 
     run()
     ctx.sync()
     t = timer.time(run, 3)
-    calls, links, tests = ctx.link_work(da.data_ptr(), db.data_ptr(), ne, True)
-    flops = FLOP_FK_LINK * links + FLOP_SEG_TEST * tests
+    wk = ctx.link_work(da.ptr, db.ptr, ne, True)
+    flops = (FLOP_FK_LINK * wk["fk"] + FLOP_GROUP * wk["group"] + FLOP_LINK_FAR * wk["far"]
+             + FLOP_LINK_EXACT * wk["exact"])
+    ref_flops = FLOP_FK_LINK * wk["ref_fk"] + FLOP_REF_TEST * wk["ref_tests"]
+    calls = wk["calls"]
     res["edge_batch"] = {
         "edges": ne, "kernel_ms": t * 1e3, "edge_checks_per_s": ne / t, "valid_calls": calls,
-        "colliding_frac": float(1 - v.float().mean().item()),
+        "colliding_frac": float(1 - v.numpy().mean()),
     }
     roof = {"bound": "fp64", "achieved": flops / t / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": flops / t / FP64_PEAK, "traffic": None, "kernel": "link_bisect",
-            "flops_per_launch": flops}
+            "frac": flops / t / FP64_PEAK, "traffic": None, "kernel": "arm_link_wave<20>",
+            "flops_per_launch": flops, "work": wk,
+            "reference_equivalent_tflops": ref_flops / t / 1e12}
     return res, roof, sc
 
 
@@ -145,15 +136,14 @@ def bench_nn(ctx, args, timer):
     ctx.tree_clear()
     ctx.space(scenes.METRIC_L2, 7)
     ctx.tree_insert(pts)
-    import torch
-    dq = dev(q)
+    dq = ctx.to_device(q)
     out = {}
     for k in (1, 20):
-        di = torch.zeros((65536, k), dtype=torch.int32, device="cuda")
-        dd = torch.zeros((65536, k), dtype=torch.float64, device="cuda")
+        di = ctx.array((65536, k), np.int32)
+        dd = ctx.array((65536, k), np.float64)
 
         def run():
-            ctx.knn_dev(dq.data_ptr(), 65536, k, math.inf, di.data_ptr(), dd.data_ptr())
+            ctx.knn_dev(dq.ptr, 65536, k, math.inf, di.ptr, dd.ptr)
 
         run()
         ctx.sync()
@@ -167,22 +157,21 @@ def bench_nn(ctx, args, timer):
 def bench_grid(ctx, args, timer):
     from mpt_amd import scenes
     import mpt_amd
-    import torch
     sc = scenes.grid_scene(size=4096, seed=4)
     ctx.scene(sc)
     n = 1 << 20
     a, b = scenes.grid_edges(sc, n, seed=5)
-    da, db = dev(a), dev(b)
-    v = torch.zeros(n, dtype=torch.uint8, device="cuda")
-    c = torch.zeros(n, dtype=torch.int32, device="cuda")
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    v = ctx.array(n, np.uint8)
+    c = ctx.array(n, np.uint32)
 
     def run():
-        ctx.link_dev(da.data_ptr(), db.data_ptr(), n, mpt_amd.ORDER_REFERENCE, False, v.data_ptr(), c.data_ptr())
+        ctx.link_dev(da.ptr, db.ptr, n, mpt_amd.ORDER_REFERENCE, False, v.ptr, c.ptr)
 
     run()
     ctx.sync()
     t = timer.time(run, 5)
-    w = int(c.to(torch.int64).sum().item())
+    w = int(c.numpy().astype(np.int64).sum())
     byts = n * (32 + 1) + w
     return {"edges": n, "ms": t * 1e3, "edge_checks_per_s": n / t, "valid_calls": w,
             "algorithmic_bytes": byts, "achieved_gbs": byts / t / 1e9, "frac_hbm": byts / t / HBM_PEAK,
@@ -229,7 +218,6 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
-    import torch  # noqa: F401  (HIP events, device buffers)
     import mpt_amd
     ctx = mpt_amd.Context(0)
     timer = Timer(ctx)

@@ -70,6 +70,19 @@ const char* mptg_last_error(const mptg_ctx* ctx);
 int mptg_sync(mptg_ctx* ctx);
 void* mptg_stream(mptg_ctx* ctx); /* the context's hipStream_t */
 
+/* ---- device memory and timing on the context's device/stream (plumbing for
+ *      callers without their own HIP runtime; no reference counterpart).
+ *      upload/download are synchronous; memset is asynchronous on the stream.
+ *      timer_stop returns the milliseconds between the two events recorded
+ *      on the context's stream by timer_start and timer_stop. */
+int mptg_dev_alloc(mptg_ctx* ctx, uint64_t bytes, void** d_out);
+int mptg_dev_free(mptg_ctx* ctx, void* d_ptr);
+int mptg_dev_upload(mptg_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
+int mptg_dev_download(mptg_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+int mptg_dev_memset(mptg_ctx* ctx, void* d_dst, int value, uint64_t bytes);
+int mptg_timer_start(mptg_ctx* ctx);
+int mptg_timer_stop(mptg_ctx* ctx, float* ms);
+
 /* ---- space: replaces the Space given to nigh::Nigh(space)
  *      src/mpt/impl/prrt/prrt.hpp:122,151.  w0/w1 = SO(3)/R^3 weights for
  *      MPTG_METRIC_SE3 (ignored otherwise). dim: 2..32 (L1/L2), 7 (SE3). */
@@ -151,15 +164,48 @@ int mptg_prrt_stats_get(mptg_ctx* ctx, mptg_prrt_stats* out);
 /* ids of goal nodes in discovery order (foundGoal, prrt.hpp:139-146); returns count via *n */
 int mptg_prrt_goals(mptg_ctx* ctx, int32_t* h_ids, int64_t cap, int64_t* n);
 
+/* ---- collision surrogate (no counterpart in the reference): a
+ *      64-128-128-128-1 ReLU MLP (BF16 operands, FP32 accumulation) trained
+ *      on the device on exact-checker labels.  It only orders the waypoint
+ *      checks of link() (MPTG_ORDER_SURROGATE, arm scene): verdicts still come
+ *      from the exact checker.  Parameters are FP32, laid out
+ *      W1[64][128] b1 W2[128][128] b2 W3[128][128] b3 w4[128] b4. */
+int64_t mptg_surrogate_nparams(void);
+int mptg_surrogate_init(mptg_ctx* ctx, uint64_t seed);  /* He-uniform weights, zero biases */
+int mptg_surrogate_get(mptg_ctx* ctx, float* h_params, int64_t n);
+int mptg_surrogate_set(mptg_ctx* ctx, const float* h_params, int64_t n);
+/* 64 BF16 features per arm state (joint positions / total length, cos/sin of
+ * the first 12 cumulative angles) */
+int mptg_surrogate_features(mptg_ctx* ctx, const double* d_states, int64_t n, uint16_t* d_feat);
+/* one training sample per edge: a waypoint of link(from_i, to_i), its
+ * features and its exact label (1 = collides) */
+int mptg_surrogate_samples(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, uint64_t seed,
+                           uint16_t* d_feat, float* d_label);
+int mptg_surrogate_forward(mptg_ctx* ctx, const uint16_t* d_feat, int64_t n, float* d_logits);
+/* mean binary cross-entropy over n rows (multiple of 16) and its gradient */
+int mptg_surrogate_loss_grad(mptg_ctx* ctx, const uint16_t* d_feat, const float* d_label, int64_t n,
+                             float* h_loss, float* h_grad);
+/* Adam (0.9, 0.999, 1e-8) over `epochs` passes of batches of `batch` rows */
+int mptg_surrogate_train(mptg_ctx* ctx, const uint16_t* d_feat, const float* d_label, int64_t n, int epochs,
+                         int batch, float lr, float* h_epoch_losses);
+/* scores of every waypoint of every edge: d_scores[i * 512 + reference ordinal] */
+int mptg_surrogate_score_edges(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
+                               float* d_scores);
+
 /* ---- diagnostics (no counterpart in the reference): evaluates the device
  *      build of the project math used by the exact path, for bit-equality
  *      tests against the host build. */
 enum mptg_diag_op { MPTG_DIAG_SIN = 0, MPTG_DIAG_COS = 1, MPTG_DIAG_ACOS = 2, MPTG_DIAG_SQRT = 3 };
 int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y);
-/* Work a reference-order link() batch performs on the arm scene (untimed,
- * instrumented copy of the link kernel): h_work = [valid() calls, forward-
- * kinematics links evaluated, segment-disk tests evaluated].  Used to price a
- * launch in algorithmic FLOPs. */
+/* Work of a reference-order link() batch on the arm scene, from untimed
+ * instrumented copies of the kernels (used to price a launch in FLOPs):
+ *   h_work[0..4] = the GPU kernel's executed work: valid() calls, forward-
+ *                  kinematics links, group broad-phase tests, link tests
+ *                  resolved by the far-disk bound, link tests that ran the
+ *                  exact expression;
+ *   h_work[5..7] = the reference algorithm's work on the same edges: valid()
+ *                  calls, FK links, segment-disk tests (every disk until the
+ *                  first hit). */
 int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
                    int endpoints_known_valid, uint64_t* h_work);
 

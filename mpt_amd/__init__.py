@@ -23,7 +23,7 @@ from ._native import PrrtParams, PrrtStats
 from .scenes import (LINK_BISECT, LINK_DMV, METRIC_L1, METRIC_L2, METRIC_SE3, ORDER_REFERENCE,
                      ORDER_SURROGATE, SCENE_ARM, SCENE_BOXES, SCENE_GRID, Scene)
 
-__all__ = ["Context", "PRRT", "MptgError", "Scene", "METRIC_L1", "METRIC_L2", "METRIC_SE3", "LINK_DMV",
+__all__ = ["Context", "DeviceArray", "PRRT", "MptgError", "Scene", "METRIC_L1", "METRIC_L2", "METRIC_SE3", "LINK_DMV",
            "LINK_BISECT", "ORDER_REFERENCE", "ORDER_SURROGATE"]
 
 class _Lazy:
@@ -55,6 +55,53 @@ def _f64(a, dim=None) -> np.ndarray:
     return a
 
 
+class DeviceArray:
+    """Device buffer of a context (``mptg_dev_alloc``) with numpy transfers."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx._chk(_lib.mptg_dev_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = int(p.value or 0)
+
+    @classmethod
+    def from_numpy(cls, ctx: "Context", a) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        self.ctx._chk(_lib.mptg_dev_upload(self.ctx.handle, C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
+                                           self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        a = np.empty(self.shape, dtype=self.dtype)
+        self.ctx._chk(_lib.mptg_dev_download(self.ctx.handle, a.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                             self.nbytes))
+        return a
+
+    def zero(self):
+        self.ctx._chk(_lib.mptg_dev_memset(self.ctx.handle, C.c_void_p(self.ptr), 0, self.nbytes))
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            _lib.mptg_dev_free(self.ctx.handle, C.c_void_p(self.ptr))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """One device, one HIP stream, one tree, one scene (``mptg_ctx``)."""
 
@@ -70,6 +117,10 @@ class Context:
         if getattr(self, "_h", None):
             _lib.mptg_destroy(self._h)
             self._h = None
+
+    @property
+    def alive(self) -> bool:
+        return bool(getattr(self, "_h", None))
 
     def __del__(self):
         self.close()
@@ -94,6 +145,22 @@ class Context:
 
     def sync(self):
         self._chk(_lib.mptg_sync(self._h))
+
+    def array(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a) -> DeviceArray:
+        return DeviceArray.from_numpy(self, a)
+
+    def time_ms(self, fn, reps: int = 1) -> float:
+        """Average milliseconds of fn() over reps, from HIP events on this
+        context's stream."""
+        self._chk(_lib.mptg_timer_start(self._h))
+        for _ in range(reps):
+            fn()
+        ms = C.c_float()
+        self._chk(_lib.mptg_timer_stop(self._h, C.byref(ms)))
+        return float(ms.value) / reps
 
     # ---- space / tree
     def space(self, metric: int, dim: int, w0: float = 1.0, w1: float = 1.0):
@@ -186,12 +253,56 @@ class Context:
         self._chk(_lib.mptg_diag_math(self._h, op, _p(x, C.c_double), x.size, _p(y, C.c_double)))
         return y
 
-    def link_work(self, a_ptr: int, b_ptr: int, n: int, known_valid: bool = False):
-        """[valid() calls, FK links, segment-disk tests] of a reference-order
-        link() batch on device buffers (arm scene)."""
-        w = (C.c_uint64 * 3)()
+    def link_work(self, a_ptr: int, b_ptr: int, n: int, known_valid: bool = False) -> dict:
+        """Executed work of the arm link kernel and the reference algorithm's
+        work on the same edges (device buffers, reference order)."""
+        w = (C.c_uint64 * 8)()
         self._chk(_lib.mptg_link_work(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, 1 if known_valid else 0, w))
-        return [int(x) for x in w]
+        keys = ("calls", "fk", "group", "far", "exact", "ref_calls", "ref_fk", "ref_tests")
+        return {k: int(v) for k, v in zip(keys, w)}
+
+    # ---- collision surrogate (device buffers are passed as integer pointers)
+    def surrogate_nparams(self) -> int:
+        return int(_lib.mptg_surrogate_nparams())
+
+    def surrogate_init(self, seed: int = 0):
+        self._chk(_lib.mptg_surrogate_init(self._h, seed))
+
+    def surrogate_get(self) -> np.ndarray:
+        p = np.zeros(self.surrogate_nparams(), dtype=np.float32)
+        self._chk(_lib.mptg_surrogate_get(self._h, _p(p, C.c_float), p.size))
+        return p
+
+    def surrogate_set(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._chk(_lib.mptg_surrogate_set(self._h, _p(p, C.c_float), p.size))
+
+    def surrogate_features(self, states_ptr: int, n: int, feat_ptr: int):
+        self._chk(_lib.mptg_surrogate_features(self._h, C.c_void_p(states_ptr), n, C.c_void_p(feat_ptr)))
+
+    def surrogate_samples(self, a_ptr: int, b_ptr: int, n: int, seed: int, feat_ptr: int, label_ptr: int):
+        self._chk(_lib.mptg_surrogate_samples(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, seed,
+                                              C.c_void_p(feat_ptr), C.c_void_p(label_ptr)))
+
+    def surrogate_forward(self, feat_ptr: int, n: int, logits_ptr: int):
+        self._chk(_lib.mptg_surrogate_forward(self._h, C.c_void_p(feat_ptr), n, C.c_void_p(logits_ptr)))
+
+    def surrogate_loss_grad(self, feat_ptr: int, label_ptr: int, n: int):
+        loss = C.c_float()
+        g = np.zeros(self.surrogate_nparams(), dtype=np.float32)
+        self._chk(_lib.mptg_surrogate_loss_grad(self._h, C.c_void_p(feat_ptr), C.c_void_p(label_ptr), n,
+                                                C.byref(loss), _p(g, C.c_float)))
+        return float(loss.value), g
+
+    def surrogate_train(self, feat_ptr: int, label_ptr: int, n: int, epochs: int, batch: int, lr: float):
+        losses = np.zeros(epochs, dtype=np.float32)
+        self._chk(_lib.mptg_surrogate_train(self._h, C.c_void_p(feat_ptr), C.c_void_p(label_ptr), n, epochs, batch,
+                                            lr, _p(losses, C.c_float)))
+        return losses
+
+    def surrogate_score_edges(self, a_ptr: int, b_ptr: int, n: int, scores_ptr: int):
+        self._chk(_lib.mptg_surrogate_score_edges(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n,
+                                                  C.c_void_p(scores_ptr)))
 
     # ---- planner
     def prrt_init(self, sc: Scene, seed: int, batch: int, order: int = ORDER_REFERENCE, range_: Optional[float] = None,

@@ -44,6 +44,13 @@ _SIGNATURES = [
     ("mptg_last_error", C.c_char_p, [C.c_void_p]),
     ("mptg_sync", C.c_int, [C.c_void_p]),
     ("mptg_stream", C.c_void_p, [C.c_void_p]),
+    ("mptg_dev_alloc", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("mptg_dev_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mptg_dev_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("mptg_dev_download", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("mptg_dev_memset", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64]),
+    ("mptg_timer_start", C.c_int, [C.c_void_p]),
+    ("mptg_timer_stop", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("mptg_space", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]),
     ("mptg_tree_reserve", C.c_int, [C.c_void_p, C.c_int64]),
     ("mptg_tree_clear", C.c_int, [C.c_void_p]),
@@ -65,6 +72,19 @@ _SIGNATURES = [
     ("mptg_prrt_run", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(PrrtStats)]),
     ("mptg_prrt_stats_get", C.c_int, [C.c_void_p, C.POINTER(PrrtStats)]),
     ("mptg_prrt_goals", C.c_int, [C.c_void_p, c_int32_p, C.c_int64, c_int64_p]),
+    ("mptg_surrogate_nparams", C.c_int64, []),
+    ("mptg_surrogate_init", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("mptg_surrogate_get", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
+    ("mptg_surrogate_set", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
+    ("mptg_surrogate_features", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("mptg_surrogate_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, C.c_void_p,
+                                         C.c_void_p]),
+    ("mptg_surrogate_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("mptg_surrogate_loss_grad", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float)]),
+    ("mptg_surrogate_train", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_float,
+                                       C.POINTER(C.c_float)]),
+    ("mptg_surrogate_score_edges", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("mptg_diag_math", C.c_int, [C.c_void_p, C.c_int, c_double_p, C.c_int64, c_double_p]),
     ("mptg_link_work", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,
                                  C.POINTER(C.c_uint64)]),

@@ -44,7 +44,7 @@ HIP_FLAGS = [
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall"]
 
-LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "diag.hip", "arm.hip"]
+LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "diag.hip", "arm.hip", "mlp.hip"]
 ORACLE_SOURCES = ["orc_core.cpp", "orc_prrt.cpp", "orc_pool.cpp", "orc_mlp.cpp"]
 
 

@@ -26,55 +26,77 @@
 // bound check with a relative margin of 2^-30 that dominates the rounding of
 // the exact expression, so the skip never changes a verdict.  Disks that pass
 // the bound get the reference expression bit-for-bit.
-#include "space_dev.hpp"
+#include "bisect_dev.hpp"
 
 namespace mptg {
 namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
-constexpr int kMaxArmDepth = 30;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Pre-order ordinal -> (path, depth) in a perfect tree with h internal levels.
-__device__ __forceinline__ void preorder_path(int h, uint32_t o, uint32_t* path, int* depth) {
-    uint32_t p = 0;
-    int d = 0;
-    uint32_t sub = (1u << h) - 1u;
-    while (o != 0) {
-        o -= 1;
-        uint32_t half = (sub - 1u) >> 1;
-        if (o >= half) {
-            o -= half;
-            p |= 1u << d;
+// Sort key of a waypoint in surrogate order: ascending key = descending
+// score, ties by ascending reference ordinal (-0 counts as +0, NaN last).
+__device__ __forceinline__ uint64_t score_key(float s, uint32_t ordinal) {
+    uint32_t hi;
+    if (s != s) {
+        hi = 0xffffffffu;
+    } else {
+        if (s == 0.0f) s = 0.0f;
+        uint32_t u = bits_of(s);
+        uint32_t mono = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // ascending in s
+        hi = ~mono;
+    }
+    return ((uint64_t)hi << 32) | ordinal;
+}
+
+// Ascending bitonic sort of the wave's 64 * NE keys; element i = e * 64 +
+// lane lives in key[e] of that lane, so after the sort rank r = c * 64 + lane
+// is key[c] of that lane.  Partners within a lane are registers, partners in
+// other lanes come by __shfl_xor.
+template <int NE>
+__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[NE], int lane) {
+    constexpr int N = 64 * NE;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint64_t nk[NE];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const int i = e * 64 + lane;
+                uint64_t other;
+                if (j >= 64) {
+                    other = key[e ^ (j >> 6)];
+                } else {
+                    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)key[e], j);
+                    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(key[e] >> 32), j);
+                    other = ((uint64_t)hi << 32) | lo;
+                }
+                const bool lower = (i & j) == 0, asc = (i & k) == 0;
+                const uint64_t mn = key[e] < other ? key[e] : other;
+                const uint64_t mx = key[e] < other ? other : key[e];
+                nk[e] = (lower == asc) ? mn : mx;
+            }
+#pragma unroll
+            for (int e = 0; e < NE; ++e) key[e] = nk[e];
         }
-        ++d;
-        sub = half;
     }
-    *path = p;
-    *depth = d;
 }
 
-// One coordinate of the midpoint of the node at `path`: the reference's
-// recursion (a+b)/2 restricted to that coordinate.
-__device__ __forceinline__ double coord_mid(double lo, double hi, uint32_t path, int depth) {
-    for (int k = 0; k < depth; ++k) {
-        double m = (lo + hi) / 2;
-        bool right = (path >> k) & 1u;
-        lo = right ? m : lo;
-        hi = right ? hi : m;
-    }
-    return (lo + hi) / 2;
-}
-
-template <int D>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_wave(
-    const double* __restrict__ from, const double* __restrict__ to, int64_t n, int known_valid,
-    const uint8_t* __restrict__ mask, double tol, SceneArgs sc, uint8_t* __restrict__ verdict,
-    uint32_t* __restrict__ checks, int64_t* __restrict__ irregular, uint32_t* __restrict__ n_irregular) {
+// One edge per wave (the kernel body; COUNT adds the work tally; ORD
+// selects the visiting order: reference pre-order, or descending surrogate
+// score from scores[e * kScoreStride + ordinal]).
+template <int D, bool COUNT, int ORD>
+__device__ __forceinline__ void arm_link_edge(const double* __restrict__ from, const double* __restrict__ to, int64_t n,
+                                              int known_valid, const uint8_t* __restrict__ mask, double tol,
+                                              const SceneArgs& sc, const float* __restrict__ scores,
+                                              uint8_t* __restrict__ verdict,
+                                              uint32_t* __restrict__ checks, int64_t* __restrict__ irregular,
+                                              uint32_t* __restrict__ n_irregular, ArmWork* wk) {
     // no block-wide barriers in this kernel: waves leave independently
     const int lane = lane_id();
     const int w = uniform((int)(threadIdx.x >> 6));
@@ -86,8 +108,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_wave(
     uint32_t base = 2;
     if (!known_valid) {
         bool okA = true, okB = true;
-        if (lane == 0) okA = arm_ok<D>(sc, a);
-        if (lane == 1) okB = arm_ok<D>(sc, b);
+        if (lane == 0) okA = arm_ok_fn<D, COUNT>(sc, [&](int i) { return a[i]; }, wk);
+        if (lane == 1) okB = arm_ok_fn<D, COUNT>(sc, [&](int i) { return b[i]; }, wk);
         uint64_t badA = __ballot(!okA), badB = __ballot(!okB);
         if (badA | badB) {
             if (lane == 0) {
@@ -98,41 +120,48 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_wave(
         }
     }
     // tree height from the widest coordinate, and the regularity test
-    double wmax = 0.0, M = 0.0;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-        wmax = __builtin_fmax(wmax, __builtin_fabs(a[j] - b[j]));
-        M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(a[j]), __builtin_fabs(b[j])));
-    }
     int h = 0;
-    double wd = wmax;  // w_max / 2^h, exact scaling
-    while (!(wd < tol) && h <= kMaxArmDepth) {
-        wd *= 0.5;
-        ++h;
-    }
-    if (h > kMaxArmDepth) {
+    bool regular = false;
+    if (!bisect_shape<D>(a, b, tol, &h, &regular)) {
         if (lane == 0) {
             verdict[e] = 0;
             checks[e] = MPTG_CHECKS_OVER_LIMIT;
         }
         return;
     }
-    const double err = (double)(h + 1) * 0x1.0p-50 * M;
-    // h == 0: the root's width is exactly wmax < tol, a leaf
-    const bool regular = (h == 0) || ((2.0 * wd - err >= tol) && (wd + err < tol));
     if (!regular) {  // exact per-edge walk in a second launch
         if (lane == 0) irregular[atomicAdd(n_irregular, 1u)] = e;
         return;
     }
     const uint32_t W = (1u << h) - 1u;
-    for (uint32_t c0 = 0; c0 < W; c0 += kWave) {
-        const uint32_t o = c0 + (uint32_t)lane;
+    uint64_t key[kScoreStride / kWave];
+    if constexpr (ORD == MPTG_ORDER_SURROGATE) {
+        if (W > kScoreStride) {  // deeper than the scored set: reference order
+            if (lane == 0) irregular[atomicAdd(n_irregular, 1u)] = e;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < kScoreStride / kWave; ++k) {
+            const uint32_t o = (uint32_t)(k * kWave + lane);
+            key[k] = o < W ? score_key(scores[e * kScoreStride + o], o) : ~0ull;
+        }
+        wave_bitonic_sort<kScoreStride / kWave>(key, lane);
+    }
+    for (uint32_t c0 = 0, c = 0; c0 < W; c0 += kWave, ++c) {
+        uint32_t o = c0 + (uint32_t)lane;
+        if constexpr (ORD == MPTG_ORDER_SURROGATE) {
+            uint64_t pick = key[0];
+#pragma unroll
+            for (int k = 1; k < kScoreStride / kWave; ++k)
+                if (c == (uint32_t)k) pick = key[k];
+            o = (uint32_t)pick;  // rank c0 + lane -> reference ordinal
+        }
         bool bad = false;
-        if (o < W) {
+        if (c0 + (uint32_t)lane < W) {
             uint32_t path;
             int depth;
             preorder_path(h, o, &path, &depth);
-            bad = !arm_ok_fn<D>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); });
+            bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, wk);
         }
         uint64_t m = __ballot(bad);
         if (m) {
@@ -149,26 +178,63 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_wave(
     }
 }
 
+
+template <int D, bool COUNT, int ORD>
+__global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_wave(
+    const double* __restrict__ from, const double* __restrict__ to, int64_t n, int known_valid,
+    const uint8_t* __restrict__ mask, double tol, SceneArgs sc, const float* __restrict__ scores,
+    uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks, int64_t* __restrict__ irregular,
+    uint32_t* __restrict__ n_irregular, unsigned long long* __restrict__ work) {
+    ArmWork wk{0, 0, 0, 0, 0};
+    arm_link_edge<D, COUNT, ORD>(from, to, n, known_valid, mask, tol, sc, scores, verdict, checks, irregular,
+                                 n_irregular, &wk);
+    if constexpr (COUNT) {
+        atomicAdd(&work[0], (unsigned long long)wk.calls);
+        atomicAdd(&work[1], (unsigned long long)wk.fk);
+        atomicAdd(&work[2], (unsigned long long)wk.group);
+        atomicAdd(&work[3], (unsigned long long)wk.far);
+        atomicAdd(&work[4], (unsigned long long)wk.exact);
+    }
+}
+
 }  // namespace
 
 template <int D>
 static void launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv, const uint8_t* mask,
-                         uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr) {
+                         const float* scores, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
+                         unsigned long long* work) {
     unsigned blocks = (unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((arm_link_wave<D>), dim3(blocks), dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n,
-                       kv, mask, ctx->link_param, scene_args(ctx), verdict, checks, irr, nirr);
+    dim3 g(blocks), t(kWave * kWavesPerBlock);
+    const double tol = ctx->link_param;
+    const SceneArgs sa = scene_args(ctx);
+    if (scores) {
+        if (work)
+            hipLaunchKernelGGL((arm_link_wave<D, true, MPTG_ORDER_SURROGATE>), g, t, 0, ctx->stream, from, to, n, kv,
+                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
+        else
+            hipLaunchKernelGGL((arm_link_wave<D, false, MPTG_ORDER_SURROGATE>), g, t, 0, ctx->stream, from, to, n, kv,
+                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
+    } else {
+        if (work)
+            hipLaunchKernelGGL((arm_link_wave<D, true, MPTG_ORDER_REFERENCE>), g, t, 0, ctx->stream, from, to, n, kv,
+                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
+        else
+            hipLaunchKernelGGL((arm_link_wave<D, false, MPTG_ORDER_REFERENCE>), g, t, 0, ctx->stream, from, to, n, kv,
+                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
+    }
 }
 
 // Launches the wave kernel; edges it lists as irregular are left for the
 // caller's exact per-edge kernel.  Returns MPTG_E_STATE when no wave kernel
 // exists for this arm.
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
-                    const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr) {
+                    const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
+                    unsigned long long* work, const float* scores) {
     if (ctx->dim != ctx->scene.nlinks) return MPTG_E_STATE;
     switch (ctx->dim) {
-        case 5: launch_arm_d<5>(ctx, from, to, n, known_valid, mask, verdict, checks, irr, nirr); break;
-        case 10: launch_arm_d<10>(ctx, from, to, n, known_valid, mask, verdict, checks, irr, nirr); break;
-        case 20: launch_arm_d<20>(ctx, from, to, n, known_valid, mask, verdict, checks, irr, nirr); break;
+        case 5: launch_arm_d<5>(ctx, from, to, n, known_valid, mask, scores, verdict, checks, irr, nirr, work); break;
+        case 10: launch_arm_d<10>(ctx, from, to, n, known_valid, mask, scores, verdict, checks, irr, nirr, work); break;
+        case 20: launch_arm_d<20>(ctx, from, to, n, known_valid, mask, scores, verdict, checks, irr, nirr, work); break;
         default: return MPTG_E_STATE;
     }
     MPTG_HIP(ctx, hipGetLastError());

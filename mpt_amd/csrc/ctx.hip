@@ -9,6 +9,7 @@ namespace mptg {
 
 thread_local std::string g_create_err;
 void prrt_free(mptg_ctx* ctx);
+void surrogate_free(mptg_ctx* ctx);
 
 int ensure(mptg_ctx* ctx, DevBuf& b, size_t bytes) {
     if (b.bytes >= bytes && b.p) return MPTG_OK;
@@ -129,7 +130,10 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.arm_lim.p);
     for (auto& b : ctx->scratch) (void)hipFree(b.p);
     prrt_free(ctx);
+    surrogate_free(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -146,6 +150,68 @@ int mptg_sync(mptg_ctx* ctx) {
 }
 
 void* mptg_stream(mptg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mptg_dev_alloc(mptg_ctx* ctx, uint64_t bytes, void** d_out) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (!d_out) return fail(ctx, MPTG_E_ARG, "null output pointer");
+    *d_out = nullptr;
+    MPTG_HIP(ctx, hipMalloc(d_out, bytes ? bytes : 1));
+    return MPTG_OK;
+}
+
+int mptg_dev_free(mptg_ctx* ctx, void* d_ptr) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MPTG_HIP(ctx, hipFree(d_ptr));
+    return MPTG_OK;
+}
+
+int mptg_dev_upload(mptg_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return MPTG_OK;
+    MPTG_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_dev_download(mptg_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return MPTG_OK;
+    MPTG_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_dev_memset(mptg_ctx* ctx, void* d_dst, int value, uint64_t bytes) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return MPTG_OK;
+    MPTG_HIP(ctx, hipMemsetAsync(d_dst, value, bytes, ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_timer_start(mptg_ctx* ctx) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (!ctx->ev_start) MPTG_HIP(ctx, hipEventCreate(&ctx->ev_start));
+    if (!ctx->ev_stop) MPTG_HIP(ctx, hipEventCreate(&ctx->ev_stop));
+    MPTG_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_timer_stop(mptg_ctx* ctx, float* ms) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (!ctx->ev_start || !ms) return fail(ctx, MPTG_E_STATE, "mptg_timer_start was not called");
+    MPTG_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+    MPTG_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
+    MPTG_HIP(ctx, hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop));
+    return MPTG_OK;
+}
 
 int mptg_space(mptg_ctx* ctx, int metric, int dim, double w0, double w1) {
     int rc = check_device_ready(ctx);
@@ -307,7 +373,11 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
     if (rc) return rc;
     // kernel layout: [cx, cy, (r+R)^2, r+R] and a per-link segment-length
     // bound L (1 + 2^-30) + 1e-12 for the far-disk skip
-    std::vector<double> ad((size_t)std::max(1, ndisks) * 4, 0.0), lim((size_t)nlinks);
+    // then per group of kArmGroup links a bound on its half-length,
+    // S_g/2 (1 + 2^-30) + 1e-12
+    constexpr int kGroup = 5;
+    const int ngroups = (nlinks + kGroup - 1) / kGroup;
+    std::vector<double> ad((size_t)std::max(1, ndisks) * 4, 0.0), lim((size_t)(nlinks + ngroups));
     for (int i = 0; i < ndisks; ++i) {
         double rt = h_disks[3 * i + 2] + robot_radius;
         ad[4 * i + 0] = h_disks[3 * i + 0];
@@ -316,6 +386,11 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
         ad[4 * i + 3] = rt;
     }
     for (int i = 0; i < nlinks; ++i) lim[i] = h_lengths[i] * (1.0 + 0x1.0p-30) + 1e-12;
+    for (int g = 0; g < ngroups; ++g) {
+        double S = 0.0;
+        for (int i = g * kGroup; i < (g + 1) * kGroup && i < nlinks; ++i) S += h_lengths[i];
+        lim[nlinks + g] = S / 2 * (1.0 + 0x1.0p-30) + 1e-12;
+    }
     if ((rc = ensure(ctx, ctx->scene.arm_disks, sizeof(double) * ad.size()))) return rc;
     if ((rc = ensure(ctx, ctx->scene.arm_lim, sizeof(double) * lim.size()))) return rc;
     MPTG_HIP(ctx, hipMemcpyAsync(ctx->scene.arm_disks.p, ad.data(), sizeof(double) * ad.size(),
@@ -324,6 +399,8 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
                                  hipMemcpyHostToDevice, ctx->stream));
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->scene.kind = MPTG_SCENE_ARM;
+    ctx->scene.total_length = 0;
+    for (int i = 0; i < nlinks; ++i) ctx->scene.total_length += h_lengths[i];
     ctx->scene.nlinks = nlinks;
     ctx->scene.ndisks = ndisks;
     ctx->scene.robot_radius = robot_radius;

@@ -12,6 +12,8 @@
 
 namespace mptg {
 
+struct Surrogate;
+
 constexpr int kMaxDim = 32;
 constexpr int kMaxK = 32;
 
@@ -36,6 +38,7 @@ struct SceneDev {
     double* disks = nullptr;  // [cx, cy, (r+R)^2] per disk
     int ndisks = 0;
     DevBuf arm_disks, arm_lim;  // kernel layout of the disks, per-link bounds
+    double total_length = 0;    // sum of link lengths (feature scale)
 };
 
 struct PrrtState {
@@ -77,13 +80,15 @@ struct mptg_ctx {
     double link_param = 1.0;
 
     // scratch (grow-only)
-    mptg::DevBuf scratch[8];
+    mptg::DevBuf scratch[12];  // 0-3 host staging, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores
     // pinned host staging
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
 
     mptg::PrrtState prrt;
     bool force_generic = false;  // route link() to the per-edge reference kernels (testing)
+    mptg::Surrogate* surrogate = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 
 namespace mptg {
@@ -120,6 +125,11 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
                        int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
 int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
-                    const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr);
+                    const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
+                    unsigned long long* work = nullptr, const float* scores = nullptr);
+// surrogate scores of every waypoint of every (masked) arm edge:
+// scores[e * 512 + reference ordinal]
+int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                          float* scores);
 
 }  // namespace mptg

@@ -1,0 +1,731 @@
+// Collision surrogate: a 64 -> 128 -> 128 -> 128 -> 1 ReLU MLP with BF16
+// operands and FP32 accumulation on gfx950 matrix cores, trained on labels
+// from the exact checker.  It has no counterpart in the reference; it only
+// orders the waypoint checks of link() so that likely collisions are tested
+// first — every verdict still comes from the exact checker.
+//
+// Matrix products use v_mfma_f32_16x16x32_bf16, one wave per 16x16 output
+// tile.  Operand layout (checked on the device): lane l supplies row l%16 of
+// A and column l%16 of B, k = 8*(l/16) .. +7; it receives column l%16 and
+// rows 4*(l/16) .. +3 of D.  Every product is written C = A * Bt^T with A and
+// Bt row-major, so both operands are contiguous 16-byte loads.  Epilogues:
+//   FWD  : + bias, ReLU, store H (BF16) and H^T (BF16, for the weight grads)
+//   DX   : zero where the layer input was 0 (ReLU'), store dH and dH^T (BF16)
+//   F32  : store FP32 (weight gradients)
+// Rounding points are those of the CPU oracle (oracle/orc_mlp.cpp).
+//
+// Parameter layout (FP32 master copy, same as the oracle):
+//   W1[64][128] b1[128] W2[128][128] b2[128] W3[128][128] b3[128] w4[128] b4
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "bisect_dev.hpp"
+
+namespace mptg {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int IN = 64, HID = 128;
+constexpr int64_t OFF_W1 = 0, OFF_B1 = OFF_W1 + IN * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
+                  OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * HID, OFF_W4 = OFF_B3 + HID,
+                  OFF_B4 = OFF_W4 + HID, NPARAM = OFF_B4 + 1;
+
+enum { EPI_FWD = 0, EPI_DX = 1, EPI_F32 = 2 };
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+    uint4 v = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+struct Epi {
+    const float* bias;     // FWD
+    const uint16_t* mask;  // DX: layer input, [M][N]
+    uint16_t* out;         // FWD/DX: [M][N] BF16
+    uint16_t* out_t;       // FWD/DX: [N][M] BF16 (may be null)
+    float* out_f;          // F32: [M][N]
+};
+
+// C[M][N] = A[M][K] * Bt[N][K]^T; M, N multiples of 16, K multiple of 32.
+template <int MODE>
+__global__ __launch_bounds__(256) void gemm_nt(int M, int N, int K, const uint16_t* __restrict__ A,
+                                               const uint16_t* __restrict__ Bt, Epi ep) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tiles_n = N / 16;
+    const int tm = tile / tiles_n, tn = tile % tiles_n;
+    if (tm * 16 >= M) return;
+    const int r = lane & 15, kq = lane >> 4;
+    const uint16_t* pa = A + (int64_t)(tm * 16 + r) * K + 8 * kq;
+    const uint16_t* pb = Bt + (int64_t)(tn * 16 + r) * K + 8 * kq;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 32) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load8(pa + k0), load8(pb + k0), acc, 0, 0, 0);
+    const int n = tn * 16 + r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = tm * 16 + 4 * kq + i;
+        float v = acc[i];
+        if constexpr (MODE == EPI_F32) {
+            ep.out_f[(int64_t)m * N + n] = v;
+        } else {
+            if constexpr (MODE == EPI_FWD) {
+                v = v + ep.bias[n];
+                v = v > 0.f ? v : 0.f;
+            } else {
+                if (ep.mask[(int64_t)m * N + n] == 0) v = 0.f;  // ReLU output is 0 or positive
+            }
+            uint16_t h = float_to_bf16(v);
+            ep.out[(int64_t)m * N + n] = h;
+            if (ep.out_t) ep.out_t[(int64_t)n * M + m] = h;
+        }
+    }
+}
+
+// Output layer and loss: z = H3 . w4 + b4; dz = (sigmoid(z) - y) / N;
+// dH3 = bf16(dz * w4 where H3 > 0).
+__global__ __launch_bounds__(256) void head_kernel(int M, const uint16_t* __restrict__ H3,
+                                                   const uint16_t* __restrict__ w4, const float* __restrict__ b4,
+                                                   const float* __restrict__ y, float inv_n, float* __restrict__ z,
+                                                   float* __restrict__ dz, uint16_t* __restrict__ dH3,
+                                                   uint16_t* __restrict__ dH3t, float* __restrict__ loss) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (r < M) {
+        float acc = 0.f;
+        for (int k = 0; k < HID; ++k) acc += bf16_to_float(H3[(int64_t)r * HID + k]) * bf16_to_float(w4[k]);
+        float zz = acc + b4[0];
+        if (z) z[r] = zz;
+        if (y) {
+            float yy = y[r];
+            l = fmaxf(zz, 0.f) - zz * yy + log1pf(expf(-fabsf(zz)));
+            float d = (1.f / (1.f + expf(-zz)) - yy) * inv_n;
+            dz[r] = d;
+            for (int k = 0; k < HID; ++k) {
+                uint16_t h = H3[(int64_t)r * HID + k];
+                uint16_t g = float_to_bf16(h != 0 ? d * bf16_to_float(w4[k]) : 0.f);
+                dH3[(int64_t)r * HID + k] = g;
+                dH3t[(int64_t)k * M + r] = g;
+            }
+        }
+    }
+    if (y) {
+        // block sum of the loss
+        __shared__ float red[256];
+        red[threadIdx.x] = l;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_n);
+    }
+}
+
+// out[row] = sum_j X[row][j] * (w ? w[j] : 1), X BF16 [rows][len]: bias
+// gradients from dH^T rows, and dw4 from H3^T rows weighted by dz.
+__global__ __launch_bounds__(256) void row_sums(const uint16_t* __restrict__ X, int len, const float* __restrict__ w,
+                                                float* __restrict__ out) {
+    const uint16_t* x = X + (int64_t)blockIdx.x * len;
+    float acc = 0.f;
+    for (int j = threadIdx.x; j < len; j += blockDim.x) acc += bf16_to_float(x[j]) * (w ? w[j] : 1.f);
+    __shared__ float red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Transpose of a BF16 matrix [rows][cols] -> [cols][rows].
+__global__ void transpose_bf16(const uint16_t* __restrict__ X, int rows, int cols, uint16_t* __restrict__ Xt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * cols) return;
+    int r = (int)(i / cols), c = (int)(i % cols);
+    Xt[(int64_t)c * rows + r] = X[i];
+}
+
+// BF16 operand copies of the FP32 master weights: W[in][out] and W^T[out][in].
+__global__ void cast_weights(const float* __restrict__ p, uint16_t* __restrict__ wb, uint16_t* __restrict__ wtb) {
+    // wb/wtb hold W1, W2, W3, w4 back to back (W4 has no transpose copy)
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n1 = IN * HID, n2 = HID * HID;
+    if (i < n1) {
+        int a = (int)(i / HID), o = (int)(i % HID);
+        uint16_t h = float_to_bf16(p[OFF_W1 + i]);
+        wb[i] = h;
+        wtb[(int64_t)o * IN + a] = h;
+    } else if (i < n1 + 2 * n2) {
+        int64_t j = i - n1;
+        int layer = (int)(j / n2);
+        int64_t k = j % n2;
+        int a = (int)(k / HID), o = (int)(k % HID);
+        uint16_t h = float_to_bf16(p[(layer == 0 ? OFF_W2 : OFF_W3) + k]);
+        wb[n1 + layer * n2 + k] = h;
+        wtb[n1 + layer * n2 + (int64_t)o * HID + a] = h;
+    } else if (i < n1 + 2 * n2 + HID) {
+        int64_t k = i - n1 - 2 * n2;
+        wb[n1 + 2 * n2 + k] = float_to_bf16(p[OFF_W4 + k]);
+    }
+}
+
+__global__ void adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.f - b1) * gi;
+    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void init_params(float* __restrict__ p, uint64_t seed) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM) return;
+    float fan_in = 0.f;
+    if (i < OFF_B1) fan_in = IN;
+    else if ((i >= OFF_W2 && i < OFF_B2) || (i >= OFF_W3 && i < OFF_B3) || (i >= OFF_W4 && i < OFF_B4)) fan_in = HID;
+    if (fan_in == 0.f) {
+        p[i] = 0.f;  // biases
+        return;
+    }
+    double u = uniform_at(seed, (uint64_t)i, 0x4d4c5030u, 0);  // He-uniform
+    float lim = sqrtf(6.f / fan_in);
+    p[i] = (float)(u * 2.0 - 1.0) * lim;
+}
+
+// Surrogate features of an arm state (FP32): joint positions p_1..p_20
+// scaled by 1/total length (40 values), then cos/sin of the first 12
+// cumulative angles (24 values).  Hardware sine/cosine: the features only
+// steer the check order, never a verdict.
+template <int D>
+__device__ __forceinline__ void arm_features(const double* q, const double* lengths, float inv_total, uint16_t* out) {
+    float x = 0.f, y = 0.f, ang = 0.f;
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        ang += (float)q[i];
+        float s, c;
+        __sincosf(ang, &s, &c);
+        x += (float)lengths[i] * c;
+        y += (float)lengths[i] * s;
+        if (2 * i + 1 < 40) {
+            out[2 * i] = float_to_bf16(x * inv_total);
+            out[2 * i + 1] = float_to_bf16(y * inv_total);
+        }
+        if (i < 12) {
+            out[40 + 2 * i] = float_to_bf16(c);
+            out[40 + 2 * i + 1] = float_to_bf16(s);
+        }
+        ++k;
+    }
+    for (int j = 2 * D; j < 40; ++j) out[j] = 0;
+    for (int j = 40 + 2 * (D < 12 ? D : 12); j < 64; ++j) out[j] = 0;
+}
+
+}  // namespace
+}  // namespace mptg
+
+namespace mptg {
+namespace {
+
+__global__ void sum_f32(const float* __restrict__ x, int n, float* __restrict__ out) {
+    float acc = 0.f;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) acc += x[j];
+    __shared__ float red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+template <int D>
+__global__ void features_kernel(const double* __restrict__ q, int64_t n, const double* __restrict__ lengths,
+                                float inv_total, uint16_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) s[j] = q[i * D + j];
+    uint16_t f[IN];
+    arm_features<D>(s, lengths, inv_total, f);
+#pragma unroll
+    for (int j = 0; j < IN; ++j) out[i * IN + j] = f[j];
+}
+
+}  // namespace
+
+// ---- fused inference over the waypoints of arm edges ----------------------
+// One wave (= one block) scores 64 waypoints of one edge: it rebuilds each
+// waypoint with the reference's recursion (FP64), computes its features,
+// and runs the three hidden layers on the matrix cores with activations in
+// LDS.  Rows are 64 waypoints; a layer is 4 row tiles x 8 column tiles of
+// 16x16, K in steps of 32.  Weights come from the BF16 W^T copies (L2/L1
+// resident, 80 KiB).
+constexpr int kLdsRow = HID + 8;  // padded row (BF16 elements)
+
+template <int K>
+__device__ __forceinline__ void wave_layer(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
+                                           const float* __restrict__ bias, uint16_t* __restrict__ out, int lane) {
+    const int r = lane & 15, kq = lane >> 4;
+#pragma unroll 1
+    for (int tm = 0; tm < 4; ++tm) {
+        f32x4 acc[8];
+#pragma unroll
+        for (int tn = 0; tn < 8; ++tn) acc[tn] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            bf16x8 a = load8(in + (tm * 16 + r) * kLdsRow + k0 + 8 * kq);
+#pragma unroll
+            for (int tn = 0; tn < 8; ++tn) {
+                bf16x8 b = load8(wt + (tn * 16 + r) * K + k0 + 8 * kq);
+                acc[tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tn], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int tn = 0; tn < 8; ++tn) {
+            const int col = tn * 16 + r;
+            const float bb = bias[col];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[tn][i] + bb;
+                out[(tm * 16 + 4 * kq + i) * kLdsRow + col] = float_to_bf16(v > 0.f ? v : 0.f);
+            }
+        }
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__ from, const double* __restrict__ to,
+                                                      int64_t n, const uint8_t* __restrict__ mask, double tol,
+                                                      const double* __restrict__ lengths, float inv_total,
+                                                      const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
+                                                      const float* __restrict__ p, float* __restrict__ scores) {
+    __shared__ __attribute__((aligned(16))) uint16_t bufA[64 * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t bufB[64 * kLdsRow];
+    const int lane = threadIdx.x;
+    const int64_t e = blockIdx.x / (kScoreStride / 64);
+    const uint32_t c0 = (uint32_t)(blockIdx.x % (kScoreStride / 64)) * 64;
+    if (e >= n || (mask && !mask[e])) return;
+    const double* a = from + e * D;
+    const double* b = to + e * D;
+    int h = 0;
+    bool regular = false;
+    if (!bisect_shape<D>(a, b, tol, &h, &regular)) return;
+    const uint32_t W = (1u << h) - 1u;
+    if (c0 >= W || W > (uint32_t)kScoreStride) return;
+    // features of this lane's waypoint (rows beyond W are zero)
+    const uint32_t o = c0 + (uint32_t)lane;
+    uint16_t f[IN];
+    if (o < W) {
+        uint32_t path;
+        int depth;
+        preorder_path(h, o, &path, &depth);
+        double q[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) q[i] = coord_mid(a[i], b[i], path, depth);
+        arm_features<D>(q, lengths, inv_total, f);
+    } else {
+#pragma unroll
+        for (int j = 0; j < IN; ++j) f[j] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < IN; j += 8) {
+        uint4 v;
+        v.x = (uint32_t)f[j] | ((uint32_t)f[j + 1] << 16);
+        v.y = (uint32_t)f[j + 2] | ((uint32_t)f[j + 3] << 16);
+        v.z = (uint32_t)f[j + 4] | ((uint32_t)f[j + 5] << 16);
+        v.w = (uint32_t)f[j + 6] | ((uint32_t)f[j + 7] << 16);
+        *reinterpret_cast<uint4*>(bufA + lane * kLdsRow + j) = v;
+    }
+    __syncthreads();
+    wave_layer<IN>(bufA, wt, p + OFF_B1, bufB, lane);
+    __syncthreads();
+    wave_layer<HID>(bufB, wt + IN * HID, p + OFF_B2, bufA, lane);
+    __syncthreads();
+    wave_layer<HID>(bufA, wt + IN * HID + HID * HID, p + OFF_B3, bufB, lane);
+    __syncthreads();
+    if (o < W) {
+        const uint16_t* w4 = wb + IN * HID + 2 * HID * HID;
+        float acc = 0.f;
+        for (int k = 0; k < HID; ++k) acc += bf16_to_float(bufB[lane * kLdsRow + k]) * bf16_to_float(w4[k]);
+        scores[e * kScoreStride + o] = acc + p[OFF_B4];
+    }
+}
+
+// Training samples: one waypoint per edge (ordinal uniform over the nominal
+// bisection tree, Philox(seed, edge)), its features and the exact label
+// (1 = the waypoint collides).
+template <int D>
+__global__ __launch_bounds__(256) void sample_waypoints(const double* __restrict__ from, const double* __restrict__ to,
+                                                        int64_t n, double tol, SceneArgs sc, float inv_total,
+                                                        uint64_t seed, uint16_t* __restrict__ feat,
+                                                        float* __restrict__ label) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double* a = from + e * D;
+    const double* b = to + e * D;
+    int h = 0;
+    bool regular = false;
+    double q[D];
+    if (!bisect_shape<D>(a, b, tol, &h, &regular) || h == 0) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) q[i] = b[i];
+    } else {
+        const uint32_t W = (1u << h) - 1u;
+        uint32_t o = (uint32_t)(uniform_at(seed, (uint64_t)e, 0x53414d57u, 0) * (double)W);
+        if (o >= W) o = W - 1;
+        uint32_t path;
+        int depth;
+        preorder_path(h, o, &path, &depth);
+#pragma unroll
+        for (int i = 0; i < D; ++i) q[i] = coord_mid(a[i], b[i], path, depth);
+    }
+    uint16_t f[IN];
+    arm_features<D>(q, sc.lengths, inv_total, f);
+#pragma unroll
+    for (int j = 0; j < IN; ++j) feat[e * IN + j] = f[j];
+    label[e] = arm_ok<D>(sc, q) ? 0.f : 1.f;
+}
+
+// Device state of the surrogate (owned by the context).
+struct Surrogate {
+    bool init = false;
+    int64_t cap = 0;  // rows the batch buffers hold
+    int32_t adam_t = 0;
+    DevBuf params, m, v, grad, wb, wtb, loss;
+    DevBuf x, xt, y, h1, h1t, h2, h2t, h3, h3t, z, dz, dh3, dh3t, dh2, dh2t, dh1, dh1t;
+};
+
+Surrogate& surrogate(mptg_ctx* ctx) {
+    if (!ctx->surrogate) ctx->surrogate = new Surrogate();
+    return *ctx->surrogate;
+}
+
+void surrogate_free(mptg_ctx* ctx) {
+    if (!ctx->surrogate) return;
+    Surrogate& S = *ctx->surrogate;
+    for (DevBuf* b : {&S.params, &S.m, &S.v, &S.grad, &S.wb, &S.wtb, &S.loss, &S.x, &S.xt, &S.y, &S.h1, &S.h1t,
+                      &S.h2, &S.h2t, &S.h3, &S.h3t, &S.z, &S.dz, &S.dh3, &S.dh3t, &S.dh2, &S.dh2t, &S.dh1, &S.dh1t})
+        (void)hipFree(b->p);
+    delete ctx->surrogate;
+    ctx->surrogate = nullptr;
+}
+
+static constexpr int64_t kWb = IN * HID + 2 * HID * HID + HID;  // BF16 weight copies
+
+static int sur_reserve(mptg_ctx* ctx, Surrogate& S, int64_t rows) {
+    if (rows <= S.cap) return MPTG_OK;
+    int rc;
+    const size_t bh = sizeof(uint16_t) * HID * rows;
+    if ((rc = ensure(ctx, S.x, sizeof(uint16_t) * IN * rows))) return rc;
+    if ((rc = ensure(ctx, S.xt, sizeof(uint16_t) * IN * rows))) return rc;
+    if ((rc = ensure(ctx, S.y, sizeof(float) * rows))) return rc;
+    for (DevBuf* b : {&S.h1, &S.h1t, &S.h2, &S.h2t, &S.h3, &S.h3t, &S.dh3, &S.dh3t, &S.dh2, &S.dh2t, &S.dh1, &S.dh1t})
+        if ((rc = ensure(ctx, *b, bh))) return rc;
+    if ((rc = ensure(ctx, S.z, sizeof(float) * rows))) return rc;
+    if ((rc = ensure(ctx, S.dz, sizeof(float) * rows))) return rc;
+    S.cap = rows;
+    return MPTG_OK;
+}
+
+static int sur_alloc(mptg_ctx* ctx, Surrogate& S) {
+    int rc;
+    for (DevBuf* b : {&S.params, &S.m, &S.v, &S.grad})
+        if ((rc = ensure(ctx, *b, sizeof(float) * NPARAM))) return rc;
+    if ((rc = ensure(ctx, S.wb, sizeof(uint16_t) * kWb))) return rc;
+    if ((rc = ensure(ctx, S.wtb, sizeof(uint16_t) * kWb))) return rc;
+    if ((rc = ensure(ctx, S.loss, sizeof(float) * 4))) return rc;
+    return MPTG_OK;
+}
+
+static int sur_cast(mptg_ctx* ctx, Surrogate& S) {
+    hipLaunchKernelGGL(cast_weights, dim3((unsigned)((kWb + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float*)S.params.p, (uint16_t*)S.wb.p, (uint16_t*)S.wtb.p);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+static void gemm(mptg_ctx* ctx, int mode, int M, int N, int K, const void* A, const void* Bt, Epi ep) {
+    unsigned blocks = (unsigned)(((int64_t)(M / 16) * (N / 16) + 3) / 4);
+    if (mode == EPI_FWD)
+        hipLaunchKernelGGL(gemm_nt<EPI_FWD>, dim3(blocks), dim3(256), 0, ctx->stream, M, N, K, (const uint16_t*)A,
+                           (const uint16_t*)Bt, ep);
+    else if (mode == EPI_DX)
+        hipLaunchKernelGGL(gemm_nt<EPI_DX>, dim3(blocks), dim3(256), 0, ctx->stream, M, N, K, (const uint16_t*)A,
+                           (const uint16_t*)Bt, ep);
+    else
+        hipLaunchKernelGGL(gemm_nt<EPI_F32>, dim3(blocks), dim3(256), 0, ctx->stream, M, N, K, (const uint16_t*)A,
+                           (const uint16_t*)Bt, ep);
+}
+
+// Forward over M rows of S.x (M multiple of 16); logits to S.z.  With
+// labels (S.y) also the loss, dz and dH3.
+static void sur_forward(mptg_ctx* ctx, Surrogate& S, int M, bool with_loss) {
+    const uint16_t* wt = (const uint16_t*)S.wtb.p;
+    const float* p = (const float*)S.params.p;
+    gemm(ctx, EPI_FWD, M, HID, IN, S.x.p, wt, Epi{p + OFF_B1, nullptr, (uint16_t*)S.h1.p, (uint16_t*)S.h1t.p, nullptr});
+    gemm(ctx, EPI_FWD, M, HID, HID, S.h1.p, wt + IN * HID,
+         Epi{p + OFF_B2, nullptr, (uint16_t*)S.h2.p, (uint16_t*)S.h2t.p, nullptr});
+    gemm(ctx, EPI_FWD, M, HID, HID, S.h2.p, wt + IN * HID + HID * HID,
+         Epi{p + OFF_B3, nullptr, (uint16_t*)S.h3.p, (uint16_t*)S.h3t.p, nullptr});
+    hipLaunchKernelGGL(head_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ctx->stream, M,
+                       (const uint16_t*)S.h3.p, (const uint16_t*)S.wb.p + IN * HID + 2 * HID * HID, p + OFF_B4,
+                       with_loss ? (const float*)S.y.p : nullptr, 1.0f / (float)M, (float*)S.z.p, (float*)S.dz.p,
+                       (uint16_t*)S.dh3.p, (uint16_t*)S.dh3t.p, (float*)S.loss.p);
+}
+
+// Gradient of the mean loss over the M rows already forwarded; into S.grad.
+static void sur_backward(mptg_ctx* ctx, Surrogate& S, int M) {
+    float* g = (float*)S.grad.p;
+    const uint16_t* wb = (const uint16_t*)S.wb.p;
+    hipLaunchKernelGGL(row_sums, dim3(HID), dim3(256), 0, ctx->stream, (const uint16_t*)S.h3t.p, M,
+                       (const float*)S.dz.p, g + OFF_W4);
+    hipLaunchKernelGGL(sum_f32, dim3(1), dim3(256), 0, ctx->stream, (const float*)S.dz.p, M, g + OFF_B4);
+    // layer 3
+    gemm(ctx, EPI_F32, HID, HID, M, S.h2t.p, S.dh3t.p, Epi{nullptr, nullptr, nullptr, nullptr, g + OFF_W3});
+    hipLaunchKernelGGL(row_sums, dim3(HID), dim3(256), 0, ctx->stream, (const uint16_t*)S.dh3t.p, M,
+                       (const float*)nullptr, g + OFF_B3);
+    gemm(ctx, EPI_DX, M, HID, HID, S.dh3.p, wb + IN * HID + HID * HID,
+         Epi{nullptr, (const uint16_t*)S.h2.p, (uint16_t*)S.dh2.p, (uint16_t*)S.dh2t.p, nullptr});
+    // layer 2
+    gemm(ctx, EPI_F32, HID, HID, M, S.h1t.p, S.dh2t.p, Epi{nullptr, nullptr, nullptr, nullptr, g + OFF_W2});
+    hipLaunchKernelGGL(row_sums, dim3(HID), dim3(256), 0, ctx->stream, (const uint16_t*)S.dh2t.p, M,
+                       (const float*)nullptr, g + OFF_B2);
+    gemm(ctx, EPI_DX, M, HID, HID, S.dh2.p, wb + IN * HID,
+         Epi{nullptr, (const uint16_t*)S.h1.p, (uint16_t*)S.dh1.p, (uint16_t*)S.dh1t.p, nullptr});
+    // layer 1
+    hipLaunchKernelGGL(transpose_bf16, dim3((unsigned)(((int64_t)M * IN + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const uint16_t*)S.x.p, M, IN, (uint16_t*)S.xt.p);
+    gemm(ctx, EPI_F32, IN, HID, M, S.xt.p, S.dh1t.p, Epi{nullptr, nullptr, nullptr, nullptr, g + OFF_W1});
+    hipLaunchKernelGGL(row_sums, dim3(HID), dim3(256), 0, ctx->stream, (const uint16_t*)S.dh1t.p, M,
+                       (const float*)nullptr, g + OFF_B1);
+}
+
+static int sur_check(mptg_ctx* ctx, bool need_init) {
+    Surrogate& S = surrogate(ctx);
+    if (need_init && !S.init) return fail(ctx, MPTG_E_STATE, "surrogate not initialised (mptg_surrogate_init)");
+    return MPTG_OK;
+}
+
+int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                          float* scores) {
+    Surrogate& S = surrogate(ctx);
+    if (!S.init) return fail(ctx, MPTG_E_STATE, "surrogate order needs an initialised surrogate");
+    if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks)
+        return fail(ctx, MPTG_E_STATE, "surrogate order is defined for the arm scene");
+    const float inv_total = (float)(1.0 / ctx->scene.total_length);
+    const unsigned blocks = (unsigned)(n * (kScoreStride / 64));
+    const uint16_t* wt = (const uint16_t*)S.wtb.p;
+    const uint16_t* wb = (const uint16_t*)S.wb.p;
+    const float* p = (const float*)S.params.p;
+    const double* L = (const double*)ctx->scene.lengths;
+    switch (ctx->dim) {
+        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
+        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
+        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
+        default: return fail(ctx, MPTG_E_STATE, "surrogate order: arms of 5, 10 or 20 links");
+    }
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+}  // namespace mptg
+
+using namespace mptg;
+
+extern "C" {
+
+int64_t mptg_surrogate_nparams(void) { return NPARAM; }
+
+// One training sample per edge: a waypoint of link(from_i, to_i), its
+// features and its exact label (device buffers).
+int mptg_surrogate_samples(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, uint64_t seed,
+                           uint16_t* d_feat, float* d_label) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks || ctx->link_mode != MPTG_LINK_BISECT)
+        return fail(ctx, MPTG_E_STATE, "surrogate samples are defined for the arm scene with bisection links");
+    if (n < 0 || (n > 0 && (!d_from || !d_to || !d_feat || !d_label))) return fail(ctx, MPTG_E_ARG, "bad buffers");
+    if (n == 0) return MPTG_OK;
+    const float inv_total = (float)(1.0 / ctx->scene.total_length);
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    const SceneArgs sa = scene_args(ctx);
+    switch (ctx->dim) {
+        case 5: hipLaunchKernelGGL(sample_waypoints<5>, dim3(blocks), dim3(256), 0, ctx->stream, d_from, d_to, n, ctx->link_param, sa, inv_total, seed, d_feat, d_label); break;
+        case 10: hipLaunchKernelGGL(sample_waypoints<10>, dim3(blocks), dim3(256), 0, ctx->stream, d_from, d_to, n, ctx->link_param, sa, inv_total, seed, d_feat, d_label); break;
+        case 20: hipLaunchKernelGGL(sample_waypoints<20>, dim3(blocks), dim3(256), 0, ctx->stream, d_from, d_to, n, ctx->link_param, sa, inv_total, seed, d_feat, d_label); break;
+        default: return fail(ctx, MPTG_E_ARG, "surrogate samples: arms of 5, 10 or 20 links");
+    }
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+// Surrogate scores of every waypoint of every edge: d_scores[i * 512 + o]
+// for reference ordinal o (slots beyond the edge's waypoint count untouched).
+int mptg_surrogate_score_edges(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
+                               float* d_scores) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!d_from || !d_to || !d_scores))) return fail(ctx, MPTG_E_ARG, "bad buffers");
+    if (n == 0) return MPTG_OK;
+    return surrogate_score_edges(ctx, d_from, d_to, n, nullptr, d_scores);
+}
+
+int mptg_surrogate_init(mptg_ctx* ctx, uint64_t seed) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    Surrogate& S = surrogate(ctx);
+    if ((rc = sur_alloc(ctx, S))) return rc;
+    hipLaunchKernelGGL(init_params, dim3((unsigned)((NPARAM + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (float*)S.params.p, seed);
+    MPTG_HIP(ctx, hipMemsetAsync(S.m.p, 0, sizeof(float) * NPARAM, ctx->stream));
+    MPTG_HIP(ctx, hipMemsetAsync(S.v.p, 0, sizeof(float) * NPARAM, ctx->stream));
+    if ((rc = sur_cast(ctx, S))) return rc;
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    S.adam_t = 0;
+    S.init = true;
+    return MPTG_OK;
+}
+
+int mptg_surrogate_get(mptg_ctx* ctx, float* h_params, int64_t n) {
+    int rc = check_device_ready(ctx);
+    if (rc || (rc = sur_check(ctx, true))) return rc;
+    if (n != NPARAM || !h_params) return fail(ctx, MPTG_E_ARG, "parameter buffer must hold mptg_surrogate_nparams()");
+    MPTG_HIP(ctx, hipMemcpyAsync(h_params, surrogate(ctx).params.p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+int mptg_surrogate_set(mptg_ctx* ctx, const float* h_params, int64_t n) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n != NPARAM || !h_params) return fail(ctx, MPTG_E_ARG, "parameter buffer must hold mptg_surrogate_nparams()");
+    Surrogate& S = surrogate(ctx);
+    if ((rc = sur_alloc(ctx, S))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(S.params.p, h_params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemsetAsync(S.m.p, 0, sizeof(float) * NPARAM, ctx->stream));
+    MPTG_HIP(ctx, hipMemsetAsync(S.v.p, 0, sizeof(float) * NPARAM, ctx->stream));
+    if ((rc = sur_cast(ctx, S))) return rc;
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    S.adam_t = 0;
+    S.init = true;
+    return MPTG_OK;
+}
+
+int mptg_surrogate_features(mptg_ctx* ctx, const double* d_states, int64_t n, uint16_t* d_feat) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks)
+        return fail(ctx, MPTG_E_STATE, "surrogate features are defined for the arm scene");
+    if (n < 0 || (n > 0 && (!d_states || !d_feat))) return fail(ctx, MPTG_E_ARG, "bad buffers");
+    if (n == 0) return MPTG_OK;
+    float inv_total = (float)(1.0 / ctx->scene.total_length);
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    switch (ctx->dim) {
+        case 5: hipLaunchKernelGGL(features_kernel<5>, dim3(blocks), dim3(256), 0, ctx->stream, d_states, n, (const double*)ctx->scene.lengths, inv_total, d_feat); break;
+        case 10: hipLaunchKernelGGL(features_kernel<10>, dim3(blocks), dim3(256), 0, ctx->stream, d_states, n, (const double*)ctx->scene.lengths, inv_total, d_feat); break;
+        case 20: hipLaunchKernelGGL(features_kernel<20>, dim3(blocks), dim3(256), 0, ctx->stream, d_states, n, (const double*)ctx->scene.lengths, inv_total, d_feat); break;
+        default: return fail(ctx, MPTG_E_ARG, "surrogate features: arms of 5, 10 or 20 links");
+    }
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+// Logits of n feature rows (device buffers).
+int mptg_surrogate_forward(mptg_ctx* ctx, const uint16_t* d_feat, int64_t n, float* d_logits) {
+    int rc = check_device_ready(ctx);
+    if (rc || (rc = sur_check(ctx, true))) return rc;
+    if (n < 0 || (n > 0 && (!d_feat || !d_logits))) return fail(ctx, MPTG_E_ARG, "bad buffers");
+    Surrogate& S = surrogate(ctx);
+    const int64_t chunk = 1 << 16;
+    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+        int64_t m = std::min(chunk, n - r0);
+        int M = (int)((m + 15) / 16 * 16);
+        if ((rc = sur_reserve(ctx, S, M))) return rc;
+        MPTG_HIP(ctx, hipMemsetAsync(S.x.p, 0, sizeof(uint16_t) * IN * M, ctx->stream));
+        MPTG_HIP(ctx, hipMemcpyAsync(S.x.p, d_feat + r0 * IN, sizeof(uint16_t) * IN * m, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        sur_forward(ctx, S, M, false);
+        MPTG_HIP(ctx, hipMemcpyAsync(d_logits + r0, S.z.p, sizeof(float) * m, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+// Mean loss and its gradient over n rows (one batch; n multiple of 16).
+int mptg_surrogate_loss_grad(mptg_ctx* ctx, const uint16_t* d_feat, const float* d_label, int64_t n, float* h_loss,
+                             float* h_grad) {
+    int rc = check_device_ready(ctx);
+    if (rc || (rc = sur_check(ctx, true))) return rc;
+    if (n <= 0 || n % 16 || n > (1 << 20) || !d_feat || !d_label)
+        return fail(ctx, MPTG_E_ARG, "batch rows must be a positive multiple of 16 (at most 2^20)");
+    Surrogate& S = surrogate(ctx);
+    if ((rc = sur_reserve(ctx, S, n))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(S.x.p, d_feat, sizeof(uint16_t) * IN * n, hipMemcpyDeviceToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(S.y.p, d_label, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemsetAsync(S.loss.p, 0, sizeof(float), ctx->stream));
+    sur_forward(ctx, S, (int)n, true);
+    sur_backward(ctx, S, (int)n);
+    MPTG_HIP(ctx, hipGetLastError());
+    if (h_loss) MPTG_HIP(ctx, hipMemcpyAsync(h_loss, S.loss.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (h_grad)
+        MPTG_HIP(ctx, hipMemcpyAsync(h_grad, S.grad.p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+// Adam training over n rows (device features/labels), in order, `epochs`
+// passes of batches of `batch` rows (a multiple of 16; a short last batch
+// is dropped).  h_losses (optional) receives the mean loss of each epoch.
+int mptg_surrogate_train(mptg_ctx* ctx, const uint16_t* d_feat, const float* d_label, int64_t n, int epochs,
+                         int batch, float lr, float* h_losses) {
+    int rc = check_device_ready(ctx);
+    if (rc || (rc = sur_check(ctx, true))) return rc;
+    if (batch <= 0 || batch % 16 || batch > (1 << 16)) return fail(ctx, MPTG_E_ARG, "batch must be a multiple of 16");
+    if (n < batch || epochs < 1 || !(lr > 0) || !d_feat || !d_label) return fail(ctx, MPTG_E_ARG, "bad training args");
+    Surrogate& S = surrogate(ctx);
+    if ((rc = sur_reserve(ctx, S, batch))) return rc;
+    const int64_t nb = n / batch;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs; ++ep) {
+        MPTG_HIP(ctx, hipMemsetAsync(S.loss.p, 0, sizeof(float) * 2, ctx->stream));
+        for (int64_t bi = 0; bi < nb; ++bi) {
+            const int64_t r0 = bi * batch;
+            MPTG_HIP(ctx, hipMemcpyAsync(S.x.p, d_feat + r0 * IN, sizeof(uint16_t) * IN * batch,
+                                         hipMemcpyDeviceToDevice, ctx->stream));
+            MPTG_HIP(ctx, hipMemcpyAsync(S.y.p, d_label + r0, sizeof(float) * batch, hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+            sur_forward(ctx, S, batch, true);
+            sur_backward(ctx, S, batch);
+            S.adam_t += 1;
+            float c1 = 1.f - std::pow(b1, (float)S.adam_t), c2 = 1.f - std::pow(b2, (float)S.adam_t);
+            hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((NPARAM + 255) / 256)), dim3(256), 0, ctx->stream,
+                               (float*)S.params.p, (float*)S.m.p, (float*)S.v.p, (const float*)S.grad.p,
+                               (int64_t)NPARAM, lr, b1, b2, eps, c1, c2);
+            if ((rc = sur_cast(ctx, S))) return rc;
+        }
+        MPTG_HIP(ctx, hipGetLastError());
+        if (h_losses) {
+            float tot = 0.f;
+            MPTG_HIP(ctx, hipMemcpyAsync(&tot, S.loss.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            h_losses[ep] = tot / (float)nb;
+        }
+    }
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+}  // extern "C"

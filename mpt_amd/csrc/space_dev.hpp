@@ -205,27 +205,87 @@ __device__ __forceinline__ bool disk_clear(const ArmDisk& dk, double fx, double 
     return d > dk.rr2;
 }
 
-// Arm validity for a state of D coordinates (nlinks <= D): same verdict as
-// arm_valid, registers only.  coord(i) yields the state's i-th coordinate, so
-// callers can produce coordinates on the fly instead of holding the state.
-template <int D, typename Coord>
-__device__ __forceinline__ bool arm_ok_fn(const SceneArgs& sc, Coord&& coord) {
-    double fx = 0.0, fy = 0.0, angle = 0.0;
+// Arm validity for a state of D = nlinks coordinates: same verdict as
+// arm_valid (the reference's FK and segment-vs-disk predicate), computed
+// disk-major with a sound broad phase:
+//   * forward kinematics with the reference's expression
+//     p_{i+1} = p_i + L_i (cos, sin)(sum of angles), one group of kArmGroup
+//     links at a time (group-major keeps only that group's joints live);
+//   * a group's points all lie within its
+//     half-length S_g/2 of m_g = (p_first + p_last)/2 (triangle inequality);
+//   * for every disk, a group whose circle (m_g, rho_g + r + R) misses the
+//     disk centre is skipped; rho_g = S_g/2 (1 + 2^-30) + 1e-12 dominates the
+//     FK rounding, so every skipped (disk, link) pair would have passed the
+//     exact test;
+//   * the remaining pairs run disk_clear (the reference expression).
+// coord(i) yields the state's i-th coordinate, so callers can generate the
+// coordinates on the fly instead of holding the state in registers.
+constexpr int kArmGroup = 5;
+
+// Work tally of arm_ok_fn (instrumented builds only): FK links, group tests,
+// link tests resolved by the far bound, link tests that ran the exact
+// expression, valid() calls.
+struct ArmWork {
+    uint32_t fk, group, far, exact, calls;
+};
+
+template <int D, bool COUNT = false, typename Coord>
+__device__ __forceinline__ bool arm_ok_fn(const SceneArgs& sc, Coord&& coord, ArmWork* wk = nullptr) {
+    constexpr int G = (D + kArmGroup - 1) / kArmGroup;
+    if constexpr (COUNT) wk->calls += 1;
+    double px = 0.0, py = 0.0, angle = 0.0;  // first joint of the current group
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-        if (i >= sc.nlinks) break;
-        angle += coord(i);
-        double s, c;
-        pm::sincos(angle, &s, &c);
-        double L = sc.lengths[i];
-        double tx = fx + L * c;
-        double ty = fy + L * s;
-        double vx = tx - fx, vy = ty - fy;
-        double li = sc.arm_lim[i];
-        for (int j = 0; j < sc.ndisks; ++j)
-            if (!disk_clear(sc.arm_disks[j], fx, fy, tx, ty, vx, vy, li)) return false;
-        fx = tx;
-        fy = ty;
+    for (int g = 0; g < G; ++g) {
+        constexpr int GS = kArmGroup;
+        const int i0 = g * GS;
+        const int n = (i0 + GS <= D) ? GS : D - i0;
+        double jx[GS + 1], jy[GS + 1];
+        jx[0] = px;
+        jy[0] = py;
+#pragma unroll
+        for (int k = 0; k < GS; ++k) {
+            if (k < n) {
+                angle += coord(i0 + k);
+                double s, c;
+                pm::sincos(angle, &s, &c);
+                double L = sc.lengths[i0 + k];
+                jx[k + 1] = jx[k] + L * c;
+                jy[k + 1] = jy[k] + L * s;
+                if constexpr (COUNT) wk->fk += 1;
+            }
+        }
+        double last_x = jx[0], last_y = jy[0];
+#pragma unroll
+        for (int k = 1; k <= GS; ++k)
+            if (k <= n) {
+                last_x = jx[k];
+                last_y = jy[k];
+            }
+        const double mx = (jx[0] + last_x) / 2, my = (jy[0] + last_y) / 2;
+        const double rho = sc.arm_lim[D + g];
+        for (int j = 0; j < sc.ndisks; ++j) {
+            const ArmDisk dk = sc.arm_disks[j];
+            double dx = dk.cx - mx, dy = dk.cy - my;
+            double R = rho + dk.rt;
+            if constexpr (COUNT) wk->group += 1;
+            if (dx * dx + dy * dy > R * R) continue;
+#pragma unroll
+            for (int k = 0; k < GS; ++k) {
+                if (k < n) {
+                    double vx = jx[k + 1] - jx[k], vy = jy[k + 1] - jy[k];
+                    if constexpr (COUNT) {
+                        double wx = dk.cx - jx[k], wy = dk.cy - jy[k];
+                        double far = sc.arm_lim[i0 + k] + dk.rt;
+                        if (wx * wx + wy * wy > far * far) wk->far += 1;
+                        else wk->exact += 1;
+                    }
+                    if (!disk_clear(dk, jx[k], jy[k], jx[k + 1], jy[k + 1], vx, vy, sc.arm_lim[i0 + k]))
+                        return false;
+                }
+            }
+        }
+        px = last_x;
+        py = last_y;
     }
     return true;
 }
@@ -240,7 +300,7 @@ __device__ __forceinline__ bool arm_ok(const SceneArgs& sc, const double* q) {
 template <int D>
 __device__ __forceinline__ bool state_valid_t(const SceneArgs& sc, const double* q) {
     if constexpr (D > 0) {
-        if (sc.kind == MPTG_SCENE_ARM) return arm_ok<D>(sc, q);
+        if (sc.kind == MPTG_SCENE_ARM && sc.nlinks == D) return arm_ok<D>(sc, q);
     }
     switch (sc.kind) {
         case MPTG_SCENE_BOXES: return boxes_valid(sc, q);

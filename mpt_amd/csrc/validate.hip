@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTableMax = 255;  // largest n = steps-1 served by the BFS table
+constexpr int64_t kScoreSlots = 512;  // surrogate scores per edge
 // Work limits of one link() call; an edge beyond them is not checked and
 // reports verdict 0 with checks = MPTG_CHECKS_OVER_LIMIT.
 constexpr int64_t kMaxDmvWaypoints = (int64_t)1 << 24;
@@ -419,7 +420,10 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > ctx->dim)
         return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
     if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
-    if (order != MPTG_ORDER_REFERENCE) return fail(ctx, MPTG_E_STATE, "surrogate order needs a trained surrogate");
+    if (order != MPTG_ORDER_REFERENCE && order != MPTG_ORDER_SURROGATE) return fail(ctx, MPTG_E_ARG, "unknown order");
+    if (order == MPTG_ORDER_SURROGATE &&
+        (ctx->scene.kind != MPTG_SCENE_ARM || ctx->link_mode != MPTG_LINK_BISECT || ctx->force_generic))
+        return fail(ctx, MPTG_E_STATE, "surrogate order is implemented for the arm scene with bisection links");
     if (n > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 edges per call");
     unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     if (ctx->link_mode == MPTG_LINK_BISECT) {
@@ -429,7 +433,14 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
             uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
             int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
             MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
-            rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks, irr, nirr);
+            const float* scores = nullptr;
+            if (order == MPTG_ORDER_SURROGATE) {
+                if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreSlots * n))) return rc;
+                if ((rc = surrogate_score_edges(ctx, d_from, d_to, n, d_mask, (float*)ctx->scratch[8].p))) return rc;
+                scores = (const float*)ctx->scratch[8].p;
+            }
+            rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks, irr, nirr, nullptr,
+                                 scores);
             if (rc == MPTG_OK) {
                 // listed (irregular) edges: exact per-edge walk; usually an empty launch
                 hipLaunchKernelGGL(link_bisect_list, dim3(16), dim3(kBlock), 0, ctx->stream, ctx->dim,
@@ -464,13 +475,23 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     return MPTG_OK;
 }
 
+// d_work[0..4]: the wave kernel's executed work (instrumented twin of the
+// timed kernel); d_work[5..7]: the reference algorithm's work on the same
+// edges (reference order, every disk tested until the first hit).
 int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int known_valid,
-                     unsigned long long* d_work) {
+                     unsigned long long* d_work, uint8_t* d_verdict, uint32_t* d_checks) {
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->link_mode != MPTG_LINK_BISECT)
         return fail(ctx, MPTG_E_STATE, "work counting is implemented for the arm scene with bisection links");
-    MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    int rc;
+    if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
+    uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
+    int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
+    MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
+    rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks, irr, nirr, d_work);
+    if (rc) return rc == MPTG_E_STATE ? fail(ctx, rc, "no wave kernel for this arm") : rc;
     hipLaunchKernelGGL(link_bisect_work, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
-                       ctx->dim, scene_args(ctx), ctx->link_param, d_from, d_to, n, known_valid, d_work);
+                       ctx->dim, scene_args(ctx), ctx->link_param, d_from, d_to, n, known_valid, d_work + 5);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

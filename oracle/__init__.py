@@ -72,6 +72,18 @@ _SIG = [
                                  C.c_int32, C.POINTER(OrcPoolResult)]),
 ]
 
+_SIG += [
+    ("orc_mlp_nparams", C.c_int64, []),
+    ("orc_bf16_round", None, [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_uint16)]),
+    ("orc_mlp_loss_grad", C.c_float, [C.POINTER(C.c_float), C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_int64,
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("orc_mlp_loss_grad_exact", C.c_double, [C.POINTER(C.c_float), C.POINTER(C.c_uint16), C.POINTER(C.c_float),
+                                             C.c_int64, _dp]),
+    ("orc_adam_step", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                             C.POINTER(C.c_float), C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
+                             C.c_int32]),
+]
+
 _lib = None
 
 
@@ -235,3 +247,50 @@ def dmv_order(steps: int) -> np.ndarray:
 
 def uniform_at(seed, index, purpose, j) -> float:
     return lib().orc_uniform_at(seed, index, purpose, j)
+
+
+# ---- surrogate MLP oracle ----------------------------------------------------
+def mlp_nparams() -> int:
+    return int(lib().orc_mlp_nparams())
+
+
+def bf16(x) -> np.ndarray:
+    """FP32 -> BF16 bits (round to nearest even)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(x.shape, dtype=np.uint16)
+    lib().orc_bf16_round(_p(x, C.c_float), x.size, _p(out, C.c_uint16))
+    return out
+
+
+def mlp_loss_grad(params, feat_bf16, labels):
+    """FP32 mode with the GPU's rounding points: (loss, grad, logits)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(feat_bf16, dtype=np.uint16)
+    y = np.ascontiguousarray(labels, dtype=np.float32)
+    n = y.size
+    g = np.zeros(p.size, dtype=np.float32)
+    z = np.zeros(n, dtype=np.float32)
+    loss = lib().orc_mlp_loss_grad(_p(p, C.c_float), _p(X, C.c_uint16), _p(y, C.c_float), n, _p(g, C.c_float),
+                                   _p(z, C.c_float))
+    return float(loss), g, z
+
+
+def mlp_loss_grad_exact(params, feat_bf16, labels, want_grad=True):
+    """FP64, no rounding."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(feat_bf16, dtype=np.uint16)
+    y = np.ascontiguousarray(labels, dtype=np.float32)
+    g = np.zeros(p.size) if want_grad else None
+    loss = lib().orc_mlp_loss_grad_exact(_p(p, C.c_float), _p(X, C.c_uint16), _p(y, C.c_float), y.size,
+                                         _p(g, C.c_double) if want_grad else None)
+    return float(loss), g
+
+
+def adam_step(params, m, v, grad, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):
+    p = np.array(params, dtype=np.float32)
+    m = np.array(m, dtype=np.float32)
+    v = np.array(v, dtype=np.float32)
+    g = np.ascontiguousarray(grad, dtype=np.float32)
+    lib().orc_adam_step(_p(p, C.c_float), _p(m, C.c_float), _p(v, C.c_float), _p(g, C.c_float), p.size, lr, beta1,
+                        beta2, eps, t)
+    return p, m, v

@@ -1,0 +1,210 @@
+"""GPU surrogate: forward/backward kernels against the CPU oracle, training on
+exact-checker labels, and the surrogate-ordered link() — same verdicts as the
+reference order, check counts equal to the oracle's count for the GPU's own
+scores, fewer checks in total."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import mpt_amd
+import oracle
+from mpt_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+NP = 41473
+OFF = {"W1": 0, "b1": 8192, "W2": 8320, "b2": 24704, "W3": 24832, "b3": 41216, "w4": 41344, "b4": 41472}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _bits_to_f32(u16):
+    return (u16.astype(np.uint32) << 16).view(np.float32)
+
+
+def _net(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NP, dtype=np.float32)
+    for w, b, fan in (("W1", "b1", 64), ("W2", "b2", 128), ("W3", "b3", 128), ("w4", "b4", 128)):
+        p[OFF[w]:OFF[b]] = rng.uniform(-1, 1, OFF[b] - OFF[w]) * math.sqrt(6 / fan)
+    p[OFF["b1"]:OFF["W2"]] = rng.normal(0, 0.1, 128)
+    p[OFF["b2"]:OFF["W3"]] = rng.normal(0, 0.1, 128)
+    p[OFF["b3"]:OFF["w4"]] = rng.normal(0, 0.1, 128)
+    p[OFF["b4"]] = 0.05
+    return p
+
+
+def _data(seed, n):
+    rng = np.random.default_rng(seed)
+    X = oracle.bf16(rng.normal(0, 1, (n, 64)).astype(np.float32))
+    y = (rng.uniform(size=n) < 0.4).astype(np.float32)
+    return X, y
+
+
+def _tensor_slices():
+    return [(k, OFF[k], OFF[n]) for k, n in (("W1", "b1"), ("b1", "W2"), ("W2", "b2"), ("b2", "W3"), ("W3", "b3"),
+                                              ("b3", "w4"), ("w4", "b4"))] + [("b4", OFF["b4"], NP)]
+
+
+def test_loss_grad_parity(ctx_factory):
+    _say("surrogate loss/grad parity")
+    ctx = ctx_factory()
+    p = _net(11)
+    X, y = _data(12, 512)
+    ctx.surrogate_set(p)
+    dX, dy = ctx.to_device(X), ctx.to_device(y)
+    loss, g = ctx.surrogate_loss_grad(dX.ptr, dy.ptr, 512)
+    ol, og, _ = oracle.mlp_loss_grad(p, X, y)
+    assert abs(loss - ol) <= 1e-4 * abs(ol)
+    for name, a, b in _tensor_slices():
+        gg, oo = g[a:b].astype(np.float64), og[a:b].astype(np.float64)
+        # FP32 accumulation order and rare BF16 rounding flips differ
+        assert np.linalg.norm(gg - oo) <= 1e-2 * np.linalg.norm(oo) + 1e-7, name
+        if b - a > 1:
+            assert float(np.dot(gg, oo)) >= 0.9999 * np.linalg.norm(gg) * np.linalg.norm(oo), name
+
+
+def test_forward_parity(ctx_factory):
+    _say("surrogate forward parity")
+    ctx = ctx_factory()
+    p = _net(13)
+    X, y = _data(14, 1000)  # not a multiple of 16: padded inside
+    ctx.surrogate_set(p)
+    dX = ctx.to_device(X)
+    dz = ctx.array(1000, np.float32)
+    ctx.surrogate_forward(dX.ptr, 1000, dz.ptr)
+    _, _, oz = oracle.mlp_loss_grad(p, X, y)
+    z = dz.numpy()
+    assert np.allclose(z, oz, rtol=2e-2, atol=2e-2)
+    assert np.corrcoef(z, oz)[0, 1] > 0.9999
+
+
+def test_adam_step_parity(ctx_factory):
+    _say("surrogate adam parity")
+    ctx = ctx_factory()
+    p = _net(15)
+    X, y = _data(16, 256)
+    ctx.surrogate_set(p)
+    dX, dy = ctx.to_device(X), ctx.to_device(y)
+    _, g = ctx.surrogate_loss_grad(dX.ptr, dy.ptr, 256)
+    ctx.surrogate_set(p)
+    ctx.surrogate_train(dX.ptr, dy.ptr, 256, epochs=1, batch=256, lr=1e-3)
+    p1 = ctx.surrogate_get()
+    e1, _, _ = oracle.adam_step(p, np.zeros(NP), np.zeros(NP), g, lr=1e-3, t=1)
+    assert np.mean(np.abs(p1 - e1) <= 1e-6) > 0.999
+
+
+def test_features_match_forward_kinematics(ctx_factory):
+    _say("surrogate features")
+    ctx = ctx_factory()
+    sc = scenes.arm_scene()
+    ctx.scene(sc)
+    rng = np.random.default_rng(17)
+    q = rng.uniform(-math.pi, math.pi, (500, 20))
+    f = ctx.array((500, 64), np.uint16)
+    dq = ctx.to_device(q)
+    ctx.surrogate_features(dq.ptr, 500, f.ptr)
+    got = _bits_to_f32(f.numpy())
+    ang = np.cumsum(q, axis=1)
+    x = np.cumsum(sc.lengths * np.cos(ang), axis=1) / sc.lengths.sum()
+    yy = np.cumsum(sc.lengths * np.sin(ang), axis=1) / sc.lengths.sum()
+    exp = np.zeros((500, 64))
+    exp[:, 0:40:2], exp[:, 1:40:2] = x, yy
+    exp[:, 40:64:2], exp[:, 41:64:2] = np.cos(ang[:, :12]), np.sin(ang[:, :12])
+    assert np.allclose(got, exp, rtol=1e-2, atol=2e-3)
+
+
+def _valid_states(ctx, sc, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    while sum(len(o) for o in out) < n:
+        q = rng.uniform(sc.lo, sc.hi, (4 * n, sc.dim))
+        out.append(q[ctx.valid(q)])
+    return np.concatenate(out)[:n]
+
+
+def _auc(scores, labels):
+    order = np.argsort(scores, kind="stable")
+    ranks = np.empty(len(scores))
+    ranks[order] = np.arange(1, len(scores) + 1)
+    pos = labels > 0.5
+    npos, nneg = pos.sum(), (~pos).sum()
+    return (ranks[pos].sum() - npos * (npos + 1) / 2) / (npos * nneg)
+
+
+@pytest.fixture(scope="module")
+def trained(ctx_factory):
+    """Arm scene + a surrogate trained on 2^17 exact-checker labels."""
+    ctx = ctx_factory()
+    sc = scenes.arm_scene()
+    ctx.scene(sc)
+    n = 1 << 17
+    a = _valid_states(ctx, sc, n, 21)
+    b = np.clip(a + np.random.default_rng(22).normal(0, 0.8, a.shape), -math.pi, math.pi)
+    feat = ctx.array((n, 64), np.uint16)
+    lab = ctx.array(n, np.float32)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    ctx.surrogate_samples(da.ptr, db.ptr, n, 23, feat.ptr, lab.ptr)
+    ctx.surrogate_init(24)
+    losses = ctx.surrogate_train(feat.ptr, lab.ptr, n, epochs=6, batch=1024, lr=2e-3)
+    return ctx, sc, losses, feat, lab
+
+
+def test_training_learns_collision_labels(trained):
+    _say("surrogate training")
+    ctx, sc, losses, _, lab = trained
+    _say("epoch losses", losses)
+    assert losses[-1] < 0.85 * losses[0]
+    m = 1 << 14
+    a = _valid_states(ctx, sc, m, 31)
+    b = np.clip(a + np.random.default_rng(32).normal(0, 0.8, a.shape), -math.pi, math.pi)
+    feat = ctx.array((m, 64), np.uint16)
+    y = ctx.array(m, np.float32)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    ctx.surrogate_samples(da.ptr, db.ptr, m, 33, feat.ptr, y.ptr)
+    z = ctx.array(m, np.float32)
+    ctx.surrogate_forward(feat.ptr, m, z.ptr)
+    yy = y.numpy()
+    auc = _auc(z.numpy(), yy)
+    _say("held-out AUC", auc, "positives", float(yy.mean()))
+    assert auc > 0.8
+
+
+def test_surrogate_order_parity_and_savings(trained):
+    _say("surrogate-ordered link")
+    ctx, sc, _, _, _ = trained
+    m = 512
+    a = _valid_states(ctx, sc, m, 41)
+    b = _valid_states(ctx, sc, m, 42)
+    v_ref, c_ref = ctx.link(a, b, mpt_amd.ORDER_REFERENCE, True)
+    v_sur, c_sur = ctx.link(a, b, mpt_amd.ORDER_SURROGATE, True)
+    assert np.array_equal(v_ref, v_sur)
+    ov, oc = oracle.Oracle(sc).link(a, b, known_valid=True, threads=16)
+    assert np.array_equal(v_ref, ov) and np.array_equal(c_ref, oc)
+    scores = ctx.array((m, 512), np.float32)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    ctx.surrogate_score_edges(da.ptr, db.ptr, m, scores.ptr)
+    sc_np = scores.numpy()
+    o = oracle.Oracle(sc)
+    for e in range(0, m, 4):
+        W = len(o.waypoints(a[e], b[e]))
+        assert c_sur[e] == o.checks_in_score_order(a[e], b[e], sc_np[e, :W], known_valid=True), e
+    # fused scores == generic forward on the same waypoints
+    for e in range(0, 24):
+        wp = o.waypoints(a[e], b[e])
+        W = len(wp)
+        f = ctx.array((W, 64), np.uint16)
+        dwp = ctx.to_device(wp)
+        ctx.surrogate_features(dwp.ptr, W, f.ptr)
+        z = ctx.array(W, np.float32)
+        ctx.surrogate_forward(f.ptr, W, z.ptr)
+        assert np.allclose(z.numpy(), sc_np[e, :W], rtol=1e-5, atol=1e-5)
+    coll = ~v_ref
+    _say("colliding", int(coll.sum()), "checks ref", int(c_ref[coll].sum()), "sur", int(c_sur[coll].sum()))
+    assert c_sur[coll].sum() < 0.8 * c_ref[coll].sum()
+    assert np.array_equal(c_sur[~coll], c_ref[~coll])  # valid edges check everything in both orders
