@@ -2,11 +2,14 @@
 
 Headline workload (default): the 20-link planar arm among 200 disks, batched
 PRRT with 8192 samples per iteration; one step = one iteration (sample ->
-nearest -> steer -> valid -> link -> insert for 8192 samples).  `value` is
-edge checks (link() calls decided) per second over the timed steps; the JSON
-line also carries extension attempts per second, the roofline of the dominant
-kernel (the arm link() kernel, FP64 VALU), and the CPU baseline (the
-reference-style std::thread worker pool, all host cores).  Extra lines for the
+nearest -> steer -> valid -> link -> insert for 8192 samples).  The planner
+runs twice from the same seed: with the reference waypoint order ("plain")
+and with the surrogate order (after training the surrogate on 2^22 exact
+labels, untimed).  `value` is the surrogate-ordered planner's edge checks
+(link() calls decided) per second; the line also carries the plain rate, the
+gain, extension attempts per second, the roofline of the dominant kernel (the
+arm link() kernel, FP64 VALU), the MFMA rate of the score kernel, and the CPU
+baseline (the reference-style std::thread worker pool, all host cores).  Extra lines for the
 nearest-neighbour kernel (1e6 x R^7, 65536 queries), the occupancy-grid edge
 kernel (4096^2 grid, 2^20 edges) and the boxes planner are reported under
 "extra".
@@ -31,6 +34,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK = 8.0e12        # B/s
 FP64_PEAK = 78.6e12      # FLOP/s (vector and matrix)
 FP32_PEAK = 157.3e12     # FLOP/s
+BF16_PEAK = 2.5e15       # FLOP/s, dense matrix
 # FP64 FLOP prices of the arm check's units of work
 FLOP_FK_LINK = 50        # angle add, pm::sincos (reduction + both kernels), 2 mul + 2 add
 FLOP_GROUP = 8           # group circle test: 2 sub, 3 mul, 2 add
@@ -53,30 +57,10 @@ class Timer:
         return self.ctx.time_ms(fn, reps) / 1e3
 
 
-def arm_edges(ctx, sc, tree_states, n, seed):
-    """Edges distributed like the planner's link() calls: (nearest tree node,
-    valid uniform sample)."""
-    rng = np.random.default_rng(seed)
-    out_a, out_b = [], []
-    need = n
-    while need > 0:
-        q = rng.uniform(sc.lo, sc.hi, (max(4 * need, 4096), sc.dim))
-        q = q[ctx.valid(q)]
-        idx, _ = ctx.knn(q, 1)
-        out_a.append(tree_states[idx[:, 0]])
-        out_b.append(q)
-        need -= q.shape[0]
-    return np.concatenate(out_a)[:n], np.concatenate(out_b)[:n]
-
-
-def bench_arm(ctx, args, timer):
-    import mpt_amd
-    from mpt_amd import scenes
-    sc = scenes.arm_scene()
-    B = 8192
-    ctx.scene(sc)
+def run_planner(ctx, sc, args, order, batch):
+    """Grows the arm tree from the start: warmup steps, then the timed steps."""
     ctx.tree_clear()
-    ctx.prrt_init(sc, seed=args.seed, batch=B)
+    ctx.prrt_init(sc, seed=args.seed, batch=batch, order=order)
     ctx.prrt_add_start(sc.start)
     ctx.prrt_run(1 << 30, args.warmup)
     s0 = ctx.prrt_stats()
@@ -87,43 +71,101 @@ def bench_arm(ctx, args, timer):
     dt = time.perf_counter() - t0
     s1 = ctx.prrt_stats()
     d = {k: s1[k] - s0[k] for k in s1}
+    nodes, parents = ctx.tree_read()
     res = {
         "ms_per_step": dt / args.steps * 1e3,
         "edge_checks_per_s": d["link_calls"] / dt,
         "extensions_per_s": d["samples"] / dt,
         "accepted_per_s": d["link_ok"] / dt,
         "valid_calls_per_s": d["valid_calls"] / dt,
+        "link_calls": d["link_calls"],
+        "valid_calls": d["valid_calls"],
         "tree_size": s1["size"],
-        "steps": args.steps,
-        "samples_per_step": B,
     }
-    # dominant kernel: link() over a planner-like edge batch
-    states, _ = ctx.tree_read()
+    return res, nodes, parents
+
+
+def fk_flops(wk):
+    return (FLOP_FK_LINK * wk["fk"] + FLOP_GROUP * wk["group"] + FLOP_LINK_FAR * wk["far"]
+            + FLOP_LINK_EXACT * wk["exact"])
+
+
+def bench_arm(ctx, args, timer):
+    import mpt_amd
+    from mpt_amd import scenes, surrogate
+    sc = scenes.arm_scene()
+    B = 8192
+    ctx.scene(sc)
+    log("  plain planner ...")
+    plain, nodes_p, par_p = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B)
+    log("  surrogate training ...")
+    train = surrogate.train(ctx, sc, nodes_p, n_samples=args.train_samples, epochs=args.train_epochs, seed=5)
+    log("  surrogate-ordered planner ...")
+    sur, nodes_s, par_s = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B)
+    same_tree = bool(np.array_equal(nodes_p, nodes_s) and np.array_equal(par_p, par_s))
+
+    # dominant kernels on a fixed batch of planner-like edges
     ne = 8192
-    a, b = arm_edges(ctx, sc, states, ne, seed=99)
+    a, b = surrogate.planner_edges(ctx, sc, nodes_p, ne, seed=99)
     da, db = ctx.to_device(a), ctx.to_device(b)
     v = ctx.array(ne, np.uint8)
     c = ctx.array(ne, np.uint32)
+    scores = ctx.array((ne, 512), np.float32)
+    out = {}
+    for name, order in (("plain", mpt_amd.ORDER_REFERENCE), ("surrogate", mpt_amd.ORDER_SURROGATE)):
+        def run(order=order):
+            ctx.link_dev(da.ptr, db.ptr, ne, order, True, v.ptr, c.ptr)
 
-    def run():
-        ctx.link_dev(da.ptr, db.ptr, ne, mpt_amd.ORDER_REFERENCE, True, v.ptr, c.ptr)
-
-    run()
-    ctx.sync()
-    t = timer.time(run, 3)
-    wk = ctx.link_work(da.ptr, db.ptr, ne, True)
-    flops = (FLOP_FK_LINK * wk["fk"] + FLOP_GROUP * wk["group"] + FLOP_LINK_FAR * wk["far"]
-             + FLOP_LINK_EXACT * wk["exact"])
-    ref_flops = FLOP_FK_LINK * wk["ref_fk"] + FLOP_REF_TEST * wk["ref_tests"]
-    calls = wk["calls"]
-    res["edge_batch"] = {
-        "edges": ne, "kernel_ms": t * 1e3, "edge_checks_per_s": ne / t, "valid_calls": calls,
-        "colliding_frac": float(1 - v.numpy().mean()),
+        run()
+        ctx.sync()
+        t = timer.time(run, 5)
+        out[name] = {"ms": t * 1e3, "edge_checks_per_s": ne / t, "verdicts": v.numpy().copy(),
+                     "checks": c.numpy().astype(np.int64)}
+    t_score = timer.time(lambda: ctx.surrogate_score_edges(da.ptr, db.ptr, ne, scores.ptr), 5)
+    wk_p = ctx.link_work(da.ptr, db.ptr, ne, True, mpt_amd.ORDER_REFERENCE)
+    wk_s = ctx.link_work(da.ptr, db.ptr, ne, True, mpt_amd.ORDER_SURROGATE)
+    same_verdicts = bool(np.array_equal(out["plain"]["verdicts"], out["surrogate"]["verdicts"]))
+    coll = ~out["plain"]["verdicts"].astype(bool)
+    wmax = np.abs(a - b).max(axis=1)
+    h = np.ceil(np.log2(np.maximum(wmax / sc.link_param, 1.0)) + 1e-12).astype(int)
+    h = np.where(wmax < sc.link_param, 0, h)
+    rows = int(((1 << h) - 1).sum())
+    mlp_flops = rows * 2 * (64 * 128 + 2 * 128 * 128 + 128)
+    t_link_sur = out["surrogate"]["ms"] / 1e3 - t_score
+    flops_p, flops_s = fk_flops(wk_p), fk_flops(wk_s)
+    t_plain = out["plain"]["ms"] / 1e3
+    edge_batch = {
+        "edges": ne,
+        "colliding_frac": float(coll.mean()),
+        "plain_ms": out["plain"]["ms"],
+        "surrogate_ms": out["surrogate"]["ms"],
+        "score_ms": t_score * 1e3,
+        "plain_edge_checks_per_s": out["plain"]["edge_checks_per_s"],
+        "surrogate_edge_checks_per_s": out["surrogate"]["edge_checks_per_s"],
+        "gain": out["surrogate"]["edge_checks_per_s"] / out["plain"]["edge_checks_per_s"],
+        "verdicts_identical": same_verdicts,
+        "reference_order_checks": int(out["plain"]["checks"].sum()),
+        "surrogate_order_checks": int(out["surrogate"]["checks"].sum()),
+        "work_saved": 1.0 - out["surrogate"]["checks"].sum() / out["plain"]["checks"].sum(),
+        "work_saved_colliding": 1.0 - out["surrogate"]["checks"][coll].sum() / max(1, out["plain"]["checks"][coll].sum()),
+        "gpu_waypoint_checks_plain": wk_p["calls"],
+        "gpu_waypoint_checks_surrogate": wk_s["calls"],
+        "plain_kernel_fp64_tflops": flops_p / t_plain / 1e12,
+        "surrogate_link_fp64_tflops": flops_s / max(t_link_sur, 1e-9) / 1e12,
+        "score_rows": rows,
+        "score_mfma_tflops": mlp_flops / t_score / 1e12,
+        "score_frac_bf16": mlp_flops / t_score / BF16_PEAK,
     }
-    roof = {"bound": "fp64", "achieved": flops / t / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": flops / t / FP64_PEAK, "traffic": None, "kernel": "arm_link_wave<20>",
-            "flops_per_launch": flops, "work": wk,
-            "reference_equivalent_tflops": ref_flops / t / 1e12}
+    roof = {"bound": "fp64", "achieved": flops_s / max(t_link_sur, 1e-9) / 1e12, "peak": FP64_PEAK / 1e12,
+            "unit": "TFLOP/s", "frac": flops_s / max(t_link_sur, 1e-9) / FP64_PEAK, "traffic": None,
+            "kernel": "arm_link_wave<20, surrogate order>", "flops_per_launch": flops_s, "work": wk_s,
+            "reference_equivalent_tflops": (FLOP_FK_LINK * wk_s["ref_fk"] + FLOP_REF_TEST * wk_s["ref_tests"])
+            / max(t_link_sur, 1e-9) / 1e12}
+    res = {"plain": plain, "surrogate": sur, "trees_identical": same_tree, "training": train,
+           "edge_batch": edge_batch,
+           "planner_gain": sur["edge_checks_per_s"] / plain["edge_checks_per_s"]}
+    for x in (da, db, v, c, scores):
+        x.free()
     return res, roof, sc
 
 
@@ -214,6 +256,8 @@ def main():
     ap.add_argument("--seed", type=int, default=17)
     ap.add_argument("--quick", action="store_true", help="skip extras and the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--train-samples", type=int, default=1 << 22)
+    ap.add_argument("--train-epochs", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -235,23 +279,26 @@ def main():
     if not args.quick:
         log("cpu baseline ...")
         cpu = cpu_baseline(sc, args.cpu_seconds)
+    sur = arm["surrogate"]
     line = {
         "metric": "RRT extensions/sec + edge checks/sec, surrogate-ordered vs plain, 1 MI355X",
-        "value": arm["edge_checks_per_s"],
+        "value": sur["edge_checks_per_s"],
         "unit": "edge checks/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": arm["ms_per_step"],
+        "ms_per_step": sur["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded scenes and Philox samples)",
         "config": {"workload": "arm20_disks200_prrt_b8192", "links": 20, "disks": 200, "batch": 8192,
-                   "check_order": "reference"},
-        "extensions_per_s": arm["extensions_per_s"],
-        "accepted_per_s": arm["accepted_per_s"],
+                   "check_order": "surrogate", "surrogate": "mlp 64-3x128-1 relu bf16"},
+        "extensions_per_s": sur["extensions_per_s"],
+        "plain_edge_checks_per_s": arm["plain"]["edge_checks_per_s"],
+        "plain_extensions_per_s": arm["plain"]["extensions_per_s"],
+        "surrogate_gain": arm["planner_gain"],
         "planner": arm,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -197,7 +197,7 @@ int mptg_surrogate_score_edges(mptg_ctx* ctx, const double* d_from, const double
  *      tests against the host build. */
 enum mptg_diag_op { MPTG_DIAG_SIN = 0, MPTG_DIAG_COS = 1, MPTG_DIAG_ACOS = 2, MPTG_DIAG_SQRT = 3 };
 int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y);
-/* Work of a reference-order link() batch on the arm scene, from untimed
+/* Work of a link() batch on the arm scene in the given order, from untimed
  * instrumented copies of the kernels (used to price a launch in FLOPs):
  *   h_work[0..4] = the GPU kernel's executed work: valid() calls, forward-
  *                  kinematics links, group broad-phase tests, link tests
@@ -206,7 +206,7 @@ int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* 
  *   h_work[5..7] = the reference algorithm's work on the same edges: valid()
  *                  calls, FK links, segment-disk tests (every disk until the
  *                  first hit). */
-int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
+int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                    int endpoints_known_valid, uint64_t* h_work);
 
 #ifdef __cplusplus

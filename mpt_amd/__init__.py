@@ -253,11 +253,13 @@ class Context:
         self._chk(_lib.mptg_diag_math(self._h, op, _p(x, C.c_double), x.size, _p(y, C.c_double)))
         return y
 
-    def link_work(self, a_ptr: int, b_ptr: int, n: int, known_valid: bool = False) -> dict:
-        """Executed work of the arm link kernel and the reference algorithm's
-        work on the same edges (device buffers, reference order)."""
+    def link_work(self, a_ptr: int, b_ptr: int, n: int, known_valid: bool = False,
+                  order: int = ORDER_REFERENCE) -> dict:
+        """Executed work of the arm link kernel in the given order and the
+        reference algorithm's work on the same edges (device buffers)."""
         w = (C.c_uint64 * 8)()
-        self._chk(_lib.mptg_link_work(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, 1 if known_valid else 0, w))
+        self._chk(_lib.mptg_link_work(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, order,
+                                      1 if known_valid else 0, w))
         keys = ("calls", "fk", "group", "far", "exact", "ref_calls", "ref_fk", "ref_tests")
         return {k: int(v) for k, v in zip(keys, w)}
 

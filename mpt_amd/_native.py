@@ -86,7 +86,7 @@ _SIGNATURES = [
                                        C.POINTER(C.c_float)]),
     ("mptg_surrogate_score_edges", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     ("mptg_diag_math", C.c_int, [C.c_void_p, C.c_int, c_double_p, C.c_int64, c_double_p]),
-    ("mptg_link_work", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,
+    ("mptg_link_work", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int,
                                  C.POINTER(C.c_uint64)]),
 ]
 

@@ -86,9 +86,23 @@ def build_oracle() -> Path:
     return out
 
 
+def build_examples() -> list[Path]:
+    """C++ programs on the header layer (include/mptg.h +
+    mpt_amd/include/mpt_amd/*.hpp), linked against libmptg.so."""
+    cxx = shutil.which("g++") or "g++"
+    outs = []
+    for src in sorted((PKG / "examples").glob("*.cpp")):
+        out = src.with_suffix(".bin")
+        _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"), "-I", str(PKG / "include"), str(src),
+              "-L", str(PKG), "-lmptg", f"-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-o", str(out)])
+        outs.append(out)
+    return outs
+
+
 def build_all() -> None:
     build_oracle()
     build_lib()
+    build_examples()
 
 
 if __name__ == "__main__":

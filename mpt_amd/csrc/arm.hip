@@ -87,107 +87,175 @@ __device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[NE], int lane)
     }
 }
 
-// One edge per wave (the kernel body; COUNT adds the work tally; ORD
-// selects the visiting order: reference pre-order, or descending surrogate
-// score from scores[e * kScoreStride + ordinal]).
-template <int D, bool COUNT, int ORD>
-__device__ __forceinline__ void arm_link_edge(const double* __restrict__ from, const double* __restrict__ to, int64_t n,
-                                              int known_valid, const uint8_t* __restrict__ mask, double tol,
-                                              const SceneArgs& sc, const float* __restrict__ scores,
-                                              uint8_t* __restrict__ verdict,
-                                              uint32_t* __restrict__ checks, int64_t* __restrict__ irregular,
-                                              uint32_t* __restrict__ n_irregular, ArmWork* wk) {
-    // no block-wide barriers in this kernel: waves leave independently
-    const int lane = lane_id();
-    const int w = uniform((int)(threadIdx.x >> 6));
-    const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + w;
-    if (e >= n) return;
-    if (mask && !mask[e]) return;
-    const double* a = from + e * D;  // wave-uniform: scalar loads
-    const double* b = to + e * D;
-    uint32_t base = 2;
-    if (!known_valid) {
-        bool okA = true, okB = true;
-        if (lane == 0) okA = arm_ok_fn<D, COUNT>(sc, [&](int i) { return a[i]; }, wk);
-        if (lane == 1) okB = arm_ok_fn<D, COUNT>(sc, [&](int i) { return b[i]; }, wk);
-        uint64_t badA = __ballot(!okA), badB = __ballot(!okB);
-        if (badA | badB) {
-            if (lane == 0) {
-                verdict[e] = 0;
-                checks[e] = badA ? 1u : 2u;
-            }
-            return;
-        }
-    }
-    // tree height from the widest coordinate, and the regularity test
+// Ranked ordinals of one edge per wave: sort the edge's waypoint scores
+// (descending, ties by reference ordinal) and write the reference ordinal of
+// every rank to ranked[e * kScoreStride + rank].
+__global__ __launch_bounds__(64) void rank_scores(const double* __restrict__ from, const double* __restrict__ to,
+                                                  int64_t n, int dim, const uint8_t* __restrict__ mask, double tol,
+                                                  const float* __restrict__ scores, uint16_t* __restrict__ ranked) {
+    const int64_t e = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (e >= n || (mask && !mask[e])) return;
     int h = 0;
     bool regular = false;
-    if (!bisect_shape<D>(a, b, tol, &h, &regular)) {
-        if (lane == 0) {
-            verdict[e] = 0;
-            checks[e] = MPTG_CHECKS_OVER_LIMIT;
-        }
-        return;
+    // shape from the widest coordinate (same rule as the link kernels)
+    double wmax = 0.0, M = 0.0;
+    for (int j = 0; j < dim; ++j) {
+        double x = from[e * dim + j], y = to[e * dim + j];
+        wmax = __builtin_fmax(wmax, __builtin_fabs(x - y));
+        M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(x), __builtin_fabs(y)));
     }
-    if (!regular) {  // exact per-edge walk in a second launch
-        if (lane == 0) irregular[atomicAdd(n_irregular, 1u)] = e;
-        return;
+    double wd = wmax;
+    while (!(wd < tol) && h <= kMaxArmDepth) {
+        wd *= 0.5;
+        ++h;
     }
+    (void)M;
+    (void)regular;
+    if (h > 9) return;  // more waypoints than score slots: reference order
     const uint32_t W = (1u << h) - 1u;
     uint64_t key[kScoreStride / kWave];
-    if constexpr (ORD == MPTG_ORDER_SURROGATE) {
-        if (W > kScoreStride) {  // deeper than the scored set: reference order
-            if (lane == 0) irregular[atomicAdd(n_irregular, 1u)] = e;
-            return;
-        }
 #pragma unroll
-        for (int k = 0; k < kScoreStride / kWave; ++k) {
-            const uint32_t o = (uint32_t)(k * kWave + lane);
-            key[k] = o < W ? score_key(scores[e * kScoreStride + o], o) : ~0ull;
-        }
-        wave_bitonic_sort<kScoreStride / kWave>(key, lane);
+    for (int k = 0; k < kScoreStride / kWave; ++k) {
+        const uint32_t o = (uint32_t)(k * kWave + lane);
+        key[k] = o < W ? score_key(scores[e * kScoreStride + o], o) : ~0ull;
     }
-    for (uint32_t c0 = 0, c = 0; c0 < W; c0 += kWave, ++c) {
-        uint32_t o = c0 + (uint32_t)lane;
-        if constexpr (ORD == MPTG_ORDER_SURROGATE) {
-            uint64_t pick = key[0];
+    wave_bitonic_sort<kScoreStride / kWave>(key, lane);
 #pragma unroll
-            for (int k = 1; k < kScoreStride / kWave; ++k)
-                if (c == (uint32_t)k) pick = key[k];
-            o = (uint32_t)pick;  // rank c0 + lane -> reference ordinal
-        }
-        bool bad = false;
-        if (c0 + (uint32_t)lane < W) {
-            uint32_t path;
-            int depth;
-            preorder_path(h, o, &path, &depth);
-            bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, wk);
-        }
-        uint64_t m = __ballot(bad);
-        if (m) {
-            if (lane == 0) {
-                verdict[e] = 0;
-                checks[e] = base + c0 + (uint32_t)__builtin_ctzll(m) + 1u;
-            }
-            return;
-        }
-    }
-    if (lane == 0) {
-        verdict[e] = 1;
-        checks[e] = base + W;
+    for (int k = 0; k < kScoreStride / kWave; ++k) {
+        const uint32_t r = (uint32_t)(k * kWave + lane);
+        if (r < W) ranked[e * kScoreStride + r] = (uint16_t)key[k];
     }
 }
 
+// Reference ordinal of the waypoint at `rank` in the chosen order.
+template <int ORD>
+__device__ __forceinline__ uint32_t ordinal_at(const uint16_t* __restrict__ ranked, int64_t e, uint32_t rank) {
+    if constexpr (ORD == MPTG_ORDER_SURROGATE) return ranked[e * kScoreStride + rank];
+    return rank;
+}
 
+constexpr int kHeadRanks = 4;                     // ranks per edge in the packed round
+constexpr int kHeadEdges = kWave / kHeadRanks;    // edges per wave in the packed round
+
+// Packed round: 16 edges per wave, 4 lanes per edge.  Endpoint checks (when
+// not vouched for), the tree shape, then ranks 0..3 of the chosen order.
+// Edges still undecided go to the continuation list.
 template <int D, bool COUNT, int ORD>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_wave(
+__global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const double* __restrict__ from, const double* __restrict__ to, int64_t n, int known_valid,
-    const uint8_t* __restrict__ mask, double tol, SceneArgs sc, const float* __restrict__ scores,
+    const uint8_t* __restrict__ mask, double tol, SceneArgs sc, const uint16_t* __restrict__ ranked,
     uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks, int64_t* __restrict__ irregular,
-    uint32_t* __restrict__ n_irregular, unsigned long long* __restrict__ work) {
+    uint32_t* __restrict__ n_irregular, int64_t* __restrict__ cont, uint32_t* __restrict__ n_cont,
+    unsigned long long* __restrict__ work) {
     ArmWork wk{0, 0, 0, 0, 0};
-    arm_link_edge<D, COUNT, ORD>(from, to, n, known_valid, mask, tol, sc, scores, verdict, checks, irregular,
-                                 n_irregular, &wk);
+    const int lane = lane_id();
+    const int slot = lane / kHeadRanks, rk = lane % kHeadRanks;
+    const int64_t e = ((int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) * kHeadEdges + slot;
+    bool live = e < n && (!mask || mask[e]);
+    const int64_t ec = live ? e : 0;
+    const double* a = from + ec * D;
+    const double* b = to + ec * D;
+    const uint32_t base = 2;
+    if (!known_valid) {
+        bool bad = false;
+        if (live && rk < 2)
+            bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return rk == 0 ? a[i] : b[i]; }, &wk);
+        const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
+        if (live && (g & 3u)) {
+            if (rk == 0) {
+                verdict[e] = 0;
+                checks[e] = (g & 1u) ? 1u : 2u;
+            }
+            live = false;
+        }
+    }
+    int h = 0;
+    bool regular = false;
+    if (live) {
+        if (!bisect_shape<D>(a, b, tol, &h, &regular)) {
+            if (rk == 0) {
+                verdict[e] = 0;
+                checks[e] = MPTG_CHECKS_OVER_LIMIT;
+            }
+            live = false;
+        } else if (!regular || (ORD == MPTG_ORDER_SURROGATE && h > 9)) {
+            if (rk == 0) irregular[atomicAdd(n_irregular, 1u)] = e;  // exact per-edge walk later
+            live = false;
+        }
+    }
+    const uint32_t W = live ? (1u << h) - 1u : 0u;
+    bool bad = false;
+    if (live && (uint32_t)rk < W) {
+        uint32_t path;
+        int depth;
+        preorder_path(h, ordinal_at<ORD>(ranked, e, (uint32_t)rk), &path, &depth);
+        bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, &wk);
+    }
+    const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
+    if (live && rk == 0) {
+        if (g) {
+            verdict[e] = 0;
+            checks[e] = base + (uint32_t)__builtin_ctz(g) + 1u;
+        } else if (W <= (uint32_t)kHeadRanks) {
+            verdict[e] = 1;
+            checks[e] = base + W;
+        } else {
+            cont[atomicAdd(n_cont, 1u)] = e;
+        }
+    }
+    if constexpr (COUNT) {
+        atomicAdd(&work[0], (unsigned long long)wk.calls);
+        atomicAdd(&work[1], (unsigned long long)wk.fk);
+        atomicAdd(&work[2], (unsigned long long)wk.group);
+        atomicAdd(&work[3], (unsigned long long)wk.far);
+        atomicAdd(&work[4], (unsigned long long)wk.exact);
+    }
+}
+
+// Continuation: one wave per listed edge, 64 ranks per round from rank 4.
+template <int D, bool COUNT, int ORD>
+__global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
+    const double* __restrict__ from, const double* __restrict__ to, double tol, SceneArgs sc,
+    const uint16_t* __restrict__ ranked, const int64_t* __restrict__ cont, const uint32_t* __restrict__ n_cont,
+    uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks, unsigned long long* __restrict__ work) {
+    // no block-wide barriers: waves leave independently
+    ArmWork wk{0, 0, 0, 0, 0};
+    const int lane = lane_id();
+    const int64_t widx = (int64_t)blockIdx.x * kWavesPerBlock + uniform((int)(threadIdx.x >> 6));
+    if (widx < (int64_t)*n_cont) {
+        const int64_t e = cont[widx];
+        const double* a = from + e * D;  // wave-uniform: scalar loads
+        const double* b = to + e * D;
+        int h = 0;
+        bool regular = false;
+        bisect_shape<D>(a, b, tol, &h, &regular);  // listed edges are regular
+        const uint32_t W = (1u << h) - 1u;
+        const uint32_t base = 2;
+        bool decided = false;
+        for (uint32_t c0 = kHeadRanks; c0 < W && !decided; c0 += kWave) {
+            const uint32_t r = c0 + (uint32_t)lane;
+            bool bad = false;
+            if (r < W) {
+                uint32_t path;
+                int depth;
+                preorder_path(h, ordinal_at<ORD>(ranked, e, r), &path, &depth);
+                bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, &wk);
+            }
+            const uint64_t m = __ballot(bad);
+            if (m) {
+                if (lane == 0) {
+                    verdict[e] = 0;
+                    checks[e] = base + c0 + (uint32_t)__builtin_ctzll(m) + 1u;
+                }
+                decided = true;
+            }
+        }
+        if (!decided && lane == 0) {
+            verdict[e] = 1;
+            checks[e] = base + W;
+        }
+    }
     if constexpr (COUNT) {
         atomicAdd(&work[0], (unsigned long long)wk.calls);
         atomicAdd(&work[1], (unsigned long long)wk.fk);
@@ -199,43 +267,60 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_wave(
 
 }  // namespace
 
-template <int D>
-static void launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv, const uint8_t* mask,
-                         const float* scores, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
-                         unsigned long long* work) {
-    unsigned blocks = (unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock);
-    dim3 g(blocks), t(kWave * kWavesPerBlock);
+template <int D, bool COUNT, int ORD>
+static void launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv,
+                              const uint8_t* mask, const uint16_t* ranked, uint8_t* verdict, uint32_t* checks,
+                              int64_t* irr, uint32_t* nirr, int64_t* cont, uint32_t* ncont,
+                              unsigned long long* work) {
     const double tol = ctx->link_param;
     const SceneArgs sa = scene_args(ctx);
-    if (scores) {
-        if (work)
-            hipLaunchKernelGGL((arm_link_wave<D, true, MPTG_ORDER_SURROGATE>), g, t, 0, ctx->stream, from, to, n, kv,
-                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
-        else
-            hipLaunchKernelGGL((arm_link_wave<D, false, MPTG_ORDER_SURROGATE>), g, t, 0, ctx->stream, from, to, n, kv,
-                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
+    const int64_t per_block = (int64_t)kWavesPerBlock * kHeadEdges;
+    hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD>), dim3((unsigned)((n + per_block - 1) / per_block)),
+                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, ranked, verdict,
+                       checks, irr, nirr, cont, ncont, work);
+    hipLaunchKernelGGL((arm_link_tail<D, COUNT, ORD>), dim3((unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, tol, sa, ranked, cont, ncont, verdict,
+                       checks, work);
+}
+
+template <int D>
+static void launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv, const uint8_t* mask,
+                         const uint16_t* ranked, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
+                         int64_t* cont, uint32_t* ncont, unsigned long long* work) {
+    if (ranked) {
+        if (work) launch_arm_phases<D, true, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
+        else launch_arm_phases<D, false, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
     } else {
-        if (work)
-            hipLaunchKernelGGL((arm_link_wave<D, true, MPTG_ORDER_REFERENCE>), g, t, 0, ctx->stream, from, to, n, kv,
-                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
-        else
-            hipLaunchKernelGGL((arm_link_wave<D, false, MPTG_ORDER_REFERENCE>), g, t, 0, ctx->stream, from, to, n, kv,
-                               mask, tol, sa, scores, verdict, checks, irr, nirr, work);
+        if (work) launch_arm_phases<D, true, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
+        else launch_arm_phases<D, false, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
     }
 }
 
-// Launches the wave kernel; edges it lists as irregular are left for the
-// caller's exact per-edge kernel.  Returns MPTG_E_STATE when no wave kernel
-// exists for this arm.
+// Launches the packed round and the continuation (after ranking the
+// waypoints when scores are given); edges the kernels list as irregular are
+// left for the caller's exact per-edge kernel (irr/nirr).  Returns
+// MPTG_E_STATE when no kernel exists for this arm.
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
                     const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
                     unsigned long long* work, const float* scores) {
     if (ctx->dim != ctx->scene.nlinks) return MPTG_E_STATE;
+    if (ctx->dim != 5 && ctx->dim != 10 && ctx->dim != 20) return MPTG_E_STATE;
+    int rc;
+    if ((rc = ensure(ctx, ctx->scratch[10], sizeof(int64_t) * n + 256))) return rc;
+    uint32_t* ncont = (uint32_t*)ctx->scratch[10].p;
+    int64_t* cont = (int64_t*)((char*)ctx->scratch[10].p + 256);
+    MPTG_HIP(ctx, hipMemsetAsync(ncont, 0, sizeof(uint32_t), ctx->stream));
+    const uint16_t* ranked = nullptr;
+    if (scores) {
+        if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kScoreStride * n))) return rc;
+        hipLaunchKernelGGL(rank_scores, dim3((unsigned)n), dim3(kWave), 0, ctx->stream, from, to, n, ctx->dim, mask,
+                           ctx->link_param, scores, (uint16_t*)ctx->scratch[9].p);
+        ranked = (const uint16_t*)ctx->scratch[9].p;
+    }
     switch (ctx->dim) {
-        case 5: launch_arm_d<5>(ctx, from, to, n, known_valid, mask, scores, verdict, checks, irr, nirr, work); break;
-        case 10: launch_arm_d<10>(ctx, from, to, n, known_valid, mask, scores, verdict, checks, irr, nirr, work); break;
-        case 20: launch_arm_d<20>(ctx, from, to, n, known_valid, mask, scores, verdict, checks, irr, nirr, work); break;
-        default: return MPTG_E_STATE;
+        case 5: launch_arm_d<5>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;
+        case 10: launch_arm_d<10>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;
+        default: launch_arm_d<20>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;
     }
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;

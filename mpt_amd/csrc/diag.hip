@@ -28,11 +28,11 @@ __global__ void diag_math_kernel(int op, const double* __restrict__ x, int64_t n
 using namespace mptg;
 
 namespace mptg {
-int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int known_valid,
+int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order, int known_valid,
                      unsigned long long* d_work, uint8_t* d_verdict, uint32_t* d_checks);
 }
 
-extern "C" int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
+extern "C" int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                               int endpoints_known_valid, uint64_t* h_work) {
     int rc = check_device_ready(ctx);
     if (rc) return rc;
@@ -42,7 +42,7 @@ extern "C" int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double*
     unsigned long long* w = (unsigned long long*)ctx->scratch[3].p;
     uint32_t* chk = (uint32_t*)ctx->scratch[2].p;
     uint8_t* ver = (uint8_t*)(chk + std::max<int64_t>(n, 1));
-    if ((rc = launch_link_work(ctx, d_from, d_to, n, endpoints_known_valid, w, ver, chk))) return rc;
+    if ((rc = launch_link_work(ctx, d_from, d_to, n, order, endpoints_known_valid, w, ver, chk))) return rc;
     MPTG_HIP(ctx, hipMemcpyAsync(h_work, w, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return MPTG_OK;

@@ -274,35 +274,45 @@ __global__ void features_kernel(const double* __restrict__ q, int64_t n, const d
 // resident, 80 KiB).
 constexpr int kLdsRow = HID + 8;  // padded row (BF16 elements)
 
+// One layer for the wave's 64 rows: the layer input (64 x K, BF16, in LDS)
+// is read once into registers as MFMA A fragments, then each 16-column slab
+// of the output loads its weight fragments once and reuses them for all four
+// row tiles; bias + ReLU results overwrite the LDS buffer.
 template <int K>
-__device__ __forceinline__ void wave_layer(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
-                                           const float* __restrict__ bias, uint16_t* __restrict__ out, int lane) {
+__device__ __forceinline__ void wave_layer(uint16_t* __restrict__ buf, const uint16_t* __restrict__ wt,
+                                           const float* __restrict__ bias, int lane) {
+    constexpr int KS = K / 32;
     const int r = lane & 15, kq = lane >> 4;
-#pragma unroll 1
-    for (int tm = 0; tm < 4; ++tm) {
-        f32x4 acc[8];
+    bf16x8 af[4][KS];
 #pragma unroll
-        for (int tn = 0; tn < 8; ++tn) acc[tn] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tm = 0; tm < 4; ++tm)
 #pragma unroll
-        for (int k0 = 0; k0 < K; k0 += 32) {
-            bf16x8 a = load8(in + (tm * 16 + r) * kLdsRow + k0 + 8 * kq);
+        for (int kk = 0; kk < KS; ++kk) af[tm][kk] = load8(buf + (tm * 16 + r) * kLdsRow + kk * 32 + 8 * kq);
+    __syncthreads();  // every lane has its fragments before the buffer is overwritten
+#pragma unroll 2
+    for (int tn = 0; tn < 8; ++tn) {
+        bf16x8 bf[KS];
 #pragma unroll
-            for (int tn = 0; tn < 8; ++tn) {
-                bf16x8 b = load8(wt + (tn * 16 + r) * K + k0 + 8 * kq);
-                acc[tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tn], 0, 0, 0);
-            }
+        for (int kk = 0; kk < KS; ++kk) bf[kk] = load8(wt + (tn * 16 + r) * K + kk * 32 + 8 * kq);
+        f32x4 acc[4];
+#pragma unroll
+        for (int tm = 0; tm < 4; ++tm) {
+            acc[tm] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < KS; ++kk)
+                acc[tm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[tm][kk], bf[kk], acc[tm], 0, 0, 0);
         }
+        const int col = tn * 16 + r;
+        const float bb = bias[col];
 #pragma unroll
-        for (int tn = 0; tn < 8; ++tn) {
-            const int col = tn * 16 + r;
-            const float bb = bias[col];
+        for (int tm = 0; tm < 4; ++tm)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float v = acc[tn][i] + bb;
-                out[(tm * 16 + 4 * kq + i) * kLdsRow + col] = float_to_bf16(v > 0.f ? v : 0.f);
+                float v = acc[tm][i] + bb;
+                buf[(tm * 16 + 4 * kq + i) * kLdsRow + col] = float_to_bf16(v > 0.f ? v : 0.f);
             }
-        }
     }
+    __syncthreads();
 }
 
 template <int D>
@@ -311,8 +321,7 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
                                                       const double* __restrict__ lengths, float inv_total,
                                                       const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
                                                       const float* __restrict__ p, float* __restrict__ scores) {
-    __shared__ __attribute__((aligned(16))) uint16_t bufA[64 * kLdsRow];
-    __shared__ __attribute__((aligned(16))) uint16_t bufB[64 * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t buf[64 * kLdsRow];
     const int lane = threadIdx.x;
     const int64_t e = blockIdx.x / (kScoreStride / 64);
     const uint32_t c0 = (uint32_t)(blockIdx.x % (kScoreStride / 64)) * 64;
@@ -346,19 +355,16 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
         v.y = (uint32_t)f[j + 2] | ((uint32_t)f[j + 3] << 16);
         v.z = (uint32_t)f[j + 4] | ((uint32_t)f[j + 5] << 16);
         v.w = (uint32_t)f[j + 6] | ((uint32_t)f[j + 7] << 16);
-        *reinterpret_cast<uint4*>(bufA + lane * kLdsRow + j) = v;
+        *reinterpret_cast<uint4*>(buf + lane * kLdsRow + j) = v;
     }
     __syncthreads();
-    wave_layer<IN>(bufA, wt, p + OFF_B1, bufB, lane);
-    __syncthreads();
-    wave_layer<HID>(bufB, wt + IN * HID, p + OFF_B2, bufA, lane);
-    __syncthreads();
-    wave_layer<HID>(bufA, wt + IN * HID + HID * HID, p + OFF_B3, bufB, lane);
-    __syncthreads();
+    wave_layer<IN>(buf, wt, p + OFF_B1, lane);
+    wave_layer<HID>(buf, wt + IN * HID, p + OFF_B2, lane);
+    wave_layer<HID>(buf, wt + IN * HID + HID * HID, p + OFF_B3, lane);
     if (o < W) {
         const uint16_t* w4 = wb + IN * HID + 2 * HID * HID;
         float acc = 0.f;
-        for (int k = 0; k < HID; ++k) acc += bf16_to_float(bufB[lane * kLdsRow + k]) * bf16_to_float(w4[k]);
+        for (int k = 0; k < HID; ++k) acc += bf16_to_float(buf[lane * kLdsRow + k]) * bf16_to_float(w4[k]);
         scores[e * kScoreStride + o] = acc + p[OFF_B4];
     }
 }

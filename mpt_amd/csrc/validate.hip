@@ -478,7 +478,7 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
 // d_work[0..4]: the wave kernel's executed work (instrumented twin of the
 // timed kernel); d_work[5..7]: the reference algorithm's work on the same
 // edges (reference order, every disk tested until the first hit).
-int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int known_valid,
+int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order, int known_valid,
                      unsigned long long* d_work, uint8_t* d_verdict, uint32_t* d_checks) {
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->link_mode != MPTG_LINK_BISECT)
         return fail(ctx, MPTG_E_STATE, "work counting is implemented for the arm scene with bisection links");
@@ -488,7 +488,13 @@ int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, in
     int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
     MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 8 * sizeof(unsigned long long), ctx->stream));
     MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
-    rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks, irr, nirr, d_work);
+    const float* scores = nullptr;
+    if (order == MPTG_ORDER_SURROGATE) {
+        if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreSlots * n))) return rc;
+        if ((rc = surrogate_score_edges(ctx, d_from, d_to, n, nullptr, (float*)ctx->scratch[8].p))) return rc;
+        scores = (const float*)ctx->scratch[8].p;
+    }
+    rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks, irr, nirr, d_work, scores);
     if (rc) return rc == MPTG_E_STATE ? fail(ctx, rc, "no wave kernel for this arm") : rc;
     hipLaunchKernelGGL(link_bisect_work, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                        ctx->dim, scene_args(ctx), ctx->link_param, d_from, d_to, n, known_valid, d_work + 5);

@@ -1,0 +1,297 @@
+// C++ host layer of the MI355X batched PRRT path: the reference planner's
+// template API over the C ABI of include/mptg.h.
+//
+// The reference resolves planners at compile time:
+//     unc::robotics::mpt::Planner<Scenario, PRRT<options...>>
+// (src/mpt/planner.hpp:46-47, src/mpt/prrt.hpp:56-82).  This header keeps
+// that shape:
+//     mpt_amd::Planner<Scenario, mpt_amd::PRRT<mpt_amd::gpu_batch<8192>, ...>>
+// and the public methods of the reference class
+// (src/mpt/impl/prrt/prrt.hpp:148-316): setGoalBias/getGoalBias,
+// setRange/getRange, size, addStart, solve(doneFn), solveFor, solveUntil,
+// solved, solution(), solution(fn), printStats, visitGraph.
+//
+// Scenario contract (src/mpt/scenario.hpp:93-130): space(), bounds(),
+// goal(), and — instead of valid()/link() being called per state from N
+// worker threads — a description of valid()/link() as data:
+//     void gpu_scene(mptg_ctx*) const;   // uploads space, obstacles, link form
+// States are any type with operator[] and a compile-time dimension given by
+// Scenario::kDimensions.  Errors from the C ABI are rethrown as
+// std::runtime_error (as the reference throws at prrt.hpp:198).
+#pragma once
+
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "mptg.h"
+
+namespace mpt_amd {
+
+// ---- option tags (src/mpt/planner_tags.hpp:45-64) ---------------------------
+template <int B>
+struct gpu_batch {
+    static_assert(B >= 1, "batch must be positive");
+    static constexpr int value = B;
+};
+template <bool b>
+struct report_stats {
+    static constexpr bool value = b;
+};
+// Accepted for source compatibility; one host thread drives a context.
+template <int N>
+struct max_threads {
+    static constexpr int value = N;
+};
+// Visiting order of link()'s waypoints: reference, or surrogate-ranked.
+template <int O>
+struct check_order {
+    static constexpr int value = O;
+};
+
+namespace detail {
+inline void check(mptg_ctx* ctx, int rc) {
+    if (rc != MPTG_OK) throw std::runtime_error(std::string("mptg: ") + mptg_last_error(ctx));
+}
+}  // namespace detail
+
+// Strategy type produced by PRRT<options...>; option order is free, as in
+// the reference's option parser (src/mpt/impl/packs.hpp:105-125).
+template <int Batch, bool Stats, int Order>
+struct PRRTStrategy {
+    static constexpr int batch = Batch;
+    static constexpr bool stats = Stats;
+    static constexpr int order = Order;
+};
+
+template <typename... Opts>
+struct PRRTOptions;
+template <>
+struct PRRTOptions<> {
+    static constexpr int batch = 8192;
+    static constexpr bool stats = false;
+    static constexpr int order = MPTG_ORDER_REFERENCE;
+};
+template <int B, typename... Rest>
+struct PRRTOptions<gpu_batch<B>, Rest...> : PRRTOptions<Rest...> {
+    static constexpr int batch = B;
+};
+template <bool S, typename... Rest>
+struct PRRTOptions<report_stats<S>, Rest...> : PRRTOptions<Rest...> {
+    static constexpr bool stats = S;
+};
+template <int N, typename... Rest>
+struct PRRTOptions<max_threads<N>, Rest...> : PRRTOptions<Rest...> {};
+template <int O, typename... Rest>
+struct PRRTOptions<check_order<O>, Rest...> : PRRTOptions<Rest...> {
+    static constexpr int order = O;
+};
+
+template <typename... Opts>
+using PRRT = PRRTStrategy<PRRTOptions<Opts...>::batch, PRRTOptions<Opts...>::stats, PRRTOptions<Opts...>::order>;
+
+template <typename Scenario, typename Strategy>
+class GpuPRRT;
+
+template <typename Scenario, typename Algorithm>
+struct PlannerResolver;
+template <typename Scenario, int B, bool S, int O>
+struct PlannerResolver<Scenario, PRRTStrategy<B, S, O>> {
+    using type = GpuPRRT<Scenario, PRRTStrategy<B, S, O>>;
+};
+
+template <typename Scenario, typename Algorithm>
+using Planner = typename PlannerResolver<Scenario, Algorithm>::type;
+
+// ---- the planner ---------------------------------------------------------------
+template <typename Scenario, typename Strategy>
+class GpuPRRT {
+   public:
+    using State = typename Scenario::State;
+    static constexpr int kDim = Scenario::kDimensions;
+
+   private:
+    Scenario scenario_;
+    mptg_ctx* ctx_ = nullptr;
+    uint64_t seed_;
+    double maxDistance_ = std::numeric_limits<double>::infinity();
+    double goalBias_ = 0.01;
+    bool configured_ = false;
+
+    static void toDoubles(const State& s, double* out) {
+        for (int i = 0; i < kDim; ++i) out[i] = static_cast<double>(s[i]);
+    }
+    static State fromDoubles(const double* in) {
+        State s{};
+        for (int i = 0; i < kDim; ++i) s[i] = in[i];
+        return s;
+    }
+
+    void configure() {
+        double lo[kDim], hi[kDim], goal[kDim];
+        scenario_.bounds_to(lo, hi);
+        toDoubles(scenario_.goal_state(), goal);
+        mptg_prrt_params p{};
+        p.seed = seed_;
+        p.range = maxDistance_;
+        p.goal_bias = goalBias_;
+        p.goal_radius = scenario_.goal_radius();
+        p.batch = Strategy::batch;
+        p.order = Strategy::order;
+        detail::check(ctx_, mptg_prrt_init(ctx_, &p, lo, hi, goal));
+        configured_ = true;
+    }
+
+   public:
+    explicit GpuPRRT(const Scenario& scenario = Scenario(), uint64_t seed = 0, int device = 0)
+        : scenario_(scenario), seed_(seed) {
+        if (mptg_create(device, &ctx_) != MPTG_OK)
+            throw std::runtime_error(std::string("mptg: ") + mptg_last_error(nullptr));
+        try {
+            scenario_.gpu_scene(ctx_);
+            configure();
+        } catch (...) {
+            mptg_destroy(ctx_);
+            throw;
+        }
+    }
+    GpuPRRT(const GpuPRRT&) = delete;
+    GpuPRRT& operator=(const GpuPRRT&) = delete;
+    ~GpuPRRT() { mptg_destroy(ctx_); }
+
+    mptg_ctx* context() { return ctx_; }
+
+    void setGoalBias(double bias) {
+        if (!(0 <= bias && bias <= 1)) throw std::invalid_argument("goal bias must be in [0,1]");
+        goalBias_ = bias;
+        configure();
+    }
+    double getGoalBias() const { return goalBias_; }
+    void setRange(double range) {
+        if (!(range > 0)) throw std::invalid_argument("range must be > 0");
+        maxDistance_ = range;
+        configure();
+    }
+    double getRange() const { return maxDistance_; }
+
+    std::size_t size() const { return static_cast<std::size_t>(mptg_tree_size(ctx_)); }
+
+    void addStart(const State& q) {
+        double d[kDim];
+        toDoubles(q, d);
+        detail::check(ctx_, mptg_prrt_add_start(ctx_, d));
+    }
+
+    // One batched iteration per doneFn() check (prrt.hpp:194-201).
+    template <typename DoneFn>
+    std::enable_if_t<std::is_same_v<bool, std::invoke_result_t<DoneFn>>> solve(DoneFn doneFn) {
+        if (size() == 0) throw std::runtime_error("there are no valid initial states");
+        while (!doneFn()) detail::check(ctx_, mptg_prrt_run(ctx_, std::numeric_limits<int32_t>::max(), 1, nullptr));
+    }
+
+    template <typename Duration>
+    void solveFor(const Duration& d) {
+        auto end = std::chrono::steady_clock::now() + d;
+        solve([&] { return std::chrono::steady_clock::now() >= end; });
+    }
+
+    template <typename Clock, typename Duration>
+    void solveUntil(const std::chrono::time_point<Clock, Duration>& t) {
+        solve([&] { return Clock::now() >= t; });
+    }
+
+    // Grow until the tree holds at least n nodes (the demos' done condition).
+    void solveUntilSize(std::size_t n) {
+        if (size() == 0) throw std::runtime_error("there are no valid initial states");
+        detail::check(ctx_, mptg_prrt_run(ctx_, static_cast<int64_t>(n), std::numeric_limits<int64_t>::max(),
+                                          nullptr));
+    }
+
+    bool solved() const {
+        mptg_prrt_stats st{};
+        mptg_prrt_stats_get(ctx_, &st);
+        return st.goals > 0;
+    }
+
+    mptg_prrt_stats stats() const {
+        mptg_prrt_stats st{};
+        detail::check(ctx_, mptg_prrt_stats_get(ctx_, &st));
+        return st;
+    }
+
+   private:
+    void readTree(std::vector<double>& states, std::vector<int32_t>& parents) const {
+        const int64_t n = mptg_tree_size(ctx_);
+        states.resize(static_cast<size_t>(n) * kDim);
+        parents.resize(static_cast<size_t>(n));
+        if (n) detail::check(ctx_, mptg_tree_read(ctx_, 0, n, states.data(), parents.data()));
+    }
+
+    // Best goal by path cost (prrt.hpp:208-234); returns node id or -1.
+    int64_t bestGoal(const std::vector<double>& states, const std::vector<int32_t>& parents) const {
+        int64_t ng = 0;
+        detail::check(ctx_, mptg_prrt_goals(ctx_, nullptr, 0, &ng));
+        std::vector<int32_t> goals(static_cast<size_t>(ng));
+        if (ng) detail::check(ctx_, mptg_prrt_goals(ctx_, goals.data(), ng, &ng));
+        double best = std::numeric_limits<double>::infinity();
+        int64_t bestId = -1;
+        for (int32_t g : goals) {
+            double cost = 0;
+            for (int64_t n = g; parents[n] >= 0; n = parents[n])
+                cost += scenario_.distance(&states[n * kDim], &states[parents[n] * kDim]);
+            if (bestId < 0 || cost < best) {
+                best = cost;
+                bestId = g;
+            }
+        }
+        return bestId;
+    }
+
+   public:
+    std::vector<State> solution() const {
+        std::vector<double> states;
+        std::vector<int32_t> parents;
+        readTree(states, parents);
+        std::vector<State> path;
+        for (int64_t n = bestGoal(states, parents); n >= 0; n = parents[n]) path.push_back(fromDoubles(&states[n * kDim]));
+        return std::vector<State>(path.rbegin(), path.rend());
+    }
+
+    // fn(state) once per waypoint of the best path, start first (prrt.hpp:276-285)
+    template <typename Fn>
+    void solution(Fn fn) const {
+        for (const State& s : solution()) fn(s);
+    }
+
+    // visitor.vertex(state) / visitor.edge(parentState) per node (prrt.hpp:300-316)
+    template <typename Visitor>
+    void visitGraph(Visitor&& visitor) const {
+        std::vector<double> states;
+        std::vector<int32_t> parents;
+        readTree(states, parents);
+        for (size_t n = 0; n < parents.size(); ++n) {
+            visitor.vertex(fromDoubles(&states[n * kDim]));
+            if (parents[n] >= 0) visitor.edge(fromDoubles(&states[parents[n] * kDim]));
+        }
+    }
+
+    void printStats() const {
+        mptg_prrt_stats st = stats();
+        std::cout << "nodes in graph: " << size() << "\n"
+                  << "solutions: " << st.goals << "\n";
+        if constexpr (Strategy::stats) {
+            std::cout << "iterations: " << st.iterations << "\nsamples: " << st.samples
+                      << "\nlink calls: " << st.link_calls << " (" << st.link_ok << " ok, " << st.valid_calls
+                      << " valid() calls)\n";
+        }
+    }
+};
+
+}  // namespace mpt_amd

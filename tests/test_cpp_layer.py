@@ -1,0 +1,36 @@
+"""The C++ host layer (mpt_amd/include/mpt_amd/*.hpp over include/mptg.h):
+headers compile standalone, and the planar-arm example plans on the GPU."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+EXAMPLE = ROOT / "mpt_amd" / "examples" / "planar_arm.bin"
+
+
+def test_headers_compile_standalone(tmp_path):
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "mpt_amd/planner.hpp"\n#include "mpt_amd/scenarios.hpp"\n'
+                   "using P = mpt_amd::Planner<mpt_amd::LinkManipulator<20>,\n"
+                   "    mpt_amd::PRRT<mpt_amd::report_stats<true>, mpt_amd::gpu_batch<4096>>>;\n"
+                   "static_assert(P::kDim == 20);\n"
+                   "using Q = mpt_amd::PRRT<mpt_amd::gpu_batch<64>, mpt_amd::check_order<MPTG_ORDER_SURROGATE>>;\n"
+                   "static_assert(Q::batch == 64 && Q::order == MPTG_ORDER_SURROGATE && !Q::stats);\n"
+                   "int main() { return 0; }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", str(ROOT / "include"), "-I",
+                        str(ROOT / "mpt_amd" / "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_example_is_built():
+    assert EXAMPLE.exists(), "run __graft_entry__.build()"
+
+
+@pytest.mark.gpu
+def test_example_plans_on_gpu():
+    print("planar arm example", flush=True)
+    r = subprocess.run([str(EXAMPLE), "3"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "path_valid=1" in r.stdout and "solved=1" in r.stdout

@@ -208,3 +208,19 @@ def test_surrogate_order_parity_and_savings(trained):
     _say("colliding", int(coll.sum()), "checks ref", int(c_ref[coll].sum()), "sur", int(c_sur[coll].sum()))
     assert c_sur[coll].sum() < 0.8 * c_ref[coll].sum()
     assert np.array_equal(c_sur[~coll], c_ref[~coll])  # valid edges check everything in both orders
+
+
+def test_planner_with_surrogate_order_grows_the_oracle_tree(trained):
+    """Check order never changes a verdict, so the surrogate-ordered planner
+    grows exactly the reference-order tree (and the oracle's)."""
+    _say("surrogate-ordered planner")
+    ctx, sc, _, _, _ = trained
+    ctx.tree_clear()
+    ctx.prrt_init(sc, 13, 256, order=mpt_amd.ORDER_SURROGATE)
+    ctx.prrt_add_start(sc.start)
+    st = ctx.prrt_run(400)
+    gn, gp = ctx.tree_read()
+    on, op, og, ost = oracle.Oracle(sc).prrt(13, 256, 400)
+    assert np.array_equal(gn, on) and np.array_equal(gp, op)
+    assert st["link_calls"] == ost["link_calls"] and st["link_ok"] == ost["link_ok"]
+    assert st["valid_calls"] < ost["valid_calls"]  # surrogate order: fewer checks than the reference order
