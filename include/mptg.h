@@ -152,6 +152,14 @@ typedef struct {
 typedef struct {
     int64_t iterations, samples, zero_dist, invalid_states;
     int64_t link_calls, link_ok, valid_calls, goals, size;
+    /* PRRT* only (zero for PRRT) */
+    int64_t rewire_tests;    /* neighbours considered (Stats::rewireTests, prrt_star.hpp:564) */
+    int64_t rewires;         /* neighbours moved under a new node (Stats::rewireCount) */
+    int64_t parent_changes;  /* new nodes whose parent is not the near node */
+    int64_t cp_links;        /* choose-parent link() calls evaluated (speculatively, in parallel) */
+    int64_t rw_links;        /* rewire link() calls evaluated (speculatively, in parallel) */
+    int64_t deferred;        /* rewire links evaluated one at a time because the new node's
+                                cost dropped within its own iteration */
 } mptg_prrt_stats;
 
 int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo, const double* h_hi,
@@ -163,6 +171,32 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
 int mptg_prrt_stats_get(mptg_ctx* ctx, mptg_prrt_stats* out);
 /* ids of goal nodes in discovery order (foundGoal, prrt.hpp:139-146); returns count via *n */
 int mptg_prrt_goals(mptg_ctx* ctx, int32_t* h_ids, int64_t cap, int64_t* n);
+
+/* ---- batched PRRT*: replaces PRRTStar::solve / Worker::addSample
+ *      (src/mpt/impl/prrt_star/prrt_star.hpp:447-688).  Per iteration: extend
+ *      every sample against the iteration's tree (as PRRT, with dNear
+ *      recomputed after steering, :535); choose each survivor's parent among
+ *      its k nearest in ascending cost + distance order (:552-605); append in
+ *      sample order; then rewire survivors in sample order, neighbours in
+ *      distance order, against the costs current at that point, pushing each
+ *      cost decrease down the moved subtree (:626-688).  batch = 1 is the
+ *      reference's non-concurrent single-worker loop.  After
+ *      mptg_prrt_star_init, mptg_prrt_add_start / _run / _stats_get / _goals
+ *      drive the PRRT* loop; the tree must be grown from mptg_prrt_add_start
+ *      (nodes from mptg_tree_insert carry no costs: MPTG_E_STATE). */
+typedef struct {
+    int32_t k;             /* neighbours per survivor, 1..32; 0 = the reference's
+                              ceil(kRRG * ln(n + 1)), kRRG = factor * e * (1 + 1/dim)
+                              (rrg_rewire_neighbors.hpp:46-67), n = tree size at the
+                              iteration start; MPTG_E_RANGE once that exceeds 32 */
+    int32_t _pad;
+    double rewire_factor;  /* setRewireFactor (default 1.1, prrt_star.hpp:176,224) */
+} mptg_prrt_star_params;
+
+int mptg_prrt_star_init(mptg_ctx* ctx, const mptg_prrt_params* p, const mptg_prrt_star_params* sp,
+                        const double* h_lo, const double* h_hi, const double* h_goal);
+/* path cost (Edge::cost) of nodes [first, first + n) of a PRRT* tree */
+int mptg_prrt_costs(mptg_ctx* ctx, int64_t first, int64_t n, double* h_cost);
 
 /* ---- collision surrogate (no counterpart in the reference): a
  *      64-128-128-128-1 ReLU MLP (BF16 operands, FP32 accumulation) trained

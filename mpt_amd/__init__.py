@@ -19,7 +19,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _native
-from ._native import PrrtParams, PrrtStats
+from ._native import PrrtParams, PrrtStarParams, PrrtStats
 from .scenes import (LINK_BISECT, LINK_DMV, METRIC_L1, METRIC_L2, METRIC_SE3, ORDER_REFERENCE,
                      ORDER_SURROGATE, SCENE_ARM, SCENE_BOXES, SCENE_GRID, Scene)
 
@@ -315,6 +315,23 @@ class Context:
         lo, hi, g = _f64(sc.lo), _f64(sc.hi), _f64(sc.goal)
         self._chk(_lib.mptg_prrt_init(self._h, C.byref(p), _p(lo, C.c_double), _p(hi, C.c_double), _p(g, C.c_double)))
 
+    def prrt_star_init(self, sc: Scene, seed: int, batch: int, k: int = 20, rewire_factor: float = 1.1,
+                       order: int = ORDER_REFERENCE, range_: Optional[float] = None,
+                       goal_bias: Optional[float] = None):
+        p = PrrtParams(seed=seed, range=sc.range if range_ is None else range_,
+                       goal_bias=sc.goal_bias if goal_bias is None else goal_bias, goal_radius=sc.goal_radius,
+                       batch=batch, order=order)
+        sp = PrrtStarParams(k=k, rewire_factor=rewire_factor)
+        lo, hi, g = _f64(sc.lo), _f64(sc.hi), _f64(sc.goal)
+        self._chk(_lib.mptg_prrt_star_init(self._h, C.byref(p), C.byref(sp), _p(lo, C.c_double), _p(hi, C.c_double),
+                                           _p(g, C.c_double)))
+
+    def prrt_costs(self) -> np.ndarray:
+        n = self.tree_size()
+        out = np.zeros(n)
+        self._chk(_lib.mptg_prrt_costs(self._h, 0, n, _p(out, C.c_double)))
+        return out
+
     def prrt_add_start(self, q):
         q = _f64(q)
         self._chk(_lib.mptg_prrt_add_start(self._h, _p(q, C.c_double)))
@@ -386,7 +403,7 @@ class PRRT:
         if self.size() == 0:
             raise RuntimeError("there are no valid initial states")
         while not done():
-            self.ctx.prrt_run(1 << 31 - 1, iters_per_check)
+            self.ctx.prrt_run((1 << 31) - 1, iters_per_check)
 
     def solveFor(self, seconds: float):
         t_end = time.perf_counter() + seconds
@@ -435,3 +452,62 @@ class PRRT:
         print(f"solutions: {s['goals']}")
         for k, v in s.items():
             print(f"{k}: {v}")
+
+
+class PRRTStar(PRRT):
+    """Host mirror of ``mpt::Planner<Scenario, PRRTStar<...>>`` on the batched
+    GPU path (reference surface: ``src/mpt/impl/prrt_star/prrt_star.hpp:225-430``).
+    ``k`` = 0 selects the reference's k-nearest formula."""
+
+    def __init__(self, scenario: Scene, seed: int = 0, batch: int = 8192, k: int = 20, rewire_factor: float = 1.1,
+                 order: int = ORDER_REFERENCE, device: int = 0, ctx: Optional[Context] = None):
+        self._k, self._factor = k, rewire_factor
+        super().__init__(scenario, seed, batch, order, device, ctx)
+
+    def _reinit(self):
+        self.ctx.prrt_star_init(self.scenario, self._seed, self._batch, self._k, self._factor, self._order,
+                                self._range, self._bias)
+
+    def _init(self):
+        self.ctx.tree_clear()
+        self._reinit()
+
+    def setRange(self, r: float):
+        if not r > 0:
+            raise ValueError("range must be > 0")
+        self._range = r
+        self._reinit()
+
+    def setGoalBias(self, b: float):
+        if not 0 <= b <= 1:
+            raise ValueError("goal bias must be in [0,1]")
+        self._bias = b
+        self._reinit()
+
+    def setRewireFactor(self, f: float):
+        if not f > 0:
+            raise ValueError("rewire factor must be > 0")
+        self._factor = f
+        self._reinit()
+
+    def costs(self) -> np.ndarray:
+        return self.ctx.prrt_costs()
+
+    def _best_goal(self):
+        goals = self.ctx.prrt_goals()
+        if goals.size == 0:
+            return None, math.nan
+        c = self.costs()[goals]
+        i = int(np.argmin(c))
+        return int(goals[i]), float(c[i])
+
+    def solutionCost(self) -> float:
+        return self._best_goal()[1]
+
+    def solution(self):
+        """Path to the lowest-cost goal node (prrt_star.hpp:366-380)."""
+        g, _ = self._best_goal()
+        if g is None:
+            return []
+        states, parents = self.ctx.tree_read()
+        return self._path_to(g, states, parents)

@@ -28,10 +28,15 @@ class PrrtParams(C.Structure):
     ]
 
 
+class PrrtStarParams(C.Structure):
+    _fields_ = [("k", C.c_int32), ("_pad", C.c_int32), ("rewire_factor", C.c_double)]
+
+
 class PrrtStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "iterations", "samples", "zero_dist", "invalid_states", "link_calls", "link_ok",
-        "valid_calls", "goals", "size")]
+        "valid_calls", "goals", "size", "rewire_tests", "rewires", "parent_changes", "cp_links", "rw_links",
+        "deferred")]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -72,6 +77,9 @@ _SIGNATURES = [
     ("mptg_prrt_run", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(PrrtStats)]),
     ("mptg_prrt_stats_get", C.c_int, [C.c_void_p, C.POINTER(PrrtStats)]),
     ("mptg_prrt_goals", C.c_int, [C.c_void_p, c_int32_p, C.c_int64, c_int64_p]),
+    ("mptg_prrt_star_init", C.c_int, [C.c_void_p, C.POINTER(PrrtParams), C.POINTER(PrrtStarParams), c_double_p,
+                                      c_double_p, c_double_p]),
+    ("mptg_prrt_costs", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, c_double_p]),
     ("mptg_surrogate_nparams", C.c_int64, []),
     ("mptg_surrogate_init", C.c_int, [C.c_void_p, C.c_uint64]),
     ("mptg_surrogate_get", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),

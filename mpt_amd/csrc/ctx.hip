@@ -45,34 +45,54 @@ int check_device_ready(mptg_ctx* ctx) {
     return MPTG_OK;
 }
 
+static void free_tree(mptg_ctx* ctx) {
+    (void)hipFree(ctx->x64);
+    (void)hipFree(ctx->x32);
+    (void)hipFree(ctx->parent);
+    (void)hipFree(ctx->cost);
+    (void)hipFree(ctx->child);
+    ctx->x64 = nullptr;
+    ctx->x32 = nullptr;
+    ctx->parent = nullptr;
+    ctx->cost = nullptr;
+    ctx->child = ctx->next = ctx->prev = nullptr;
+    ctx->cap = 0;
+}
+
 static int grow_tree(mptg_ctx* ctx, int64_t cap) {
     if (cap <= ctx->cap) return MPTG_OK;
     int d = ctx->dim, dp = pitch32(d);
     double* x64 = nullptr;
     float* x32 = nullptr;
     int32_t* par = nullptr;
+    double* cost = nullptr;
+    int32_t* links = nullptr;
     MPTG_HIP(ctx, hipMalloc(&x64, sizeof(double) * d * cap));
     MPTG_HIP(ctx, hipMalloc(&x32, sizeof(float) * dp * cap));
     MPTG_HIP(ctx, hipMalloc(&par, sizeof(int32_t) * cap));
+    MPTG_HIP(ctx, hipMalloc(&cost, sizeof(double) * cap));
+    MPTG_HIP(ctx, hipMalloc(&links, sizeof(int32_t) * 3 * cap));
     // zero the FP32 pad columns once so every row is fully defined
     MPTG_HIP(ctx, hipMemsetAsync(x32, 0, sizeof(float) * dp * cap, ctx->stream));
     if (ctx->size > 0) {
-        MPTG_HIP(ctx, hipMemcpyAsync(x64, ctx->x64, sizeof(double) * d * ctx->size,
-                                     hipMemcpyDeviceToDevice, ctx->stream));
-        MPTG_HIP(ctx, hipMemcpyAsync(x32, ctx->x32, sizeof(float) * dp * ctx->size,
-                                     hipMemcpyDeviceToDevice, ctx->stream));
-        MPTG_HIP(ctx, hipMemcpyAsync(par, ctx->parent, sizeof(int32_t) * ctx->size,
-                                     hipMemcpyDeviceToDevice, ctx->stream));
+        const size_t n = (size_t)ctx->size;
+        MPTG_HIP(ctx, hipMemcpyAsync(x64, ctx->x64, sizeof(double) * d * n, hipMemcpyDeviceToDevice, ctx->stream));
+        MPTG_HIP(ctx, hipMemcpyAsync(x32, ctx->x32, sizeof(float) * dp * n, hipMemcpyDeviceToDevice, ctx->stream));
+        MPTG_HIP(ctx, hipMemcpyAsync(par, ctx->parent, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        MPTG_HIP(ctx, hipMemcpyAsync(cost, ctx->cost, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        for (int l = 0; l < 3; ++l)
+            MPTG_HIP(ctx, hipMemcpyAsync(links + l * cap, ctx->child + l * ctx->cap, sizeof(int32_t) * n,
+                                         hipMemcpyDeviceToDevice, ctx->stream));
     }
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->x64) {
-        MPTG_HIP(ctx, hipFree(ctx->x64));
-        MPTG_HIP(ctx, hipFree(ctx->x32));
-        MPTG_HIP(ctx, hipFree(ctx->parent));
-    }
+    free_tree(ctx);
     ctx->x64 = x64;
     ctx->x32 = x32;
     ctx->parent = par;
+    ctx->cost = cost;
+    ctx->child = links;
+    ctx->next = links + cap;
+    ctx->prev = links + 2 * cap;
     ctx->cap = cap;
     return MPTG_OK;
 }
@@ -119,9 +139,7 @@ void mptg_destroy(mptg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->x64);
-    (void)hipFree(ctx->x32);
-    (void)hipFree(ctx->parent);
+    free_tree(ctx);
     (void)hipFree(ctx->scene.boxes);
     (void)hipFree(ctx->scene.grid);
     (void)hipFree(ctx->scene.lengths);
@@ -225,13 +243,7 @@ int mptg_space(mptg_ctx* ctx, int metric, int dim, double w0, double w1) {
         return fail(ctx, MPTG_E_STATE, "cannot change the space of a non-empty tree");
     if (dim != ctx->dim && ctx->x64) {
         MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        MPTG_HIP(ctx, hipFree(ctx->x64));
-        MPTG_HIP(ctx, hipFree(ctx->x32));
-        MPTG_HIP(ctx, hipFree(ctx->parent));
-        ctx->x64 = nullptr;
-        ctx->x32 = nullptr;
-        ctx->parent = nullptr;
-        ctx->cap = 0;
+        free_tree(ctx);
     }
     ctx->metric = metric;
     ctx->dim = dim;
@@ -251,6 +263,7 @@ int mptg_tree_clear(mptg_ctx* ctx) {
     if (rc) return rc;
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->size = 0;
+    ctx->star_ok = true;
     ctx->coord_bound = 0;
     ctx->prrt.iteration = 0;
     ctx->prrt.goals = 0;
@@ -294,6 +307,7 @@ int mptg_tree_insert(mptg_ctx* ctx, const double* h_states, const int32_t* h_par
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->size += n;
     ctx->coord_bound = bound;
+    ctx->star_ok = false;  // no path costs or child lists for these nodes
     return MPTG_OK;
 }
 

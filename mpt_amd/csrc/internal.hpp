@@ -52,6 +52,12 @@ struct PrrtState {
     int64_t goals = 0;           // goal nodes found so far
     mptg_prrt_stats stats{};
     std::vector<int32_t> goal_ids;
+    // PRRT*
+    bool star = false;
+    int32_t star_k = 20;          // 0: the reference's k(n) formula
+    double rewire_factor = 1.1;
+    DevBuf dnear, sidx, qs, nbh_idx, nbh_d, marked, pc, cpar, cp_from, cp_to, cp_ver, cp_chk, pair_cp, rw_from,
+        rw_to, rw_ver, rw_chk, pair_rw, front, pend, star_cnt;
 };
 
 }  // namespace mptg
@@ -72,6 +78,11 @@ struct mptg_ctx {
     double* x64 = nullptr;   // [cap][dim] AoS, FP64 (exact coordinates)
     float* x32 = nullptr;    // [cap][dp]  AoS, FP32 shadow for the prefilter
     int32_t* parent = nullptr;
+    // PRRT* node data: path cost and child lists (first child, next and
+    // previous sibling; -1 = none), valid for every node while star_ok
+    double* cost = nullptr;
+    int32_t *child = nullptr, *next = nullptr, *prev = nullptr;
+    bool star_ok = true;
     double coord_bound = 0;  // max |coordinate| over inserted nodes and bounds
 
     // scene

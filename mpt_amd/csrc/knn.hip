@@ -141,7 +141,40 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
     float thr = rthr;
     if (!active) thr = -1.0f;  // inactive lanes never take the slow path
 
-    for (int64_t p = p0; p < p1; ++p) {
+    // four tree nodes per step: their rows load together, one branch for all
+    constexpr int U = 4;
+    int64_t p = p0;
+    for (; p + U <= p1; p += U) {
+        float acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float* row = x32 + (p + u) * DP;
+            float a = 0.0f;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                float t = q32[i] - row[i];
+                if constexpr (METRIC == MPTG_METRIC_L1) a = a + __builtin_fabsf(t);
+                else a = __builtin_fmaf(t, t, a);
+            }
+            acc[u] = a;
+        }
+        uint32_t pass = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) pass |= (acc[u] <= thr ? 1u : 0u) << u;
+        // candidates in node order; a candidate that a lower threshold would
+        // now reject is still safe to evaluate (insert keeps the exact order)
+        while (pass) {
+            const int u = __builtin_ctz(pass);
+            pass &= pass - 1;
+            double dq = exact_dist<D, METRIC>(q + qc * D, x64 + (p + u) * D);
+            if (dq <= radius) {
+                top.insert(dq, (int32_t)(p + u));
+                float t = prefilter_threshold<D, METRIC>(top.d[K - 1], S);
+                thr = t < rthr ? t : rthr;
+            }
+        }
+    }
+    for (; p < p1; ++p) {
         const float* row = x32 + p * DP;
         float acc = 0.0f;
 #pragma unroll

@@ -16,86 +16,11 @@
 #include <algorithm>
 #include <cmath>
 
-#include "space_dev.hpp"
+#include "prrt_dev.hpp"
 
 namespace mptg {
 
 namespace {
-
-constexpr int kBlock = 256;
-
-enum : uint8_t { ST_ZERO = 0, ST_INVALID = 1, ST_LINK_FAIL = 2, ST_OK = 3 };
-
-struct Counters {
-    int64_t size;
-    int64_t goals;
-    int64_t zero, invalid, link_calls, link_ok, valid_calls;
-};
-
-struct PrrtDev {
-    double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
-};
-
-__global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int D, uint64_t seed, double goal_bias,
-                                                      int64_t it, int B, const Counters* __restrict__ cnt,
-                                                      double* __restrict__ q) {
-    int b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= B) return;
-    uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
-    bool bias = cnt->goals == 0 && goal_bias > 0;
-    bool useGoal = bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias;
-    for (int i = 0; i < D; ++i) {
-        double v = useGoal ? pd.goal[i]
-                           : uniform_in(uniform_at(seed, sidx, RNG_PURPOSE_SAMPLE, (uint32_t)i), pd.lo[i], pd.hi[i]);
-        q[(int64_t)b * D + i] = v;
-    }
-}
-
-template <int D>
-__global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc, double range, int B,
-                                                     const double* __restrict__ x64, const double* __restrict__ q,
-                                                     const int32_t* __restrict__ near,
-                                                     const double* __restrict__ ndist, double* __restrict__ from,
-                                                     double* __restrict__ to, uint8_t* __restrict__ status,
-                                                     uint8_t* __restrict__ linkmask) {
-    constexpr int DA = D > 0 ? D : kMaxDim;
-    const int dd = D > 0 ? D : sp.dim;
-    int b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= B) return;
-    double d = ndist[b];
-    int32_t nn = near[b];
-    linkmask[b] = 0;
-    if (d == 0 || nn < 0) {
-        status[b] = ST_ZERO;
-        return;
-    }
-    double a[DA], r[DA], s[DA];
-#pragma unroll
-    for (int i = 0; i < DA; ++i)
-        if (i < dd) {
-            a[i] = x64[(int64_t)nn * dd + i];
-            r[i] = q[(int64_t)b * dd + i];
-        }
-    if (d > range) {
-        space_interp_t<D>(sp, a, r, range / d, s);
-    } else {
-#pragma unroll
-        for (int i = 0; i < DA; ++i)
-            if (i < dd) s[i] = r[i];
-    }
-#pragma unroll
-    for (int i = 0; i < DA; ++i)
-        if (i < dd) {
-            from[(int64_t)b * dd + i] = a[i];
-            to[(int64_t)b * dd + i] = s[i];
-        }
-    if (!state_valid_t<D>(sc, s)) {
-        status[b] = ST_INVALID;
-        return;
-    }
-    status[b] = ST_LINK_FAIL;  // until link() says otherwise
-    linkmask[b] = 1;
-}
 
 // Single-block stable compaction: accepted samples get consecutive node ids
 // in sample order; also accumulates the counters.
@@ -221,8 +146,11 @@ int prrt_reserve(mptg_ctx* ctx, int B) {
 
 void prrt_free(mptg_ctx* ctx) {
     PrrtBufs& pb = bufs(ctx);
-    DevBuf* all[] = {&pb.q, &pb.near, &pb.ndist, &pb.from, &pb.to, &pb.status,
-                     &pb.mask, &pb.verdict, &pb.checks, &pb.counters, &pb.goal_buf};
+    DevBuf* all[] = {&pb.q,       &pb.near,    &pb.ndist,  &pb.from,    &pb.to,       &pb.status,  &pb.mask,
+                     &pb.verdict, &pb.checks,  &pb.counters, &pb.goal_buf, &pb.dnear, &pb.sidx,   &pb.qs,
+                     &pb.nbh_idx, &pb.nbh_d,   &pb.marked, &pb.pc,      &pb.cpar,     &pb.cp_from, &pb.cp_to,
+                     &pb.cp_ver,  &pb.cp_chk,  &pb.pair_cp, &pb.rw_from, &pb.rw_to,    &pb.rw_ver,  &pb.rw_chk,
+                     &pb.pair_rw, &pb.front,   &pb.pend,   &pb.star_cnt};
     for (DevBuf* b : all) {
         (void)hipFree(b->p);
         b->p = nullptr;
@@ -231,6 +159,7 @@ void prrt_free(mptg_ctx* ctx) {
 }
 
 int tree_reserve(mptg_ctx* ctx, int64_t cap);
+int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters);
 
 }  // namespace mptg
 
@@ -252,6 +181,7 @@ int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo,
         return fail(ctx, MPTG_E_ARG, "unknown check order");
     PrrtState& s = ctx->prrt;
     s.p = *p;
+    s.star = false;
     double bound = ctx->coord_bound;
     for (int i = 0; i < ctx->dim; ++i) {
         if (!(h_lo[i] <= h_hi[i]) || !std::isfinite(h_lo[i]) || !std::isfinite(h_hi[i]))
@@ -278,8 +208,19 @@ int mptg_prrt_add_start(mptg_ctx* ctx, const double* h_q) {
     if (rc) return rc;
     if (!ctx->prrt.init) return fail(ctx, MPTG_E_STATE, "mptg_prrt_init must be called first");
     int32_t par = -1;
+    const bool star_ok = ctx->star_ok;
     rc = mptg_tree_insert(ctx, h_q, &par, 1);
     if (rc) return rc;
+    // a start is a root: path cost 0 (Edge() at prrt_star.hpp:264-272), no children
+    const int64_t id = ctx->size - 1;
+    const double zero = 0.0;
+    const int32_t none[3] = {-1, -1, -1};
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->cost + id, &zero, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->child + id, &none[0], sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->next + id, &none[1], sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->prev + id, &none[2], sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->star_ok = star_ok;
     ctx->prrt.counters_valid = false;
     return MPTG_OK;
 }
@@ -291,6 +232,11 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
     if (!s.init) return fail(ctx, MPTG_E_STATE, "mptg_prrt_init must be called first");
     if (ctx->size == 0) return fail(ctx, MPTG_E_EMPTY, "there are no valid initial states");
     if (target_size > (int64_t)INT32_MAX) return fail(ctx, MPTG_E_RANGE, "target size must be < 2^31");
+    if (s.star) {
+        rc = prrt_star_run(ctx, target_size, max_iters);
+        if (out) *out = s.stats;
+        return rc;
+    }
     const int B = s.p.batch;
     const int D = ctx->dim;
     if ((rc = prrt_reserve(ctx, B))) return rc;
@@ -326,7 +272,8 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
             hipLaunchKernelGGL((prrt_steer<DD>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx),
                                scene_args(ctx), s.p.range, B, (const double*)ctx->x64, (const double*)pb.q.p,
                                (const int32_t*)pb.near.p, (const double*)pb.ndist.p, (double*)pb.from.p,
-                               (double*)pb.to.p, (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p);
+                               (double*)pb.to.p, (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p,
+                               (double*)nullptr);
         });
         MPTG_HIP(ctx, hipGetLastError());
         if ((rc = launch_link_masked(ctx, (const double*)pb.from.p, (const double*)pb.to.p, B, s.p.order, 1,

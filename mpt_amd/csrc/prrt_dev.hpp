@@ -1,0 +1,88 @@
+// Device kernels shared by the batched PRRT and PRRT* loops: sampling and
+// steering (src/mpt/impl/prrt/prrt.hpp:377-440,
+// src/mpt/impl/prrt_star/prrt_star.hpp:470-540).
+#pragma once
+#include "space_dev.hpp"
+
+namespace mptg {
+namespace {
+
+constexpr int kBlock = 256;
+
+enum : uint8_t { ST_ZERO = 0, ST_INVALID = 1, ST_LINK_FAIL = 2, ST_OK = 3 };
+
+struct Counters {
+    int64_t size;
+    int64_t goals;
+    int64_t zero, invalid, link_calls, link_ok, valid_calls;
+};
+
+struct PrrtDev {
+    double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
+};
+
+__global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int D, uint64_t seed, double goal_bias,
+                                                      int64_t it, int B, const Counters* __restrict__ cnt,
+                                                      double* __restrict__ q) {
+    int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B) return;
+    uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
+    bool bias = cnt->goals == 0 && goal_bias > 0;
+    bool useGoal = bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias;
+    for (int i = 0; i < D; ++i) {
+        double v = useGoal ? pd.goal[i]
+                           : uniform_in(uniform_at(seed, sidx, RNG_PURPOSE_SAMPLE, (uint32_t)i), pd.lo[i], pd.hi[i]);
+        q[(int64_t)b * D + i] = v;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc, double range, int B,
+                                                     const double* __restrict__ x64, const double* __restrict__ q,
+                                                     const int32_t* __restrict__ near,
+                                                     const double* ndist, double* __restrict__ from,
+                                                     double* __restrict__ to, uint8_t* __restrict__ status,
+                                                     uint8_t* __restrict__ linkmask, double* dnear) {
+    constexpr int DA = D > 0 ? D : kMaxDim;
+    const int dd = D > 0 ? D : sp.dim;
+    int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B) return;
+    double d = ndist[b];
+    int32_t nn = near[b];
+    linkmask[b] = 0;
+    if (d == 0 || nn < 0) {
+        status[b] = ST_ZERO;
+        return;
+    }
+    double a[DA], r[DA], s[DA];
+#pragma unroll
+    for (int i = 0; i < DA; ++i)
+        if (i < dd) {
+            a[i] = x64[(int64_t)nn * dd + i];
+            r[i] = q[(int64_t)b * dd + i];
+        }
+    if (d > range) {
+        space_interp_t<D>(sp, a, r, range / d, s);
+        // PRRT* carries d(near, new) on into the cost (prrt_star.hpp:535)
+        if (dnear) dnear[b] = space_dist_t<D>(sp, a, s);
+    } else {
+#pragma unroll
+        for (int i = 0; i < DA; ++i)
+            if (i < dd) s[i] = r[i];
+    }
+#pragma unroll
+    for (int i = 0; i < DA; ++i)
+        if (i < dd) {
+            from[(int64_t)b * dd + i] = a[i];
+            to[(int64_t)b * dd + i] = s[i];
+        }
+    if (!state_valid_t<D>(sc, s)) {
+        status[b] = ST_INVALID;
+        return;
+    }
+    status[b] = ST_LINK_FAIL;  // until link() says otherwise
+    linkmask[b] = 1;
+}
+
+}  // namespace
+}  // namespace mptg

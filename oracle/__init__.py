@@ -45,6 +45,11 @@ class OrcPrrtStats(C.Structure):
                                          "invalid_states", "link_calls", "link_ok", "valid_calls", "goals")]
 
 
+class OrcStarStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("rewire_tests", "cp_links", "rw_links", "rewires", "parent_changes",
+                                         "spec_cp", "spec_rw", "deferred")]
+
+
 class OrcPoolResult(C.Structure):
     _fields_ = [("seconds", C.c_double), ("size", C.c_int64), ("samples", C.c_int64),
                 ("link_calls", C.c_int64), ("valid_calls", C.c_int64), ("goals", C.c_int64)]
@@ -68,6 +73,10 @@ _SIG = [
                                                C.POINTER(C.c_float), C.c_int64]),
     ("orc_prrt_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_int64, _dp,
                                  _i32p, C.c_int64, _i32p, C.c_int64, C.POINTER(OrcPrrtStats)]),
+    ("orc_star_k", C.c_int32, [C.c_int32, C.c_double, C.c_int32, C.c_int64]),
+    ("orc_prrt_star_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), C.c_int32, C.c_double, _dp,
+                                      C.c_int64, C.c_int64, _dp, _i32p, _dp, C.c_int64, _i32p, C.c_int64,
+                                      C.POINTER(OrcPrrtStats), C.POINTER(OrcStarStats)]),
     ("orc_pool_run", C.c_int32, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_double,
                                  C.c_int32, C.POINTER(OrcPoolResult)]),
 ]
@@ -206,6 +215,28 @@ class Oracle:
             raise RuntimeError("oracle capacity exceeded")
         stats = {f: int(getattr(st, f)) for f, _ in st._fields_}
         return nodes[:n].copy(), parents[:n].copy(), goals[: min(stats["goals"], 4096)].copy(), stats
+
+    def prrt_star(self, seed, batch, target, k=20, rewire_factor=1.1, max_iters=1 << 40, range_=None,
+                  goal_bias=None):
+        """Batch-synchronous PRRT*: (nodes, parents, costs, goals, stats, star_stats)."""
+        p = self._params(seed, batch, range_, goal_bias)
+        cap = target + batch + 8
+        nodes = np.zeros((cap, self.dim))
+        parents = np.zeros(cap, dtype=np.int32)
+        costs = np.zeros(cap)
+        goals = np.zeros(4096, dtype=np.int32)
+        st = OrcPrrtStats()
+        ss = OrcStarStats()
+        start = _f64(self.scene.start)
+        n = self.L.orc_prrt_star_run(C.byref(self.s), C.byref(p), k, rewire_factor, _p(start, C.c_double), target,
+                                     max_iters, _p(nodes, C.c_double), _p(parents, C.c_int32), _p(costs, C.c_double),
+                                     cap, _p(goals, C.c_int32), 4096, C.byref(st), C.byref(ss))
+        if n < 0:
+            raise RuntimeError({-1: "oracle capacity exceeded", -2: "bad distance", -3: "k out of range"}[n])
+        stats = {f: int(getattr(st, f)) for f, _ in st._fields_}
+        sstats = {f: int(getattr(ss, f)) for f, _ in ss._fields_}
+        return (nodes[:n].copy(), parents[:n].copy(), costs[:n].copy(), goals[: min(stats["goals"], 4096)].copy(),
+                stats, sstats)
 
     def pool(self, seed, target, seconds, threads, range_=None, goal_bias=None):
         p = self._params(seed, 1, range_, goal_bias)

@@ -115,6 +115,29 @@ int64_t orc_prrt_run(const orc_scene* sc, const orc_prrt_params* p, const double
                      int64_t target, int64_t max_iters, double* nodes, int32_t* parents, int64_t cap,
                      int32_t* goal_nodes, int64_t goal_cap, orc_prrt_stats* stats);
 
+/* --- batch-synchronous PRRT* ----------------------------------------------- */
+typedef struct {
+    int64_t rewire_tests;    /* neighbours considered for rewiring */
+    int64_t cp_links;        /* link() calls of the choose-parent walk */
+    int64_t rw_links;        /* link() calls of the rewire pass */
+    int64_t rewires;         /* neighbours moved under a new node */
+    int64_t parent_changes;  /* survivors whose parent is not the near node */
+    int64_t spec_cp;         /* walk entries a batched walk queues: those before the first
+                                cost-bound or near-node stop */
+    int64_t spec_rw;         /* unmarked pairs with pc + d < cost at the rewire phase's start */
+    int64_t deferred;        /* pairs rewire-tested only because cost(new) fell in the phase */
+} orc_star_stats;
+
+/* Neighbour count: k_fixed if > 0, else ceil(factor*e*(1+1/dim)*ln(n+1)). */
+int32_t orc_star_k(int32_t k_fixed, double factor, int32_t dim, int64_t n);
+/* Grows a PRRT* tree; costs[cap] receive the path costs.  Returns the final
+ * size, -1 if cap is too small, -2 on the reference's "bad distance" abort,
+ * -3 if k exceeds 64. */
+int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params* p, int32_t k_fixed, double rewire_factor,
+                          const double* start, int64_t target, int64_t max_iters, double* nodes,
+                          int32_t* parents, double* costs, int64_t cap, int32_t* goal_nodes, int64_t goal_cap,
+                          orc_prrt_stats* stats, orc_star_stats* star_stats);
+
 /* Reference-style shared-memory worker pool (std::thread, one RNG per
  * worker, mt19937_64 + uniform_real_distribution, shared tree behind a
  * KD-tree with a reader/writer lock).  Nondeterministic by design, like the
