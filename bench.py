@@ -11,8 +11,9 @@ gain, extension attempts per second, the roofline of the dominant kernel (the
 arm link() kernel, FP64 VALU), the MFMA rate of the score kernel, and the CPU
 baseline (the reference-style std::thread worker pool, all host cores).  Extra lines for the
 nearest-neighbour kernel (1e6 x R^7, 65536 queries), the occupancy-grid edge
-kernel (4096^2 grid, 2^20 edges) and the boxes planner are reported under
-"extra".
+kernel (4096^2 grid, 2^20 edges), the boxes planner and the SE(3) mesh
+PRRT* (500 vs 50k triangles, k = 20, 4096 samples per iteration) are
+reported under "extra".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -236,6 +237,33 @@ def bench_boxes(ctx, args):
             "iterations": st["iterations"], "size": st["size"], "solved": st["goals"] > 0}
 
 
+def bench_se3(ctx, args):
+    """SE(3) rigid body (500-triangle robot, 50k-triangle environment) PRRT*
+    with k = 20 and 4096 samples per iteration: extensions and edge checks
+    (every link() call: extend, choose-parent and rewire) per second."""
+    from mpt_amd import scenes
+    sc = scenes.se3_scene()
+    ctx.scene(sc)
+    ctx.tree_clear()
+    ctx.prrt_star_init(sc, seed=args.seed, batch=4096, k=20)
+    ctx.prrt_add_start(sc.start)
+    ctx.prrt_run(1 << 30, args.warmup)
+    s0 = ctx.prrt_stats()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.prrt_run(1 << 30, args.steps)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    s1 = ctx.prrt_stats()
+    d = {k: s1[k] - s0[k] for k in s1}
+    links = d["link_calls"] + d["cp_links"] + d["rw_links"]
+    return {"workload": "se3_robot500_env50000_prrtstar_k20_b4096", "ms_per_step": dt / args.steps * 1e3,
+            "extensions_per_s": d["samples"] / dt, "accepted_per_s": d["link_ok"] / dt,
+            "edge_checks_per_s": links / dt, "extend_links": d["link_calls"], "choose_parent_links": d["cp_links"],
+            "rewire_links": d["rw_links"], "rewires": d["rewires"], "deferred": d["deferred"],
+            "tree_size": s1["size"], "goals": s1["goals"]}
+
+
 def cpu_baseline(sc, seconds):
     import oracle
     o = oracle.Oracle(sc)
@@ -275,6 +303,8 @@ def main():
         extra["grid4096_edges_2p20"] = bench_grid(ctx, args, timer)
         log("boxes planner ...")
         extra["boxes16_prrt_1e4"] = bench_boxes(ctx, args)
+        log("se3 prrt* ...")
+        extra["se3_prrtstar"] = bench_se3(ctx, args)
     cpu = None
     if not args.quick:
         log("cpu baseline ...")

@@ -48,7 +48,8 @@ enum mptg_scene_kind {
     MPTG_SCENE_NONE = 0,
     MPTG_SCENE_BOXES = 1, /* closed 2-D boxes, Rect::pointIsValid   demo/shape_hierarchy.hpp:177-182 */
     MPTG_SCENE_GRID = 2,  /* occupancy grid, PNG2dScenario::valid    demo/png_2d_scenario.hpp:104-110 */
-    MPTG_SCENE_ARM = 3    /* planar arm vs disks                     demo/link_manipulator_scenario.hpp:99-116 */
+    MPTG_SCENE_ARM = 3,   /* planar arm vs disks                     demo/link_manipulator_scenario.hpp:99-116 */
+    MPTG_SCENE_MESH = 4   /* rigid body vs triangle mesh (SE3 space) demo/se3_rigid_body_scenario.hpp:288-312 */
 };
 
 enum mptg_link_mode {
@@ -115,6 +116,14 @@ int mptg_scene_grid(mptg_ctx* ctx, const uint8_t* h_cells /* h x w, nonzero = ob
                     int width, int height);
 int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double robot_radius,
                    const double* h_disks /* n x [cx,cy,r] */, int ndisks);
+/* SE(3) rigid body among triangles (replaces SE3RigidBodyScenario::valid,
+ * demo/se3_rigid_body_scenario.hpp:288-312, whose collide() is FCL's).  The
+ * space must be MPTG_METRIC_SE3 with states [qx,qy,qz,qw,tx,ty,tz]; pose =
+ * Translation * Quaternion (:258-261).  Triangles are 9 floats (3 vertices):
+ * robot in its own frame (<= 2^16), environment in the world frame (<= 2^20).
+ * valid(q) = no (robot triangle, env triangle) pair whose boxes overlap and
+ * which intersect (FP32 separating-axis test).  The link form must be DMV. */
+int mptg_scene_mesh(mptg_ctx* ctx, const float* h_robot, int nrobot, const float* h_env, int nenv);
 /* link(): DMV step or bisection tolerance */
 int mptg_set_link(mptg_ctx* ctx, int mode, double param);
 

@@ -21,7 +21,7 @@ import numpy as np
 from . import _native
 from ._native import PrrtParams, PrrtStarParams, PrrtStats
 from .scenes import (LINK_BISECT, LINK_DMV, METRIC_L1, METRIC_L2, METRIC_SE3, ORDER_REFERENCE,
-                     ORDER_SURROGATE, SCENE_ARM, SCENE_BOXES, SCENE_GRID, Scene)
+                     ORDER_SURROGATE, SCENE_ARM, SCENE_BOXES, SCENE_GRID, SCENE_MESH, Scene)
 
 __all__ = ["Context", "DeviceArray", "PRRT", "MptgError", "Scene", "METRIC_L1", "METRIC_L2", "METRIC_SE3", "LINK_DMV",
            "LINK_BISECT", "ORDER_REFERENCE", "ORDER_SURROGATE"]
@@ -222,6 +222,10 @@ class Context:
             dk = _f64(sc.disks, 3)
             self._chk(_lib.mptg_scene_arm(self._h, _p(ln, C.c_double), ln.shape[0], sc.robot_radius,
                                           _p(dk, C.c_double), dk.shape[0]))
+        elif sc.kind == SCENE_MESH:
+            rt = np.ascontiguousarray(sc.robot_tris, dtype=np.float32).reshape(-1, 9)
+            et = np.ascontiguousarray(sc.env_tris, dtype=np.float32).reshape(-1, 9)
+            self._chk(_lib.mptg_scene_mesh(self._h, _p(rt, C.c_float), rt.shape[0], _p(et, C.c_float), et.shape[0]))
         else:
             raise ValueError(f"unknown scene kind {sc.kind}")
         self._chk(_lib.mptg_set_link(self._h, sc.link_mode, sc.link_param))

@@ -67,6 +67,7 @@ _SIGNATURES = [
     ("mptg_scene_boxes", C.c_int, [C.c_void_p, c_double_p, C.c_int]),
     ("mptg_scene_grid", C.c_int, [C.c_void_p, c_uint8_p, C.c_int, C.c_int]),
     ("mptg_scene_arm", C.c_int, [C.c_void_p, c_double_p, C.c_int, C.c_double, c_double_p, C.c_int]),
+    ("mptg_scene_mesh", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.c_int]),
     ("mptg_set_link", C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     ("mptg_valid", C.c_int, [C.c_void_p, c_double_p, C.c_int64, c_uint8_p]),
     ("mptg_link", C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_int64, C.c_int, C.c_int, c_uint8_p, c_uint32_p]),

@@ -146,6 +146,9 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.disks);
     (void)hipFree(ctx->scene.arm_disks.p);
     (void)hipFree(ctx->scene.arm_lim.p);
+    (void)hipFree(ctx->scene.mesh_nodes.p);
+    (void)hipFree(ctx->scene.mesh_tris.p);
+    (void)hipFree(ctx->scene.mesh_robot.p);
     for (auto& b : ctx->scratch) (void)hipFree(b.p);
     prrt_free(ctx);
     surrogate_free(ctx);

@@ -39,6 +39,9 @@ struct SceneDev {
     int ndisks = 0;
     DevBuf arm_disks, arm_lim;  // kernel layout of the disks, per-link bounds
     double total_length = 0;    // sum of link lengths (feature scale)
+    // mesh (SE(3) rigid body): env BVH nodes, env triangles, robot triangles
+    DevBuf mesh_nodes, mesh_tris, mesh_robot;
+    int mesh_nnodes = 0, mesh_nrobot = 0;
 };
 
 struct PrrtState {
@@ -135,6 +138,13 @@ int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t
 int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                        int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
 int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
+// mesh scene: one wave per state / edge
+int launch_mesh_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
+int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from, const double* d_to, int64_t n,
+                     int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
+// planners: states[b] with mask[b] set that collide get status[b] = invalid_code, mask[b] = 0
+int launch_mesh_filter(mptg_ctx* ctx, const double* d_states, int64_t n, uint8_t* d_mask, uint8_t* d_status,
+                       int invalid_code);
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
                     const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
                     unsigned long long* work = nullptr, const float* scores = nullptr);

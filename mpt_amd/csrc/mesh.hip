@@ -1,0 +1,535 @@
+// SE(3) rigid body among triangle meshes: the reference's valid(q) =
+// !fcl::collide(robot at pose(q), env) (demo/se3_rigid_body_scenario.hpp:288-312)
+// and its DiscreteMotionValidator link (:330-334), on the device.
+//
+// The predicate is mesh_hd.hpp's: posed robot triangle a and environment
+// triangle b collide iff their boxes overlap and the separating-axis test
+// finds no separating axis.  Culling below only skips pairs whose boxes are
+// apart, so the verdict is the predicate's exactly.
+//
+// Layout in HBM (built on the host at upload):
+//   env BVH   MeshNode[nnodes] (32 B: box + children or leaf range), depth-first
+//             order, binned-SAH splits, <= 4 triangles per leaf, depth <= 38
+//   env tris  16 floats (64 B) per triangle in leaf order: v0 v1 v2, box lo, hi
+//   robot     9 floats per triangle, Morton-sorted by centroid so that the 8
+//             consecutive triangles a lane owns are close together
+// Kernels (one 64-lane wave per block, one item per wave at a time):
+//   mesh_valid  one state per wave
+//   mesh_link   one edge per wave; waypoints in DMV order one at a time
+//               (reference checks_done counts), early exit on the first hit
+// Per state, lane l poses robot triangles [8l, 8l + 8) of each 512-triangle
+// chunk, takes their box, and walks the env BVH depth-first with a private
+// stack in LDS; a per-wave LDS flag stops every lane at the first hit.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mesh_hd.hpp"
+#include "space_dev.hpp"
+
+namespace mptg {
+
+namespace {
+
+constexpr int kLaneTris = 8;                    // robot triangles per lane
+constexpr int kChunkTris = 64 * kLaneTris;      // robot triangles per wave pass
+constexpr int kStackStride = 41;                // LDS stack slots per lane (odd: no bank conflicts)
+constexpr int kBuildDepth = 38;                 // BVH depth bound (stack needs depth + 1)
+constexpr int kMaxDmvWaypointsMesh = 1 << 24;   // same limit as the other DMV kernels
+constexpr int kTableMaxMesh = 255;
+
+// ---------------------------------------------------------------------------
+// host: BVH build
+// ---------------------------------------------------------------------------
+struct BuildRef {
+    float lo[3], hi[3], c[3];
+    int32_t id;
+};
+
+struct BvhBuilder {
+    std::vector<BuildRef> refs;
+    std::vector<MeshNode> nodes;
+
+    static void grow(float* lo, float* hi, const float* blo, const float* bhi) {
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::min(lo[i], blo[i]);
+            hi[i] = std::max(hi[i], bhi[i]);
+        }
+    }
+    static float area(const float* lo, const float* hi) {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0)) return 0.0f;
+        return dx * dy + dy * dz + dz * dx;
+    }
+
+    int build(int begin, int end, int depth) {
+        const int id = (int)nodes.size();
+        nodes.push_back(MeshNode{});
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = begin; i < end; ++i) {
+            grow(lo, hi, refs[i].lo, refs[i].hi);
+            grow(clo, chi, refs[i].c, refs[i].c);
+        }
+        auto set_box = [&](MeshNode& n) {
+            for (int i = 0; i < 3; ++i) {
+                n.lo[i] = lo[i];
+                n.hi[i] = hi[i];
+            }
+        };
+        const int count = end - begin;
+        if (count <= kLeafTris) {
+            MeshNode& n = nodes[id];
+            set_box(n);
+            n.a = begin;
+            n.b = -count;
+            return id;
+        }
+        int axis = 0;
+        for (int i = 1; i < 3; ++i)
+            if (chi[i] - clo[i] > chi[axis] - clo[axis]) axis = i;
+        int mid = -1;
+        const float ext = chi[axis] - clo[axis];
+        if (depth < 20 && ext > 0) {
+            constexpr int kBins = 16;
+            int bn[kBins] = {};
+            float blo[kBins][3], bhi[kBins][3];
+            for (int b = 0; b < kBins; ++b)
+                for (int i = 0; i < 3; ++i) {
+                    blo[b][i] = INFINITY;
+                    bhi[b][i] = -INFINITY;
+                }
+            auto bin_of = [&](const BuildRef& r) {
+                int b = (int)((r.c[axis] - clo[axis]) / ext * kBins);
+                return std::min(std::max(b, 0), kBins - 1);
+            };
+            for (int i = begin; i < end; ++i) {
+                int b = bin_of(refs[i]);
+                bn[b]++;
+                grow(blo[b], bhi[b], refs[i].lo, refs[i].hi);
+            }
+            float best = INFINITY;
+            int best_split = -1;
+            for (int s = 1; s < kBins; ++s) {
+                float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                int nl = 0, nr = 0;
+                for (int b = 0; b < s; ++b) {
+                    nl += bn[b];
+                    if (bn[b]) grow(llo, lhi, blo[b], bhi[b]);
+                }
+                for (int b = s; b < kBins; ++b) {
+                    nr += bn[b];
+                    if (bn[b]) grow(rlo, rhi, blo[b], bhi[b]);
+                }
+                if (nl == 0 || nr == 0) continue;
+                const float cost = area(llo, lhi) * nl + area(rlo, rhi) * nr;
+                if (cost < best) {
+                    best = cost;
+                    best_split = s;
+                }
+            }
+            if (best_split > 0) {
+                auto it = std::partition(refs.begin() + begin, refs.begin() + end,
+                                         [&](const BuildRef& r) { return bin_of(r) < best_split; });
+                mid = (int)(it - refs.begin());
+            }
+        }
+        if (mid <= begin || mid >= end) {  // median split: depth stays logarithmic
+            mid = begin + count / 2;
+            std::nth_element(refs.begin() + begin, refs.begin() + mid, refs.begin() + end,
+                             [&](const BuildRef& x, const BuildRef& y) {
+                                 return x.c[axis] < y.c[axis] || (x.c[axis] == y.c[axis] && x.id < y.id);
+                             });
+        }
+        const int left = build(begin, mid, depth + 1);
+        const int right = build(mid, end, depth + 1);
+        MeshNode& n = nodes[id];
+        set_box(n);
+        n.a = left;
+        n.b = right;
+        return id;
+    }
+};
+
+int bvh_depth(const std::vector<MeshNode>& nodes, int n, int d) {
+    if (nodes[n].b <= 0) return d;
+    return std::max(bvh_depth(nodes, nodes[n].a, d + 1), bvh_depth(nodes, nodes[n].b, d + 1));
+}
+
+uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+struct MeshArgs {
+    const MeshNode* nodes;
+    const float4* tris;   // 4 float4 per triangle
+    const float* robot;   // 9 floats per triangle
+    int nnodes, nrobot;
+};
+
+__device__ __forceinline__ void load_env_tri(const float4* __restrict__ t, int i, float* v, float* lo, float* hi) {
+    const float4 a = t[4 * i], b = t[4 * i + 1], c = t[4 * i + 2], d = t[4 * i + 3];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    v[8] = c.x;
+    lo[0] = c.y; lo[1] = c.z; lo[2] = c.w;
+    hi[0] = d.x; hi[1] = d.y; hi[2] = d.z;
+}
+
+// Does the robot at state q collide with the environment?  Whole wave; q is
+// the same on every lane.  stack: this wave's 64 * kStackStride LDS slots;
+// flag: this wave's LDS word.
+__device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack, volatile int* flag) {
+    const int lane = threadIdx.x & 63;
+    if (m.nnodes == 0 || m.nrobot == 0) return false;
+    const Pose32 pose = pose_from_state(q);
+    if (lane == 0) *flag = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    int32_t* stk = stack + lane * kStackStride;
+    for (int c0 = 0; c0 < m.nrobot; c0 += kChunkTris) {
+        const int first = c0 + lane * kLaneTris;
+        const int cnt = max(0, min(kLaneTris, m.nrobot - first));
+        // box of this lane's posed triangles; the triangles themselves are
+        // posed again only for environment triangles inside this box
+        float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll 1
+        for (int t = 0; t < cnt; ++t) {
+            const float* src = m.robot + (size_t)(first + t) * 9;
+            float tri[9];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) xform(pose, src + 3 * v, tri + 3 * v);
+            float tlo[3], thi[3];
+            tri_box(tri, tlo, thi);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                llo[i] = tlo[i] < llo[i] ? tlo[i] : llo[i];
+                lhi[i] = thi[i] > lhi[i] ? thi[i] : lhi[i];
+            }
+        }
+        int sp = 0;
+        if (cnt > 0) stk[sp++] = 0;
+        while (sp > 0 && *flag == 0) {
+            const MeshNode n = m.nodes[stk[--sp]];
+            if (!boxes_overlap(n.lo, n.hi, llo, lhi)) continue;
+            if (n.b > 0) {
+                stk[sp++] = n.b;
+                stk[sp++] = n.a;
+                continue;
+            }
+            bool hit = false;
+#pragma unroll 1
+            for (int i = n.a; i < n.a - n.b && !hit; ++i) {
+                float ev[9], elo[3], ehi[3];
+                load_env_tri(m.tris, i, ev, elo, ehi);
+                if (!boxes_overlap(elo, ehi, llo, lhi)) continue;
+#pragma unroll 1
+                for (int t = 0; t < cnt && !hit; ++t) {
+                    const float* src = m.robot + (size_t)(first + t) * 9;
+                    float tri[9], tlo[3], thi[3];
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) xform(pose, src + 3 * v, tri + 3 * v);
+                    tri_box(tri, tlo, thi);
+                    hit = pair_collides(tri, tlo, thi, ev, elo, ehi);
+                }
+            }
+            if (hit) *flag = 1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (*flag) return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(64) void mesh_valid(MeshArgs m, const double* __restrict__ q, int64_t n,
+                                                 uint8_t* __restrict__ ok) {
+    __shared__ int32_t stack[64 * kStackStride];
+    __shared__ int flag;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        double s[7];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) s[j] = q[i * 7 + j];
+        const bool hit = mesh_collides(m, s, stack, &flag);
+        if (threadIdx.x == 0) ok[i] = hit ? 0 : 1;
+    }
+}
+
+__global__ __launch_bounds__(64) void mesh_filter(MeshArgs m, const double* __restrict__ q, int64_t n,
+                                                  uint8_t* __restrict__ mask, uint8_t* __restrict__ status,
+                                                  int invalid_code) {
+    __shared__ int32_t stack[64 * kStackStride];
+    __shared__ int flag;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        if (!mask[i]) continue;
+        double s[7];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) s[j] = q[i * 7 + j];
+        const bool hit = mesh_collides(m, s, stack, &flag);
+        if (threadIdx.x == 0 && hit) {
+            mask[i] = 0;
+            status[i] = (uint8_t)invalid_code;
+        }
+    }
+}
+
+// DMV link (src/mpt/discrete_motion_validator.hpp:71-130), one edge per wave:
+// valid(to) unless the caller vouches for it, then the interior waypoints in
+// the reference's ring order (table for up to 255, ring replay beyond).
+__global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double inv,
+                                                const uint8_t* __restrict__ table, const double* __restrict__ from,
+                                                const double* __restrict__ to, int64_t n, int known_valid,
+                                                const uint8_t* __restrict__ mask, uint8_t* __restrict__ verdict,
+                                                uint32_t* __restrict__ checks) {
+    __shared__ int32_t stack[64 * kStackStride];
+    __shared__ int flag;
+    __shared__ uint64_t rmin[256], rmax[256];
+    __shared__ uint64_t emitted[64];
+    __shared__ int cnt_s;
+    // edge endpoints live in LDS (read back per waypoint) to keep them out of
+    // the registers the collision walk needs
+    __shared__ double ab_s[14];
+    volatile double* ab = ab_s;
+    const int lane = threadIdx.x;
+    for (int64_t e = blockIdx.x; e < n; e += gridDim.x) {
+        if (mask && !mask[e]) continue;
+        if (lane < 14) ab[lane] = lane < 7 ? from[e * 7 + lane] : to[e * 7 + lane - 7];
+        __syncthreads();
+        double a[7], b[7], q[7];
+        auto load_ab = [&]() {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                a[j] = ab[j];
+                b[j] = ab[7 + j];
+            }
+        };
+        load_ab();
+        if (!known_valid && mesh_collides(m, b, stack, &flag)) {
+            if (lane == 0) {
+                verdict[e] = 0;
+                checks[e] = 1;
+            }
+            __syncthreads();
+            continue;
+        }
+        load_ab();
+        double x = __builtin_ceil(space_dist(sp, a, b) * inv);
+        int64_t steps = 0;
+        if (x >= 2.0) steps = x < 0x1.0p62 ? (int64_t)x : ((int64_t)1 << 62);
+        const int64_t mw = steps > 0 ? steps - 1 : 0;
+        if (mw > kMaxDmvWaypointsMesh) {
+            if (lane == 0) {
+                verdict[e] = 0;
+                checks[e] = MPTG_CHECKS_OVER_LIMIT;
+            }
+            continue;
+        }
+        const double delta = steps > 0 ? 1.0 / (double)steps : 0.0;
+        bool failed = false;
+        int64_t done = 0;
+        if (mw <= kTableMaxMesh) {
+            const uint8_t* ord = table + (mw * (mw - 1)) / 2;
+            for (int64_t o = 0; o < mw; ++o) {
+                load_ab();
+                space_interp(sp, a, b, (double)ord[o] * delta, q);
+                ++done;
+                if (mesh_collides(m, q, stack, &flag)) {
+                    failed = true;
+                    break;
+                }
+            }
+        } else {
+            // ring replay: lane 0 emits up to 64 indices at a time
+            uint64_t qs = 0, qe = 1, seq_cur = 0, seq_end = 0;
+            bool seq = false;
+            if (lane == 0) {
+                rmin[0] = 1;
+                rmax[0] = (uint64_t)mw;
+            }
+            while (!failed) {
+                if (lane == 0) {
+                    int c = 0;
+                    while (c < 64 && (seq || qs != qe)) {
+                        if (seq) {
+                            emitted[c++] = seq_cur;
+                            if (seq_cur == seq_end) seq = false;
+                            else ++seq_cur;
+                            continue;
+                        }
+                        const uint64_t mn = rmin[qs % 256], mx = rmax[qs % 256];
+                        ++qs;
+                        if (mn == mx) {
+                            emitted[c++] = mn;
+                        } else if (qe + 2 < qs + 256) {
+                            const uint64_t mid = (mn + mx) / 2;
+                            emitted[c++] = mid;
+                            if (mn < mid) {
+                                rmin[qe % 256] = mn;
+                                rmax[qe % 256] = mid - 1;
+                                ++qe;
+                            }
+                            if (mid < mx) {
+                                rmin[qe % 256] = mid + 1;
+                                rmax[qe % 256] = mx;
+                                ++qe;
+                            }
+                        } else {
+                            seq = true;
+                            seq_cur = mn;
+                            seq_end = mx;
+                        }
+                    }
+                    cnt_s = c;
+                }
+                __syncthreads();
+                const int c = cnt_s;
+                if (c == 0) break;
+                for (int o = 0; o < c; ++o) {
+                    load_ab();
+                    space_interp(sp, a, b, (double)emitted[o] * delta, q);
+                    ++done;
+                    if (mesh_collides(m, q, stack, &flag)) {
+                        failed = true;
+                        break;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (lane == 0) {
+            verdict[e] = failed ? 0 : 1;
+            checks[e] = (uint32_t)(1 + done);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+MeshArgs mesh_args(const mptg_ctx* ctx) {
+    const SceneDev& s = ctx->scene;
+    return MeshArgs{(const MeshNode*)s.mesh_nodes.p, (const float4*)s.mesh_tris.p, (const float*)s.mesh_robot.p,
+                    s.mesh_nnodes, s.mesh_nrobot};
+}
+
+int launch_mesh_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
+    if (n <= 0) return MPTG_OK;
+    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 32);
+    hipLaunchKernelGGL(mesh_valid, dim3(g), dim3(64), 0, ctx->stream, mesh_args(ctx), d_q, n, d_ok);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+int launch_mesh_filter(mptg_ctx* ctx, const double* d_states, int64_t n, uint8_t* d_mask, uint8_t* d_status,
+                       int invalid_code) {
+    if (n <= 0) return MPTG_OK;
+    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 32);
+    hipLaunchKernelGGL(mesh_filter, dim3(g), dim3(64), 0, ctx->stream, mesh_args(ctx), d_states, n, d_mask, d_status,
+                       invalid_code);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from, const double* d_to, int64_t n,
+                     int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks) {
+    if (n <= 0) return MPTG_OK;
+    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 32);
+    hipLaunchKernelGGL(mesh_link, dim3(g), dim3(64), 0, ctx->stream, space_args(ctx), mesh_args(ctx),
+                       1.0 / ctx->link_param, d_table, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+}  // namespace mptg
+
+using namespace mptg;
+
+extern "C" int mptg_scene_mesh(mptg_ctx* ctx, const float* h_robot, int nrobot, const float* h_env, int nenv) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (ctx->metric != MPTG_METRIC_SE3) return fail(ctx, MPTG_E_STATE, "mesh scenes need the SE3 space");
+    if (nrobot < 0 || nenv < 0 || (nrobot > 0 && !h_robot) || (nenv > 0 && !h_env))
+        return fail(ctx, MPTG_E_ARG, "bad mesh arrays");
+    if (nenv > (1 << 20)) return fail(ctx, MPTG_E_RANGE, "at most 2^20 environment triangles");
+    if (nrobot > (1 << 16)) return fail(ctx, MPTG_E_RANGE, "at most 2^16 robot triangles");
+    for (int64_t i = 0; i < (int64_t)nrobot * 9; ++i)
+        if (!std::isfinite(h_robot[i])) return fail(ctx, MPTG_E_ARG, "non-finite robot vertex");
+    for (int64_t i = 0; i < (int64_t)nenv * 9; ++i)
+        if (!std::isfinite(h_env[i])) return fail(ctx, MPTG_E_ARG, "non-finite environment vertex");
+    // environment BVH
+    BvhBuilder bb;
+    bb.refs.resize((size_t)nenv);
+    for (int t = 0; t < nenv; ++t) {
+        BuildRef& r = bb.refs[(size_t)t];
+        tri_box(h_env + (size_t)t * 9, r.lo, r.hi);
+        for (int i = 0; i < 3; ++i) r.c[i] = 0.5f * (r.lo[i] + r.hi[i]);
+        r.id = t;
+    }
+    if (nenv > 0) bb.build(0, nenv, 0);
+    if (nenv > 0 && bvh_depth(bb.nodes, 0, 0) > kBuildDepth)
+        return fail(ctx, MPTG_E_STATE, "environment BVH deeper than the traversal stack");
+    std::vector<float> tris((size_t)nenv * kTriFloats, 0.0f);
+    for (int i = 0; i < nenv; ++i) {
+        const BuildRef& r = bb.refs[(size_t)i];
+        float* d = &tris[(size_t)i * kTriFloats];
+        std::memcpy(d, h_env + (size_t)r.id * 9, 9 * sizeof(float));
+        for (int k = 0; k < 3; ++k) {
+            d[9 + k] = r.lo[k];
+            d[12 + k] = r.hi[k];
+        }
+    }
+    // robot: Morton order of triangle centroids
+    std::vector<float> robot((size_t)nrobot * 9);
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        std::vector<float> cen((size_t)nrobot * 3);
+        for (int t = 0; t < nrobot; ++t)
+            for (int i = 0; i < 3; ++i) {
+                const float* v = h_robot + (size_t)t * 9;
+                float c = (v[i] + v[3 + i] + v[6 + i]) / 3.0f;
+                cen[(size_t)t * 3 + i] = c;
+                lo[i] = std::min(lo[i], c);
+                hi[i] = std::max(hi[i], c);
+            }
+        std::vector<std::pair<uint32_t, int>> key((size_t)nrobot);
+        for (int t = 0; t < nrobot; ++t) {
+            uint32_t code = 0;
+            for (int i = 0; i < 3; ++i) {
+                float ext = hi[i] - lo[i];
+                uint32_t qv = ext > 0 ? (uint32_t)std::min(1023.0f, (cen[(size_t)t * 3 + i] - lo[i]) / ext * 1023.0f) : 0;
+                code |= spread10(qv) << i;
+            }
+            key[(size_t)t] = {code, t};
+        }
+        std::sort(key.begin(), key.end());
+        for (int t = 0; t < nrobot; ++t)
+            std::memcpy(&robot[(size_t)t * 9], h_robot + (size_t)key[(size_t)t].second * 9, 9 * sizeof(float));
+    }
+    SceneDev& s = ctx->scene;
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = ensure(ctx, s.mesh_nodes, sizeof(MeshNode) * std::max<size_t>(1, bb.nodes.size())))) return rc;
+    if ((rc = ensure(ctx, s.mesh_tris, sizeof(float) * std::max<size_t>(16, tris.size())))) return rc;
+    if ((rc = ensure(ctx, s.mesh_robot, sizeof(float) * std::max<size_t>(9, robot.size())))) return rc;
+    if (!bb.nodes.empty())
+        MPTG_HIP(ctx, hipMemcpy(s.mesh_nodes.p, bb.nodes.data(), sizeof(MeshNode) * bb.nodes.size(),
+                                hipMemcpyHostToDevice));
+    if (!tris.empty())
+        MPTG_HIP(ctx, hipMemcpy(s.mesh_tris.p, tris.data(), sizeof(float) * tris.size(), hipMemcpyHostToDevice));
+    if (!robot.empty())
+        MPTG_HIP(ctx, hipMemcpy(s.mesh_robot.p, robot.data(), sizeof(float) * robot.size(), hipMemcpyHostToDevice));
+    s.mesh_nnodes = (int)bb.nodes.size();
+    s.mesh_nrobot = nrobot;
+    s.kind = MPTG_SCENE_MESH;
+    return MPTG_OK;
+}

@@ -262,7 +262,7 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
     for (int64_t k = 0; k < max_iters && ctx->size < target_size; ++k) {
         if ((rc = tree_reserve(ctx, ctx->size + B))) return rc;
         const int64_t it = s.iteration;
-        hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, D, s.p.seed, s.p.goal_bias, it,
+        hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, ctx->metric, D, s.p.seed, s.p.goal_bias, it,
                            B, (const Counters*)dcnt, (double*)pb.q.p);
         MPTG_HIP(ctx, hipGetLastError());
         if ((rc = launch_knn(ctx, (const double*)pb.q.p, B, 1, __builtin_inf(), (int32_t*)pb.near.p,
@@ -276,6 +276,10 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
                                (double*)nullptr);
         });
         MPTG_HIP(ctx, hipGetLastError());
+        if (ctx->scene.kind == MPTG_SCENE_MESH &&
+            (rc = launch_mesh_filter(ctx, (const double*)pb.to.p, B, (uint8_t*)pb.mask.p, (uint8_t*)pb.status.p,
+                                     ST_INVALID)))
+            return rc;
         if ((rc = launch_link_masked(ctx, (const double*)pb.from.p, (const double*)pb.to.p, B, s.p.order, 1,
                                      (const uint8_t*)pb.mask.p, (uint8_t*)pb.verdict.p, (uint32_t*)pb.checks.p)))
             return rc;

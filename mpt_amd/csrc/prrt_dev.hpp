@@ -2,6 +2,7 @@
 // steering (src/mpt/impl/prrt/prrt.hpp:377-440,
 // src/mpt/impl/prrt_star/prrt_star.hpp:470-540).
 #pragma once
+#include "sample_hd.hpp"
 #include "space_dev.hpp"
 
 namespace mptg {
@@ -21,7 +22,7 @@ struct PrrtDev {
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
 };
 
-__global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int D, uint64_t seed, double goal_bias,
+__global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int metric, int D, uint64_t seed, double goal_bias,
                                                       int64_t it, int B, const Counters* __restrict__ cnt,
                                                       double* __restrict__ q) {
     int b = blockIdx.x * kBlock + threadIdx.x;
@@ -29,11 +30,13 @@ __global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int D, uint64_
     uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
     bool bias = cnt->goals == 0 && goal_bias > 0;
     bool useGoal = bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias;
-    for (int i = 0; i < D; ++i) {
-        double v = useGoal ? pd.goal[i]
-                           : uniform_in(uniform_at(seed, sidx, RNG_PURPOSE_SAMPLE, (uint32_t)i), pd.lo[i], pd.hi[i]);
-        q[(int64_t)b * D + i] = v;
+    double v[kMaxDim];
+    if (useGoal) {
+        for (int i = 0; i < D; ++i) v[i] = pd.goal[i];
+    } else {
+        sample_state(seed, sidx, metric, D, pd.lo, pd.hi, v);
     }
+    for (int i = 0; i < D; ++i) q[(int64_t)b * D + i] = v[i];
 }
 
 template <int D>
@@ -76,7 +79,9 @@ __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc,
             from[(int64_t)b * dd + i] = a[i];
             to[(int64_t)b * dd + i] = s[i];
         }
-    if (!state_valid_t<D>(sc, s)) {
+    // mesh scenes: valid(new) is a whole-wave check, done by the filter
+    // kernel that runs next over the masked samples
+    if (sc.kind != MPTG_SCENE_MESH && !state_valid_t<D>(sc, s)) {
         status[b] = ST_INVALID;
         return;
     }

@@ -585,7 +585,7 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         if ((rc = ensure(ctx, s.front, sizeof(int32_t) * 2 * (size_t)(n0 + B)))) return rc;
         const int64_t it = s.iteration;
         // ---- extend
-        hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, D, s.p.seed, s.p.goal_bias, it,
+        hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, ctx->metric, D, s.p.seed, s.p.goal_bias, it,
                            B, (const Counters*)dcnt, (double*)s.q.p);
         MPTG_HIP(ctx, hipGetLastError());
         if ((rc = launch_knn(ctx, (const double*)s.q.p, B, 1, __builtin_inf(), (int32_t*)s.near.p,
@@ -600,6 +600,10 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
                                (double*)s.to.p, (uint8_t*)s.status.p, (uint8_t*)s.mask.p, (double*)s.dnear.p);
         });
         MPTG_HIP(ctx, hipGetLastError());
+        if (ctx->scene.kind == MPTG_SCENE_MESH &&
+            (rc = launch_mesh_filter(ctx, (const double*)s.to.p, B, (uint8_t*)s.mask.p, (uint8_t*)s.status.p,
+                                     ST_INVALID)))
+            return rc;
         if ((rc = launch_link_masked(ctx, (const double*)s.from.p, (const double*)s.to.p, B, s.p.order, 1,
                                      (const uint8_t*)s.mask.p, (uint8_t*)s.verdict.p, (uint32_t*)s.checks.p)))
             return rc;

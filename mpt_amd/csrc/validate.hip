@@ -400,6 +400,7 @@ int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
     if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > ctx->dim)
         return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
     if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
+    if (ctx->scene.kind == MPTG_SCENE_MESH) return launch_mesh_valid(ctx, d_q, n, d_ok);
     dispatch_dim(ctx->dim, [&]<int D>() {
         hipLaunchKernelGGL((valid_kernel<D>), dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            ctx->stream, scene_args(ctx), ctx->dim, d_q, n, d_ok);
@@ -426,6 +427,8 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
         return fail(ctx, MPTG_E_STATE, "surrogate order is implemented for the arm scene with bisection links");
     if (n > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 edges per call");
     unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    if (ctx->scene.kind == MPTG_SCENE_MESH && ctx->link_mode != MPTG_LINK_DMV)
+        return fail(ctx, MPTG_E_STATE, "mesh scenes use the discrete motion validator link");
     if (ctx->link_mode == MPTG_LINK_BISECT) {
         if (ctx->scene.kind == MPTG_SCENE_ARM && !ctx->force_generic) {
             int rc;
@@ -458,6 +461,9 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     }
     int rc = ensure_tables(ctx);
     if (rc) return rc;
+    if (ctx->scene.kind == MPTG_SCENE_MESH)
+        return launch_mesh_link(ctx, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n, known_valid, d_mask,
+                                d_verdict, d_checks);
     if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
     uint32_t* long_count = (uint32_t*)ctx->scratch[7].p;
     int64_t* long_list = (int64_t*)((char*)ctx->scratch[7].p + 256);

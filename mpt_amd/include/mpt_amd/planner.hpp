@@ -55,6 +55,16 @@ template <int O>
 struct check_order {
     static constexpr int value = O;
 };
+// PRRT* rewiring (src/mpt/planner_tags.hpp:49-51).  k-nearest is the
+// default, with the reference's k = ceil(kRRG ln(n + 1)); rewire_k<K> fixes
+// k (1..32).  The radius form is not implemented on the device.
+struct rewire_k_nearest {};
+struct rewire_r_nearest {};
+template <int K>
+struct rewire_k {
+    static_assert(K >= 1 && K <= 32, "rewire_k must be in 1..32");
+    static constexpr int value = K;
+};
 
 namespace detail {
 inline void check(mptg_ctx* ctx, int rc) {
@@ -97,6 +107,43 @@ struct PRRTOptions<check_order<O>, Rest...> : PRRTOptions<Rest...> {
 template <typename... Opts>
 using PRRT = PRRTStrategy<PRRTOptions<Opts...>::batch, PRRTOptions<Opts...>::stats, PRRTOptions<Opts...>::order>;
 
+// PRRT* (src/mpt/prrt_star.hpp:55-110): the PRRT options plus rewiring.
+template <int Batch, bool Stats, int Order, int K>
+struct PRRTStarStrategy {
+    static constexpr int batch = Batch;
+    static constexpr bool stats = Stats;
+    static constexpr int order = Order;
+    static constexpr int k = K;  // 0: the reference's k-nearest formula
+};
+
+template <typename... Opts>
+struct PRRTStarOptions;
+template <>
+struct PRRTStarOptions<> : PRRTOptions<> {
+    static constexpr int k = 0;
+};
+template <typename Opt, typename... Rest>
+struct PRRTStarOptions<Opt, Rest...> : PRRTStarOptions<Rest...> {
+    using Base = PRRTStarOptions<Rest...>;
+    static constexpr int batch = PRRTOptions<Opt>::batch != PRRTOptions<>::batch ? PRRTOptions<Opt>::batch : Base::batch;
+    static constexpr bool stats = PRRTOptions<Opt>::stats || Base::stats;
+    static constexpr int order = PRRTOptions<Opt>::order != PRRTOptions<>::order ? PRRTOptions<Opt>::order : Base::order;
+};
+template <typename... Rest>
+struct PRRTStarOptions<rewire_k_nearest, Rest...> : PRRTStarOptions<Rest...> {};
+template <typename... Rest>
+struct PRRTStarOptions<rewire_r_nearest, Rest...> : PRRTStarOptions<Rest...> {
+    static_assert(sizeof...(Rest) < 0, "rewire_r_nearest is not implemented by the device planner");
+};
+template <int K, typename... Rest>
+struct PRRTStarOptions<rewire_k<K>, Rest...> : PRRTStarOptions<Rest...> {
+    static constexpr int k = K;
+};
+
+template <typename... Opts>
+using PRRTStar = PRRTStarStrategy<PRRTStarOptions<Opts...>::batch, PRRTStarOptions<Opts...>::stats,
+                                  PRRTStarOptions<Opts...>::order, PRRTStarOptions<Opts...>::k>;
+
 template <typename Scenario, typename Strategy>
 class GpuPRRT;
 
@@ -106,16 +153,26 @@ template <typename Scenario, int B, bool S, int O>
 struct PlannerResolver<Scenario, PRRTStrategy<B, S, O>> {
     using type = GpuPRRT<Scenario, PRRTStrategy<B, S, O>>;
 };
+template <typename Scenario, int B, bool S, int O, int K>
+struct PlannerResolver<Scenario, PRRTStarStrategy<B, S, O, K>> {
+    using type = GpuPRRT<Scenario, PRRTStarStrategy<B, S, O, K>>;
+};
 
 template <typename Scenario, typename Algorithm>
 using Planner = typename PlannerResolver<Scenario, Algorithm>::type;
 
 // ---- the planner ---------------------------------------------------------------
+template <typename S>
+struct is_star : std::false_type {};
+template <int B, bool S, int O, int K>
+struct is_star<PRRTStarStrategy<B, S, O, K>> : std::true_type {};
+
 template <typename Scenario, typename Strategy>
 class GpuPRRT {
    public:
     using State = typename Scenario::State;
     static constexpr int kDim = Scenario::kDimensions;
+    static constexpr bool kStar = is_star<Strategy>::value;
 
    private:
     Scenario scenario_;
@@ -123,6 +180,7 @@ class GpuPRRT {
     uint64_t seed_;
     double maxDistance_ = std::numeric_limits<double>::infinity();
     double goalBias_ = 0.01;
+    double rewireFactor_ = 1.1;
     bool configured_ = false;
 
     static void toDoubles(const State& s, double* out) {
@@ -145,7 +203,14 @@ class GpuPRRT {
         p.goal_radius = scenario_.goal_radius();
         p.batch = Strategy::batch;
         p.order = Strategy::order;
-        detail::check(ctx_, mptg_prrt_init(ctx_, &p, lo, hi, goal));
+        if constexpr (kStar) {
+            mptg_prrt_star_params sp{};
+            sp.k = Strategy::k;
+            sp.rewire_factor = rewireFactor_;
+            detail::check(ctx_, mptg_prrt_star_init(ctx_, &p, &sp, lo, hi, goal));
+        } else {
+            detail::check(ctx_, mptg_prrt_init(ctx_, &p, lo, hi, goal));
+        }
         configured_ = true;
     }
 
@@ -180,6 +245,14 @@ class GpuPRRT {
         configure();
     }
     double getRange() const { return maxDistance_; }
+
+    // PRRT* only (prrt_star.hpp:225-231)
+    void setRewireFactor(double factor) {
+        static_assert(kStar, "setRewireFactor is a PRRT* method");
+        if (!(factor > 0)) throw std::invalid_argument("rewire factor must be > 0");
+        rewireFactor_ = factor;
+        configure();
+    }
 
     std::size_t size() const { return static_cast<std::size_t>(mptg_tree_size(ctx_)); }
 
@@ -234,23 +307,36 @@ class GpuPRRT {
         if (n) detail::check(ctx_, mptg_tree_read(ctx_, 0, n, states.data(), parents.data()));
     }
 
-    // Best goal by path cost (prrt.hpp:208-234); returns node id or -1.
-    int64_t bestGoal(const std::vector<double>& states, const std::vector<int32_t>& parents) const {
+    // Best goal by path cost (prrt.hpp:208-234, prrt_star.hpp:336-345); returns
+    // node id or -1.  PRRT* reads the costs the device maintains.
+    int64_t bestGoal(const std::vector<double>& states, const std::vector<int32_t>& parents,
+                     double* bestCost = nullptr) const {
         int64_t ng = 0;
         detail::check(ctx_, mptg_prrt_goals(ctx_, nullptr, 0, &ng));
         std::vector<int32_t> goals(static_cast<size_t>(ng));
         if (ng) detail::check(ctx_, mptg_prrt_goals(ctx_, goals.data(), ng, &ng));
+        std::vector<double> costs;
+        if constexpr (kStar) {
+            costs.resize(parents.size());
+            if (!costs.empty())
+                detail::check(ctx_, mptg_prrt_costs(ctx_, 0, (int64_t)costs.size(), costs.data()));
+        }
         double best = std::numeric_limits<double>::infinity();
         int64_t bestId = -1;
         for (int32_t g : goals) {
             double cost = 0;
-            for (int64_t n = g; parents[n] >= 0; n = parents[n])
-                cost += scenario_.distance(&states[n * kDim], &states[parents[n] * kDim]);
+            if constexpr (kStar) {
+                cost = costs[g];
+            } else {
+                for (int64_t n = g; parents[n] >= 0; n = parents[n])
+                    cost += scenario_.distance(&states[n * kDim], &states[parents[n] * kDim]);
+            }
             if (bestId < 0 || cost < best) {
                 best = cost;
                 bestId = g;
             }
         }
+        if (bestCost) *bestCost = bestId < 0 ? std::numeric_limits<double>::quiet_NaN() : best;
         return bestId;
     }
 
@@ -262,6 +348,16 @@ class GpuPRRT {
         std::vector<State> path;
         for (int64_t n = bestGoal(states, parents); n >= 0; n = parents[n]) path.push_back(fromDoubles(&states[n * kDim]));
         return std::vector<State>(path.rbegin(), path.rend());
+    }
+
+    // cost of the best solution, NaN if none (prrt_star.hpp:336-341)
+    double solutionCost() const {
+        std::vector<double> states;
+        std::vector<int32_t> parents;
+        readTree(states, parents);
+        double c;
+        bestGoal(states, parents, &c);
+        return c;
     }
 
     // fn(state) once per waypoint of the best path, start first (prrt.hpp:276-285)
@@ -290,6 +386,9 @@ class GpuPRRT {
             std::cout << "iterations: " << st.iterations << "\nsamples: " << st.samples
                       << "\nlink calls: " << st.link_calls << " (" << st.link_ok << " ok, " << st.valid_calls
                       << " valid() calls)\n";
+            if constexpr (kStar)
+                std::cout << "rewire tests: " << st.rewire_tests << "\nrewires: " << st.rewires
+                          << "\nparent changes: " << st.parent_changes << "\n";
         }
     }
 };

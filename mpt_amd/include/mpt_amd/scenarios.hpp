@@ -9,6 +9,9 @@
 //                       a DiscreteMotionValidator link (step given)
 //   OccupancyGrid2D     demo/png_2d_scenario.hpp:71-166 with a
 //                       DiscreteMotionValidator link (step given)
+//   RigidBodyMeshes     demo/se3_rigid_body_scenario.hpp:250-340: SE(3) with
+//                       the rotation weighted 50, robot and environment
+//                       triangle meshes, DMV step (|max - min| + 50 pi/2) * res
 //
 // Each provides State, kDimensions, gpu_scene(mptg_ctx*), bounds_to(lo, hi),
 // goal_state(), goal_radius() and distance(a, b) (used for path costs).
@@ -150,6 +153,54 @@ class OccupancyGrid2D {
     std::vector<uint8_t> cells_;
     State goal_;
     double step_;
+};
+
+class RigidBodyMeshes {
+   public:
+    static constexpr int kDimensions = 7;
+    using State = std::array<double, 7>;  // qx qy qz qw tx ty tz
+    static constexpr double kSO3Weight = 50.0;
+
+    // triangles: 9 floats each; volume [min, max]^3 bounds the translation
+    RigidBodyMeshes(std::vector<float> robot, std::vector<float> env, const State& goal, const std::array<double, 3>& min,
+                    const std::array<double, 3>& max, double checkResolution = 0.01)
+        : robot_(std::move(robot)), env_(std::move(env)), goal_(goal), min_(min), max_(max), res_(checkResolution) {
+        if (robot_.size() % 9 || env_.size() % 9) throw std::invalid_argument("triangles are 9 floats");
+    }
+
+    void gpu_scene(mptg_ctx* ctx) const {
+        detail::scene_check(ctx, mptg_space(ctx, MPTG_METRIC_SE3, 7, kSO3Weight, 1.0));
+        detail::scene_check(ctx, mptg_scene_mesh(ctx, robot_.data(), (int)(robot_.size() / 9), env_.data(),
+                                                 (int)(env_.size() / 9)));
+        double ext = 0;
+        for (int i = 0; i < 3; ++i) ext += (max_[i] - min_[i]) * (max_[i] - min_[i]);
+        detail::scene_check(ctx, mptg_set_link(ctx, MPTG_LINK_DMV, (std::sqrt(ext) + kSO3Weight * M_PI / 2) * res_));
+    }
+    void bounds_to(double* lo, double* hi) const {
+        for (int i = 0; i < 4; ++i) {
+            lo[i] = -1;
+            hi[i] = 1;
+        }
+        for (int i = 0; i < 3; ++i) {
+            lo[4 + i] = min_[i];
+            hi[4 + i] = max_[i];
+        }
+    }
+    const State& goal_state() const { return goal_; }
+    double goal_radius() const { return 1e-6; }
+    double distance(const double* a, const double* b) const {
+        double d = std::fabs(a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]);
+        double r = d >= 1.0 ? 0.0 : std::acos(d);
+        double t = 0;
+        for (int i = 4; i < 7; ++i) t += (a[i] - b[i]) * (a[i] - b[i]);
+        return kSO3Weight * r + std::sqrt(t);
+    }
+
+   private:
+    std::vector<float> robot_, env_;
+    State goal_;
+    std::array<double, 3> min_, max_;
+    double res_;
 };
 
 }  // namespace mpt_amd

@@ -16,6 +16,9 @@ synthetic obstacles:
 * :func:`arm_scene` — planar link manipulator vs disks
   (``demo/link_manipulator_scenario.hpp:54-155``), L1 metric, bisection
   tolerance 0.02 rad, no range limit (the demo sets none).
+* :func:`se3_scene` — rigid body among triangle meshes
+  (``demo/se3_rigid_body_scenario.hpp:250-340``): SE(3) metric with the
+  rotation weighted 50, DMV step ``(|max - min| + 50 pi/2) * 0.01``.
 """
 from __future__ import annotations
 
@@ -26,7 +29,7 @@ from typing import Optional
 import numpy as np
 
 METRIC_L2, METRIC_L1, METRIC_SE3 = 0, 1, 2
-SCENE_NONE, SCENE_BOXES, SCENE_GRID, SCENE_ARM = 0, 1, 2, 3
+SCENE_NONE, SCENE_BOXES, SCENE_GRID, SCENE_ARM, SCENE_MESH = 0, 1, 2, 3, 4
 LINK_DMV, LINK_BISECT = 0, 1
 ORDER_REFERENCE, ORDER_SURROGATE = 0, 1
 
@@ -52,6 +55,8 @@ class Scene:
     lengths: Optional[np.ndarray] = None    # (links,)
     robot_radius: float = 0.0
     disks: Optional[np.ndarray] = None      # (n, 3) cx cy r
+    robot_tris: Optional[np.ndarray] = None  # (n, 9) float32, robot frame
+    env_tris: Optional[np.ndarray] = None    # (n, 9) float32, world frame
     name: str = ""
     extra: dict = field(default_factory=dict)
 
@@ -189,3 +194,58 @@ def arm_scene(links: int = 20, ndisks: int = 200, seed_links: int = 7, seed_disk
     sc.start = np.array(found[0])
     sc.goal = np.array(found[1])
     return sc
+
+
+# ------------------------------------------------------------------ SE(3)
+def box_triangles(lo, hi) -> np.ndarray:
+    """12 triangles (2 per face) of the axis-aligned box [lo, hi], float32 (12, 9)."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    c = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]])
+    faces = [(0, 1, 2), (0, 2, 3), (4, 6, 5), (4, 7, 6), (0, 5, 1), (0, 4, 5),
+             (3, 2, 6), (3, 6, 7), (0, 3, 7), (0, 7, 4), (1, 5, 6), (1, 6, 2)]
+    return np.array([np.concatenate([c[a], c[b], c[d]]) for a, b, d in faces], dtype=np.float32)
+
+
+def _box_soup(rng, n_tris, centre_fn, half_fn) -> np.ndarray:
+    nb = -(-n_tris // 12)
+    out = [box_triangles(c - h, c + h) for c, h in ((centre_fn(rng), half_fn(rng)) for _ in range(nb))]
+    return np.concatenate(out)[:n_tris].astype(np.float32)
+
+
+def se3_scene(n_env_tris: int = 50000, n_robot_tris: int = 500, seed_env: int = 6, seed_robot: int = 11,
+              seed_states: int = 12, size: float = 100.0, check_resolution: float = 0.01) -> Scene:
+    """Rigid body among random boxes.  Robot: a seeded cluster of boxes
+    (centres N(0, 1) per axis, half-sides U[0.2, 0.6]), truncated to
+    n_robot_tris triangles.  Environment: boxes with centre U[0, size]^3 and
+    half-sides U[0.4, 1.6], 12 triangles each, truncated to n_env_tris
+    (about 45% of uniformly drawn poses are collision-free).  States are
+    [qx, qy, qz, qw, tx, ty, tz]; translation bounds [0, size]^3.  Start and
+    goal are drawn until the robot's bounding sphere clears every
+    environment box (so both are collision-free by construction)."""
+    rr = np.random.default_rng(seed_robot)
+    robot = _box_soup(rr, n_robot_tris, lambda r: r.normal(0, 1.0, 3), lambda r: r.uniform(0.2, 0.6, 3))
+    re = np.random.default_rng(seed_env)
+    env = _box_soup(re, n_env_tris, lambda r: r.uniform(0, size, 3), lambda r: r.uniform(0.4, 1.6, 3))
+    radius = float(np.sqrt((robot.reshape(-1, 3).astype(np.float64) ** 2).sum(axis=1)).max())
+    elo = env.reshape(-1, 3, 3).min(axis=1).astype(np.float64)
+    ehi = env.reshape(-1, 3, 3).max(axis=1).astype(np.float64)
+    rs = np.random.default_rng(seed_states)
+
+    def clear_pose():
+        while True:
+            t = rs.uniform(radius, size - radius, 3)
+            d = np.maximum(np.maximum(elo - t, t - ehi), 0.0)
+            if (np.sqrt((d ** 2).sum(axis=1)) > radius * 1.001 + 1e-3).all():
+                q = rs.normal(size=4)
+                q /= np.linalg.norm(q)
+                return np.concatenate([q, t])
+
+    lo = np.array([-1.0, -1.0, -1.0, -1.0, 0.0, 0.0, 0.0])
+    hi = np.array([1.0, 1.0, 1.0, 1.0, size, size, size])
+    step = (math.sqrt(3.0) * size + 50.0 * math.pi / 2) * check_resolution
+    start, goal = clear_pose(), clear_pose()
+    return Scene(metric=METRIC_SE3, dim=7, kind=SCENE_MESH, link_mode=LINK_DMV, link_param=step, lo=lo, hi=hi,
+                 start=start, goal=goal, w_so3=50.0, w_l2=1.0, robot_tris=robot, env_tris=env,
+                 name=f"se3_robot{n_robot_tris}_env{n_env_tris}", extra={"robot_radius": radius})

@@ -31,6 +31,8 @@ class OrcScene(C.Structure):
         ("grid", _u8p), ("width", C.c_int32), ("height", C.c_int32),
         ("lengths", _dp), ("nlinks", C.c_int32), ("robot_radius", C.c_double),
         ("disks", _dp), ("ndisks", C.c_int32),
+        ("robot_tris", C.POINTER(C.c_float)), ("nrobot", C.c_int32),
+        ("env_tris", C.POINTER(C.c_float)), ("nenv", C.c_int32), ("mesh_index", C.c_void_p),
     ]
 
 
@@ -65,6 +67,10 @@ _SIG = [
     ("orc_knn", None, [C.POINTER(OrcScene), _dp, C.c_int64, _dp, C.c_int64, C.c_int32, C.c_double, _i32p, _dp,
                        C.c_int32]),
     ("orc_valid", C.c_int32, [C.POINTER(OrcScene), _dp]),
+    ("orc_mesh_index_build", C.c_void_p, [C.POINTER(OrcScene)]),
+    ("orc_mesh_index_free", None, [C.c_void_p]),
+    ("orc_tri_tri", C.c_int32, [C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("orc_pose_robot", None, [C.POINTER(OrcScene), _dp, C.POINTER(C.c_float)]),
     ("orc_dmv_order", C.c_int64, [C.c_int64, _i64p, C.c_int64]),
     ("orc_link_waypoints", C.c_int64, [C.POINTER(OrcScene), _dp, _dp, _dp, C.c_int64]),
     ("orc_link", C.c_int32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32, _u32p]),
@@ -142,8 +148,28 @@ class Oracle:
             self._keep += [ln, dk]
             s.lengths, s.nlinks, s.robot_radius = _p(ln, C.c_double), ln.shape[0], scene.robot_radius
             s.disks, s.ndisks = _p(dk, C.c_double), dk.shape[0]
-        self.s = s
         self.L = lib()
+        self._index = None
+        if scene.robot_tris is not None:
+            rt = np.ascontiguousarray(scene.robot_tris, dtype=np.float32).reshape(-1, 9)
+            et = np.ascontiguousarray(scene.env_tris, dtype=np.float32).reshape(-1, 9)
+            self._keep += [rt, et]
+            s.robot_tris, s.nrobot = _p(rt, C.c_float), rt.shape[0]
+            s.env_tris, s.nenv = _p(et, C.c_float), et.shape[0]
+            self._index = self.L.orc_mesh_index_build(C.byref(s))
+            s.mesh_index = self._index
+        self.s = s
+
+    def __del__(self):
+        if getattr(self, "_index", None):
+            self.L.orc_mesh_index_free(self._index)
+            self._index = None
+
+    def pose_robot(self, q) -> np.ndarray:
+        q = _f64(q)
+        out = np.zeros((self.s.nrobot, 9), dtype=np.float32)
+        self.L.orc_pose_robot(C.byref(self.s), _p(q, C.c_double), _p(out, C.c_float))
+        return out
 
     def distance(self, a, b) -> float:
         a, b = _f64(a), _f64(b)
@@ -244,6 +270,12 @@ class Oracle:
         start = _f64(self.scene.start)
         self.L.orc_pool_run(C.byref(self.s), C.byref(p), _p(start, C.c_double), target, seconds, threads, C.byref(r))
         return {f: getattr(r, f) for f, _ in r._fields_}
+
+
+def tri_tri(p, q) -> bool:
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    q = np.ascontiguousarray(q, dtype=np.float32)
+    return bool(lib().orc_tri_tri(_p(p, C.c_float), _p(q, C.c_float)))
 
 
 def philox(ctr, key):

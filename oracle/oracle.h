@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 enum { ORC_METRIC_L2 = 0, ORC_METRIC_L1 = 1, ORC_METRIC_SE3 = 2 };
-enum { ORC_SCENE_NONE = 0, ORC_SCENE_BOXES = 1, ORC_SCENE_GRID = 2, ORC_SCENE_ARM = 3 };
+enum { ORC_SCENE_NONE = 0, ORC_SCENE_BOXES = 1, ORC_SCENE_GRID = 2, ORC_SCENE_ARM = 3, ORC_SCENE_MESH = 4 };
 enum { ORC_LINK_DMV = 0, ORC_LINK_BISECT = 1 };
 
 typedef struct {
@@ -41,7 +41,22 @@ typedef struct {
     double robot_radius;
     const double* disks;     /* ndisks x [cx, cy, r] */
     int32_t ndisks;
+    const float* robot_tris; /* nrobot x 9 (robot frame) */
+    int32_t nrobot;
+    const float* env_tris;   /* nenv x 9 (world frame) */
+    int32_t nenv;
+    const void* mesh_index;  /* from orc_mesh_index_build */
 } orc_scene;
+
+/* Uniform-grid index over the environment triangles of a mesh scene (the
+ * oracle's own culling, independent of the device BVH).  Build once per
+ * scene and store in sc->mesh_index; free with orc_mesh_index_free. */
+void* orc_mesh_index_build(const orc_scene* sc);
+void orc_mesh_index_free(void* index);
+/* Closed triangle-triangle intersection (the project's FP32 predicate). */
+int32_t orc_tri_tri(const float* p, const float* q);
+/* Posed robot triangles of SE(3) state q (nrobot x 9 floats). */
+void orc_pose_robot(const orc_scene* sc, const double* q, float* out);
 
 /* --- shared project math (same code as the device) ---------------------- */
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

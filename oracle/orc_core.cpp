@@ -20,6 +20,9 @@
 //     hit exits) and bisectLink (L-inf < tol, (a+b)/2, pre-order)
 //                                                          demo/link_manipulator_scenario.hpp:99-138
 //     with Circle::segmentIsValid / distPointSegmentSquared demo/shape_hierarchy.hpp:234-237,259-270
+//   * mesh: SE3RigidBodyScenario::valid (robot vs environment triangles)
+//                                                          demo/se3_rigid_body_scenario.hpp:288-312
+//     (orc_mesh.cpp)
 //
 // Parity pinning: see tests/test_oracle_*.py.  The nearest-neighbour order
 // "ascending (FP64 distance, index)" and the left-to-right summation order are
@@ -39,6 +42,10 @@
 #include <vector>
 
 #include "../mpt_amd/csrc/pmath.hpp"
+
+namespace orc_internal {
+bool valid_mesh(const orc_scene* sc, const double* q);
+}  // namespace orc_internal
 
 namespace {
 
@@ -170,6 +177,7 @@ bool valid(const orc_scene* sc, const double* q) {
         case ORC_SCENE_BOXES: return valid_boxes(sc, q);
         case ORC_SCENE_GRID: return valid_grid(sc, q);
         case ORC_SCENE_ARM: return valid_arm(sc, q);
+        case ORC_SCENE_MESH: return orc_internal::valid_mesh(sc, q);
         default: return true;
     }
 }

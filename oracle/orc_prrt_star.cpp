@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "../mpt_amd/csrc/hd.hpp"
+#include "../mpt_amd/csrc/sample_hd.hpp"
 #include "oracle.h"
 
 namespace orc_internal {
@@ -131,9 +132,7 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
             if (useGoal) {
                 std::memcpy(q, p->goal, sizeof(double) * d);
             } else {
-                for (int i = 0; i < d; ++i)
-                    q[i] = mptg::uniform_in(mptg::uniform_at(p->seed, sidx, mptg::RNG_PURPOSE_SAMPLE, i),
-                                            p->lo[i], p->hi[i]);
+                mptg::sample_state(p->seed, sidx, sc->metric, d, p->lo, p->hi, q);
             }
             int32_t best = -1;
             double bd = std::numeric_limits<double>::infinity();

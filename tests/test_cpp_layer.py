@@ -1,5 +1,6 @@
 """The C++ host layer (mpt_amd/include/mpt_amd/*.hpp over include/mptg.h):
-headers compile standalone, and the planar-arm example plans on the GPU."""
+headers compile standalone, and the examples (planar-arm PRRT, SE(3)
+rigid-body PRRT*) plan on the GPU."""
 import subprocess
 from pathlib import Path
 
@@ -7,6 +8,7 @@ import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 EXAMPLE = ROOT / "mpt_amd" / "examples" / "planar_arm.bin"
+EXAMPLE_STAR = ROOT / "mpt_amd" / "examples" / "rigid_body_star.bin"
 
 
 def test_headers_compile_standalone(tmp_path):
@@ -17,6 +19,12 @@ def test_headers_compile_standalone(tmp_path):
                    "static_assert(P::kDim == 20);\n"
                    "using Q = mpt_amd::PRRT<mpt_amd::gpu_batch<64>, mpt_amd::check_order<MPTG_ORDER_SURROGATE>>;\n"
                    "static_assert(Q::batch == 64 && Q::order == MPTG_ORDER_SURROGATE && !Q::stats);\n"
+                   "using S = mpt_amd::PRRTStar<mpt_amd::rewire_k<20>, mpt_amd::gpu_batch<512>,\n"
+                   "    mpt_amd::report_stats<true>>;\n"
+                   "static_assert(S::k == 20 && S::batch == 512 && S::stats);\n"
+                   "static_assert(mpt_amd::PRRTStar<>::k == 0 && mpt_amd::PRRTStar<>::batch == 8192);\n"
+                   "using R = mpt_amd::Planner<mpt_amd::RigidBodyMeshes, S>;\n"
+                   "static_assert(R::kStar && R::kDim == 7);\n"
                    "int main() { return 0; }\n")
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", str(ROOT / "include"), "-I",
                         str(ROOT / "mpt_amd" / "include"), str(src)], capture_output=True, text=True)
@@ -24,13 +32,22 @@ def test_headers_compile_standalone(tmp_path):
 
 
 def test_example_is_built():
-    assert EXAMPLE.exists(), "run __graft_entry__.build()"
+    assert EXAMPLE.exists() and EXAMPLE_STAR.exists(), "run __graft_entry__.build()"
 
 
 @pytest.mark.gpu
 def test_example_plans_on_gpu():
     print("planar arm example", flush=True)
     r = subprocess.run([str(EXAMPLE), "3"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "path_valid=1" in r.stdout and "solved=1" in r.stdout
+
+
+@pytest.mark.gpu
+def test_rigid_body_star_example_plans_on_gpu():
+    print("rigid body PRRT* example", flush=True)
+    r = subprocess.run([str(EXAMPLE_STAR), "5"], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], flush=True)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "path_valid=1" in r.stdout and "solved=1" in r.stdout
