@@ -276,6 +276,42 @@ def cpu_baseline(sc, seconds):
             "extensions_per_s": r["samples"] / r["seconds"]}
 
 
+def kernels_only(ctx, args):
+    """Profiling mode (--kernels): the dominant kernels alone, a few launches
+    each, so that rocprofv3 kernel traces and counter passes stay short:
+    a 256 MiB plain copy (byte-counter reference), then the arm link on 8192
+    planner edges in reference and in surrogate order, and the score kernel."""
+    import mpt_amd
+    from mpt_amd import scenes, surrogate
+    sc = scenes.arm_scene()
+    ctx.scene(sc)
+    ctx.tree_clear()
+    ctx.prrt_init(sc, seed=args.seed, batch=8192)
+    ctx.prrt_add_start(sc.start)
+    ctx.prrt_run(1 << 30, 3)
+    nodes, _ = ctx.tree_read()
+    surrogate.train(ctx, sc, nodes, n_samples=1 << 18, epochs=1, seed=5)
+    ne = 8192
+    a, b = surrogate.planner_edges(ctx, sc, nodes, ne, seed=99)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    v = ctx.array(ne, np.uint8)
+    c = ctx.array(ne, np.uint32)
+    scores = ctx.array((ne, 512), np.float32)
+    nbytes = 256 << 20
+    src = ctx.array(nbytes // 4, np.float32)
+    dst = ctx.array(nbytes // 4, np.float32)
+    src.zero()
+    timer = Timer(ctx)
+    reps = max(1, args.steps)
+    res = {"copy_bytes": nbytes,
+           "copy_ms": timer.time(lambda: ctx.diag_copy(dst.ptr, src.ptr, nbytes), reps) * 1e3}
+    for name, order in (("plain", mpt_amd.ORDER_REFERENCE), ("surrogate", mpt_amd.ORDER_SURROGATE)):
+        res[f"link_{name}_ms"] = timer.time(
+            lambda order=order: ctx.link_dev(da.ptr, db.ptr, ne, order, True, v.ptr, c.ptr), reps) * 1e3
+    res["score_ms"] = timer.time(lambda: ctx.surrogate_score_edges(da.ptr, db.ptr, ne, scores.ptr), reps) * 1e3
+    print(json.dumps({"mode": "kernels", "edges": ne, "reps": reps, **res}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,12 +322,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--train-samples", type=int, default=1 << 22)
     ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--kernels", action="store_true", help="profiling mode: the dominant kernels only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import mpt_amd
     ctx = mpt_amd.Context(0)
+    if args.kernels:
+        kernels_only(ctx, args)
+        ctx.close()
+        return
     timer = Timer(ctx)
     log("arm planner ...")
     arm, roof, sc = bench_arm(ctx, args, timer)

@@ -240,6 +240,9 @@ int mptg_surrogate_score_edges(mptg_ctx* ctx, const double* d_from, const double
  *      tests against the host build. */
 enum mptg_diag_op { MPTG_DIAG_SIN = 0, MPTG_DIAG_COS = 1, MPTG_DIAG_ACOS = 2, MPTG_DIAG_SQRT = 3 };
 int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y);
+/* plain device-to-device copy kernel of `bytes` (multiple of 16, 16-byte
+ * aligned), asynchronous: the known-size reference for hardware byte counters */
+int mptg_diag_copy(mptg_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
 /* Work of a link() batch on the arm scene in the given order, from untimed
  * instrumented copies of the kernels (used to price a launch in FLOPs):
  *   h_work[0..4] = the GPU kernel's executed work: valid() calls, forward-

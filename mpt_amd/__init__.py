@@ -250,6 +250,10 @@ class Context:
         self._chk(_lib.mptg_link_dev(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, order,
                                      1 if known_valid else 0, C.c_void_p(v_ptr), C.c_void_p(c_ptr)))
 
+    def diag_copy(self, dst: int, src: int, nbytes: int):
+        """Asynchronous plain copy kernel between device buffers (byte-counter reference)."""
+        self._chk(_lib.mptg_diag_copy(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes))
+
     def diag_math(self, op: int, x) -> np.ndarray:
         """op: 0 sin, 1 cos, 2 acos, 3 sqrt — the device build of the project math."""
         x = _f64(x).ravel()

@@ -22,10 +22,30 @@ __global__ void diag_math_kernel(int op, const double* __restrict__ x, int64_t n
     y[i] = r;
 }
 
+// Plain device copy (16-byte vector loads and stores, grid-stride): the
+// reference point for byte counters (read n bytes, write n bytes).
+__global__ __launch_bounds__(256) void diag_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                        int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 }  // namespace
 }  // namespace mptg
 
 using namespace mptg;
+
+extern "C" int mptg_diag_copy(mptg_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (bytes % 16 || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16 || (bytes && (!d_dst || !d_src)))
+        return fail(ctx, MPTG_E_ARG, "copy needs 16-byte aligned buffers and a multiple of 16 bytes");
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 == 0) return MPTG_OK;
+    const unsigned g = (unsigned)std::min<int64_t>((n16 + 255) / 256, (int64_t)ctx->cus * 32);
+    hipLaunchKernelGGL(diag_copy_kernel, dim3(g), dim3(256), 0, ctx->stream, (const uint4*)d_src, (uint4*)d_dst, n16);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
 
 namespace mptg {
 int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order, int known_valid,
