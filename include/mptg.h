@@ -240,6 +240,11 @@ int mptg_surrogate_score_edges(mptg_ctx* ctx, const double* d_from, const double
  *      tests against the host build. */
 enum mptg_diag_op { MPTG_DIAG_SIN = 0, MPTG_DIAG_COS = 1, MPTG_DIAG_ACOS = 2, MPTG_DIAG_SQRT = 3 };
 int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y);
+/* one arm-scene link() call (as mptg_link_dev) timed phase by phase with HIP
+ * events on the context stream: h_ms[0] surrogate scoring, [1] ranking,
+ * [2] packed head round, [3] continuation, [4] irregular-edge walk (ms) */
+int mptg_link_phases(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                     int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks, float* h_ms);
 /* plain device-to-device copy kernel of `bytes` (multiple of 16, 16-byte
  * aligned), asynchronous: the known-size reference for hardware byte counters */
 int mptg_diag_copy(mptg_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);

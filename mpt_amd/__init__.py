@@ -250,6 +250,13 @@ class Context:
         self._chk(_lib.mptg_link_dev(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, order,
                                      1 if known_valid else 0, C.c_void_p(v_ptr), C.c_void_p(c_ptr)))
 
+    def link_phases(self, a: int, b: int, n: int, order: int, known_valid: bool, verdict: int, checks: int) -> dict:
+        """Per-phase milliseconds of one arm link() call on device buffers."""
+        ms = (C.c_float * 5)()
+        self._chk(_lib.mptg_link_phases(self._h, C.c_void_p(a), C.c_void_p(b), n, order, 1 if known_valid else 0,
+                                        C.c_void_p(verdict), C.c_void_p(checks), ms))
+        return dict(zip(("score", "rank", "head", "tail", "list"), (float(x) for x in ms)))
+
     def diag_copy(self, dst: int, src: int, nbytes: int):
         """Asynchronous plain copy kernel between device buffers (byte-counter reference)."""
         self._chk(_lib.mptg_diag_copy(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes))

@@ -94,6 +94,8 @@ _SIGNATURES = [
     ("mptg_surrogate_train", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_float,
                                        C.POINTER(C.c_float)]),
     ("mptg_surrogate_score_edges", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("mptg_link_phases", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.POINTER(C.c_float)]),
     ("mptg_diag_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("mptg_diag_math", C.c_int, [C.c_void_p, C.c_int, c_double_p, C.c_int64, c_double_p]),
     ("mptg_link_work", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int,

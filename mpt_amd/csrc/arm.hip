@@ -20,7 +20,8 @@
 // exact walk.
 //
 // valid() per lane: forward kinematics with the project sincos, then every
-// link segment against every disk.  Disks are read with wave-uniform (scalar)
+// link segment against every disk (in the continuation, after the wave-wide
+// disk culling of arm_ok_wave: a chunk's 64 waypoints come from one edge).  Disks are read with wave-uniform (scalar)
 // loads.  A disk whose centre is farther than (link length + r + R) from the
 // segment start cannot satisfy dist <= r + R; the test skips it after a
 // bound check with a relative margin of 2^-30 that dominates the rounding of
@@ -158,9 +159,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const double* b = to + ec * D;
     const uint32_t base = 2;
     if (!known_valid) {
-        bool bad = false;
-        if (live && rk < 2)
-            bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return rk == 0 ? a[i] : b[i]; }, &wk);
+        const bool act = live && rk < 2;
+        const bool bad = act && !arm_ok_wave<D, COUNT>(sc, [&](int i) { return rk == 0 ? a[i] : b[i]; }, act, &wk);
         const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
         if (live && (g & 3u)) {
             if (rk == 0) {
@@ -185,13 +185,15 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
         }
     }
     const uint32_t W = live ? (1u << h) - 1u : 0u;
-    bool bad = false;
-    if (live && (uint32_t)rk < W) {
-        uint32_t path;
-        int depth;
-        preorder_path(h, ordinal_at<ORD>(ranked, e, (uint32_t)rk), &path, &depth);
-        bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, &wk);
-    }
+    // the whole wave takes part in the disk culling; lanes without a
+    // waypoint check the edge's end state and do not count
+    const bool act = live && (uint32_t)rk < W;
+    uint32_t path = 0;
+    int depth = 0;
+    if (act) preorder_path(h, ordinal_at<ORD>(ranked, e, (uint32_t)rk), &path, &depth);
+    const bool bad =
+        act && !arm_ok_wave<D, COUNT>(sc, [&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; },
+                                      act, &wk);
     const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
     if (live && rk == 0) {
         if (g) {
@@ -235,14 +237,15 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
         bool decided = false;
         for (uint32_t c0 = kHeadRanks; c0 < W && !decided; c0 += kWave) {
             const uint32_t r = c0 + (uint32_t)lane;
-            bool bad = false;
-            if (r < W) {
-                uint32_t path;
-                int depth;
-                preorder_path(h, ordinal_at<ORD>(ranked, e, r), &path, &depth);
-                bad = !arm_ok_fn<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, &wk);
-            }
-            const uint64_t m = __ballot(bad);
+            // every lane takes part in the wave's disk culling; lanes past
+            // the last rank recompute the last waypoint and do not count
+            const bool active = r < W;
+            uint32_t path;
+            int depth;
+            preorder_path(h, ordinal_at<ORD>(ranked, e, active ? r : W - 1u), &path, &depth);
+            const bool bad =
+                !arm_ok_wave<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, active, &wk);
+            const uint64_t m = __ballot(bad && active);
             if (m) {
                 if (lane == 0) {
                     verdict[e] = 0;
@@ -278,6 +281,7 @@ static void launch_arm_phases(mptg_ctx* ctx, const double* from, const double* t
     hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD>), dim3((unsigned)((n + per_block - 1) / per_block)),
                        dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, ranked, verdict,
                        checks, irr, nirr, cont, ncont, work);
+    phase_mark(ctx, 3);
     hipLaunchKernelGGL((arm_link_tail<D, COUNT, ORD>), dim3((unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock)),
                        dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, tol, sa, ranked, cont, ncont, verdict,
                        checks, work);
@@ -317,6 +321,7 @@ int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t
                            ctx->link_param, scores, (uint16_t*)ctx->scratch[9].p);
         ranked = (const uint16_t*)ctx->scratch[9].p;
     }
+    phase_mark(ctx, 2);
     switch (ctx->dim) {
         case 5: launch_arm_d<5>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;
         case 10: launch_arm_d<10>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "internal.hpp"
+#include "space_dev.hpp"
 
 namespace mptg {
 
@@ -139,6 +140,8 @@ void mptg_destroy(mptg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->phase_ev)
+        if (e) (void)hipEventDestroy(e);
     free_tree(ctx);
     (void)hipFree(ctx->scene.boxes);
     (void)hipFree(ctx->scene.grid);
@@ -146,6 +149,7 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.disks);
     (void)hipFree(ctx->scene.arm_disks.p);
     (void)hipFree(ctx->scene.arm_lim.p);
+    (void)hipFree(ctx->scene.arm_index.p);
     (void)hipFree(ctx->scene.mesh_nodes.p);
     (void)hipFree(ctx->scene.mesh_tris.p);
     (void)hipFree(ctx->scene.mesh_robot.p);
@@ -402,6 +406,36 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
         ad[4 * i + 2] = rt * rt;
         ad[4 * i + 3] = rt;
     }
+    // disks in ascending cx (ties by index) plus a 64-bucket index over cx, so
+    // a query x-interval maps to one contiguous range of disks; the verdict
+    // is an AND over disks, so their order changes nothing
+    {
+        std::vector<int> ord((size_t)ndisks);
+        for (int i = 0; i < ndisks; ++i) ord[(size_t)i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return ad[4 * x] < ad[4 * y]; });
+        std::vector<double> sorted(ad.size(), 0.0);
+        for (int i = 0; i < ndisks; ++i)
+            for (int k = 0; k < 4; ++k) sorted[4 * (size_t)i + k] = ad[4 * (size_t)ord[(size_t)i] + k];
+        ad.swap(sorted);
+    }
+    std::vector<double> index(4 + (kArmBuckets + 1 + 1) / 2, 0.0);
+    {
+        double xmin = ndisks ? ad[0] : 0.0, xmax = ndisks ? ad[4 * (size_t)(ndisks - 1)] : 0.0, rtmax = 0.0;
+        for (int i = 0; i < ndisks; ++i) rtmax = std::max(rtmax, ad[4 * (size_t)i + 3]);
+        const double inv_bw = xmax > xmin ? (double)kArmBuckets / (xmax - xmin) : 0.0;
+        index[0] = xmin;
+        index[1] = inv_bw;
+        index[2] = rtmax;
+        int32_t* starts = reinterpret_cast<int32_t*>(index.data() + 4);
+        int j = 0;
+        for (int k = 0; k <= kArmBuckets; ++k) {
+            while (j < ndisks && arm_bucket(ad[4 * (size_t)j], xmin, inv_bw) < k) ++j;
+            starts[k] = j;
+        }
+    }
+    if ((rc = ensure(ctx, ctx->scene.arm_index, sizeof(double) * index.size()))) return rc;
+    MPTG_HIP(ctx, hipMemcpyAsync(ctx->scene.arm_index.p, index.data(), sizeof(double) * index.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
     for (int i = 0; i < nlinks; ++i) lim[i] = h_lengths[i] * (1.0 + 0x1.0p-30) + 1e-12;
     for (int g = 0; g < ngroups; ++g) {
         double S = 0.0;

@@ -68,6 +68,24 @@ extern "C" int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double*
     return MPTG_OK;
 }
 
+extern "C" int mptg_link_phases(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                                int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks, float* h_ms) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (!h_ms) return fail(ctx, MPTG_E_ARG, "null output");
+    if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->link_mode != MPTG_LINK_BISECT || ctx->force_generic)
+        return fail(ctx, MPTG_E_STATE, "phase timing covers the arm scene's bisection link");
+    for (hipEvent_t& e : ctx->phase_ev)
+        if (!e) MPTG_HIP(ctx, hipEventCreate(&e));
+    ctx->phase_timing = true;
+    rc = launch_link(ctx, d_from, d_to, n, order, endpoints_known_valid, d_verdict, d_checks);
+    ctx->phase_timing = false;
+    if (rc) return rc;
+    MPTG_HIP(ctx, hipEventSynchronize(ctx->phase_ev[5]));
+    for (int i = 0; i < 5; ++i) MPTG_HIP(ctx, hipEventElapsedTime(&h_ms[i], ctx->phase_ev[i], ctx->phase_ev[i + 1]));
+    return MPTG_OK;
+}
+
 extern "C" int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y) {
     int rc = check_device_ready(ctx);
     if (rc) return rc;

@@ -37,7 +37,8 @@ struct SceneDev {
     double robot_radius = 0;
     double* disks = nullptr;  // [cx, cy, (r+R)^2] per disk
     int ndisks = 0;
-    DevBuf arm_disks, arm_lim;  // kernel layout of the disks, per-link bounds
+    DevBuf arm_disks, arm_lim;  // kernel layout of the disks (ascending cx), per-link bounds
+    DevBuf arm_index;           // [xmin, 1/bucket width, max (r+R), 0] + int32 bucket starts
     double total_length = 0;    // sum of link lengths (feature scale)
     // mesh (SE(3) rigid body): env BVH nodes, env triangles, robot triangles
     DevBuf mesh_nodes, mesh_tris, mesh_robot;
@@ -94,7 +95,8 @@ struct mptg_ctx {
     double link_param = 1.0;
 
     // scratch (grow-only)
-    mptg::DevBuf scratch[12];  // 0-3 host staging, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores
+    mptg::DevBuf scratch[12];  // 0-3 host staging, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
+                               // 9 ranked ordinals, 10 continuation list
     // pinned host staging
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -103,11 +105,20 @@ struct mptg_ctx {
     bool force_generic = false;  // route link() to the per-edge reference kernels (testing)
     mptg::Surrogate* surrogate = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // per-phase timing of one arm link() call (mptg_link_phases): events
+    // recorded between its launches when `phase_timing` is set
+    bool phase_timing = false;
+    hipEvent_t phase_ev[6] = {};
 };
 
 namespace mptg {
 
 extern thread_local std::string g_create_err;
+
+// Marks phase boundary i of a timed arm link() call (no-op otherwise).
+inline void phase_mark(mptg_ctx* ctx, int i) {
+    if (ctx->phase_timing) (void)hipEventRecord(ctx->phase_ev[i], ctx->stream);
+}
 
 inline int fail(mptg_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;

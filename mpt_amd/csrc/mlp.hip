@@ -85,43 +85,69 @@ __global__ __launch_bounds__(256) void gemm_nt(int M, int N, int K, const uint16
 }
 
 // Output layer and loss: z = H3 . w4 + b4; dz = (sigmoid(z) - y) / N;
-// dH3 = bf16(dz * w4 where H3 > 0).
+// dH3 = bf16(dz * w4 where H3 > 0).  A block takes 64 rows, each wave 16 of
+// them in turn: lanes hold 2 of the 128 hidden units (coalesced row loads), a
+// wave reduction gives the logit, and the gradient row goes to dH3 directly
+// and to dH3^T through an LDS tile so both stores are coalesced.
+constexpr int kHeadRows = 64;
+
 __global__ __launch_bounds__(256) void head_kernel(int M, const uint16_t* __restrict__ H3,
                                                    const uint16_t* __restrict__ w4, const float* __restrict__ b4,
                                                    const float* __restrict__ y, float inv_n, float* __restrict__ z,
                                                    float* __restrict__ dz, uint16_t* __restrict__ dH3,
                                                    uint16_t* __restrict__ dH3t, float* __restrict__ loss) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    float l = 0.f;
-    if (r < M) {
-        float acc = 0.f;
-        for (int k = 0; k < HID; ++k) acc += bf16_to_float(H3[(int64_t)r * HID + k]) * bf16_to_float(w4[k]);
-        float zz = acc + b4[0];
-        if (z) z[r] = zz;
-        if (y) {
-            float yy = y[r];
-            l = fmaxf(zz, 0.f) - zz * yy + log1pf(expf(-fabsf(zz)));
-            float d = (1.f / (1.f + expf(-zz)) - yy) * inv_n;
-            dz[r] = d;
-            for (int k = 0; k < HID; ++k) {
-                uint16_t h = H3[(int64_t)r * HID + k];
-                uint16_t g = float_to_bf16(h != 0 ? d * bf16_to_float(w4[k]) : 0.f);
-                dH3[(int64_t)r * HID + k] = g;
-                dH3t[(int64_t)k * M + r] = g;
+    __shared__ uint16_t tile[HID][kHeadRows + 2];
+    __shared__ float wl[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int r0 = blockIdx.x * kHeadRows;
+    const float wa = bf16_to_float(w4[2 * lane]), wb = bf16_to_float(w4[2 * lane + 1]);
+    float lsum = 0.f;
+    // all 16 row loads of this wave in flight before any is used
+    constexpr int kRowsPerWave = kHeadRows / 4;
+    uint32_t pairs[kRowsPerWave];
+    float ys[kRowsPerWave];
+#pragma unroll
+    for (int j = 0; j < kRowsPerWave; ++j) {
+        const int r = r0 + wid + 4 * j;
+        pairs[j] = r < M ? reinterpret_cast<const uint32_t*>(H3 + (int64_t)r * HID)[lane] : 0u;
+        ys[j] = (y && r < M) ? y[r] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kRowsPerWave; ++j) {
+        const int i = wid + 4 * j;
+        const int r = r0 + i;
+        uint16_t g0 = 0, g1 = 0;
+        if (r < M) {  // wave-uniform
+            const uint16_t h0 = (uint16_t)(pairs[j] & 0xffffu), h1 = (uint16_t)(pairs[j] >> 16);
+            float acc = bf16_to_float(h0) * wa + bf16_to_float(h1) * wb;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            const float zz = acc + b4[0];
+            if (z && lane == 0) z[r] = zz;
+            if (y) {
+                const float yy = ys[j];
+                const float d = (1.f / (1.f + expf(-zz)) - yy) * inv_n;
+                if (lane == 0) {
+                    lsum += fmaxf(zz, 0.f) - zz * yy + log1pf(expf(-fabsf(zz)));
+                    dz[r] = d;
+                }
+                g0 = float_to_bf16(h0 != 0 ? d * wa : 0.f);
+                g1 = float_to_bf16(h1 != 0 ? d * wb : 0.f);
+                reinterpret_cast<uint32_t*>(dH3 + (int64_t)r * HID)[lane] = (uint32_t)g0 | ((uint32_t)g1 << 16);
             }
         }
+        tile[2 * lane][i] = g0;
+        tile[2 * lane + 1][i] = g1;
     }
-    if (y) {
-        // block sum of the loss
-        __shared__ float red[256];
-        red[threadIdx.x] = l;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_n);
+    if (!y) return;  // block-uniform
+    if (lane == 0) wl[wid] = lsum;
+    __syncthreads();
+    // dH3^T rows k, columns r0 .. r0 + 63: 64 consecutive rows per k
+    for (int idx = threadIdx.x; idx < HID * kHeadRows; idx += 256) {
+        const int k = idx / kHeadRows, i = idx % kHeadRows;
+        if (r0 + i < M) dH3t[(int64_t)k * M + r0 + i] = tile[k][i];
     }
+    if (threadIdx.x == 0) atomicAdd(loss, (wl[0] + wl[1] + wl[2] + wl[3]) * inv_n);
 }
 
 // out[row] = sum_j X[row][j] * (w ? w[j] : 1), X BF16 [rows][len]: bias
@@ -485,7 +511,7 @@ static void sur_forward(mptg_ctx* ctx, Surrogate& S, int M, bool with_loss) {
          Epi{p + OFF_B2, nullptr, (uint16_t*)S.h2.p, (uint16_t*)S.h2t.p, nullptr});
     gemm(ctx, EPI_FWD, M, HID, HID, S.h2.p, wt + IN * HID + HID * HID,
          Epi{p + OFF_B3, nullptr, (uint16_t*)S.h3.p, (uint16_t*)S.h3t.p, nullptr});
-    hipLaunchKernelGGL(head_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ctx->stream, M,
+    hipLaunchKernelGGL(head_kernel, dim3((unsigned)((M + kHeadRows - 1) / kHeadRows)), dim3(256), 0, ctx->stream, M,
                        (const uint16_t*)S.h3.p, (const uint16_t*)S.wb.p + IN * HID + 2 * HID * HID, p + OFF_B4,
                        with_loss ? (const float*)S.y.p : nullptr, 1.0f / (float)M, (float*)S.z.p, (float*)S.dz.p,
                        (uint16_t*)S.dh3.p, (uint16_t*)S.dh3t.p, (float*)S.loss.p);

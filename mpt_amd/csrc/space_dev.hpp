@@ -34,9 +34,19 @@ struct SceneArgs {
     int nlinks;
     const double* disks;  // [cx, cy, (r + R)^2]
     int ndisks;
-    const ArmDisk* arm_disks;  // same disks, kernel layout (built by arm_prepare)
+    const ArmDisk* arm_disks;  // same disks, kernel layout, ascending cx
     const double* arm_lim;     // per-link segment-length bound
+    const double* arm_index;   // [xmin, 1/bucket width, max (r+R), 0], then int32 starts[kArmBuckets + 1]
 };
+
+// Bucket of an x coordinate in the arm scene's disk index; the host build and
+// the device query use this same expression, so it is monotone in x on both.
+constexpr int kArmBuckets = 64;
+constexpr int kDiskBatch = 4;  // disks loaded together in the culled loop
+MPTG_HD int arm_bucket(double x, double xmin, double inv_bw) {
+    const double t = (x - xmin) * inv_bw;
+    return t < 0.0 ? 0 : (t >= (double)kArmBuckets ? kArmBuckets - 1 : (int)t);
+}
 
 inline SpaceArgs space_args(const mptg_ctx* ctx) { return SpaceArgs{ctx->metric, ctx->dim, ctx->w_so3, ctx->w_l2}; }
 
@@ -44,7 +54,7 @@ inline SceneArgs scene_args(const mptg_ctx* ctx) {
     const SceneDev& s = ctx->scene;
     return SceneArgs{s.kind,    s.boxes,  s.nboxes, s.grid,   s.width,
                      s.height,  s.lengths, s.nlinks, s.disks, s.ndisks,
-                     (const ArmDisk*)s.arm_disks.p, (const double*)s.arm_lim.p};
+                     (const ArmDisk*)s.arm_disks.p, (const double*)s.arm_lim.p, (const double*)s.arm_index.p};
 }
 
 __device__ __forceinline__ double so3_dist(const double* a, const double* b) {
@@ -288,6 +298,117 @@ __device__ __forceinline__ bool arm_ok_fn(const SceneArgs& sc, Coord&& coord, Ar
         py = last_y;
     }
     return true;
+}
+
+__device__ __forceinline__ double wave_min_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// arm_ok_fn for a whole wave whose lanes check nearby states (one edge's
+// waypoints): every lane must call it (lanes with active == false compute on
+// any valid coordinates and return true).  Per group, the lanes' group-circle
+// centres span a box; a disk farther than (group half-length + r + R)
+// (1 + 2^-40) + 1e-12 outside that box fails every lane's own group test
+// (the margin dominates the rounding of dx*dx + dy*dy > R*R for coordinates
+// below 10^3), so the wave skips it with one uniform branch.  Other disks go
+// through the per-lane tests of arm_ok_fn unchanged.  Lanes that already
+// found a collision leave the box; the verdicts equal arm_ok_fn's.
+template <int D, bool COUNT = false, typename Coord>
+__device__ __forceinline__ bool arm_ok_wave(const SceneArgs& sc, Coord&& coord, bool active, ArmWork* wk = nullptr) {
+    constexpr int G = (D + kArmGroup - 1) / kArmGroup;
+    if constexpr (COUNT) wk->calls += active ? 1 : 0;
+    bool ok = true;
+    double px = 0.0, py = 0.0, angle = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        constexpr int GS = kArmGroup;
+        const int i0 = g * GS;
+        const int n = (i0 + GS <= D) ? GS : D - i0;
+        double jx[GS + 1], jy[GS + 1];
+        jx[0] = px;
+        jy[0] = py;
+#pragma unroll
+        for (int k = 0; k < GS; ++k) {
+            if (k < n) {
+                angle += coord(i0 + k);
+                double s, c;
+                pm::sincos(angle, &s, &c);
+                double L = sc.lengths[i0 + k];
+                jx[k + 1] = jx[k] + L * c;
+                jy[k + 1] = jy[k] + L * s;
+                if constexpr (COUNT) wk->fk += (active && ok) ? 1 : 0;
+            }
+        }
+        double last_x = jx[0], last_y = jy[0];
+#pragma unroll
+        for (int k = 1; k <= GS; ++k)
+            if (k <= n) {
+                last_x = jx[k];
+                last_y = jy[k];
+            }
+        const double mx = (jx[0] + last_x) / 2, my = (jy[0] + last_y) / 2;
+        const bool live = active && ok;
+        if (__ballot(live) == 0) break;  // wave-uniform
+        const double inf = __builtin_inf();
+        const double bx0 = wave_min_f64(live ? mx : inf), bx1 = wave_max_f64(live ? mx : -inf);
+        const double by0 = wave_min_f64(live ? my : inf), by1 = wave_max_f64(live ? my : -inf);
+        const double rho = sc.arm_lim[D + g];
+        // disks whose cx lies within the box's x-range widened by the largest
+        // (group half-length + r + R) form one contiguous run of the sorted
+        // disks; every other disk is farther than its own bound below
+        const double* ix = sc.arm_index;
+        const int32_t* starts = reinterpret_cast<const int32_t*>(ix + 4);
+        const double Rmax = (rho + ix[2]) * (1.0 + 0x1.0p-40) + 1e-12;
+        const int j0 = starts[arm_bucket(bx0 - Rmax, ix[0], ix[1])];
+        const int j1 = starts[arm_bucket(bx1 + Rmax, ix[0], ix[1]) + 1];
+        for (int jb = j0; jb < j1; jb += kDiskBatch) {
+            ArmDisk batch[kDiskBatch];
+#pragma unroll
+            for (int u = 0; u < kDiskBatch; ++u) batch[u] = sc.arm_disks[jb + u < j1 ? jb + u : j1 - 1];
+#pragma unroll
+            for (int u = 0; u < kDiskBatch; ++u) {
+                if (jb + u >= j1) break;  // uniform
+                const ArmDisk dk = batch[u];
+                const double Rw = (rho + dk.rt) * (1.0 + 0x1.0p-40) + 1e-12;
+                if (dk.cx < bx0 - Rw || dk.cx > bx1 + Rw || dk.cy < by0 - Rw || dk.cy > by1 + Rw) continue;  // uniform
+                if (!(active && ok)) continue;
+                double dx = dk.cx - mx, dy = dk.cy - my;
+                double R = rho + dk.rt;
+                if constexpr (COUNT) wk->group += 1;
+                if (dx * dx + dy * dy > R * R) continue;
+#pragma unroll
+                for (int k = 0; k < GS; ++k) {
+                    if (k < n && ok) {
+                        double vx = jx[k + 1] - jx[k], vy = jy[k + 1] - jy[k];
+                        if constexpr (COUNT) {
+                            double wx = dk.cx - jx[k], wy = dk.cy - jy[k];
+                            double far = sc.arm_lim[i0 + k] + dk.rt;
+                            if (wx * wx + wy * wy > far * far) wk->far += 1;
+                            else wk->exact += 1;
+                        }
+                        if (!disk_clear(dk, jx[k], jy[k], jx[k + 1], jy[k + 1], vx, vy, sc.arm_lim[i0 + k]))
+                            ok = false;
+                    }
+                }
+            }
+        }
+        px = last_x;
+        py = last_y;
+    }
+    return ok;
 }
 
 template <int D>

@@ -437,19 +437,23 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
             int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
             MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
             const float* scores = nullptr;
+            phase_mark(ctx, 0);
             if (order == MPTG_ORDER_SURROGATE) {
                 if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreSlots * n))) return rc;
                 if ((rc = surrogate_score_edges(ctx, d_from, d_to, n, d_mask, (float*)ctx->scratch[8].p))) return rc;
                 scores = (const float*)ctx->scratch[8].p;
             }
+            phase_mark(ctx, 1);
             rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks, irr, nirr, nullptr,
                                  scores);
             if (rc == MPTG_OK) {
+                phase_mark(ctx, 4);
                 // listed (irregular) edges: exact per-edge walk; usually an empty launch
                 hipLaunchKernelGGL(link_bisect_list, dim3(16), dim3(kBlock), 0, ctx->stream, ctx->dim,
                                    scene_args(ctx), ctx->link_param, d_from, d_to, known_valid, (const int64_t*)irr,
                                    (const uint32_t*)nirr, d_verdict, d_checks);
                 MPTG_HIP(ctx, hipGetLastError());
+                phase_mark(ctx, 5);
                 return MPTG_OK;
             }
             if (rc != MPTG_E_STATE) return rc;  // E_STATE: no wave kernel for this arm, use the generic one
