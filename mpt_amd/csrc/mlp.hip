@@ -292,53 +292,125 @@ __global__ void features_kernel(const double* __restrict__ q, int64_t n, const d
 }  // namespace
 
 // ---- fused inference over the waypoints of arm edges ----------------------
-// One wave (= one block) scores 64 waypoints of one edge: it rebuilds each
-// waypoint with the reference's recursion (FP64), computes its features,
-// and runs the three hidden layers on the matrix cores with activations in
-// LDS.  Rows are 64 waypoints; a layer is 4 row tiles x 8 column tiles of
-// 16x16, K in steps of 32.  Weights come from the BF16 W^T copies (L2/L1
-// resident, 80 KiB).
+// One wave (= one block) scores 64 waypoints of one edge.  The features come
+// from an FP32 interpolation of the waypoint (its dyadic position along the
+// edge) rather than the reference's FP64 midpoint recursion: scores only
+// order the exact checks, so a 1e-7 relative difference in a feature is
+// immaterial.  Each hidden layer is computed transposed, H^T = W . X^T, with
+// the weights as the MFMA A operand (16 output units x 32 inputs) and the
+// activations as the B operand (32 inputs x 16 waypoints): a lane's output
+// fragment is then 4 consecutive units of one waypoint, stored as one 8-byte
+// LDS write after a hardware BF16 conversion, and read back by the next
+// layer as 16-byte B fragments.  The output unit's dot product is folded
+// into the last layer's epilogue.  Weights come from the BF16 W^T copies
+// (L2/L1 resident, 80 KiB).
 constexpr int kLdsRow = HID + 8;  // padded row (BF16 elements)
 
-// One layer for the wave's 64 rows: the layer input (64 x K, BF16, in LDS)
-// is read once into registers as MFMA A fragments, then each 16-column slab
-// of the output loads its weight fragments once and reuses them for all four
-// row tiles; bias + ReLU results overwrite the LDS buffer.
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    const __bf16 x = (__bf16)a, y = (__bf16)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+}
+
+// buf[64 waypoints][K] (BF16, row stride kLdsRow) -> buf[64][128] = relu(X W^T + b)
 template <int K>
-__device__ __forceinline__ void wave_layer(uint16_t* __restrict__ buf, const uint16_t* __restrict__ wt,
-                                           const float* __restrict__ bias, int lane) {
+__device__ __forceinline__ void wave_layer_t(uint16_t* __restrict__ buf, const uint16_t* __restrict__ wt,
+                                             const float* __restrict__ bias, int lane) {
     constexpr int KS = K / 32;
     const int r = lane & 15, kq = lane >> 4;
-    bf16x8 af[4][KS];
+    bf16x8 xf[4][KS];
 #pragma unroll
-    for (int tm = 0; tm < 4; ++tm)
+    for (int st = 0; st < 4; ++st)
 #pragma unroll
-        for (int kk = 0; kk < KS; ++kk) af[tm][kk] = load8(buf + (tm * 16 + r) * kLdsRow + kk * 32 + 8 * kq);
-    __syncthreads();  // every lane has its fragments before the buffer is overwritten
+        for (int kk = 0; kk < KS; ++kk) xf[st][kk] = load8(buf + (st * 16 + r) * kLdsRow + kk * 32 + 8 * kq);
+    __syncthreads();  // every lane holds its fragments before the buffer is overwritten
 #pragma unroll 2
-    for (int tn = 0; tn < 8; ++tn) {
-        bf16x8 bf[KS];
+    for (int ot = 0; ot < HID / 16; ++ot) {
+        bf16x8 wf[KS];
 #pragma unroll
-        for (int kk = 0; kk < KS; ++kk) bf[kk] = load8(wt + (tn * 16 + r) * K + kk * 32 + 8 * kq);
-        f32x4 acc[4];
+        for (int kk = 0; kk < KS; ++kk) wf[kk] = load8(wt + (ot * 16 + r) * K + kk * 32 + 8 * kq);
+        const float4 bb = *reinterpret_cast<const float4*>(bias + ot * 16 + 4 * kq);
 #pragma unroll
-        for (int tm = 0; tm < 4; ++tm) {
-            acc[tm] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int st = 0; st < 4; ++st) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kk = 0; kk < KS; ++kk)
-                acc[tm] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[tm][kk], bf[kk], acc[tm], 0, 0, 0);
+            for (int kk = 0; kk < KS; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[kk], xf[st][kk], acc, 0, 0, 0);
+            const float v0 = fmaxf(acc[0] + bb.x, 0.f), v1 = fmaxf(acc[1] + bb.y, 0.f);
+            const float v2 = fmaxf(acc[2] + bb.z, 0.f), v3 = fmaxf(acc[3] + bb.w, 0.f);
+            uint2 pk;
+            pk.x = pack_bf16x2(v0, v1);
+            pk.y = pack_bf16x2(v2, v3);
+            *reinterpret_cast<uint2*>(buf + (st * 16 + r) * kLdsRow + ot * 16 + 4 * kq) = pk;
         }
-        const int col = tn * 16 + r;
-        const float bb = bias[col];
-#pragma unroll
-        for (int tm = 0; tm < 4; ++tm)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float v = acc[tm][i] + bb;
-                buf[(tm * 16 + 4 * kq + i) * kLdsRow + col] = float_to_bf16(v > 0.f ? v : 0.f);
-            }
     }
     __syncthreads();
+}
+
+// Last hidden layer and the output unit: logit of waypoint st * 16 + (lane & 15)
+// in out[st] (all lanes), H3 rounded to BF16 as the training forward does.
+__device__ __forceinline__ void wave_layer_logit(const uint16_t* __restrict__ buf, const uint16_t* __restrict__ wt,
+                                                 const float* __restrict__ bias, const uint16_t* __restrict__ w4,
+                                                 int lane, float out[4]) {
+    constexpr int KS = HID / 32;
+    const int r = lane & 15, kq = lane >> 4;
+    bf16x8 xf[4][KS];
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) xf[st][kk] = load8(buf + (st * 16 + r) * kLdsRow + kk * 32 + 8 * kq);
+    float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int ot = 0; ot < HID / 16; ++ot) {
+        bf16x8 wf[KS];
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) wf[kk] = load8(wt + (ot * 16 + r) * HID + kk * 32 + 8 * kq);
+        const float4 bb = *reinterpret_cast<const float4*>(bias + ot * 16 + 4 * kq);
+        const uint2 wq = *reinterpret_cast<const uint2*>(w4 + ot * 16 + 4 * kq);
+        const float w0 = bf16_to_float((uint16_t)(wq.x & 0xffffu)), w1 = bf16_to_float((uint16_t)(wq.x >> 16));
+        const float w2 = bf16_to_float((uint16_t)(wq.y & 0xffffu)), w3 = bf16_to_float((uint16_t)(wq.y >> 16));
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < KS; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[kk], xf[st][kk], acc, 0, 0, 0);
+            const float h0 = (float)(__bf16)fmaxf(acc[0] + bb.x, 0.f), h1 = (float)(__bf16)fmaxf(acc[1] + bb.y, 0.f);
+            const float h2 = (float)(__bf16)fmaxf(acc[2] + bb.z, 0.f), h3 = (float)(__bf16)fmaxf(acc[3] + bb.w, 0.f);
+            part[st] += ((h0 * w0 + h1 * w1) + h2 * w2) + h3 * w3;
+        }
+    }
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        float v = part[st];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        out[st] = v;
+    }
+}
+
+// FP32 features of a waypoint given as FP32 coordinates (see arm_features).
+template <int D>
+__device__ __forceinline__ void arm_features_f(const float* q, const double* lengths, float inv_total,
+                                               uint16_t* out) {
+    float x = 0.f, y = 0.f, ang = 0.f;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        ang += q[i];
+        float s, c;
+        __sincosf(ang, &s, &c);
+        x += (float)lengths[i] * c;
+        y += (float)lengths[i] * s;
+        if (2 * i + 1 < 40) {
+            out[2 * i] = __builtin_bit_cast(uint16_t, (__bf16)(x * inv_total));
+            out[2 * i + 1] = __builtin_bit_cast(uint16_t, (__bf16)(y * inv_total));
+        }
+        if (i < 12) {
+            out[40 + 2 * i] = __builtin_bit_cast(uint16_t, (__bf16)c);
+            out[40 + 2 * i + 1] = __builtin_bit_cast(uint16_t, (__bf16)s);
+        }
+    }
+#pragma unroll
+    for (int j = 2 * D; j < 40; ++j) out[j] = 0;
+#pragma unroll
+    for (int j = 40 + 2 * (D < 12 ? D : 12); j < 64; ++j) out[j] = 0;
 }
 
 template <int D>
@@ -366,10 +438,19 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
         uint32_t path;
         int depth;
         preorder_path(h, o, &path, &depth);
-        double q[D];
+        // dyadic position of the node's midpoint along the edge
+        float t = 0.f, step = 0.5f;
+#pragma unroll 1
+        for (int k = 0; k < depth; ++k, step *= 0.5f)
+            if ((path >> k) & 1u) t += step;
+        t += step;
+        float q[D];
 #pragma unroll
-        for (int i = 0; i < D; ++i) q[i] = coord_mid(a[i], b[i], path, depth);
-        arm_features<D>(q, lengths, inv_total, f);
+        for (int i = 0; i < D; ++i) {
+            const float ai = (float)a[i], bi = (float)b[i];
+            q[i] = ai + t * (bi - ai);
+        }
+        arm_features_f<D>(q, lengths, inv_total, f);
     } else {
 #pragma unroll
         for (int j = 0; j < IN; ++j) f[j] = 0;
@@ -384,14 +465,18 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
         *reinterpret_cast<uint4*>(buf + lane * kLdsRow + j) = v;
     }
     __syncthreads();
-    wave_layer<IN>(buf, wt, p + OFF_B1, lane);
-    wave_layer<HID>(buf, wt + IN * HID, p + OFF_B2, lane);
-    wave_layer<HID>(buf, wt + IN * HID + HID * HID, p + OFF_B3, lane);
-    if (o < W) {
-        const uint16_t* w4 = wb + IN * HID + 2 * HID * HID;
-        float acc = 0.f;
-        for (int k = 0; k < HID; ++k) acc += bf16_to_float(buf[lane * kLdsRow + k]) * bf16_to_float(w4[k]);
-        scores[e * kScoreStride + o] = acc + p[OFF_B4];
+    wave_layer_t<IN>(buf, wt, p + OFF_B1, lane);
+    wave_layer_t<HID>(buf, wt + IN * HID, p + OFF_B2, lane);
+    float logit[4];
+    wave_layer_logit(buf, wt + IN * HID + HID * HID, p + OFF_B3, wb + IN * HID + 2 * HID * HID, lane, logit);
+    // lanes 0..15 write waypoints st * 16 + lane
+    if (lane < 16) {
+        const float b4 = p[OFF_B4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            const uint32_t oo = c0 + (uint32_t)(st * 16 + lane);
+            if (oo < W) scores[e * kScoreStride + oo] = logit[st] + b4;
+        }
     }
 }
 
