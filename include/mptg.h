@@ -169,6 +169,7 @@ typedef struct {
     int64_t rw_links;        /* rewire link() calls evaluated (speculatively, in parallel) */
     int64_t deferred;        /* rewire links evaluated one at a time because the new node's
                                 cost dropped within its own iteration */
+    int64_t graph_launches;  /* hipGraph launches (mptg_prrt_graph), not in the reference */
 } mptg_prrt_stats;
 
 int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo, const double* h_hi,
@@ -177,6 +178,14 @@ int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo,
 int mptg_prrt_add_start(mptg_ctx* ctx, const double* h_q);
 /* Runs iterations until size >= target_size or max_iters iterations. */
 int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_prrt_stats* out);
+/* Plain PRRT only: run `iterations_per_launch` iterations per hipGraph launch
+   (0, the default: enqueue every iteration and read the counters after each).
+   The device stops itself at the size target or the iteration bound, so the
+   tree, the counters and the statistics equal the step-by-step run's; only
+   the host sees them every `iterations_per_launch` iterations.  Replaces the
+   per-iteration loop of Worker::solve (src/mpt/impl/prrt/prrt.hpp:358-375)
+   when the host has nothing to do between iterations. */
+int mptg_prrt_graph(mptg_ctx* ctx, int iterations_per_launch);
 int mptg_prrt_stats_get(mptg_ctx* ctx, mptg_prrt_stats* out);
 /* ids of goal nodes in discovery order (foundGoal, prrt.hpp:139-146); returns count via *n */
 int mptg_prrt_goals(mptg_ctx* ctx, int32_t* h_ids, int64_t cap, int64_t* n);

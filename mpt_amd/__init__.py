@@ -356,6 +356,11 @@ class Context:
         self._chk(_lib.mptg_prrt_run(self._h, target_size, max_iters, C.byref(st)))
         return st.as_dict()
 
+    def prrt_graph(self, iterations_per_launch: int):
+        """Plain PRRT: iterations per hipGraph launch (0 = one launch sequence
+        per iteration).  Trees and statistics do not depend on it."""
+        self._chk(_lib.mptg_prrt_graph(self._h, iterations_per_launch))
+
     def prrt_stats(self) -> dict:
         st = PrrtStats()
         self._chk(_lib.mptg_prrt_stats_get(self._h, C.byref(st)))
@@ -405,6 +410,10 @@ class PRRT:
 
     def getGoalBias(self) -> float:
         return self._bias
+
+    def setGraphIterations(self, k: int):
+        """Iterations captured per hipGraph launch (0: none)."""
+        self.ctx.prrt_graph(k)
 
     def addStart(self, q):
         self.ctx.prrt_add_start(q)

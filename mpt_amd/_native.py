@@ -36,7 +36,7 @@ class PrrtStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "iterations", "samples", "zero_dist", "invalid_states", "link_calls", "link_ok",
         "valid_calls", "goals", "size", "rewire_tests", "rewires", "parent_changes", "cp_links", "rw_links",
-        "deferred")]
+        "deferred", "graph_launches")]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -77,6 +77,7 @@ _SIGNATURES = [
     ("mptg_prrt_add_start", C.c_int, [C.c_void_p, c_double_p]),
     ("mptg_prrt_run", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(PrrtStats)]),
     ("mptg_prrt_stats_get", C.c_int, [C.c_void_p, C.POINTER(PrrtStats)]),
+    ("mptg_prrt_graph", C.c_int, [C.c_void_p, C.c_int]),
     ("mptg_prrt_goals", C.c_int, [C.c_void_p, c_int32_p, C.c_int64, c_int64_p]),
     ("mptg_prrt_star_init", C.c_int, [C.c_void_p, C.POINTER(PrrtParams), C.POINTER(PrrtStarParams), c_double_p,
                                       c_double_p, c_double_p]),

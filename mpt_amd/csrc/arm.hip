@@ -313,7 +313,7 @@ int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t
     if ((rc = ensure(ctx, ctx->scratch[10], sizeof(int64_t) * n + 256))) return rc;
     uint32_t* ncont = (uint32_t*)ctx->scratch[10].p;
     int64_t* cont = (int64_t*)((char*)ctx->scratch[10].p + 256);
-    MPTG_HIP(ctx, hipMemsetAsync(ncont, 0, sizeof(uint32_t), ctx->stream));
+    if ((rc = launch_zero_u32(ctx, ncont, 1))) return rc;
     const uint16_t* ranked = nullptr;
     if (scores) {
         if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kScoreStride * n))) return rc;

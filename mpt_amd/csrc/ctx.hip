@@ -12,14 +12,30 @@ thread_local std::string g_create_err;
 void prrt_free(mptg_ctx* ctx);
 void surrogate_free(mptg_ctx* ctx);
 
+namespace {
+__global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+}  // namespace
+
+int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n) {
+    if (n <= 0) return MPTG_OK;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
 int ensure(mptg_ctx* ctx, DevBuf& b, size_t bytes) {
     if (b.bytes >= bytes && b.p) return MPTG_OK;
+    if (ctx->capturing) return fail(ctx, MPTG_E_STATE, "device buffer would grow during graph capture");
     if (b.p) {
         MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         MPTG_HIP(ctx, hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
     }
+    ++ctx->epoch;
     size_t want = std::max<size_t>(bytes, 256);
     MPTG_HIP(ctx, hipMalloc(&b.p, want));
     b.bytes = want;
@@ -42,6 +58,7 @@ int ensure_pinned(mptg_ctx* ctx, size_t bytes) {
 
 int check_device_ready(mptg_ctx* ctx) {
     if (!ctx) return fail(nullptr, MPTG_E_ARG, "null context");
+    ++ctx->epoch;  // any entry point may change what a captured graph holds
     MPTG_HIP(ctx, hipSetDevice(ctx->device));
     return MPTG_OK;
 }
@@ -62,6 +79,7 @@ static void free_tree(mptg_ctx* ctx) {
 
 static int grow_tree(mptg_ctx* ctx, int64_t cap) {
     if (cap <= ctx->cap) return MPTG_OK;
+    ++ctx->epoch;
     int d = ctx->dim, dp = pitch32(d);
     double* x64 = nullptr;
     float* x32 = nullptr;

@@ -54,6 +54,15 @@ struct PrrtState {
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
     int64_t iteration = 0;       // next iteration index
     int64_t goals = 0;           // goal nodes found so far
+    // iterations per graph launch (0: one launch sequence per iteration)
+    int32_t graph_iters = 0;
+    // the captured iterations, kept across runs while nothing else ran:
+    // valid at context epoch g_epoch for g_iters iterations, tree capacity
+    // g_cap and a kNN split planned for g_plan nodes
+    hipGraphExec_t g_exec = nullptr;
+    uint64_t g_epoch = 0;
+    int32_t g_iters = 0;
+    int64_t g_cap = 0, g_plan = 0;
     mptg_prrt_stats stats{};
     std::vector<int32_t> goal_ids;
     // PRRT*
@@ -108,6 +117,11 @@ struct mptg_ctx {
     // per-phase timing of one arm link() call (mptg_link_phases): events
     // recorded between its launches when `phase_timing` is set
     bool phase_timing = false;
+    // a stream capture is open: no allocation, synchronisation or memset
+    bool capturing = false;
+    // bumped by every entry point and every (re)allocation: a captured
+    // graph is replayed only while the epoch it was captured at holds
+    uint64_t epoch = 0;
     hipEvent_t phase_ev[6] = {};
 };
 
@@ -142,8 +156,15 @@ int ensure_pinned(mptg_ctx* ctx, size_t bytes);
 int check_device_ready(mptg_ctx* ctx);
 
 // kernels launched from other translation units
+// n_dev: tree size read on the device (tree_n then only plans the split);
+// skip: device flag that turns the launch into a no-op
+// Allocates the kNN partial-result buffers for nq queries over a tree of
+// tree_n nodes (what launch_knn would allocate).
+int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n);
+// Zeroes n 32-bit words with a kernel (graph-capturable, unlike a memset).
+int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n);
 int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
-               double* d_dist, int64_t tree_n);
+               double* d_dist, int64_t tree_n, const int64_t* n_dev = nullptr, const int32_t* skip = nullptr);
 int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                 int known_valid, uint8_t* d_verdict, uint32_t* d_checks);
 int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,

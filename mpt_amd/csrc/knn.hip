@@ -114,12 +114,25 @@ __device__ __forceinline__ float prefilter_threshold(double dk, double S) {
     return (double)f < thr ? float_of(bits_of(f) + 1u) : f;
 }
 
+// Tree size: n, or *n_dev when given (launches captured in a graph read the
+// size the previous iteration left); *skip != 0 makes the launch a no-op.
+__device__ __forceinline__ bool knn_extent(const int64_t* n_dev, const int32_t* skip, int64_t* n, int64_t* chunk) {
+    if (skip && *skip) return false;
+    if (n_dev) {
+        *n = *n_dev;
+        *chunk = (*n + gridDim.y - 1) / gridDim.y;
+    }
+    return true;
+}
+
 template <int D, int METRIC, int K>
 __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32, const double* __restrict__ x64,
                                                    int64_t n, int64_t chunk, const double* __restrict__ q,
                                                    int64_t nq, double radius, double pbound,
-                                                   double* __restrict__ out_d, int32_t* __restrict__ out_i) {
+                                                   double* __restrict__ out_d, int32_t* __restrict__ out_i,
+                                                   const int64_t* n_dev, const int32_t* skip) {
     constexpr int DP = (D <= 2) ? 2 : (D <= 4 ? 4 : (D <= 8 ? 8 : ((D + 3) / 4) * 4));
+    if (!knn_extent(n_dev, skip, &n, &chunk)) return;
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = qi < nq;
     const int64_t qc = active ? qi : 0;
@@ -208,14 +221,15 @@ template <int K>
 __global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict__ x64, int64_t n, int64_t chunk,
                                                        const double* __restrict__ q, int64_t nq, double radius,
                                                        double w_so3, double w_l2, double* __restrict__ out_d,
-                                                       int32_t* __restrict__ out_i);
+                                                       int32_t* __restrict__ out_i, const int64_t* n_dev,
+                                                       const int32_t* skip);
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                     int splits, int64_t nq, int k, double* __restrict__ out_d,
-                                                    int32_t* __restrict__ out_i) {
+                                                    int32_t* __restrict__ out_i, const int32_t* skip) {
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (qi >= nq) return;
+    if (qi >= nq || (skip && *skip)) return;
     TopK<K> top;
     top.init();
     for (int s = 0; s < splits; ++s) {
@@ -240,9 +254,9 @@ __global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ p
 template <int K>
 __global__ __launch_bounds__(kBlock) void knn_finish(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                      int64_t nq, int k, double* __restrict__ out_d,
-                                                     int32_t* __restrict__ out_i) {
+                                                     int32_t* __restrict__ out_i, const int32_t* skip) {
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (qi >= nq) return;
+    if (qi >= nq || (skip && *skip)) return;
     for (int j = 0; j < k; ++j) {
         int32_t id = pi[qi * K + j];
         bool have = id != INT_MAX;
@@ -276,7 +290,9 @@ template <int K>
 __global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict__ x64, int64_t n, int64_t chunk,
                                                        const double* __restrict__ q, int64_t nq, double radius,
                                                        double w_so3, double w_l2, double* __restrict__ out_d,
-                                                       int32_t* __restrict__ out_i) {
+                                                       int32_t* __restrict__ out_i, const int64_t* n_dev,
+                                                       const int32_t* skip) {
+    if (!knn_extent(n_dev, skip, &n, &chunk)) return;
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (qi >= nq) return;
     const int64_t p0 = (int64_t)blockIdx.y * chunk;
@@ -297,42 +313,48 @@ __global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict_
     }
 }
 
+struct ScanExtent {
+    int64_t n, chunk;
+    const int64_t* n_dev;
+    const int32_t* skip;
+};
+
 template <int D, int METRIC, int K>
-void launch_scan(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq, double radius,
+void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q, int64_t nq, double radius,
                  double* pd, int32_t* pi) {
-    hipLaunchKernelGGL((knn_scan<D, METRIC, K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, n, chunk,
-                       q, nq, radius, ctx->coord_bound, pd, pi);
+    hipLaunchKernelGGL((knn_scan<D, METRIC, K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n, x.chunk,
+                       q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip);
 }
 
 template <int D, int K>
-bool dispatch_metric(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq,
-                     double radius, double* pd, int32_t* pi) {
-    if (ctx->metric == MPTG_METRIC_L1) launch_scan<D, MPTG_METRIC_L1, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-    else launch_scan<D, MPTG_METRIC_L2, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+bool dispatch_metric(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q, int64_t nq, double radius,
+                     double* pd, int32_t* pi) {
+    if (ctx->metric == MPTG_METRIC_L1) launch_scan<D, MPTG_METRIC_L1, K>(ctx, grid, x, q, nq, radius, pd, pi);
+    else launch_scan<D, MPTG_METRIC_L2, K>(ctx, grid, x, q, nq, radius, pd, pi);
     return true;
 }
 
 template <int K>
-bool dispatch_dim(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq, double radius,
+bool dispatch_dim(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q, int64_t nq, double radius,
                   double* pd, int32_t* pi) {
     if (ctx->metric == MPTG_METRIC_SE3) {
-        hipLaunchKernelGGL((knn_scan_se3<K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x64, n, chunk, q, nq,
-                           radius, ctx->w_so3, ctx->w_l2, pd, pi);
+        hipLaunchKernelGGL((knn_scan_se3<K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, x.chunk, q, nq,
+                           radius, ctx->w_so3, ctx->w_l2, pd, pi, x.n_dev, x.skip);
         return true;
     }
     switch (ctx->dim) {
-        case 1: return dispatch_metric<1, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 2: return dispatch_metric<2, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 3: return dispatch_metric<3, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 4: return dispatch_metric<4, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 5: return dispatch_metric<5, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 6: return dispatch_metric<6, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 7: return dispatch_metric<7, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 8: return dispatch_metric<8, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 10: return dispatch_metric<10, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 12: return dispatch_metric<12, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 16: return dispatch_metric<16, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
-        case 20: return dispatch_metric<20, K>(ctx, grid, n, chunk, q, nq, radius, pd, pi);
+        case 1: return dispatch_metric<1, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 2: return dispatch_metric<2, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 3: return dispatch_metric<3, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 4: return dispatch_metric<4, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 5: return dispatch_metric<5, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 6: return dispatch_metric<6, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 7: return dispatch_metric<7, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 8: return dispatch_metric<8, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 10: return dispatch_metric<10, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 12: return dispatch_metric<12, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 16: return dispatch_metric<16, K>(ctx, grid, x, q, nq, radius, pd, pi);
+        case 20: return dispatch_metric<20, K>(ctx, grid, x, q, nq, radius, pd, pi);
         default: return false;
     }
 }
@@ -352,16 +374,31 @@ static int64_t choose_splits(mptg_ctx* ctx, int64_t nq, int64_t n) {
     return s;
 }
 
+static int64_t plan_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int64_t* chunk) {
+    int64_t splits = n > 0 ? choose_splits(ctx, nq, n) : 1;
+    *chunk = n > 0 ? (n + splits - 1) / splits : 0;
+    return n > 0 ? (n + *chunk - 1) / *chunk : 1;
+}
+
+int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
+    const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
+    int64_t chunk;
+    const int64_t splits = plan_splits(ctx, nq, tree_n, &chunk);
+    int rc;
+    if ((rc = ensure(ctx, ctx->scratch[4], sizeof(double) * K * nq * splits))) return rc;
+    return ensure(ctx, ctx->scratch[5], sizeof(int32_t) * K * nq * splits);
+}
+
 int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
-               double* d_dist, int64_t tree_n) {
+               double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip) {
     if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
     if (nq > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 queries");
     if (!(radius >= 0)) return fail(ctx, MPTG_E_ARG, "radius must be >= 0");
     const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
-    int64_t n = tree_n;
-    int64_t splits = n > 0 ? choose_splits(ctx, nq, n) : 1;
-    int64_t chunk = n > 0 ? (n + splits - 1) / splits : 0;
-    if (n > 0) splits = (n + chunk - 1) / chunk;
+    // with n_dev, tree_n is only the size the split count is planned for
+    int64_t n = tree_n, chunk;
+    const int64_t splits = plan_splits(ctx, nq, n, &chunk);
+    const ScanExtent ext{n, chunk, n_dev, skip};
     size_t pbytes_d = sizeof(double) * K * nq * splits, pbytes_i = sizeof(int32_t) * K * nq * splits;
     int rc;
     if ((rc = ensure(ctx, ctx->scratch[4], pbytes_d))) return rc;
@@ -370,20 +407,20 @@ int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radiu
     int32_t* pi = (int32_t*)ctx->scratch[5].p;
     dim3 grid((unsigned)((nq + kBlock - 1) / kBlock), (unsigned)splits);
     bool ok;
-    if (K == 1) ok = dispatch_dim<1>(ctx, grid, n, chunk, d_q, nq, radius, pd, pi);
-    else if (K == 20) ok = dispatch_dim<20>(ctx, grid, n, chunk, d_q, nq, radius, pd, pi);
-    else ok = dispatch_dim<32>(ctx, grid, n, chunk, d_q, nq, radius, pd, pi);
+    if (K == 1) ok = dispatch_dim<1>(ctx, grid, ext, d_q, nq, radius, pd, pi);
+    else if (K == 20) ok = dispatch_dim<20>(ctx, grid, ext, d_q, nq, radius, pd, pi);
+    else ok = dispatch_dim<32>(ctx, grid, ext, d_q, nq, radius, pd, pi);
     if (!ok) return fail(ctx, MPTG_E_ARG, "unsupported dimension for the nearest-neighbour kernel");
     MPTG_HIP(ctx, hipGetLastError());
     dim3 g1((unsigned)((nq + kBlock - 1) / kBlock));
     if (splits > 1) {
-        if (K == 1) hipLaunchKernelGGL((knn_merge<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx);
-        else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx);
-        else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx);
+        if (K == 1) hipLaunchKernelGGL((knn_merge<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip);
+        else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip);
+        else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip);
     } else {
-        if (K == 1) hipLaunchKernelGGL((knn_finish<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx);
-        else if (K == 20) hipLaunchKernelGGL((knn_finish<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx);
-        else hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx);
+        if (K == 1) hipLaunchKernelGGL((knn_finish<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);
+        else if (K == 20) hipLaunchKernelGGL((knn_finish<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);
+        else hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);
     }
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;

@@ -37,6 +37,7 @@ __global__ __launch_bounds__(1024) void prrt_append(SpaceArgs sp, PrrtDev pd, do
     __shared__ int64_t zero_s, inv_s, lc_s, ok_s, vc_s;
     const int D = sp.dim;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    if (cnt->done) return;  // an iteration past the stop (block-uniform)
     if (t == 0) {
         base_s = cnt->size;
         gbase_s = cnt->goals;
@@ -119,6 +120,8 @@ __global__ __launch_bounds__(1024) void prrt_append(SpaceArgs sp, PrrtDev pd, do
         cnt->link_calls += lc_s;
         cnt->link_ok += ok_s;
         cnt->valid_calls += vc_s;
+        cnt->iteration += 1;
+        cnt->done = (base_s >= cnt->target || cnt->iteration >= cnt->iter_end) ? 1 : 0;
     }
 }
 
@@ -144,7 +147,14 @@ int prrt_reserve(mptg_ctx* ctx, int B) {
     return MPTG_OK;
 }
 
+void prrt_graph_drop(mptg_ctx* ctx) {
+    PrrtState& s = ctx->prrt;
+    if (s.g_exec) (void)hipGraphExecDestroy(s.g_exec);
+    s.g_exec = nullptr;
+}
+
 void prrt_free(mptg_ctx* ctx) {
+    prrt_graph_drop(ctx);
     PrrtBufs& pb = bufs(ctx);
     DevBuf* all[] = {&pb.q,       &pb.near,    &pb.ndist,  &pb.from,    &pb.to,       &pb.status,  &pb.mask,
                      &pb.verdict, &pb.checks,  &pb.counters, &pb.goal_buf, &pb.dnear, &pb.sidx,   &pb.qs,
@@ -160,6 +170,103 @@ void prrt_free(mptg_ctx* ctx) {
 
 int tree_reserve(mptg_ctx* ctx, int64_t cap);
 int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters);
+
+// Writes the stop condition of one run into the device counters: the next
+// iteration index, the size target and the iteration bound; clears `done`.
+int counters_arm(mptg_ctx* ctx, void* dcnt, int64_t iteration, int64_t target, int64_t max_iters) {
+    struct Tail {
+        int64_t iteration, target, iter_end;
+        int32_t done, pad;
+    } t;
+    static_assert(sizeof(Tail) == sizeof(Counters) - offsetof(Counters, iteration), "counter tail layout");
+    t.iteration = iteration;
+    t.target = target;
+    t.iter_end = max_iters > INT64_MAX - iteration ? INT64_MAX : iteration + max_iters;
+    t.done = 0;
+    t.pad = 0;
+    MPTG_HIP(ctx, hipMemcpyAsync((char*)dcnt + offsetof(Counters, iteration), &t, sizeof(t),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MPTG_OK;
+}
+
+namespace {
+
+// Enqueues one PRRT iteration.  it >= 0: eager (host-known iteration index
+// and tree size); it < 0: graph body (both read from the device counters,
+// the kNN split planned for plan_n nodes).
+int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan_n) {
+    PrrtState& s = ctx->prrt;
+    PrrtBufs& pb = bufs(ctx);
+    const int B = s.p.batch, D = ctx->dim;
+    Counters* dcnt = (Counters*)pb.counters.p;
+    const bool dev = it < 0;
+    const unsigned gb = (unsigned)((B + kBlock - 1) / kBlock);
+    int rc;
+    hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, ctx->metric, D, s.p.seed,
+                       s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)pb.q.p);
+    MPTG_HIP(ctx, hipGetLastError());
+    if ((rc = launch_knn(ctx, (const double*)pb.q.p, B, 1, __builtin_inf(), (int32_t*)pb.near.p,
+                         (double*)pb.ndist.p, dev ? plan_n : ctx->size, dev ? &dcnt->size : nullptr,
+                         dev ? &dcnt->done : nullptr)))
+        return rc;
+    dispatch_dim(D, [&]<int DD>() {
+        hipLaunchKernelGGL((prrt_steer<DD>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx),
+                           scene_args(ctx), s.p.range, B, (const double*)ctx->x64, (const double*)pb.q.p,
+                           (const int32_t*)pb.near.p, (const double*)pb.ndist.p, (double*)pb.from.p,
+                           (double*)pb.to.p, (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p, (double*)nullptr,
+                           (const Counters*)dcnt);
+    });
+    MPTG_HIP(ctx, hipGetLastError());
+    if (ctx->scene.kind == MPTG_SCENE_MESH &&
+        (rc = launch_mesh_filter(ctx, (const double*)pb.to.p, B, (uint8_t*)pb.mask.p, (uint8_t*)pb.status.p,
+                                 ST_INVALID)))
+        return rc;
+    if ((rc = launch_link_masked(ctx, (const double*)pb.from.p, (const double*)pb.to.p, B, s.p.order, 1,
+                                 (const uint8_t*)pb.mask.p, (uint8_t*)pb.verdict.p, (uint32_t*)pb.checks.p)))
+        return rc;
+    hipLaunchKernelGGL(prrt_append, dim3(1), dim3(1024), 0, ctx->stream, space_args(ctx), pd, s.p.goal_radius, B,
+                       pitch32(D), (const double*)pb.to.p, (const int32_t*)pb.near.p, (uint8_t*)pb.status.p,
+                       (const uint8_t*)pb.verdict.p, (const uint32_t*)pb.checks.p, ctx->x64, ctx->x32,
+                       ctx->parent, dcnt, (int32_t*)pb.goal_buf.p, (int64_t)4096);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+// Captures `iters` iterations into an executable graph.  Everything the
+// body allocates must already be allocated: the capture flag turns any
+// growth into an error instead of a synchronising reallocation.
+int capture_iterations(mptg_ctx* ctx, const PrrtDev& pd, int iters, int64_t plan_n, hipGraphExec_t* out) {
+    *out = nullptr;
+    const bool timing = ctx->phase_timing;
+    ctx->phase_timing = false;
+    ctx->capturing = true;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+    int rc = MPTG_OK;
+    if (e != hipSuccess) {
+        ctx->capturing = false;
+        ctx->phase_timing = timing;
+        return fail(ctx, MPTG_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    for (int k = 0; k < iters && rc == MPTG_OK; ++k) rc = enqueue_iteration(ctx, pd, -1, plan_n);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->capturing = false;
+    ctx->phase_timing = timing;
+    if (rc == MPTG_OK && e != hipSuccess)
+        rc = fail(ctx, MPTG_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (rc == MPTG_OK) {
+        e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            *out = nullptr;
+            rc = fail(ctx, MPTG_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+}  // namespace
 
 }  // namespace mptg
 
@@ -226,9 +333,12 @@ int mptg_prrt_add_start(mptg_ctx* ctx, const double* h_q) {
 }
 
 int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_prrt_stats* out) {
+    const uint64_t entry_epoch = ctx ? ctx->epoch : 0;
     int rc = check_device_ready(ctx);
     if (rc) return rc;
     PrrtState& s = ctx->prrt;
+    // the graph of the previous run stays valid if no entry point ran since
+    if (s.g_exec && s.g_epoch == entry_epoch) s.g_epoch = ctx->epoch;
     if (!s.init) return fail(ctx, MPTG_E_STATE, "mptg_prrt_init must be called first");
     if (ctx->size == 0) return fail(ctx, MPTG_E_EMPTY, "there are no valid initial states");
     if (target_size > (int64_t)INT32_MAX) return fail(ctx, MPTG_E_RANGE, "target size must be < 2^31");
@@ -250,6 +360,11 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
         s.counters_valid = true;
         s.stats_base = s.stats;
     }
+    if (max_iters <= 0 || ctx->size >= target_size) {
+        if (out) *out = s.stats;
+        return MPTG_OK;
+    }
+    if ((rc = counters_arm(ctx, dcnt, s.iteration, target_size, max_iters))) return rc;
     if ((rc = ensure_pinned(ctx, sizeof(Counters)))) return rc;
     Counters* hc = (Counters*)ctx->pinned;
     PrrtDev pd;
@@ -258,46 +373,67 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
         pd.hi[i] = s.hi[i];
         pd.goal[i] = s.goal[i];
     }
-    const unsigned gb = (unsigned)((B + kBlock - 1) / kBlock);
-    for (int64_t k = 0; k < max_iters && ctx->size < target_size; ++k) {
-        if ((rc = tree_reserve(ctx, ctx->size + B))) return rc;
-        const int64_t it = s.iteration;
-        hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, ctx->metric, D, s.p.seed, s.p.goal_bias, it,
-                           B, (const Counters*)dcnt, (double*)pb.q.p);
-        MPTG_HIP(ctx, hipGetLastError());
-        if ((rc = launch_knn(ctx, (const double*)pb.q.p, B, 1, __builtin_inf(), (int32_t*)pb.near.p,
-                             (double*)pb.ndist.p, ctx->size)))
-            return rc;
-        dispatch_dim(D, [&]<int DD>() {
-            hipLaunchKernelGGL((prrt_steer<DD>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx),
-                               scene_args(ctx), s.p.range, B, (const double*)ctx->x64, (const double*)pb.q.p,
-                               (const int32_t*)pb.near.p, (const double*)pb.ndist.p, (double*)pb.from.p,
-                               (double*)pb.to.p, (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p,
-                               (double*)nullptr);
-        });
-        MPTG_HIP(ctx, hipGetLastError());
-        if (ctx->scene.kind == MPTG_SCENE_MESH &&
-            (rc = launch_mesh_filter(ctx, (const double*)pb.to.p, B, (uint8_t*)pb.mask.p, (uint8_t*)pb.status.p,
-                                     ST_INVALID)))
-            return rc;
-        if ((rc = launch_link_masked(ctx, (const double*)pb.from.p, (const double*)pb.to.p, B, s.p.order, 1,
-                                     (const uint8_t*)pb.mask.p, (uint8_t*)pb.verdict.p, (uint32_t*)pb.checks.p)))
-            return rc;
-        hipLaunchKernelGGL(prrt_append, dim3(1), dim3(1024), 0, ctx->stream, space_args(ctx), pd, s.p.goal_radius, B,
-                           pitch32(D), (const double*)pb.to.p, (const int32_t*)pb.near.p, (uint8_t*)pb.status.p,
-                           (const uint8_t*)pb.verdict.p, (const uint32_t*)pb.checks.p, ctx->x64, ctx->x32,
-                           ctx->parent, dcnt, (int32_t*)pb.goal_buf.p, (int64_t)4096);
-        MPTG_HIP(ctx, hipGetLastError());
-        MPTG_HIP(ctx, hipMemcpyAsync(hc, dcnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
-        MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // graph mode: `graph_iters` iterations per launch, the device stopping
+    // itself at the target or the iteration bound.  A graph holds tree and
+    // scratch addresses and by-value arguments, so it is replayed only while
+    // the context epoch it was captured at holds (no entry point, no
+    // allocation since), for the same tree capacity and iteration count.
+    // Before a capture one step-by-step iteration sizes every buffer.
+    const int G = s.graph_iters;
+    int64_t k = 0;
+    bool warm = false;
+    while (k < max_iters && ctx->size < target_size) {
+        int64_t ran = 1;
+        const int64_t span = (int64_t)G * B;
+        bool replay = G > 0 && (warm || (s.g_exec && s.g_epoch == ctx->epoch && s.g_iters == G));
+        if (replay && (rc = tree_reserve(ctx, ctx->size + span))) break;
+        if (!replay) {
+            if ((rc = tree_reserve(ctx, ctx->size + B))) break;
+            if ((rc = enqueue_iteration(ctx, pd, s.iteration, 0))) break;
+            warm = G > 0;
+        } else {
+            if (!s.g_exec || s.g_epoch != ctx->epoch || s.g_iters != G || s.g_cap != ctx->cap ||
+                ctx->size + span > s.g_plan) {
+                prrt_graph_drop(ctx);
+                const int64_t plan = std::min<int64_t>(ctx->cap, 2 * (ctx->size + span));
+                if ((rc = knn_reserve(ctx, B, 1, plan))) break;
+                if ((rc = capture_iterations(ctx, pd, G, plan, &s.g_exec))) break;
+                s.g_epoch = ctx->epoch;
+                s.g_iters = G;
+                s.g_cap = ctx->cap;
+                s.g_plan = plan;
+            }
+            hipError_t e = hipGraphLaunch(s.g_exec, ctx->stream);
+            if (e != hipSuccess) {
+                rc = fail(ctx, MPTG_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+                break;
+            }
+            s.stats.graph_launches++;
+            ran = -1;  // read back below
+        }
+        if (hipMemcpyAsync(hc, dcnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = fail(ctx, MPTG_E_HIP, "reading the planner counters failed");
+            break;
+        }
+        if (ran < 0) ran = hc->iteration - s.iteration;
+        if (ran <= 0) {
+            rc = fail(ctx, MPTG_E_STATE, "a graph launch ran no iteration");
+            break;
+        }
         ctx->size = hc->size;
-        s.iteration++;
-        s.stats.iterations++;
-        s.stats.samples += B;
+        s.iteration += ran;
+        k += ran;
+        s.stats.iterations += ran;
+        s.stats.samples += ran * B;
         if (hc->goals > s.goals) {
             int64_t ng = std::min<int64_t>(hc->goals, 4096);
             s.goal_ids.resize((size_t)ng);
-            MPTG_HIP(ctx, hipMemcpy(s.goal_ids.data(), pb.goal_buf.p, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
+            if (hipMemcpy(s.goal_ids.data(), pb.goal_buf.p, sizeof(int32_t) * ng, hipMemcpyDeviceToHost) !=
+                hipSuccess) {
+                rc = fail(ctx, MPTG_E_HIP, "reading the goal ids failed");
+                break;
+            }
             s.goals = hc->goals;
         }
         s.stats.zero_dist = s.stats_base.zero_dist + hc->zero;
@@ -308,7 +444,20 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
         s.stats.goals = s.goals;
         s.stats.size = ctx->size;
     }
+    if (rc) {
+        prrt_graph_drop(ctx);
+        return rc;
+    }
     if (out) *out = s.stats;
+    return MPTG_OK;
+}
+
+int mptg_prrt_graph(mptg_ctx* ctx, int iterations_per_launch) {
+    if (!ctx) return fail(ctx, MPTG_E_ARG, "null context");
+    if (iterations_per_launch < 0 || iterations_per_launch > 1024)
+        return fail(ctx, MPTG_E_RANGE, "iterations per graph launch must be in 0..1024");
+    ctx->prrt.graph_iters = iterations_per_launch;
+    if (iterations_per_launch == 0) prrt_graph_drop(ctx);
     return MPTG_OK;
 }
 

@@ -12,21 +12,29 @@ constexpr int kBlock = 256;
 
 enum : uint8_t { ST_ZERO = 0, ST_INVALID = 1, ST_LINK_FAIL = 2, ST_OK = 3 };
 
+// Planner counters, resident on the device.  `iteration` is the index of the
+// next iteration; `done` turns every planner kernel of an iteration into a
+// no-op once size >= target or iteration >= iter_end, so a captured graph of
+// several iterations never runs past where the step-by-step loop stops.
 struct Counters {
     int64_t size;
     int64_t goals;
     int64_t zero, invalid, link_calls, link_ok, valid_calls;
+    int64_t iteration, target, iter_end;
+    int32_t done, pad;
 };
 
 struct PrrtDev {
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
 };
 
+// it < 0: the iteration index is the device counter's (graph replays).
 __global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int metric, int D, uint64_t seed, double goal_bias,
                                                       int64_t it, int B, const Counters* __restrict__ cnt,
                                                       double* __restrict__ q) {
     int b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= B) return;
+    if (b >= B || cnt->done) return;
+    if (it < 0) it = cnt->iteration;
     uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
     bool bias = cnt->goals == 0 && goal_bias > 0;
     bool useGoal = bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias;
@@ -45,11 +53,16 @@ __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc,
                                                      const int32_t* __restrict__ near,
                                                      const double* ndist, double* __restrict__ from,
                                                      double* __restrict__ to, uint8_t* __restrict__ status,
-                                                     uint8_t* __restrict__ linkmask, double* dnear) {
+                                                     uint8_t* __restrict__ linkmask, double* dnear,
+                                                     const Counters* __restrict__ cnt) {
     constexpr int DA = D > 0 ? D : kMaxDim;
     const int dd = D > 0 ? D : sp.dim;
     int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= B) return;
+    if (cnt->done) {  // nothing to link in an iteration past the stop
+        linkmask[b] = 0;
+        return;
+    }
     double d = ndist[b];
     int32_t nn = near[b];
     linkmask[b] = 0;

@@ -597,7 +597,8 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
             hipLaunchKernelGGL((prrt_steer<DD>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx),
                                scene_args(ctx), s.p.range, B, (const double*)ctx->x64, (const double*)s.q.p,
                                (const int32_t*)s.near.p, (const double*)s.ndist.p, (double*)s.from.p,
-                               (double*)s.to.p, (uint8_t*)s.status.p, (uint8_t*)s.mask.p, (double*)s.dnear.p);
+                               (double*)s.to.p, (uint8_t*)s.status.p, (uint8_t*)s.mask.p, (double*)s.dnear.p,
+                               (const Counters*)dcnt);
         });
         MPTG_HIP(ctx, hipGetLastError());
         if (ctx->scene.kind == MPTG_SCENE_MESH &&

@@ -435,7 +435,7 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
             if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
             uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
             int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
-            MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
+            if ((rc = launch_zero_u32(ctx, nirr, 1))) return rc;
             const float* scores = nullptr;
             phase_mark(ctx, 0);
             if (order == MPTG_ORDER_SURROGATE) {
@@ -471,7 +471,7 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
     uint32_t* long_count = (uint32_t*)ctx->scratch[7].p;
     int64_t* long_list = (int64_t*)((char*)ctx->scratch[7].p + 256);
-    MPTG_HIP(ctx, hipMemsetAsync(long_count, 0, sizeof(uint32_t), ctx->stream));
+    if ((rc = launch_zero_u32(ctx, long_count, 1))) return rc;
     double inv = 1.0 / ctx->link_param;
     unsigned lg = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 16);
     dispatch_dim(ctx->dim, [&]<int D>() {

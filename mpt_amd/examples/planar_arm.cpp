@@ -1,7 +1,8 @@
 // The reference's planar-arm demo (demo/link_manipulator_planning.cpp:57-96)
 // on the GPU planner: same arm, disks, start and goal, driven through
 // mpt_amd::Planner<Scenario, PRRT<gpu_batch<B>>>.  Prints the solution and
-// re-validates every solution edge through the C ABI.
+// re-validates every solution edge through the C ABI; then grows the same
+// tree with iterations captured in hipGraphs and checks that it is the same.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -50,5 +51,17 @@ int main(int argc, char** argv) {
     double gd = 0;
     for (int j = 0; j < 5; ++j) gd += std::fabs(path.back()[j] - goal[j]);
     std::printf("path_valid=1 starts_at_start=%d goal_distance=%.3g\n", at_start ? 1 : 0, gd);
-    return (at_start && gd <= 1e-6) ? 0 : 5;
+    if (!(at_start && gd <= 1e-6)) return 5;
+
+    // the same planner with 4 iterations per graph launch, grown to the same
+    // size: same iterations, same tree, same solution
+    using Graphed = mpt_amd::PRRT<mpt_amd::gpu_batch<1024>, mpt_amd::graph_iterations<4>>;
+    mpt_amd::Planner<Scenario, Graphed> graphed(scenario, seed);
+    graphed.addStart(start);
+    graphed.solveUntilSize(planner.size());
+    const mptg_prrt_stats gs = graphed.stats();
+    const bool same = graphed.size() == planner.size() && graphed.solution() == path &&
+                      gs.iterations == planner.stats().iterations;
+    std::printf("graph_equal=%d graph_launches=%lld\n", same ? 1 : 0, (long long)gs.graph_launches);
+    return same && gs.graph_launches > 0 ? 0 : 6;
 }

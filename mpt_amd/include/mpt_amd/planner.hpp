@@ -55,6 +55,13 @@ template <int O>
 struct check_order {
     static constexpr int value = O;
 };
+// PRRT only: iterations per hipGraph launch; solve() then checks its done
+// condition every G iterations instead of after each one.
+template <int G>
+struct graph_iterations {
+    static_assert(G >= 0 && G <= 1024, "graph_iterations must be in 0..1024");
+    static constexpr int value = G;
+};
 // PRRT* rewiring (src/mpt/planner_tags.hpp:49-51).  k-nearest is the
 // default, with the reference's k = ceil(kRRG ln(n + 1)); rewire_k<K> fixes
 // k (1..32).  The radius form is not implemented on the device.
@@ -74,11 +81,12 @@ inline void check(mptg_ctx* ctx, int rc) {
 
 // Strategy type produced by PRRT<options...>; option order is free, as in
 // the reference's option parser (src/mpt/impl/packs.hpp:105-125).
-template <int Batch, bool Stats, int Order>
+template <int Batch, bool Stats, int Order, int Graph = 0>
 struct PRRTStrategy {
     static constexpr int batch = Batch;
     static constexpr bool stats = Stats;
     static constexpr int order = Order;
+    static constexpr int graph = Graph;
 };
 
 template <typename... Opts>
@@ -88,6 +96,7 @@ struct PRRTOptions<> {
     static constexpr int batch = 8192;
     static constexpr bool stats = false;
     static constexpr int order = MPTG_ORDER_REFERENCE;
+    static constexpr int graph = 0;
 };
 template <int B, typename... Rest>
 struct PRRTOptions<gpu_batch<B>, Rest...> : PRRTOptions<Rest...> {
@@ -103,9 +112,14 @@ template <int O, typename... Rest>
 struct PRRTOptions<check_order<O>, Rest...> : PRRTOptions<Rest...> {
     static constexpr int order = O;
 };
+template <int G, typename... Rest>
+struct PRRTOptions<graph_iterations<G>, Rest...> : PRRTOptions<Rest...> {
+    static constexpr int graph = G;
+};
 
 template <typename... Opts>
-using PRRT = PRRTStrategy<PRRTOptions<Opts...>::batch, PRRTOptions<Opts...>::stats, PRRTOptions<Opts...>::order>;
+using PRRT = PRRTStrategy<PRRTOptions<Opts...>::batch, PRRTOptions<Opts...>::stats, PRRTOptions<Opts...>::order,
+                          PRRTOptions<Opts...>::graph>;
 
 // PRRT* (src/mpt/prrt_star.hpp:55-110): the PRRT options plus rewiring.
 template <int Batch, bool Stats, int Order, int K>
@@ -131,6 +145,10 @@ struct PRRTStarOptions<Opt, Rest...> : PRRTStarOptions<Rest...> {
 };
 template <typename... Rest>
 struct PRRTStarOptions<rewire_k_nearest, Rest...> : PRRTStarOptions<Rest...> {};
+template <int G, typename... Rest>
+struct PRRTStarOptions<graph_iterations<G>, Rest...> : PRRTStarOptions<Rest...> {
+    static_assert(G < 0, "graph_iterations applies to PRRT; PRRT* enqueues its iterations one by one");
+};
 template <typename... Rest>
 struct PRRTStarOptions<rewire_r_nearest, Rest...> : PRRTStarOptions<Rest...> {
     static_assert(sizeof...(Rest) < 0, "rewire_r_nearest is not implemented by the device planner");
@@ -149,9 +167,9 @@ class GpuPRRT;
 
 template <typename Scenario, typename Algorithm>
 struct PlannerResolver;
-template <typename Scenario, int B, bool S, int O>
-struct PlannerResolver<Scenario, PRRTStrategy<B, S, O>> {
-    using type = GpuPRRT<Scenario, PRRTStrategy<B, S, O>>;
+template <typename Scenario, int B, bool S, int O, int G>
+struct PlannerResolver<Scenario, PRRTStrategy<B, S, O, G>> {
+    using type = GpuPRRT<Scenario, PRRTStrategy<B, S, O, G>>;
 };
 template <typename Scenario, int B, bool S, int O, int K>
 struct PlannerResolver<Scenario, PRRTStarStrategy<B, S, O, K>> {
@@ -210,6 +228,7 @@ class GpuPRRT {
             detail::check(ctx_, mptg_prrt_star_init(ctx_, &p, &sp, lo, hi, goal));
         } else {
             detail::check(ctx_, mptg_prrt_init(ctx_, &p, lo, hi, goal));
+            detail::check(ctx_, mptg_prrt_graph(ctx_, Strategy::graph));
         }
         configured_ = true;
     }
@@ -262,11 +281,15 @@ class GpuPRRT {
         detail::check(ctx_, mptg_prrt_add_start(ctx_, d));
     }
 
-    // One batched iteration per doneFn() check (prrt.hpp:194-201).
+    // One batched iteration per doneFn() check (prrt.hpp:194-201); with
+    // graph_iterations<G>, one graph launch of G iterations per check.
     template <typename DoneFn>
     std::enable_if_t<std::is_same_v<bool, std::invoke_result_t<DoneFn>>> solve(DoneFn doneFn) {
         if (size() == 0) throw std::runtime_error("there are no valid initial states");
-        while (!doneFn()) detail::check(ctx_, mptg_prrt_run(ctx_, std::numeric_limits<int32_t>::max(), 1, nullptr));
+        int64_t per_check = 1;
+        if constexpr (!kStar) per_check = Strategy::graph > 0 ? Strategy::graph : 1;
+        while (!doneFn())
+            detail::check(ctx_, mptg_prrt_run(ctx_, std::numeric_limits<int32_t>::max(), per_check, nullptr));
     }
 
     template <typename Duration>

@@ -19,6 +19,10 @@ def test_headers_compile_standalone(tmp_path):
                    "static_assert(P::kDim == 20);\n"
                    "using Q = mpt_amd::PRRT<mpt_amd::gpu_batch<64>, mpt_amd::check_order<MPTG_ORDER_SURROGATE>>;\n"
                    "static_assert(Q::batch == 64 && Q::order == MPTG_ORDER_SURROGATE && !Q::stats);\n"
+                   "using G = mpt_amd::PRRT<mpt_amd::graph_iterations<8>, mpt_amd::gpu_batch<64>>;\n"
+                   "static_assert(G::graph == 8 && G::batch == 64 && mpt_amd::PRRT<>::graph == 0);\n"
+                   "using PG = mpt_amd::Planner<mpt_amd::LinkManipulator<5>, G>;\n"
+                   "static_assert(!PG::kStar);\n"
                    "using S = mpt_amd::PRRTStar<mpt_amd::rewire_k<20>, mpt_amd::gpu_batch<512>,\n"
                    "    mpt_amd::report_stats<true>>;\n"
                    "static_assert(S::k == 20 && S::batch == 512 && S::stats);\n"
@@ -42,6 +46,7 @@ def test_example_plans_on_gpu():
     print(r.stdout[-2000:], flush=True)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "path_valid=1" in r.stdout and "solved=1" in r.stdout
+    assert "graph_equal=1" in r.stdout
 
 
 @pytest.mark.gpu

@@ -230,3 +230,27 @@ def test_planner_with_surrogate_order_grows_the_oracle_tree(trained):
     assert np.array_equal(gn, on) and np.array_equal(gp, op)
     assert st["link_calls"] == ost["link_calls"] and st["link_ok"] == ost["link_ok"]
     assert st["valid_calls"] < ost["valid_calls"]  # surrogate order: fewer checks than the reference order
+
+
+def test_surrogate_planner_in_graph_mode_equals_step_by_step(trained):
+    """Captured iterations with the surrogate order (score, rank, head, tail
+    kernels inside the graph) grow the step-by-step tree with the same
+    counters, check counts included."""
+    _say("surrogate planner graph")
+    ctx, sc, _, _, _ = trained
+    res = []
+    for g in (0, 4):
+        ctx.tree_clear()
+        ctx.prrt_init(sc, 14, 512, order=mpt_amd.ORDER_SURROGATE)
+        ctx.prrt_graph(g)
+        ctx.prrt_add_start(sc.start)
+        st = ctx.prrt_run(1200)
+        gn, gp = ctx.tree_read()
+        res.append((gn, gp, st))
+    ctx.prrt_graph(0)
+    (an, ap, ast), (bn, bp, bst) = res
+    assert ast["graph_launches"] == 0 and bst["graph_launches"] > 0
+    assert np.array_equal(an, bn) and np.array_equal(ap, bp)
+    for key in ("iterations", "samples", "zero_dist", "invalid_states", "link_calls", "link_ok", "valid_calls",
+                "goals"):
+        assert ast[key] == bst[key], key
