@@ -249,6 +249,46 @@ int mptg_surrogate_score_edges(mptg_ctx* ctx, const double* d_from, const double
  *      tests against the host build. */
 enum mptg_diag_op { MPTG_DIAG_SIN = 0, MPTG_DIAG_COS = 1, MPTG_DIAG_ACOS = 2, MPTG_DIAG_SQRT = 3 };
 int mptg_diag_math(mptg_ctx* ctx, int op, const double* h_x, int64_t n, double* h_y);
+/* ---- autotuner (no reference counterpart: the reference has no launch
+   shapes to choose) --------------------------------------------------------
+   Launch shapes the device kernels take from the context.  None of them
+   changes a result: they only split the same work differently. */
+typedef struct mptg_tune_params {
+    int32_t knn_blocks_per_cu;      /* kNN scan blocks per CU the tree split aims at (1..256) */
+    int32_t dmv_long_waves_per_cu;  /* grid of the persistent DMV long-edge kernel (1..256) */
+    int32_t mesh_waves_per_cu;      /* grid of the persistent mesh kernels (1..256) */
+    int32_t _pad;
+} mptg_tune_params;
+
+enum { MPTG_TUNE_KNN = 0, MPTG_TUNE_DMV_LONG = 1, MPTG_TUNE_MESH = 2, MPTG_TUNE_GRAPH = 3 };
+
+/* What one tuning call tried and chose: the knob, each candidate value and
+   its median time (ms per call; per planner iteration for MPTG_TUNE_GRAPH). */
+typedef struct mptg_tune_report {
+    int32_t knob;
+    int32_t ncand;
+    int32_t chosen;
+    int32_t _pad;
+    int32_t value[8];
+    float ms[8];
+} mptg_tune_report;
+
+int mptg_tune_get(mptg_ctx* ctx, mptg_tune_params* out);
+int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p);
+/* Times kNN of the nq device queries d_q against the current tree for each
+   candidate split and keeps the fastest (replaces the nigh KD-tree's fixed
+   traversal, src/mpt/impl/prrt/prrt.hpp:406-409, which has nothing to tune). */
+int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_tune_report* rep);
+/* Times link() of n device edges for each candidate grid of the scene's
+   persistent link kernel (DMV long edges, or the mesh kernels) and keeps the
+   fastest; other scenes have no such kernel (MPTG_E_STATE). */
+int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
+                       mptg_tune_report* rep);
+/* Plain PRRT: grows the tree (these iterations count, as in mptg_prrt_run)
+   with each candidate number of iterations per graph launch, 0 = step by
+   step, and keeps the fastest per iteration; stops early at target_size. */
+int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep);
+
 /* one arm-scene link() call (as mptg_link_dev) timed phase by phase with HIP
  * events on the context stream: h_ms[0] surrogate scoring, [1] ranking,
  * [2] packed head round, [3] continuation, [4] irregular-edge walk (ms) */

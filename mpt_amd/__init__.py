@@ -361,6 +361,34 @@ class Context:
         per iteration).  Trees and statistics do not depend on it."""
         self._chk(_lib.mptg_prrt_graph(self._h, iterations_per_launch))
 
+    # ---- launch-shape autotuner (results never depend on these) ----
+    def tune_get(self) -> dict:
+        t = _native.TuneParams()
+        self._chk(_lib.mptg_tune_get(self._h, C.byref(t)))
+        return {"knn_blocks_per_cu": t.knn_blocks_per_cu, "dmv_long_waves_per_cu": t.dmv_long_waves_per_cu,
+                "mesh_waves_per_cu": t.mesh_waves_per_cu}
+
+    def tune_set(self, **kw):
+        cur = self.tune_get()
+        cur.update(kw)
+        t = _native.TuneParams(cur["knn_blocks_per_cu"], cur["dmv_long_waves_per_cu"], cur["mesh_waves_per_cu"], 0)
+        self._chk(_lib.mptg_tune_set(self._h, C.byref(t)))
+
+    def autotune_knn(self, d_q: int, nq: int, k: int) -> dict:
+        r = _native.TuneReport()
+        self._chk(_lib.mptg_autotune_knn(self._h, C.c_void_p(d_q), nq, k, C.byref(r)))
+        return r.as_dict()
+
+    def autotune_link(self, d_from: int, d_to: int, n: int) -> dict:
+        r = _native.TuneReport()
+        self._chk(_lib.mptg_autotune_link(self._h, C.c_void_p(d_from), C.c_void_p(d_to), n, C.byref(r)))
+        return r.as_dict()
+
+    def autotune_prrt(self, target_size: int = (1 << 31) - 1) -> dict:
+        r = _native.TuneReport()
+        self._chk(_lib.mptg_autotune_prrt(self._h, target_size, C.byref(r)))
+        return r.as_dict()
+
     def prrt_stats(self) -> dict:
         st = PrrtStats()
         self._chk(_lib.mptg_prrt_stats_get(self._h, C.byref(st)))

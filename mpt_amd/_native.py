@@ -28,6 +28,21 @@ class PrrtParams(C.Structure):
     ]
 
 
+class TuneParams(C.Structure):
+    _fields_ = [("knn_blocks_per_cu", C.c_int32), ("dmv_long_waves_per_cu", C.c_int32),
+                ("mesh_waves_per_cu", C.c_int32), ("_pad", C.c_int32)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("knob", C.c_int32), ("ncand", C.c_int32), ("chosen", C.c_int32), ("_pad", C.c_int32),
+                ("value", C.c_int32 * 8), ("ms", C.c_float * 8)]
+
+    def as_dict(self) -> dict:
+        n = int(self.ncand)
+        return {"knob": int(self.knob), "chosen": int(self.chosen),
+                "candidates": {int(self.value[i]): float(self.ms[i]) for i in range(n)}}
+
+
 class PrrtStarParams(C.Structure):
     _fields_ = [("k", C.c_int32), ("_pad", C.c_int32), ("rewire_factor", C.c_double)]
 
@@ -78,6 +93,11 @@ _SIGNATURES = [
     ("mptg_prrt_run", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(PrrtStats)]),
     ("mptg_prrt_stats_get", C.c_int, [C.c_void_p, C.POINTER(PrrtStats)]),
     ("mptg_prrt_graph", C.c_int, [C.c_void_p, C.c_int]),
+    ("mptg_tune_get", C.c_int, [C.c_void_p, C.POINTER(TuneParams)]),
+    ("mptg_tune_set", C.c_int, [C.c_void_p, C.POINTER(TuneParams)]),
+    ("mptg_autotune_knn", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.POINTER(TuneReport)]),
+    ("mptg_autotune_link", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(TuneReport)]),
+    ("mptg_autotune_prrt", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TuneReport)]),
     ("mptg_prrt_goals", C.c_int, [C.c_void_p, c_int32_p, C.c_int64, c_int64_p]),
     ("mptg_prrt_star_init", C.c_int, [C.c_void_p, C.POINTER(PrrtParams), C.POINTER(PrrtStarParams), c_double_p,
                                       c_double_p, c_double_p]),

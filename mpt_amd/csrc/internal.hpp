@@ -122,6 +122,9 @@ struct mptg_ctx {
     // bumped by every entry point and every (re)allocation: a captured
     // graph is replayed only while the epoch it was captured at holds
     uint64_t epoch = 0;
+    // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
+    // persistent DMV long-edge kernel and of the mesh kernels
+    mptg_tune_params tune{8, 16, 32, 0};
     hipEvent_t phase_ev[6] = {};
 };
 

@@ -365,7 +365,7 @@ bool dispatch_dim(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q
 // worth of work, with at least 256 nodes per chunk.
 static int64_t choose_splits(mptg_ctx* ctx, int64_t nq, int64_t n) {
     int64_t qblocks = (nq + kBlock - 1) / kBlock;
-    int64_t want_blocks = (int64_t)ctx->cus * 8;
+    int64_t want_blocks = (int64_t)ctx->cus * ctx->tune.knn_blocks_per_cu;
     int64_t s = (want_blocks + qblocks - 1) / qblocks;
     int64_t max_s = (n + 255) / 256;
     if (s > max_s) s = max_s;

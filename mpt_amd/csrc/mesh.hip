@@ -425,7 +425,7 @@ MeshArgs mesh_args(const mptg_ctx* ctx) {
 
 int launch_mesh_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
     if (n <= 0) return MPTG_OK;
-    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 32);
+    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * ctx->tune.mesh_waves_per_cu);
     hipLaunchKernelGGL(mesh_valid, dim3(g), dim3(64), 0, ctx->stream, mesh_args(ctx), d_q, n, d_ok);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
@@ -434,7 +434,7 @@ int launch_mesh_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok
 int launch_mesh_filter(mptg_ctx* ctx, const double* d_states, int64_t n, uint8_t* d_mask, uint8_t* d_status,
                        int invalid_code) {
     if (n <= 0) return MPTG_OK;
-    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 32);
+    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * ctx->tune.mesh_waves_per_cu);
     hipLaunchKernelGGL(mesh_filter, dim3(g), dim3(64), 0, ctx->stream, mesh_args(ctx), d_states, n, d_mask, d_status,
                        invalid_code);
     MPTG_HIP(ctx, hipGetLastError());
@@ -444,7 +444,7 @@ int launch_mesh_filter(mptg_ctx* ctx, const double* d_states, int64_t n, uint8_t
 int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from, const double* d_to, int64_t n,
                      int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks) {
     if (n <= 0) return MPTG_OK;
-    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 32);
+    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * ctx->tune.mesh_waves_per_cu);
     hipLaunchKernelGGL(mesh_link, dim3(g), dim3(64), 0, ctx->stream, space_args(ctx), mesh_args(ctx),
                        1.0 / ctx->link_param, d_table, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks);
     MPTG_HIP(ctx, hipGetLastError());

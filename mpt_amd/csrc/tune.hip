@@ -1,0 +1,184 @@
+// Launch-shape autotuner.  Each knob only changes how the same work is split
+// over the chip (kNN tree splits, grids of the persistent link kernels,
+// planner iterations per graph launch), never a result, so the tuner picks
+// by time alone: HIP events around device work, the host clock around whole
+// planner iterations (their launch and synchronisation cost is the point).
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace mptg {
+
+namespace {
+
+constexpr int kReps = 3;
+
+// Median event time (ms) of kReps calls of fn on the context stream, after
+// one untimed call that also makes every allocation.
+template <typename F>
+int time_median(mptg_ctx* ctx, F&& fn, float* ms) {
+    int rc = fn();
+    if (rc) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    MPTG_HIP(ctx, hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(ctx, MPTG_E_HIP, "hipEventCreate failed");
+    }
+    std::vector<float> t;
+    for (int r = 0; r < kReps && rc == MPTG_OK; ++r) {
+        float x = 0;
+        if (hipEventRecord(a, ctx->stream) != hipSuccess) rc = fail(ctx, MPTG_E_HIP, "hipEventRecord failed");
+        if (rc == MPTG_OK) rc = fn();
+        if (rc == MPTG_OK && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                              hipEventElapsedTime(&x, a, b) != hipSuccess))
+            rc = fail(ctx, MPTG_E_HIP, "event timing failed");
+        t.push_back(x);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return MPTG_OK;
+}
+
+// Tries each candidate through set(value) + the timed call, keeps the
+// fastest (the earliest on a tie) and leaves it set.
+template <typename Set, typename Call>
+int sweep(mptg_ctx* ctx, int knob, const std::vector<int>& cand, Set&& set, Call&& call, mptg_tune_report* rep) {
+    mptg_tune_report r{};
+    r.knob = knob;
+    r.ncand = (int32_t)cand.size();
+    int best = -1;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        set(cand[i]);
+        float ms = 0;
+        int rc = time_median(ctx, call, &ms);
+        if (rc) return rc;
+        r.value[i] = cand[i];
+        r.ms[i] = ms;
+        if (best < 0 || ms < r.ms[best]) best = (int)i;
+    }
+    set(cand[best]);
+    r.chosen = cand[best];
+    if (rep) *rep = r;
+    return MPTG_OK;
+}
+
+bool in_range(int v) { return v >= 1 && v <= 256; }
+
+}  // namespace
+
+}  // namespace mptg
+
+using namespace mptg;
+
+extern "C" {
+
+int mptg_tune_get(mptg_ctx* ctx, mptg_tune_params* out) {
+    if (!ctx || !out) return fail(ctx, MPTG_E_ARG, "null argument");
+    *out = ctx->tune;
+    return MPTG_OK;
+}
+
+int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
+    int rc = check_device_ready(ctx);  // also retires captured graphs, which hold the old grids
+    if (rc) return rc;
+    if (!p) return fail(ctx, MPTG_E_ARG, "null argument");
+    if (!in_range(p->knn_blocks_per_cu) || !in_range(p->dmv_long_waves_per_cu) || !in_range(p->mesh_waves_per_cu))
+        return fail(ctx, MPTG_E_RANGE, "launch shapes must be in 1..256");
+    ctx->tune = *p;
+    ctx->tune._pad = 0;
+    return MPTG_OK;
+}
+
+int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_tune_report* rep) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (nq <= 0 || !d_q) return fail(ctx, MPTG_E_ARG, "autotune_knn needs device queries");
+    if (k < 1 || k > 32) return fail(ctx, MPTG_E_RANGE, "k must be in 1..32");
+    if (ctx->size <= 0) return fail(ctx, MPTG_E_EMPTY, "the tree is empty");
+    DevBuf& out = ctx->scratch[11];
+    if ((rc = ensure(ctx, out, (sizeof(double) + sizeof(int32_t)) * (size_t)k * nq))) return rc;
+    double* dist = (double*)out.p;
+    int32_t* idx = (int32_t*)(dist + (size_t)k * nq);
+    return sweep(
+        ctx, MPTG_TUNE_KNN, {1, 2, 4, 8, 16, 32, 64}, [&](int v) { ctx->tune.knn_blocks_per_cu = v; },
+        [&] { return launch_knn(ctx, d_q, nq, k, __builtin_inf(), idx, dist, ctx->size); }, rep);
+}
+
+int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, mptg_tune_report* rep) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (n <= 0 || !d_from || !d_to) return fail(ctx, MPTG_E_ARG, "autotune_link needs device edges");
+    if (ctx->link_mode != MPTG_LINK_DMV || ctx->force_generic)
+        return fail(ctx, MPTG_E_STATE, "only the discrete-motion-validator link has a persistent kernel to tune");
+    DevBuf& out = ctx->scratch[11];
+    if ((rc = ensure(ctx, out, (sizeof(uint32_t) + 1) * (size_t)n))) return rc;
+    uint32_t* checks = (uint32_t*)out.p;
+    uint8_t* verdict = (uint8_t*)(checks + n);
+    auto call = [&] {
+        return launch_link_masked(ctx, d_from, d_to, n, MPTG_ORDER_REFERENCE, 0, nullptr, verdict, checks);
+    };
+    if (ctx->scene.kind == MPTG_SCENE_MESH)
+        return sweep(ctx, MPTG_TUNE_MESH, {8, 16, 32, 64, 128}, [&](int v) { ctx->tune.mesh_waves_per_cu = v; },
+                     call, rep);
+    return sweep(ctx, MPTG_TUNE_DMV_LONG, {2, 4, 8, 16, 32, 64},
+                 [&](int v) { ctx->tune.dmv_long_waves_per_cu = v; }, call, rep);
+}
+
+int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    PrrtState& s = ctx->prrt;
+    if (!s.init) return fail(ctx, MPTG_E_STATE, "mptg_prrt_init must be called first");
+    if (s.star) return fail(ctx, MPTG_E_STATE, "PRRT* enqueues its iterations one by one: nothing to tune");
+    if (ctx->size == 0) return fail(ctx, MPTG_E_EMPTY, "there are no valid initial states");
+    // each candidate: one call that captures (max(G,1) iterations), then the
+    // same number of timed iterations for every candidate; two passes in
+    // opposite orders, best of the two, so tree growth favours none
+    const std::vector<int> cand = {0, 2, 4, 8, 16};
+    constexpr int kTimedIters = 16;
+    std::vector<double> best(cand.size(), -1.0);
+    const int32_t keep = s.graph_iters;
+    bool stop = false;
+    for (int pass = 0; pass < 2 && !stop; ++pass) {
+        for (size_t j = 0; j < cand.size() && !stop; ++j) {
+            const size_t i = pass == 0 ? j : cand.size() - 1 - j;
+            s.graph_iters = cand[i];
+            if ((rc = mptg_prrt_run(ctx, target_size, std::max(cand[i], 1), nullptr))) break;
+            const int64_t it0 = s.iteration;
+            const auto t0 = std::chrono::steady_clock::now();
+            if ((rc = mptg_prrt_run(ctx, target_size, kTimedIters, nullptr))) break;
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            const int64_t ran = s.iteration - it0;
+            if (ran > 0) {
+                const double per = sec * 1e3 / (double)ran;
+                if (best[i] < 0 || per < best[i]) best[i] = per;
+            }
+            stop = ctx->size >= target_size;
+        }
+    }
+    if (rc) {
+        s.graph_iters = keep;
+        return rc;
+    }
+    mptg_tune_report r{};
+    r.knob = MPTG_TUNE_GRAPH;
+    int pick = -1;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        r.value[r.ncand] = cand[i];
+        r.ms[r.ncand] = (float)best[i];
+        ++r.ncand;
+        if (best[i] >= 0 && (pick < 0 || best[i] < best[pick])) pick = (int)i;
+    }
+    s.graph_iters = pick >= 0 ? cand[pick] : keep;
+    r.chosen = s.graph_iters;
+    if (rep) *rep = r;
+    return MPTG_OK;
+}
+
+}  // extern "C"

@@ -473,7 +473,7 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     int64_t* long_list = (int64_t*)((char*)ctx->scratch[7].p + 256);
     if ((rc = launch_zero_u32(ctx, long_count, 1))) return rc;
     double inv = 1.0 / ctx->link_param;
-    unsigned lg = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * 16);
+    unsigned lg = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * ctx->tune.dmv_long_waves_per_cu);
     dispatch_dim(ctx->dim, [&]<int D>() {
         hipLaunchKernelGGL((link_dmv_short<D>), dim3(blocks), dim3(kBlock), 0, ctx->stream, space_args(ctx),
                            scene_args(ctx), inv, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n, known_valid, d_mask,

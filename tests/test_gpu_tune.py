@@ -1,0 +1,131 @@
+"""Launch-shape autotuner (mptg_autotune_*, mptg_tune_set): every candidate
+shape gives the oracle's bit-exact results, the tuner reports a time for each
+candidate and leaves the fastest set, and a planner tuned while it grows
+still grows the oracle's tree."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from mpt_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _plain(metric, dim):
+    return scenes.Scene(metric=metric, dim=dim, kind=scenes.SCENE_NONE, link_mode=scenes.LINK_DMV, link_param=1.0,
+                        lo=np.zeros(dim), hi=np.ones(dim), start=np.zeros(dim), goal=np.zeros(dim))
+
+
+def _check_report(rep, knob_values):
+    assert sorted(rep["candidates"]) == sorted(knob_values)
+    assert all(ms > 0 for ms in rep["candidates"].values())
+    assert rep["chosen"] == min(rep["candidates"], key=lambda v: (rep["candidates"][v], knob_values.index(v)))
+
+
+def test_knn_every_split_is_exact_and_tuner_picks_fastest(ctx_factory):
+    _say("tune knn")
+    ctx = ctx_factory()
+    rng = np.random.default_rng(71)
+    n, dim, k = 60000, 7, 20
+    pts = rng.uniform(-math.pi, math.pi, (n, dim))
+    q = rng.uniform(-math.pi, math.pi, (4096, dim))
+    ctx.space(scenes.METRIC_L2, dim)
+    ctx.tree_insert(pts)
+    oi, od = oracle.Oracle(_plain(scenes.METRIC_L2, dim)).knn(pts, q[:600], k)
+    for bpc in (1, 3, 8, 64, 256):
+        ctx.tune_set(knn_blocks_per_cu=bpc)
+        gi, gd = ctx.knn(q[:600], k)
+        assert np.array_equal(gi, oi) and np.array_equal(gd, od), bpc
+    dq = ctx.to_device(q)
+    rep = ctx.autotune_knn(dq.ptr, q.shape[0], k)
+    _say(" ", rep)
+    _check_report(rep, [1, 2, 4, 8, 16, 32, 64])
+    assert ctx.tune_get()["knn_blocks_per_cu"] == rep["chosen"]
+    gi, gd = ctx.knn(q[:600], k)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+def test_dmv_long_grid_every_shape_is_exact(ctx_factory):
+    _say("tune dmv")
+    ctx = ctx_factory()
+    sc = scenes.grid_scene(size=1024, seed=3)
+    ctx.scene(sc)
+    rng = np.random.default_rng(72)
+    ang = rng.uniform(0, 2 * math.pi, 3000)
+    ln = rng.uniform(100, 1400, 3000)
+    a = rng.uniform(0, 1024, (3000, 2))
+    b = np.clip(a + ln[:, None] * np.stack([np.cos(ang), np.sin(ang)], 1), 0, 1023)
+    ov, oc = oracle.Oracle(sc).link(a, b, threads=16)
+    for w in (1, 5, 64):
+        ctx.tune_set(dmv_long_waves_per_cu=w)
+        gv, gc = ctx.link(a, b)
+        assert np.array_equal(gv, ov) and np.array_equal(gc, oc), w
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    rep = ctx.autotune_link(da.ptr, db.ptr, a.shape[0])
+    _say(" ", rep)
+    _check_report(rep, [2, 4, 8, 16, 32, 64])
+    assert ctx.tune_get()["dmv_long_waves_per_cu"] == rep["chosen"]
+    gv, gc = ctx.link(a, b)
+    assert np.array_equal(gv, ov) and np.array_equal(gc, oc)
+
+
+def test_mesh_grid_every_shape_is_exact(ctx_factory):
+    _say("tune mesh")
+    ctx = ctx_factory()
+    sc = scenes.se3_scene()
+    ctx.scene(sc)
+    rng = np.random.default_rng(73)
+    qa = rng.normal(size=(800, 4))
+    qa /= np.linalg.norm(qa, axis=1, keepdims=True)
+    a = np.concatenate([qa, rng.uniform(0, 100, (800, 3))], axis=1)
+    b = a.copy()
+    b[:, 4:] += rng.normal(0, 3, (800, 3))
+    o = oracle.Oracle(sc)
+    ov, oc = o.link(a, b, threads=16)
+    ok = o.valid(a)
+    for w in (1, 7, 128):
+        ctx.tune_set(mesh_waves_per_cu=w)
+        gv, gc = ctx.link(a, b)
+        assert np.array_equal(gv, ov) and np.array_equal(gc, oc), w
+        assert np.array_equal(ctx.valid(a), ok), w
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    rep = ctx.autotune_link(da.ptr, db.ptr, a.shape[0])
+    _say(" ", rep)
+    _check_report(rep, [8, 16, 32, 64, 128])
+
+
+def test_planner_tuned_while_growing_equals_oracle(ctx_factory):
+    _say("tune prrt")
+    sc = scenes.boxes_scene()
+    ctx = ctx_factory()
+    ctx.scene(sc)
+    ctx.tree_clear()
+    ctx.prrt_init(sc, 19, 64)
+    ctx.prrt_add_start(sc.start)
+    rep = ctx.autotune_prrt(9000)
+    _say(" ", rep)
+    assert sorted(rep["candidates"]) == [0, 2, 4, 8, 16]
+    assert rep["chosen"] in rep["candidates"]
+    st = ctx.prrt_run(9000)
+    gn, gp = ctx.tree_read()
+    on, op, og, ost = oracle.Oracle(sc).prrt(19, 64, 9000)
+    assert np.array_equal(gn, on) and np.array_equal(gp, op)
+    for key in ("iterations", "samples", "zero_dist", "invalid_states", "link_calls", "link_ok", "valid_calls",
+                "goals"):
+        assert st[key] == ost[key], key
+
+
+def test_tune_set_rejects_out_of_range(ctx_factory):
+    ctx = ctx_factory()
+    with pytest.raises(Exception):
+        ctx.tune_set(knn_blocks_per_cu=0)
+    with pytest.raises(Exception):
+        ctx.tune_set(mesh_waves_per_cu=257)
