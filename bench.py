@@ -58,10 +58,12 @@ class Timer:
         return self.ctx.time_ms(fn, reps) / 1e3
 
 
-def run_planner(ctx, sc, args, order, batch):
-    """Grows the arm tree from the start: warmup steps, then the timed steps."""
+def run_planner(ctx, sc, args, order, batch, graph=0):
+    """Grows the arm tree from the start: warmup steps, then the timed steps
+    (`graph` iterations per hipGraph launch, 0 = step by step)."""
     ctx.tree_clear()
     ctx.prrt_init(sc, seed=args.seed, batch=batch, order=order)
+    ctx.prrt_graph(graph)
     ctx.prrt_add_start(sc.start)
     ctx.prrt_run(1 << 30, args.warmup)
     s0 = ctx.prrt_stats()
@@ -82,8 +84,22 @@ def run_planner(ctx, sc, args, order, batch):
         "link_calls": d["link_calls"],
         "valid_calls": d["valid_calls"],
         "tree_size": s1["size"],
+        "graph_iterations": graph,
+        "graph_launches": d["graph_launches"],
     }
     return res, nodes, parents
+
+
+def tune_planner(ctx, sc, args, batch):
+    """Iterations per graph launch chosen by the autotuner on a tree of its
+    own (the choice does not change any tree)."""
+    import mpt_amd
+    ctx.tree_clear()
+    ctx.prrt_init(sc, seed=args.seed + 1000, batch=batch, order=mpt_amd.ORDER_REFERENCE)
+    ctx.prrt_add_start(sc.start)
+    rep = ctx.autotune_prrt()
+    ctx.prrt_graph(0)
+    return rep
 
 
 def arm_link_traffic():
@@ -110,13 +126,19 @@ def bench_arm(ctx, args, timer):
     sc = scenes.arm_scene()
     B = 8192
     ctx.scene(sc)
+    log("  graph autotune ...")
+    tune = tune_planner(ctx, sc, args, B)
+    G = tune["chosen"]
     log("  plain planner ...")
-    plain, nodes_p, par_p = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B)
+    plain, nodes_p, par_p = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B, G)
+    plain_steps, nodes_ps, par_ps = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B, 0)
     log("  surrogate training ...")
     train = surrogate.train(ctx, sc, nodes_p, n_samples=args.train_samples, epochs=args.train_epochs, seed=5)
     log("  surrogate-ordered planner ...")
-    sur, nodes_s, par_s = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B)
-    same_tree = bool(np.array_equal(nodes_p, nodes_s) and np.array_equal(par_p, par_s))
+    sur, nodes_s, par_s = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B, G)
+    sur_steps, nodes_ss, par_ss = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B, 0)
+    same_tree = all(bool(np.array_equal(nodes_p, n) and np.array_equal(par_p, p))
+                    for n, p in ((nodes_s, par_s), (nodes_ps, par_ps), (nodes_ss, par_ss)))
 
     # dominant kernels on a fixed batch of planner-like edges
     ne = 8192
@@ -185,7 +207,9 @@ def bench_arm(ctx, args, timer):
             "launch_ms": t_kernel * 1e3, "flops_per_launch": flops_s, "work": wk_s,
             "reference_equivalent_tflops": (FLOP_FK_LINK * wk_s["ref_fk"] + FLOP_REF_TEST * wk_s["ref_tests"])
             / t_kernel / 1e12}
-    res = {"plain": plain, "surrogate": sur, "trees_identical": same_tree, "training": train,
+    res = {"plain": plain, "surrogate": sur, "plain_step_by_step": plain_steps,
+           "surrogate_step_by_step": sur_steps, "graph_autotune": tune,
+           "trees_identical": same_tree, "training": train,
            "edge_batch": edge_batch,
            "planner_gain": sur["edge_checks_per_s"] / plain["edge_checks_per_s"]}
     for x in (da, db, v, c, scores):
@@ -207,6 +231,7 @@ def bench_nn(ctx, args, timer):
     for k in (1, 20):
         di = ctx.array((65536, k), np.int32)
         dd = ctx.array((65536, k), np.float64)
+        tune = ctx.autotune_knn(dq.ptr, 65536, k)
 
         def run():
             ctx.knn_dev(dq.ptr, 65536, k, math.inf, di.ptr, dd.ptr)
@@ -216,7 +241,7 @@ def bench_nn(ctx, args, timer):
         t = timer.time(run, 2 if args.quick else 3)
         flops = 65536 * 3 * 7 * n
         out[f"k{k}"] = {"ms": t * 1e3, "queries_per_s": 65536 / t, "achieved_tflops": flops / t / 1e12,
-                        "frac_fp32": flops / t / FP32_PEAK}
+                        "frac_fp32": flops / t / FP32_PEAK, "autotune_blocks_per_cu": tune}
     return out
 
 
@@ -234,6 +259,7 @@ def bench_grid(ctx, args, timer):
     def run():
         ctx.link_dev(da.ptr, db.ptr, n, mpt_amd.ORDER_REFERENCE, False, v.ptr, c.ptr)
 
+    tune = ctx.autotune_link(da.ptr, db.ptr, n)
     run()
     ctx.sync()
     t = timer.time(run, 5)
@@ -241,7 +267,7 @@ def bench_grid(ctx, args, timer):
     byts = n * (32 + 1) + w
     return {"edges": n, "ms": t * 1e3, "edge_checks_per_s": n / t, "valid_calls": w,
             "algorithmic_bytes": byts, "achieved_gbs": byts / t / 1e9, "frac_hbm": byts / t / HBM_PEAK,
-            "occupancy": sc.extra["occupancy"]}
+            "occupancy": sc.extra["occupancy"], "autotune_long_waves_per_cu": tune}
 
 
 def bench_boxes(ctx, args):
@@ -271,6 +297,13 @@ def bench_se3(ctx, args):
     ctx.prrt_star_init(sc, seed=args.seed, batch=4096, k=20)
     ctx.prrt_add_start(sc.start)
     ctx.prrt_run(1 << 30, args.warmup)
+    # mesh kernel grid tuned on the warm tree's own edges (parent -> node)
+    nodes, parents = ctx.tree_read()
+    ea, eb = nodes[parents[1:]], nodes[1:]
+    da, db = ctx.to_device(ea), ctx.to_device(eb)
+    tune = ctx.autotune_link(da.ptr, db.ptr, ea.shape[0])
+    da.free()
+    db.free()
     s0 = ctx.prrt_stats()
     ctx.sync()
     t0 = time.perf_counter()
@@ -284,7 +317,7 @@ def bench_se3(ctx, args):
             "extensions_per_s": d["samples"] / dt, "accepted_per_s": d["link_ok"] / dt,
             "edge_checks_per_s": links / dt, "extend_links": d["link_calls"], "choose_parent_links": d["cp_links"],
             "rewire_links": d["rw_links"], "rewires": d["rewires"], "deferred": d["deferred"],
-            "tree_size": s1["size"], "goals": s1["goals"]}
+            "tree_size": s1["size"], "goals": s1["goals"], "autotune_mesh_waves_per_cu": tune}
 
 
 def cpu_baseline(sc, seconds):

@@ -56,11 +56,15 @@ int ensure_pinned(mptg_ctx* ctx, size_t bytes) {
     return MPTG_OK;
 }
 
-int check_device_ready(mptg_ctx* ctx) {
+int check_device(mptg_ctx* ctx) {
     if (!ctx) return fail(nullptr, MPTG_E_ARG, "null context");
-    ++ctx->epoch;  // any entry point may change what a captured graph holds
     MPTG_HIP(ctx, hipSetDevice(ctx->device));
     return MPTG_OK;
+}
+
+int check_device_ready(mptg_ctx* ctx) {
+    if (ctx) ++ctx->epoch;  // the entry point may change what a captured graph holds
+    return check_device(ctx);
 }
 
 static void free_tree(mptg_ctx* ctx) {
@@ -186,7 +190,7 @@ const char* mptg_last_error(const mptg_ctx* ctx) {
 }
 
 int mptg_sync(mptg_ctx* ctx) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return MPTG_OK;
@@ -195,7 +199,7 @@ int mptg_sync(mptg_ctx* ctx) {
 void* mptg_stream(mptg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int mptg_dev_alloc(mptg_ctx* ctx, uint64_t bytes, void** d_out) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (!d_out) return fail(ctx, MPTG_E_ARG, "null output pointer");
     *d_out = nullptr;
@@ -204,7 +208,7 @@ int mptg_dev_alloc(mptg_ctx* ctx, uint64_t bytes, void** d_out) {
 }
 
 int mptg_dev_free(mptg_ctx* ctx, void* d_ptr) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     MPTG_HIP(ctx, hipFree(d_ptr));
@@ -212,7 +216,7 @@ int mptg_dev_free(mptg_ctx* ctx, void* d_ptr) {
 }
 
 int mptg_dev_upload(mptg_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (bytes == 0) return MPTG_OK;
     MPTG_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -221,7 +225,7 @@ int mptg_dev_upload(mptg_ctx* ctx, void* d_dst, const void* h_src, uint64_t byte
 }
 
 int mptg_dev_download(mptg_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (bytes == 0) return MPTG_OK;
     MPTG_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -230,7 +234,7 @@ int mptg_dev_download(mptg_ctx* ctx, void* h_dst, const void* d_src, uint64_t by
 }
 
 int mptg_dev_memset(mptg_ctx* ctx, void* d_dst, int value, uint64_t bytes) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (bytes == 0) return MPTG_OK;
     MPTG_HIP(ctx, hipMemsetAsync(d_dst, value, bytes, ctx->stream));
@@ -238,7 +242,7 @@ int mptg_dev_memset(mptg_ctx* ctx, void* d_dst, int value, uint64_t bytes) {
 }
 
 int mptg_timer_start(mptg_ctx* ctx) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (!ctx->ev_start) MPTG_HIP(ctx, hipEventCreate(&ctx->ev_start));
     if (!ctx->ev_stop) MPTG_HIP(ctx, hipEventCreate(&ctx->ev_stop));
@@ -247,7 +251,7 @@ int mptg_timer_start(mptg_ctx* ctx) {
 }
 
 int mptg_timer_stop(mptg_ctx* ctx, float* ms) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (!ctx->ev_start || !ms) return fail(ctx, MPTG_E_STATE, "mptg_timer_start was not called");
     MPTG_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
@@ -339,7 +343,7 @@ int mptg_tree_insert(mptg_ctx* ctx, const double* h_states, const int32_t* h_par
 int64_t mptg_tree_size(const mptg_ctx* ctx) { return ctx ? ctx->size : -1; }
 
 int mptg_tree_read(mptg_ctx* ctx, int64_t first, int64_t n, double* h_states, int32_t* h_parents) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (first < 0 || n < 0 || first + n > ctx->size) return fail(ctx, MPTG_E_RANGE, "read beyond tree size");
     if (n == 0) return MPTG_OK;
@@ -489,7 +493,7 @@ int mptg_set_link(mptg_ctx* ctx, int mode, double param) {
 // ---- host-pointer wrappers --------------------------------------------------
 int mptg_knn(mptg_ctx* ctx, const double* h_q, int64_t nq, int k, double radius, int32_t* h_idx,
              double* h_dist) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (nq < 0 || (nq > 0 && (!h_q || !h_idx || !h_dist))) return fail(ctx, MPTG_E_ARG, "bad knn buffers");
     if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..32");
@@ -511,7 +515,7 @@ int mptg_knn(mptg_ctx* ctx, const double* h_q, int64_t nq, int k, double radius,
 
 int mptg_knn_dev(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
                  double* d_dist) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (nq < 0 || (nq > 0 && (!d_q || !d_idx || !d_dist))) return fail(ctx, MPTG_E_ARG, "bad knn buffers");
     if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..32");
@@ -520,7 +524,7 @@ int mptg_knn_dev(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double rad
 }
 
 int mptg_valid(mptg_ctx* ctx, const double* h_q, int64_t n, uint8_t* h_ok) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!h_q || !h_ok))) return fail(ctx, MPTG_E_ARG, "bad valid buffers");
     if (n == 0) return MPTG_OK;
@@ -537,7 +541,7 @@ int mptg_valid(mptg_ctx* ctx, const double* h_q, int64_t n, uint8_t* h_ok) {
 
 int mptg_link(mptg_ctx* ctx, const double* h_from, const double* h_to, int64_t n, int order,
               int endpoints_known_valid, uint8_t* h_verdict, uint32_t* h_checks) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!h_from || !h_to || !h_verdict || !h_checks)))
         return fail(ctx, MPTG_E_ARG, "bad link buffers");
@@ -564,7 +568,7 @@ int mptg_link(mptg_ctx* ctx, const double* h_from, const double* h_to, int64_t n
 
 int mptg_link_dev(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                   int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!d_from || !d_to || !d_verdict || !d_checks)))
         return fail(ctx, MPTG_E_ARG, "bad link buffers");

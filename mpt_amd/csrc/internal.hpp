@@ -49,7 +49,7 @@ struct PrrtState {
     bool init = false;
     bool counters_valid = false;  // device counters mirror the host state
     mptg_prrt_stats stats_base{};  // host stats when the device counters were reset
-    DevBuf q, near, ndist, from, to, status, mask, verdict, checks, counters, goal_buf;
+    DevBuf q, near, ndist, from, to, status, mask, verdict, checks, counters, goal_buf, blk;
     mptg_prrt_params p{};
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
     int64_t iteration = 0;       // next iteration index
@@ -156,7 +156,12 @@ inline int pitch32(int dim) { return dim <= 2 ? 2 : (dim <= 4 ? 4 : (dim <= 8 ? 
 
 int ensure(mptg_ctx* ctx, DevBuf& b, size_t bytes);
 int ensure_pinned(mptg_ctx* ctx, size_t bytes);
+// Entry-point prologue: selects the context's device and retires captured
+// graphs (the call may change state they hold).
 int check_device_ready(mptg_ctx* ctx);
+// The same without retiring graphs, for entry points that change nothing a
+// graph holds (their scratch growth retires graphs through ensure()).
+int check_device(mptg_ctx* ctx);
 
 // kernels launched from other translation units
 // n_dev: tree size read on the device (tree_n then only plans the split);

@@ -114,13 +114,25 @@ __device__ __forceinline__ float prefilter_threshold(double dk, double S) {
     return (double)f < thr ? float_of(bits_of(f) + 1u) : f;
 }
 
+// Splits actually used for a tree of n nodes when the launch was planned
+// for `planned` (a graph captured for a larger tree): at least 256 nodes
+// per split, as the host plan, and no empty split.
+__device__ __forceinline__ int64_t knn_used_splits(int64_t n, int64_t planned, int64_t* chunk) {
+    int64_t s = (n + 255) / 256;
+    s = s < planned ? s : planned;
+    s = s > 1 ? s : 1;
+    *chunk = n > 0 ? (n + s - 1) / s : 0;
+    return n > 0 ? (n + *chunk - 1) / *chunk : 1;
+}
+
 // Tree size: n, or *n_dev when given (launches captured in a graph read the
-// size the previous iteration left); *skip != 0 makes the launch a no-op.
+// size the previous iteration left; splits beyond those the size needs
+// return at once); *skip != 0 makes the launch a no-op.
 __device__ __forceinline__ bool knn_extent(const int64_t* n_dev, const int32_t* skip, int64_t* n, int64_t* chunk) {
     if (skip && *skip) return false;
     if (n_dev) {
         *n = *n_dev;
-        *chunk = (*n + gridDim.y - 1) / gridDim.y;
+        if ((int64_t)blockIdx.y >= knn_used_splits(*n, gridDim.y, chunk)) return false;
     }
     return true;
 }
@@ -227,9 +239,14 @@ __global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict_
 template <int K>
 __global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                     int splits, int64_t nq, int k, double* __restrict__ out_d,
-                                                    int32_t* __restrict__ out_i, const int32_t* skip) {
+                                                    int32_t* __restrict__ out_i, const int32_t* skip,
+                                                    const int64_t* n_dev) {
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (qi >= nq || (skip && *skip)) return;
+    if (n_dev) {
+        int64_t chunk;
+        splits = (int)knn_used_splits(*n_dev, splits, &chunk);
+    }
     TopK<K> top;
     top.init();
     for (int s = 0; s < splits; ++s) {
@@ -414,9 +431,9 @@ int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radiu
     MPTG_HIP(ctx, hipGetLastError());
     dim3 g1((unsigned)((nq + kBlock - 1) / kBlock));
     if (splits > 1) {
-        if (K == 1) hipLaunchKernelGGL((knn_merge<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip);
-        else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip);
-        else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip);
+        if (K == 1) hipLaunchKernelGGL((knn_merge<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
+        else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
+        else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
     } else {
         if (K == 1) hipLaunchKernelGGL((knn_finish<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);
         else if (K == 20) hipLaunchKernelGGL((knn_finish<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);

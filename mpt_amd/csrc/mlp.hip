@@ -692,7 +692,7 @@ int mptg_surrogate_samples(mptg_ctx* ctx, const double* d_from, const double* d_
 // for reference ordinal o (slots beyond the edge's waypoint count untouched).
 int mptg_surrogate_score_edges(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
                                float* d_scores) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!d_from || !d_to || !d_scores))) return fail(ctx, MPTG_E_ARG, "bad buffers");
     if (n == 0) return MPTG_OK;
@@ -716,7 +716,7 @@ int mptg_surrogate_init(mptg_ctx* ctx, uint64_t seed) {
 }
 
 int mptg_surrogate_get(mptg_ctx* ctx, float* h_params, int64_t n) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc || (rc = sur_check(ctx, true))) return rc;
     if (n != NPARAM || !h_params) return fail(ctx, MPTG_E_ARG, "parameter buffer must hold mptg_surrogate_nparams()");
     MPTG_HIP(ctx, hipMemcpyAsync(h_params, surrogate(ctx).params.p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost,
@@ -742,7 +742,7 @@ int mptg_surrogate_set(mptg_ctx* ctx, const float* h_params, int64_t n) {
 }
 
 int mptg_surrogate_features(mptg_ctx* ctx, const double* d_states, int64_t n, uint16_t* d_feat) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks)
         return fail(ctx, MPTG_E_STATE, "surrogate features are defined for the arm scene");
@@ -762,7 +762,7 @@ int mptg_surrogate_features(mptg_ctx* ctx, const double* d_states, int64_t n, ui
 
 // Logits of n feature rows (device buffers).
 int mptg_surrogate_forward(mptg_ctx* ctx, const uint16_t* d_feat, int64_t n, float* d_logits) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc || (rc = sur_check(ctx, true))) return rc;
     if (n < 0 || (n > 0 && (!d_feat || !d_logits))) return fail(ctx, MPTG_E_ARG, "bad buffers");
     Surrogate& S = surrogate(ctx);

@@ -22,107 +22,164 @@ namespace mptg {
 
 namespace {
 
-// Single-block stable compaction: accepted samples get consecutive node ids
-// in sample order; also accumulates the counters.
-__global__ __launch_bounds__(1024) void prrt_append(SpaceArgs sp, PrrtDev pd, double goal_radius, int B, int dp,
-                                                    const double* __restrict__ to, const int32_t* __restrict__ near,
-                                                    uint8_t* __restrict__ status, const uint8_t* __restrict__ verdict,
-                                                    const uint32_t* __restrict__ checks, double* __restrict__ x64,
-                                                    float* __restrict__ x32, int32_t* __restrict__ parent,
-                                                    Counters* __restrict__ cnt, int32_t* __restrict__ goal_ids,
-                                                    int64_t goal_cap) {
-    __shared__ int wsum[16];
-    __shared__ int gsum[16];
-    __shared__ int64_t base_s, gbase_s;
-    __shared__ int64_t zero_s, inv_s, lc_s, ok_s, vc_s;
-    const int D = sp.dim;
+// Append in three launches, one sample per thread: count (per-block
+// accepted and goal counts, the counters' sums), write (node ids in sample
+// order: the block's base is the sum over the blocks before it) and commit
+// (tree size, goals, the next iteration and the stop flag).  Every kernel is
+// a no-op once the stop flag is set.
+constexpr int kAppendBlock = 1024;
+
+__device__ __forceinline__ int block_rank(bool flag, int* wsum, int* total) {
+    // rank of this thread's flag among the block's set flags (thread order);
+    // wsum: 16 ints of LDS.  Contains __syncthreads: call from every thread.
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    if (cnt->done) return;  // an iteration past the stop (block-uniform)
-    if (t == 0) {
-        base_s = cnt->size;
-        gbase_s = cnt->goals;
-        zero_s = inv_s = lc_s = ok_s = vc_s = 0;
+    const uint64_t m = __ballot(flag);
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    if (lane == 0) wsum[wid] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int w = 0; w < kAppendBlock / 64; ++w) {
+        if (w < wid) base += wsum[w];
+        tot += wsum[w];
+    }
+    *total = tot;
+    return base + __popcll(m & below);
+}
+
+// Sum over the block (kAppendBlock threads) of two int64 values; lds: 32
+// int64 of LDS.  Contains __syncthreads: call from every thread.
+__device__ __forceinline__ void block_sum2(int64_t* a, int64_t* b, int64_t* lds) {
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    int64_t x = *a, y = *b;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        x += __shfl_xor(x, o);
+        y += __shfl_xor(y, o);
+    }
+    if (lane == 0) {
+        lds[wid] = x;
+        lds[16 + wid] = y;
     }
     __syncthreads();
-    int64_t lzero = 0, linv = 0, llc = 0, lok = 0, lvc = 0;
-    for (int c0 = 0; c0 < B; c0 += 1024) {
-        int b = c0 + t;
-        bool ok = false, goal = false;
-        if (b < B) {
-            uint8_t st = status[b];
-            if (st == ST_LINK_FAIL) {
-                ++llc;
-                lvc += checks[b];
-                if (verdict[b]) {
-                    st = ST_OK;
-                    status[b] = ST_OK;
-                }
-            }
-            lzero += st == ST_ZERO;
-            linv += st == ST_INVALID;
-            if (st == ST_OK) {
-                ++lok;
-                ok = true;
-                goal = space_dist(sp, to + (int64_t)b * D, pd.goal) <= goal_radius;
-            }
-        }
-        // wave-level ballots then block prefix over 16 waves
-        uint64_t m = __ballot(ok);
-        uint64_t gm = __ballot(goal);
-        uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        int wrank = __popcll(m & below);
-        int grank = __popcll(gm & below);
-        if (lane == 0) {
-            wsum[wid] = __popcll(m);
-            gsum[wid] = __popcll(gm);
-        }
-        __syncthreads();
-        int wbase = 0, gwbase = 0, tot = 0, gtot = 0;
-        for (int w = 0; w < 16; ++w) {
-            if (w < wid) {
-                wbase += wsum[w];
-                gwbase += gsum[w];
-            }
-            tot += wsum[w];
-            gtot += gsum[w];
-        }
-        if (ok) {
-            int64_t id = base_s + wbase + wrank;
-            for (int i = 0; i < D; ++i) {
-                double v = to[(int64_t)b * D + i];
-                x64[id * D + i] = v;
-                x32[id * dp + i] = (float)v;
-            }
-            parent[id] = near[b];
-            if (goal) {
-                int64_t g = gbase_s + gwbase + grank;
-                if (g < goal_cap) goal_ids[g] = (int32_t)id;
-            }
-        }
-        __syncthreads();
-        if (t == 0) {
-            base_s += tot;
-            gbase_s += gtot;
-        }
-        __syncthreads();
+    x = y = 0;
+    for (int w = 0; w < kAppendBlock / 64; ++w) {
+        x += lds[w];
+        y += lds[16 + w];
     }
-    atomicAdd((unsigned long long*)&zero_s, (unsigned long long)lzero);
-    atomicAdd((unsigned long long*)&inv_s, (unsigned long long)linv);
-    atomicAdd((unsigned long long*)&lc_s, (unsigned long long)llc);
-    atomicAdd((unsigned long long*)&ok_s, (unsigned long long)lok);
-    atomicAdd((unsigned long long*)&vc_s, (unsigned long long)lvc);
+    *a = x;
+    *b = y;
+}
+
+__global__ __launch_bounds__(kAppendBlock) void prrt_count(SpaceArgs sp, PrrtDev pd, double goal_radius, int B,
+                                                           const double* __restrict__ to,
+                                                           uint8_t* __restrict__ status,
+                                                           const uint8_t* __restrict__ verdict,
+                                                           const uint32_t* __restrict__ checks,
+                                                           uint8_t* __restrict__ flags, int32_t* __restrict__ blk,
+                                                           Counters* __restrict__ cnt) {
+    __shared__ int wsum[16];
+    __shared__ unsigned long long part[5];
+    const int D = sp.dim;
+    const int t = threadIdx.x;
+    if (cnt->done) return;  // block-uniform
+    if (t < 5) part[t] = 0;
+    const int b = blockIdx.x * kAppendBlock + t;
+    bool ok = false, goal = false;
+    unsigned long long lz = 0, li = 0, lc = 0, lo = 0, lv = 0;
+    if (b < B) {
+        uint8_t st = status[b];
+        if (st == ST_LINK_FAIL) {
+            lc = 1;
+            lv = checks[b];
+            if (verdict[b]) {
+                st = ST_OK;
+                status[b] = ST_OK;
+            }
+        }
+        lz = st == ST_ZERO;
+        li = st == ST_INVALID;
+        if (st == ST_OK) {
+            lo = 1;
+            ok = true;
+            goal = space_dist(sp, to + (int64_t)b * D, pd.goal) <= goal_radius;
+        }
+        flags[b] = (uint8_t)((ok ? 1 : 0) | (goal ? 2 : 0));
+    }
+    int nok, ngoal;
+    (void)block_rank(ok, wsum, &nok);
+    __syncthreads();
+    (void)block_rank(goal, wsum, &ngoal);
+    if (lz) atomicAdd(&part[0], lz);
+    if (li) atomicAdd(&part[1], li);
+    if (lc) atomicAdd(&part[2], lc);
+    if (lo) atomicAdd(&part[3], lo);
+    if (lv) atomicAdd(&part[4], lv);
     __syncthreads();
     if (t == 0) {
-        cnt->size = base_s;
-        cnt->goals = gbase_s;
-        cnt->zero += zero_s;
-        cnt->invalid += inv_s;
-        cnt->link_calls += lc_s;
-        cnt->link_ok += ok_s;
-        cnt->valid_calls += vc_s;
-        cnt->iteration += 1;
-        cnt->done = (base_s >= cnt->target || cnt->iteration >= cnt->iter_end) ? 1 : 0;
+        blk[2 * blockIdx.x] = nok;
+        blk[2 * blockIdx.x + 1] = ngoal;
+        atomicAdd((unsigned long long*)&cnt->zero, part[0]);
+        atomicAdd((unsigned long long*)&cnt->invalid, part[1]);
+        atomicAdd((unsigned long long*)&cnt->link_calls, part[2]);
+        atomicAdd((unsigned long long*)&cnt->link_ok, part[3]);
+        atomicAdd((unsigned long long*)&cnt->valid_calls, part[4]);
     }
+}
+
+__global__ __launch_bounds__(kAppendBlock) void prrt_write(int D, int dp, int B, const double* __restrict__ to,
+                                                           const int32_t* __restrict__ near,
+                                                           const uint8_t* __restrict__ flags,
+                                                           const int32_t* __restrict__ blk, double* __restrict__ x64,
+                                                           float* __restrict__ x32, int32_t* __restrict__ parent,
+                                                           const Counters* __restrict__ cnt,
+                                                           int32_t* __restrict__ goal_ids, int64_t goal_cap) {
+    __shared__ int wsum[16];
+    __shared__ int64_t red[32];
+    const int t = threadIdx.x;
+    if (cnt->done) return;  // block-uniform
+    // this block's first node id and goal slot: the counts of the blocks
+    // before it (at most kAppendBlock of them) on top of the tree
+    int64_t base = t < (int)blockIdx.x ? blk[2 * t] : 0;
+    int64_t gbase = t < (int)blockIdx.x ? blk[2 * t + 1] : 0;
+    block_sum2(&base, &gbase, red);
+    base += cnt->size;
+    gbase += cnt->goals;
+    __syncthreads();
+    const int b = blockIdx.x * kAppendBlock + t;
+    const uint8_t f = b < B ? flags[b] : 0;
+    const bool ok = f & 1, goal = f & 2;
+    int tot;
+    const int r = block_rank(ok, wsum, &tot);
+    __syncthreads();
+    const int gr = block_rank(goal, wsum, &tot);
+    if (ok) {
+        const int64_t id = base + r;
+        for (int i = 0; i < D; ++i) {
+            const double v = to[(int64_t)b * D + i];
+            x64[id * D + i] = v;
+            x32[id * dp + i] = (float)v;
+        }
+        parent[id] = near[b];
+        if (goal) {
+            const int64_t g = gbase + gr;
+            if (g < goal_cap) goal_ids[g] = (int32_t)id;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kAppendBlock) void prrt_commit(int nblk, const int32_t* __restrict__ blk,
+                                                            Counters* __restrict__ cnt) {
+    __shared__ int64_t red[32];
+    const int t = threadIdx.x;
+    if (cnt->done) return;  // block-uniform
+    int64_t add = t < nblk ? blk[2 * t] : 0, gadd = t < nblk ? blk[2 * t + 1] : 0;
+    block_sum2(&add, &gadd, red);
+    if (t != 0) return;
+    const int64_t size = cnt->size + add;
+    cnt->size = size;
+    cnt->goals += gadd;
+    cnt->iteration += 1;
+    cnt->done = (size >= cnt->target || cnt->iteration >= cnt->iter_end) ? 1 : 0;
 }
 
 }  // namespace
@@ -144,6 +201,7 @@ int prrt_reserve(mptg_ctx* ctx, int B) {
     if ((rc = ensure(ctx, pb.checks, sizeof(uint32_t) * B))) return rc;
     if ((rc = ensure(ctx, pb.counters, sizeof(Counters)))) return rc;
     if ((rc = ensure(ctx, pb.goal_buf, sizeof(int32_t) * 4096))) return rc;
+    if ((rc = ensure(ctx, pb.blk, sizeof(int32_t) * 2 * ((B + kAppendBlock - 1) / kAppendBlock)))) return rc;
     return MPTG_OK;
 }
 
@@ -160,7 +218,7 @@ void prrt_free(mptg_ctx* ctx) {
                      &pb.verdict, &pb.checks,  &pb.counters, &pb.goal_buf, &pb.dnear, &pb.sidx,   &pb.qs,
                      &pb.nbh_idx, &pb.nbh_d,   &pb.marked, &pb.pc,      &pb.cpar,     &pb.cp_from, &pb.cp_to,
                      &pb.cp_ver,  &pb.cp_chk,  &pb.pair_cp, &pb.rw_from, &pb.rw_to,    &pb.rw_ver,  &pb.rw_chk,
-                     &pb.pair_rw, &pb.front,   &pb.pend,   &pb.star_cnt};
+                     &pb.pair_rw, &pb.front,   &pb.pend,   &pb.star_cnt, &pb.blk};
     for (DevBuf* b : all) {
         (void)hipFree(b->p);
         b->p = nullptr;
@@ -211,11 +269,9 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
                          dev ? &dcnt->done : nullptr)))
         return rc;
     dispatch_dim(D, [&]<int DD>() {
-        hipLaunchKernelGGL((prrt_steer<DD>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx),
-                           scene_args(ctx), s.p.range, B, (const double*)ctx->x64, (const double*)pb.q.p,
-                           (const int32_t*)pb.near.p, (const double*)pb.ndist.p, (double*)pb.from.p,
-                           (double*)pb.to.p, (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p, (double*)nullptr,
-                           (const Counters*)dcnt);
+        launch_steer<DD>(ctx, s.p.range, B, (const double*)pb.q.p, (const int32_t*)pb.near.p,
+                         (const double*)pb.ndist.p, (double*)pb.from.p, (double*)pb.to.p, (uint8_t*)pb.status.p,
+                         (uint8_t*)pb.mask.p, (double*)nullptr, (const Counters*)dcnt);
     });
     MPTG_HIP(ctx, hipGetLastError());
     if (ctx->scene.kind == MPTG_SCENE_MESH &&
@@ -225,10 +281,16 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
     if ((rc = launch_link_masked(ctx, (const double*)pb.from.p, (const double*)pb.to.p, B, s.p.order, 1,
                                  (const uint8_t*)pb.mask.p, (uint8_t*)pb.verdict.p, (uint32_t*)pb.checks.p)))
         return rc;
-    hipLaunchKernelGGL(prrt_append, dim3(1), dim3(1024), 0, ctx->stream, space_args(ctx), pd, s.p.goal_radius, B,
-                       pitch32(D), (const double*)pb.to.p, (const int32_t*)pb.near.p, (uint8_t*)pb.status.p,
-                       (const uint8_t*)pb.verdict.p, (const uint32_t*)pb.checks.p, ctx->x64, ctx->x32,
-                       ctx->parent, dcnt, (int32_t*)pb.goal_buf.p, (int64_t)4096);
+    const unsigned ab = (unsigned)((B + kAppendBlock - 1) / kAppendBlock);
+    hipLaunchKernelGGL(prrt_count, dim3(ab), dim3(kAppendBlock), 0, ctx->stream, space_args(ctx), pd,
+                       s.p.goal_radius, B, (const double*)pb.to.p, (uint8_t*)pb.status.p, (const uint8_t*)pb.verdict.p,
+                       (const uint32_t*)pb.checks.p, (uint8_t*)pb.mask.p, (int32_t*)pb.blk.p, dcnt);
+    hipLaunchKernelGGL(prrt_write, dim3(ab), dim3(kAppendBlock), 0, ctx->stream, D, pitch32(D), B,
+                       (const double*)pb.to.p, (const int32_t*)pb.near.p, (const uint8_t*)pb.mask.p,
+                       (const int32_t*)pb.blk.p, ctx->x64, ctx->x32, ctx->parent, (const Counters*)dcnt,
+                       (int32_t*)pb.goal_buf.p, (int64_t)4096);
+    hipLaunchKernelGGL(prrt_commit, dim3(1), dim3(kAppendBlock), 0, ctx->stream, (int)ab, (const int32_t*)pb.blk.p,
+                       dcnt);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

@@ -102,5 +102,86 @@ __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc,
     linkmask[b] = 1;
 }
 
+// prrt_steer for arm scenes with D = nlinks: kArmLanes lanes per sample
+// share its disk tests (arm_ok_split); lane 0 of each group writes.  Same
+// outputs as prrt_steer.
+constexpr int kArmLanes = 8;
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void prrt_steer_arm(SpaceArgs sp, SceneArgs sc, double range, int B,
+                                                         const double* __restrict__ x64, const double* __restrict__ q,
+                                                         const int32_t* __restrict__ near,
+                                                         const double* ndist, double* __restrict__ from,
+                                                         double* __restrict__ to, uint8_t* __restrict__ status,
+                                                         uint8_t* __restrict__ linkmask, double* dnear,
+                                                         const Counters* __restrict__ cnt) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int b = (int)(t / kArmLanes);
+    const bool lead = (t % kArmLanes) == 0;
+    const bool inb = b < B;
+    bool need = false;
+    double s[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) s[i] = 0.0;
+    if (inb && cnt->done) {
+        if (lead) linkmask[b] = 0;
+    } else if (inb) {
+        const double d = ndist[b];
+        const int32_t nn = near[b];
+        if (d == 0 || nn < 0) {
+            if (lead) {
+                linkmask[b] = 0;
+                status[b] = ST_ZERO;
+            }
+        } else {
+            double a[D], r[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                a[i] = x64[(int64_t)nn * D + i];
+                r[i] = q[(int64_t)b * D + i];
+            }
+            if (d > range) {
+                space_interp_t<D>(sp, a, r, range / d, s);
+                if (dnear && lead) dnear[b] = space_dist_t<D>(sp, a, s);
+            } else {
+#pragma unroll
+                for (int i = 0; i < D; ++i) s[i] = r[i];
+            }
+            if (lead) {
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    from[(int64_t)b * D + i] = a[i];
+                    to[(int64_t)b * D + i] = s[i];
+                }
+            }
+            need = true;
+        }
+    }
+    const bool ok = arm_ok_split<D, kArmLanes>(sc, [&](int i) { return s[i]; }, need);
+    if (need && lead) {
+        status[b] = ok ? ST_LINK_FAIL : ST_INVALID;  // ST_LINK_FAIL until link() says otherwise
+        linkmask[b] = ok ? 1 : 0;
+    }
+}
+
+// Launches the steer kernel for a batch of B samples (arm scenes with
+// D = nlinks in {5, 10, 20}: the split-lane kernel).
+template <int D>
+void launch_steer(mptg_ctx* ctx, double range, int B, const double* q, const int32_t* near, const double* ndist,
+                  double* from, double* to, uint8_t* status, uint8_t* linkmask, double* dnear, const Counters* cnt) {
+    if constexpr (D == 5 || D == 10 || D == 20) {
+        if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks == D) {
+            const unsigned g = (unsigned)(((int64_t)B * kArmLanes + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL((prrt_steer_arm<D>), dim3(g), dim3(kBlock), 0, ctx->stream, space_args(ctx),
+                               scene_args(ctx), range, B, (const double*)ctx->x64, q, near, ndist, from, to, status,
+                               linkmask, dnear, cnt);
+            return;
+        }
+    }
+    const unsigned gb = (unsigned)((B + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((prrt_steer<D>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx), scene_args(ctx),
+                       range, B, (const double*)ctx->x64, q, near, ndist, from, to, status, linkmask, dnear, cnt);
+}
+
 }  // namespace
 }  // namespace mptg

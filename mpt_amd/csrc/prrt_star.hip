@@ -594,11 +594,9 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         MPTG_HIP(ctx, hipMemcpyAsync(s.dnear.p, s.ndist.p, sizeof(double) * B, hipMemcpyDeviceToDevice,
                                      ctx->stream));
         dispatch_dim(D, [&]<int DD>() {
-            hipLaunchKernelGGL((prrt_steer<DD>), dim3(gb), dim3(kBlock), 0, ctx->stream, space_args(ctx),
-                               scene_args(ctx), s.p.range, B, (const double*)ctx->x64, (const double*)s.q.p,
-                               (const int32_t*)s.near.p, (const double*)s.ndist.p, (double*)s.from.p,
-                               (double*)s.to.p, (uint8_t*)s.status.p, (uint8_t*)s.mask.p, (double*)s.dnear.p,
-                               (const Counters*)dcnt);
+            launch_steer<DD>(ctx, s.p.range, B, (const double*)s.q.p, (const int32_t*)s.near.p,
+                             (const double*)s.ndist.p, (double*)s.from.p, (double*)s.to.p, (uint8_t*)s.status.p,
+                             (uint8_t*)s.mask.p, (double*)s.dnear.p, (const Counters*)dcnt);
         });
         MPTG_HIP(ctx, hipGetLastError());
         if (ctx->scene.kind == MPTG_SCENE_MESH &&
@@ -724,7 +722,7 @@ int mptg_prrt_star_init(mptg_ctx* ctx, const mptg_prrt_params* p, const mptg_prr
 }
 
 int mptg_prrt_costs(mptg_ctx* ctx, int64_t first, int64_t n, double* h_cost) {
-    int rc = check_device_ready(ctx);
+    int rc = check_device(ctx);
     if (rc) return rc;
     if (!ctx->prrt.star || !ctx->star_ok) return fail(ctx, MPTG_E_STATE, "the tree has no PRRT* path costs");
     if (first < 0 || n < 0 || first + n > ctx->size) return fail(ctx, MPTG_E_RANGE, "read beyond tree size");

@@ -411,9 +411,145 @@ __device__ __forceinline__ bool arm_ok_wave(const SceneArgs& sc, Coord&& coord, 
     return ok;
 }
 
+// arm_ok_fn for one lane on its own (states unrelated to their neighbours'):
+// per group, only the disks whose cx lies within (group half-length + the
+// largest r + R) (1 + 2^-40) + 1e-12 of the lane's group-circle centre can
+// pass the group test; they form one contiguous run of the x-sorted disks,
+// found through the bucket index.  Every other disk fails arm_ok_fn's group
+// test (the margin dominates the rounding of dx*dx + dy*dy > R*R for
+// coordinates below 10^3), so the verdict equals arm_ok_fn's.
+template <int D, typename Coord>
+__device__ __forceinline__ bool arm_ok_lane(const SceneArgs& sc, Coord&& coord) {
+    constexpr int G = (D + kArmGroup - 1) / kArmGroup;
+    const double* ix = sc.arm_index;
+    const int32_t* starts = reinterpret_cast<const int32_t*>(ix + 4);
+    double px = 0.0, py = 0.0, angle = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        constexpr int GS = kArmGroup;
+        const int i0 = g * GS;
+        const int n = (i0 + GS <= D) ? GS : D - i0;
+        double jx[GS + 1], jy[GS + 1];
+        jx[0] = px;
+        jy[0] = py;
+#pragma unroll
+        for (int k = 0; k < GS; ++k) {
+            if (k < n) {
+                angle += coord(i0 + k);
+                double s, c;
+                pm::sincos(angle, &s, &c);
+                double L = sc.lengths[i0 + k];
+                jx[k + 1] = jx[k] + L * c;
+                jy[k + 1] = jy[k] + L * s;
+            }
+        }
+        double last_x = jx[0], last_y = jy[0];
+#pragma unroll
+        for (int k = 1; k <= GS; ++k)
+            if (k <= n) {
+                last_x = jx[k];
+                last_y = jy[k];
+            }
+        const double mx = (jx[0] + last_x) / 2, my = (jy[0] + last_y) / 2;
+        const double rho = sc.arm_lim[D + g];
+        const double Rmax = (rho + ix[2]) * (1.0 + 0x1.0p-40) + 1e-12;
+        const int j0 = starts[arm_bucket(mx - Rmax, ix[0], ix[1])];
+        const int j1 = starts[arm_bucket(mx + Rmax, ix[0], ix[1]) + 1];
+        for (int j = j0; j < j1; ++j) {
+            const ArmDisk dk = sc.arm_disks[j];
+            double dx = dk.cx - mx, dy = dk.cy - my;
+            double R = rho + dk.rt;
+            if (dx * dx + dy * dy > R * R) continue;
+#pragma unroll
+            for (int k = 0; k < GS; ++k) {
+                if (k < n) {
+                    double vx = jx[k + 1] - jx[k], vy = jy[k + 1] - jy[k];
+                    if (!disk_clear(dk, jx[k], jy[k], jx[k + 1], jy[k + 1], vx, vy, sc.arm_lim[i0 + k]))
+                        return false;
+                }
+            }
+        }
+        px = last_x;
+        py = last_y;
+    }
+    return true;
+}
+
+// arm_ok_lane with L lanes per state: the L lanes of an aligned group of the
+// wave check the same state, each a 1/L share of every group's disk run
+// (disks j0 + u, j0 + u + L, ...), and OR their hits after each group of
+// links.  Same disks tested as arm_ok_lane, so the same verdict.  Every lane
+// of the wave must call it (the OR is a shuffle); lanes with active == false
+// test no disk and return true.
+template <int D, int L, typename Coord>
+__device__ __forceinline__ bool arm_ok_split(const SceneArgs& sc, Coord&& coord, bool active) {
+    static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L must be a power of two up to 64");
+    constexpr int G = (D + kArmGroup - 1) / kArmGroup;
+    const int u = (int)(threadIdx.x & 63) & (L - 1);
+    const double* ix = sc.arm_index;
+    const int32_t* starts = reinterpret_cast<const int32_t*>(ix + 4);
+    bool ok = true;
+    double px = 0.0, py = 0.0, angle = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        constexpr int GS = kArmGroup;
+        const int i0 = g * GS;
+        const int n = (i0 + GS <= D) ? GS : D - i0;
+        double jx[GS + 1], jy[GS + 1];
+        jx[0] = px;
+        jy[0] = py;
+#pragma unroll
+        for (int k = 0; k < GS; ++k) {
+            if (k < n) {
+                angle += coord(i0 + k);
+                double s, c;
+                pm::sincos(angle, &s, &c);
+                double L_ = sc.lengths[i0 + k];
+                jx[k + 1] = jx[k] + L_ * c;
+                jy[k + 1] = jy[k] + L_ * s;
+            }
+        }
+        double last_x = jx[0], last_y = jy[0];
+#pragma unroll
+        for (int k = 1; k <= GS; ++k)
+            if (k <= n) {
+                last_x = jx[k];
+                last_y = jy[k];
+            }
+        const double mx = (jx[0] + last_x) / 2, my = (jy[0] + last_y) / 2;
+        const double rho = sc.arm_lim[D + g];
+        const double Rmax = (rho + ix[2]) * (1.0 + 0x1.0p-40) + 1e-12;
+        int j0 = 0, j1 = 0;
+        if (active && ok) {
+            j0 = starts[arm_bucket(mx - Rmax, ix[0], ix[1])];
+            j1 = starts[arm_bucket(mx + Rmax, ix[0], ix[1]) + 1];
+        }
+        bool hit = false;
+        for (int j = j0 + u; j < j1 && !hit; j += L) {
+            const ArmDisk dk = sc.arm_disks[j];
+            double dx = dk.cx - mx, dy = dk.cy - my;
+            double R = rho + dk.rt;
+            if (dx * dx + dy * dy > R * R) continue;
+#pragma unroll
+            for (int k = 0; k < GS; ++k) {
+                if (k < n && !hit) {
+                    double vx = jx[k + 1] - jx[k], vy = jy[k + 1] - jy[k];
+                    if (!disk_clear(dk, jx[k], jy[k], jx[k + 1], jy[k + 1], vx, vy, sc.arm_lim[i0 + k])) hit = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < L; o <<= 1) hit = (__shfl_xor(hit ? 1 : 0, o) != 0) || hit;
+        ok = ok && !hit;
+        px = last_x;
+        py = last_y;
+    }
+    return ok;
+}
+
 template <int D>
 __device__ __forceinline__ bool arm_ok(const SceneArgs& sc, const double* q) {
-    return arm_ok_fn<D>(sc, [&](int i) { return q[i]; });
+    return arm_ok_lane<D>(sc, [&](int i) { return q[i]; });
 }
 
 // Validity with the state size known at compile time (D > 0) or at run time

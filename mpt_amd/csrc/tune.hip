@@ -137,17 +137,17 @@ int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep
     if (!s.init) return fail(ctx, MPTG_E_STATE, "mptg_prrt_init must be called first");
     if (s.star) return fail(ctx, MPTG_E_STATE, "PRRT* enqueues its iterations one by one: nothing to tune");
     if (ctx->size == 0) return fail(ctx, MPTG_E_EMPTY, "there are no valid initial states");
-    // each candidate: one call that captures (max(G,1) iterations), then the
-    // same number of timed iterations for every candidate; two passes in
-    // opposite orders, best of the two, so tree growth favours none
+    // each candidate per round: one call that captures (max(G,1)
+    // iterations), then kTimedIters timed iterations; rounds alternate the
+    // candidate order so that tree growth favours none; median over rounds
     const std::vector<int> cand = {0, 2, 4, 8, 16};
-    constexpr int kTimedIters = 16;
-    std::vector<double> best(cand.size(), -1.0);
+    constexpr int kTimedIters = 16, kRounds = 3;
+    std::vector<std::vector<double>> per(cand.size());
     const int32_t keep = s.graph_iters;
     bool stop = false;
-    for (int pass = 0; pass < 2 && !stop; ++pass) {
+    for (int round = 0; round < kRounds && !stop; ++round) {
         for (size_t j = 0; j < cand.size() && !stop; ++j) {
-            const size_t i = pass == 0 ? j : cand.size() - 1 - j;
+            const size_t i = round % 2 == 0 ? j : cand.size() - 1 - j;
             s.graph_iters = cand[i];
             if ((rc = mptg_prrt_run(ctx, target_size, std::max(cand[i], 1), nullptr))) break;
             const int64_t it0 = s.iteration;
@@ -155,13 +155,17 @@ int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep
             if ((rc = mptg_prrt_run(ctx, target_size, kTimedIters, nullptr))) break;
             const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             const int64_t ran = s.iteration - it0;
-            if (ran > 0) {
-                const double per = sec * 1e3 / (double)ran;
-                if (best[i] < 0 || per < best[i]) best[i] = per;
-            }
+            if (ran > 0) per[i].push_back(sec * 1e3 / (double)ran);
             stop = ctx->size >= target_size;
         }
+        if (rc) break;
     }
+    std::vector<double> best(cand.size(), -1.0);
+    for (size_t i = 0; i < cand.size(); ++i)
+        if (!per[i].empty()) {
+            std::sort(per[i].begin(), per[i].end());
+            best[i] = per[i][per[i].size() / 2];
+        }
     if (rc) {
         s.graph_iters = keep;
         return rc;

@@ -39,6 +39,21 @@ constexpr int64_t kScoreSlots = 512;  // surrogate scores per edge
 constexpr int64_t kMaxDmvWaypoints = (int64_t)1 << 24;
 constexpr int kMaxBisectDepth = 30;
 
+// valid() of n arm states, 8 lanes per state (arm_ok_split).
+template <int D>
+__global__ __launch_bounds__(kBlock) void valid_kernel_arm(SceneArgs sc, const double* __restrict__ q, int64_t n,
+                                                           uint8_t* __restrict__ ok) {
+    constexpr int L = 8;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = t / L;
+    const bool active = i < n;
+    double s[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) s[j] = active ? q[i * D + j] : 0.0;
+    const bool v = arm_ok_split<D, L>(sc, [&](int j) { return s[j]; }, active);
+    if (active && (t % L) == 0) ok[i] = v ? 1 : 0;
+}
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void valid_kernel(SceneArgs sc, int dim, const double* __restrict__ q,
                                                        int64_t n, uint8_t* __restrict__ ok) {
@@ -401,6 +416,17 @@ int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
         return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
     if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
     if (ctx->scene.kind == MPTG_SCENE_MESH) return launch_mesh_valid(ctx, d_q, n, d_ok);
+    if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks == ctx->dim &&
+        (ctx->dim == 5 || ctx->dim == 10 || ctx->dim == 20)) {
+        const unsigned g = (unsigned)((n * 8 + kBlock - 1) / kBlock);
+        switch (ctx->dim) {
+            case 5: hipLaunchKernelGGL(valid_kernel_arm<5>, dim3(g), dim3(kBlock), 0, ctx->stream, scene_args(ctx), d_q, n, d_ok); break;
+            case 10: hipLaunchKernelGGL(valid_kernel_arm<10>, dim3(g), dim3(kBlock), 0, ctx->stream, scene_args(ctx), d_q, n, d_ok); break;
+            default: hipLaunchKernelGGL(valid_kernel_arm<20>, dim3(g), dim3(kBlock), 0, ctx->stream, scene_args(ctx), d_q, n, d_ok); break;
+        }
+        MPTG_HIP(ctx, hipGetLastError());
+        return MPTG_OK;
+    }
     dispatch_dim(ctx->dim, [&]<int D>() {
         hipLaunchKernelGGL((valid_kernel<D>), dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            ctx->stream, scene_args(ctx), ctx->dim, d_q, n, d_ok);

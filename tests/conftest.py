@@ -9,6 +9,11 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 
+# Iteration bound of every planner run in the GPU tests: a kernel that broke
+# the tree's growth makes its test fail instead of running without end.
+GPU_ITER_CAP = 200_000
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
     config.addinivalue_line("markers", "slow: longer CPU test")

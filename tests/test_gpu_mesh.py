@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import GPU_ITER_CAP
 from mpt_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -83,7 +84,7 @@ def test_mesh_prrt_tree_parity(ctx_factory, se3):
     ctx.tree_clear()
     ctx.prrt_init(se3, 31, 256)
     ctx.prrt_add_start(se3.start)
-    st = ctx.prrt_run(1500)
+    st = ctx.prrt_run(1500, GPU_ITER_CAP)
     gn, gp = ctx.tree_read()
     on, op, og, ost = oracle.Oracle(se3).prrt(31, 256, 1500)
     assert np.array_equal(gn, on) and np.array_equal(gp, op)
@@ -100,7 +101,7 @@ def test_mesh_prrt_star_tree_parity(ctx_factory, se3):
     ctx.tree_clear()
     ctx.prrt_star_init(se3, 32, 512, k=20)
     ctx.prrt_add_start(se3.start)
-    st = ctx.prrt_run(2000)
+    st = ctx.prrt_run(2000, GPU_ITER_CAP)
     gn, gp = ctx.tree_read()
     gc = ctx.prrt_costs()
     on, op, oc, og, ost, oss = oracle.Oracle(se3).prrt_star(32, 512, 2000, k=20)

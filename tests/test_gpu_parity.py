@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import GPU_ITER_CAP
 from mpt_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -205,7 +206,7 @@ def _planner_parity(ctx, sc, seed, batch, target, max_iters=1 << 30):
     ctx.tree_clear()
     ctx.prrt_init(sc, seed, batch)
     ctx.prrt_add_start(sc.start)
-    st = ctx.prrt_run(target, max_iters)
+    st = ctx.prrt_run(target, min(max_iters, GPU_ITER_CAP))
     gn, gp = ctx.tree_read()
     on, op, og, ost = oracle.Oracle(sc).prrt(seed, batch, target, max_iters)
     assert gn.shape == on.shape

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import GPU_ITER_CAP
 from mpt_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -26,7 +27,7 @@ def _star_parity(ctx, sc, seed, batch, target, k=20, factor=1.1, max_iters=1 << 
     ctx.tree_clear()
     ctx.prrt_star_init(sc, seed, batch, k=k, rewire_factor=factor)
     ctx.prrt_add_start(sc.start)
-    st = ctx.prrt_run(target, max_iters)
+    st = ctx.prrt_run(target, min(max_iters, GPU_ITER_CAP))
     gn, gp = ctx.tree_read()
     gc = ctx.prrt_costs()
     on, op, oc, og, ost, oss = oracle.Oracle(sc).prrt_star(seed, batch, target, k=k, rewire_factor=factor,
@@ -90,10 +91,10 @@ def test_prrt_star_k_limit_and_plain_tree(ctx_factory):
     ctx.tree_insert(np.array([sc.start]))
     ctx.prrt_star_init(sc, 1, 8, k=20)
     with pytest.raises(mpt_amd.MptgError):
-        ctx.prrt_run(100)
+        ctx.prrt_run(100, GPU_ITER_CAP)
     # the reference formula outgrows 32 neighbours for D = 2 past ~1250 nodes
     ctx.tree_clear()
     ctx.prrt_star_init(sc, 1, 256, k=0)
     ctx.prrt_add_start(sc.start)
     with pytest.raises(mpt_amd.MptgError):
-        ctx.prrt_run(5000)
+        ctx.prrt_run(5000, GPU_ITER_CAP)

@@ -10,6 +10,7 @@ import pytest
 
 import mpt_amd
 import oracle
+from conftest import GPU_ITER_CAP
 from mpt_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -224,7 +225,7 @@ def test_planner_with_surrogate_order_grows_the_oracle_tree(trained):
     ctx.tree_clear()
     ctx.prrt_init(sc, 13, 256, order=mpt_amd.ORDER_SURROGATE)
     ctx.prrt_add_start(sc.start)
-    st = ctx.prrt_run(400)
+    st = ctx.prrt_run(400, GPU_ITER_CAP)
     gn, gp = ctx.tree_read()
     on, op, og, ost = oracle.Oracle(sc).prrt(13, 256, 400)
     assert np.array_equal(gn, on) and np.array_equal(gp, op)
@@ -244,7 +245,7 @@ def test_surrogate_planner_in_graph_mode_equals_step_by_step(trained):
         ctx.prrt_init(sc, 14, 512, order=mpt_amd.ORDER_SURROGATE)
         ctx.prrt_graph(g)
         ctx.prrt_add_start(sc.start)
-        st = ctx.prrt_run(1200)
+        st = ctx.prrt_run(1200, GPU_ITER_CAP)
         gn, gp = ctx.tree_read()
         res.append((gn, gp, st))
     ctx.prrt_graph(0)

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import GPU_ITER_CAP
 from mpt_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -114,7 +115,7 @@ def test_planner_tuned_while_growing_equals_oracle(ctx_factory):
     _say(" ", rep)
     assert sorted(rep["candidates"]) == [0, 2, 4, 8, 16]
     assert rep["chosen"] in rep["candidates"]
-    st = ctx.prrt_run(9000)
+    st = ctx.prrt_run(9000, GPU_ITER_CAP)
     gn, gp = ctx.tree_read()
     on, op, og, ost = oracle.Oracle(sc).prrt(19, 64, 9000)
     assert np.array_equal(gn, on) and np.array_equal(gp, op)
