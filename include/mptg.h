@@ -257,10 +257,10 @@ typedef struct mptg_tune_params {
     int32_t knn_blocks_per_cu;      /* kNN scan blocks per CU the tree split aims at (1..256) */
     int32_t dmv_long_waves_per_cu;  /* grid of the persistent DMV long-edge kernel (1..256) */
     int32_t mesh_waves_per_cu;      /* grid of the persistent mesh kernels (1..256) */
-    int32_t _pad;
+    int32_t arm_tail_waves;         /* waves sharing one arm edge's continuation rounds (1..8) */
 } mptg_tune_params;
 
-enum { MPTG_TUNE_KNN = 0, MPTG_TUNE_DMV_LONG = 1, MPTG_TUNE_MESH = 2, MPTG_TUNE_GRAPH = 3 };
+enum { MPTG_TUNE_KNN = 0, MPTG_TUNE_DMV_LONG = 1, MPTG_TUNE_MESH = 2, MPTG_TUNE_GRAPH = 3, MPTG_TUNE_ARM_TAIL = 4 };
 
 /* What one tuning call tried and chose: the knob, each candidate value and
    its median time (ms per call; per planner iteration for MPTG_TUNE_GRAPH). */
@@ -279,9 +279,10 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p);
    candidate split and keeps the fastest (replaces the nigh KD-tree's fixed
    traversal, src/mpt/impl/prrt/prrt.hpp:406-409, which has nothing to tune). */
 int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_tune_report* rep);
-/* Times link() of n device edges for each candidate grid of the scene's
-   persistent link kernel (DMV long edges, or the mesh kernels) and keeps the
-   fastest; other scenes have no such kernel (MPTG_E_STATE). */
+/* Times link() of n device edges (reference order) for each candidate of the
+   scene's link launch shape -- the grid of the persistent DMV long-edge or
+   mesh kernel, or the waves per edge of the arm continuation -- and keeps
+   the fastest; the per-edge bisection kernels have none (MPTG_E_STATE). */
 int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
                        mptg_tune_report* rep);
 /* Plain PRRT: grows the tree (these iterations count, as in mptg_prrt_run)

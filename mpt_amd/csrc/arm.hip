@@ -203,6 +203,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
             verdict[e] = 1;
             checks[e] = base + W;
         } else {
+            checks[e] = 0xffffffffu;  // the continuation's first-failure slot
             cont[atomicAdd(n_cont, 1u)] = e;
         }
     }
@@ -215,27 +216,36 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     }
 }
 
-// Continuation: one wave per listed edge, 64 ranks per round from rank 4.
+// Continuation: P waves per listed edge (P = waves_per_edge), 64 ranks per
+// round from rank 4; wave p of an edge takes rounds p, p + P, p + 2P, ...
+// A wave stops at its first failing waypoint and lowers the edge's slot in
+// `checks` (set to ~0 by the packed round) to that rank with atomicMin; it
+// also stops when the slot already holds a rank below its next round.  The
+// first failing rank of the edge lies in one wave's rounds, and that wave
+// cannot stop before it (every rank the others report is larger), so the
+// slot ends at exactly the first failing rank of the chosen order.  No wave
+// waits for another: the slot is read and lowered, never awaited.
 template <int D, bool COUNT, int ORD>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
     const double* __restrict__ from, const double* __restrict__ to, double tol, SceneArgs sc,
     const uint16_t* __restrict__ ranked, const int64_t* __restrict__ cont, const uint32_t* __restrict__ n_cont,
-    uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks, unsigned long long* __restrict__ work) {
+    int waves_per_edge, uint32_t* __restrict__ checks, unsigned long long* __restrict__ work) {
     // no block-wide barriers: waves leave independently
     ArmWork wk{0, 0, 0, 0, 0};
     const int lane = lane_id();
     const int64_t widx = (int64_t)blockIdx.x * kWavesPerBlock + uniform((int)(threadIdx.x >> 6));
-    if (widx < (int64_t)*n_cont) {
-        const int64_t e = cont[widx];
+    const int64_t item = widx / waves_per_edge;
+    const uint32_t P = (uint32_t)waves_per_edge, p = (uint32_t)(widx % waves_per_edge);
+    if (item < (int64_t)*n_cont) {
+        const int64_t e = cont[item];
         const double* a = from + e * D;  // wave-uniform: scalar loads
         const double* b = to + e * D;
         int h = 0;
         bool regular = false;
         bisect_shape<D>(a, b, tol, &h, &regular);  // listed edges are regular
         const uint32_t W = (1u << h) - 1u;
-        const uint32_t base = 2;
-        bool decided = false;
-        for (uint32_t c0 = kHeadRanks; c0 < W && !decided; c0 += kWave) {
+        for (uint32_t c0 = kHeadRanks + kWave * p; c0 < W; c0 += kWave * P) {
+            if (__hip_atomic_load(&checks[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < c0) break;  // uniform
             const uint32_t r = c0 + (uint32_t)lane;
             // every lane takes part in the wave's disk culling; lanes past
             // the last rank recompute the last waypoint and do not count
@@ -247,16 +257,9 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
                 !arm_ok_wave<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, active, &wk);
             const uint64_t m = __ballot(bad && active);
             if (m) {
-                if (lane == 0) {
-                    verdict[e] = 0;
-                    checks[e] = base + c0 + (uint32_t)__builtin_ctzll(m) + 1u;
-                }
-                decided = true;
+                if (lane == 0) atomicMin(&checks[e], c0 + (uint32_t)__builtin_ctzll(m));
+                break;
             }
-        }
-        if (!decided && lane == 0) {
-            verdict[e] = 1;
-            checks[e] = base + W;
         }
     }
     if constexpr (COUNT) {
@@ -265,6 +268,29 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
         atomicAdd(&work[2], (unsigned long long)wk.group);
         atomicAdd(&work[3], (unsigned long long)wk.far);
         atomicAdd(&work[4], (unsigned long long)wk.exact);
+    }
+}
+
+// Verdicts of the listed edges from their slots: ~0 = no failing waypoint.
+template <int D>
+__global__ __launch_bounds__(256) void arm_link_finish(const double* __restrict__ from, const double* __restrict__ to,
+                                                       double tol, const int64_t* __restrict__ cont,
+                                                       const uint32_t* __restrict__ n_cont,
+                                                       uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)*n_cont) return;
+    const int64_t e = cont[i];
+    const uint32_t base = 2;
+    const uint32_t first = checks[e];
+    if (first != 0xffffffffu) {
+        verdict[e] = 0;
+        checks[e] = base + first + 1u;
+    } else {
+        int h = 0;
+        bool regular = false;
+        bisect_shape<D>(from + e * D, to + e * D, tol, &h, &regular);
+        verdict[e] = 1;
+        checks[e] = base + ((1u << h) - 1u);
     }
 }
 
@@ -282,9 +308,13 @@ static void launch_arm_phases(mptg_ctx* ctx, const double* from, const double* t
                        dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, ranked, verdict,
                        checks, irr, nirr, cont, ncont, work);
     phase_mark(ctx, 3);
-    hipLaunchKernelGGL((arm_link_tail<D, COUNT, ORD>), dim3((unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock)),
-                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, tol, sa, ranked, cont, ncont, verdict,
-                       checks, work);
+    int P = ctx->tune.arm_tail_waves;
+    if ((n * P + kWavesPerBlock - 1) / kWavesPerBlock > (int64_t)INT32_MAX) P = 1;  // grid limit
+    hipLaunchKernelGGL((arm_link_tail<D, COUNT, ORD>), dim3((unsigned)((n * P + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, tol, sa, ranked, cont, ncont, P, checks,
+                       work);
+    hipLaunchKernelGGL((arm_link_finish<D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, from, to,
+                       tol, cont, ncont, verdict, checks);
 }
 
 template <int D>

@@ -123,8 +123,9 @@ struct mptg_ctx {
     // graph is replayed only while the epoch it was captured at holds
     uint64_t epoch = 0;
     // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
-    // persistent DMV long-edge kernel and of the mesh kernels
-    mptg_tune_params tune{8, 16, 32, 0};
+    // persistent DMV long-edge kernel and of the mesh kernels, waves per edge
+    // of the arm continuation
+    mptg_tune_params tune{8, 16, 32, 2};
     hipEvent_t phase_ev[6] = {};
 };
 

@@ -90,8 +90,9 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
     if (!p) return fail(ctx, MPTG_E_ARG, "null argument");
     if (!in_range(p->knn_blocks_per_cu) || !in_range(p->dmv_long_waves_per_cu) || !in_range(p->mesh_waves_per_cu))
         return fail(ctx, MPTG_E_RANGE, "launch shapes must be in 1..256");
+    if (p->arm_tail_waves < 1 || p->arm_tail_waves > 8)
+        return fail(ctx, MPTG_E_RANGE, "arm continuation waves per edge must be in 1..8");
     ctx->tune = *p;
-    ctx->tune._pad = 0;
     return MPTG_OK;
 }
 
@@ -114,8 +115,10 @@ int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     int rc = check_device_ready(ctx);
     if (rc) return rc;
     if (n <= 0 || !d_from || !d_to) return fail(ctx, MPTG_E_ARG, "autotune_link needs device edges");
-    if (ctx->link_mode != MPTG_LINK_DMV || ctx->force_generic)
-        return fail(ctx, MPTG_E_STATE, "only the discrete-motion-validator link has a persistent kernel to tune");
+    const bool arm_wave = ctx->scene.kind == MPTG_SCENE_ARM && ctx->link_mode == MPTG_LINK_BISECT &&
+                          ctx->scene.nlinks == ctx->dim && (ctx->dim == 5 || ctx->dim == 10 || ctx->dim == 20);
+    if (ctx->force_generic || !(ctx->link_mode == MPTG_LINK_DMV || arm_wave))
+        return fail(ctx, MPTG_E_STATE, "this scene's link kernels have no launch shape to tune");
     DevBuf& out = ctx->scratch[11];
     if ((rc = ensure(ctx, out, (sizeof(uint32_t) + 1) * (size_t)n))) return rc;
     uint32_t* checks = (uint32_t*)out.p;
@@ -123,6 +126,8 @@ int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     auto call = [&] {
         return launch_link_masked(ctx, d_from, d_to, n, MPTG_ORDER_REFERENCE, 0, nullptr, verdict, checks);
     };
+    if (arm_wave)
+        return sweep(ctx, MPTG_TUNE_ARM_TAIL, {1, 2, 4, 8}, [&](int v) { ctx->tune.arm_tail_waves = v; }, call, rep);
     if (ctx->scene.kind == MPTG_SCENE_MESH)
         return sweep(ctx, MPTG_TUNE_MESH, {8, 16, 32, 64, 128}, [&](int v) { ctx->tune.mesh_waves_per_cu = v; },
                      call, rep);

@@ -103,6 +103,34 @@ def test_mesh_grid_every_shape_is_exact(ctx_factory):
     _check_report(rep, [8, 16, 32, 64, 128])
 
 
+def test_arm_continuation_every_wave_count_is_exact(ctx_factory):
+    """Waves per edge of the arm continuation (1 = one wave walks all the
+    edge's rounds): verdicts and first-failure check counts equal the
+    oracle's for every count, from endpoints checked or vouched for, and
+    the surrogate order's verdicts equal the reference order's."""
+    _say("tune arm tail")
+    import mpt_amd
+    ctx = ctx_factory()
+    sc = scenes.arm_scene()
+    ctx.scene(sc)
+    rng = np.random.default_rng(74)
+    a = rng.uniform(-math.pi, math.pi, (3000, 20))
+    b = np.clip(a + rng.normal(0, 0.5, (3000, 20)), -math.pi, math.pi)
+    b[-800:] = rng.uniform(-math.pi, math.pi, (800, 20))  # long edges: 8 rounds past the packed one
+    o = oracle.Oracle(sc)
+    want = {kv: o.link(a, b, known_valid=kv, threads=16) for kv in (False, True)}
+    for waves in (1, 2, 3, 5, 8):
+        ctx.tune_set(arm_tail_waves=waves)
+        for kv in (False, True):
+            gv, gc = ctx.link(a, b, endpoints_known_valid=kv)
+            assert np.array_equal(gv, want[kv][0]) and np.array_equal(gc, want[kv][1]), (waves, kv)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    rep = ctx.autotune_link(da.ptr, db.ptr, a.shape[0])
+    _say(" ", rep)
+    _check_report(rep, [1, 2, 4, 8])
+    assert ctx.tune_get()["arm_tail_waves"] == rep["chosen"]
+
+
 def test_planner_tuned_while_growing_equals_oracle(ctx_factory):
     _say("tune prrt")
     sc = scenes.boxes_scene()
@@ -130,3 +158,5 @@ def test_tune_set_rejects_out_of_range(ctx_factory):
         ctx.tune_set(knn_blocks_per_cu=0)
     with pytest.raises(Exception):
         ctx.tune_set(mesh_waves_per_cu=257)
+    with pytest.raises(Exception):
+        ctx.tune_set(arm_tail_waves=9)
