@@ -126,6 +126,14 @@ def bench_arm(ctx, args, timer):
     sc = scenes.arm_scene()
     B = 8192
     ctx.scene(sc)
+    log("  link autotune ...")
+    # on planner edges from a tree of the timed runs' size (grown untimed)
+    _, nodes_t, _ = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B, 0)
+    ta, tb = surrogate.planner_edges(ctx, sc, nodes_t, 8192, seed=98)
+    dta, dtb = ctx.to_device(ta), ctx.to_device(tb)
+    tune_link = ctx.autotune_link(dta.ptr, dtb.ptr, ta.shape[0])
+    dta.free()
+    dtb.free()
     log("  graph autotune ...")
     tune = tune_planner(ctx, sc, args, B)
     G = tune["chosen"]
@@ -208,7 +216,7 @@ def bench_arm(ctx, args, timer):
             "reference_equivalent_tflops": (FLOP_FK_LINK * wk_s["ref_fk"] + FLOP_REF_TEST * wk_s["ref_tests"])
             / t_kernel / 1e12}
     res = {"plain": plain, "surrogate": sur, "plain_step_by_step": plain_steps,
-           "surrogate_step_by_step": sur_steps, "graph_autotune": tune,
+           "surrogate_step_by_step": sur_steps, "graph_autotune": tune, "tail_autotune": tune_link,
            "trees_identical": same_tree, "training": train,
            "edge_batch": edge_batch,
            "planner_gain": sur["edge_checks_per_s"] / plain["edge_checks_per_s"]}
