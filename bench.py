@@ -367,13 +367,14 @@ def kernels_only(ctx, args):
     src.zero()
     timer = Timer(ctx)
     reps = max(1, args.steps)
+    tune = ctx.autotune_link(da.ptr, db.ptr, ne)  # the continuation's waves per edge, as the headline run
     res = {"copy_bytes": nbytes,
            "copy_ms": timer.time(lambda: ctx.diag_copy(dst.ptr, src.ptr, nbytes), reps) * 1e3}
     for name, order in (("plain", mpt_amd.ORDER_REFERENCE), ("surrogate", mpt_amd.ORDER_SURROGATE)):
         res[f"link_{name}_ms"] = timer.time(
             lambda order=order: ctx.link_dev(da.ptr, db.ptr, ne, order, True, v.ptr, c.ptr), reps) * 1e3
     res["score_ms"] = timer.time(lambda: ctx.surrogate_score_edges(da.ptr, db.ptr, ne, scores.ptr), reps) * 1e3
-    print(json.dumps({"mode": "kernels", "edges": ne, "reps": reps, **res}))
+    print(json.dumps({"mode": "kernels", "edges": ne, "reps": reps, "tail_autotune": tune, **res}))
 
 
 def main():
