@@ -88,15 +88,19 @@ __device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[NE], int lane)
     }
 }
 
-// Ranked ordinals of one edge per wave: sort the edge's waypoint scores
-// (descending, ties by reference ordinal) and write the reference ordinal of
-// every rank to ranked[e * kScoreStride + rank].
+// Ranked ordinals of one edge per wave (edge list[item] for item <
+// *n_list): reference ordinals 0 .. lead-1 first, then the others by
+// descending score (ties by reference ordinal); writes the reference ordinal
+// of every rank to ranked[e * kScoreStride + rank].
 __global__ __launch_bounds__(64) void rank_scores(const double* __restrict__ from, const double* __restrict__ to,
-                                                  int64_t n, int dim, const uint8_t* __restrict__ mask, double tol,
-                                                  const float* __restrict__ scores, uint16_t* __restrict__ ranked) {
-    const int64_t e = blockIdx.x;
+                                                  const int64_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ n_list, int dim, double tol,
+                                                  uint32_t lead, const float* __restrict__ scores,
+                                                  uint16_t* __restrict__ ranked) {
+    const int64_t item = blockIdx.x;
     const int lane = threadIdx.x;
-    if (e >= n || (mask && !mask[e])) return;
+    if (item >= (int64_t)*n_list) return;
+    const int64_t e = list[item];
     int h = 0;
     bool regular = false;
     // shape from the widest coordinate (same rule as the link kernels)
@@ -119,7 +123,8 @@ __global__ __launch_bounds__(64) void rank_scores(const double* __restrict__ fro
 #pragma unroll
     for (int k = 0; k < kScoreStride / kWave; ++k) {
         const uint32_t o = (uint32_t)(k * kWave + lane);
-        key[k] = o < W ? score_key(scores[e * kScoreStride + o], o) : ~0ull;
+        // led ordinals sort first: every score key has a nonzero high word
+        key[k] = o < W ? (o < lead ? (uint64_t)o : score_key(scores[e * kScoreStride + o], o)) : ~0ull;
     }
     wave_bitonic_sort<kScoreStride / kWave>(key, lane);
 #pragma unroll
@@ -140,12 +145,13 @@ constexpr int kHeadRanks = 4;                     // ranks per edge in the packe
 constexpr int kHeadEdges = kWave / kHeadRanks;    // edges per wave in the packed round
 
 // Packed round: 16 edges per wave, 4 lanes per edge.  Endpoint checks (when
-// not vouched for), the tree shape, then ranks 0..3 of the chosen order.
+// not vouched for), the tree shape, then reference ranks 0..3 (the surrogate
+// order leads with them too, so that no edge this round decides is scored).
 // Edges still undecided go to the continuation list.
 template <int D, bool COUNT, int ORD>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const double* __restrict__ from, const double* __restrict__ to, int64_t n, int known_valid,
-    const uint8_t* __restrict__ mask, double tol, SceneArgs sc, const uint16_t* __restrict__ ranked,
+    const uint8_t* __restrict__ mask, double tol, SceneArgs sc,
     uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks, int64_t* __restrict__ irregular,
     uint32_t* __restrict__ n_irregular, int64_t* __restrict__ cont, uint32_t* __restrict__ n_cont,
     unsigned long long* __restrict__ work) {
@@ -190,7 +196,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const bool act = live && (uint32_t)rk < W;
     uint32_t path = 0;
     int depth = 0;
-    if (act) preorder_path(h, ordinal_at<ORD>(ranked, e, (uint32_t)rk), &path, &depth);
+    if (act) preorder_path(h, (uint32_t)rk, &path, &depth);  // reference ranks in both orders
     const bool bad =
         act && !arm_ok_wave<D, COUNT>(sc, [&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; },
                                       act, &wk);
@@ -297,16 +303,31 @@ __global__ __launch_bounds__(256) void arm_link_finish(const double* __restrict_
 }  // namespace
 
 template <int D, bool COUNT, int ORD>
-static void launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv,
-                              const uint8_t* mask, const uint16_t* ranked, uint8_t* verdict, uint32_t* checks,
-                              int64_t* irr, uint32_t* nirr, int64_t* cont, uint32_t* ncont,
-                              unsigned long long* work) {
+static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv,
+                             const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
+                             int64_t* cont, uint32_t* ncont, unsigned long long* work) {
     const double tol = ctx->link_param;
     const SceneArgs sa = scene_args(ctx);
     const int64_t per_block = (int64_t)kWavesPerBlock * kHeadEdges;
     hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD>), dim3((unsigned)((n + per_block - 1) / per_block)),
-                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, ranked, verdict,
-                       checks, irr, nirr, cont, ncont, work);
+                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, verdict, checks,
+                       irr, nirr, cont, ncont, work);
+    phase_mark(ctx, 1);
+    const uint16_t* ranked = nullptr;
+    if constexpr (ORD == MPTG_ORDER_SURROGATE) {
+        int rc;
+        if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreStride * n))) return rc;
+        if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kScoreStride * n))) return rc;
+        float* scores = (float*)ctx->scratch[8].p;
+        if ((rc = surrogate_score_list(ctx, from, to, n, nullptr, cont, ncont, scores))) return rc;
+        phase_mark(ctx, 2);
+        hipLaunchKernelGGL(rank_scores, dim3((unsigned)n), dim3(kWave), 0, ctx->stream, from, to,
+                           (const int64_t*)cont, (const uint32_t*)ncont, D, tol, (uint32_t)kHeadRanks,
+                           (const float*)scores, (uint16_t*)ctx->scratch[9].p);
+        ranked = (const uint16_t*)ctx->scratch[9].p;
+    } else {
+        phase_mark(ctx, 2);
+    }
     phase_mark(ctx, 3);
     int P = ctx->tune.arm_tail_waves;
     if ((n * P + kWavesPerBlock - 1) / kWavesPerBlock > (int64_t)INT32_MAX) P = 1;  // grid limit
@@ -315,28 +336,28 @@ static void launch_arm_phases(mptg_ctx* ctx, const double* from, const double* t
                        work);
     hipLaunchKernelGGL((arm_link_finish<D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, from, to,
                        tol, cont, ncont, verdict, checks);
+    return MPTG_OK;
 }
 
 template <int D>
-static void launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv, const uint8_t* mask,
-                         const uint16_t* ranked, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
-                         int64_t* cont, uint32_t* ncont, unsigned long long* work) {
-    if (ranked) {
-        if (work) launch_arm_phases<D, true, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
-        else launch_arm_phases<D, false, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
-    } else {
-        if (work) launch_arm_phases<D, true, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
-        else launch_arm_phases<D, false, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work);
+static int launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv, const uint8_t* mask,
+                        int order, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr, int64_t* cont,
+                        uint32_t* ncont, unsigned long long* work) {
+    if (order == MPTG_ORDER_SURROGATE) {
+        if (work) return launch_arm_phases<D, true, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
+        return launch_arm_phases<D, false, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
     }
+    if (work) return launch_arm_phases<D, true, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
+    return launch_arm_phases<D, false, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
 }
 
-// Launches the packed round and the continuation (after ranking the
-// waypoints when scores are given); edges the kernels list as irregular are
-// left for the caller's exact per-edge kernel (irr/nirr).  Returns
-// MPTG_E_STATE when no kernel exists for this arm.
+// Launches the packed round and the continuation (scoring and ranking the
+// continued edges in between for the surrogate order); edges the kernels
+// list as irregular are left for the caller's exact per-edge kernel
+// (irr/nirr).  Returns MPTG_E_STATE when no kernel exists for this arm.
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
                     const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
-                    unsigned long long* work, const float* scores) {
+                    unsigned long long* work, int order) {
     if (ctx->dim != ctx->scene.nlinks) return MPTG_E_STATE;
     if (ctx->dim != 5 && ctx->dim != 10 && ctx->dim != 20) return MPTG_E_STATE;
     int rc;
@@ -344,19 +365,13 @@ int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t
     uint32_t* ncont = (uint32_t*)ctx->scratch[10].p;
     int64_t* cont = (int64_t*)((char*)ctx->scratch[10].p + 256);
     if ((rc = launch_zero_u32(ctx, ncont, 1))) return rc;
-    const uint16_t* ranked = nullptr;
-    if (scores) {
-        if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kScoreStride * n))) return rc;
-        hipLaunchKernelGGL(rank_scores, dim3((unsigned)n), dim3(kWave), 0, ctx->stream, from, to, n, ctx->dim, mask,
-                           ctx->link_param, scores, (uint16_t*)ctx->scratch[9].p);
-        ranked = (const uint16_t*)ctx->scratch[9].p;
-    }
-    phase_mark(ctx, 2);
+    phase_mark(ctx, 0);
     switch (ctx->dim) {
-        case 5: launch_arm_d<5>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;
-        case 10: launch_arm_d<10>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;
-        default: launch_arm_d<20>(ctx, from, to, n, known_valid, mask, ranked, verdict, checks, irr, nirr, cont, ncont, work); break;
+        case 5: rc = launch_arm_d<5>(ctx, from, to, n, known_valid, mask, order, verdict, checks, irr, nirr, cont, ncont, work); break;
+        case 10: rc = launch_arm_d<10>(ctx, from, to, n, known_valid, mask, order, verdict, checks, irr, nirr, cont, ncont, work); break;
+        default: rc = launch_arm_d<20>(ctx, from, to, n, known_valid, mask, order, verdict, checks, irr, nirr, cont, ncont, work); break;
     }
+    if (rc) return rc;
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

@@ -186,12 +186,18 @@ int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from
 // planners: states[b] with mask[b] set that collide get status[b] = invalid_code, mask[b] = 0
 int launch_mesh_filter(mptg_ctx* ctx, const double* d_states, int64_t n, uint8_t* d_mask, uint8_t* d_status,
                        int invalid_code);
+// Arm link kernels in `order`; with the surrogate order the edges left after
+// the packed round (reference ranks 0..3) are scored and ranked first.
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
                     const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
-                    unsigned long long* work = nullptr, const float* scores = nullptr);
+                    unsigned long long* work, int order);
 // surrogate scores of every waypoint of every (masked) arm edge:
 // scores[e * 512 + reference ordinal]
 int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
                           float* scores);
+// Scores of the edges list[0 .. *n_list) (device count; at most n of them),
+// written at scores[e * 512 ...] as surrogate_score_edges does.
+int surrogate_score_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                         const int64_t* list, const uint32_t* n_list, float* scores);
 
 }  // namespace mptg

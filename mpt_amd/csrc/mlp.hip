@@ -413,17 +413,27 @@ __device__ __forceinline__ void arm_features_f(const float* q, const double* len
     for (int j = 40 + 2 * (D < 12 ? D : 12); j < 64; ++j) out[j] = 0;
 }
 
+// Edges: e = item (all n, or those with mask[e] set), or e = list[item] for
+// item < *n_list when a list is given.
 template <int D>
 __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__ from, const double* __restrict__ to,
-                                                      int64_t n, const uint8_t* __restrict__ mask, double tol,
+                                                      int64_t n, const uint8_t* __restrict__ mask,
+                                                      const int64_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ n_list, double tol,
                                                       const double* __restrict__ lengths, float inv_total,
                                                       const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
                                                       const float* __restrict__ p, float* __restrict__ scores) {
     __shared__ __attribute__((aligned(16))) uint16_t buf[64 * kLdsRow];
     const int lane = threadIdx.x;
-    const int64_t e = blockIdx.x / (kScoreStride / 64);
+    const int64_t item = blockIdx.x / (kScoreStride / 64);
     const uint32_t c0 = (uint32_t)(blockIdx.x % (kScoreStride / 64)) * 64;
-    if (e >= n || (mask && !mask[e])) return;
+    int64_t e = item;
+    if (list) {
+        if (item >= (int64_t)*n_list) return;
+        e = list[item];
+    } else if (e >= n || (mask && !mask[e])) {
+        return;
+    }
     const double* a = from + e * D;
     const double* b = to + e * D;
     int h = 0;
@@ -637,6 +647,11 @@ static int sur_check(mptg_ctx* ctx, bool need_init) {
 
 int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
                           float* scores) {
+    return surrogate_score_list(ctx, from, to, n, mask, nullptr, nullptr, scores);
+}
+
+int surrogate_score_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                         const int64_t* list, const uint32_t* n_list, float* scores) {
     Surrogate& S = surrogate(ctx);
     if (!S.init) return fail(ctx, MPTG_E_STATE, "surrogate order needs an initialised surrogate");
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks)
@@ -648,9 +663,9 @@ int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, i
     const float* p = (const float*)S.params.p;
     const double* L = (const double*)ctx->scene.lengths;
     switch (ctx->dim) {
-        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
-        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
-        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
+        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
+        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
+        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
         default: return fail(ctx, MPTG_E_STATE, "surrogate order: arms of 5, 10 or 20 links");
     }
     MPTG_HIP(ctx, hipGetLastError());

@@ -33,7 +33,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTableMax = 255;  // largest n = steps-1 served by the BFS table
-constexpr int64_t kScoreSlots = 512;  // surrogate scores per edge
 // Work limits of one link() call; an edge beyond them is not checked and
 // reports verdict 0 with checks = MPTG_CHECKS_OVER_LIMIT.
 constexpr int64_t kMaxDmvWaypoints = (int64_t)1 << 24;
@@ -462,16 +461,8 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
             uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
             int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
             if ((rc = launch_zero_u32(ctx, nirr, 1))) return rc;
-            const float* scores = nullptr;
-            phase_mark(ctx, 0);
-            if (order == MPTG_ORDER_SURROGATE) {
-                if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreSlots * n))) return rc;
-                if ((rc = surrogate_score_edges(ctx, d_from, d_to, n, d_mask, (float*)ctx->scratch[8].p))) return rc;
-                scores = (const float*)ctx->scratch[8].p;
-            }
-            phase_mark(ctx, 1);
             rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks, irr, nirr, nullptr,
-                                 scores);
+                                 order);
             if (rc == MPTG_OK) {
                 phase_mark(ctx, 4);
                 // listed (irregular) edges: exact per-edge walk; usually an empty launch
@@ -524,13 +515,7 @@ int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, in
     int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
     MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 8 * sizeof(unsigned long long), ctx->stream));
     MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
-    const float* scores = nullptr;
-    if (order == MPTG_ORDER_SURROGATE) {
-        if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreSlots * n))) return rc;
-        if ((rc = surrogate_score_edges(ctx, d_from, d_to, n, nullptr, (float*)ctx->scratch[8].p))) return rc;
-        scores = (const float*)ctx->scratch[8].p;
-    }
-    rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks, irr, nirr, d_work, scores);
+    rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks, irr, nirr, d_work, order);
     if (rc) return rc == MPTG_E_STATE ? fail(ctx, rc, "no wave kernel for this arm") : rc;
     hipLaunchKernelGGL(link_bisect_work, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                        ctx->dim, scene_args(ctx), ctx->link_param, d_from, d_to, n, known_valid, d_work + 5);

@@ -92,11 +92,12 @@ int32_t orc_link(const orc_scene* sc, const double* a, const double* b, int32_t 
 void orc_link_batch(const orc_scene* sc, const double* from, const double* to, int64_t n,
                     int32_t endpoints_known_valid, uint8_t* verdict, uint32_t* checks_done,
                     int32_t nthreads);
-/* Number of valid() calls when the link's waypoints are checked in the order
- * of descending score (ties: ascending reference ordinal), after the same
- * endpoint checks.  scores[] is indexed by reference ordinal. */
+/* Number of valid() calls when the link's waypoints are checked in this
+ * order, after the same endpoint checks: reference ordinals 0 .. lead-1 first,
+ * then the others by descending score (ties: ascending reference ordinal).
+ * scores[] is indexed by reference ordinal. */
 uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const double* b,
-                                   int32_t endpoints_known_valid, const float* scores, int64_t nscores);
+                                   int32_t known_valid, const float* scores, int64_t nscores, int32_t lead);
 
 /* --- batch-synchronous PRRT ------------------------------------------------ */
 typedef struct {

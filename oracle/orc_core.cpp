@@ -399,7 +399,7 @@ void orc_link_batch(const orc_scene* sc, const double* from, const double* to, i
 }
 
 uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const double* b,
-                                   int32_t known_valid, const float* scores, int64_t nscores) {
+                                   int32_t known_valid, const float* scores, int64_t nscores, int32_t lead) {
     int d = sc->dim;
     uint32_t checks = 0;
     int nend = sc->link_mode == ORC_LINK_BISECT ? 2 : 1;
@@ -415,7 +415,10 @@ uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const d
     orc_link_waypoints(sc, a, b, wp.data(), n);
     std::vector<int64_t> order((size_t)n);
     for (int64_t i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
+    // the first `lead` ordinals keep the reference order; the rest follow by
+    // descending score (ties: ascending ordinal)
+    const int64_t l = std::min<int64_t>(std::max<int32_t>(lead, 0), n);
+    std::stable_sort(order.begin() + l, order.end(),
                      [&](int64_t x, int64_t y) { return scores[x] > scores[y]; });
     for (int64_t o : order) {
         ++checks;

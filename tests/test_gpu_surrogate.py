@@ -192,9 +192,11 @@ def test_surrogate_order_parity_and_savings(trained):
     ctx.surrogate_score_edges(da.ptr, db.ptr, m, scores.ptr)
     sc_np = scores.numpy()
     o = oracle.Oracle(sc)
+    # the surrogate order: reference ordinals 0..3 (the packed round), then
+    # the rest by descending score
     for e in range(0, m, 4):
         W = len(o.waypoints(a[e], b[e]))
-        assert c_sur[e] == o.checks_in_score_order(a[e], b[e], sc_np[e, :W], known_valid=True), e
+        assert c_sur[e] == o.checks_in_score_order(a[e], b[e], sc_np[e, :W], known_valid=True, lead=4), e
     # fused scores == generic forward on the same waypoints, up to the BF16
     # rounding of features built from FP32-interpolated waypoints
     zs, ss = [], []

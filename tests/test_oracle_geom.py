@@ -281,3 +281,14 @@ def test_score_order_counts_checks():
         perfect = (~ok).astype(np.float32)
         expect = 3 if not ok.all() else 2 + len(wp)
         assert o.checks_in_score_order(a, b, perfect, known_valid=True) == expect
+        # with a reference lead of 4 ordinals: those first, in reference
+        # order, then the rest by descending score (ties by ordinal)
+        rnd = rng.uniform(-1, 1, len(wp)).astype(np.float32)
+        rnd[::7] = rnd[0]  # ties
+        for lead in (0, 1, 4, len(wp), len(wp) + 3):
+            for s in (perfect, rnd, dec):
+                head = list(range(min(lead, len(wp))))
+                rest = sorted(range(len(head), len(wp)), key=lambda i: (-float(s[i]), i))
+                bad = [pos for pos, i in enumerate(head + rest) if not ok[i]]
+                want = 2 + (bad[0] + 1 if bad else len(wp))
+                assert o.checks_in_score_order(a, b, s, known_valid=True, lead=lead) == want, lead
