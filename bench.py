@@ -217,6 +217,7 @@ def bench_arm(ctx, args, timer):
             / t_kernel / 1e12}
     res = {"plain": plain, "surrogate": sur, "plain_step_by_step": plain_steps,
            "surrogate_step_by_step": sur_steps, "graph_autotune": tune, "tail_autotune": tune_link,
+           "launch_shapes": ctx.tune_get(),
            "trees_identical": same_tree, "training": train,
            "edge_batch": edge_batch,
            "planner_gain": sur["edge_checks_per_s"] / plain["edge_checks_per_s"]}

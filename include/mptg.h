@@ -258,9 +258,19 @@ typedef struct mptg_tune_params {
     int32_t dmv_long_waves_per_cu;  /* grid of the persistent DMV long-edge kernel (1..256) */
     int32_t mesh_waves_per_cu;      /* grid of the persistent mesh kernels (1..256) */
     int32_t arm_tail_waves;         /* waves sharing one arm edge's continuation rounds (1..8) */
+    int32_t arm_cull;               /* arm link disk culling: 0 one walk per wave over the wave's
+                                       box, 1 each lane over its own grid cells */
+    int32_t _pad;
 } mptg_tune_params;
 
-enum { MPTG_TUNE_KNN = 0, MPTG_TUNE_DMV_LONG = 1, MPTG_TUNE_MESH = 2, MPTG_TUNE_GRAPH = 3, MPTG_TUNE_ARM_TAIL = 4 };
+enum {
+    MPTG_TUNE_KNN = 0,
+    MPTG_TUNE_DMV_LONG = 1,
+    MPTG_TUNE_MESH = 2,
+    MPTG_TUNE_GRAPH = 3,
+    MPTG_TUNE_ARM_TAIL = 4,
+    MPTG_TUNE_ARM_CULL = 5
+};
 
 /* What one tuning call tried and chose: the knob, each candidate value and
    its median time (ms per call; per planner iteration for MPTG_TUNE_GRAPH). */
@@ -281,8 +291,9 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p);
 int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_tune_report* rep);
 /* Times link() of n device edges (reference order) for each candidate of the
    scene's link launch shape -- the grid of the persistent DMV long-edge or
-   mesh kernel, or the waves per edge of the arm continuation -- and keeps
-   the fastest; the per-edge bisection kernels have none (MPTG_E_STATE). */
+   mesh kernel, or for the arm the disk culling and then the waves per edge
+   of the continuation (rep: the last knob swept) -- and keeps the fastest;
+   the per-edge bisection kernels have none (MPTG_E_STATE). */
 int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
                        mptg_tune_report* rep);
 /* Plain PRRT: grows the tree (these iterations count, as in mptg_prrt_run)

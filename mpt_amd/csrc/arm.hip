@@ -148,7 +148,15 @@ constexpr int kHeadEdges = kWave / kHeadRanks;    // edges per wave in the packe
 // not vouched for), the tree shape, then reference ranks 0..3 (the surrogate
 // order leads with them too, so that no edge this round decides is scored).
 // Edges still undecided go to the continuation list.
-template <int D, bool COUNT, int ORD>
+// Waypoint validity in the link kernels: the wave-box culling (LANE false:
+// all lanes share one disk walk) or each lane's own grid cells (LANE true).
+template <int D, bool COUNT, bool LANE, typename Coord>
+__device__ __forceinline__ bool arm_ok_link(const SceneArgs& sc, Coord&& coord, bool active, ArmWork* wk) {
+    if constexpr (LANE) return arm_ok_lane<D, COUNT>(sc, coord, active, wk);
+    return arm_ok_wave<D, COUNT>(sc, coord, active, wk);
+}
+
+template <int D, bool COUNT, int ORD, bool LANE>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const double* __restrict__ from, const double* __restrict__ to, int64_t n, int known_valid,
     const uint8_t* __restrict__ mask, double tol, SceneArgs sc,
@@ -166,7 +174,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const uint32_t base = 2;
     if (!known_valid) {
         const bool act = live && rk < 2;
-        const bool bad = act && !arm_ok_wave<D, COUNT>(sc, [&](int i) { return rk == 0 ? a[i] : b[i]; }, act, &wk);
+        const bool bad =
+            act && !arm_ok_link<D, COUNT, LANE>(sc, [&](int i) { return rk == 0 ? a[i] : b[i]; }, act, &wk);
         const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
         if (live && (g & 3u)) {
             if (rk == 0) {
@@ -197,9 +206,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     uint32_t path = 0;
     int depth = 0;
     if (act) preorder_path(h, (uint32_t)rk, &path, &depth);  // reference ranks in both orders
-    const bool bad =
-        act && !arm_ok_wave<D, COUNT>(sc, [&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; },
-                                      act, &wk);
+    const bool bad = act && !arm_ok_link<D, COUNT, LANE>(
+                                sc, [&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; }, act, &wk);
     const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
     if (live && rk == 0) {
         if (g) {
@@ -231,7 +239,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
 // cannot stop before it (every rank the others report is larger), so the
 // slot ends at exactly the first failing rank of the chosen order.  No wave
 // waits for another: the slot is read and lowered, never awaited.
-template <int D, bool COUNT, int ORD>
+template <int D, bool COUNT, int ORD, bool LANE>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
     const double* __restrict__ from, const double* __restrict__ to, double tol, SceneArgs sc,
     const uint16_t* __restrict__ ranked, const int64_t* __restrict__ cont, const uint32_t* __restrict__ n_cont,
@@ -259,8 +267,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
             uint32_t path;
             int depth;
             preorder_path(h, ordinal_at<ORD>(ranked, e, active ? r : W - 1u), &path, &depth);
-            const bool bad =
-                !arm_ok_wave<D, COUNT>(sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, active, &wk);
+            const bool bad = !arm_ok_link<D, COUNT, LANE>(
+                sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, active, &wk);
             const uint64_t m = __ballot(bad && active);
             if (m) {
                 if (lane == 0) atomicMin(&checks[e], c0 + (uint32_t)__builtin_ctzll(m));
@@ -302,14 +310,14 @@ __global__ __launch_bounds__(256) void arm_link_finish(const double* __restrict_
 
 }  // namespace
 
-template <int D, bool COUNT, int ORD>
+template <int D, bool COUNT, int ORD, bool LANE>
 static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv,
                              const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
                              int64_t* cont, uint32_t* ncont, unsigned long long* work) {
     const double tol = ctx->link_param;
     const SceneArgs sa = scene_args(ctx);
     const int64_t per_block = (int64_t)kWavesPerBlock * kHeadEdges;
-    hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD>), dim3((unsigned)((n + per_block - 1) / per_block)),
+    hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD, LANE>), dim3((unsigned)((n + per_block - 1) / per_block)),
                        dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, verdict, checks,
                        irr, nirr, cont, ncont, work);
     phase_mark(ctx, 1);
@@ -331,7 +339,7 @@ static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to
     phase_mark(ctx, 3);
     int P = ctx->tune.arm_tail_waves;
     if ((n * P + kWavesPerBlock - 1) / kWavesPerBlock > (int64_t)INT32_MAX) P = 1;  // grid limit
-    hipLaunchKernelGGL((arm_link_tail<D, COUNT, ORD>), dim3((unsigned)((n * P + kWavesPerBlock - 1) / kWavesPerBlock)),
+    hipLaunchKernelGGL((arm_link_tail<D, COUNT, ORD, LANE>), dim3((unsigned)((n * P + kWavesPerBlock - 1) / kWavesPerBlock)),
                        dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, tol, sa, ranked, cont, ncont, P, checks,
                        work);
     hipLaunchKernelGGL((arm_link_finish<D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, from, to,
@@ -339,17 +347,27 @@ static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to
     return MPTG_OK;
 }
 
+template <int D, bool COUNT>
+static int launch_arm_cull(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv,
+                           const uint8_t* mask, int order, uint8_t* verdict, uint32_t* checks, int64_t* irr,
+                           uint32_t* nirr, int64_t* cont, uint32_t* ncont, unsigned long long* work) {
+    const bool lane = ctx->tune.arm_cull != 0;
+    if (order == MPTG_ORDER_SURROGATE) {
+        if (lane) return launch_arm_phases<D, COUNT, MPTG_ORDER_SURROGATE, true>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
+        return launch_arm_phases<D, COUNT, MPTG_ORDER_SURROGATE, false>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
+    }
+    if (lane) return launch_arm_phases<D, COUNT, MPTG_ORDER_REFERENCE, true>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
+    return launch_arm_phases<D, COUNT, MPTG_ORDER_REFERENCE, false>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
+}
+
 template <int D>
 static int launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int kv, const uint8_t* mask,
                         int order, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr, int64_t* cont,
                         uint32_t* ncont, unsigned long long* work) {
-    if (order == MPTG_ORDER_SURROGATE) {
-        if (work) return launch_arm_phases<D, true, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
-        return launch_arm_phases<D, false, MPTG_ORDER_SURROGATE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
-    }
-    if (work) return launch_arm_phases<D, true, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
-    return launch_arm_phases<D, false, MPTG_ORDER_REFERENCE>(ctx, from, to, n, kv, mask, verdict, checks, irr, nirr, cont, ncont, work);
+    if (work) return launch_arm_cull<D, true>(ctx, from, to, n, kv, mask, order, verdict, checks, irr, nirr, cont, ncont, work);
+    return launch_arm_cull<D, false>(ctx, from, to, n, kv, mask, order, verdict, checks, irr, nirr, cont, ncont, work);
 }
+
 
 // Launches the packed round and the continuation (scoring and ranking the
 // continued edges in between for the surrogate order); edges the kernels

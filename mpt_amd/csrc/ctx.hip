@@ -428,31 +428,56 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
         ad[4 * i + 2] = rt * rt;
         ad[4 * i + 3] = rt;
     }
-    // disks in ascending cx (ties by index) plus a 64-bucket index over cx, so
-    // a query x-interval maps to one contiguous range of disks; the verdict
-    // is an AND over disks, so their order changes nothing
+    // disk grid: about one disk per cell on average (at most kArmGridMax per
+    // axis) over the disk centres' bounding box; disks stored by cell,
+    // row-major, ties by index.  The verdict is an AND over disks, so their
+    // order changes nothing.
+    int gnx = 1, gny = 1;
+    double gx0 = 0.0, gy0 = 0.0, gix = 0.0, giy = 0.0, rtmax = 0.0;
+    if (ndisks > 0) {
+        double xmin = ad[0], xmax = ad[0], ymin = ad[1], ymax = ad[1];
+        for (int i = 0; i < ndisks; ++i) {
+            xmin = std::min(xmin, ad[4 * (size_t)i]);
+            xmax = std::max(xmax, ad[4 * (size_t)i]);
+            ymin = std::min(ymin, ad[4 * (size_t)i + 1]);
+            ymax = std::max(ymax, ad[4 * (size_t)i + 1]);
+            rtmax = std::max(rtmax, ad[4 * (size_t)i + 3]);
+        }
+        const int side = std::max(1, std::min(kArmGridMax, (int)std::lround(std::sqrt((double)ndisks))));
+        gnx = xmax > xmin ? side : 1;
+        gny = ymax > ymin ? side : 1;
+        gx0 = xmin;
+        gy0 = ymin;
+        gix = xmax > xmin ? (double)gnx / (xmax - xmin) : 0.0;
+        giy = ymax > ymin ? (double)gny / (ymax - ymin) : 0.0;
+    }
+    auto key = [&](size_t i) {
+        return arm_cell(ad[4 * i + 1], gy0, giy, gny) * gnx + arm_cell(ad[4 * i], gx0, gix, gnx);
+    };
     {
         std::vector<int> ord((size_t)ndisks);
         for (int i = 0; i < ndisks; ++i) ord[(size_t)i] = i;
-        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return ad[4 * x] < ad[4 * y]; });
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key((size_t)x) < key((size_t)y); });
         std::vector<double> sorted(ad.size(), 0.0);
         for (int i = 0; i < ndisks; ++i)
             for (int k = 0; k < 4; ++k) sorted[4 * (size_t)i + k] = ad[4 * (size_t)ord[(size_t)i] + k];
         ad.swap(sorted);
     }
-    std::vector<double> index(4 + (kArmBuckets + 1 + 1) / 2, 0.0);
+    const int ncells = gnx * gny;
+    std::vector<double> index(8 + (ncells + 1 + 1) / 2, 0.0);
     {
-        double xmin = ndisks ? ad[0] : 0.0, xmax = ndisks ? ad[4 * (size_t)(ndisks - 1)] : 0.0, rtmax = 0.0;
-        for (int i = 0; i < ndisks; ++i) rtmax = std::max(rtmax, ad[4 * (size_t)i + 3]);
-        const double inv_bw = xmax > xmin ? (double)kArmBuckets / (xmax - xmin) : 0.0;
-        index[0] = xmin;
-        index[1] = inv_bw;
-        index[2] = rtmax;
-        int32_t* starts = reinterpret_cast<int32_t*>(index.data() + 4);
+        index[0] = gx0;
+        index[1] = gy0;
+        index[2] = gix;
+        index[3] = giy;
+        index[4] = rtmax;
+        index[5] = gnx;
+        index[6] = gny;
+        int32_t* starts = reinterpret_cast<int32_t*>(index.data() + 8);
         int j = 0;
-        for (int k = 0; k <= kArmBuckets; ++k) {
-            while (j < ndisks && arm_bucket(ad[4 * (size_t)j], xmin, inv_bw) < k) ++j;
-            starts[k] = j;
+        for (int c = 0; c <= ncells; ++c) {
+            while (j < ndisks && key((size_t)j) < c) ++j;
+            starts[c] = j;
         }
     }
     if ((rc = ensure(ctx, ctx->scene.arm_index, sizeof(double) * index.size()))) return rc;

@@ -124,8 +124,8 @@ struct mptg_ctx {
     uint64_t epoch = 0;
     // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
     // persistent DMV long-edge kernel and of the mesh kernels, waves per edge
-    // of the arm continuation
-    mptg_tune_params tune{8, 16, 32, 2};
+    // of the arm continuation, arm link disk culling (0 wave box, 1 per lane)
+    mptg_tune_params tune{8, 16, 32, 2, 0, 0};
     hipEvent_t phase_ev[6] = {};
 };
 

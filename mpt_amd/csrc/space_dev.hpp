@@ -34,18 +34,34 @@ struct SceneArgs {
     int nlinks;
     const double* disks;  // [cx, cy, (r + R)^2]
     int ndisks;
-    const ArmDisk* arm_disks;  // same disks, kernel layout, ascending cx
+    const ArmDisk* arm_disks;  // same disks, kernel layout, ordered by grid cell (row-major)
     const double* arm_lim;     // per-link segment-length bound
-    const double* arm_index;   // [xmin, 1/bucket width, max (r+R), 0], then int32 starts[kArmBuckets + 1]
+    const double* arm_index;   // disk grid, see ArmGrid
 };
 
-// Bucket of an x coordinate in the arm scene's disk index; the host build and
-// the device query use this same expression, so it is monotone in x on both.
-constexpr int kArmBuckets = 64;
-constexpr int kDiskBatch = 4;  // disks loaded together in the culled loop
-MPTG_HD int arm_bucket(double x, double xmin, double inv_bw) {
-    const double t = (x - xmin) * inv_bw;
-    return t < 0.0 ? 0 : (t >= (double)kArmBuckets ? kArmBuckets - 1 : (int)t);
+// Uniform grid over the disk centres.  Disks are stored by cell, row-major
+// (cell y, then cell x), so the cells x0..x1 of one row hold one contiguous
+// run of disks: starts[row * nx + x0] .. starts[row * nx + x1 + 1].  Layout
+// of arm_index (doubles): [x0, y0, 1/cell width, 1/cell height, max (r+R),
+// nx, ny, 0], then int32 starts[nx * ny + 1].
+constexpr int kArmGridMax = 32;  // cells per axis at most
+constexpr int kDiskBatch = 4;    // disks loaded together in the culled loop
+
+// Cell of a coordinate along one axis; the host build and the device query
+// use this same expression, so it is monotone in v on both.
+MPTG_HD int arm_cell(double v, double v0, double inv, int n) {
+    const double t = (v - v0) * inv;
+    return t < 0.0 ? 0 : (t >= (double)n ? n - 1 : (int)t);
+}
+
+struct ArmGrid {
+    double x0, y0, ix, iy, rtmax;
+    int nx, ny;
+    const int32_t* starts;
+};
+
+__device__ __forceinline__ ArmGrid arm_grid(const double* g) {
+    return ArmGrid{g[0], g[1], g[2], g[3], g[4], (int)g[5], (int)g[6], reinterpret_cast<const int32_t*>(g + 8)};
 }
 
 inline SpaceArgs space_args(const mptg_ctx* ctx) { return SpaceArgs{ctx->metric, ctx->dim, ctx->w_so3, ctx->w_l2}; }
@@ -366,14 +382,17 @@ __device__ __forceinline__ bool arm_ok_wave(const SceneArgs& sc, Coord&& coord, 
         const double bx0 = wave_min_f64(live ? mx : inf), bx1 = wave_max_f64(live ? mx : -inf);
         const double by0 = wave_min_f64(live ? my : inf), by1 = wave_max_f64(live ? my : -inf);
         const double rho = sc.arm_lim[D + g];
-        // disks whose cx lies within the box's x-range widened by the largest
-        // (group half-length + r + R) form one contiguous run of the sorted
-        // disks; every other disk is farther than its own bound below
-        const double* ix = sc.arm_index;
-        const int32_t* starts = reinterpret_cast<const int32_t*>(ix + 4);
-        const double Rmax = (rho + ix[2]) * (1.0 + 0x1.0p-40) + 1e-12;
-        const int j0 = starts[arm_bucket(bx0 - Rmax, ix[0], ix[1])];
-        const int j1 = starts[arm_bucket(bx1 + Rmax, ix[0], ix[1]) + 1];
+        // disks whose centre lies within the box widened by the largest
+        // (group half-length + r + R) sit in the grid cells that box covers,
+        // one contiguous run per cell row; every other disk is farther than
+        // its own bound below
+        const ArmGrid ag = arm_grid(sc.arm_index);
+        const double Rmax = (rho + ag.rtmax) * (1.0 + 0x1.0p-40) + 1e-12;
+        const int gx0 = arm_cell(bx0 - Rmax, ag.x0, ag.ix, ag.nx), gx1 = arm_cell(bx1 + Rmax, ag.x0, ag.ix, ag.nx);
+        const int gy0 = arm_cell(by0 - Rmax, ag.y0, ag.iy, ag.ny), gy1 = arm_cell(by1 + Rmax, ag.y0, ag.iy, ag.ny);
+        for (int gy = gy0; gy <= gy1; ++gy) {
+        const int j0 = ag.starts[gy * ag.nx + gx0];
+        const int j1 = ag.starts[gy * ag.nx + gx1 + 1];
         for (int jb = j0; jb < j1; jb += kDiskBatch) {
             ArmDisk batch[kDiskBatch];
 #pragma unroll
@@ -405,6 +424,7 @@ __device__ __forceinline__ bool arm_ok_wave(const SceneArgs& sc, Coord&& coord, 
                 }
             }
         }
+        }  // grid rows
         px = last_x;
         py = last_y;
     }
@@ -412,17 +432,21 @@ __device__ __forceinline__ bool arm_ok_wave(const SceneArgs& sc, Coord&& coord, 
 }
 
 // arm_ok_fn for one lane on its own (states unrelated to their neighbours'):
-// per group, only the disks whose cx lies within (group half-length + the
-// largest r + R) (1 + 2^-40) + 1e-12 of the lane's group-circle centre can
-// pass the group test; they form one contiguous run of the x-sorted disks,
-// found through the bucket index.  Every other disk fails arm_ok_fn's group
-// test (the margin dominates the rounding of dx*dx + dy*dy > R*R for
-// coordinates below 10^3), so the verdict equals arm_ok_fn's.
-template <int D, typename Coord>
-__device__ __forceinline__ bool arm_ok_lane(const SceneArgs& sc, Coord&& coord) {
+// per group, only the disks whose centre lies within (group half-length +
+// the largest r + R) (1 + 2^-40) + 1e-12 of the lane's group-circle centre,
+// per axis, can pass the group test; they sit in the grid cells that box
+// covers, one contiguous run per cell row.  Every other disk fails
+// arm_ok_fn's group test (the margin dominates the rounding of
+// dx*dx + dy*dy > R*R for coordinates below 10^3), so the verdict equals
+// arm_ok_fn's.  No cross-lane operation: lanes may call it divergently;
+// active == false returns true at once.
+template <int D, bool COUNT = false, typename Coord>
+__device__ __forceinline__ bool arm_ok_lane(const SceneArgs& sc, Coord&& coord, bool active = true,
+                                            ArmWork* wk = nullptr) {
+    if (!active) return true;
     constexpr int G = (D + kArmGroup - 1) / kArmGroup;
-    const double* ix = sc.arm_index;
-    const int32_t* starts = reinterpret_cast<const int32_t*>(ix + 4);
+    if constexpr (COUNT) wk->calls += 1;
+    const ArmGrid ag = arm_grid(sc.arm_index);
     double px = 0.0, py = 0.0, angle = 0.0;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -441,6 +465,7 @@ __device__ __forceinline__ bool arm_ok_lane(const SceneArgs& sc, Coord&& coord) 
                 double L = sc.lengths[i0 + k];
                 jx[k + 1] = jx[k] + L * c;
                 jy[k + 1] = jy[k] + L * s;
+                if constexpr (COUNT) wk->fk += 1;
             }
         }
         double last_x = jx[0], last_y = jy[0];
@@ -452,20 +477,30 @@ __device__ __forceinline__ bool arm_ok_lane(const SceneArgs& sc, Coord&& coord) 
             }
         const double mx = (jx[0] + last_x) / 2, my = (jy[0] + last_y) / 2;
         const double rho = sc.arm_lim[D + g];
-        const double Rmax = (rho + ix[2]) * (1.0 + 0x1.0p-40) + 1e-12;
-        const int j0 = starts[arm_bucket(mx - Rmax, ix[0], ix[1])];
-        const int j1 = starts[arm_bucket(mx + Rmax, ix[0], ix[1]) + 1];
-        for (int j = j0; j < j1; ++j) {
-            const ArmDisk dk = sc.arm_disks[j];
-            double dx = dk.cx - mx, dy = dk.cy - my;
-            double R = rho + dk.rt;
-            if (dx * dx + dy * dy > R * R) continue;
+        const double Rmax = (rho + ag.rtmax) * (1.0 + 0x1.0p-40) + 1e-12;
+        const int gx0 = arm_cell(mx - Rmax, ag.x0, ag.ix, ag.nx), gx1 = arm_cell(mx + Rmax, ag.x0, ag.ix, ag.nx);
+        const int gy0 = arm_cell(my - Rmax, ag.y0, ag.iy, ag.ny), gy1 = arm_cell(my + Rmax, ag.y0, ag.iy, ag.ny);
+        for (int gy = gy0; gy <= gy1; ++gy) {
+            const int j1 = ag.starts[gy * ag.nx + gx1 + 1];
+            for (int j = ag.starts[gy * ag.nx + gx0]; j < j1; ++j) {
+                const ArmDisk dk = sc.arm_disks[j];
+                double dx = dk.cx - mx, dy = dk.cy - my;
+                double R = rho + dk.rt;
+                if constexpr (COUNT) wk->group += 1;
+                if (dx * dx + dy * dy > R * R) continue;
 #pragma unroll
-            for (int k = 0; k < GS; ++k) {
-                if (k < n) {
-                    double vx = jx[k + 1] - jx[k], vy = jy[k + 1] - jy[k];
-                    if (!disk_clear(dk, jx[k], jy[k], jx[k + 1], jy[k + 1], vx, vy, sc.arm_lim[i0 + k]))
-                        return false;
+                for (int k = 0; k < GS; ++k) {
+                    if (k < n) {
+                        double vx = jx[k + 1] - jx[k], vy = jy[k + 1] - jy[k];
+                        if constexpr (COUNT) {
+                            double wx = dk.cx - jx[k], wy = dk.cy - jy[k];
+                            double far = sc.arm_lim[i0 + k] + dk.rt;
+                            if (wx * wx + wy * wy > far * far) wk->far += 1;
+                            else wk->exact += 1;
+                        }
+                        if (!disk_clear(dk, jx[k], jy[k], jx[k + 1], jy[k + 1], vx, vy, sc.arm_lim[i0 + k]))
+                            return false;
+                    }
                 }
             }
         }
@@ -486,8 +521,7 @@ __device__ __forceinline__ bool arm_ok_split(const SceneArgs& sc, Coord&& coord,
     static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L must be a power of two up to 64");
     constexpr int G = (D + kArmGroup - 1) / kArmGroup;
     const int u = (int)(threadIdx.x & 63) & (L - 1);
-    const double* ix = sc.arm_index;
-    const int32_t* starts = reinterpret_cast<const int32_t*>(ix + 4);
+    const ArmGrid ag = arm_grid(sc.arm_index);
     bool ok = true;
     double px = 0.0, py = 0.0, angle = 0.0;
 #pragma unroll
@@ -518,13 +552,18 @@ __device__ __forceinline__ bool arm_ok_split(const SceneArgs& sc, Coord&& coord,
             }
         const double mx = (jx[0] + last_x) / 2, my = (jy[0] + last_y) / 2;
         const double rho = sc.arm_lim[D + g];
-        const double Rmax = (rho + ix[2]) * (1.0 + 0x1.0p-40) + 1e-12;
-        int j0 = 0, j1 = 0;
+        const double Rmax = (rho + ag.rtmax) * (1.0 + 0x1.0p-40) + 1e-12;
+        int gx0 = 0, gx1 = -1, gy0 = 0, gy1 = -1;
         if (active && ok) {
-            j0 = starts[arm_bucket(mx - Rmax, ix[0], ix[1])];
-            j1 = starts[arm_bucket(mx + Rmax, ix[0], ix[1]) + 1];
+            gx0 = arm_cell(mx - Rmax, ag.x0, ag.ix, ag.nx);
+            gx1 = arm_cell(mx + Rmax, ag.x0, ag.ix, ag.nx);
+            gy0 = arm_cell(my - Rmax, ag.y0, ag.iy, ag.ny);
+            gy1 = arm_cell(my + Rmax, ag.y0, ag.iy, ag.ny);
         }
         bool hit = false;
+        // the group's disk runs, row after row, dealt to its L lanes in turn
+        for (int gy = gy0; gy <= gy1 && !hit; ++gy) {
+        const int j0 = ag.starts[gy * ag.nx + gx0], j1 = ag.starts[gy * ag.nx + gx1 + 1];
         for (int j = j0 + u; j < j1 && !hit; j += L) {
             const ArmDisk dk = sc.arm_disks[j];
             double dx = dk.cx - mx, dy = dk.cy - my;
@@ -538,6 +577,7 @@ __device__ __forceinline__ bool arm_ok_split(const SceneArgs& sc, Coord&& coord,
                 }
             }
         }
+        }  // grid rows
 #pragma unroll
         for (int o = 1; o < L; o <<= 1) hit = (__shfl_xor(hit ? 1 : 0, o) != 0) || hit;
         ok = ok && !hit;

@@ -92,7 +92,9 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
         return fail(ctx, MPTG_E_RANGE, "launch shapes must be in 1..256");
     if (p->arm_tail_waves < 1 || p->arm_tail_waves > 8)
         return fail(ctx, MPTG_E_RANGE, "arm continuation waves per edge must be in 1..8");
+    if (p->arm_cull != 0 && p->arm_cull != 1) return fail(ctx, MPTG_E_RANGE, "arm culling is 0 or 1");
     ctx->tune = *p;
+    ctx->tune._pad = 0;
     return MPTG_OK;
 }
 
@@ -126,8 +128,11 @@ int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     auto call = [&] {
         return launch_link_masked(ctx, d_from, d_to, n, MPTG_ORDER_REFERENCE, 0, nullptr, verdict, checks);
     };
-    if (arm_wave)
+    if (arm_wave) {
+        if ((rc = sweep(ctx, MPTG_TUNE_ARM_CULL, {0, 1}, [&](int v) { ctx->tune.arm_cull = v; }, call, nullptr)))
+            return rc;
         return sweep(ctx, MPTG_TUNE_ARM_TAIL, {1, 2, 4, 8}, [&](int v) { ctx->tune.arm_tail_waves = v; }, call, rep);
+    }
     if (ctx->scene.kind == MPTG_SCENE_MESH)
         return sweep(ctx, MPTG_TUNE_MESH, {8, 16, 32, 64, 128}, [&](int v) { ctx->tune.mesh_waves_per_cu = v; },
                      call, rep);

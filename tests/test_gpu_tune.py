@@ -119,16 +119,52 @@ def test_arm_continuation_every_wave_count_is_exact(ctx_factory):
     b[-800:] = rng.uniform(-math.pi, math.pi, (800, 20))  # long edges: 8 rounds past the packed one
     o = oracle.Oracle(sc)
     want = {kv: o.link(a, b, known_valid=kv, threads=16) for kv in (False, True)}
-    for waves in (1, 2, 3, 5, 8):
-        ctx.tune_set(arm_tail_waves=waves)
-        for kv in (False, True):
-            gv, gc = ctx.link(a, b, endpoints_known_valid=kv)
-            assert np.array_equal(gv, want[kv][0]) and np.array_equal(gc, want[kv][1]), (waves, kv)
+    for cull in (0, 1):
+        for waves in (1, 2, 3, 5, 8):
+            ctx.tune_set(arm_tail_waves=waves, arm_cull=cull)
+            for kv in (False, True):
+                gv, gc = ctx.link(a, b, endpoints_known_valid=kv)
+                assert np.array_equal(gv, want[kv][0]) and np.array_equal(gc, want[kv][1]), (cull, waves, kv)
     da, db = ctx.to_device(a), ctx.to_device(b)
     rep = ctx.autotune_link(da.ptr, db.ptr, a.shape[0])
-    _say(" ", rep)
+    _say(" ", rep, ctx.tune_get())
     _check_report(rep, [1, 2, 4, 8])
     assert ctx.tune_get()["arm_tail_waves"] == rep["chosen"]
+    assert ctx.tune_get()["arm_cull"] in (0, 1)
+
+
+@pytest.mark.parametrize("ndisks", [0, 1, 7, 2000])
+def test_arm_disk_grid_every_size_and_culling_is_exact(ctx_factory, ndisks):
+    """The disk grid (1 x 1 up to 32 x 32 cells) and both link cullings
+    (wave box, per-lane cells) and the split-lane validity: verdicts and
+    check counts equal the oracle's brute force over every disk."""
+    _say("arm grid", ndisks)
+    import dataclasses
+    base = scenes.arm_scene(links=10, ndisks=3)
+    rng = np.random.default_rng(75 + ndisks)
+    total = float(base.lengths.sum())
+    disks = np.zeros((ndisks, 3))
+    if ndisks:
+        spread, rmax = (3.0, 0.5) if ndisks > 100 else (1.0, 4.0)  # many disks: sparser and smaller
+        disks[:, :2] = rng.uniform(-spread * total, spread * total, (ndisks, 2))
+        disks[:, 2] = rng.uniform(0.1, rmax, ndisks)
+        disks[: ndisks // 3, 0] = np.round(disks[: ndisks // 3, 0])  # shared centre coordinates
+    sc = dataclasses.replace(base, disks=disks)
+    ctx = ctx_factory()
+    ctx.scene(sc)
+    o = oracle.Oracle(sc)
+    q = rng.uniform(-math.pi, math.pi, (3000, 10))
+    ov = o.valid(q)
+    a = q[:1500]
+    b = np.clip(a + rng.normal(0, 0.7, a.shape), -math.pi, math.pi)
+    lv, lc = o.link(a, b, known_valid=False, threads=16)
+    for cull in (0, 1):
+        ctx.tune_set(arm_cull=cull)
+        assert np.array_equal(ctx.valid(q), ov), cull
+        gv, gc = ctx.link(a, b, endpoints_known_valid=False)
+        assert np.array_equal(gv, lv) and np.array_equal(gc, lc), cull
+    if ndisks == 2000:
+        assert 0.01 < ov.mean() < 0.99
 
 
 def test_planner_tuned_while_growing_equals_oracle(ctx_factory):
@@ -160,3 +196,5 @@ def test_tune_set_rejects_out_of_range(ctx_factory):
         ctx.tune_set(mesh_waves_per_cu=257)
     with pytest.raises(Exception):
         ctx.tune_set(arm_tail_waves=9)
+    with pytest.raises(Exception):
+        ctx.tune_set(arm_cull=2)
