@@ -312,16 +312,15 @@ def bench_se3(ctx, args):
     sc = scenes.se3_scene()
     ctx.scene(sc)
     ctx.tree_clear()
+    # waves per mesh edge, tuned by the autotuner on a PRRT* tree of its own
+    # (the planner's own mix of extend, choose-parent and rewire links)
+    ctx.prrt_star_init(sc, seed=args.seed + 1000, batch=4096, k=20)
+    ctx.prrt_add_start(sc.start)
+    tune = ctx.autotune_prrt()
+    ctx.tree_clear()
     ctx.prrt_star_init(sc, seed=args.seed, batch=4096, k=20)
     ctx.prrt_add_start(sc.start)
     ctx.prrt_run(1 << 30, args.warmup)
-    # mesh kernel grid tuned on the warm tree's own edges (parent -> node)
-    nodes, parents = ctx.tree_read()
-    ea, eb = nodes[parents[1:]], nodes[1:]
-    da, db = ctx.to_device(ea), ctx.to_device(eb)
-    tune = ctx.autotune_link(da.ptr, db.ptr, ea.shape[0])
-    da.free()
-    db.free()
     s0 = ctx.prrt_stats()
     ctx.sync()
     t0 = time.perf_counter()
@@ -335,7 +334,8 @@ def bench_se3(ctx, args):
             "extensions_per_s": d["samples"] / dt, "accepted_per_s": d["link_ok"] / dt,
             "edge_checks_per_s": links / dt, "extend_links": d["link_calls"], "choose_parent_links": d["cp_links"],
             "rewire_links": d["rw_links"], "rewires": d["rewires"], "deferred": d["deferred"],
-            "tree_size": s1["size"], "goals": s1["goals"], "autotune_mesh_waves_per_cu": tune}
+            "tree_size": s1["size"], "goals": s1["goals"], "autotune_mesh_edge_waves": tune,
+            "launch_shapes": ctx.tune_get()}
 
 
 def cpu_baseline(sc, seconds):

@@ -260,7 +260,7 @@ typedef struct mptg_tune_params {
     int32_t arm_tail_waves;         /* waves sharing one arm edge's continuation rounds (1..8) */
     int32_t arm_cull;               /* arm link disk culling: 0 one walk per wave over the wave's
                                        box, 1 each lane over its own grid cells */
-    int32_t _pad;
+    int32_t mesh_edge_waves;        /* waves sharing one mesh edge's waypoints (1..32) */
 } mptg_tune_params;
 
 enum {
@@ -269,7 +269,8 @@ enum {
     MPTG_TUNE_MESH = 2,
     MPTG_TUNE_GRAPH = 3,
     MPTG_TUNE_ARM_TAIL = 4,
-    MPTG_TUNE_ARM_CULL = 5
+    MPTG_TUNE_ARM_CULL = 5,
+    MPTG_TUNE_MESH_EDGE = 6
 };
 
 /* What one tuning call tried and chose: the knob, each candidate value and
@@ -290,15 +291,18 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p);
    traversal, src/mpt/impl/prrt/prrt.hpp:406-409, which has nothing to tune). */
 int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_tune_report* rep);
 /* Times link() of n device edges (reference order) for each candidate of the
-   scene's link launch shape -- the grid of the persistent DMV long-edge or
-   mesh kernel, or for the arm the disk culling and then the waves per edge
-   of the continuation (rep: the last knob swept) -- and keeps the fastest;
+   scene's link launch shape -- the grid of the persistent DMV long-edge
+   kernel, for meshes the grid and then the waves per edge, for the arm the
+   disk culling and then the waves per continued edge (rep: the last knob
+   swept) -- and keeps the fastest;
    the per-edge bisection kernels have none (MPTG_E_STATE). */
 int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n,
                        mptg_tune_report* rep);
-/* Plain PRRT: grows the tree (these iterations count, as in mptg_prrt_run)
-   with each candidate number of iterations per graph launch, 0 = step by
-   step, and keeps the fastest per iteration; stops early at target_size. */
+/* Grows the tree (these iterations count, as in mptg_prrt_run) with each
+   candidate of the planner's launch knob and keeps the fastest per
+   iteration; stops early at target_size.  Plain PRRT: iterations per graph
+   launch (0 = step by step).  PRRT*: waves per mesh edge, or per continued
+   arm edge (other scenes: MPTG_E_STATE). */
 int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep);
 
 /* one arm-scene link() call (as mptg_link_dev) timed phase by phase with HIP

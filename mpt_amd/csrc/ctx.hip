@@ -17,11 +17,22 @@ __global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t* __restrict__ p,
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 }  // namespace
 
 int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n) {
     if (n <= 0) return MPTG_OK;
     hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+int launch_fill_u32(mptg_ctx* ctx, uint32_t* p, int64_t n, uint32_t v) {
+    if (n <= 0) return MPTG_OK;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n, v);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

@@ -125,7 +125,7 @@ struct mptg_ctx {
     // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
     // persistent DMV long-edge kernel and of the mesh kernels, waves per edge
     // of the arm continuation, arm link disk culling (0 wave box, 1 per lane)
-    mptg_tune_params tune{8, 16, 32, 2, 0, 0};
+    mptg_tune_params tune{8, 16, 32, 2, 0, 2};
     hipEvent_t phase_ev[6] = {};
 };
 
@@ -172,6 +172,8 @@ int check_device(mptg_ctx* ctx);
 int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n);
 // Zeroes n 32-bit words with a kernel (graph-capturable, unlike a memset).
 int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n);
+// Sets n 32-bit words to v with a kernel.
+int launch_fill_u32(mptg_ctx* ctx, uint32_t* p, int64_t n, uint32_t v);
 int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
                double* d_dist, int64_t tree_n, const int64_t* n_dev = nullptr, const int32_t* skip = nullptr);
 int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,

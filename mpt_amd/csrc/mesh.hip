@@ -287,11 +287,23 @@ __global__ __launch_bounds__(64) void mesh_filter(MeshArgs m, const double* __re
 // DMV link (src/mpt/discrete_motion_validator.hpp:71-130), one edge per wave:
 // valid(to) unless the caller vouches for it, then the interior waypoints in
 // the reference's ring order (table for up to 255, ring replay beyond).
+// link() of n edges, P waves per edge (P = waves_per_edge; each wave is one
+// workgroup, and the whole wave checks one waypoint at a time).  The check
+// sequence of an edge is valid(to) at position 0 (counted even when vouched
+// for, as the reference's validator counts it), then the DMV waypoints in
+// table order at positions 1..mw.  Wave p takes valid(to) (p = 0) and the
+// waypoints of rank p, p + P, ...; a failure lowers the edge's slot in
+// `checks` (set to ~0 beforehand) to its position with atomicMin, and a wave
+// stops when the slot is already below its next position.  The first
+// failing position lies in one wave's share, which that wave reaches, so the
+// slot ends at exactly the first failing position; no wave waits for
+// another.  Edges beyond 255 waypoints replay the 256-slot ring on wave 0
+// alone.  mesh_link_finish turns the slots into verdicts and check counts.
 __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double inv,
                                                 const uint8_t* __restrict__ table, const double* __restrict__ from,
                                                 const double* __restrict__ to, int64_t n, int known_valid,
-                                                const uint8_t* __restrict__ mask, uint8_t* __restrict__ verdict,
-                                                uint32_t* __restrict__ checks) {
+                                                const uint8_t* __restrict__ mask, int waves_per_edge,
+                                                uint32_t* __restrict__ slot) {
     __shared__ int32_t stack[64 * kStackStride];
     __shared__ int flag;
     __shared__ uint64_t rmin[256], rmax[256];
@@ -302,7 +314,14 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
     __shared__ double ab_s[14];
     volatile double* ab = ab_s;
     const int lane = threadIdx.x;
-    for (int64_t e = blockIdx.x; e < n; e += gridDim.x) {
+    const int64_t P = waves_per_edge;
+    auto current = [&](int64_t e) {  // the edge's slot, wave-uniform
+        return (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)__hip_atomic_load(&slot[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    };
+    for (int64_t item = blockIdx.x; item < n * P; item += gridDim.x) {
+        const int64_t e = item / P;
+        const int64_t p = item % P;
         if (mask && !mask[e]) continue;
         if (lane < 14) ab[lane] = lane < 7 ? from[e * 7 + lane] : to[e * 7 + lane - 7];
         __syncthreads();
@@ -315,11 +334,8 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
             }
         };
         load_ab();
-        if (!known_valid && mesh_collides(m, b, stack, &flag)) {
-            if (lane == 0) {
-                verdict[e] = 0;
-                checks[e] = 1;
-            }
+        if (p == 0 && !known_valid && mesh_collides(m, b, stack, &flag)) {
+            if (lane == 0) atomicMin(&slot[e], 0u);
             __syncthreads();
             continue;
         }
@@ -328,29 +344,27 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
         int64_t steps = 0;
         if (x >= 2.0) steps = x < 0x1.0p62 ? (int64_t)x : ((int64_t)1 << 62);
         const int64_t mw = steps > 0 ? steps - 1 : 0;
-        if (mw > kMaxDmvWaypointsMesh) {
-            if (lane == 0) {
-                verdict[e] = 0;
-                checks[e] = MPTG_CHECKS_OVER_LIMIT;
-            }
+        if (mw > kMaxDmvWaypointsMesh) {  // not checked; the finish kernel reports it
+            __syncthreads();
             continue;
         }
         const double delta = steps > 0 ? 1.0 / (double)steps : 0.0;
-        bool failed = false;
-        int64_t done = 0;
         if (mw <= kTableMaxMesh) {
             const uint8_t* ord = table + (mw * (mw - 1)) / 2;
-            for (int64_t o = 0; o < mw; ++o) {
+            for (int64_t o = p; o < mw; o += P) {
+                const uint32_t pos = (uint32_t)(o + 1);
+                if (current(e) < pos) break;  // an earlier position already failed
                 load_ab();
                 space_interp(sp, a, b, (double)ord[o] * delta, q);
-                ++done;
                 if (mesh_collides(m, q, stack, &flag)) {
-                    failed = true;
+                    if (lane == 0) atomicMin(&slot[e], pos);
                     break;
                 }
             }
-        } else {
+        } else if (p == 0) {
             // ring replay: lane 0 emits up to 64 indices at a time
+            bool failed = false;
+            int64_t done = 0;
             uint64_t qs = 0, qe = 1, seq_cur = 0, seq_end = 0;
             bool seq = false;
             if (lane == 0) {
@@ -406,12 +420,41 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
                 }
                 __syncthreads();
             }
-        }
-        if (lane == 0) {
-            verdict[e] = failed ? 0 : 1;
-            checks[e] = (uint32_t)(1 + done);
+            if (failed && lane == 0) atomicMin(&slot[e], (uint32_t)done);
         }
         __syncthreads();
+    }
+}
+
+// Verdicts and check counts of the edges from their slots.
+__global__ __launch_bounds__(256) void mesh_link_finish(SpaceArgs sp, double inv, const double* __restrict__ from,
+                                                        const double* __restrict__ to, int64_t n,
+                                                        const uint8_t* __restrict__ mask,
+                                                        uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n || (mask && !mask[e])) return;
+    const uint32_t s = checks[e];
+    if (s != 0xffffffffu) {
+        verdict[e] = 0;
+        checks[e] = s + 1u;
+        return;
+    }
+    double a[7], b[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        a[j] = from[e * 7 + j];
+        b[j] = to[e * 7 + j];
+    }
+    const double x = __builtin_ceil(space_dist(sp, a, b) * inv);
+    int64_t steps = 0;
+    if (x >= 2.0) steps = x < 0x1.0p62 ? (int64_t)x : ((int64_t)1 << 62);
+    const int64_t mw = steps > 0 ? steps - 1 : 0;
+    if (mw > kMaxDmvWaypointsMesh) {
+        verdict[e] = 0;
+        checks[e] = MPTG_CHECKS_OVER_LIMIT;
+    } else {
+        verdict[e] = 1;
+        checks[e] = (uint32_t)(1 + mw);
     }
 }
 
@@ -444,9 +487,14 @@ int launch_mesh_filter(mptg_ctx* ctx, const double* d_states, int64_t n, uint8_t
 int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from, const double* d_to, int64_t n,
                      int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks) {
     if (n <= 0) return MPTG_OK;
-    const unsigned g = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * ctx->tune.mesh_waves_per_cu);
-    hipLaunchKernelGGL(mesh_link, dim3(g), dim3(64), 0, ctx->stream, space_args(ctx), mesh_args(ctx),
-                       1.0 / ctx->link_param, d_table, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks);
+    int rc;
+    if ((rc = launch_fill_u32(ctx, d_checks, n, 0xffffffffu))) return rc;  // first-failure slots
+    const int P = ctx->tune.mesh_edge_waves;  // n <= 2^31 and P <= 32: n * P fits
+    const unsigned gp = (unsigned)std::min<int64_t>(n * P, (int64_t)ctx->cus * ctx->tune.mesh_waves_per_cu);
+    hipLaunchKernelGGL(mesh_link, dim3(gp), dim3(64), 0, ctx->stream, space_args(ctx), mesh_args(ctx),
+                       1.0 / ctx->link_param, d_table, d_from, d_to, n, known_valid, d_mask, P, d_checks);
+    hipLaunchKernelGGL(mesh_link_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, space_args(ctx),
+                       1.0 / ctx->link_param, d_from, d_to, n, d_mask, d_verdict, d_checks);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

@@ -93,8 +93,9 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
     if (p->arm_tail_waves < 1 || p->arm_tail_waves > 8)
         return fail(ctx, MPTG_E_RANGE, "arm continuation waves per edge must be in 1..8");
     if (p->arm_cull != 0 && p->arm_cull != 1) return fail(ctx, MPTG_E_RANGE, "arm culling is 0 or 1");
+    if (p->mesh_edge_waves < 1 || p->mesh_edge_waves > 32)
+        return fail(ctx, MPTG_E_RANGE, "mesh waves per edge must be in 1..32");
     ctx->tune = *p;
-    ctx->tune._pad = 0;
     return MPTG_OK;
 }
 
@@ -133,9 +134,13 @@ int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, 
             return rc;
         return sweep(ctx, MPTG_TUNE_ARM_TAIL, {1, 2, 4, 8}, [&](int v) { ctx->tune.arm_tail_waves = v; }, call, rep);
     }
-    if (ctx->scene.kind == MPTG_SCENE_MESH)
+    if (ctx->scene.kind == MPTG_SCENE_MESH) {
+        if ((rc = sweep(ctx, MPTG_TUNE_MESH_EDGE, {1, 2, 4, 8, 16, 32},
+                        [&](int v) { ctx->tune.mesh_edge_waves = v; }, call, nullptr)))
+            return rc;
         return sweep(ctx, MPTG_TUNE_MESH, {8, 16, 32, 64, 128}, [&](int v) { ctx->tune.mesh_waves_per_cu = v; },
                      call, rep);
+    }
     return sweep(ctx, MPTG_TUNE_DMV_LONG, {2, 4, 8, 16, 32, 64},
                  [&](int v) { ctx->tune.dmv_long_waves_per_cu = v; }, call, rep);
 }
@@ -145,24 +150,51 @@ int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep
     if (rc) return rc;
     PrrtState& s = ctx->prrt;
     if (!s.init) return fail(ctx, MPTG_E_STATE, "mptg_prrt_init must be called first");
-    if (s.star) return fail(ctx, MPTG_E_STATE, "PRRT* enqueues its iterations one by one: nothing to tune");
     if (ctx->size == 0) return fail(ctx, MPTG_E_EMPTY, "there are no valid initial states");
-    // each candidate per round: one call that captures (max(G,1)
-    // iterations), then kTimedIters timed iterations; rounds alternate the
-    // candidate order so that tree growth favours none; median over rounds
-    const std::vector<int> cand = {0, 2, 4, 8, 16};
-    constexpr int kTimedIters = 16, kRounds = 3;
+    // the knob: plain PRRT, iterations per graph launch; PRRT* (step by step),
+    // the scene's per-edge link split (waves per mesh edge or per continued
+    // arm edge), whose best value depends on the planner's own edge mix
+    int knob;
+    std::vector<int> cand;
+    int32_t* field;
+    int timed;
+    if (!s.star) {
+        knob = MPTG_TUNE_GRAPH;
+        cand = {0, 2, 4, 8, 16};
+        field = &s.graph_iters;
+        timed = 16;
+    } else if (ctx->scene.kind == MPTG_SCENE_MESH) {
+        knob = MPTG_TUNE_MESH_EDGE;
+        cand = {1, 2, 4, 8, 16};
+        field = &ctx->tune.mesh_edge_waves;
+        timed = 4;
+    } else if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->link_mode == MPTG_LINK_BISECT) {
+        knob = MPTG_TUNE_ARM_TAIL;
+        cand = {1, 2, 4, 8};
+        field = &ctx->tune.arm_tail_waves;
+        timed = 4;
+    } else {
+        return fail(ctx, MPTG_E_STATE, "this PRRT* scene has no launch shape to tune");
+    }
+    // each candidate per round: one untimed call (a graph capture, or the
+    // first launches of the new shape), then `timed` timed iterations;
+    // rounds alternate the candidate order so that tree growth favours none;
+    // median over rounds.  Every iteration counts, as in mptg_prrt_run.
+    constexpr int kRounds = 3;
     std::vector<std::vector<double>> per(cand.size());
-    const int32_t keep = s.graph_iters;
+    const int32_t keep = *field;
     bool stop = false;
     for (int round = 0; round < kRounds && !stop; ++round) {
         for (size_t j = 0; j < cand.size() && !stop; ++j) {
             const size_t i = round % 2 == 0 ? j : cand.size() - 1 - j;
-            s.graph_iters = cand[i];
-            if ((rc = mptg_prrt_run(ctx, target_size, std::max(cand[i], 1), nullptr))) break;
+            *field = cand[i];
+            ++ctx->epoch;  // a changed launch shape retires captured graphs
+            if ((rc = mptg_prrt_run(ctx, target_size, knob == MPTG_TUNE_GRAPH ? std::max(cand[i], 1) : 1,
+                                    nullptr)))
+                break;
             const int64_t it0 = s.iteration;
             const auto t0 = std::chrono::steady_clock::now();
-            if ((rc = mptg_prrt_run(ctx, target_size, kTimedIters, nullptr))) break;
+            if ((rc = mptg_prrt_run(ctx, target_size, timed, nullptr))) break;
             const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             const int64_t ran = s.iteration - it0;
             if (ran > 0) per[i].push_back(sec * 1e3 / (double)ran);
@@ -170,27 +202,27 @@ int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep
         }
         if (rc) break;
     }
-    std::vector<double> best(cand.size(), -1.0);
-    for (size_t i = 0; i < cand.size(); ++i)
-        if (!per[i].empty()) {
-            std::sort(per[i].begin(), per[i].end());
-            best[i] = per[i][per[i].size() / 2];
-        }
     if (rc) {
-        s.graph_iters = keep;
+        *field = keep;
         return rc;
     }
     mptg_tune_report r{};
-    r.knob = MPTG_TUNE_GRAPH;
+    r.knob = knob;
     int pick = -1;
     for (size_t i = 0; i < cand.size(); ++i) {
+        double best = -1.0;
+        if (!per[i].empty()) {
+            std::sort(per[i].begin(), per[i].end());
+            best = per[i][per[i].size() / 2];
+        }
         r.value[r.ncand] = cand[i];
-        r.ms[r.ncand] = (float)best[i];
+        r.ms[r.ncand] = (float)best;
         ++r.ncand;
-        if (best[i] >= 0 && (pick < 0 || best[i] < best[pick])) pick = (int)i;
+        if (best >= 0 && (pick < 0 || best < r.ms[pick])) pick = (int)i;
     }
-    s.graph_iters = pick >= 0 ? cand[pick] : keep;
-    r.chosen = s.graph_iters;
+    *field = pick >= 0 ? cand[pick] : keep;
+    ++ctx->epoch;
+    r.chosen = *field;
     if (rep) *rep = r;
     return MPTG_OK;
 }

@@ -89,18 +89,23 @@ def test_mesh_grid_every_shape_is_exact(ctx_factory):
     a = np.concatenate([qa, rng.uniform(0, 100, (800, 3))], axis=1)
     b = a.copy()
     b[:, 4:] += rng.normal(0, 3, (800, 3))
+    # the ring replay of edges beyond 255 waypoints (wave 0 only) is covered
+    # by test_gpu_mesh.py at the default waves per edge
     o = oracle.Oracle(sc)
-    ov, oc = o.link(a, b, threads=16)
+    want = {kv: o.link(a, b, known_valid=kv, threads=16) for kv in (False, True)}
     ok = o.valid(a)
     for w in (1, 7, 128):
-        ctx.tune_set(mesh_waves_per_cu=w)
-        gv, gc = ctx.link(a, b)
-        assert np.array_equal(gv, ov) and np.array_equal(gc, oc), w
+        for ew in (1, 3, 32):
+            ctx.tune_set(mesh_waves_per_cu=w, mesh_edge_waves=ew)
+            for kv in (False, True):
+                gv, gc = ctx.link(a, b, endpoints_known_valid=kv)
+                assert np.array_equal(gv, want[kv][0]) and np.array_equal(gc, want[kv][1]), (w, ew, kv)
         assert np.array_equal(ctx.valid(a), ok), w
     da, db = ctx.to_device(a), ctx.to_device(b)
     rep = ctx.autotune_link(da.ptr, db.ptr, a.shape[0])
-    _say(" ", rep)
-    _check_report(rep, [8, 16, 32, 64, 128])
+    _say(" ", rep, ctx.tune_get())
+    _check_report(rep, [8, 16, 32, 64, 128])  # the grid, swept after the waves per edge
+    assert ctx.tune_get()["mesh_waves_per_cu"] == rep["chosen"]
 
 
 def test_arm_continuation_every_wave_count_is_exact(ctx_factory):
@@ -188,6 +193,38 @@ def test_planner_tuned_while_growing_equals_oracle(ctx_factory):
         assert st[key] == ost[key], key
 
 
+def test_prrt_star_tuned_while_growing_equals_oracle(ctx_factory):
+    """PRRT* on the mesh scene: the tuner times planner iterations for each
+    count of waves per mesh edge on the live tree; the tree it leaves, grown
+    on to the target, is the oracle's."""
+    _say("tune prrt* mesh")
+    sc = scenes.se3_scene()
+    ctx = ctx_factory()
+    ctx.scene(sc)
+    ctx.tree_clear()
+    ctx.prrt_star_init(sc, 33, 256, k=20)
+    ctx.prrt_add_start(sc.start)
+    rep = ctx.autotune_prrt(2500)
+    _say(" ", rep)
+    assert rep["knob"] == 6 and sorted(rep["candidates"]) == [1, 2, 4, 8, 16]
+    assert ctx.tune_get()["mesh_edge_waves"] == rep["chosen"]
+    st = ctx.prrt_run(2500, GPU_ITER_CAP)
+    gn, gp = ctx.tree_read()
+    gc = ctx.prrt_costs()
+    on, op, oc, og, ost, oss = oracle.Oracle(sc).prrt_star(33, 256, 2500, k=20)
+    assert np.array_equal(gn, on) and np.array_equal(gp, op) and np.array_equal(gc, oc)
+    for key in ("rewire_tests", "rewires", "parent_changes", "deferred"):
+        assert st[key] == oss[key], key
+    # PRRT* on a scene with no per-edge split to tune is refused
+    boxes = scenes.boxes_scene()
+    ctx.scene(boxes)
+    ctx.tree_clear()
+    ctx.prrt_star_init(boxes, 1, 64, k=20)
+    ctx.prrt_add_start(boxes.start)
+    with pytest.raises(Exception):
+        ctx.autotune_prrt(1000)
+
+
 def test_tune_set_rejects_out_of_range(ctx_factory):
     ctx = ctx_factory()
     with pytest.raises(Exception):
@@ -198,3 +235,5 @@ def test_tune_set_rejects_out_of_range(ctx_factory):
         ctx.tune_set(arm_tail_waves=9)
     with pytest.raises(Exception):
         ctx.tune_set(arm_cull=2)
+    with pytest.raises(Exception):
+        ctx.tune_set(mesh_edge_waves=33)
