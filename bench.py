@@ -410,18 +410,24 @@ def main():
         ctx.close()
         return
     timer = Timer(ctx)
+    defaults = ctx.tune_get()  # each leg tunes for its own workload, from the defaults
+
+    def leg(fn, *a):
+        ctx.tune_set(**defaults)
+        return fn(ctx, *a)
+
     log("arm planner ...")
-    arm, roof, sc = bench_arm(ctx, args, timer)
+    arm, roof, sc = leg(bench_arm, args, timer)
     extra = {}
     if not args.quick:
         log("nearest-neighbour kernel ...")
-        extra["nn_1e6x7_q65536"] = bench_nn(ctx, args, timer)
+        extra["nn_1e6x7_q65536"] = leg(bench_nn, args, timer)
         log("grid edges ...")
-        extra["grid4096_edges_2p20"] = bench_grid(ctx, args, timer)
+        extra["grid4096_edges_2p20"] = leg(bench_grid, args, timer)
         log("boxes planner ...")
-        extra["boxes16_prrt_1e4"] = bench_boxes(ctx, args)
+        extra["boxes16_prrt_1e4"] = leg(bench_boxes, args)
         log("se3 prrt* ...")
-        extra["se3_prrtstar"] = bench_se3(ctx, args)
+        extra["se3_prrtstar"] = leg(bench_se3, args)
     cpu = None
     if not args.quick:
         log("cpu baseline ...")
