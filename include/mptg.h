@@ -261,6 +261,9 @@ typedef struct mptg_tune_params {
     int32_t arm_cull;               /* arm link disk culling: 0 one walk per wave over the wave's
                                        box, 1 each lane over its own grid cells */
     int32_t mesh_edge_waves;        /* waves sharing one mesh edge's waypoints (1..32) */
+    int32_t knn_select;             /* k > 1 on trees of 2^16 nodes or more: 1 bound, collect and
+                                       select (registers of k = 1 in the scan), 0 the k-list scan */
+    int32_t _pad;
 } mptg_tune_params;
 
 enum {

@@ -187,6 +187,7 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.mesh_tris.p);
     (void)hipFree(ctx->scene.mesh_robot.p);
     for (auto& b : ctx->scratch) (void)hipFree(b.p);
+    for (auto& b : ctx->knn_sel) (void)hipFree(b.p);
     prrt_free(ctx);
     surrogate_free(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -304,7 +305,9 @@ int mptg_tree_clear(mptg_ctx* ctx) {
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->size = 0;
     ctx->star_ok = true;
-    ctx->coord_bound = 0;
+    // the prefilter's coordinate bound: the planner's sampling box and goal
+    // (every node a planner adds lies within them), grown again by inserts
+    ctx->coord_bound = planner_coord_bound(ctx);
     ctx->prrt.iteration = 0;
     ctx->prrt.goals = 0;
     ctx->prrt.goal_ids.clear();

@@ -104,6 +104,9 @@ struct mptg_ctx {
     double link_param = 1.0;
 
     // scratch (grow-only)
+    // kNN k > 1 over large trees (bound, collect, select): subset results,
+    // bounds + counts, candidates, overflow list, gathered queries, results
+    mptg::DevBuf knn_sel[6];
     mptg::DevBuf scratch[12];  // 0-3 host staging, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
                                // 9 ranked ordinals, 10 continuation list
     // pinned host staging
@@ -125,7 +128,7 @@ struct mptg_ctx {
     // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
     // persistent DMV long-edge kernel and of the mesh kernels, waves per edge
     // of the arm continuation, arm link disk culling (0 wave box, 1 per lane)
-    mptg_tune_params tune{8, 16, 32, 2, 0, 2};
+    mptg_tune_params tune{8, 16, 32, 2, 0, 2, 1, 0};
     hipEvent_t phase_ev[6] = {};
 };
 
@@ -163,6 +166,9 @@ int check_device_ready(mptg_ctx* ctx);
 // The same without retiring graphs, for entry points that change nothing a
 // graph holds (their scratch growth retires graphs through ensure()).
 int check_device(mptg_ctx* ctx);
+// Largest |coordinate| of the initialised planner's sampling box and goal
+// (0 without a planner), with a 1e-12 relative margin.
+double planner_coord_bound(const mptg_ctx* ctx);
 
 // kernels launched from other translation units
 // n_dev: tree size read on the device (tree_n then only plans the split);

@@ -282,6 +282,127 @@ __global__ __launch_bounds__(kBlock) void knn_finish(const double* __restrict__ 
     }
 }
 
+// ---- k > 1 over large trees: bound, collect, select -------------------------
+// knn_collect: every tree node whose exact distance to the query is at most
+// the query's bound tau[q] (the k-th distance over a subset of the tree, so
+// no smaller than the k-th over all of it) goes to the query's candidate
+// list, cap entries at most; cnt[q] counts them all, so cnt[q] > cap marks an
+// overflow.  Each lane holds one bound, not a k-list: the scan runs with the
+// registers of k = 1.
+template <int D, int METRIC>
+__global__ __launch_bounds__(kBlock) void knn_collect(const float* __restrict__ x32, const double* __restrict__ x64,
+                                                      int64_t n, int64_t chunk, const double* __restrict__ q,
+                                                      int64_t nq, const double* __restrict__ tau, double pbound,
+                                                      int cap, double* __restrict__ cand_d,
+                                                      int32_t* __restrict__ cand_i, uint32_t* __restrict__ cnt) {
+    constexpr int DP = (D <= 2) ? 2 : (D <= 4 ? 4 : (D <= 8 ? 8 : ((D + 3) / 4) * 4));
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = qi < nq;
+    const int64_t qc = active ? qi : 0;
+    const int64_t p0 = (int64_t)blockIdx.y * chunk;
+    const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
+    float q32[D];
+    double mq = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double v = q[qc * D + i];
+        q32[i] = (float)v;
+        mq = __builtin_fmax(mq, __builtin_fabs(v));
+    }
+    const double t = tau[qc];
+    const float thr = active ? prefilter_threshold<D, METRIC>(t, mq + pbound) : -1.0f;
+    constexpr int U = 4;
+    int64_t p = p0;
+    auto take = [&](int64_t j) {
+        const double dq = exact_dist<D, METRIC>(q + qc * D, x64 + j * D);
+        if (dq <= t) {
+            const uint32_t s = atomicAdd(&cnt[qc], 1u);
+            if ((int)s < cap) {
+                cand_d[qc * cap + s] = dq;
+                cand_i[qc * cap + s] = (int32_t)j;
+            }
+        }
+    };
+    for (; p + U <= p1; p += U) {
+        float acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float* row = x32 + (p + u) * DP;
+            float a = 0.0f;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                float dd = q32[i] - row[i];
+                if constexpr (METRIC == MPTG_METRIC_L1) a = a + __builtin_fabsf(dd);
+                else a = __builtin_fmaf(dd, dd, a);
+            }
+            acc[u] = a;
+        }
+        uint32_t pass = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) pass |= (acc[u] <= thr ? 1u : 0u) << u;
+        while (pass) {
+            const int u = __builtin_ctz(pass);
+            pass &= pass - 1;
+            take(p + u);
+        }
+    }
+    for (; p < p1; ++p) {
+        const float* row = x32 + p * DP;
+        float a = 0.0f;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            float dd = q32[i] - row[i];
+            if constexpr (METRIC == MPTG_METRIC_L1) a = a + __builtin_fabsf(dd);
+            else a = __builtin_fmaf(dd, dd, a);
+        }
+        if (a <= thr) take(p);
+    }
+}
+
+// k best (distance, index) of each query's candidates; a query whose list
+// overflowed goes to `over` for the full scan instead.
+template <int K>
+__global__ __launch_bounds__(kBlock) void knn_select(const double* __restrict__ cand_d,
+                                                     const int32_t* __restrict__ cand_i,
+                                                     const uint32_t* __restrict__ cnt, int cap, int64_t nq, int k,
+                                                     double* __restrict__ out_d, int32_t* __restrict__ out_i,
+                                                     int64_t* __restrict__ over, uint32_t* __restrict__ n_over) {
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (qi >= nq) return;
+    const uint32_t c = cnt[qi];
+    if (c > (uint32_t)cap) {
+        over[atomicAdd(n_over, 1u)] = qi;
+        return;
+    }
+    TopK<K> top;
+    top.init();
+    for (uint32_t j = 0; j < c; ++j) top.insert(cand_d[qi * cap + j], cand_i[qi * cap + j]);
+    for (int j = 0; j < k; ++j) {
+        const bool have = top.i[j] != INT_MAX;
+        out_d[qi * k + j] = have ? top.d[j] : __builtin_inf();
+        out_i[qi * k + j] = have ? top.i[j] : -1;
+    }
+}
+
+// Gathers the listed queries (D coordinates each) into a compact array, and
+// scatters their k results back.
+__global__ __launch_bounds__(kBlock) void knn_gather(const double* __restrict__ q, int D,
+                                                     const int64_t* __restrict__ list, int64_t m,
+                                                     double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= m * D) return;
+    out[t] = q[list[t / D] * D + t % D];
+}
+__global__ __launch_bounds__(kBlock) void knn_scatter(const double* __restrict__ d, const int32_t* __restrict__ i,
+                                                      const int64_t* __restrict__ list, int64_t m, int k,
+                                                      double* __restrict__ out_d, int32_t* __restrict__ out_i) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= m * k) return;
+    const int64_t row = list[t / k];
+    out_d[row * k + t % k] = d[t];
+    out_i[row * k + t % k] = i[t];
+}
+
 // ---- SE(3): Cartesian[Scaled(SO3, w_so3), Scaled(L2, w_l2)] ----------------
 }  // namespace
 }  // namespace mptg
@@ -406,11 +527,9 @@ int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
     return ensure(ctx, ctx->scratch[5], sizeof(int32_t) * K * nq * splits);
 }
 
-int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
-               double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip) {
-    if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
-    if (nq > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 queries");
-    if (!(radius >= 0)) return fail(ctx, MPTG_E_ARG, "radius must be >= 0");
+// The k-list scan: each lane keeps its query's k best in registers.
+static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+                         double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip) {
     const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
     // with n_dev, tree_n is only the size the split count is planned for
     int64_t n = tree_n, chunk;
@@ -441,6 +560,118 @@ int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radiu
     }
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
+}
+
+namespace {
+__global__ __launch_bounds__(kBlock) void knn_bound(const double* __restrict__ sub_d, int64_t nq, int k,
+                                                    double radius, double* __restrict__ tau) {
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (qi < nq) tau[qi] = __builtin_fmin(sub_d[qi * k + k - 1], radius);
+}
+
+template <int D>
+bool dispatch_collect(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq,
+                      const double* tau, int cap, double* cd, int32_t* ci, uint32_t* cnt) {
+    if (ctx->metric == MPTG_METRIC_L1)
+        hipLaunchKernelGGL((knn_collect<D, MPTG_METRIC_L1>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64,
+                           n, chunk, q, nq, tau, ctx->coord_bound, cap, cd, ci, cnt);
+    else
+        hipLaunchKernelGGL((knn_collect<D, MPTG_METRIC_L2>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64,
+                           n, chunk, q, nq, tau, ctx->coord_bound, cap, cd, ci, cnt);
+    return true;
+}
+}  // namespace
+
+// k > 1 over a large tree: bound each query by its k-th distance over the
+// first 1/16 of the tree (a subset: its k-th distance is no smaller than the
+// whole tree's), collect every node within the bound, keep the k best by
+// (distance, index); the exact k best are all within the bound, so the result
+// equals the k-list scan's.  Queries whose candidates overflow take the
+// k-list scan.
+static int knn_select_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+                           double* d_dist, int64_t n) {
+    const int K = k <= 20 ? 20 : 32;
+    const int cap = 32 * K;
+    const int D = ctx->dim;
+    int rc;
+    DevBuf* b = ctx->knn_sel;
+    if ((rc = ensure(ctx, b[0], (sizeof(double) + sizeof(int32_t)) * (size_t)nq * k))) return rc;
+    if ((rc = ensure(ctx, b[1], (sizeof(double) + sizeof(uint32_t)) * (size_t)nq))) return rc;
+    if ((rc = ensure(ctx, b[2], (sizeof(double) + sizeof(int32_t)) * (size_t)nq * cap))) return rc;
+    if ((rc = ensure(ctx, b[3], sizeof(int64_t) * (size_t)nq + 256))) return rc;
+    double* sub_d = (double*)b[0].p;
+    int32_t* sub_i = (int32_t*)(sub_d + (size_t)nq * k);
+    double* tau = (double*)b[1].p;
+    uint32_t* cnt = (uint32_t*)(tau + nq);
+    double* cd = (double*)b[2].p;
+    int32_t* ci = (int32_t*)(cd + (size_t)nq * cap);
+    uint32_t* n_over = (uint32_t*)b[3].p;
+    int64_t* over = (int64_t*)((char*)b[3].p + 256);
+    const unsigned g1 = (unsigned)((nq + kBlock - 1) / kBlock);
+    // 1. bound from the subset [0, n / 16)
+    if ((rc = knn_scan_path(ctx, d_q, nq, k, radius, sub_i, sub_d, n / 16, nullptr, nullptr))) return rc;
+    hipLaunchKernelGGL(knn_bound, dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)sub_d, nq, k, radius, tau);
+    // 2. collect over the whole tree
+    if ((rc = launch_zero_u32(ctx, cnt, nq))) return rc;
+    if ((rc = launch_zero_u32(ctx, n_over, 1))) return rc;
+    int64_t chunk;
+    const int64_t splits = plan_splits(ctx, nq, n, &chunk);
+    const dim3 grid(g1, (unsigned)splits);
+    bool ok = false;
+    switch (D) {
+        case 2: ok = dispatch_collect<2>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 3: ok = dispatch_collect<3>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 4: ok = dispatch_collect<4>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 5: ok = dispatch_collect<5>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 6: ok = dispatch_collect<6>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 7: ok = dispatch_collect<7>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 8: ok = dispatch_collect<8>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 10: ok = dispatch_collect<10>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        case 20: ok = dispatch_collect<20>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
+        default: break;
+    }
+    if (!ok) return fail(ctx, MPTG_E_STATE, "no collect kernel for this dimension");
+    MPTG_HIP(ctx, hipGetLastError());
+    // 3. select
+    if (K == 20)
+        hipLaunchKernelGGL((knn_select<20>), dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)cd,
+                           (const int32_t*)ci, (const uint32_t*)cnt, cap, nq, k, d_dist, d_idx, over, n_over);
+    else
+        hipLaunchKernelGGL((knn_select<32>), dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)cd,
+                           (const int32_t*)ci, (const uint32_t*)cnt, cap, nq, k, d_dist, d_idx, over, n_over);
+    MPTG_HIP(ctx, hipGetLastError());
+    // 4. overflowed queries: the k-list scan, gathered and scattered back
+    uint32_t m = 0;
+    MPTG_HIP(ctx, hipMemcpyAsync(&m, n_over, sizeof(m), hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (m == 0) return MPTG_OK;
+    if ((rc = ensure(ctx, b[4], sizeof(double) * (size_t)m * D))) return rc;
+    if ((rc = ensure(ctx, b[5], (sizeof(double) + sizeof(int32_t)) * (size_t)m * k))) return rc;
+    double* gq = (double*)b[4].p;
+    double* gd = (double*)b[5].p;
+    int32_t* gi = (int32_t*)(gd + (size_t)m * k);
+    hipLaunchKernelGGL(knn_gather, dim3((unsigned)(((int64_t)m * D + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, d_q, D, (const int64_t*)over, (int64_t)m, gq);
+    if ((rc = knn_scan_path(ctx, gq, m, k, radius, gi, gd, n, nullptr, nullptr))) return rc;
+    hipLaunchKernelGGL(knn_scatter, dim3((unsigned)(((int64_t)m * k + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, (const double*)gd, (const int32_t*)gi, (const int64_t*)over, (int64_t)m, k, d_dist,
+                       d_idx);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+constexpr int64_t kSelectMinTree = 1 << 16;  // smaller trees: the k-list scan
+
+int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+               double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip) {
+    if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
+    if (nq > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 queries");
+    if (!(radius >= 0)) return fail(ctx, MPTG_E_ARG, "radius must be >= 0");
+    const bool sel_dim = ctx->dim <= 8 || ctx->dim == 10 || ctx->dim == 20;
+    if (k > 1 && nq > 0 && tree_n >= kSelectMinTree && !n_dev && !skip && !ctx->capturing && ctx->tune.knn_select &&
+        ctx->metric != MPTG_METRIC_SE3 && ctx->dim >= 2 && sel_dim)
+        return knn_select_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n);
+    return knn_scan_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n, n_dev, skip);
 }
 
 }  // namespace mptg

@@ -229,6 +229,18 @@ void prrt_free(mptg_ctx* ctx) {
 int tree_reserve(mptg_ctx* ctx, int64_t cap);
 int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters);
 
+double planner_coord_bound(const mptg_ctx* ctx) {
+    const PrrtState& s = ctx->prrt;
+    if (!s.init) return 0.0;
+    double bound = 0.0;
+    for (int i = 0; i < ctx->dim; ++i) {
+        bound = std::max(bound, std::max(std::fabs(s.lo[i]), std::fabs(s.hi[i])));
+        bound = std::max(bound, std::fabs(s.goal[i]));
+    }
+    if (ctx->metric == MPTG_METRIC_SE3) bound = std::max(bound, 1.0);
+    return bound * (1.0 + 1e-12);
+}
+
 // Writes the stop condition of one run into the device counters: the next
 // iteration index, the size target and the iteration bound; clears `done`.
 int counters_arm(mptg_ctx* ctx, void* dcnt, int64_t iteration, int64_t target, int64_t max_iters) {
@@ -348,22 +360,19 @@ int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo,
     if (!(p->goal_bias >= 0 && p->goal_bias <= 1)) return fail(ctx, MPTG_E_ARG, "goal bias must be in [0,1]");
     if (p->order != MPTG_ORDER_REFERENCE && p->order != MPTG_ORDER_SURROGATE)
         return fail(ctx, MPTG_E_ARG, "unknown check order");
+    for (int i = 0; i < ctx->dim; ++i)
+        if (!(h_lo[i] <= h_hi[i]) || !std::isfinite(h_lo[i]) || !std::isfinite(h_hi[i]) || !std::isfinite(h_goal[i]))
+            return fail(ctx, MPTG_E_ARG, "bounds and goal must be finite with lo <= hi");
     PrrtState& s = ctx->prrt;
     s.p = *p;
     s.star = false;
-    double bound = ctx->coord_bound;
     for (int i = 0; i < ctx->dim; ++i) {
-        if (!(h_lo[i] <= h_hi[i]) || !std::isfinite(h_lo[i]) || !std::isfinite(h_hi[i]))
-            return fail(ctx, MPTG_E_ARG, "bounds must be finite with lo <= hi");
         s.lo[i] = h_lo[i];
         s.hi[i] = h_hi[i];
         s.goal[i] = h_goal[i];
-        bound = std::max(bound, std::max(std::fabs(h_lo[i]), std::fabs(h_hi[i])));
-        bound = std::max(bound, std::fabs(h_goal[i]));
     }
-    if (ctx->metric == MPTG_METRIC_SE3) bound = std::max(bound, 1.0);
-    ctx->coord_bound = bound * (1.0 + 1e-12);
     s.init = true;
+    ctx->coord_bound = std::max(ctx->coord_bound, planner_coord_bound(ctx));
     s.iteration = 0;
     s.goals = 0;
     s.goal_ids.clear();

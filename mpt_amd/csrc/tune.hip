@@ -95,7 +95,9 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
     if (p->arm_cull != 0 && p->arm_cull != 1) return fail(ctx, MPTG_E_RANGE, "arm culling is 0 or 1");
     if (p->mesh_edge_waves < 1 || p->mesh_edge_waves > 32)
         return fail(ctx, MPTG_E_RANGE, "mesh waves per edge must be in 1..32");
+    if (p->knn_select != 0 && p->knn_select != 1) return fail(ctx, MPTG_E_RANGE, "knn_select is 0 or 1");
     ctx->tune = *p;
+    ctx->tune._pad = 0;
     return MPTG_OK;
 }
 

@@ -235,3 +235,21 @@ def test_prrt_tree_parity_grid(ctx_factory, grid1024):
 def test_prrt_tree_parity_arm(ctx_factory):
     _say("prrt arm")
     _planner_parity(ctx_factory(), scenes.arm_scene(), seed=13, batch=256, target=400)
+
+
+def test_prrt_after_init_then_clear(ctx_factory):
+    """Planner initialised, then the tree cleared and re-seeded: the nearest-
+    neighbour prefilter keeps the planner's sampling box in its coordinate
+    bound, and the tree equals the oracle's."""
+    _say("prrt init then clear")
+    sc = scenes.boxes_scene()
+    ctx = ctx_factory()
+    ctx.scene(sc)
+    ctx.prrt_init(sc, 23, 128)
+    ctx.tree_clear()
+    ctx.prrt_add_start(sc.start)
+    st = ctx.prrt_run(5000, GPU_ITER_CAP)
+    gn, gp = ctx.tree_read()
+    on, op, og, ost = oracle.Oracle(sc).prrt(23, 128, 5000)
+    assert np.array_equal(gn, on) and np.array_equal(gp, op)
+    assert st["iterations"] == ost["iterations"]

@@ -54,6 +54,41 @@ def test_knn_every_split_is_exact_and_tuner_picks_fastest(ctx_factory):
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
 
 
+@pytest.mark.parametrize("metric,dim,k", [(scenes.METRIC_L2, 7, 20), (scenes.METRIC_L1, 20, 5),
+                                          (scenes.METRIC_L2, 3, 32)])
+def test_knn_select_path_equals_klist_scan_and_oracle(ctx_factory, metric, dim, k):
+    """k > 1 on trees of 2^16 nodes or more: bound from the first 1/16 of the
+    tree, collect, select.  Equals the k-list scan on every query and the
+    oracle on a sample, with duplicates and ties, a finite radius, and a tree
+    whose first 1/16 lies far from the queries (loose bounds: candidate lists
+    overflow and those queries take the k-list scan)."""
+    _say("knn select", metric, dim, k)
+    ctx = ctx_factory()
+    rng = np.random.default_rng(80 + dim)
+    n = 1 << 17
+    pts = rng.uniform(-math.pi, math.pi, (n, dim))
+    pts[5000:5100] = pts[7000:7100]          # duplicates: ties by index
+    far = pts.copy()
+    far[: n // 16] += 40.0                   # the subset the bound comes from sits far away
+    q = rng.uniform(-math.pi, math.pi, (3000, dim))
+    q[:50] = pts[7000:7050]
+    sample = rng.choice(q.shape[0], 120, replace=False)
+    for tree in (pts, far):
+        ctx.space(metric, dim)
+        ctx.tree_clear()
+        ctx.tree_insert(tree)
+        o = oracle.Oracle(_plain(metric, dim))
+        for radius in (math.inf, 4.0 if metric == scenes.METRIC_L2 else 20.0):
+            res = {}
+            for sel in (0, 1):
+                ctx.tune_set(knn_select=sel)
+                res[sel] = ctx.knn(q, k, radius)
+            assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), radius
+            oi, od = o.knn(tree, q[sample], k, radius=radius, threads=16)
+            assert np.array_equal(res[1][0][sample], oi) and np.array_equal(res[1][1][sample], od), radius
+    ctx.tune_set(knn_select=1)
+
+
 def test_dmv_long_grid_every_shape_is_exact(ctx_factory):
     _say("tune dmv")
     ctx = ctx_factory()
@@ -237,3 +272,5 @@ def test_tune_set_rejects_out_of_range(ctx_factory):
         ctx.tune_set(arm_cull=2)
     with pytest.raises(Exception):
         ctx.tune_set(mesh_edge_waves=33)
+    with pytest.raises(Exception):
+        ctx.tune_set(knn_select=2)
