@@ -283,17 +283,18 @@ __global__ __launch_bounds__(kBlock) void knn_finish(const double* __restrict__ 
 }
 
 // ---- k > 1 over large trees: bound, collect, select -------------------------
-// knn_collect: every tree node whose exact distance to the query is at most
-// the query's bound tau[q] (the k-th distance over a subset of the tree, so
-// no smaller than the k-th over all of it) goes to the query's candidate
-// list, cap entries at most; cnt[q] counts them all, so cnt[q] > cap marks an
-// overflow.  Each lane holds one bound, not a k-list: the scan runs with the
-// registers of k = 1.
+// knn_collect: every tree node of the block's chunk that passes the FP32
+// prefilter for the query's bound tau[q] (a k-th distance over a subset of
+// the tree, so no smaller than the k-th over all of it; the prefilter never
+// drops a node whose exact distance is within the bound) goes, as an index,
+// to the (chunk, query) candidate region, cap entries at most; cnt[chunk][q]
+// counts them all, so a count above cap marks an overflow.  Each lane holds
+// one bound and a count, owns its region (no atomics), and issues no
+// dependent load in the scan: exact distances are the select kernel's.
 template <int D, int METRIC>
-__global__ __launch_bounds__(kBlock) void knn_collect(const float* __restrict__ x32, const double* __restrict__ x64,
-                                                      int64_t n, int64_t chunk, const double* __restrict__ q,
-                                                      int64_t nq, const double* __restrict__ tau, double pbound,
-                                                      int cap, double* __restrict__ cand_d,
+__global__ __launch_bounds__(kBlock) void knn_collect(const float* __restrict__ x32, int64_t n, int64_t chunk,
+                                                      const double* __restrict__ q, int64_t nq,
+                                                      const double* __restrict__ tau, double pbound, int cap,
                                                       int32_t* __restrict__ cand_i, uint32_t* __restrict__ cnt) {
     constexpr int DP = (D <= 2) ? 2 : (D <= 4 ? 4 : (D <= 8 ? 8 : ((D + 3) / 4) * 4));
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -309,20 +310,15 @@ __global__ __launch_bounds__(kBlock) void knn_collect(const float* __restrict__ 
         q32[i] = (float)v;
         mq = __builtin_fmax(mq, __builtin_fabs(v));
     }
-    const double t = tau[qc];
-    const float thr = active ? prefilter_threshold<D, METRIC>(t, mq + pbound) : -1.0f;
+    const float thr = active ? prefilter_threshold<D, METRIC>(tau[qc], mq + pbound) : -1.0f;
+    const int64_t region = ((int64_t)blockIdx.y * nq + qc) * cap;
+    uint32_t c = 0;
+    auto take = [&](int64_t j) {
+        if (c < (uint32_t)cap) cand_i[region + c] = (int32_t)j;
+        ++c;
+    };
     constexpr int U = 4;
     int64_t p = p0;
-    auto take = [&](int64_t j) {
-        const double dq = exact_dist<D, METRIC>(q + qc * D, x64 + j * D);
-        if (dq <= t) {
-            const uint32_t s = atomicAdd(&cnt[qc], 1u);
-            if ((int)s < cap) {
-                cand_d[qc * cap + s] = dq;
-                cand_i[qc * cap + s] = (int32_t)j;
-            }
-        }
-    };
     for (; p + U <= p1; p += U) {
         float acc[U];
 #pragma unroll
@@ -337,6 +333,7 @@ __global__ __launch_bounds__(kBlock) void knn_collect(const float* __restrict__ 
             }
             acc[u] = a;
         }
+        // the four rows accumulate independently; one branch for all
         uint32_t pass = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) pass |= (acc[u] <= thr ? 1u : 0u) << u;
@@ -357,30 +354,48 @@ __global__ __launch_bounds__(kBlock) void knn_collect(const float* __restrict__ 
         }
         if (a <= thr) take(p);
     }
+    if (active) cnt[(int64_t)blockIdx.y * nq + qi] = c;
 }
 
-// k best (distance, index) of each query's candidates; a query whose list
-// overflowed goes to `over` for the full scan instead.
-template <int K>
-__global__ __launch_bounds__(kBlock) void knn_select(const double* __restrict__ cand_d,
+// k best (exact distance, index) of each query's candidates over the chunks,
+// among those within the radius; a query with an overflowed region goes to
+// `over` for the k-list scan instead.  Candidates beyond the bound cannot
+// displace the k best: at least k tree nodes lie within it.
+template <int D, int METRIC, int K>
+__global__ __launch_bounds__(kBlock) void knn_select(const double* __restrict__ x64, const double* __restrict__ q,
                                                      const int32_t* __restrict__ cand_i,
-                                                     const uint32_t* __restrict__ cnt, int cap, int64_t nq, int k,
-                                                     double* __restrict__ out_d, int32_t* __restrict__ out_i,
-                                                     int64_t* __restrict__ over, uint32_t* __restrict__ n_over) {
+                                                     const uint32_t* __restrict__ cnt, int cap, int splits,
+                                                     int64_t nq, int k, double radius, double* __restrict__ out_d,
+                                                     int32_t* __restrict__ out_i, int64_t* __restrict__ over,
+                                                     uint32_t* __restrict__ n_over) {
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (qi >= nq) return;
-    const uint32_t c = cnt[qi];
-    if (c > (uint32_t)cap) {
-        over[atomicAdd(n_over, 1u)] = qi;
-        return;
-    }
+    for (int sp = 0; sp < splits; ++sp)
+        if (cnt[(int64_t)sp * nq + qi] > (uint32_t)cap) {
+            over[atomicAdd(n_over, 1u)] = qi;
+            return;
+        }
+    double qq[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) qq[i] = q[qi * D + i];
     TopK<K> top;
     top.init();
-    for (uint32_t j = 0; j < c; ++j) top.insert(cand_d[qi * cap + j], cand_i[qi * cap + j]);
-    for (int j = 0; j < k; ++j) {
-        const bool have = top.i[j] != INT_MAX;
-        out_d[qi * k + j] = have ? top.d[j] : __builtin_inf();
-        out_i[qi * k + j] = have ? top.i[j] : -1;
+    for (int sp = 0; sp < splits; ++sp) {
+        const int64_t region = ((int64_t)sp * nq + qi) * cap;
+        const uint32_t c = cnt[(int64_t)sp * nq + qi];
+        for (uint32_t j = 0; j < c; ++j) {
+            const int32_t id = cand_i[region + j];
+            const double dq = exact_dist<D, METRIC>(qq, x64 + (int64_t)id * D);
+            if (dq <= radius) top.insert(dq, id);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {  // static indices: the list stays in registers
+        if (j < k) {
+            const bool have = top.i[j] != INT_MAX;
+            out_d[qi * k + j] = have ? top.d[j] : __builtin_inf();
+            out_i[qi * k + j] = have ? top.i[j] : -1;
+        }
     }
 }
 
@@ -501,19 +516,21 @@ bool dispatch_dim(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q
 
 // Chooses the number of tree chunks so that a launch has ~8 waves per SIMD
 // worth of work, with at least 256 nodes per chunk.
-static int64_t choose_splits(mptg_ctx* ctx, int64_t nq, int64_t n) {
+static int64_t choose_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int K) {
     int64_t qblocks = (nq + kBlock - 1) / kBlock;
     int64_t want_blocks = (int64_t)ctx->cus * ctx->tune.knn_blocks_per_cu;
     int64_t s = (want_blocks + qblocks - 1) / qblocks;
     int64_t max_s = (n + 255) / 256;
     if (s > max_s) s = max_s;
     if (s < 1) s = 1;
-    if (s > 4096) s = 4096;
+    // the merge reads splits * K candidates per query, one latency each
+    const int64_t cap = K == 1 ? 4096 : 64;
+    if (s > cap) s = cap;
     return s;
 }
 
-static int64_t plan_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int64_t* chunk) {
-    int64_t splits = n > 0 ? choose_splits(ctx, nq, n) : 1;
+static int64_t plan_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int64_t* chunk, int K = 1) {
+    int64_t splits = n > 0 ? choose_splits(ctx, nq, n, K) : 1;
     *chunk = n > 0 ? (n + splits - 1) / splits : 0;
     return n > 0 ? (n + *chunk - 1) / *chunk : 1;
 }
@@ -521,7 +538,7 @@ static int64_t plan_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int64_t* chunk)
 int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
     const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
     int64_t chunk;
-    const int64_t splits = plan_splits(ctx, nq, tree_n, &chunk);
+    const int64_t splits = plan_splits(ctx, nq, tree_n, &chunk, K);
     int rc;
     if ((rc = ensure(ctx, ctx->scratch[4], sizeof(double) * K * nq * splits))) return rc;
     return ensure(ctx, ctx->scratch[5], sizeof(int32_t) * K * nq * splits);
@@ -533,7 +550,7 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
     // with n_dev, tree_n is only the size the split count is planned for
     int64_t n = tree_n, chunk;
-    const int64_t splits = plan_splits(ctx, nq, n, &chunk);
+    const int64_t splits = plan_splits(ctx, nq, n, &chunk, K);
     const ScanExtent ext{n, chunk, n_dev, skip};
     size_t pbytes_d = sizeof(double) * K * nq * splits, pbytes_i = sizeof(int32_t) * K * nq * splits;
     int rc;
@@ -569,78 +586,72 @@ __global__ __launch_bounds__(kBlock) void knn_bound(const double* __restrict__ s
     if (qi < nq) tau[qi] = __builtin_fmin(sub_d[qi * k + k - 1], radius);
 }
 
-template <int D>
-bool dispatch_collect(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq,
-                      const double* tau, int cap, double* cd, int32_t* ci, uint32_t* cnt) {
-    if (ctx->metric == MPTG_METRIC_L1)
-        hipLaunchKernelGGL((knn_collect<D, MPTG_METRIC_L1>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64,
-                           n, chunk, q, nq, tau, ctx->coord_bound, cap, cd, ci, cnt);
+template <int D, int METRIC>
+void launch_collect_select(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq,
+                           const double* tau, int cap, int splits, int k, double radius, int32_t* ci, uint32_t* cnt,
+                           double* out_d, int32_t* out_i, int64_t* over, uint32_t* n_over) {
+    hipLaunchKernelGGL((knn_collect<D, METRIC>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, n, chunk, q, nq, tau,
+                       ctx->coord_bound, cap, ci, cnt);
+    if (k <= 20)
+        hipLaunchKernelGGL((knn_select<D, METRIC, 20>), dim3(grid.x), dim3(kBlock), 0, ctx->stream, ctx->x64, q,
+                           (const int32_t*)ci, (const uint32_t*)cnt, cap, splits, nq, k, radius, out_d, out_i, over,
+                           n_over);
     else
-        hipLaunchKernelGGL((knn_collect<D, MPTG_METRIC_L2>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64,
-                           n, chunk, q, nq, tau, ctx->coord_bound, cap, cd, ci, cnt);
-    return true;
+        hipLaunchKernelGGL((knn_select<D, METRIC, 32>), dim3(grid.x), dim3(kBlock), 0, ctx->stream, ctx->x64, q,
+                           (const int32_t*)ci, (const uint32_t*)cnt, cap, splits, nq, k, radius, out_d, out_i, over,
+                           n_over);
+}
+
+template <int D>
+void dispatch_collect(mptg_ctx* ctx, dim3 grid, int64_t n, int64_t chunk, const double* q, int64_t nq,
+                      const double* tau, int cap, int splits, int k, double radius, int32_t* ci, uint32_t* cnt,
+                      double* out_d, int32_t* out_i, int64_t* over, uint32_t* n_over) {
+    if (ctx->metric == MPTG_METRIC_L1)
+        launch_collect_select<D, MPTG_METRIC_L1>(ctx, grid, n, chunk, q, nq, tau, cap, splits, k, radius, ci, cnt,
+                                                 out_d, out_i, over, n_over);
+    else
+        launch_collect_select<D, MPTG_METRIC_L2>(ctx, grid, n, chunk, q, nq, tau, cap, splits, k, radius, ci, cnt,
+                                                 out_d, out_i, over, n_over);
 }
 }  // namespace
 
-// k > 1 over a large tree: bound each query by its k-th distance over the
-// first 1/16 of the tree (a subset: its k-th distance is no smaller than the
-// whole tree's), collect every node within the bound, keep the k best by
-// (distance, index); the exact k best are all within the bound, so the result
-// equals the k-list scan's.  Queries whose candidates overflow take the
-// k-list scan.
-static int knn_select_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
-                           double* d_dist, int64_t n) {
+// One refinement level over the tree prefix [0, n): collect within the
+// bounds tau, select the k best; queries whose candidates overflowed take
+// the k-list scan over the prefix (gathered, scanned, scattered back).
+static int knn_refine(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, const double* tau,
+                      int64_t n, double* out_d, int32_t* out_i) {
     const int K = k <= 20 ? 20 : 32;
-    const int cap = 32 * K;
     const int D = ctx->dim;
-    int rc;
-    DevBuf* b = ctx->knn_sel;
-    if ((rc = ensure(ctx, b[0], (sizeof(double) + sizeof(int32_t)) * (size_t)nq * k))) return rc;
-    if ((rc = ensure(ctx, b[1], (sizeof(double) + sizeof(uint32_t)) * (size_t)nq))) return rc;
-    if ((rc = ensure(ctx, b[2], (sizeof(double) + sizeof(int32_t)) * (size_t)nq * cap))) return rc;
-    if ((rc = ensure(ctx, b[3], sizeof(int64_t) * (size_t)nq + 256))) return rc;
-    double* sub_d = (double*)b[0].p;
-    int32_t* sub_i = (int32_t*)(sub_d + (size_t)nq * k);
-    double* tau = (double*)b[1].p;
-    uint32_t* cnt = (uint32_t*)(tau + nq);
-    double* cd = (double*)b[2].p;
-    int32_t* ci = (int32_t*)(cd + (size_t)nq * cap);
-    uint32_t* n_over = (uint32_t*)b[3].p;
-    int64_t* over = (int64_t*)((char*)b[3].p + 256);
-    const unsigned g1 = (unsigned)((nq + kBlock - 1) / kBlock);
-    // 1. bound from the subset [0, n / 16)
-    if ((rc = knn_scan_path(ctx, d_q, nq, k, radius, sub_i, sub_d, n / 16, nullptr, nullptr))) return rc;
-    hipLaunchKernelGGL(knn_bound, dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)sub_d, nq, k, radius, tau);
-    // 2. collect over the whole tree
-    if ((rc = launch_zero_u32(ctx, cnt, nq))) return rc;
-    if ((rc = launch_zero_u32(ctx, n_over, 1))) return rc;
     int64_t chunk;
     const int64_t splits = plan_splits(ctx, nq, n, &chunk);
+    // per (chunk, query) region: about twice a chunk's share of the 16 k
+    // nodes a bound from a 1/16 prefix admits on spread-out trees; 32 at least
+    const int cap = (int)std::max<int64_t>(32, (32 * (int64_t)K + splits - 1) / splits);
+    int rc;
+    DevBuf* b = ctx->knn_sel;
+    if ((rc = ensure(ctx, b[2], sizeof(int32_t) * (size_t)nq * splits * cap))) return rc;
+    if ((rc = ensure(ctx, b[3], sizeof(int64_t) * (size_t)nq + sizeof(uint32_t) * (size_t)nq * splits + 256)))
+        return rc;
+    int32_t* ci = (int32_t*)b[2].p;
+    uint32_t* n_over = (uint32_t*)b[3].p;
+    int64_t* over = (int64_t*)((char*)b[3].p + 256);
+    uint32_t* cnt = (uint32_t*)(over + nq);
+    if ((rc = launch_zero_u32(ctx, n_over, 1))) return rc;
+    const unsigned g1 = (unsigned)((nq + kBlock - 1) / kBlock);
     const dim3 grid(g1, (unsigned)splits);
-    bool ok = false;
     switch (D) {
-        case 2: ok = dispatch_collect<2>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 3: ok = dispatch_collect<3>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 4: ok = dispatch_collect<4>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 5: ok = dispatch_collect<5>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 6: ok = dispatch_collect<6>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 7: ok = dispatch_collect<7>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 8: ok = dispatch_collect<8>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 10: ok = dispatch_collect<10>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        case 20: ok = dispatch_collect<20>(ctx, grid, n, chunk, d_q, nq, tau, cap, cd, ci, cnt); break;
-        default: break;
+        case 2: dispatch_collect<2>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 3: dispatch_collect<3>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 4: dispatch_collect<4>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 5: dispatch_collect<5>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 6: dispatch_collect<6>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 7: dispatch_collect<7>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 8: dispatch_collect<8>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 10: dispatch_collect<10>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        case 20: dispatch_collect<20>(ctx, grid, n, chunk, d_q, nq, tau, cap, (int)splits, k, radius, ci, cnt, out_d, out_i, over, n_over); break;
+        default: return fail(ctx, MPTG_E_STATE, "no collect kernel for this dimension");
     }
-    if (!ok) return fail(ctx, MPTG_E_STATE, "no collect kernel for this dimension");
     MPTG_HIP(ctx, hipGetLastError());
-    // 3. select
-    if (K == 20)
-        hipLaunchKernelGGL((knn_select<20>), dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)cd,
-                           (const int32_t*)ci, (const uint32_t*)cnt, cap, nq, k, d_dist, d_idx, over, n_over);
-    else
-        hipLaunchKernelGGL((knn_select<32>), dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)cd,
-                           (const int32_t*)ci, (const uint32_t*)cnt, cap, nq, k, d_dist, d_idx, over, n_over);
-    MPTG_HIP(ctx, hipGetLastError());
-    // 4. overflowed queries: the k-list scan, gathered and scattered back
     uint32_t m = 0;
     MPTG_HIP(ctx, hipMemcpyAsync(&m, n_over, sizeof(m), hipMemcpyDeviceToHost, ctx->stream));
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -654,10 +665,35 @@ static int knn_select_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, 
                        ctx->stream, d_q, D, (const int64_t*)over, (int64_t)m, gq);
     if ((rc = knn_scan_path(ctx, gq, m, k, radius, gi, gd, n, nullptr, nullptr))) return rc;
     hipLaunchKernelGGL(knn_scatter, dim3((unsigned)(((int64_t)m * k + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       ctx->stream, (const double*)gd, (const int32_t*)gi, (const int64_t*)over, (int64_t)m, k, d_dist,
-                       d_idx);
+                       ctx->stream, (const double*)gd, (const int32_t*)gi, (const int64_t*)over, (int64_t)m, k, out_d,
+                       out_i);
     MPTG_HIP(ctx, hipGetLastError());
+    (void)K;
     return MPTG_OK;
+}
+
+// k > 1 over a large tree, refined over growing tree prefixes: the exact k
+// best over the first 1/256 of the tree (k-list scan) bound every query;
+// collect within the bound over the first 1/16 and select its exact k best,
+// which bound the whole tree's; collect and select over the whole tree.  A
+// prefix's k-th distance is no smaller than the whole tree's, the collect
+// keeps every node within the bound and the select orders candidates by
+// exact (distance, index), so the result equals the k-list scan's.
+static int knn_select_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+                           double* d_dist, int64_t n) {
+    int rc;
+    DevBuf* b = ctx->knn_sel;
+    if ((rc = ensure(ctx, b[0], (sizeof(double) + sizeof(int32_t)) * (size_t)nq * k))) return rc;
+    if ((rc = ensure(ctx, b[1], sizeof(double) * (size_t)nq))) return rc;
+    double* sub_d = (double*)b[0].p;
+    int32_t* sub_i = (int32_t*)(sub_d + (size_t)nq * k);
+    double* tau = (double*)b[1].p;
+    const unsigned g1 = (unsigned)((nq + kBlock - 1) / kBlock);
+    if ((rc = knn_scan_path(ctx, d_q, nq, k, radius, sub_i, sub_d, n / 256, nullptr, nullptr))) return rc;
+    hipLaunchKernelGGL(knn_bound, dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)sub_d, nq, k, radius, tau);
+    if ((rc = knn_refine(ctx, d_q, nq, k, radius, tau, n / 16, sub_d, sub_i))) return rc;
+    hipLaunchKernelGGL(knn_bound, dim3(g1), dim3(kBlock), 0, ctx->stream, (const double*)sub_d, nq, k, radius, tau);
+    return knn_refine(ctx, d_q, nq, k, radius, tau, n, d_dist, d_idx);
 }
 
 constexpr int64_t kSelectMinTree = 1 << 16;  // smaller trees: the k-list scan
