@@ -107,7 +107,7 @@ struct mptg_ctx {
     // kNN k > 1 over large trees (bound, collect, select): subset results,
     // bounds + counts, candidates, overflow list, gathered queries, results
     mptg::DevBuf knn_sel[6];
-    mptg::DevBuf scratch[12];  // 0-3 host staging, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
+    mptg::DevBuf scratch[12];  // 0-2 host staging, 3 kNN seed bounds, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
                                // 9 ranked ordinals, 10 continuation list
     // pinned host staging
     void* pinned = nullptr;

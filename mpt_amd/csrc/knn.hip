@@ -36,6 +36,11 @@ namespace mptg {
 namespace {
 
 constexpr int kBlock = 256;
+// Fewest tree nodes per split of the scan: small trees still spread over
+// many waves, while every split's partial list stays worth its merge.
+constexpr int64_t kMinSplitNodes = 64;
+// Tree nodes every k = 1 query measures first, for a starting bound.
+constexpr int64_t kSeedNodes = 32;
 
 __device__ __forceinline__ bool better(double a, int ia, double b, int ib) {
     return a < b || (a == b && ia < ib);
@@ -115,10 +120,10 @@ __device__ __forceinline__ float prefilter_threshold(double dk, double S) {
 }
 
 // Splits actually used for a tree of n nodes when the launch was planned
-// for `planned` (a graph captured for a larger tree): at least 256 nodes
+// for `planned` (a graph captured for a larger tree): at least kMinSplitNodes
 // per split, as the host plan, and no empty split.
 __device__ __forceinline__ int64_t knn_used_splits(int64_t n, int64_t planned, int64_t* chunk) {
-    int64_t s = (n + 255) / 256;
+    int64_t s = (n + kMinSplitNodes - 1) / kMinSplitNodes;
     s = s < planned ? s : planned;
     s = s > 1 ? s : 1;
     *chunk = n > 0 ? (n + s - 1) / s : 0;
@@ -137,12 +142,29 @@ __device__ __forceinline__ bool knn_extent(const int64_t* n_dev, const int32_t* 
     return true;
 }
 
+// Nearest of the tree's first kSeedNodes nodes to each query (exact FP64
+// distance): a starting bound for the k = 1 scan.
+template <int D, int METRIC>
+__global__ __launch_bounds__(kBlock) void knn_seed(const double* __restrict__ x64, int64_t n,
+                                                   const double* __restrict__ q, int64_t nq,
+                                                   double* __restrict__ seed, const int64_t* n_dev,
+                                                   const int32_t* skip) {
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (qi >= nq || (skip && *skip)) return;
+    if (n_dev) n = *n_dev;
+    const int64_t ns = n < kSeedNodes ? n : kSeedNodes;
+    double best = __builtin_inf();
+    for (int64_t j = 0; j < ns; ++j) best = __builtin_fmin(best, exact_dist<D, METRIC>(q + qi * D, x64 + j * D));
+    seed[qi] = best;
+}
+
 template <int D, int METRIC, int K>
 __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32, const double* __restrict__ x64,
                                                    int64_t n, int64_t chunk, const double* __restrict__ q,
                                                    int64_t nq, double radius, double pbound,
                                                    double* __restrict__ out_d, int32_t* __restrict__ out_i,
-                                                   const int64_t* n_dev, const int32_t* skip) {
+                                                   const int64_t* n_dev, const int32_t* skip,
+                                                   const double* __restrict__ seed) {
     constexpr int DP = (D <= 2) ? 2 : (D <= 4 ? 4 : (D <= 8 ? 8 : ((D + 3) / 4) * 4));
     if (!knn_extent(n_dev, skip, &n, &chunk)) return;
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -162,6 +184,12 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
     const double S = mq + pbound;
     TopK<K> top;
     top.init();
+    // seed bound (k = 1): the nearest of the tree's first nodes, measured by
+    // knn_seed.  The nearest node overall is no farther, so it still passes
+    // the prefilter in its own chunk; other chunks skip the nodes beyond the
+    // bound (their partial lists may then hold nothing) and the merge of the
+    // chunks' minima is unchanged
+    if (seed) radius = __builtin_fmin(radius, seed[qc]);
     float rthr = prefilter_threshold<D, METRIC>(radius, S);
     float thr = rthr;
     if (!active) thr = -1.0f;  // inactive lanes never take the slow path
@@ -249,13 +277,41 @@ __global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ p
     }
     TopK<K> top;
     top.init();
-    for (int s = 0; s < splits; ++s) {
-        const int64_t base = ((int64_t)s * nq + qi) * K;
-        for (int j = 0; j < K; ++j) {
-            double d = pd[base + j];
-            int32_t id = pi[base + j];
-            if (id == INT_MAX) break;
-            top.insert(d, id);
+    if constexpr (K == 1) {
+        // the minimum over the splits, eight loads in flight at a time
+        int s = 0;
+        for (; s + 8 <= splits; s += 8) {
+            double d[8];
+            int32_t id[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                d[u] = pd[(int64_t)(s + u) * nq + qi];
+                id[u] = pi[(int64_t)(s + u) * nq + qi];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (better(d[u], id[u], top.d[0], top.i[0])) {
+                    top.d[0] = d[u];
+                    top.i[0] = id[u];
+                }
+        }
+        for (; s < splits; ++s) {
+            const double d = pd[(int64_t)s * nq + qi];
+            const int32_t id = pi[(int64_t)s * nq + qi];
+            if (better(d, id, top.d[0], top.i[0])) {
+                top.d[0] = d;
+                top.i[0] = id;
+            }
+        }
+    } else {
+        for (int s = 0; s < splits; ++s) {
+            const int64_t base = ((int64_t)s * nq + qi) * K;
+            for (int j = 0; j < K; ++j) {
+                double d = pd[base + j];
+                int32_t id = pi[base + j];
+                if (id == INT_MAX) break;
+                top.insert(d, id);
+            }
         }
     }
 #pragma unroll
@@ -470,13 +526,17 @@ struct ScanExtent {
     int64_t n, chunk;
     const int64_t* n_dev;
     const int32_t* skip;
+    double* seed;  // k = 1: per-query starting bound (nullptr: none)
 };
 
 template <int D, int METRIC, int K>
 void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q, int64_t nq, double radius,
                  double* pd, int32_t* pi) {
+    if (K == 1 && x.seed)
+        hipLaunchKernelGGL((knn_seed<D, METRIC>), dim3(grid.x), dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq,
+                           x.seed, x.n_dev, x.skip);
     hipLaunchKernelGGL((knn_scan<D, METRIC, K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n, x.chunk,
-                       q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip);
+                       q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip, (const double*)(K == 1 ? x.seed : nullptr));
 }
 
 template <int D, int K>
@@ -515,12 +575,12 @@ bool dispatch_dim(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q
 }  // namespace
 
 // Chooses the number of tree chunks so that a launch has ~8 waves per SIMD
-// worth of work, with at least 256 nodes per chunk.
+// worth of work, with at least kMinSplitNodes nodes per chunk.
 static int64_t choose_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int K) {
     int64_t qblocks = (nq + kBlock - 1) / kBlock;
     int64_t want_blocks = (int64_t)ctx->cus * ctx->tune.knn_blocks_per_cu;
     int64_t s = (want_blocks + qblocks - 1) / qblocks;
-    int64_t max_s = (n + 255) / 256;
+    int64_t max_s = (n + kMinSplitNodes - 1) / kMinSplitNodes;
     if (s > max_s) s = max_s;
     if (s < 1) s = 1;
     // the merge reads splits * K candidates per query, one latency each
@@ -541,6 +601,7 @@ int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
     const int64_t splits = plan_splits(ctx, nq, tree_n, &chunk, K);
     int rc;
     if ((rc = ensure(ctx, ctx->scratch[4], sizeof(double) * K * nq * splits))) return rc;
+    if (K == 1 && (rc = ensure(ctx, ctx->scratch[3], sizeof(double) * nq))) return rc;
     return ensure(ctx, ctx->scratch[5], sizeof(int32_t) * K * nq * splits);
 }
 
@@ -551,11 +612,18 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     // with n_dev, tree_n is only the size the split count is planned for
     int64_t n = tree_n, chunk;
     const int64_t splits = plan_splits(ctx, nq, n, &chunk, K);
-    const ScanExtent ext{n, chunk, n_dev, skip};
     size_t pbytes_d = sizeof(double) * K * nq * splits, pbytes_i = sizeof(int32_t) * K * nq * splits;
     int rc;
     if ((rc = ensure(ctx, ctx->scratch[4], pbytes_d))) return rc;
     if ((rc = ensure(ctx, ctx->scratch[5], pbytes_i))) return rc;
+    // a seed bound pays off when the tree is split many ways: every chunk
+    // starts from it
+    double* seed = nullptr;
+    if (K == 1 && splits >= 8 && ctx->metric != MPTG_METRIC_SE3) {
+        if ((rc = ensure(ctx, ctx->scratch[3], sizeof(double) * nq))) return rc;
+        seed = (double*)ctx->scratch[3].p;
+    }
+    const ScanExtent ext{n, chunk, n_dev, skip, seed};
     double* pd = (double*)ctx->scratch[4].p;
     int32_t* pi = (int32_t*)ctx->scratch[5].p;
     dim3 grid((unsigned)((nq + kBlock - 1) / kBlock), (unsigned)splits);
