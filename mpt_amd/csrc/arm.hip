@@ -1,4 +1,5 @@
-// Planar-arm link() on the GPU: one wave per edge, 64 waypoints per round.
+// Planar-arm link() on the GPU: a packed round (16 edges per wave, ranks
+// 0..3), then a continuation of 64-rank rounds over several waves per edge.
 //
 // Reference semantics (demo/link_manipulator_scenario.hpp:118-138): valid(a),
 // valid(b), then the pre-order of recursive midpoints (a+b)/2 down to
@@ -12,21 +13,19 @@
 //     e = (h + 1) 2^-50 M,
 // every node above depth h splits and every node at depth h is a leaf: the
 // tree is perfect with 2^h - 1 waypoints and pre-order ordinal -> path is a
-// closed form.  Lanes then take 64 consecutive ordinals of the chosen order,
-// rebuild their midpoint from (a, b) along the path (bit-identical to the
-// reference's recursion), run valid() and the wave ballots for the first
-// failure.  Edges whose widths fall within e of tol at depth h-1 or h (never
+// closed form.  Lanes take consecutive ranks of the chosen order, rebuild
+// their midpoint from (a, b) along the path (bit-identical to the
+// reference's recursion), run valid() and ballot for the first failure.
+// Edges whose widths fall within e of tol at depth h-1 or h (never
 // observed; kept exact anyway) are walked by one lane with the stackless
 // exact walk.
 //
 // valid() per lane: forward kinematics with the project sincos, then every
-// link segment against every disk (in the continuation, after the wave-wide
-// disk culling of arm_ok_wave: a chunk's 64 waypoints come from one edge).  Disks are read with wave-uniform (scalar)
-// loads.  A disk whose centre is farther than (link length + r + R) from the
-// segment start cannot satisfy dist <= r + R; the test skips it after a
-// bound check with a relative margin of 2^-30 that dominates the rounding of
-// the exact expression, so the skip never changes a verdict.  Disks that pass
-// the bound get the reference expression bit-for-bit.
+// link segment against the disks that the grid culling leaves (the wave's
+// box or the lane's own cells, arm_ok_link), behind a far-disk bound with a
+// relative margin of 2^-30 that dominates the rounding of the exact
+// expression, so a skip never changes a verdict.  Disks that pass the bound
+// get the reference expression bit-for-bit.
 #include "bisect_dev.hpp"
 
 namespace mptg {

@@ -736,7 +736,6 @@ static int knn_refine(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, doubl
                        ctx->stream, (const double*)gd, (const int32_t*)gi, (const int64_t*)over, (int64_t)m, k, out_d,
                        out_i);
     MPTG_HIP(ctx, hipGetLastError());
-    (void)K;
     return MPTG_OK;
 }
 
