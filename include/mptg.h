@@ -263,7 +263,8 @@ typedef struct mptg_tune_params {
     int32_t mesh_edge_waves;        /* waves sharing one mesh edge's waypoints (1..32) */
     int32_t knn_select;             /* k > 1 on trees of 2^16 nodes or more: 1 bound, collect and
                                        select (registers of k = 1 in the scan), 0 the k-list scan */
-    int32_t _pad;
+    int32_t knn_grid;               /* L1/L2 in 2..8 dimensions on trees of 2^16 nodes or more: 1 a
+                                       uniform grid built per call, searched cell by cell; 0 the scans */
 } mptg_tune_params;
 
 enum {

@@ -107,6 +107,10 @@ struct mptg_ctx {
     // kNN k > 1 over large trees (bound, collect, select): subset results,
     // bounds + counts, candidates, overflow list, gathered queries, results
     mptg::DevBuf knn_sel[6];
+    // kNN over a uniform grid: cell of each node, counting-sort arrays,
+    // sorted rows and ids, overflow list, gathered overflow queries +
+    // results, query cells and order
+    mptg::DevBuf knn_grid[6];
     mptg::DevBuf scratch[12];  // 0-2 host staging, 3 kNN seed bounds, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
                                // 9 ranked ordinals, 10 continuation list
     // pinned host staging
@@ -128,7 +132,7 @@ struct mptg_ctx {
     // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
     // persistent DMV long-edge kernel and of the mesh kernels, waves per edge
     // of the arm continuation, arm link disk culling (0 wave box, 1 per lane)
-    mptg_tune_params tune{8, 16, 32, 2, 0, 2, 1, 0};
+    mptg_tune_params tune{8, 16, 32, 2, 0, 2, 1, 1};
     hipEvent_t phase_ev[6] = {};
 };
 
@@ -176,6 +180,12 @@ double planner_coord_bound(const mptg_ctx* ctx);
 // Allocates the kNN partial-result buffers for nq queries over a tree of
 // tree_n nodes (what launch_knn would allocate).
 int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n);
+// Grid kNN (knn_grid.hip): whether it serves a tree of n nodes in this
+// context, and the search itself; the m queries it leaves (*m, listed at
+// *over on the device) are the caller's to answer.
+bool knn_grid_applies(const mptg_ctx* ctx, int64_t n);
+int knn_grid_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx, double* d_dist,
+                  int64_t n, int64_t** over, uint32_t* m);
 // Zeroes n 32-bit words with a kernel (graph-capturable, unlike a memset).
 int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n);
 // Sets n 32-bit words to v with a kernel.

@@ -771,9 +771,37 @@ int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radiu
     if (nq > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 queries");
     if (!(radius >= 0)) return fail(ctx, MPTG_E_ARG, "radius must be >= 0");
     const bool sel_dim = ctx->dim <= 8 || ctx->dim == 10 || ctx->dim == 20;
-    if (k > 1 && nq > 0 && tree_n >= kSelectMinTree && !n_dev && !skip && !ctx->capturing && ctx->tune.knn_select &&
-        ctx->metric != MPTG_METRIC_SE3 && ctx->dim >= 2 && sel_dim)
-        return knn_select_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n);
+    const bool eager = nq > 0 && !n_dev && !skip && !ctx->capturing;
+    const bool select = eager && k > 1 && tree_n >= kSelectMinTree && ctx->tune.knn_select &&
+                        ctx->metric != MPTG_METRIC_SE3 && ctx->dim >= 2 && sel_dim;
+    if (eager && knn_grid_applies(ctx, tree_n)) {
+        // the grid answers most queries; the rest take the scans, gathered
+        // into a compact batch and scattered back
+        int64_t* over = nullptr;
+        uint32_t m = 0;
+        int rc = knn_grid_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n, &over, &m);
+        if (rc || m == 0) return rc;
+        const int D = ctx->dim;
+        DevBuf& g = ctx->knn_grid[4];
+        if ((rc = ensure(ctx, g, (sizeof(double) * D + sizeof(double) * k + sizeof(int32_t) * k) * (size_t)m)))
+            return rc;
+        double* gq = (double*)g.p;
+        double* gd = gq + (size_t)m * D;
+        int32_t* gi = (int32_t*)(gd + (size_t)m * k);
+        hipLaunchKernelGGL(knn_gather, dim3((unsigned)(((int64_t)m * D + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, d_q, D, (const int64_t*)over, (int64_t)m, gq);
+        // the select path streams the tree three times whatever the batch:
+        // few queries take the k-list scan
+        rc = select && m >= 2048 ? knn_select_path(ctx, gq, m, k, radius, gi, gd, tree_n)
+                                 : knn_scan_path(ctx, gq, m, k, radius, gi, gd, tree_n, nullptr, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(knn_scatter, dim3((unsigned)(((int64_t)m * k + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, (const double*)gd, (const int32_t*)gi, (const int64_t*)over, (int64_t)m, k,
+                           d_dist, d_idx);
+        MPTG_HIP(ctx, hipGetLastError());
+        return MPTG_OK;
+    }
+    if (select) return knn_select_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n);
     return knn_scan_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n, n_dev, skip);
 }
 

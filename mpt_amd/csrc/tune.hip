@@ -97,7 +97,7 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
         return fail(ctx, MPTG_E_RANGE, "mesh waves per edge must be in 1..32");
     if (p->knn_select != 0 && p->knn_select != 1) return fail(ctx, MPTG_E_RANGE, "knn_select is 0 or 1");
     ctx->tune = *p;
-    ctx->tune._pad = 0;
+    if (p->knn_grid != 0 && p->knn_grid != 1) return fail(ctx, MPTG_E_RANGE, "knn_grid is 0 or 1");
     return MPTG_OK;
 }
 

@@ -73,6 +73,7 @@ def test_knn_select_path_equals_klist_scan_and_oracle(ctx_factory, metric, dim, 
     q = rng.uniform(-math.pi, math.pi, (3000, dim))
     q[:50] = pts[7000:7050]
     sample = rng.choice(q.shape[0], 120, replace=False)
+    ctx.tune_set(knn_grid=0)  # the grid would answer these trees first
     for tree in (pts, far):
         ctx.space(metric, dim)
         ctx.tree_clear()
@@ -86,7 +87,50 @@ def test_knn_select_path_equals_klist_scan_and_oracle(ctx_factory, metric, dim, 
             assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), radius
             oi, od = o.knn(tree, q[sample], k, radius=radius, threads=16)
             assert np.array_equal(res[1][0][sample], oi) and np.array_equal(res[1][1][sample], od), radius
-    ctx.tune_set(knn_select=1)
+    ctx.tune_set(knn_select=1, knn_grid=1)
+
+
+@pytest.mark.parametrize("metric,dim", [(scenes.METRIC_L2, 2), (scenes.METRIC_L2, 5), (scenes.METRIC_L1, 3),
+                                        (scenes.METRIC_L2, 8)])
+def test_knn_grid_equals_scans_and_oracle(ctx_factory, metric, dim):
+    """The uniform-grid search (trees of 2^16 nodes or more, 2..8 dimensions)
+    equals the scans on every query and the oracle on a sample: uniform and
+    clustered trees (a cell holding a tenth of the tree), duplicates, nodes on
+    cell edges and at the coordinate bound, queries outside the bound, finite
+    radii, and k up to 32 (own cells with fewer than k nodes go to the scans)."""
+    _say("knn grid", metric, dim)
+    ctx = ctx_factory()
+    rng = np.random.default_rng(90 + dim)
+    n = (1 << 16) + 777
+    uni = rng.uniform(-math.pi, math.pi, (n, dim))
+    uni[100:200] = uni[300:400]                                   # duplicates
+    uni[400:420] = np.round(uni[400:420] * 4) / 4                 # on cell edges of power-of-two grids
+    uni[420, :] = math.pi
+    uni[421, :] = -math.pi
+    clu = uni.copy()
+    clu[: n // 10] = 0.5 + rng.normal(0, 1e-3, (n // 10, dim))    # one dense cluster
+    q = rng.uniform(-math.pi, math.pi, (2500, dim))
+    q[:40] = uni[300:340]
+    q[40:60] = clu[:20]
+    q[60:70] = rng.uniform(-30, 30, (10, dim))                    # outside the bound
+    q[70:80] = np.round(q[70:80] * 4) / 4
+    sample = np.concatenate([np.arange(80), rng.choice(np.arange(80, q.shape[0]), 70, replace=False)])
+    o = oracle.Oracle(_plain(metric, dim))
+    for tree in (uni, clu):
+        ctx.space(metric, dim)
+        ctx.tree_clear()
+        ctx.tree_insert(tree)
+        for k in (1, 7, 20, 32):
+            for radius in (math.inf, 0.7):
+                res = {}
+                for grid in (0, 1):
+                    ctx.tune_set(knn_grid=grid)
+                    res[grid] = ctx.knn(q, k, radius)
+                assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), (k, radius)
+                if k in (1, 20):
+                    oi, od = o.knn(tree, q[sample], k, radius=radius, threads=16)
+                    assert np.array_equal(res[1][0][sample], oi) and np.array_equal(res[1][1][sample], od), (k, radius)
+    ctx.tune_set(knn_grid=1)
 
 
 def test_dmv_long_grid_every_shape_is_exact(ctx_factory):
