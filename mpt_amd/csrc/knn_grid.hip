@@ -25,6 +25,7 @@
 // so results equal the brute-force scan's bit for bit.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 #include "internal.hpp"
 #include "space_dev.hpp"
@@ -151,30 +152,29 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 
 // The wave's k-list: lane j < k holds the j-th best (distance, node id),
 // ascending; (kd, ki) mirror lane k - 1's entry (the k-th best) as
-// wave-uniform values.
+// wave-uniform values.  +inf comes in as a value (a kernel argument), not a
+// constant: this toolchain encodes a 64-bit scalar move of the constant
+// 0x7ff0000000000000 with a 32-bit literal holding its low half, so a
+// uniform +inf held in scalar registers would read as 0.0.
 struct WaveList {
     double ld;
     int32_t li;
     double kd;
     int32_t ki;
     int k;
-    __device__ __forceinline__ void init(int k_) {
-        ld = __builtin_inf();
+    __device__ __forceinline__ void init(int k_, double inf) {
+        ld = inf;
         li = INT_MAX;
-        kd = __builtin_inf();
+        kd = inf;
         ki = INT_MAX;
         k = k_;
     }
     // Inserts a wave-uniform candidate that beats (kd, ki): every entry
     // after it moves one lane up.
     __device__ __forceinline__ void insert(double cd, int32_t ci, int lane) {
-        double pd = __shfl_up(ld, 1);
-        int32_t pi = __shfl_up(li, 1);
-        if (lane == 0) {
-            pd = -__builtin_inf();
-            pi = INT_MIN;
-        }
-        const bool shift = better(cd, ci, pd, pi);
+        const double pd = __shfl_up(ld, 1);
+        const int32_t pi = __shfl_up(li, 1);
+        const bool shift = lane != 0 && better(cd, ci, pd, pi);
         const bool here = !shift && better(cd, ci, ld, li);
         ld = shift ? pd : (here ? cd : ld);
         li = shift ? pi : (here ? ci : li);
@@ -244,9 +244,9 @@ __global__ __launch_bounds__(kB) void grid_query(const double* __restrict__ sx, 
                                                  const int32_t* __restrict__ sid,
                                                  const uint32_t* __restrict__ start, GridArgs ga,
                                                  const double* __restrict__ q, const uint32_t* __restrict__ qorder,
-                                                 int64_t nq, int k, double radius, double* __restrict__ out_d,
-                                                 int32_t* __restrict__ out_i, int64_t* __restrict__ over,
-                                                 uint32_t* __restrict__ n_over) {
+                                                 int64_t nq, int k, double radius, double inf,
+                                                 double* __restrict__ out_d, int32_t* __restrict__ out_i,
+                                                 int64_t* __restrict__ over, uint32_t* __restrict__ n_over) {
     const int lane = threadIdx.x & 63;
     const int64_t t = (int64_t)blockIdx.x * (kB / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (t >= nq) return;
@@ -263,13 +263,13 @@ __global__ __launch_bounds__(kB) void grid_query(const double* __restrict__ sx, 
     }
     own = (uint32_t)__builtin_amdgcn_readfirstlane((int)own);
     WaveList wl;
-    wl.init(k);
+    wl.init(k, inf);
     wave_scan_cell<D, METRIC>(sx, n, sid, start, own, qq, radius, wl, lane);
 
     const double r = __builtin_fmin(wl.kd, radius);
     int lo[D], ext[D];
     int64_t cells = 1;
-    if (r < __builtin_inf()) {
+    if (r < inf) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             // a node within r has |x - q| <= r (1 + 2^-50) per axis; the
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kB) void grid_query(const double* __restrict__ sx, 
             cells *= ext[d];
         }
     }
-    if (!(r < __builtin_inf())) {
+    if (!(r < inf)) {
         if (lane == 0) over[atomicAdd(n_over, 1u)] = qi;
         return;
     }
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kB) void grid_query(const double* __restrict__ sx, 
     }
     if (lane < k) {
         const bool have = wl.li != INT_MAX;
-        out_d[qi * k + lane] = have ? wl.ld : __builtin_inf();
+        out_d[qi * k + lane] = have ? wl.ld : inf;
         out_i[qi * k + lane] = have ? wl.li : -1;
     }
 }
@@ -412,7 +412,7 @@ void launch_query(mptg_ctx* ctx, int64_t n, const GridBufs& gb, const GridArgs& 
     const unsigned g = (unsigned)((nq + kB / 64 - 1) / (kB / 64));
     hipLaunchKernelGGL((grid_query<D, METRIC>), dim3(g), dim3(kB), 0, ctx->stream, (const double*)gb.sx, n,
                        (const int32_t*)gb.sid, (const uint32_t*)gb.start, ga, q, (const uint32_t*)gb.qorder, nq, k,
-                       radius, od, oi, over, n_over);
+                       radius, std::numeric_limits<double>::infinity(), od, oi, over, n_over);
 }
 
 template <int D>

@@ -115,8 +115,12 @@ def test_knn_grid_equals_scans_and_oracle(ctx_factory, metric, dim):
     q[60:70] = rng.uniform(-30, 30, (10, dim))                    # outside the bound
     q[70:80] = np.round(q[70:80] * 4) / 4
     sample = np.concatenate([np.arange(80), rng.choice(np.arange(80, q.shape[0]), 70, replace=False)])
+    # every node in one corner but one: most queries' own cells are empty,
+    # and with an infinite radius over 2048 of them take the scans as a batch
+    corner = rng.uniform(-math.pi, -math.pi + 0.4, (n, dim))
+    corner[-1] = math.pi
     o = oracle.Oracle(_plain(metric, dim))
-    for tree in (uni, clu):
+    for tree in (uni, clu, corner):
         ctx.space(metric, dim)
         ctx.tree_clear()
         ctx.tree_insert(tree)
