@@ -407,6 +407,8 @@ def main():
     ap.add_argument("--train-samples", type=int, default=1 << 22)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--kernels", action="store_true", help="profiling mode: the dominant kernels only")
+    ap.add_argument("--leg", choices=["nn", "grid", "boxes", "se3"],
+                    help="profiling mode: one extra workload alone, its JSON on one line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -423,6 +425,14 @@ def main():
     def leg(fn, *a):
         ctx.tune_set(**defaults)
         return fn(ctx, *a)
+
+    if args.leg:
+        fns = {"nn": (bench_nn, args, timer), "grid": (bench_grid, args, timer), "boxes": (bench_boxes, args),
+               "se3": (bench_se3, args)}
+        fn, *a = fns[args.leg]
+        print(json.dumps({"leg": args.leg, "result": leg(fn, *a)}))
+        ctx.close()
+        return
 
     log("arm planner ...")
     arm, roof, sc = leg(bench_arm, args, timer)
