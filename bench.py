@@ -90,12 +90,13 @@ def run_planner(ctx, sc, args, order, batch, graph=0):
     return res, nodes, parents
 
 
-def tune_planner(ctx, sc, args, batch):
+def tune_planner(ctx, sc, args, batch, order=None):
     """Iterations per graph launch chosen by the autotuner on a tree of its
-    own (the choice does not change any tree)."""
+    own (the choice does not change any tree), for the planner in `order`."""
     import mpt_amd
     ctx.tree_clear()
-    ctx.prrt_init(sc, seed=args.seed + 1000, batch=batch, order=mpt_amd.ORDER_REFERENCE)
+    ctx.prrt_init(sc, seed=args.seed + 1000, batch=batch,
+                  order=mpt_amd.ORDER_REFERENCE if order is None else order)
     ctx.prrt_add_start(sc.start)
     rep = ctx.autotune_prrt()
     ctx.prrt_graph(0)
@@ -143,7 +144,11 @@ def bench_arm(ctx, args, timer):
     log("  surrogate training ...")
     train = surrogate.train(ctx, sc, nodes_p, n_samples=args.train_samples, epochs=args.train_epochs, seed=5)
     log("  surrogate-ordered planner ...")
-    sur, nodes_s, par_s = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B, G)
+    # the surrogate planner has launches of its own (scoring, ranking): its
+    # graph size is tuned for it
+    tune_s = tune_planner(ctx, sc, args, B, mpt_amd.ORDER_SURROGATE)
+    Gs = tune_s["chosen"]
+    sur, nodes_s, par_s = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B, Gs)
     sur_steps, nodes_ss, par_ss = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B, 0)
     same_tree = all(bool(np.array_equal(nodes_p, n) and np.array_equal(par_p, p))
                     for n, p in ((nodes_s, par_s), (nodes_ps, par_ps), (nodes_ss, par_ss)))
@@ -225,7 +230,8 @@ def bench_arm(ctx, args, timer):
             "executed_tflops": flops_p / t_kernel_p / 1e12, "executed_frac": flops_p / t_kernel_p / FP64_PEAK,
             "work": wk_p}
     res = {"plain": plain, "surrogate": sur, "plain_step_by_step": plain_steps,
-           "surrogate_step_by_step": sur_steps, "graph_autotune": tune, "tail_autotune": tune_link,
+           "surrogate_step_by_step": sur_steps, "graph_autotune": tune, "graph_autotune_surrogate": tune_s,
+           "tail_autotune": tune_link,
            "launch_shapes": ctx.tune_get(),
            "trees_identical": same_tree, "training": train,
            "edge_batch": edge_batch,

@@ -59,7 +59,8 @@ enum mptg_link_mode {
 
 enum mptg_check_order {
     MPTG_ORDER_REFERENCE = 0, /* the reference's waypoint order */
-    MPTG_ORDER_SURROGATE = 1  /* descending surrogate score (ties: reference order) */
+    MPTG_ORDER_SURROGATE = 1  /* reference ordinals 0-3 first, then 4-63 by descending surrogate
+                                 score (ties: reference order), then the rest in reference order */
 };
 
 typedef struct mptg_ctx mptg_ctx;

@@ -87,10 +87,19 @@ __device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[NE], int lane)
     }
 }
 
+// The surrogate order of an edge's waypoints (trees of h <= 9 levels):
+// reference ordinals 0 .. lead-1 first, then the other waypoints of the top
+// kTopLevels levels (63 spread evenly along the edge) by descending score
+// (ties by reference ordinal), then the deeper waypoints in reference order.
+// The collision region of an edge is usually wide, so the top levels are
+// where a score finds it; ordering the deeper waypoints buys no device time
+// (a colliding edge costs one continuation round whichever rank fails) and
+// would cost up to eight scored tiles per edge instead of one.
+//
 // Ranked ordinals of one edge per wave (edge list[item] for item <
-// *n_list): reference ordinals 0 .. lead-1 first, then the others by
-// descending score (ties by reference ordinal); writes the reference ordinal
-// of every rank to ranked[e * kScoreStride + rank].
+// *n_list): lane i holds top-tree node i (breadth-first); writes the
+// reference ordinal of ranks 0 .. 2^min(h, kTopLevels) - 2 to
+// ranked[e * kScoreStride + rank].
 __global__ __launch_bounds__(64) void rank_scores(const double* __restrict__ from, const double* __restrict__ to,
                                                   const int64_t* __restrict__ list,
                                                   const uint32_t* __restrict__ n_list, int dim, double tol,
@@ -116,28 +125,37 @@ __global__ __launch_bounds__(64) void rank_scores(const double* __restrict__ fro
     }
     (void)M;
     (void)regular;
+    static_assert((1 << kTopLevels) - 1 < kWave, "one top-tree node per lane");
     if (h > 9) return;  // more waypoints than score slots: reference order
-    const uint32_t W = (1u << h) - 1u;
-    uint64_t key[kScoreStride / kWave];
-#pragma unroll
-    for (int k = 0; k < kScoreStride / kWave; ++k) {
-        const uint32_t o = (uint32_t)(k * kWave + lane);
+    const uint32_t T = (1u << (h < kTopLevels ? h : kTopLevels)) - 1u;
+    uint64_t key[1] = {~0ull};
+    if ((uint32_t)lane < T) {
+        uint32_t path;
+        int depth;
+        bfs_path((uint32_t)lane, &path, &depth);
+        const uint32_t o = path_preorder(h, path, depth);
         // led ordinals sort first: every score key has a nonzero high word
-        key[k] = o < W ? (o < lead ? (uint64_t)o : score_key(scores[e * kScoreStride + o], o)) : ~0ull;
+        key[0] = o < lead ? (uint64_t)o : score_key(scores[e * kScoreStride + o], o);
     }
-    wave_bitonic_sort<kScoreStride / kWave>(key, lane);
-#pragma unroll
-    for (int k = 0; k < kScoreStride / kWave; ++k) {
-        const uint32_t r = (uint32_t)(k * kWave + lane);
-        if (r < W) ranked[e * kScoreStride + r] = (uint16_t)key[k];
-    }
+    wave_bitonic_sort<1>(key, lane);
+    if ((uint32_t)lane < T) ranked[e * kScoreStride + lane] = (uint16_t)key[0];
 }
 
-// Reference ordinal of the waypoint at `rank` in the chosen order.
+// (path, depth) of the waypoint at `rank` in the chosen order (tree of h
+// levels): surrogate ranks below the top tree's size come from the ranked
+// list, later ones are the deeper waypoints in reference order.
 template <int ORD>
-__device__ __forceinline__ uint32_t ordinal_at(const uint16_t* __restrict__ ranked, int64_t e, uint32_t rank) {
-    if constexpr (ORD == MPTG_ORDER_SURROGATE) return ranked[e * kScoreStride + rank];
-    return rank;
+__device__ __forceinline__ void node_at(int h, const uint16_t* __restrict__ ranked, int64_t e, uint32_t rank,
+                                        uint32_t* path, int* depth) {
+    if constexpr (ORD == MPTG_ORDER_SURROGATE) {
+        if (h <= 9) {
+            const uint32_t T = (1u << (h < kTopLevels ? h : kTopLevels)) - 1u;
+            if (rank < T) preorder_path(h, ranked[e * kScoreStride + rank], path, depth);
+            else deep_path(h, rank - T, path, depth);
+            return;
+        }
+    }
+    preorder_path(h, rank, path, depth);
 }
 
 constexpr int kHeadRanks = 4;                     // ranks per edge in the packed round
@@ -265,7 +283,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
             const bool active = r < W;
             uint32_t path;
             int depth;
-            preorder_path(h, ordinal_at<ORD>(ranked, e, active ? r : W - 1u), &path, &depth);
+            node_at<ORD>(h, ranked, e, active ? r : W - 1u, &path, &depth);
             const bool bad = !arm_ok_link<D, COUNT, LANE>(
                 sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, active, &wk);
             const uint64_t m = __ballot(bad && active);
@@ -326,7 +344,7 @@ static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to
         if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreStride * n))) return rc;
         if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kScoreStride * n))) return rc;
         float* scores = (float*)ctx->scratch[8].p;
-        if ((rc = surrogate_score_list(ctx, from, to, n, nullptr, cont, ncont, scores))) return rc;
+        if ((rc = surrogate_score_list(ctx, from, to, n, nullptr, cont, ncont, scores, true))) return rc;
         phase_mark(ctx, 2);
         hipLaunchKernelGGL(rank_scores, dim3((unsigned)n), dim3(kWave), 0, ctx->stream, from, to,
                            (const int64_t*)cont, (const uint32_t*)ncont, D, tol, (uint32_t)kHeadRanks,

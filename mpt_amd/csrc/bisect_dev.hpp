@@ -34,6 +34,46 @@ __device__ __forceinline__ void preorder_path(int h, uint32_t o, uint32_t* path,
     *depth = d;
 }
 
+// (path, depth) -> pre-order ordinal in a perfect tree with h levels (the
+// inverse of preorder_path: each step down adds one, a right step also
+// skips the left subtree of 2^(h-1-k) - 1 nodes).
+__device__ __forceinline__ uint32_t path_preorder(int h, uint32_t path, int depth) {
+    uint32_t o = 0;
+    for (int k = 0; k < depth; ++k) o += 1u + (((path >> k) & 1u) ? (1u << (h - 1 - k)) - 1u : 0u);
+    return o;
+}
+
+// Levels of the bisection tree the surrogate order ranks by score: the
+// 63 waypoints of depths 0..5 spread evenly along the edge.
+constexpr int kTopLevels = 6;
+
+// Node i of the top tree in breadth-first order (root 0, then each level
+// left to right) -> (path, depth); path bit k is step k (1 = right).
+__device__ __forceinline__ void bfs_path(uint32_t i, uint32_t* path, int* depth) {
+    const int d = 31 - __builtin_clz(i + 1u);
+    const uint32_t idx = i + 1u - (1u << d);  // first step in the top bit
+    uint32_t p = 0;
+    for (int k = 0; k < d; ++k) p |= ((idx >> (d - 1 - k)) & 1u) << k;
+    *path = p;
+    *depth = d;
+}
+
+// The k-th node of depth >= kTopLevels in pre-order (tree of h > kTopLevels
+// levels): pre-order visits the 2^kTopLevels subtrees rooted at depth
+// kTopLevels left to right, each one contiguously, so node k is node
+// k mod S of subtree k / S (S = 2^(h - kTopLevels) - 1 nodes each).
+__device__ __forceinline__ void deep_path(int h, uint32_t k, uint32_t* path, int* depth) {
+    const uint32_t S = (1u << (h - kTopLevels)) - 1u;
+    const uint32_t j = k / S;
+    uint32_t lp;
+    int ld;
+    preorder_path(h - kTopLevels, k % S, &lp, &ld);
+    uint32_t p = 0;
+    for (int t = 0; t < kTopLevels; ++t) p |= ((j >> (kTopLevels - 1 - t)) & 1u) << t;
+    *path = p | (lp << kTopLevels);
+    *depth = kTopLevels + ld;
+}
+
 // One coordinate of the midpoint of the node at `path`: the reference's
 // recursion (a+b)/2 restricted to that coordinate.
 __device__ __forceinline__ double coord_mid(double lo, double hi, uint32_t path, int depth) {

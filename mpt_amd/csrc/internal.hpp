@@ -214,8 +214,10 @@ int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t
 int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
                           float* scores);
 // Scores of the edges list[0 .. *n_list) (device count; at most n of them),
-// written at scores[e * 512 ...] as surrogate_score_edges does.
+// written at scores[e * 512 + reference ordinal] as surrogate_score_edges
+// does: every waypoint, or (top) those of the bisection tree's top
+// kTopLevels levels only.
 int surrogate_score_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
-                         const int64_t* list, const uint32_t* n_list, float* scores);
+                         const int64_t* list, const uint32_t* n_list, float* scores, bool top);
 
 }  // namespace mptg

@@ -422,11 +422,12 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
                                                       const uint32_t* __restrict__ n_list, double tol,
                                                       const double* __restrict__ lengths, float inv_total,
                                                       const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
-                                                      const float* __restrict__ p, float* __restrict__ scores) {
+                                                      const float* __restrict__ p, float* __restrict__ scores,
+                                                      int tiles, int top) {
     __shared__ __attribute__((aligned(16))) uint16_t buf[64 * kLdsRow];
     const int lane = threadIdx.x;
-    const int64_t item = blockIdx.x / (kScoreStride / 64);
-    const uint32_t c0 = (uint32_t)(blockIdx.x % (kScoreStride / 64)) * 64;
+    const int64_t item = blockIdx.x / tiles;
+    const uint32_t c0 = (uint32_t)(blockIdx.x % tiles) * 64;
     int64_t e = item;
     if (list) {
         if (item >= (int64_t)*n_list) return;
@@ -439,15 +440,19 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
     int h = 0;
     bool regular = false;
     if (!bisect_shape<D>(a, b, tol, &h, &regular)) return;
-    const uint32_t W = (1u << h) - 1u;
+    uint32_t W = (1u << h) - 1u;
     if (c0 >= W || W > (uint32_t)kScoreStride) return;
+    // top: row i is node i of the top tree (breadth-first), else reference
+    // ordinal c0 + i
+    if (top) W = (1u << (h < kTopLevels ? h : kTopLevels)) - 1u;
     // features of this lane's waypoint (rows beyond W are zero)
     const uint32_t o = c0 + (uint32_t)lane;
     uint16_t f[IN];
     if (o < W) {
         uint32_t path;
         int depth;
-        preorder_path(h, o, &path, &depth);
+        if (top) bfs_path(o, &path, &depth);
+        else preorder_path(h, o, &path, &depth);
         // dyadic position of the node's midpoint along the edge
         float t = 0.f, step = 0.5f;
 #pragma unroll 1
@@ -485,7 +490,16 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
             const uint32_t oo = c0 + (uint32_t)(st * 16 + lane);
-            if (oo < W) scores[e * kScoreStride + oo] = logit[st] + b4;
+            if (oo < W) {
+                uint32_t ord = oo;
+                if (top) {
+                    uint32_t path;
+                    int depth;
+                    bfs_path(oo, &path, &depth);
+                    ord = path_preorder(h, path, depth);
+                }
+                scores[e * kScoreStride + ord] = logit[st] + b4;
+            }
         }
     }
 }
@@ -647,25 +661,27 @@ static int sur_check(mptg_ctx* ctx, bool need_init) {
 
 int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
                           float* scores) {
-    return surrogate_score_list(ctx, from, to, n, mask, nullptr, nullptr, scores);
+    return surrogate_score_list(ctx, from, to, n, mask, nullptr, nullptr, scores, false);
 }
 
 int surrogate_score_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
-                         const int64_t* list, const uint32_t* n_list, float* scores) {
+                         const int64_t* list, const uint32_t* n_list, float* scores, bool top) {
     Surrogate& S = surrogate(ctx);
     if (!S.init) return fail(ctx, MPTG_E_STATE, "surrogate order needs an initialised surrogate");
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks)
         return fail(ctx, MPTG_E_STATE, "surrogate order is defined for the arm scene");
     const float inv_total = (float)(1.0 / ctx->scene.total_length);
-    const unsigned blocks = (unsigned)(n * (kScoreStride / 64));
+    // 64-waypoint tiles per edge (the top tree is one)
+    const int tiles = top ? 1 : kScoreStride / 64;
+    const unsigned blocks = (unsigned)(n * tiles);
     const uint16_t* wt = (const uint16_t*)S.wtb.p;
     const uint16_t* wb = (const uint16_t*)S.wb.p;
     const float* p = (const float*)S.params.p;
     const double* L = (const double*)ctx->scene.lengths;
     switch (ctx->dim) {
-        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
-        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
-        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores); break;
+        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, top ? 1 : 0); break;
+        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, top ? 1 : 0); break;
+        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, top ? 1 : 0); break;
         default: return fail(ctx, MPTG_E_STATE, "surrogate order: arms of 5, 10 or 20 links");
     }
     MPTG_HIP(ctx, hipGetLastError());

@@ -76,7 +76,7 @@ _SIG = [
     ("orc_link", C.c_int32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32, _u32p]),
     ("orc_link_batch", None, [C.POINTER(OrcScene), _dp, _dp, C.c_int64, C.c_int32, _u8p, _u32p, C.c_int32]),
     ("orc_checks_in_score_order", C.c_uint32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32,
-                                               C.POINTER(C.c_float), C.c_int64, C.c_int32]),
+                                               C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32]),
     ("orc_prrt_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_int64, _dp,
                                  _i32p, C.c_int64, _i32p, C.c_int64, C.POINTER(OrcPrrtStats)]),
     ("orc_star_k", C.c_int32, [C.c_int32, C.c_double, C.c_int32, C.c_int64]),
@@ -211,14 +211,16 @@ class Oracle:
                               _p(v, C.c_uint8), _p(c, C.c_uint32), threads)
         return v.astype(bool), c
 
-    def checks_in_score_order(self, a, b, scores, known_valid=False, lead=0) -> int:
+    def checks_in_score_order(self, a, b, scores, known_valid=False, lead=0, top_levels=None) -> int:
         """valid() calls of link(a, b) with reference ordinals 0..lead-1 first,
-        then the rest by descending score."""
+        then the other waypoints of bisection depth < top_levels by descending
+        score, then the deeper ones in reference order (top_levels None: all
+        the rest by score)."""
         a, b = _f64(a), _f64(b)
         sc = np.ascontiguousarray(scores, dtype=np.float32)
         return int(self.L.orc_checks_in_score_order(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double),
                                                     1 if known_valid else 0, _p(sc, C.c_float), sc.shape[0],
-                                                    lead))
+                                                    lead, -1 if top_levels is None else top_levels))
 
     def _params(self, seed, batch, range_=None, goal_bias=None):
         sc = self.scene

@@ -94,10 +94,13 @@ void orc_link_batch(const orc_scene* sc, const double* from, const double* to, i
                     int32_t nthreads);
 /* Number of valid() calls when the link's waypoints are checked in this
  * order, after the same endpoint checks: reference ordinals 0 .. lead-1 first,
- * then the others by descending score (ties: ascending reference ordinal).
- * scores[] is indexed by reference ordinal. */
+ * then the other waypoints of bisection depth < top_levels by descending
+ * score (ties: ascending reference ordinal), then the deeper ones in reference
+ * order (top_levels < 0: every waypoint after the lead by score).  scores[] is
+ * indexed by reference ordinal (pre-order of the bisection tree). */
 uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const double* b,
-                                   int32_t known_valid, const float* scores, int64_t nscores, int32_t lead);
+                                   int32_t known_valid, const float* scores, int64_t nscores, int32_t lead,
+                                   int32_t top_levels);
 
 /* --- batch-synchronous PRRT ------------------------------------------------ */
 typedef struct {

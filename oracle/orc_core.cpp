@@ -399,7 +399,8 @@ void orc_link_batch(const orc_scene* sc, const double* from, const double* to, i
 }
 
 uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const double* b,
-                                   int32_t known_valid, const float* scores, int64_t nscores, int32_t lead) {
+                                   int32_t known_valid, const float* scores, int64_t nscores, int32_t lead,
+                                   int32_t top_levels) {
     int d = sc->dim;
     uint32_t checks = 0;
     int nend = sc->link_mode == ORC_LINK_BISECT ? 2 : 1;
@@ -413,13 +414,37 @@ uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const d
     if (n != nscores) return 0xffffffffu;
     wp.resize((size_t)n * d);
     orc_link_waypoints(sc, a, b, wp.data(), n);
-    std::vector<int64_t> order((size_t)n);
-    for (int64_t i = 0; i < n; ++i) order[i] = i;
-    // the first `lead` ordinals keep the reference order; the rest follow by
-    // descending score (ties: ascending ordinal)
+    // bisection depth of each pre-order ordinal: a subtree of 2^k - 1 nodes
+    // is its root, then the left and the right subtree of 2^(k-1) - 1 each
+    std::vector<int> depth((size_t)n, 0);
+    {
+        std::vector<std::pair<int64_t, int64_t>> stack{{0, n}};  // (first ordinal, size)
+        std::vector<int> sd{0};
+        while (!stack.empty()) {
+            auto [o, sz] = stack.back();
+            int d = sd.back();
+            stack.pop_back();
+            sd.pop_back();
+            if (sz <= 0) continue;
+            depth[(size_t)o] = d;
+            const int64_t half = (sz - 1) / 2;
+            stack.push_back({o + 1 + half, half});
+            sd.push_back(d + 1);
+            stack.push_back({o + 1, half});
+            sd.push_back(d + 1);
+        }
+    }
+    // the first `lead` ordinals keep the reference order; the other top
+    // waypoints follow by descending score (ties: ascending ordinal); the
+    // deeper ones keep the reference order
     const int64_t l = std::min<int64_t>(std::max<int32_t>(lead, 0), n);
-    std::stable_sort(order.begin() + l, order.end(),
+    std::vector<int64_t> order, deep;
+    for (int64_t i = 0; i < l; ++i) order.push_back(i);
+    const size_t top0 = order.size();
+    for (int64_t i = l; i < n; ++i) (top_levels < 0 || depth[(size_t)i] < top_levels ? order : deep).push_back(i);
+    std::stable_sort(order.begin() + (ptrdiff_t)top0, order.end(),
                      [&](int64_t x, int64_t y) { return scores[x] > scores[y]; });
+    order.insert(order.end(), deep.begin(), deep.end());
     for (int64_t o : order) {
         ++checks;
         if (!valid(sc, wp.data() + o * d)) return checks;

@@ -193,10 +193,12 @@ def test_surrogate_order_parity_and_savings(trained):
     sc_np = scores.numpy()
     o = oracle.Oracle(sc)
     # the surrogate order: reference ordinals 0..3 (the packed round), then
-    # the rest by descending score
+    # the other waypoints of the top 6 bisection levels by descending score,
+    # then the deeper ones in reference order
     for e in range(0, m, 4):
         W = len(o.waypoints(a[e], b[e]))
-        assert c_sur[e] == o.checks_in_score_order(a[e], b[e], sc_np[e, :W], known_valid=True, lead=4), e
+        assert c_sur[e] == o.checks_in_score_order(a[e], b[e], sc_np[e, :W], known_valid=True, lead=4,
+                                                   top_levels=6), e
     # fused scores == generic forward on the same waypoints, up to the BF16
     # rounding of features built from FP32-interpolated waypoints
     zs, ss = [], []

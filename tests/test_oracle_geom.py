@@ -35,6 +35,23 @@ def _np_knn(pts, q, k, metric, radius=math.inf):
     return np.array(out_i), np.array(out_d)
 
 
+
+def _preorder_depths(n):
+    """Depth of every node of a perfect binary tree of n nodes, in pre-order
+    (built by walking the tree, independently of the oracle's index rule)."""
+    out = []
+
+    def walk(d, size):
+        if size == 0:
+            return
+        out.append(d)
+        walk(d + 1, (size - 1) // 2)
+        walk(d + 1, (size - 1) // 2)
+
+    walk(0, n)
+    return out
+
+
 @pytest.mark.parametrize("metric,dim,k", [(scenes.METRIC_L2, 2, 1), (scenes.METRIC_L2, 7, 20),
                                           (scenes.METRIC_L1, 20, 5), (scenes.METRIC_L2, 3, 32)])
 def test_knn_matches_numpy_bruteforce(metric, dim, k):
@@ -292,3 +309,15 @@ def test_score_order_counts_checks():
                 bad = [pos for pos, i in enumerate(head + rest) if not ok[i]]
                 want = 2 + (bad[0] + 1 if bad else len(wp))
                 assert o.checks_in_score_order(a, b, s, known_valid=True, lead=lead) == want, lead
+                # only the waypoints of the top levels by score, the deeper
+                # ones after them in reference order
+                depth = _preorder_depths(len(wp))
+                for top in (0, 1, 3, 6, 40):
+                    mid = sorted([i for i in range(len(head), len(wp)) if depth[i] < top],
+                                 key=lambda i: (-float(s[i]), i))
+                    order = head + mid + [i for i in range(len(head), len(wp)) if depth[i] >= top]
+                    assert sorted(order) == list(range(len(wp)))
+                    bad = [pos for pos, i in enumerate(order) if not ok[i]]
+                    want = 2 + (bad[0] + 1 if bad else len(wp))
+                    got = o.checks_in_score_order(a, b, s, known_valid=True, lead=lead, top_levels=top)
+                    assert got == want, (lead, top)
