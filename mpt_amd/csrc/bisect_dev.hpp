@@ -16,7 +16,7 @@ constexpr int kMaxArmDepth = 30;
 constexpr int kScoreStride = 512;
 
 // Pre-order ordinal -> (path, depth) in a perfect tree with h internal levels.
-__device__ __forceinline__ void preorder_path(int h, uint32_t o, uint32_t* path, int* depth) {
+__host__ __device__ __forceinline__ void preorder_path(int h, uint32_t o, uint32_t* path, int* depth) {
     uint32_t p = 0;
     int d = 0;
     uint32_t sub = (1u << h) - 1u;
@@ -37,7 +37,7 @@ __device__ __forceinline__ void preorder_path(int h, uint32_t o, uint32_t* path,
 // (path, depth) -> pre-order ordinal in a perfect tree with h levels (the
 // inverse of preorder_path: each step down adds one, a right step also
 // skips the left subtree of 2^(h-1-k) - 1 nodes).
-__device__ __forceinline__ uint32_t path_preorder(int h, uint32_t path, int depth) {
+__host__ __device__ __forceinline__ uint32_t path_preorder(int h, uint32_t path, int depth) {
     uint32_t o = 0;
     for (int k = 0; k < depth; ++k) o += 1u + (((path >> k) & 1u) ? (1u << (h - 1 - k)) - 1u : 0u);
     return o;
@@ -49,7 +49,7 @@ constexpr int kTopLevels = 6;
 
 // Node i of the top tree in breadth-first order (root 0, then each level
 // left to right) -> (path, depth); path bit k is step k (1 = right).
-__device__ __forceinline__ void bfs_path(uint32_t i, uint32_t* path, int* depth) {
+__host__ __device__ __forceinline__ void bfs_path(uint32_t i, uint32_t* path, int* depth) {
     const int d = 31 - __builtin_clz(i + 1u);
     const uint32_t idx = i + 1u - (1u << d);  // first step in the top bit
     uint32_t p = 0;
@@ -62,7 +62,7 @@ __device__ __forceinline__ void bfs_path(uint32_t i, uint32_t* path, int* depth)
 // levels): pre-order visits the 2^kTopLevels subtrees rooted at depth
 // kTopLevels left to right, each one contiguously, so node k is node
 // k mod S of subtree k / S (S = 2^(h - kTopLevels) - 1 nodes each).
-__device__ __forceinline__ void deep_path(int h, uint32_t k, uint32_t* path, int* depth) {
+__host__ __device__ __forceinline__ void deep_path(int h, uint32_t k, uint32_t* path, int* depth) {
     const uint32_t S = (1u << (h - kTopLevels)) - 1u;
     const uint32_t j = k / S;
     uint32_t lp;
