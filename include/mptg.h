@@ -312,8 +312,9 @@ int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep
 
 /* one arm-scene link() call (as mptg_link_dev) timed phase by phase with HIP
  * events on the context stream: h_ms[0] packed head round, [1] surrogate
- * scoring and [2] ranking of the edges it left (surrogate order), [3]
- * continuation, [4] irregular-edge walk (ms) */
+ * scoring and ranking of the edges it left (surrogate order; one kernel),
+ * [2] the gap before the continuation, [3] continuation, [4] irregular-edge
+ * walk (ms) */
 int mptg_link_phases(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                      int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks, float* h_ms);
 /* plain device-to-device copy kernel of `bytes` (multiple of 16, 16-byte

@@ -38,109 +38,15 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Sort key of a waypoint in surrogate order: ascending key = descending
-// score, ties by ascending reference ordinal (-0 counts as +0, NaN last).
-__device__ __forceinline__ uint64_t score_key(float s, uint32_t ordinal) {
-    uint32_t hi;
-    if (s != s) {
-        hi = 0xffffffffu;
-    } else {
-        if (s == 0.0f) s = 0.0f;
-        uint32_t u = bits_of(s);
-        uint32_t mono = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // ascending in s
-        hi = ~mono;
-    }
-    return ((uint64_t)hi << 32) | ordinal;
-}
-
-// Ascending bitonic sort of the wave's 64 * NE keys; element i = e * 64 +
-// lane lives in key[e] of that lane, so after the sort rank r = c * 64 + lane
-// is key[c] of that lane.  Partners within a lane are registers, partners in
-// other lanes come by __shfl_xor.
-template <int NE>
-__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[NE], int lane) {
-    constexpr int N = 64 * NE;
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            uint64_t nk[NE];
-#pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                const int i = e * 64 + lane;
-                uint64_t other;
-                if (j >= 64) {
-                    other = key[e ^ (j >> 6)];
-                } else {
-                    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)key[e], j);
-                    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(key[e] >> 32), j);
-                    other = ((uint64_t)hi << 32) | lo;
-                }
-                const bool lower = (i & j) == 0, asc = (i & k) == 0;
-                const uint64_t mn = key[e] < other ? key[e] : other;
-                const uint64_t mx = key[e] < other ? other : key[e];
-                nk[e] = (lower == asc) ? mn : mx;
-            }
-#pragma unroll
-            for (int e = 0; e < NE; ++e) key[e] = nk[e];
-        }
-    }
-}
-
 // The surrogate order of an edge's waypoints (trees of h <= 9 levels):
-// reference ordinals 0 .. lead-1 first, then the other waypoints of the top
-// kTopLevels levels (63 spread evenly along the edge) by descending score
+// reference ordinals 0 .. kHeadRanks-1 first, then the other waypoints of the
+// top kTopLevels levels (63 spread evenly along the edge) by descending score
 // (ties by reference ordinal), then the deeper waypoints in reference order.
 // The collision region of an edge is usually wide, so the top levels are
 // where a score finds it; ordering the deeper waypoints buys no device time
 // (a colliding edge costs one continuation round whichever rank fails) and
-// would cost up to eight scored tiles per edge instead of one.
-//
-// Ranked ordinals of one edge per wave (edge list[item] for item <
-// *n_list): lane i holds top-tree node i (breadth-first); writes the
-// reference ordinal of ranks 0 .. 2^min(h, kTopLevels) - 2 to
-// ranked[e * kScoreStride + rank].
-__global__ __launch_bounds__(64) void rank_scores(const double* __restrict__ from, const double* __restrict__ to,
-                                                  const int64_t* __restrict__ list,
-                                                  const uint32_t* __restrict__ n_list, int dim, double tol,
-                                                  uint32_t lead, const float* __restrict__ scores,
-                                                  uint16_t* __restrict__ ranked) {
-    const int64_t item = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (item >= (int64_t)*n_list) return;
-    const int64_t e = list[item];
-    int h = 0;
-    bool regular = false;
-    // shape from the widest coordinate (same rule as the link kernels)
-    double wmax = 0.0, M = 0.0;
-    for (int j = 0; j < dim; ++j) {
-        double x = from[e * dim + j], y = to[e * dim + j];
-        wmax = __builtin_fmax(wmax, __builtin_fabs(x - y));
-        M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(x), __builtin_fabs(y)));
-    }
-    double wd = wmax;
-    while (!(wd < tol) && h <= kMaxArmDepth) {
-        wd *= 0.5;
-        ++h;
-    }
-    (void)M;
-    (void)regular;
-    static_assert((1 << kTopLevels) - 1 < kWave, "one top-tree node per lane");
-    if (h > 9) return;  // more waypoints than score slots: reference order
-    const uint32_t T = (1u << (h < kTopLevels ? h : kTopLevels)) - 1u;
-    uint64_t key[1] = {~0ull};
-    if ((uint32_t)lane < T) {
-        uint32_t path;
-        int depth;
-        bfs_path((uint32_t)lane, &path, &depth);
-        const uint32_t o = path_preorder(h, path, depth);
-        // led ordinals sort first: every score key has a nonzero high word
-        key[0] = o < lead ? (uint64_t)o : score_key(scores[e * kScoreStride + o], o);
-    }
-    wave_bitonic_sort<1>(key, lane);
-    if ((uint32_t)lane < T) ranked[e * kScoreStride + lane] = (uint16_t)key[0];
-}
-
+// would cost up to eight scored tiles per edge instead of one.  The scoring
+// kernel ranks each edge's top tree itself (surrogate_rank_list).
 // (path, depth) of the waypoint at `rank` in the chosen order (tree of h
 // levels): surrogate ranks below the top tree's size come from the ranked
 // list, later ones are the deeper waypoints in reference order.
@@ -150,7 +56,7 @@ __device__ __forceinline__ void node_at(int h, const uint16_t* __restrict__ rank
     if constexpr (ORD == MPTG_ORDER_SURROGATE) {
         if (h <= 9) {
             const uint32_t T = (1u << (h < kTopLevels ? h : kTopLevels)) - 1u;
-            if (rank < T) preorder_path(h, ranked[e * kScoreStride + rank], path, depth);
+            if (rank < T) preorder_path(h, ranked[e * kRankStride + rank], path, depth);
             else deep_path(h, rank - T, path, depth);
             return;
         }
@@ -341,15 +247,11 @@ static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to
     const uint16_t* ranked = nullptr;
     if constexpr (ORD == MPTG_ORDER_SURROGATE) {
         int rc;
-        if ((rc = ensure(ctx, ctx->scratch[8], sizeof(float) * kScoreStride * n))) return rc;
-        if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kScoreStride * n))) return rc;
-        float* scores = (float*)ctx->scratch[8].p;
-        if ((rc = surrogate_score_list(ctx, from, to, n, nullptr, cont, ncont, scores, true))) return rc;
+        if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kRankStride * n))) return rc;
+        uint16_t* rk = (uint16_t*)ctx->scratch[9].p;
+        if ((rc = surrogate_rank_list(ctx, from, to, n, cont, ncont, (uint32_t)kHeadRanks, rk))) return rc;
         phase_mark(ctx, 2);
-        hipLaunchKernelGGL(rank_scores, dim3((unsigned)n), dim3(kWave), 0, ctx->stream, from, to,
-                           (const int64_t*)cont, (const uint32_t*)ncont, D, tol, (uint32_t)kHeadRanks,
-                           (const float*)scores, (uint16_t*)ctx->scratch[9].p);
-        ranked = (const uint16_t*)ctx->scratch[9].p;
+        ranked = rk;
     } else {
         phase_mark(ctx, 2);
     }

@@ -46,6 +46,8 @@ __host__ __device__ __forceinline__ uint32_t path_preorder(int h, uint32_t path,
 // Levels of the bisection tree the surrogate order ranks by score: the
 // 63 waypoints of depths 0..5 spread evenly along the edge.
 constexpr int kTopLevels = 6;
+// Ranked ordinals kept per edge (the top tree, one per lane of a wave).
+constexpr int kRankStride = 64;
 
 // Node i of the top tree in breadth-first order (root 0, then each level
 // left to right) -> (path, depth); path bit k is step k (1 = right).
@@ -110,5 +112,55 @@ __device__ __forceinline__ bool bisect_shape(const double* a, const double* b, d
     *h_out = h;
     return true;
 }
+
+// Sort key of a waypoint in surrogate order: ascending key = descending
+// score, ties by ascending reference ordinal (-0 counts as +0, NaN last).
+__device__ __forceinline__ uint64_t score_key(float s, uint32_t ordinal) {
+    uint32_t hi;
+    if (s != s) {
+        hi = 0xffffffffu;
+    } else {
+        if (s == 0.0f) s = 0.0f;
+        uint32_t u = bits_of(s);
+        uint32_t mono = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // ascending in s
+        hi = ~mono;
+    }
+    return ((uint64_t)hi << 32) | ordinal;
+}
+
+// Ascending bitonic sort of the wave's 64 * NE keys; element i = e * 64 +
+// lane lives in key[e] of that lane, so after the sort rank r = c * 64 + lane
+// is key[c] of that lane.  Partners within a lane are registers, partners in
+// other lanes come by __shfl_xor.
+template <int NE>
+__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[NE], int lane) {
+    constexpr int N = 64 * NE;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint64_t nk[NE];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const int i = e * 64 + lane;
+                uint64_t other;
+                if (j >= 64) {
+                    other = key[e ^ (j >> 6)];
+                } else {
+                    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)key[e], j);
+                    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(key[e] >> 32), j);
+                    other = ((uint64_t)hi << 32) | lo;
+                }
+                const bool lower = (i & j) == 0, asc = (i & k) == 0;
+                const uint64_t mn = key[e] < other ? key[e] : other;
+                const uint64_t mx = key[e] < other ? other : key[e];
+                nk[e] = (lower == asc) ? mn : mx;
+            }
+#pragma unroll
+            for (int e = 0; e < NE; ++e) key[e] = nk[e];
+        }
+    }
+}
+
 
 }  // namespace mptg

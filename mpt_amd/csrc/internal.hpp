@@ -213,11 +213,11 @@ int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t
 // scores[e * 512 + reference ordinal]
 int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
                           float* scores);
-// Scores of the edges list[0 .. *n_list) (device count; at most n of them),
-// written at scores[e * 512 + reference ordinal] as surrogate_score_edges
-// does: every waypoint, or (top) those of the bisection tree's top
-// kTopLevels levels only.
-int surrogate_score_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
-                         const int64_t* list, const uint32_t* n_list, float* scores, bool top);
+// Surrogate order of the edges list[0 .. *n_list) (device count; at most n
+// of them): scores the waypoints of each edge's top bisection levels and
+// ranks them in the same kernel, writing the reference ordinal of ranks
+// 0 .. 62 to ranked[e * 64 + rank] (ordinals below `lead` first).
+int surrogate_rank_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const int64_t* list,
+                        const uint32_t* n_list, uint32_t lead, uint16_t* ranked);
 
 }  // namespace mptg

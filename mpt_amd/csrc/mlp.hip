@@ -414,7 +414,12 @@ __device__ __forceinline__ void arm_features_f(const float* q, const double* len
 }
 
 // Edges: e = item (all n, or those with mask[e] set), or e = list[item] for
-// item < *n_list when a list is given.
+// item < *n_list when a list is given.  Without `ranked`: the scores of every
+// waypoint (tile = 64 reference ordinals from c0) at scores[e * 512 +
+// ordinal].  With `ranked` (one tile per edge): row i is node i of the top
+// tree (breadth-first), and the wave sorts its rows' keys -- reference
+// ordinals below `lead` first, then descending score, ties by ordinal -- and
+// writes the ordinal of each rank to ranked[e * kRankStride + rank].
 template <int D>
 __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__ from, const double* __restrict__ to,
                                                       int64_t n, const uint8_t* __restrict__ mask,
@@ -423,7 +428,8 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
                                                       const double* __restrict__ lengths, float inv_total,
                                                       const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
                                                       const float* __restrict__ p, float* __restrict__ scores,
-                                                      int tiles, int top) {
+                                                      int tiles, uint16_t* __restrict__ ranked, uint32_t lead) {
+    const bool top = ranked != nullptr;
     __shared__ __attribute__((aligned(16))) uint16_t buf[64 * kLdsRow];
     const int lane = threadIdx.x;
     const int64_t item = blockIdx.x / tiles;
@@ -484,22 +490,31 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
     wave_layer_t<HID>(buf, wt + IN * HID, p + OFF_B2, lane);
     float logit[4];
     wave_layer_logit(buf, wt + IN * HID + HID * HID, p + OFF_B3, wb + IN * HID + 2 * HID * HID, lane, logit);
+    const float b4 = p[OFF_B4];
+    if (top) {
+        // every lane holds the logits of waypoints st * 16 + (lane & 15):
+        // lane i takes its own row, i = (i >> 4) * 16 + (i & 15)
+        const int q = lane >> 4;
+        const float s = (q == 0 ? logit[0] : q == 1 ? logit[1] : q == 2 ? logit[2] : logit[3]) + b4;
+        uint64_t key[1] = {~0ull};
+        if (o < W) {
+            uint32_t path;
+            int depth;
+            bfs_path(o, &path, &depth);
+            const uint32_t ord = path_preorder(h, path, depth);
+            // led ordinals sort first: every score key has a nonzero high word
+            key[0] = ord < lead ? (uint64_t)ord : score_key(s, ord);
+        }
+        wave_bitonic_sort<1>(key, lane);
+        if (o < W) ranked[e * kRankStride + lane] = (uint16_t)key[0];
+        return;
+    }
     // lanes 0..15 write waypoints st * 16 + lane
     if (lane < 16) {
-        const float b4 = p[OFF_B4];
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
             const uint32_t oo = c0 + (uint32_t)(st * 16 + lane);
-            if (oo < W) {
-                uint32_t ord = oo;
-                if (top) {
-                    uint32_t path;
-                    int depth;
-                    bfs_path(oo, &path, &depth);
-                    ord = path_preorder(h, path, depth);
-                }
-                scores[e * kScoreStride + ord] = logit[st] + b4;
-            }
+            if (oo < W) scores[e * kScoreStride + oo] = logit[st] + b4;
         }
     }
 }
@@ -659,33 +674,40 @@ static int sur_check(mptg_ctx* ctx, bool need_init) {
     return MPTG_OK;
 }
 
-int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
-                          float* scores) {
-    return surrogate_score_list(ctx, from, to, n, mask, nullptr, nullptr, scores, false);
-}
-
-int surrogate_score_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
-                         const int64_t* list, const uint32_t* n_list, float* scores, bool top) {
+// One launch of the fused scoring kernel (ranked: the top-tree ranking mode).
+static int score_launch(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                        const int64_t* list, const uint32_t* n_list, float* scores, uint16_t* ranked,
+                        uint32_t lead) {
     Surrogate& S = surrogate(ctx);
     if (!S.init) return fail(ctx, MPTG_E_STATE, "surrogate order needs an initialised surrogate");
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks)
         return fail(ctx, MPTG_E_STATE, "surrogate order is defined for the arm scene");
     const float inv_total = (float)(1.0 / ctx->scene.total_length);
     // 64-waypoint tiles per edge (the top tree is one)
-    const int tiles = top ? 1 : kScoreStride / 64;
+    const int tiles = ranked ? 1 : kScoreStride / 64;
     const unsigned blocks = (unsigned)(n * tiles);
     const uint16_t* wt = (const uint16_t*)S.wtb.p;
     const uint16_t* wb = (const uint16_t*)S.wb.p;
     const float* p = (const float*)S.params.p;
     const double* L = (const double*)ctx->scene.lengths;
     switch (ctx->dim) {
-        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, top ? 1 : 0); break;
-        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, top ? 1 : 0); break;
-        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, top ? 1 : 0); break;
+        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, ranked, lead); break;
+        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, ranked, lead); break;
+        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, ranked, lead); break;
         default: return fail(ctx, MPTG_E_STATE, "surrogate order: arms of 5, 10 or 20 links");
     }
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
+}
+
+int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                          float* scores) {
+    return score_launch(ctx, from, to, n, mask, nullptr, nullptr, scores, nullptr, 0);
+}
+
+int surrogate_rank_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const int64_t* list,
+                        const uint32_t* n_list, uint32_t lead, uint16_t* ranked) {
+    return score_launch(ctx, from, to, n, nullptr, list, n_list, nullptr, ranked, lead);
 }
 
 }  // namespace mptg
