@@ -293,7 +293,10 @@ int mptg_tune_get(mptg_ctx* ctx, mptg_tune_params* out);
 int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p);
 /* Times kNN of the nq device queries d_q against the current tree for each
    candidate split and keeps the fastest (replaces the nigh KD-tree's fixed
-   traversal, src/mpt/impl/prrt/prrt.hpp:406-409, which has nothing to tune). */
+   traversal, src/mpt/impl/prrt/prrt.hpp:406-409, which has nothing to tune).
+   The split shapes the scans; a tree the uniform grid serves (knn_grid on)
+   sends every candidate through the grid, so switch knn_grid off first to
+   tune the scans themselves. */
 int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_tune_report* rep);
 /* Times link() of n device edges (reference order) for each candidate of the
    scene's link launch shape -- the grid of the persistent DMV long-edge

@@ -91,7 +91,7 @@ def test_knn_select_path_equals_klist_scan_and_oracle(ctx_factory, metric, dim, 
 
 
 @pytest.mark.parametrize("metric,dim", [(scenes.METRIC_L2, 2), (scenes.METRIC_L2, 5), (scenes.METRIC_L1, 3),
-                                        (scenes.METRIC_L2, 8)])
+                                        (scenes.METRIC_L2, 8), (scenes.METRIC_L1, 8)])
 def test_knn_grid_equals_scans_and_oracle(ctx_factory, metric, dim):
     """The uniform-grid search (trees of 2^16 nodes or more, 2..8 dimensions)
     equals the scans on every query and the oracle on a sample: uniform and
