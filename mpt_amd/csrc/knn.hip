@@ -143,19 +143,29 @@ __device__ __forceinline__ bool knn_extent(const int64_t* n_dev, const int32_t* 
 }
 
 // Nearest of the tree's first kSeedNodes nodes to each query (exact FP64
-// distance): a starting bound for the k = 1 scan.
+// distance): a starting bound for the k = 1 scan.  kSeedLanes lanes share a
+// query, each measuring every kSeedLanes-th node; the minimum is exact in
+// any order.
+constexpr int kSeedLanes = 8;
 template <int D, int METRIC>
 __global__ __launch_bounds__(kBlock) void knn_seed(const double* __restrict__ x64, int64_t n,
                                                    const double* __restrict__ q, int64_t nq,
                                                    double* __restrict__ seed, const int64_t* n_dev,
                                                    const int32_t* skip) {
-    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (qi >= nq || (skip && *skip)) return;
+    static_assert(64 % kSeedLanes == 0, "lane groups within a wave");
+    if (skip && *skip) return;  // uniform: every thread reads the same flag
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t qi = t / kSeedLanes;
+    const int part = (int)(t % kSeedLanes);
     if (n_dev) n = *n_dev;
     const int64_t ns = n < kSeedNodes ? n : kSeedNodes;
     double best = __builtin_inf();
-    for (int64_t j = 0; j < ns; ++j) best = __builtin_fmin(best, exact_dist<D, METRIC>(q + qi * D, x64 + j * D));
-    seed[qi] = best;
+    if (qi < nq)
+        for (int64_t j = part; j < ns; j += kSeedLanes)
+            best = __builtin_fmin(best, exact_dist<D, METRIC>(q + qi * D, x64 + j * D));
+#pragma unroll
+    for (int o = 1; o < kSeedLanes; o <<= 1) best = __builtin_fmin(best, __shfl_xor(best, o));
+    if (qi < nq && part == 0) seed[qi] = best;
 }
 
 template <int D, int METRIC, int K>
@@ -533,8 +543,8 @@ template <int D, int METRIC, int K>
 void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q, int64_t nq, double radius,
                  double* pd, int32_t* pi) {
     if (K == 1 && x.seed)
-        hipLaunchKernelGGL((knn_seed<D, METRIC>), dim3(grid.x), dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq,
-                           x.seed, x.n_dev, x.skip);
+        hipLaunchKernelGGL((knn_seed<D, METRIC>), dim3((unsigned)((nq * kSeedLanes + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, x.seed, x.n_dev, x.skip);
     hipLaunchKernelGGL((knn_scan<D, METRIC, K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n, x.chunk,
                        q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip, (const double*)(K == 1 ? x.seed : nullptr));
 }

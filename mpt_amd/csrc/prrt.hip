@@ -271,10 +271,9 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
     const int B = s.p.batch, D = ctx->dim;
     Counters* dcnt = (Counters*)pb.counters.p;
     const bool dev = it < 0;
-    const unsigned gb = (unsigned)((B + kBlock - 1) / kBlock);
     int rc;
-    hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, ctx->metric, D, s.p.seed,
-                       s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)pb.q.p);
+    hipLaunchKernelGGL(prrt_sample, dim3(sample_blocks(ctx->metric, D, B)), dim3(kBlock), 0, ctx->stream, pd,
+                       ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)pb.q.p);
     MPTG_HIP(ctx, hipGetLastError());
     if ((rc = launch_knn(ctx, (const double*)pb.q.p, B, 1, __builtin_inf(), (int32_t*)pb.near.p,
                          (double*)pb.ndist.p, dev ? plan_n : ctx->size, dev ? &dcnt->size : nullptr,

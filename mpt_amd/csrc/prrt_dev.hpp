@@ -28,23 +28,40 @@ struct PrrtDev {
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
 };
 
+// Threads per sample of prrt_sample: one per coordinate in the boxes (each
+// coordinate is its own counter of the stream), one per state for SO(3) x R^3.
+__host__ __device__ inline int sample_threads(int metric, int D) { return metric == kSampleMetricSE3 ? 1 : D; }
+inline unsigned sample_blocks(int metric, int D, int B) {
+    return (unsigned)(((int64_t)B * sample_threads(metric, D) + kBlock - 1) / kBlock);
+}
+
 // it < 0: the iteration index is the device counter's (graph replays).
+// Thread t draws coordinate t % D of sample t / D (boxes; coalesced
+// stores), or the whole state for SO(3) x R^3.
 __global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int metric, int D, uint64_t seed, double goal_bias,
                                                       int64_t it, int B, const Counters* __restrict__ cnt,
                                                       double* __restrict__ q) {
-    int b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= B || cnt->done) return;
+    const int per = sample_threads(metric, D);
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)B * per || cnt->done) return;
+    const int64_t b = t / per;
     if (it < 0) it = cnt->iteration;
-    uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
-    bool bias = cnt->goals == 0 && goal_bias > 0;
-    bool useGoal = bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias;
-    double v[kMaxDim];
-    if (useGoal) {
-        for (int i = 0; i < D; ++i) v[i] = pd.goal[i];
-    } else {
-        sample_state(seed, sidx, metric, D, pd.lo, pd.hi, v);
+    const uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
+    const bool bias = cnt->goals == 0 && goal_bias > 0;
+    const bool useGoal = bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias;
+    if (per == D) {
+        const int i = (int)(t - b * per);
+        q[t] = useGoal ? pd.goal[i]
+                       : uniform_in(uniform_at(seed, sidx, RNG_PURPOSE_SAMPLE, (uint32_t)i), pd.lo[i], pd.hi[i]);
+        return;
     }
-    for (int i = 0; i < D; ++i) q[(int64_t)b * D + i] = v[i];
+    double v[7];
+    if (useGoal) {
+        for (int i = 0; i < 7; ++i) v[i] = pd.goal[i];
+    } else {
+        sample_state(seed, sidx, metric, 7, pd.lo, pd.hi, v);
+    }
+    for (int i = 0; i < 7; ++i) q[b * 7 + i] = v[i];
 }
 
 template <int D>

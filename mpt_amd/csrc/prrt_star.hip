@@ -572,7 +572,6 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         pd.hi[i] = s.hi[i];
         pd.goal[i] = s.goal[i];
     }
-    const unsigned gb = (unsigned)((B + kBlock - 1) / kBlock);
     const int dp = pitch32(D);
     for (int64_t iter = 0; iter < max_iters && ctx->size < target_size; ++iter) {
         const int64_t n0 = ctx->size;
@@ -585,8 +584,8 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         if ((rc = ensure(ctx, s.front, sizeof(int32_t) * 2 * (size_t)(n0 + B)))) return rc;
         const int64_t it = s.iteration;
         // ---- extend
-        hipLaunchKernelGGL(prrt_sample, dim3(gb), dim3(kBlock), 0, ctx->stream, pd, ctx->metric, D, s.p.seed, s.p.goal_bias, it,
-                           B, (const Counters*)dcnt, (double*)s.q.p);
+        hipLaunchKernelGGL(prrt_sample, dim3(sample_blocks(ctx->metric, D, B)), dim3(kBlock), 0, ctx->stream, pd,
+                           ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)s.q.p);
         MPTG_HIP(ctx, hipGetLastError());
         if ((rc = launch_knn(ctx, (const double*)s.q.p, B, 1, __builtin_inf(), (int32_t*)s.near.p,
                              (double*)s.ndist.p, n0)))

@@ -609,8 +609,10 @@ __device__ __forceinline__ bool state_valid_t(const SceneArgs& sc, const double*
 
 template <int D>
 __device__ __forceinline__ double space_dist_t(const SpaceArgs& sp, const double* a, const double* b) {
+    // SE(3) states have 7 coordinates (mptg_space enforces it): any other
+    // compile-time size is L1 or L2, and the arrays never leave registers
     if constexpr (D > 0) {
-        if (sp.metric != MPTG_METRIC_SE3) {
+        if (D != 7 || sp.metric != MPTG_METRIC_SE3) {
             double s = 0.0;
             if (sp.metric == MPTG_METRIC_L1) {
 #pragma unroll
@@ -632,13 +634,13 @@ template <int D>
 __device__ __forceinline__ void space_interp_t(const SpaceArgs& sp, const double* a, const double* b, double t,
                                                double* q) {
     if constexpr (D > 0) {
-        if (sp.metric != MPTG_METRIC_SE3) {
+        if (D != 7 || sp.metric != MPTG_METRIC_SE3) {
 #pragma unroll
             for (int i = 0; i < D; ++i) q[i] = (b[i] - a[i]) * t + a[i];
             return;
         }
     }
-    space_interp(sp, a, b, t, q);
+    if constexpr (D == 0 || D == 7) space_interp(sp, a, b, t, q);
 }
 
 // Runs f.template operator()<D>() with D the compile-time state size, or
