@@ -291,7 +291,8 @@ static int launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int
 // Launches the packed round and the continuation (scoring and ranking the
 // continued edges in between for the surrogate order); edges the kernels
 // list as irregular are left for the caller's exact per-edge kernel
-// (irr/nirr).  Returns MPTG_E_STATE when no kernel exists for this arm.
+// (irr/nirr; *nirr is zeroed here).  Returns MPTG_E_STATE when no kernel
+// exists for this arm.
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
                     const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
                     unsigned long long* work, int order) {
@@ -301,7 +302,7 @@ int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t
     if ((rc = ensure(ctx, ctx->scratch[10], sizeof(int64_t) * n + 256))) return rc;
     uint32_t* ncont = (uint32_t*)ctx->scratch[10].p;
     int64_t* cont = (int64_t*)((char*)ctx->scratch[10].p + 256);
-    if ((rc = launch_zero_u32(ctx, ncont, 1))) return rc;
+    if ((rc = launch_zero_pair(ctx, ncont, nirr))) return rc;
     phase_mark(ctx, 0);
     switch (ctx->dim) {
         case 5: rc = launch_arm_d<5>(ctx, from, to, n, known_valid, mask, order, verdict, checks, irr, nirr, cont, ncont, work); break;

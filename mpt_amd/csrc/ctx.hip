@@ -17,6 +17,10 @@ __global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t* __restrict__ p,
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
+__global__ __launch_bounds__(64) void zero_pair_kernel(uint32_t* __restrict__ a, uint32_t* __restrict__ b) {
+    if (threadIdx.x == 0) *a = 0u;
+    if (threadIdx.x == 1) *b = 0u;
+}
 __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;
@@ -26,6 +30,12 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p,
 int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n) {
     if (n <= 0) return MPTG_OK;
     hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+int launch_zero_pair(mptg_ctx* ctx, uint32_t* a, uint32_t* b) {
+    hipLaunchKernelGGL(zero_pair_kernel, dim3(1), dim3(64), 0, ctx->stream, a, b);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

@@ -188,6 +188,8 @@ int knn_grid_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double ra
                   int64_t n, int64_t** over, uint32_t* m);
 // Zeroes n 32-bit words with a kernel (graph-capturable, unlike a memset).
 int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n);
+// Zeroes the 32-bit words *a and *b with one small kernel.
+int launch_zero_pair(mptg_ctx* ctx, uint32_t* a, uint32_t* b);
 // Sets n 32-bit words to v with a kernel.
 int launch_fill_u32(mptg_ctx* ctx, uint32_t* p, int64_t n, uint32_t v);
 int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,

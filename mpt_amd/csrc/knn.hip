@@ -29,6 +29,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <limits>
 
 #include "internal.hpp"
 
@@ -331,6 +332,52 @@ __global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ p
             out_d[qi * k + j] = have ? top.d[j] : __builtin_inf();
             out_i[qi * k + j] = have ? top.i[j] : -1;
         }
+    }
+}
+
+// k = 1: the nearest over the splits.  kMergeLanes lanes share a query,
+// each reading every kMergeLanes-th split's partial result, then a shuffle
+// minimum by (distance, node id), which is exact in any order.
+constexpr int kMergeLanes = 8;
+__global__ __launch_bounds__(kBlock) void knn_merge1(const double* __restrict__ pd, const int32_t* __restrict__ pi,
+                                                     int splits, int64_t nq, double* __restrict__ out_d,
+                                                     int32_t* __restrict__ out_i, const int32_t* skip,
+                                                     const int64_t* n_dev, double inf) {
+    static_assert(64 % kMergeLanes == 0, "lane groups within a wave");
+    if (skip && *skip) return;  // uniform: every thread reads the same flag
+    if (n_dev) {
+        int64_t chunk;
+        splits = (int)knn_used_splits(*n_dev, splits, &chunk);
+    }
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t qi = t / kMergeLanes;
+    const int part = (int)(t % kMergeLanes);
+    double bd = inf;
+    int32_t bi = INT_MAX;
+    if (qi < nq) {
+#pragma unroll 4
+        for (int s = part; s < splits; s += kMergeLanes) {
+            const double d = pd[(int64_t)s * nq + qi];
+            const int32_t id = pi[(int64_t)s * nq + qi];
+            if (better(d, id, bd, bi)) {
+                bd = d;
+                bi = id;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < kMergeLanes; o <<= 1) {
+        const double od = __shfl_xor(bd, o);
+        const int32_t oi = __shfl_xor(bi, o);
+        if (better(od, oi, bd, bi)) {
+            bd = od;
+            bi = oi;
+        }
+    }
+    if (qi < nq && part == 0) {
+        const bool have = bi != INT_MAX;
+        out_d[qi] = have ? bd : inf;
+        out_i[qi] = have ? bi : -1;
     }
 }
 
@@ -645,7 +692,7 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     MPTG_HIP(ctx, hipGetLastError());
     dim3 g1((unsigned)((nq + kBlock - 1) / kBlock));
     if (splits > 1) {
-        if (K == 1) hipLaunchKernelGGL((knn_merge<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
+        if (K == 1) hipLaunchKernelGGL(knn_merge1, dim3((unsigned)((nq * kMergeLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, d_dist, d_idx, skip, n_dev, std::numeric_limits<double>::infinity());
         else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
         else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
     } else {

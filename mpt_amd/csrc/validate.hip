@@ -460,7 +460,6 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
             if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
             uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
             int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
-            if ((rc = launch_zero_u32(ctx, nirr, 1))) return rc;
             rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks, irr, nirr, nullptr,
                                  order);
             if (rc == MPTG_OK) {
@@ -514,7 +513,6 @@ int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, in
     uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
     int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
     MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 8 * sizeof(unsigned long long), ctx->stream));
-    MPTG_HIP(ctx, hipMemsetAsync(nirr, 0, sizeof(uint32_t), ctx->stream));
     rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks, irr, nirr, d_work, order);
     if (rc) return rc == MPTG_E_STATE ? fail(ctx, rc, "no wave kernel for this arm") : rc;
     hipLaunchKernelGGL(link_bisect_work, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
