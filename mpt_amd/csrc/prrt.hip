@@ -27,7 +27,7 @@ namespace {
 // order: the block's base is the sum over the blocks before it) and commit
 // (tree size, goals, the next iteration and the stop flag).  Every kernel is
 // a no-op once the stop flag is set.
-constexpr int kAppendBlock = 1024;
+constexpr int kAppendBlock = 256;
 
 __device__ __forceinline__ int block_rank(bool flag, int* wsum, int* total) {
     // rank of this thread's flag among the block's set flags (thread order);
@@ -138,9 +138,12 @@ __global__ __launch_bounds__(kAppendBlock) void prrt_write(int D, int dp, int B,
     const int t = threadIdx.x;
     if (cnt->done) return;  // block-uniform
     // this block's first node id and goal slot: the counts of the blocks
-    // before it (at most kAppendBlock of them) on top of the tree
-    int64_t base = t < (int)blockIdx.x ? blk[2 * t] : 0;
-    int64_t gbase = t < (int)blockIdx.x ? blk[2 * t + 1] : 0;
+    // before it on top of the tree
+    int64_t base = 0, gbase = 0;
+    for (int j = t; j < (int)blockIdx.x; j += kAppendBlock) {
+        base += blk[2 * j];
+        gbase += blk[2 * j + 1];
+    }
     block_sum2(&base, &gbase, red);
     base += cnt->size;
     gbase += cnt->goals;
@@ -172,7 +175,11 @@ __global__ __launch_bounds__(kAppendBlock) void prrt_commit(int nblk, const int3
     __shared__ int64_t red[32];
     const int t = threadIdx.x;
     if (cnt->done) return;  // block-uniform
-    int64_t add = t < nblk ? blk[2 * t] : 0, gadd = t < nblk ? blk[2 * t + 1] : 0;
+    int64_t add = 0, gadd = 0;
+    for (int j = t; j < nblk; j += kAppendBlock) {
+        add += blk[2 * j];
+        gadd += blk[2 * j + 1];
+    }
     block_sum2(&add, &gadd, red);
     if (t != 0) return;
     const int64_t size = cnt->size + add;
