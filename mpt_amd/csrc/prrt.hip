@@ -81,6 +81,11 @@ __global__ __launch_bounds__(kAppendBlock) void prrt_count(SpaceArgs sp, PrrtDev
     __shared__ unsigned long long part[5];
     const int D = sp.dim;
     const int t = threadIdx.x;
+    if (blockIdx.x == 0 && t == 0) {  // what prrt_write builds on
+        cnt->size_seen = cnt->size;
+        cnt->goals_seen = cnt->goals;
+        cnt->done_seen = cnt->done;
+    }
     if (cnt->done) return;  // block-uniform
     if (t < 5) part[t] = 0;
     const int b = blockIdx.x * kAppendBlock + t;
@@ -131,30 +136,31 @@ __global__ __launch_bounds__(kAppendBlock) void prrt_write(int D, int dp, int B,
                                                            const uint8_t* __restrict__ flags,
                                                            const int32_t* __restrict__ blk, double* __restrict__ x64,
                                                            float* __restrict__ x32, int32_t* __restrict__ parent,
-                                                           const Counters* __restrict__ cnt,
+                                                           Counters* __restrict__ cnt,
                                                            int32_t* __restrict__ goal_ids, int64_t goal_cap) {
     __shared__ int wsum[16];
     __shared__ int64_t red[32];
     const int t = threadIdx.x;
-    if (cnt->done) return;  // block-uniform
+    // the *_seen values stay fixed while this kernel runs; size, goals,
+    // iteration and done change only in its last block, after its writes
+    if (cnt->done_seen) return;  // block-uniform
     // this block's first node id and goal slot: the counts of the blocks
     // before it on top of the tree
-    int64_t base = 0, gbase = 0;
+    int64_t before = 0, gbefore = 0;
     for (int j = t; j < (int)blockIdx.x; j += kAppendBlock) {
-        base += blk[2 * j];
-        gbase += blk[2 * j + 1];
+        before += blk[2 * j];
+        gbefore += blk[2 * j + 1];
     }
-    block_sum2(&base, &gbase, red);
-    base += cnt->size;
-    gbase += cnt->goals;
+    block_sum2(&before, &gbefore, red);
+    const int64_t base = cnt->size_seen + before, gbase = cnt->goals_seen + gbefore;
     __syncthreads();
     const int b = blockIdx.x * kAppendBlock + t;
     const uint8_t f = b < B ? flags[b] : 0;
     const bool ok = f & 1, goal = f & 2;
-    int tot;
-    const int r = block_rank(ok, wsum, &tot);
+    int tot_ok, tot_goal;
+    const int r = block_rank(ok, wsum, &tot_ok);
     __syncthreads();
-    const int gr = block_rank(goal, wsum, &tot);
+    const int gr = block_rank(goal, wsum, &tot_goal);
     if (ok) {
         const int64_t id = base + r;
         for (int i = 0; i < D; ++i) {
@@ -168,25 +174,14 @@ __global__ __launch_bounds__(kAppendBlock) void prrt_write(int D, int dp, int B,
             if (g < goal_cap) goal_ids[g] = (int32_t)id;
         }
     }
-}
-
-__global__ __launch_bounds__(kAppendBlock) void prrt_commit(int nblk, const int32_t* __restrict__ blk,
-                                                            Counters* __restrict__ cnt) {
-    __shared__ int64_t red[32];
-    const int t = threadIdx.x;
-    if (cnt->done) return;  // block-uniform
-    int64_t add = 0, gadd = 0;
-    for (int j = t; j < nblk; j += kAppendBlock) {
-        add += blk[2 * j];
-        gadd += blk[2 * j + 1];
+    // the last block commits: every block before it is counted in `before`
+    if (blockIdx.x == gridDim.x - 1 && t == 0) {
+        const int64_t size = base + tot_ok;
+        cnt->size = size;
+        cnt->goals = gbase + tot_goal;
+        cnt->iteration += 1;
+        cnt->done = (size >= cnt->target || cnt->iteration >= cnt->iter_end) ? 1 : 0;
     }
-    block_sum2(&add, &gadd, red);
-    if (t != 0) return;
-    const int64_t size = cnt->size + add;
-    cnt->size = size;
-    cnt->goals += gadd;
-    cnt->iteration += 1;
-    cnt->done = (size >= cnt->target || cnt->iteration >= cnt->iter_end) ? 1 : 0;
 }
 
 }  // namespace
@@ -253,14 +248,17 @@ double planner_coord_bound(const mptg_ctx* ctx) {
 int counters_arm(mptg_ctx* ctx, void* dcnt, int64_t iteration, int64_t target, int64_t max_iters) {
     struct Tail {
         int64_t iteration, target, iter_end;
-        int32_t done, pad;
+        int32_t done, done_seen;
+        int64_t size_seen, goals_seen;
     } t;
     static_assert(sizeof(Tail) == sizeof(Counters) - offsetof(Counters, iteration), "counter tail layout");
     t.iteration = iteration;
     t.target = target;
     t.iter_end = max_iters > INT64_MAX - iteration ? INT64_MAX : iteration + max_iters;
     t.done = 0;
-    t.pad = 0;
+    t.done_seen = 0;
+    t.size_seen = 0;
+    t.goals_seen = 0;
     MPTG_HIP(ctx, hipMemcpyAsync((char*)dcnt + offsetof(Counters, iteration), &t, sizeof(t),
                                  hipMemcpyHostToDevice, ctx->stream));
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -305,10 +303,8 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
                        (const uint32_t*)pb.checks.p, (uint8_t*)pb.mask.p, (int32_t*)pb.blk.p, dcnt);
     hipLaunchKernelGGL(prrt_write, dim3(ab), dim3(kAppendBlock), 0, ctx->stream, D, pitch32(D), B,
                        (const double*)pb.to.p, (const int32_t*)pb.near.p, (const uint8_t*)pb.mask.p,
-                       (const int32_t*)pb.blk.p, ctx->x64, ctx->x32, ctx->parent, (const Counters*)dcnt,
+                       (const int32_t*)pb.blk.p, ctx->x64, ctx->x32, ctx->parent, dcnt,
                        (int32_t*)pb.goal_buf.p, (int64_t)4096);
-    hipLaunchKernelGGL(prrt_commit, dim3(1), dim3(kAppendBlock), 0, ctx->stream, (int)ab, (const int32_t*)pb.blk.p,
-                       dcnt);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

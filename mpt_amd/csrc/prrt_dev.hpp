@@ -21,7 +21,12 @@ struct Counters {
     int64_t goals;
     int64_t zero, invalid, link_calls, link_ok, valid_calls;
     int64_t iteration, target, iter_end;
-    int32_t done, pad;
+    int32_t done;
+    // size, goals and done as the iteration's append step found them
+    // (prrt_count writes them, prrt_write reads them while its last block
+    // commits the new size)
+    int32_t done_seen;
+    int64_t size_seen, goals_seen;
 };
 
 struct PrrtDev {
