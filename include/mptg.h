@@ -107,10 +107,11 @@ int mptg_tree_read(mptg_ctx* ctx, int64_t first, int64_t n, double* h_states, in
  *      plain kNN. */
 int mptg_knn(mptg_ctx* ctx, const double* h_q, int64_t nq, int k, double radius,
              int32_t* h_idx, double* h_dist);
-/* Device buffers, on the context stream.  Trees of 2^16 nodes or more in
- * 2..8 dimensions (L1/L2, knn_grid on) go through a uniform grid built in the
- * call, which reads its count of unanswered queries back to the host: the
- * call then returns with its work done; otherwise it only enqueues. */
+/* Device buffers, on the context stream.  On trees of 2^16 nodes or more the
+ * uniform grid (L1/L2 in 2..8 dimensions, knn_grid on) and the select path
+ * (k > 1, knn_select on) read counts of queries left for the scans back to
+ * the host, so such calls return with their work done; other calls only
+ * enqueue. */
 int mptg_knn_dev(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius,
                  int32_t* d_idx, double* d_dist);
 
