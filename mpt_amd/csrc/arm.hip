@@ -65,9 +65,9 @@ __device__ __forceinline__ void node_at(int h, const uint16_t* __restrict__ rank
 }
 
 constexpr int kHeadRanks = 4;                     // ranks per edge in the packed round
-constexpr int kHeadEdges = kWave / kHeadRanks;    // edges per wave in the packed round
 
-// Packed round: 16 edges per wave, 4 lanes per edge.  Endpoint checks (when
+// Packed round: 4 ranks per edge, 16 edges per wave (8 when each check takes
+// 2 lanes, head_lanes).  Endpoint checks (when
 // not vouched for), the tree shape, then reference ranks 0..3 (the surrogate
 // order leads with them too, so that no edge this round decides is scored).
 // Edges still undecided go to the continuation list.
@@ -79,6 +79,30 @@ __device__ __forceinline__ bool arm_ok_link(const SceneArgs& sc, Coord&& coord, 
     return arm_ok_wave<D, COUNT>(sc, coord, active, wk);
 }
 
+// Lanes per waypoint check in the packed round: the timed per-lane-culling
+// kernel splits each check's disk runs over 2 lanes (arm_ok_split: the same
+// disks, so the same verdicts), which doubles its waves; the work-counting
+// and wave-box variants keep one lane per check.
+template <bool COUNT, bool LANE>
+constexpr int head_lanes() {
+    return (!COUNT && LANE) ? 2 : 1;
+}
+template <bool COUNT, bool LANE>
+constexpr int head_edges() {
+    return kWave / (kHeadRanks * head_lanes<COUNT, LANE>());
+}
+
+// The 4 rank flags of edge slot `slot` from a wave ballot (one bit per lane,
+// HL lanes per rank holding the same flag).
+template <int HL>
+__device__ __forceinline__ uint32_t head_group(uint64_t m, int slot) {
+    const uint64_t w = m >> (kHeadRanks * HL * slot);
+    uint32_t g = 0;
+#pragma unroll
+    for (int k = 0; k < kHeadRanks; ++k) g |= (uint32_t)((w >> (k * HL)) & 1u) << k;
+    return g;
+}
+
 template <int D, bool COUNT, int ORD, bool LANE>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const double* __restrict__ from, const double* __restrict__ to, int64_t n, int known_valid,
@@ -86,22 +110,29 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks, int64_t* __restrict__ irregular,
     uint32_t* __restrict__ n_irregular, int64_t* __restrict__ cont, uint32_t* __restrict__ n_cont,
     unsigned long long* __restrict__ work) {
+    constexpr int HL = head_lanes<COUNT, LANE>();
+    constexpr int HE = head_edges<COUNT, LANE>();
     ArmWork wk{0, 0, 0, 0, 0};
     const int lane = lane_id();
-    const int slot = lane / kHeadRanks, rk = lane % kHeadRanks;
-    const int64_t e = ((int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) * kHeadEdges + slot;
+    const int slot = lane / (kHeadRanks * HL), rk = (lane / HL) % kHeadRanks;
+    const bool first = lane % HL == 0;  // the lane of its group that writes
+    const int64_t e = ((int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) * HE + slot;
     bool live = e < n && (!mask || mask[e]);
+    // one waypoint check per group of HL lanes
+    auto check = [&](auto&& coord, bool act) {
+        if constexpr (HL == 1) return arm_ok_link<D, COUNT, LANE>(sc, coord, act, &wk);
+        else return arm_ok_split<D, HL>(sc, coord, act);
+    };
     const int64_t ec = live ? e : 0;
     const double* a = from + ec * D;
     const double* b = to + ec * D;
     const uint32_t base = 2;
     if (!known_valid) {
         const bool act = live && rk < 2;
-        const bool bad =
-            act && !arm_ok_link<D, COUNT, LANE>(sc, [&](int i) { return rk == 0 ? a[i] : b[i]; }, act, &wk);
-        const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
+        const bool bad = act && !check([&](int i) { return rk == 0 ? a[i] : b[i]; }, act);
+        const uint32_t g = head_group<HL>(__ballot(bad), slot);
         if (live && (g & 3u)) {
-            if (rk == 0) {
+            if (rk == 0 && first) {
                 verdict[e] = 0;
                 checks[e] = (g & 1u) ? 1u : 2u;
             }
@@ -112,13 +143,13 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     bool regular = false;
     if (live) {
         if (!bisect_shape<D>(a, b, tol, &h, &regular)) {
-            if (rk == 0) {
+            if (rk == 0 && first) {
                 verdict[e] = 0;
                 checks[e] = MPTG_CHECKS_OVER_LIMIT;
             }
             live = false;
         } else if (!regular || (ORD == MPTG_ORDER_SURROGATE && h > 9)) {
-            if (rk == 0) irregular[atomicAdd(n_irregular, 1u)] = e;  // exact per-edge walk later
+            if (rk == 0 && first) irregular[atomicAdd(n_irregular, 1u)] = e;  // exact per-edge walk later
             live = false;
         }
     }
@@ -129,10 +160,9 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     uint32_t path = 0;
     int depth = 0;
     if (act) preorder_path(h, (uint32_t)rk, &path, &depth);  // reference ranks in both orders
-    const bool bad = act && !arm_ok_link<D, COUNT, LANE>(
-                                sc, [&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; }, act, &wk);
-    const uint32_t g = (uint32_t)(__ballot(bad) >> (kHeadRanks * slot)) & 0xfu;
-    if (live && rk == 0) {
+    const bool bad = act && !check([&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; }, act);
+    const uint32_t g = head_group<HL>(__ballot(bad), slot);
+    if (live && rk == 0 && first) {
         if (g) {
             verdict[e] = 0;
             checks[e] = base + (uint32_t)__builtin_ctz(g) + 1u;
@@ -239,7 +269,7 @@ static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to
                              int64_t* cont, uint32_t* ncont, unsigned long long* work) {
     const double tol = ctx->link_param;
     const SceneArgs sa = scene_args(ctx);
-    const int64_t per_block = (int64_t)kWavesPerBlock * kHeadEdges;
+    const int64_t per_block = (int64_t)kWavesPerBlock * head_edges<COUNT, LANE>();
     hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD, LANE>), dim3((unsigned)((n + per_block - 1) / per_block)),
                        dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, verdict, checks,
                        irr, nirr, cont, ncont, work);
