@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
 // slot ends at exactly the first failing rank of the chosen order.  No wave
 // waits for another: the slot is read and lowered, never awaited.
 template <int D, bool COUNT, int ORD, bool LANE>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_tail(
+__global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(ORD == MPTG_ORDER_REFERENCE ? 5 : 4))) void arm_link_tail(
     const double* __restrict__ from, const double* __restrict__ to, double tol, SceneArgs sc,
     const uint16_t* __restrict__ ranked, const int64_t* __restrict__ cont, const uint32_t* __restrict__ n_cont,
     int waves_per_edge, uint32_t* __restrict__ checks, unsigned long long* __restrict__ work) {
