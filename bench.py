@@ -2,18 +2,22 @@
 
 Headline workload (default): the 20-link planar arm among 200 disks, batched
 PRRT with 8192 samples per iteration; one step = one iteration (sample ->
-nearest -> steer -> valid -> link -> insert for 8192 samples).  The planner
-runs twice from the same seed: with the reference waypoint order ("plain")
-and with the surrogate order (after training the surrogate on 2^22 exact
-labels, untimed).  `value` is the surrogate-ordered planner's edge checks
+nearest -> steer -> valid -> link -> insert for 8192 samples) on a tree of
+--tree-size nodes (10^4 by default, grown untimed from the start and
+reloaded before every step, so a step's cost does not depend on --steps).
+The steps run twice with the same seeds: with the reference waypoint order
+("plain") and with the surrogate order (after training the surrogate on 2^22
+exact labels, untimed); the trees they append must be identical.  `value` is the surrogate-ordered planner's edge checks
 (link() calls decided) per second; the line also carries the plain rate, the
 gain, extension attempts per second, the roofline of the dominant kernel (the
-arm link() kernel, FP64 VALU), the MFMA rate of the score kernel, and the CPU
-baseline (the reference-style std::thread worker pool, all host cores).  Extra lines for the
-nearest-neighbour kernel (1e6 x R^7, 65536 queries), the occupancy-grid edge
-kernel (4096^2 grid, 2^20 edges), the boxes planner and the SE(3) mesh
-PRRT* (500 vs 50k triangles, k = 20, 4096 samples per iteration) are
-reported under "extra".
+arm link() kernels, FP64 vector ALU), the MFMA rate of the score kernel, and
+the CPU baseline (the reference-style std::thread worker pool from the same
+tree, all cores of the affinity mask and one thread, medians of 5).  Extra
+lines for the nearest-neighbour kernel (1e6 x R^7, 65536 queries), the
+occupancy-grid edge kernel (4096^2 grid, 2^20 edges), the grid planner (8192
+samples per iteration), the boxes planner and the SE(3) mesh PRRT* (500 vs
+50k triangles, k = 20, 4096 samples per iteration) are reported under
+"extra".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -58,49 +62,58 @@ class Timer:
         return self.ctx.time_ms(fn, reps) / 1e3
 
 
-def run_planner(ctx, sc, args, order, batch, graph=0):
-    """Grows the arm tree from the start: warmup steps, then the timed steps
-    (`graph` iterations per hipGraph launch, 0 = step by step)."""
+def grow_tree(ctx, sc, seed, batch, size):
+    """Grows a planner tree from the start (reference order) until it holds
+    at least `size` nodes; returns (nodes, parents)."""
     ctx.tree_clear()
-    ctx.prrt_init(sc, seed=args.seed, batch=batch, order=order)
-    ctx.prrt_graph(graph)
+    ctx.prrt_init(sc, seed=seed, batch=batch)
     ctx.prrt_add_start(sc.start)
-    ctx.prrt_run(1 << 30, args.warmup)
-    s0 = ctx.prrt_stats()
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.prrt_run(1 << 30, args.steps)
-    ctx.sync()
-    dt = time.perf_counter() - t0
-    s1 = ctx.prrt_stats()
-    d = {k: s1[k] - s0[k] for k in s1}
-    nodes, parents = ctx.tree_read()
-    res = {
-        "ms_per_step": dt / args.steps * 1e3,
-        "edge_checks_per_s": d["link_calls"] / dt,
-        "extensions_per_s": d["samples"] / dt,
-        "accepted_per_s": d["link_ok"] / dt,
-        "valid_calls_per_s": d["valid_calls"] / dt,
-        "link_calls": d["link_calls"],
-        "valid_calls": d["valid_calls"],
-        "tree_size": s1["size"],
-        "graph_iterations": graph,
-        "graph_launches": d["graph_launches"],
+    ctx.prrt_run(size)
+    return ctx.tree_read()
+
+
+def run_steps(ctx, sc, tree, args, order, batch):
+    """Warmup steps, then the timed steps.  Every step is one planner
+    iteration (sample -> nearest -> steer -> valid -> link -> insert for
+    `batch` samples) on the same tree of len(tree[0]) nodes, with its own
+    seed: the tree is reloaded (untimed) before each step, so the cost of a
+    step does not depend on how many steps run.  The timed region is the
+    sum of the steps' wall times, each bracketed by stream synchronisation."""
+    import hashlib
+    nodes0, par0 = tree
+    n0 = nodes0.shape[0]
+    tot = {}
+    secs = 0.0
+    digest = hashlib.sha256()
+    for i in range(args.warmup + args.steps):
+        ctx.tree_clear()
+        ctx.tree_insert(nodes0, par0)
+        ctx.prrt_init(sc, seed=args.seed + 1 + i, batch=batch, order=order)
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = ctx.prrt_run(1 << 30, 1)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        nodes, parents = ctx.tree_read(n0)
+        digest.update(nodes.tobytes())
+        digest.update(parents.tobytes())
+        if i < args.warmup:
+            continue
+        secs += dt
+        for k, v in st.items():
+            tot[k] = tot.get(k, 0) + v
+    return {
+        "ms_per_step": secs / args.steps * 1e3,
+        "edge_checks_per_s": tot["link_calls"] / secs,
+        "extensions_per_s": tot["samples"] / secs,
+        "accepted_per_s": tot["link_ok"] / secs,
+        "valid_calls_per_s": tot["valid_calls"] / secs,
+        "link_calls": tot["link_calls"],
+        "valid_calls": tot["valid_calls"],
+        "accepted": tot["link_ok"],
+        "tree_size": n0,
+        "trees_sha256": digest.hexdigest(),
     }
-    return res, nodes, parents
-
-
-def tune_planner(ctx, sc, args, batch, order=None):
-    """Iterations per graph launch chosen by the autotuner on a tree of its
-    own (the choice does not change any tree), for the planner in `order`."""
-    import mpt_amd
-    ctx.tree_clear()
-    ctx.prrt_init(sc, seed=args.seed + 1000, batch=batch,
-                  order=mpt_amd.ORDER_REFERENCE if order is None else order)
-    ctx.prrt_add_start(sc.start)
-    rep = ctx.autotune_prrt()
-    ctx.prrt_graph(0)
-    return rep
 
 
 def arm_link_traffic():
@@ -127,31 +140,23 @@ def bench_arm(ctx, args, timer):
     sc = scenes.arm_scene()
     B = 8192
     ctx.scene(sc)
+    log(f"  tree of {args.tree_size} nodes ...")
+    tree = grow_tree(ctx, sc, args.seed, B, args.tree_size)
+    nodes_p = tree[0]
     log("  link autotune ...")
-    # on planner edges from a tree of the timed runs' size (grown untimed)
-    _, nodes_t, _ = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B, 0)
-    ta, tb = surrogate.planner_edges(ctx, sc, nodes_t, 8192, seed=98)
+    # on planner edges from the timed steps' tree
+    ta, tb = surrogate.planner_edges(ctx, sc, nodes_p, 8192, seed=98)
     dta, dtb = ctx.to_device(ta), ctx.to_device(tb)
     tune_link = ctx.autotune_link(dta.ptr, dtb.ptr, ta.shape[0])
     dta.free()
     dtb.free()
-    log("  graph autotune ...")
-    tune = tune_planner(ctx, sc, args, B)
-    G = tune["chosen"]
     log("  plain planner ...")
-    plain, nodes_p, par_p = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B, G)
-    plain_steps, nodes_ps, par_ps = run_planner(ctx, sc, args, mpt_amd.ORDER_REFERENCE, B, 0)
+    plain = run_steps(ctx, sc, tree, args, mpt_amd.ORDER_REFERENCE, B)
     log("  surrogate training ...")
     train = surrogate.train(ctx, sc, nodes_p, n_samples=args.train_samples, epochs=args.train_epochs, seed=5)
     log("  surrogate-ordered planner ...")
-    # the surrogate planner has launches of its own (scoring, ranking): its
-    # graph size is tuned for it
-    tune_s = tune_planner(ctx, sc, args, B, mpt_amd.ORDER_SURROGATE)
-    Gs = tune_s["chosen"]
-    sur, nodes_s, par_s = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B, Gs)
-    sur_steps, nodes_ss, par_ss = run_planner(ctx, sc, args, mpt_amd.ORDER_SURROGATE, B, 0)
-    same_tree = all(bool(np.array_equal(nodes_p, n) and np.array_equal(par_p, p))
-                    for n, p in ((nodes_s, par_s), (nodes_ps, par_ps), (nodes_ss, par_ss)))
+    sur = run_steps(ctx, sc, tree, args, mpt_amd.ORDER_SURROGATE, B)
+    same_tree = plain["trees_sha256"] == sur["trees_sha256"]
 
     # dominant kernels on a fixed batch of planner-like edges
     ne = 8192
@@ -214,31 +219,38 @@ def bench_arm(ctx, args, timer):
         "score_frac_bf16": mlp_flops / t_score / BF16_PEAK,
     }
     # roofline of the dominant kernels, the reference-order link (packed round
-    # + continuation + finish): algorithmic work = the reference algorithm's
-    # own count on these edges (FK per link, every disk tested until the first
-    # hit: the oracle's count), as the headline workload prices it; the
-    # kernel's broad phase skips most of those disk tests exactly, so this
-    # rate can exceed the FP64 peak.  executed_* is the work the kernel itself
-    # issues (its instrumented twin's count), against the same peak.
+    # + continuation + finish, one HIP-event span on the context stream).
+    # Algorithmic work: the reference loop's forward kinematics (every link of
+    # every reference-order waypoint check up to the first hit) plus the
+    # segment-disk tests whose disk lies within link length + r + R of the
+    # link's start joint -- the pairs no exact per-link distance bound clears,
+    # so an exact checker must test them; counted on the device by the
+    # reference walk (pinned against the oracle by the tests).  The brute-force
+    # rate (every disk tested, as the reference's loop does) is reported as
+    # effective_tflops; executed_* is the work the kernels themselves issue
+    # (their instrumented twin's count).
+    alg_flops = FLOP_FK_LINK * wk_p["ref_fk"] + FLOP_REF_TEST * wk_p["ref_near_tests"]
     ref_flops = FLOP_FK_LINK * wk_p["ref_fk"] + FLOP_REF_TEST * wk_p["ref_tests"]
-    roof = {"bound": "mfma", "pipe": "FP64 VALU (compute-bound; the FP64 dense peak)",
-            "achieved": ref_flops / t_kernel_p / 1e12, "peak": FP64_PEAK / 1e12,
-            "unit": "TFLOP/s", "frac": ref_flops / t_kernel_p / FP64_PEAK, "traffic": arm_link_traffic(),
+    roof = {"bound": "fp64-valu", "pipe": "FP64 vector ALU (the kernels issue no MFMA); peak = MI355X FP64 vector",
+            "achieved": alg_flops / t_kernel_p / 1e12, "peak": FP64_PEAK / 1e12,
+            "unit": "TFLOP/s", "frac": alg_flops / t_kernel_p / FP64_PEAK, "traffic": arm_link_traffic(),
             "kernel": "arm_link_head + arm_link_tail + arm_link_finish <20, reference order>, 8192 planner edges",
-            "launch_ms": t_kernel_p * 1e3, "flops_per_launch": ref_flops,
-            "work_unit": "per waypoint check: 20 FK links x 50 flop + reference disk tests x 20 flop",
+            "launch_ms": t_kernel_p * 1e3, "flops_per_launch": alg_flops,
+            "work_unit": "per reference-order waypoint check: 20 FK links x 50 flop, plus 20 flop per segment-disk "
+                         "test whose disk lies within link length + r + R of the link's start joint",
+            "effective_tflops": ref_flops / t_kernel_p / 1e12,
+            "effective_unit": "reference brute force: 20 FK links x 50 flop + every disk tested until the first "
+                              "hit x 20 flop",
             "executed_tflops": flops_p / t_kernel_p / 1e12, "executed_frac": flops_p / t_kernel_p / FP64_PEAK,
             "work": wk_p}
-    res = {"plain": plain, "surrogate": sur, "plain_step_by_step": plain_steps,
-           "surrogate_step_by_step": sur_steps, "graph_autotune": tune, "graph_autotune_surrogate": tune_s,
-           "tail_autotune": tune_link,
+    res = {"plain": plain, "surrogate": sur, "tail_autotune": tune_link,
            "launch_shapes": ctx.tune_get(),
            "trees_identical": same_tree, "training": train,
            "edge_batch": edge_batch,
            "planner_gain": sur["edge_checks_per_s"] / plain["edge_checks_per_s"]}
     for x in (da, db, v, c, scores):
         x.free()
-    return res, roof, sc
+    return res, roof, sc, tree
 
 
 def bench_nn(ctx, args, timer):
@@ -302,6 +314,35 @@ def bench_grid(ctx, args, timer):
             "occupancy": sc.extra["occupancy"], "autotune_long_waves_per_cu": tune}
 
 
+def bench_grid_planner(ctx, args):
+    """Occupancy-grid PRRT: 4096^2 grid (seed 4), 8192 samples per iteration,
+    range 40 px, 1 px DMV step; steps as the headline's, each one iteration on
+    the same tree of 2 * 10^4 nodes."""
+    import dataclasses
+    import mpt_amd
+    from mpt_amd import scenes
+    sc = dataclasses.replace(scenes.grid_scene(size=4096, seed=4), range=40.0)
+    ctx.scene(sc)
+    tree = grow_tree(ctx, sc, args.seed, 8192, 20_000)
+    # lanes per DMV edge and the long-edge grid, tuned on planner-like edges
+    # of this tree: nearest tree node to a uniform free sample, steered to 40 px
+    rng = np.random.default_rng(41)
+    q = rng.uniform(0, 4096, (8192, 2))
+    q = q[ctx.valid(q)]
+    idx, dist = ctx.knn(q, 1)
+    a = tree[0][idx[:, 0]]
+    t = np.minimum(1.0, 40.0 / np.maximum(dist[:, 0], 1e-300))[:, None]
+    b = (q - a) * t + a
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    tune = ctx.autotune_link(da.ptr, db.ptr, a.shape[0])
+    da.free()
+    db.free()
+    res = run_steps(ctx, sc, tree, args, mpt_amd.ORDER_REFERENCE, 8192)
+    res.update({"workload": "grid4096_disks_prrt_b8192_range40", "autotune_dmv": tune,
+                "launch_shapes": ctx.tune_get()})
+    return res
+
+
 def bench_boxes(ctx, args):
     from mpt_amd import scenes
     sc = scenes.boxes_scene()
@@ -352,16 +393,50 @@ def bench_se3(ctx, args):
             "launch_shapes": ctx.tune_get()}
 
 
-def cpu_baseline(sc, seconds):
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, tree, seconds):
+    """The reference's std::thread worker pool, restated (oracle/orc_pool.cpp),
+    on the headline scene from the GPU steps' tree: one untimed warm-up run,
+    then 5 runs on every core of this process's affinity and 5 runs on one
+    thread; each run starts from the same tree and lasts `seconds`; medians."""
     import oracle
     o = oracle.Oracle(sc)
-    # the box's CPU share is 16 cores whatever os.cpu_count() reports
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    r = o.pool(seed=7, target=200_000, seconds=seconds, threads=threads)
-    return {"value": r["link_calls"] / r["seconds"], "unit": "edge checks/s", "cores": threads, "kind": "port",
-            "sample": f"std::thread worker pool (reference loop restated), {threads} threads, arm scene, "
-                      f"{r['seconds']:.1f} s wall, {r['link_calls']} link() calls, tree of {r['size']} nodes",
-            "extensions_per_s": r["samples"] / r["seconds"]}
+    cores = len(os.sched_getaffinity(0))
+    n0 = tree[0].shape[0]
+
+    def runs(threads):
+        out = []
+        for i in range(5):
+            r = o.pool(seed=7 + i, target=n0 + 1_000_000, seconds=seconds, threads=threads, tree=tree)
+            out.append(r)
+        rates = sorted(r["link_calls"] / r["seconds"] for r in out)
+        med = rates[len(rates) // 2]
+        ext = sorted(r["samples"] / r["seconds"] for r in out)[len(out) // 2]
+        grown = sorted(r["size"] for r in out)[len(out) // 2]
+        return med, ext, rates, grown
+
+    o.pool(seed=6, target=n0 + 1_000_000, seconds=min(1.0, seconds), threads=cores, tree=tree)
+    med, ext, rates, grown = runs(cores)
+    med1, ext1, rates1, grown1 = runs(1)
+    return {"value": med, "unit": "edge checks/s", "cores": cores, "kind": "port",
+            "sample": f"std::thread worker pool (reference PRRT loop restated, project KD-tree standing in for nigh), "
+                      f"arm scene, each run from the GPU steps' tree of {n0} nodes for {seconds:.1f} s wall; "
+                      f"median of 5 runs after 1 warm-up, on {cores} threads (all cores of the affinity mask) and "
+                      f"on 1 thread",
+            "cpu_model": cpu_model(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "runs_edge_checks_per_s": rates, "extensions_per_s": ext, "tree_size_start": n0,
+            "tree_size_end_median": grown,
+            "one_thread": {"value": med1, "runs_edge_checks_per_s": rates1, "extensions_per_s": ext1,
+                           "tree_size_end_median": grown1}}
 
 
 def kernels_only(ctx, args):
@@ -373,11 +448,8 @@ def kernels_only(ctx, args):
     from mpt_amd import scenes, surrogate
     sc = scenes.arm_scene()
     ctx.scene(sc)
-    ctx.tree_clear()
-    ctx.prrt_init(sc, seed=args.seed, batch=8192)
-    ctx.prrt_add_start(sc.start)
-    ctx.prrt_run(1 << 30, 3)
-    nodes, _ = ctx.tree_read()
+    # the headline's edge batch: planner edges from its tree
+    nodes, _ = grow_tree(ctx, sc, args.seed, 8192, args.tree_size)
     surrogate.train(ctx, sc, nodes, n_samples=1 << 18, epochs=1, seed=5)
     ne = 8192
     a, b = surrogate.planner_edges(ctx, sc, nodes, ne, seed=99)
@@ -409,11 +481,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=17)
     ap.add_argument("--quick", action="store_true", help="skip extras and the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=2.0, help="wall seconds of each CPU baseline run")
+    ap.add_argument("--tree-size", type=int, default=10_000,
+                    help="nodes of the tree every timed planner step starts from")
     ap.add_argument("--train-samples", type=int, default=1 << 22)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--kernels", action="store_true", help="profiling mode: the dominant kernels only")
-    ap.add_argument("--leg", choices=["nn", "grid", "boxes", "se3"],
+    ap.add_argument("--leg", choices=["nn", "grid", "grid_planner", "boxes", "se3"],
                     help="profiling mode: one extra workload alone, its JSON on one line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -434,20 +508,22 @@ def main():
 
     if args.leg:
         fns = {"nn": (bench_nn, args, timer), "grid": (bench_grid, args, timer), "boxes": (bench_boxes, args),
-               "se3": (bench_se3, args)}
+               "se3": (bench_se3, args), "grid_planner": (bench_grid_planner, args)}
         fn, *a = fns[args.leg]
         print(json.dumps({"leg": args.leg, "result": leg(fn, *a)}))
         ctx.close()
         return
 
     log("arm planner ...")
-    arm, roof, sc = leg(bench_arm, args, timer)
+    arm, roof, sc, tree = leg(bench_arm, args, timer)
     extra = {}
     if not args.quick:
         log("nearest-neighbour kernel ...")
         extra["nn_1e6x7_q65536"] = leg(bench_nn, args, timer)
         log("grid edges ...")
         extra["grid4096_edges_2p20"] = leg(bench_grid, args, timer)
+        log("grid planner ...")
+        extra["grid4096_prrt_b8192"] = leg(bench_grid_planner, args)
         log("boxes planner ...")
         extra["boxes16_prrt_1e4"] = leg(bench_boxes, args)
         log("se3 prrt* ...")
@@ -455,7 +531,7 @@ def main():
     cpu = None
     if not args.quick:
         log("cpu baseline ...")
-        cpu = cpu_baseline(sc, args.cpu_seconds)
+        cpu = cpu_baseline(sc, tree, args.cpu_seconds)
     sur = arm["surrogate"]
     line = {
         "metric": "RRT extensions/sec + edge checks/sec, surrogate-ordered vs plain, 1 MI355X",
@@ -471,6 +547,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded scenes and Philox samples)",
         "config": {"workload": "arm20_disks200_prrt_b8192", "links": 20, "disks": 200, "batch": 8192,
+                   "tree_size": arm["plain"]["tree_size"],
+                   "step": "one planner iteration of 8192 samples on the same tree of tree_size nodes",
                    "check_order": "surrogate", "surrogate": "mlp 64-3x128-1 relu bf16"},
         "extensions_per_s": sur["extensions_per_s"],
         "plain_edge_checks_per_s": arm["plain"]["edge_checks_per_s"],

@@ -271,6 +271,7 @@ typedef struct mptg_tune_params {
                                        select (registers of k = 1 in the scan), 0 the k-list scan */
     int32_t knn_grid;               /* L1/L2 in 2..8 dimensions on trees of 2^16 nodes or more: 1 a
                                        uniform grid built per call, searched cell by cell; 0 the scans */
+    int32_t dmv_edge_lanes;         /* DMV link: lanes sharing one edge's checks (1, 2, 4, .., 64) */
 } mptg_tune_params;
 
 enum {
@@ -280,7 +281,8 @@ enum {
     MPTG_TUNE_GRAPH = 3,
     MPTG_TUNE_ARM_TAIL = 4,
     MPTG_TUNE_ARM_CULL = 5,
-    MPTG_TUNE_MESH_EDGE = 6
+    MPTG_TUNE_MESH_EDGE = 6,
+    MPTG_TUNE_DMV_LANES = 7
 };
 
 /* What one tuning call tried and chose: the knob, each candidate value and
@@ -304,8 +306,8 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p);
    tune the scans themselves. */
 int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_tune_report* rep);
 /* Times link() of n device edges (reference order) for each candidate of the
-   scene's link launch shape -- the grid of the persistent DMV long-edge
-   kernel, for meshes the grid and then the waves per edge, for the arm the
+   scene's link launch shape -- for DMV scenes the lanes per edge and then
+   the grid of the persistent long-edge kernel, for meshes the grid and then the waves per edge, for the arm the
    disk culling and then the waves per continued edge (rep: the last knob
    swept) -- and keeps the fastest;
    the per-edge bisection kernels have none (MPTG_E_STATE). */
@@ -334,9 +336,11 @@ int mptg_diag_copy(mptg_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes
  *                  kinematics links, group broad-phase tests, link tests
  *                  resolved by the far-disk bound, link tests that ran the
  *                  exact expression;
- *   h_work[5..7] = the reference algorithm's work on the same edges: valid()
+ *   h_work[5..8] = the reference algorithm's work on the same edges: valid()
  *                  calls, FK links, segment-disk tests (every disk until the
- *                  first hit). */
+ *                  first hit), and those of the tests whose disk centre lies
+ *                  within (link length + r + R) of the link's start joint,
+ *                  the pairs no exact per-link distance bound clears. */
 int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                    int endpoints_known_valid, uint64_t* h_work);
 

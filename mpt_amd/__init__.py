@@ -272,10 +272,10 @@ class Context:
                   order: int = ORDER_REFERENCE) -> dict:
         """Executed work of the arm link kernel in the given order and the
         reference algorithm's work on the same edges (device buffers)."""
-        w = (C.c_uint64 * 8)()
+        w = (C.c_uint64 * 9)()
         self._chk(_lib.mptg_link_work(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, order,
                                       1 if known_valid else 0, w))
-        keys = ("calls", "fk", "group", "far", "exact", "ref_calls", "ref_fk", "ref_tests")
+        keys = ("calls", "fk", "group", "far", "exact", "ref_calls", "ref_fk", "ref_tests", "ref_near_tests")
         return {k: int(v) for k, v in zip(keys, w)}
 
     # ---- collision surrogate (device buffers are passed as integer pointers)
@@ -368,14 +368,14 @@ class Context:
         return {"knn_blocks_per_cu": t.knn_blocks_per_cu, "dmv_long_waves_per_cu": t.dmv_long_waves_per_cu,
                 "mesh_waves_per_cu": t.mesh_waves_per_cu, "arm_tail_waves": t.arm_tail_waves,
                 "arm_cull": t.arm_cull, "mesh_edge_waves": t.mesh_edge_waves, "knn_select": t.knn_select,
-                "knn_grid": t.knn_grid}
+                "knn_grid": t.knn_grid, "dmv_edge_lanes": t.dmv_edge_lanes}
 
     def tune_set(self, **kw):
         cur = self.tune_get()
         cur.update(kw)
         t = _native.TuneParams(cur["knn_blocks_per_cu"], cur["dmv_long_waves_per_cu"], cur["mesh_waves_per_cu"],
                                cur["arm_tail_waves"], cur["arm_cull"], cur["mesh_edge_waves"], cur["knn_select"],
-                               cur["knn_grid"])
+                               cur["knn_grid"], cur["dmv_edge_lanes"])
         self._chk(_lib.mptg_tune_set(self._h, C.byref(t)))
 
     def autotune_knn(self, d_q: int, nq: int, k: int) -> dict:

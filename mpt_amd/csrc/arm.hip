@@ -129,7 +129,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const uint32_t base = 2;
     if (!known_valid) {
         const bool act = live && rk < 2;
-        const bool bad = act && !check([&](int i) { return rk == 0 ? a[i] : b[i]; }, act);
+        const bool okv = check([&](int i) { return rk == 0 ? a[i] : b[i]; }, act);  // every lane calls it
+        const bool bad = act && !okv;
         const uint32_t g = head_group<HL>(__ballot(bad), slot);
         if (live && (g & 3u)) {
             if (rk == 0 && first) {
@@ -160,7 +161,10 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     uint32_t path = 0;
     int depth = 0;
     if (act) preorder_path(h, (uint32_t)rk, &path, &depth);  // reference ranks in both orders
-    const bool bad = act && !check([&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; }, act);
+    // every lane calls the check (its wave-wide culling shuffles read every
+    // lane), and only lanes with a waypoint count
+    const bool okv = check([&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; }, act);
+    const bool bad = act && !okv;
     const uint32_t g = head_group<HL>(__ballot(bad), slot);
     if (live && rk == 0 && first) {
         if (g) {

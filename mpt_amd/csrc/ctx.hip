@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "internal.hpp"
 #include "space_dev.hpp"
@@ -414,11 +415,24 @@ int mptg_scene_grid(mptg_ctx* ctx, const uint8_t* h_cells, int width, int height
     if (rc) return rc;
     if (width <= 0 || height <= 0 || !h_cells) return fail(ctx, MPTG_E_ARG, "bad grid");
     if ((int64_t)width * height > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "grid larger than 2^31 cells");
-    rc = upload(ctx, (void**)&ctx->scene.grid, h_cells, (size_t)width * height);
+    // one bit per cell in 8 x 8 tiles (the layout grid_valid reads); cells of
+    // the last tile row / column beyond the grid stay 0 and are never read
+    // (grid_valid rejects them by the bounds check first)
+    const int tx = (width + 7) / 8, ty = (height + 7) / 8;
+    std::vector<uint64_t> bits((size_t)tx * ty, 0);
+    for (int y = 0; y < height; ++y) {
+        const uint8_t* row = h_cells + (size_t)width * y;
+        uint64_t* trow = bits.data() + (size_t)(y >> 3) * tx;
+        const int sh = (y & 7) << 3;
+        for (int x = 0; x < width; ++x)
+            if (row[x]) trow[x >> 3] |= 1ull << (sh | (x & 7));
+    }
+    rc = upload(ctx, (void**)&ctx->scene.grid, bits.data(), bits.size() * sizeof(uint64_t));
     if (rc) return rc;
     ctx->scene.kind = MPTG_SCENE_GRID;
     ctx->scene.width = width;
     ctx->scene.height = height;
+    ctx->scene.grid_tx = tx;
     return MPTG_OK;
 }
 
@@ -428,16 +442,18 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
     if (rc) return rc;
     if (nlinks < 1 || nlinks > kMaxDim || !h_lengths) return fail(ctx, MPTG_E_ARG, "arm needs 1..32 links");
     if (ndisks < 0 || ndisks > 65536 || (ndisks > 0 && !h_disks)) return fail(ctx, MPTG_E_ARG, "bad disks");
-    std::vector<double> dk((size_t)ndisks * 3);
+    // reference order: [cx, cy, (r+R)^2, r+R]
+    std::vector<double> dk((size_t)ndisks * 4);
     for (int i = 0; i < ndisks; ++i) {
-        dk[3 * i + 0] = h_disks[3 * i + 0];
-        dk[3 * i + 1] = h_disks[3 * i + 1];
+        dk[4 * i + 0] = h_disks[3 * i + 0];
+        dk[4 * i + 1] = h_disks[3 * i + 1];
         double rr = h_disks[3 * i + 2] + robot_radius;  // (radius_ + robotRadius), squared as the reference
-        dk[3 * i + 2] = rr * rr;
+        dk[4 * i + 2] = rr * rr;
+        dk[4 * i + 3] = rr;
     }
     rc = upload(ctx, (void**)&ctx->scene.lengths, h_lengths, sizeof(double) * nlinks);
     if (rc) return rc;
-    rc = upload(ctx, (void**)&ctx->scene.disks, dk.data(), sizeof(double) * 3 * ndisks);
+    rc = upload(ctx, (void**)&ctx->scene.disks, dk.data(), sizeof(double) * 4 * ndisks);
     if (rc) return rc;
     // kernel layout: [cx, cy, (r+R)^2, r+R] and a per-link segment-length
     // bound L (1 + 2^-30) + 1e-12 for the far-disk skip

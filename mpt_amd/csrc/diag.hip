@@ -57,13 +57,13 @@ extern "C" int mptg_link_work(mptg_ctx* ctx, const double* d_from, const double*
     int rc = check_device_ready(ctx);
     if (rc) return rc;
     if (n < 0 || !h_work || (n > 0 && (!d_from || !d_to))) return fail(ctx, MPTG_E_ARG, "bad buffers");
-    if ((rc = ensure(ctx, ctx->scratch[3], 8 * sizeof(uint64_t)))) return rc;
+    if ((rc = ensure(ctx, ctx->scratch[3], 9 * sizeof(uint64_t)))) return rc;
     if ((rc = ensure(ctx, ctx->scratch[2], (sizeof(uint32_t) + 1) * (size_t)std::max<int64_t>(n, 1)))) return rc;
     unsigned long long* w = (unsigned long long*)ctx->scratch[3].p;
     uint32_t* chk = (uint32_t*)ctx->scratch[2].p;
     uint8_t* ver = (uint8_t*)(chk + std::max<int64_t>(n, 1));
     if ((rc = launch_link_work(ctx, d_from, d_to, n, order, endpoints_known_valid, w, ver, chk))) return rc;
-    MPTG_HIP(ctx, hipMemcpyAsync(h_work, w, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipMemcpyAsync(h_work, w, 9 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return MPTG_OK;
 }

@@ -28,14 +28,15 @@ struct SceneDev {
     // boxes
     double* boxes = nullptr;
     int nboxes = 0;
-    // grid
-    uint8_t* grid = nullptr;
-    int width = 0, height = 0;
+    // grid: one bit per cell (1 = obstacle) in 8 x 8 tiles, tile (tx, ty) at
+    // ty * grid_tx + tx, cell (x, y) at bit 8 * (y % 8) + x % 8
+    uint64_t* grid = nullptr;
+    int width = 0, height = 0, grid_tx = 0;
     // arm
     double* lengths = nullptr;
     int nlinks = 0;
     double robot_radius = 0;
-    double* disks = nullptr;  // [cx, cy, (r+R)^2] per disk
+    double* disks = nullptr;  // [cx, cy, (r+R)^2, r+R] per disk, reference order
     int ndisks = 0;
     DevBuf arm_disks, arm_lim;  // kernel layout of the disks (ascending cx), per-link bounds
     DevBuf arm_index;           // [xmin, 1/bucket width, max (r+R), 0] + int32 bucket starts
@@ -131,8 +132,9 @@ struct mptg_ctx {
     uint64_t epoch = 0;
     // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
     // persistent DMV long-edge kernel and of the mesh kernels, waves per edge
-    // of the arm continuation, arm link disk culling (0 wave box, 1 per lane)
-    mptg_tune_params tune{8, 16, 32, 2, 0, 2, 1, 1};
+    // of the arm continuation, arm link disk culling (0 wave box, 1 per lane),
+    // waves per mesh edge, kNN select and grid paths, lanes per DMV edge
+    mptg_tune_params tune{8, 16, 32, 2, 0, 2, 1, 1, 8};
     hipEvent_t phase_ev[6] = {};
 };
 

@@ -52,7 +52,9 @@ __global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int metric, in
     const int64_t b = t / per;
     if (it < 0) it = cnt->iteration;
     const uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
-    const bool bias = cnt->goals == 0 && goal_bias > 0;
+    // only sample slot 0 of an iteration (the reference's worker 0) takes the
+    // goal-bias coin, and only while no goal is known (prrt.hpp:365-388)
+    const bool bias = b == 0 && cnt->goals == 0 && goal_bias > 0;
     const bool useGoal = bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias;
     if (per == D) {
         const int i = (int)(t - b * per);

@@ -28,11 +28,11 @@ struct SceneArgs {
     int kind;
     const double* boxes;
     int nboxes;
-    const uint8_t* grid;
-    int width, height;
+    const uint64_t* grid;  // 8 x 8 bit tiles, see grid_valid
+    int width, height, grid_tx;
     const double* lengths;
     int nlinks;
-    const double* disks;  // [cx, cy, (r + R)^2]
+    const double* disks;  // [cx, cy, (r + R)^2, r + R], reference order
     int ndisks;
     const ArmDisk* arm_disks;  // same disks, kernel layout, ordered by grid cell (row-major)
     const double* arm_lim;     // per-link segment-length bound
@@ -69,7 +69,7 @@ inline SpaceArgs space_args(const mptg_ctx* ctx) { return SpaceArgs{ctx->metric,
 inline SceneArgs scene_args(const mptg_ctx* ctx) {
     const SceneDev& s = ctx->scene;
     return SceneArgs{s.kind,    s.boxes,  s.nboxes, s.grid,   s.width,
-                     s.height,  s.lengths, s.nlinks, s.disks, s.ndisks,
+                     s.height,  s.grid_tx, s.lengths, s.nlinks, s.disks, s.ndisks,
                      (const ArmDisk*)s.arm_disks.p, (const double*)s.arm_lim.p, (const double*)s.arm_index.p};
 }
 
@@ -150,7 +150,7 @@ __device__ __forceinline__ bool arm_valid(const SceneArgs& sc, const double* q) 
         double tx = fx + L * c;
         double ty = fy + L * s;
         for (int j = 0; j < sc.ndisks; ++j) {
-            const double* dk = sc.disks + 3 * j;
+            const double* dk = sc.disks + 4 * j;
             if (!(seg_dist_sq(dk[0], dk[1], fx, fy, tx, ty) > dk[2])) return false;
         }
         fx = tx;
@@ -159,10 +159,12 @@ __device__ __forceinline__ bool arm_valid(const SceneArgs& sc, const double* q) 
     return true;
 }
 
-// arm_valid with work counters: FK links evaluated and segment-disk tests
-// evaluated (same early exits); used to price a launch in FLOPs.
+// arm_valid with work counters: FK links evaluated, segment-disk tests
+// evaluated (same early exits), and of those the tests whose disk centre lies
+// within (link length + r + R) of the link's start joint, which no exact
+// per-link distance bound can clear; used to price a launch in FLOPs.
 __device__ __forceinline__ bool arm_valid_counted(const SceneArgs& sc, const double* q, uint64_t* links,
-                                                  uint64_t* tests) {
+                                                  uint64_t* tests, uint64_t* near) {
     double fx = 0.0, fy = 0.0, angle = 0.0;
     for (int i = 0; i < sc.nlinks; ++i) {
         angle += q[i];
@@ -173,8 +175,10 @@ __device__ __forceinline__ bool arm_valid_counted(const SceneArgs& sc, const dou
         double tx = fx + L * c;
         double ty = fy + L * s;
         for (int j = 0; j < sc.ndisks; ++j) {
-            const double* dk = sc.disks + 3 * j;
+            const double* dk = sc.disks + 4 * j;
             *tests += 1;
+            const double dx = dk[0] - fx, dy = dk[1] - fy, reach = L + dk[3];
+            if (dx * dx + dy * dy <= reach * reach) *near += 1;
             if (!(seg_dist_sq(dk[0], dk[1], fx, fy, tx, ty) > dk[2])) return false;
         }
         fx = tx;
@@ -193,12 +197,17 @@ __device__ __forceinline__ bool boxes_valid(const SceneArgs& sc, const double* q
 }
 
 // PNG2dScenario::valid, demo/png_2d_scenario.hpp:104-110, plus a bounds check.
+// The grid is one bit per cell in 8 x 8 tiles (a 4096^2 grid is 2 MiB, so it
+// stays in one XCD's L2), tile-row-major, cell (x, y) at bit 8 * (y % 8) +
+// x % 8 of tile (y / 8) * grid_tx + x / 8: a waypoint's 8 x 8 neighbourhood
+// is one 8-byte word, and 16 tiles of a tile row share a 128-byte line.
 __device__ __forceinline__ bool grid_valid(const SceneArgs& sc, const double* q) {
     double fx = q[0] + 0.5, fy = q[1] + 0.5;
     if (!(fx > -1.0 && fx < (double)sc.width + 1.0 && fy > -1.0 && fy < (double)sc.height + 1.0)) return false;
     int x = (int)fx, y = (int)fy;
     if (x < 0 || y < 0 || x >= sc.width || y >= sc.height) return false;
-    return sc.grid[(int64_t)sc.width * y + x] == 0;
+    const uint64_t w = sc.grid[(int64_t)(y >> 3) * sc.grid_tx + (x >> 3)];
+    return ((w >> (((y & 7) << 3) | (x & 7))) & 1ull) == 0;
 }
 
 // distPointSegmentSquared(c, s0, s1) > (r+R)^2, skipping disks whose centre

@@ -96,8 +96,10 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
     if (p->mesh_edge_waves < 1 || p->mesh_edge_waves > 32)
         return fail(ctx, MPTG_E_RANGE, "mesh waves per edge must be in 1..32");
     if (p->knn_select != 0 && p->knn_select != 1) return fail(ctx, MPTG_E_RANGE, "knn_select is 0 or 1");
-    ctx->tune = *p;
     if (p->knn_grid != 0 && p->knn_grid != 1) return fail(ctx, MPTG_E_RANGE, "knn_grid is 0 or 1");
+    if (p->dmv_edge_lanes < 1 || p->dmv_edge_lanes > 64 || (p->dmv_edge_lanes & (p->dmv_edge_lanes - 1)))
+        return fail(ctx, MPTG_E_RANGE, "lanes per DMV edge must be a power of two in 1..64");
+    ctx->tune = *p;
     return MPTG_OK;
 }
 
@@ -143,6 +145,9 @@ int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, 
         return sweep(ctx, MPTG_TUNE_MESH, {8, 16, 32, 64, 128}, [&](int v) { ctx->tune.mesh_waves_per_cu = v; },
                      call, rep);
     }
+    if ((rc = sweep(ctx, MPTG_TUNE_DMV_LANES, {1, 2, 4, 8, 16, 32, 64},
+                    [&](int v) { ctx->tune.dmv_edge_lanes = v; }, call, nullptr)))
+        return rc;
     return sweep(ctx, MPTG_TUNE_DMV_LONG, {2, 4, 8, 16, 32, 64},
                  [&](int v) { ctx->tune.dmv_long_waves_per_cu = v; }, call, rep);
 }

@@ -8,9 +8,11 @@
 //    ranges that find the ring full are visited sequentially.  For
 //    n = steps-1 <= 255 the ring can never fill (255 disjoint non-empty
 //    ranges plus a popped range of size >= 2 need n >= 256), so the order is
-//    a pure BFS and is read from a 32 KiB table built once per context.
-//    Longer edges go to a second kernel in which one wave walks the ring in
-//    LDS and checks 64 waypoints per round.
+//    a pure BFS and is read from a 32 KiB table built once per context.  A
+//    group of lanes shares each such edge and checks a run of consecutive
+//    positions of that order per round (link_dmv_lanes).  Longer edges go to
+//    a second kernel in which one wave walks the ring in LDS and checks 64
+//    waypoints per round.
 //  * BISECT — LinkManipulatorScenario::link/bisectLink,
 //    demo/link_manipulator_scenario.hpp:118-138: valid(a), valid(b), then the
 //    pre-order of recursive midpoints (a+b)/2 until L-inf(a-b) < tol.  The
@@ -80,8 +82,17 @@ __device__ __forceinline__ int64_t dmv_count(const SpaceArgs& sp, const double* 
     return steps - 1;
 }
 
+// G = 2^lg lanes per edge (1..64; lanes of one edge sit in one wave).  An
+// edge's check sequence is position 0 = valid(to) (counted, and evaluated
+// unless the endpoints are known valid) followed by positions 1..m = the
+// interior waypoints in the BFS table's order.  The group takes positions G
+// at a time; a ballot over the group's lanes gives the first failing
+// position f, and the reference's count is f + 1 (m + 1 when none fails).
+// Every loop is bounded by m <= 255.  Edges with more waypoints go to the
+// long-edge list after valid(to), as the reference checks `to` first
+// (discrete_motion_validator.hpp:75).
 template <int D>
-__global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs sc, double inv,
+__global__ __launch_bounds__(kBlock) void link_dmv_lanes(SpaceArgs sp, SceneArgs sc, double inv, int lg,
                                                          const uint8_t* __restrict__ table,
                                                          const double* __restrict__ from,
                                                          const double* __restrict__ to, int64_t n, int known_valid,
@@ -91,8 +102,14 @@ __global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs
                                                          uint32_t* __restrict__ long_count) {
     constexpr int DA = D > 0 ? D : kMaxDim;
     const int dd = D > 0 ? D : sp.dim;
-    int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n || (mask && !mask[e])) return;
+    const int G = 1 << lg;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t e = t >> lg;
+    const int g = (int)(t & (G - 1));
+    // the group's lanes within the wave's 64-bit ballot
+    const int gbase = (int)(threadIdx.x & 63) & ~(G - 1);
+    const uint64_t gbits = G == 64 ? ~0ull : ((1ull << G) - 1) << gbase;
+    if (e >= n || (mask && !mask[e])) return;  // the whole group leaves together
     double a[DA], b[DA], q[DA];
 #pragma unroll
     for (int j = 0; j < DA; ++j)
@@ -100,35 +117,46 @@ __global__ __launch_bounds__(kBlock) void link_dmv_short(SpaceArgs sp, SceneArgs
             a[j] = from[e * dd + j];
             b[j] = to[e * dd + j];
         }
-    if (!known_valid && !state_valid_t<D>(sc, b)) {
-        verdict[e] = 0;
-        checks[e] = 1;
-        return;
-    }
     double delta = 0.0;
-    int64_t m = dmv_count<D>(sp, a, b, inv, &delta);
-    if (m > kMaxDmvWaypoints) {
-        verdict[e] = 0;
-        checks[e] = MPTG_CHECKS_OVER_LIMIT;
-        return;
-    }
+    const int64_t m = dmv_count<D>(sp, a, b, inv, &delta);
     if (m > kTableMax) {
-        uint32_t slot = atomicAdd(long_count, 1u);
-        long_list[slot] = e;
+        if (g == 0) {
+            if (!known_valid && !state_valid_t<D>(sc, b)) {
+                verdict[e] = 0;
+                checks[e] = 1;
+            } else if (m > kMaxDmvWaypoints) {
+                verdict[e] = 0;
+                checks[e] = MPTG_CHECKS_OVER_LIMIT;
+            } else {
+                long_list[atomicAdd(long_count, 1u)] = e;
+            }
+        }
         return;
     }
-    const uint8_t* ord = table + (m * (m - 1)) / 2;
-    for (int64_t o = 0; o < m; ++o) {
-        int64_t i = ord[o];
-        space_interp_t<D>(sp, a, b, (double)i * delta, q);
-        if (!state_valid_t<D>(sc, q)) {
-            verdict[e] = 0;
-            checks[e] = (uint32_t)(2 + o);
+    const int last = (int)m;
+    const uint8_t* ord = table + (last * (last - 1)) / 2;
+    for (int p0 = 0; p0 <= last; p0 += G) {
+        const int p = p0 + g;
+        bool bad = false;
+        if (p == 0) {
+            bad = !known_valid && !state_valid_t<D>(sc, b);
+        } else if (p <= last) {
+            space_interp_t<D>(sp, a, b, (double)ord[p - 1] * delta, q);
+            bad = !state_valid_t<D>(sc, q);
+        }
+        const uint64_t hit = __ballot(bad) & gbits;
+        if (hit) {
+            if (g == 0) {
+                verdict[e] = 0;
+                checks[e] = (uint32_t)(p0 + (__builtin_ctzll(hit) - gbase) + 1);
+            }
             return;
         }
     }
-    verdict[e] = 1;
-    checks[e] = (uint32_t)(1 + m);
+    if (g == 0) {
+        verdict[e] = 1;
+        checks[e] = (uint32_t)(1 + last);
+    }
 }
 
 // One wave per long edge: lane 0 replays the reference's ring in LDS and
@@ -328,8 +356,8 @@ __global__ __launch_bounds__(kBlock) void link_bisect_list(int D, SceneArgs sc, 
 }
 
 // Instrumented bisection link (not timed): same walk and verdicts, and
-// accumulates [waypoint valid() calls, FK links, segment-disk tests] for the
-// arm scene into work[0..2].
+// accumulates [waypoint valid() calls, FK links, segment-disk tests, tests
+// within the per-link reach bound] for the arm scene into work[0..3].
 __global__ __launch_bounds__(kBlock) void link_bisect_work(int D, SceneArgs sc, double tol,
                                                            const double* __restrict__ from,
                                                            const double* __restrict__ to, int64_t n, int known_valid,
@@ -341,14 +369,14 @@ __global__ __launch_bounds__(kBlock) void link_bisect_work(int D, SceneArgs sc, 
         a[j] = from[e * D + j];
         b[j] = to[e * D + j];
     }
-    uint64_t calls = 0, links = 0, tests = 0;
+    uint64_t calls = 0, links = 0, tests = 0, near = 0;
     bool ok = true;
     if (!known_valid) {
         ++calls;
-        ok = arm_valid_counted(sc, a, &links, &tests);
+        ok = arm_valid_counted(sc, a, &links, &tests, &near);
         if (ok) {
             ++calls;
-            ok = arm_valid_counted(sc, b, &links, &tests);
+            ok = arm_valid_counted(sc, b, &links, &tests, &near);
         }
     }
     uint64_t path = 0;
@@ -358,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void link_bisect_work(int D, SceneArgs sc, 
         if (bisect_internal(D, lo, hi, tol)) {
             for (int j = 0; j < D; ++j) lo[j] = (lo[j] + hi[j]) / 2;
             ++calls;
-            if (!arm_valid_counted(sc, lo, &links, &tests)) break;
+            if (!arm_valid_counted(sc, lo, &links, &tests, &near)) break;
             if (depth >= kMaxBisectDepth) break;
             ++depth;
         } else {
@@ -373,6 +401,7 @@ __global__ __launch_bounds__(kBlock) void link_bisect_work(int D, SceneArgs sc, 
     atomicAdd(&work[0], (unsigned long long)calls);
     atomicAdd(&work[1], (unsigned long long)links);
     atomicAdd(&work[2], (unsigned long long)tests);
+    atomicAdd(&work[3], (unsigned long long)near);
 }
 
 // Host replay of the reference ring for n <= 255 (never overflows): the BFS
@@ -490,10 +519,14 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     if ((rc = launch_zero_u32(ctx, long_count, 1))) return rc;
     double inv = 1.0 / ctx->link_param;
     unsigned lg = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * ctx->tune.dmv_long_waves_per_cu);
+    // lanes per edge, fewer where n edges at the tuned count would pass 2^31 threads
+    int lanes_log2 = __builtin_ctz((unsigned)ctx->tune.dmv_edge_lanes);
+    while (lanes_log2 > 0 && (n << lanes_log2) > ((int64_t)1 << 31)) --lanes_log2;
+    const unsigned sblocks = (unsigned)(((n << lanes_log2) + kBlock - 1) / kBlock);
     dispatch_dim(ctx->dim, [&]<int D>() {
-        hipLaunchKernelGGL((link_dmv_short<D>), dim3(blocks), dim3(kBlock), 0, ctx->stream, space_args(ctx),
-                           scene_args(ctx), inv, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n, known_valid, d_mask,
-                           d_verdict, d_checks, long_list, long_count);
+        hipLaunchKernelGGL((link_dmv_lanes<D>), dim3(sblocks), dim3(kBlock), 0, ctx->stream, space_args(ctx),
+                           scene_args(ctx), inv, lanes_log2, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n,
+                           known_valid, d_mask, d_verdict, d_checks, long_list, long_count);
         hipLaunchKernelGGL((link_dmv_long<D>), dim3(lg), dim3(64), 0, ctx->stream, space_args(ctx), scene_args(ctx),
                            inv, d_from, d_to, long_list, long_count, d_verdict, d_checks);
     });
@@ -502,8 +535,9 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
 }
 
 // d_work[0..4]: the wave kernel's executed work (instrumented twin of the
-// timed kernel); d_work[5..7]: the reference algorithm's work on the same
-// edges (reference order, every disk tested until the first hit).
+// timed kernel); d_work[5..8]: the reference algorithm's work on the same
+// edges (reference order, every disk tested until the first hit, and the
+// tests within the per-link reach bound).
 int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order, int known_valid,
                      unsigned long long* d_work, uint8_t* d_verdict, uint32_t* d_checks) {
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->link_mode != MPTG_LINK_BISECT)
@@ -512,7 +546,7 @@ int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, in
     if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
     uint32_t* nirr = (uint32_t*)ctx->scratch[7].p;
     int64_t* irr = (int64_t*)((char*)ctx->scratch[7].p + 256);
-    MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    MPTG_HIP(ctx, hipMemsetAsync(d_work, 0, 9 * sizeof(unsigned long long), ctx->stream));
     rc = launch_arm_link(ctx, d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks, irr, nirr, d_work, order);
     if (rc) return rc == MPTG_E_STATE ? fail(ctx, rc, "no wave kernel for this arm") : rc;
     hipLaunchKernelGGL(link_bisect_work, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,

@@ -75,6 +75,10 @@ _SIG = [
     ("orc_link_waypoints", C.c_int64, [C.POINTER(OrcScene), _dp, _dp, _dp, C.c_int64]),
     ("orc_link", C.c_int32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32, _u32p]),
     ("orc_link_batch", None, [C.POINTER(OrcScene), _dp, _dp, C.c_int64, C.c_int32, _u8p, _u32p, C.c_int32]),
+    ("orc_arm_link_work", C.c_int32, [C.POINTER(OrcScene), _dp, _dp, C.c_int64, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_uint64)]),
+    ("orc_pool_run_tree", C.c_int32, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, _i32p, C.c_int64,
+                                      C.c_int64, C.c_double, C.c_int32, C.POINTER(OrcPoolResult)]),
     ("orc_checks_in_score_order", C.c_uint32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32,
                                                C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32]),
     ("orc_prrt_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_int64, _dp,
@@ -211,6 +215,19 @@ class Oracle:
                               _p(v, C.c_uint8), _p(c, C.c_uint32), threads)
         return v.astype(bool), c
 
+    def arm_link_work(self, a, b, known_valid=False, threads=8) -> dict:
+        """Work of link() over arm edges in the reference order: valid() calls
+        evaluated, forward-kinematics links, segment-disk tests (every disk
+        until the first hit) and those whose disk lies within link length +
+        r + R of the link's start joint."""
+        a = _f64(a).reshape(-1, self.dim)
+        b = _f64(b).reshape(-1, self.dim)
+        out = (C.c_uint64 * 4)()
+        if self.L.orc_arm_link_work(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double), a.shape[0],
+                                    1 if known_valid else 0, threads, out) != 0:
+            raise ValueError("arm link work needs an arm scene with bisection links")
+        return {"calls": int(out[0]), "fk": int(out[1]), "tests": int(out[2]), "near_tests": int(out[3])}
+
     def checks_in_score_order(self, a, b, scores, known_valid=False, lead=0, top_levels=None) -> int:
         """valid() calls of link(a, b) with reference ordinals 0..lead-1 first,
         then the other waypoints of bisection depth < top_levels by descending
@@ -269,11 +286,23 @@ class Oracle:
         return (nodes[:n].copy(), parents[:n].copy(), costs[:n].copy(), goals[: min(stats["goals"], 4096)].copy(),
                 stats, sstats)
 
-    def pool(self, seed, target, seconds, threads, range_=None, goal_bias=None):
+    def pool(self, seed, target, seconds, threads, range_=None, goal_bias=None, tree=None):
+        """The std::thread worker pool (reference loop restated) from the
+        start state, or from `tree` = (nodes, parents), until `target` nodes
+        or `seconds`."""
         p = self._params(seed, 1, range_, goal_bias)
         r = OrcPoolResult()
-        start = _f64(self.scene.start)
-        self.L.orc_pool_run(C.byref(self.s), C.byref(p), _p(start, C.c_double), target, seconds, threads, C.byref(r))
+        if tree is None:
+            start = _f64(self.scene.start)
+            self.L.orc_pool_run(C.byref(self.s), C.byref(p), _p(start, C.c_double), target, seconds, threads,
+                                C.byref(r))
+        else:
+            nodes = _f64(tree[0]).reshape(-1, self.dim)
+            par = np.ascontiguousarray(tree[1], dtype=np.int32)
+            if self.L.orc_pool_run_tree(C.byref(self.s), C.byref(p), _p(nodes, C.c_double), _p(par, C.c_int32),
+                                        nodes.shape[0], max(target, nodes.shape[0]), seconds, threads,
+                                        C.byref(r)) != 0:
+                raise ValueError("bad initial tree")
         return {f: getattr(r, f) for f, _ in r._fields_}
 
 

@@ -92,6 +92,15 @@ int32_t orc_link(const orc_scene* sc, const double* a, const double* b, int32_t 
 void orc_link_batch(const orc_scene* sc, const double* from, const double* to, int64_t n,
                     int32_t endpoints_known_valid, uint8_t* verdict, uint32_t* checks_done,
                     int32_t nthreads);
+/* Work of link() over n arm edges in the reference order (bisectLink,
+ * demo/link_manipulator_scenario.hpp:99-138): out[0] valid() calls evaluated
+ * (endpoint checks of known-valid edges are not evaluated), out[1] links
+ * whose forward kinematics ran, out[2] segment-disk tests (every disk until
+ * the first hit), out[3] those of out[2] whose disk centre lies within
+ * (link length + r + R) of the link's start joint.  -1 unless an arm scene
+ * with bisection links. */
+int32_t orc_arm_link_work(const orc_scene* sc, const double* from, const double* to, int64_t n,
+                          int32_t known_valid, int32_t nthreads, uint64_t* out);
 /* Number of valid() calls when the link's waypoints are checked in this
  * order, after the same endpoint checks: reference ordinals 0 .. lead-1 first,
  * then the other waypoints of bisection depth < top_levels by descending
@@ -171,6 +180,11 @@ typedef struct {
 } orc_pool_result;
 int32_t orc_pool_run(const orc_scene* sc, const orc_prrt_params* p, const double* start, int64_t target,
                      double max_seconds, int32_t nthreads, orc_pool_result* out);
+/* The same from a tree of n0 nodes (parents[0] = -1, parents[i] < i; NULL:
+ * every node's parent is node 0), stopping at `target` nodes or max_seconds. */
+int32_t orc_pool_run_tree(const orc_scene* sc, const orc_prrt_params* p, const double* nodes,
+                          const int32_t* parents, int64_t n0, int64_t target, double max_seconds,
+                          int32_t nthreads, orc_pool_result* out);
 
 #ifdef __cplusplus
 }

@@ -153,6 +153,37 @@ bool valid_arm(const orc_scene* sc, const double* q) {
     return true;
 }
 
+// valid_arm with its work counted, in the reference's loop order: links whose
+// forward kinematics ran, segment-disk tests made (every disk until the first
+// hit), and of those the tests whose disk centre lies within (link length +
+// r + R) of the link's start joint -- the pairs that no exact per-link
+// distance bound can clear, since every point of the link lies within its
+// length of that joint.
+bool valid_arm_work(const orc_scene* sc, const double* q, uint64_t* fk, uint64_t* tests, uint64_t* near) {
+    double fx = 0.0, fy = 0.0, angle = 0.0;
+    double R = sc->robot_radius;
+    for (int i = 0; i < sc->nlinks; ++i) {
+        angle += q[i];
+        double s, c;
+        mptg::pm::sincos(angle, &s, &c);
+        double L = sc->lengths[i];
+        double tx = fx + L * c;
+        double ty = fy + L * s;
+        ++*fk;
+        for (int j = 0; j < sc->ndisks; ++j) {
+            const double* dk = sc->disks + 3 * j;
+            double rr = dk[2] + R;
+            ++*tests;
+            double dx = dk[0] - fx, dy = dk[1] - fy, reach = L + rr;
+            if (dx * dx + dy * dy <= reach * reach) ++*near;
+            if (!(dist_point_segment_sq(dk[0], dk[1], fx, fy, tx, ty) > rr * rr)) return false;
+        }
+        fx = tx;
+        fy = ty;
+    }
+    return true;
+}
+
 bool valid_boxes(const orc_scene* sc, const double* q) {
     for (int i = 0; i < sc->nboxes; ++i) {
         const double* b = sc->boxes + 4 * i;
@@ -396,6 +427,29 @@ void orc_link_batch(const orc_scene* sc, const double* from, const double* to, i
         verdict[i] = r.ok ? 1 : 0;
         checks_done[i] = r.checks;
     });
+}
+
+int32_t orc_arm_link_work(const orc_scene* sc, const double* from, const double* to, int64_t n,
+                          int32_t known_valid, int32_t nthreads, uint64_t* out) {
+    if (sc->kind != ORC_SCENE_ARM || sc->link_mode != ORC_LINK_BISECT) return -1;
+    const int d = sc->dim;
+    std::vector<std::array<uint64_t, 4>> w((size_t)n);
+    parallel_for(n, nthreads, [&](int64_t e) {
+        std::array<uint64_t, 4> c{};
+        auto check = [&](const double* q) {
+            ++c[0];
+            return valid_arm_work(sc, q, &c[1], &c[2], &c[3]);
+        };
+        const double* a = from + e * d;
+        const double* b = to + e * d;
+        bool ok = known_valid || (check(a) && check(b));
+        if (ok) bisect_visit(sc, a, b, check);
+        w[(size_t)e] = c;
+    });
+    for (int i = 0; i < 4; ++i) out[i] = 0;
+    for (const auto& c : w)
+        for (int i = 0; i < 4; ++i) out[i] += c[i];
+    return 0;
 }
 
 uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const double* b,

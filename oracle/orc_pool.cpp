@@ -170,12 +170,19 @@ struct SharedTree {
 
 }  // namespace
 
-extern "C" int32_t orc_pool_run(const orc_scene* sc, const orc_prrt_params* p, const double* start, int64_t target,
-                                double max_seconds, int32_t nthreads, orc_pool_result* out) {
+// Runs the pool from a tree of n0 nodes (parents[i] < i, parents[0] = -1)
+// until the tree holds `target` nodes or max_seconds have passed.  The
+// nearest-neighbour index over the given nodes is built before the clock
+// starts.
+extern "C" int32_t orc_pool_run_tree(const orc_scene* sc, const orc_prrt_params* p, const double* nodes,
+                                     const int32_t* parents, int64_t n0, int64_t target, double max_seconds,
+                                     int32_t nthreads, orc_pool_result* out) {
     if (nthreads < 1) nthreads = (int32_t)std::max(1u, std::thread::hardware_concurrency());
+    if (n0 < 1 || target < n0) return -1;
     const int d = sc->dim;
     SharedTree tree(sc, target + (int64_t)nthreads + 16);
-    tree.append(start, -1);
+    for (int64_t i = 0; i < n0; ++i) tree.append(nodes + i * d, parents ? parents[i] : (i ? 0 : -1));
+    tree.maybe_rebuild(n0);
     std::atomic<bool> done{false};
     std::atomic<int64_t> samples{0}, links{0}, valids{0}, goals{0};
     auto t0 = std::chrono::steady_clock::now();
@@ -240,4 +247,9 @@ extern "C" int32_t orc_pool_run(const orc_scene* sc, const orc_prrt_params* p, c
         out->goals = goals.load();
     }
     return 0;
+}
+
+extern "C" int32_t orc_pool_run(const orc_scene* sc, const orc_prrt_params* p, const double* start, int64_t target,
+                                double max_seconds, int32_t nthreads, orc_pool_result* out) {
+    return orc_pool_run_tree(sc, p, start, nullptr, 1, std::max<int64_t>(target, 1), max_seconds, nthreads, out);
 }

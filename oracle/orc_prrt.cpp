@@ -8,10 +8,11 @@
 // with batch-synchronous semantics: in iteration `it` the B samples with
 // global indices it*B .. it*B+B-1 are each extended against the tree as it
 // stood at the start of the iteration; accepted nodes are appended in sample
-// order.  Goal biasing is on while no goal has been found at the start of the
-// iteration (the reference's worker 0 re-checks goalCount_ each iteration,
-// prrt.hpp:379) and uses the unscaled bias (prrt.hpp:381; the scaled value at
-// :373 is never used).  With B = 1 this is exactly the reference's
+// order.  Only sample slot 0 of an iteration takes the goal-bias coin, as
+// only the reference's worker 0 biases (prrt.hpp:365-388); it does so while no
+// goal has been found at the start of the iteration (worker 0 re-checks
+// goalCount_ each iteration, prrt.hpp:379) and with the unscaled bias
+// (prrt.hpp:381; the scaled value at :373 is never used).  With B = 1 this is exactly the reference's
 // single-worker sequential loop, with the project's Philox stream in place of
 // mt19937_64:
 //   coin   = uniform(seed, sample, BIAS, 0) < goalBias  -> sample = goal
@@ -71,7 +72,9 @@ extern "C" int64_t orc_prrt_run(const orc_scene* sc, const orc_prrt_params* p, c
             e = Ext();
             uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
             double q[64];
-            bool useGoal = bias && mptg::uniform_at(p->seed, sidx, mptg::RNG_PURPOSE_BIAS, 0) < p->goal_bias;
+            // slot 0 of the iteration plays the reference's worker 0, the only
+            // worker that biases (prrt.hpp:365-388)
+            bool useGoal = bias && b == 0 && mptg::uniform_at(p->seed, sidx, mptg::RNG_PURPOSE_BIAS, 0) < p->goal_bias;
             if (useGoal) {
                 std::memcpy(q, p->goal, sizeof(double) * d);
             } else {

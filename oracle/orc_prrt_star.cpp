@@ -128,7 +128,8 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
             e = StarExt();
             uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
             double q[64];
-            bool useGoal = bias && mptg::uniform_at(p->seed, sidx, mptg::RNG_PURPOSE_BIAS, 0) < p->goal_bias;
+            // slot 0 plays worker 0, the only biasing worker (prrt_star.hpp:457-487)
+            bool useGoal = bias && b == 0 && mptg::uniform_at(p->seed, sidx, mptg::RNG_PURPOSE_BIAS, 0) < p->goal_bias;
             if (useGoal) {
                 std::memcpy(q, p->goal, sizeof(double) * d);
             } else {

@@ -123,8 +123,9 @@ def test_graph_mesh(ctx_factory):
 
 
 def test_graph_matches_eager_at_large_batch(ctx_factory):
-    """B = 8192 on the arm, beyond what the oracle runs quickly: the graph
-    run equals the step-by-step run node for node."""
+    """B = 8192 on the arm, graphs of 5 iterations (the last launch stops
+    inside its span): the graph run equals the step-by-step run and the
+    oracle node for node."""
     _say("graph arm 8192")
     sc = scenes.arm_scene()
     ctx = ctx_factory()
@@ -133,6 +134,10 @@ def test_graph_matches_eager_at_large_batch(ctx_factory):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
     for key in STAT_KEYS:
         assert a[3][key] == b[3][key], key
+    on, op, og, ost = oracle.Oracle(sc).prrt(21, 8192, 60000, max_iters=12)
+    assert np.array_equal(a[0], on) and np.array_equal(a[1], op) and np.array_equal(a[2], og)
+    for key in STAT_KEYS:
+        assert a[3][key] == ost[key], key
     assert a[3]["graph_launches"] == 0 and b[3]["graph_launches"] == -(-(b[3]["iterations"] - 1) // 5)
 
 

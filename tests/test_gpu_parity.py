@@ -200,6 +200,27 @@ def test_link_parity_arm(ctx_factory):
         assert np.array_equal(gv, ov) and np.array_equal(gc, oc)
 
 
+def test_arm_link_work_counts_equal_oracle(ctx_factory):
+    """The device's count of the reference algorithm's work (the FLOP price
+    of the arm link roofline) equals the oracle's walk of the reference loop."""
+    _say("arm link work")
+    ctx = ctx_factory()
+    sc = scenes.arm_scene()
+    ctx.scene(sc)
+    rng = np.random.default_rng(36)
+    a = rng.uniform(-math.pi, math.pi, (3000, 20))
+    b = np.clip(a + rng.normal(0, 0.4, (3000, 20)), -math.pi, math.pi)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    o = oracle.Oracle(sc)
+    for kv in (False, True):
+        w = ctx.link_work(da.ptr, db.ptr, a.shape[0], known_valid=kv)
+        ow = o.arm_link_work(a, b, known_valid=kv, threads=16)
+        assert (w["ref_calls"], w["ref_fk"], w["ref_tests"], w["ref_near_tests"]) == \
+            (ow["calls"], ow["fk"], ow["tests"], ow["near_tests"]), kv
+    da.free()
+    db.free()
+
+
 # ---- whole trees -------------------------------------------------------------
 def _planner_parity(ctx, sc, seed, batch, target, max_iters=1 << 30):
     ctx.scene(sc)

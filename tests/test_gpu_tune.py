@@ -161,6 +161,43 @@ def test_dmv_long_grid_every_shape_is_exact(ctx_factory):
     assert np.array_equal(gv, ov) and np.array_equal(gc, oc)
 
 
+@pytest.mark.parametrize("kind", ["grid", "boxes"])
+def test_dmv_every_lane_count_is_exact(ctx_factory, kind):
+    """Every lanes-per-edge split of the DMV link kernel (1..64 lanes per
+    edge) gives the oracle's verdicts and check counts: zero-length and
+    one-step edges, invalid ends, edges out of the grid, short and long
+    (ring-replay) edges; then the tuner's choice."""
+    _say("tune dmv lanes", kind)
+    ctx = ctx_factory()
+    rng = np.random.default_rng(73)
+    if kind == "grid":
+        sc = scenes.grid_scene(size=1024, seed=3)
+        a, b = scenes.grid_edges(sc, 40000, seed=74, mean_len=40.0)
+        b[:50] = a[:50]                                   # zero length
+        b[50:100] = a[50:100] + rng.uniform(-1, 1, (50, 2))  # at most two steps
+        a[100:150] = rng.uniform(-20, 1044, (50, 2))     # ends outside the grid
+        b[150:200] = rng.uniform(0, 1024, (50, 2))       # long edges (ring replay)
+    else:
+        sc = scenes.boxes_scene()
+        a = rng.uniform(0, 100, (40000, 2))
+        b = a + rng.normal(0, 4, (40000, 2))
+        b[:100] = a[:100]
+    ctx.scene(sc)
+    o = oracle.Oracle(sc)
+    for kv in (False, True):
+        ov, oc = o.link(a, b, known_valid=kv, threads=16)
+        for lanes in (1, 2, 4, 8, 16, 32, 64):
+            ctx.tune_set(dmv_edge_lanes=lanes)
+            gv, gc = ctx.link(a, b, endpoints_known_valid=kv)
+            assert np.array_equal(gv, ov) and np.array_equal(gc, oc), (kv, lanes)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    ctx.autotune_link(da.ptr, db.ptr, a.shape[0])
+    assert ctx.tune_get()["dmv_edge_lanes"] in (1, 2, 4, 8, 16, 32, 64)
+    gv, gc = ctx.link(a, b)
+    ov, oc = o.link(a, b, threads=16)
+    assert np.array_equal(gv, ov) and np.array_equal(gc, oc)
+
+
 def test_mesh_grid_every_shape_is_exact(ctx_factory):
     _say("tune mesh")
     ctx = ctx_factory()
@@ -322,3 +359,7 @@ def test_tune_set_rejects_out_of_range(ctx_factory):
         ctx.tune_set(mesh_edge_waves=33)
     with pytest.raises(Exception):
         ctx.tune_set(knn_select=2)
+    with pytest.raises(Exception):
+        ctx.tune_set(dmv_edge_lanes=3)
+    with pytest.raises(Exception):
+        ctx.tune_set(dmv_edge_lanes=128)

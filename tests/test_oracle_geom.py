@@ -260,6 +260,58 @@ def test_arm_bisect_link_equals_waypoint_conjunction():
         _check_link_against_waypoints(o, a, b, 2)
 
 
+def _arm_work_np(sc, q):
+    """FK links, segment-disk tests until the first hit and the tests whose
+    disk lies within link length + r + R of the link's start, for one state
+    (numpy restatement of LinkManipulatorScenario::valid's loop,
+    demo/link_manipulator_scenario.hpp:99-116)."""
+    fk = tests = near = 0
+    fx = fy = ang = 0.0
+    rr = sc.disks[:, 2] + sc.robot_radius
+    for i in range(sc.dim):
+        ang += q[i]
+        tx, ty = fx + sc.lengths[i] * math.cos(ang), fy + sc.lengths[i] * math.sin(ang)
+        fk += 1
+        vx, vy = tx - fx, ty - fy
+        wx, wy = sc.disks[:, 0] - fx, sc.disks[:, 1] - fy
+        c1 = vx * wx + vy * wy
+        c2 = vx * vx + vy * vy
+        t = np.clip(c1 / c2, 0, 1)
+        d2 = (wx - vx * t) ** 2 + (wy - vy * t) ** 2
+        hit = np.flatnonzero(d2 <= rr * rr)
+        upto = len(rr) if len(hit) == 0 else hit[0] + 1
+        tests += upto
+        near += int((wx[:upto] ** 2 + wy[:upto] ** 2 <= (sc.lengths[i] + rr[:upto]) ** 2).sum())
+        if len(hit):
+            return fk, tests, near, False
+        fx, fy = tx, ty
+    return fk, tests, near, True
+
+
+def test_arm_link_work_counts_reference_loop():
+    """The oracle's work count of link() equals a numpy walk over the
+    reference-order waypoints: every waypoint up to the first invalid one,
+    each counted link by link and disk by disk up to its first hit."""
+    sc = scenes.arm_scene()
+    o = oracle.Oracle(sc)
+    rng = np.random.default_rng(19)
+    a = rng.uniform(-math.pi, math.pi, (40, 20))
+    b = np.clip(a + rng.normal(0, 0.2, (40, 20)), -math.pi, math.pi)
+    tot = np.zeros(4, dtype=np.int64)
+    for i in range(40):
+        for q in o.waypoints(a[i], b[i]):
+            fk, tests, near, ok = _arm_work_np(sc, q)
+            tot += (1, fk, tests, near)
+            if not ok:
+                break
+    w = o.arm_link_work(a, b, known_valid=True)
+    assert (w["calls"], w["fk"], w["tests"], w["near_tests"]) == tuple(int(x) for x in tot)
+    assert 0 < w["near_tests"] < w["tests"]
+    # endpoint checks are evaluated unless known valid
+    v, c = o.link(a, b)
+    assert o.arm_link_work(a, b)["calls"] == int(c.sum())
+
+
 def test_arm_bisect_waypoint_set_is_dyadic():
     sc = scenes.arm_scene()
     o = oracle.Oracle(sc)

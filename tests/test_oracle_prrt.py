@@ -1,5 +1,7 @@
-"""Batch-synchronous PRRT oracle: structural properties of the grown tree
-(mirroring test/planner_integration_test.hpp:219-291) and the CPU worker pool."""
+"""Batch-synchronous PRRT oracle: a literal restatement of the reference's
+worker loop (B = 1, and the batch semantics at small B), structural
+properties of the grown tree (mirroring
+test/planner_integration_test.hpp:219-291) and the CPU worker pool."""
 import math
 
 import numpy as np
@@ -7,6 +9,78 @@ import pytest
 
 import oracle
 from mpt_amd import scenes
+
+
+BIAS = 0x42494153
+SAMPLE = 0x53414D50
+
+
+def literal_prrt(o, sc, seed, batch, target, goal_bias):
+    """Worker::solve + Worker::addSample (src/mpt/impl/prrt/prrt.hpp:358-452)
+    restated line by line on the separately pinned primitives (brute-force
+    kNN, distance, interpolate, valid, link).  The `batch` samples of one
+    iteration all see the tree as it stood at the iteration's start and are
+    appended in sample order; slot 0 plays worker 0, the only worker that
+    takes the goal-bias coin, and only while no goal is known (:365-388).
+    batch = 1 is the reference's single-worker loop."""
+    nodes = [np.array(sc.start, dtype=np.float64)]
+    parents = [-1]
+    goals = []
+    it = 0
+    while len(nodes) < target:
+        pts = np.array(nodes)
+        biased = len(goals) == 0 and goal_bias > 0                     # :379
+        new_nodes = []
+        for b in range(batch):
+            sidx = it * batch + b
+            if b == 0 and biased and oracle.uniform_at(seed, sidx, BIAS, 0) < goal_bias:  # :381
+                q = np.array(sc.goal, dtype=np.float64)
+            else:
+                q = np.array([oracle.uniform_at(seed, sidx, SAMPLE, i) * (sc.hi[i] - sc.lo[i]) + sc.lo[i]
+                              for i in range(sc.dim)])
+            idx, dist = o.knn(pts, q, 1, threads=1)                    # :416
+            near, d = int(idx[0, 0]), float(dist[0, 0])
+            if d == 0:                                                 # :427
+                continue
+            new = q
+            if d > sc.range:                                           # :430-434
+                new = o.interpolate(pts[near], q, sc.range / d)
+            if not o.valid(new)[0]:                                    # :439
+                continue
+            if not o.link(pts[near], new, known_valid=True)[0][0]:     # :441, 454-457
+                continue
+            new_nodes.append((new, near, o.distance(new, sc.goal) <= sc.goal_radius))  # :442-450
+        for new, near, is_goal in new_nodes:
+            if is_goal:
+                goals.append(len(nodes))
+            nodes.append(new)
+            parents.append(near)
+        it += 1
+    return np.array(nodes), np.array(parents, dtype=np.int32), np.array(goals, dtype=np.int32)
+
+
+@pytest.mark.parametrize("batch,bias", [(1, 0.01), (1, 0.3), (3, 0.3), (16, 0.5)])
+def test_oracle_equals_literal_worker_loop(batch, bias):
+    sc = scenes.boxes_scene()
+    o = oracle.Oracle(sc)
+    nodes, parents, goals, st = o.prrt(seed=21, batch=batch, target=250, goal_bias=bias)
+    ln, lp, lg = literal_prrt(o, sc, 21, batch, 250, bias)
+    assert np.array_equal(nodes, ln)
+    assert np.array_equal(parents, lp)
+    assert np.array_equal(goals, lg)
+    assert st["link_ok"] == len(nodes) - 1
+
+
+def test_goal_bias_takes_only_slot_zero():
+    """A large batch must not append copies of the goal sample: only slot 0
+    of an iteration draws the goal, as only worker 0 does in the reference."""
+    sc = scenes.boxes_scene()
+    o = oracle.Oracle(sc)
+    nodes, parents, goals, st = o.prrt(seed=11, batch=8192, target=16385, goal_bias=0.01)
+    assert len(np.unique(nodes, axis=0)) == len(nodes)
+    # with the bias at 1, every iteration's slot 0 draws the goal and the rest sample
+    nodes, parents, goals, st = o.prrt(seed=11, batch=256, target=2000, goal_bias=1.0)
+    assert len(np.unique(nodes, axis=0)) == len(nodes)
 
 
 def _check_tree(o, sc, nodes, parents, rng_limit):
