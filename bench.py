@@ -102,6 +102,9 @@ def run_steps(ctx, sc, tree, args, order, batch):
         secs += dt
         for k, v in st.items():
             tot[k] = tot.get(k, 0) + v
+    # leave the context holding the given tree (callers search it next)
+    ctx.tree_clear()
+    ctx.tree_insert(nodes0, par0)
     return {
         "ms_per_step": secs / args.steps * 1e3,
         "edge_checks_per_s": tot["link_calls"] / secs,

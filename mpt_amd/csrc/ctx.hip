@@ -198,6 +198,7 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.mesh_tris.p);
     (void)hipFree(ctx->scene.mesh_robot.p);
     for (auto& b : ctx->scratch) (void)hipFree(b.p);
+    (void)hipFree(ctx->dmv_table16.p);
     for (auto& b : ctx->knn_sel) (void)hipFree(b.p);
     for (auto& b : ctx->knn_grid) (void)hipFree(b.p);
     prrt_free(ctx);

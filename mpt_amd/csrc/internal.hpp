@@ -112,6 +112,7 @@ struct mptg_ctx {
     // sorted rows and ids, overflow list, gathered overflow queries +
     // results, query cells and order
     mptg::DevBuf knn_grid[6];
+    mptg::DevBuf dmv_table16;  // the DMV ring order for 256..1023 interior waypoints (uint16 rows)
     mptg::DevBuf scratch[12];  // 0-2 host staging, 3 kNN seed bounds, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
                                // 9 ranked ordinals, 10 continuation list
     // pinned host staging

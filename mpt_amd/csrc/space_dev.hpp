@@ -201,14 +201,20 @@ __device__ __forceinline__ bool boxes_valid(const SceneArgs& sc, const double* q
 // stays in one XCD's L2), tile-row-major, cell (x, y) at bit 8 * (y % 8) +
 // x % 8 of tile (y / 8) * grid_tx + x / 8: a waypoint's 8 x 8 neighbourhood
 // is one 8-byte word, and 16 tiles of a tile row share a 128-byte line.
-__device__ __forceinline__ bool grid_valid(const SceneArgs& sc, const double* q) {
-    double fx = q[0] + 0.5, fy = q[1] + 0.5;
-    if (!(fx > -1.0 && fx < (double)sc.width + 1.0 && fy > -1.0 && fy < (double)sc.height + 1.0)) return false;
-    int x = (int)fx, y = (int)fy;
-    if (x < 0 || y < 0 || x >= sc.width || y >= sc.height) return false;
+// The cell is x = (int)(q0 + 0.5) (truncation), and x lies in [0, width)
+// exactly when q0 + 0.5 lies in (-1, width); likewise y.  Written without
+// branches: every lane loads a cell word (cell (0, 0) for points outside the
+// grid) and the bounds decide afterwards, so unrolled callers keep their
+// loads in flight together.
+__device__ __forceinline__ bool grid_free(const SceneArgs& sc, double qx, double qy) {
+    const double fx = qx + 0.5, fy = qy + 0.5;
+    const bool in = (fx > -1.0) & (fx < (double)sc.width) & (fy > -1.0) & (fy < (double)sc.height);
+    const int x = (int)(in ? fx : 0.0), y = (int)(in ? fy : 0.0);
     const uint64_t w = sc.grid[(int64_t)(y >> 3) * sc.grid_tx + (x >> 3)];
-    return ((w >> (((y & 7) << 3) | (x & 7))) & 1ull) == 0;
+    return in & (((w >> (((y & 7) << 3) | (x & 7))) & 1ull) == 0);
 }
+
+__device__ __forceinline__ bool grid_valid(const SceneArgs& sc, const double* q) { return grid_free(sc, q[0], q[1]); }
 
 // distPointSegmentSquared(c, s0, s1) > (r+R)^2, skipping disks whose centre
 // is farther than lim + r + R from the segment start.  lim bounds the FP

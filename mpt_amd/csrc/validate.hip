@@ -35,6 +35,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTableMax = 255;  // largest n = steps-1 served by the BFS table
+// The 16-bit table holds the ring's order for 256 <= n <= kTable16Max, row n
+// at n(n-1)/2 - 255*256/2 (1 MiB).
+constexpr int kTable16Max = 1023;
+__host__ __device__ constexpr int64_t table16_row(int64_t m) {
+    return m * (m - 1) / 2 - (int64_t)kTableMax * (kTableMax + 1) / 2;
+}
 // Work limits of one link() call; an edge beyond them is not checked and
 // reports verdict 0 with checks = MPTG_CHECKS_OVER_LIMIT.
 constexpr int64_t kMaxDmvWaypoints = (int64_t)1 << 24;
@@ -82,16 +88,35 @@ __device__ __forceinline__ int64_t dmv_count(const SpaceArgs& sp, const double* 
     return steps - 1;
 }
 
+// valid() with the scene kind fixed at compile time (K = a scene kind, or 0
+// for any kind, chosen at run time): the DMV kernels inline only the scene
+// they check, which keeps their code and register count small.
+template <int D, int K>
+__device__ __forceinline__ bool valid_k(const SceneArgs& sc, const double* q) {
+    if constexpr (K == MPTG_SCENE_GRID) return grid_valid(sc, q);
+    else if constexpr (K == MPTG_SCENE_BOXES) return boxes_valid(sc, q);
+    else return state_valid_t<D>(sc, q);
+}
+
+// Positions a lane checks per round: independent (table, cell) load chains in
+// flight together, which the per-round latency hides.  Only for states of
+// up to 3 coordinates, whose waypoints stay in registers.
+template <int D>
+constexpr int dmv_unroll() {
+    return D > 0 && D <= 3 ? 4 : 1;
+}
+
 // G = 2^lg lanes per edge (1..64; lanes of one edge sit in one wave).  An
 // edge's check sequence is position 0 = valid(to) (counted, and evaluated
 // unless the endpoints are known valid) followed by positions 1..m = the
-// interior waypoints in the BFS table's order.  The group takes positions G
-// at a time; a ballot over the group's lanes gives the first failing
-// position f, and the reference's count is f + 1 (m + 1 when none fails).
-// Every loop is bounded by m <= 255.  Edges with more waypoints go to the
-// long-edge list after valid(to), as the reference checks `to` first
-// (discrete_motion_validator.hpp:75).
-template <int D>
+// interior waypoints in the BFS table's order.  Round r of the group covers
+// positions [r G U, (r + 1) G U): lane g takes r G U + u G + g for u < U.  A
+// ballot per u over the group's lanes gives the first failing position f
+// (blocks of smaller u hold smaller positions), and the reference's count is
+// f + 1 (m + 1 when none fails).  Every loop is bounded by m <= 255.  Edges
+// with more waypoints go to the long-edge list after valid(to), as the
+// reference checks `to` first (discrete_motion_validator.hpp:75).
+template <int D, int K>
 __global__ __launch_bounds__(kBlock) void link_dmv_lanes(SpaceArgs sp, SceneArgs sc, double inv, int lg,
                                                          const uint8_t* __restrict__ table,
                                                          const double* __restrict__ from,
@@ -101,6 +126,7 @@ __global__ __launch_bounds__(kBlock) void link_dmv_lanes(SpaceArgs sp, SceneArgs
                                                          int64_t* __restrict__ long_list,
                                                          uint32_t* __restrict__ long_count) {
     constexpr int DA = D > 0 ? D : kMaxDim;
+    constexpr int U = dmv_unroll<D>();
     const int dd = D > 0 ? D : sp.dim;
     const int G = 1 << lg;
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -110,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void link_dmv_lanes(SpaceArgs sp, SceneArgs
     const int gbase = (int)(threadIdx.x & 63) & ~(G - 1);
     const uint64_t gbits = G == 64 ? ~0ull : ((1ull << G) - 1) << gbase;
     if (e >= n || (mask && !mask[e])) return;  // the whole group leaves together
-    double a[DA], b[DA], q[DA];
+    double a[DA], b[DA];
 #pragma unroll
     for (int j = 0; j < DA; ++j)
         if (j < dd) {
@@ -121,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void link_dmv_lanes(SpaceArgs sp, SceneArgs
     const int64_t m = dmv_count<D>(sp, a, b, inv, &delta);
     if (m > kTableMax) {
         if (g == 0) {
-            if (!known_valid && !state_valid_t<D>(sc, b)) {
+            if (!known_valid && !valid_k<D, K>(sc, b)) {
                 verdict[e] = 0;
                 checks[e] = 1;
             } else if (m > kMaxDmvWaypoints) {
@@ -135,20 +161,47 @@ __global__ __launch_bounds__(kBlock) void link_dmv_lanes(SpaceArgs sp, SceneArgs
     }
     const int last = (int)m;
     const uint8_t* ord = table + (last * (last - 1)) / 2;
-    for (int p0 = 0; p0 <= last; p0 += G) {
-        const int p = p0 + g;
-        bool bad = false;
-        if (p == 0) {
-            bad = !known_valid && !state_valid_t<D>(sc, b);
-        } else if (p <= last) {
-            space_interp_t<D>(sp, a, b, (double)ord[p - 1] * delta, q);
-            bad = !state_valid_t<D>(sc, q);
+    // 2-D grids: every position evaluated without branches (the waypoint
+    // (b - a) * t + a of space_interp_t, position 0 = valid(to) decided up
+    // front), so a round's U table loads and U cell loads are in flight
+    // together
+    [[maybe_unused]] bool to_bad = false;
+    if constexpr (K == MPTG_SCENE_GRID) to_bad = !known_valid && !grid_free(sc, b[0], b[1]);
+    for (int p0 = 0; p0 <= last; p0 += G * U) {
+        bool bad[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * G + g;
+            if constexpr (K == MPTG_SCENE_GRID) {
+                const bool inr = p >= 1 && p <= last;
+                const double tt = (double)ord[inr ? p - 1 : 0] * delta;
+                const bool ok = grid_free(sc, (b[0] - a[0]) * tt + a[0], (b[1] - a[1]) * tt + a[1]);
+                bad[u] = p == 0 ? to_bad : (inr && !ok);
+            } else {
+                bad[u] = false;
+                if (p == 0) {
+                    bad[u] = !known_valid && !valid_k<D, K>(sc, b);
+                } else if (p <= last) {
+                    double q[DA];
+                    space_interp_t<D>(sp, a, b, (double)ord[p - 1] * delta, q);
+                    bad[u] = !valid_k<D, K>(sc, q);
+                }
+            }
         }
-        const uint64_t hit = __ballot(bad) & gbits;
-        if (hit) {
+        // every ballot before any branch on them, so no position's loads
+        // wait behind another's verdict
+        uint64_t hits[U], any = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            hits[u] = __ballot(bad[u]) & gbits;
+            any |= hits[u];
+        }
+        if (any) {
+            int u = 0;
+            while (!hits[u]) ++u;  // ends: some hits[u] is nonzero
             if (g == 0) {
                 verdict[e] = 0;
-                checks[e] = (uint32_t)(p0 + (__builtin_ctzll(hit) - gbase) + 1);
+                checks[e] = (uint32_t)(p0 + u * G + (__builtin_ctzll(hits[u]) - gbase) + 1);
             }
             return;
         }
@@ -159,10 +212,14 @@ __global__ __launch_bounds__(kBlock) void link_dmv_lanes(SpaceArgs sp, SceneArgs
     }
 }
 
-// One wave per long edge: lane 0 replays the reference's ring in LDS and
-// emits up to 64 waypoint indices per round; all lanes check them.
-template <int D>
+// One wave per long edge (more than 255 interior waypoints).  Up to
+// kTable16Max waypoints the order comes from the 16-bit table of the ring's
+// order and every lane takes U positions per round; longer edges replay the
+// reference's ring: lane 0 walks it in LDS and emits up to 64 waypoint
+// indices per round, which all lanes check.
+template <int D, int K>
 __global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, double inv,
+                                                    const uint16_t* __restrict__ table16,
                                                     const double* __restrict__ from, const double* __restrict__ to,
                                                     const int64_t* __restrict__ long_list,
                                                     const uint32_t* __restrict__ long_count,
@@ -175,6 +232,7 @@ __global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, 
     const int lane = threadIdx.x;
     for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
         constexpr int DA = D > 0 ? D : kMaxDim;
+        constexpr int U = dmv_unroll<D>();
         const int dd = D > 0 ? D : sp.dim;
         const int64_t e = long_list[w];
         double a[DA], b[DA], q[DA];
@@ -186,6 +244,38 @@ __global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, 
             }
         double delta = 0.0;
         int64_t m = dmv_count<D>(sp, a, b, inv, &delta);
+        if (m <= kTable16Max) {  // wave-uniform branch: m is the edge's
+            const uint16_t* ord = table16 + table16_row(m);
+            int64_t done = m;
+            bool hit_any = false;
+            for (int64_t p0 = 0; p0 < m && !hit_any; p0 += 64 * U) {
+                bool bad[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t o = p0 + u * 64 + lane;
+                    bad[u] = false;
+                    if (o < m) {
+                        double qq[DA];
+                        space_interp_t<D>(sp, a, b, (double)ord[o] * delta, qq);
+                        bad[u] = !valid_k<D, K>(sc, qq);
+                    }
+                }
+                uint64_t hits[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) hits[u] = __ballot(bad[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (hits[u] && !hit_any) {
+                        done = p0 + u * 64 + __builtin_ctzll(hits[u]) + 1;
+                        hit_any = true;
+                    }
+            }
+            if (lane == 0) {
+                verdict[e] = hit_any ? 0 : 1;
+                checks[e] = (uint32_t)(1 + done);
+            }
+            continue;
+        }
         // ring state lives in lane 0's registers
         uint64_t qs = 0, qe = 1, seq_cur = 0, seq_end = 0;
         bool seq = false;
@@ -237,7 +327,7 @@ __global__ __launch_bounds__(64) void link_dmv_long(SpaceArgs sp, SceneArgs sc, 
             bool bad = false;
             if (lane < c) {
                 space_interp_t<D>(sp, a, b, (double)emitted[lane] * delta, q);
-                bad = !state_valid_t<D>(sc, q);
+                bad = !valid_k<D, K>(sc, q);
             }
             if (bad) atomicMin(&fail_lane, lane);
             __syncthreads();
@@ -424,16 +514,63 @@ std::vector<uint8_t> build_dmv_table() {
     return t;
 }
 
+// Host replay of the reference's 256-slot ring with its sequential fallback
+// (discrete_motion_validator.hpp:102-127) for 256 <= n <= kTable16Max: row n
+// of the 16-bit table lists the waypoint indices 1..n in check order.
+std::vector<uint16_t> build_dmv_table16() {
+    std::vector<uint16_t> t((size_t)table16_row(kTable16Max + 1));
+    for (int m = kTableMax + 1; m <= kTable16Max; ++m) {
+        size_t c = (size_t)table16_row(m);
+        int rmin[256], rmax[256];
+        uint64_t qs = 0, qe = 1;
+        rmin[0] = 1;
+        rmax[0] = m;
+        while (qs != qe) {
+            const int mn = rmin[qs % 256], mx = rmax[qs % 256];
+            ++qs;
+            if (mn == mx) {
+                t[c++] = (uint16_t)mn;
+            } else if (qe + 2 < qs + 256) {
+                const int mid = (mn + mx) / 2;
+                t[c++] = (uint16_t)mid;
+                if (mn < mid) {
+                    rmin[qe % 256] = mn;
+                    rmax[qe % 256] = mid - 1;
+                    ++qe;
+                }
+                if (mid < mx) {
+                    rmin[qe % 256] = mid + 1;
+                    rmax[qe % 256] = mx;
+                    ++qe;
+                }
+            } else {
+                for (int i = mn; i <= mx; ++i) t[c++] = (uint16_t)i;
+            }
+        }
+    }
+    return t;
+}
+
 }  // namespace
 
 static int ensure_tables(mptg_ctx* ctx) {
     DevBuf& tb = ctx->scratch[6];
-    if (tb.p) return MPTG_OK;
-    std::vector<uint8_t> t = build_dmv_table();
-    int rc = ensure(ctx, tb, t.size());
-    if (rc) return rc;
-    MPTG_HIP(ctx, hipMemcpyAsync(tb.p, t.data(), t.size(), hipMemcpyHostToDevice, ctx->stream));
-    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!tb.p) {
+        std::vector<uint8_t> t = build_dmv_table();
+        int rc = ensure(ctx, tb, t.size());
+        if (rc) return rc;
+        MPTG_HIP(ctx, hipMemcpyAsync(tb.p, t.data(), t.size(), hipMemcpyHostToDevice, ctx->stream));
+        MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    DevBuf& t16 = ctx->dmv_table16;
+    if (!t16.p) {
+        std::vector<uint16_t> t = build_dmv_table16();
+        int rc = ensure(ctx, t16, t.size() * sizeof(uint16_t));
+        if (rc) return rc;
+        MPTG_HIP(ctx, hipMemcpyAsync(t16.p, t.data(), t.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                     ctx->stream));
+        MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return MPTG_OK;
 }
 
@@ -523,13 +660,21 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     int lanes_log2 = __builtin_ctz((unsigned)ctx->tune.dmv_edge_lanes);
     while (lanes_log2 > 0 && (n << lanes_log2) > ((int64_t)1 << 31)) --lanes_log2;
     const unsigned sblocks = (unsigned)(((n << lanes_log2) + kBlock - 1) / kBlock);
-    dispatch_dim(ctx->dim, [&]<int D>() {
-        hipLaunchKernelGGL((link_dmv_lanes<D>), dim3(sblocks), dim3(kBlock), 0, ctx->stream, space_args(ctx),
+    auto launch = [&]<int D, int K>() {
+        hipLaunchKernelGGL((link_dmv_lanes<D, K>), dim3(sblocks), dim3(kBlock), 0, ctx->stream, space_args(ctx),
                            scene_args(ctx), inv, lanes_log2, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n,
                            known_valid, d_mask, d_verdict, d_checks, long_list, long_count);
-        hipLaunchKernelGGL((link_dmv_long<D>), dim3(lg), dim3(64), 0, ctx->stream, space_args(ctx), scene_args(ctx),
-                           inv, d_from, d_to, long_list, long_count, d_verdict, d_checks);
-    });
+        hipLaunchKernelGGL((link_dmv_long<D, K>), dim3(lg), dim3(64), 0, ctx->stream, space_args(ctx),
+                           scene_args(ctx), inv, (const uint16_t*)ctx->dmv_table16.p, d_from, d_to, long_list,
+                           long_count, d_verdict, d_checks);
+    };
+    // 2-D grids and boxes get kernels of their own scene kind
+    if (ctx->dim == 2 && ctx->scene.kind == MPTG_SCENE_GRID)
+        launch.template operator()<2, MPTG_SCENE_GRID>();
+    else if (ctx->dim == 2 && ctx->scene.kind == MPTG_SCENE_BOXES)
+        launch.template operator()<2, MPTG_SCENE_BOXES>();
+    else
+        dispatch_dim(ctx->dim, [&]<int D>() { launch.template operator()<D, 0>(); });
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }

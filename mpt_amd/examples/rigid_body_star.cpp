@@ -1,6 +1,7 @@
 // SE(3) rigid-body PRRT* (the shape of demo/se3_rigid_body_planning.cpp) on
 // the GPU planner: a box-cluster robot among random boxes, driven through
-// mpt_amd::Planner<Scenario, PRRTStar<gpu_batch<B>, rewire_k<20>>>.  Checks
+// mpt_amd::Planner<Scenario, PRRTStar<gpu_batch<B>, rewire_k<20>>> until a
+// solution is found.  Checks
 // that every solution edge passes link() again and that path costs add up.
 #include <chrono>
 #include <cmath>
@@ -48,9 +49,13 @@ int main(int argc, char** argv) {
     Scenario scenario(robot, env, goal, {0, 0, 0}, {50, 50, 50});
 
     mpt_amd::Planner<Scenario, Algorithm> planner(scenario, seed);
+    // one goal-biased draw per iteration at most (the reference's worker 0),
+    // so a larger bias than the default 0.01 for a batch of 2048
+    planner.setGoalBias(0.2);
     planner.addStart(start);
     auto t0 = std::chrono::steady_clock::now();
-    planner.solve([&] { return planner.size() >= 20000; });
+    // until solved, as demo/se3_rigid_body_planning.cpp:195-198 with a node cap
+    planner.solve([&] { return planner.solved() || planner.size() >= 200000; });
     double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     planner.printStats();
     std::vector<Scenario::State> path = planner.solution();
