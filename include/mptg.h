@@ -148,6 +148,20 @@ int mptg_link(mptg_ctx* ctx, const double* h_from, const double* h_to, int64_t n
               int endpoints_known_valid, uint8_t* h_verdict, uint32_t* h_checks);
 int mptg_link_dev(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                   int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks);
+/* mptg_link_dev without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It covers
+ * discrete-motion-validator scenes (boxes, occupancy grids, arms with DMV
+ * links; not meshes or bisection links: MPTG_E_STATE) in the reference order,
+ * with the same kernels, verdicts and counts as mptg_link_dev.  The buffers
+ * the kernels need beyond the inputs and outputs (order tables, long-edge
+ * list and count) live in one static device array of 3.1 MB per GPU,
+ * allocated with the library and held until the process ends; the first
+ * call on a device writes its order tables there (synchronously).  At most
+ * 2^18 edges per call (MPTG_E_RANGE beyond).  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (several contexts or streams) race. */
+int mptg_link_dev_noalloc(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                          int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks);
 
 /* ---- batched PRRT: replaces PRRT::solve / Worker::solve / addSample
  *      (src/mpt/impl/prrt/prrt.hpp:194-201,358-452) with batch-synchronous

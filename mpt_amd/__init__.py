@@ -250,6 +250,13 @@ class Context:
         self._chk(_lib.mptg_link_dev(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, order,
                                      1 if known_valid else 0, C.c_void_p(v_ptr), C.c_void_p(c_ptr)))
 
+    def link_dev_noalloc(self, a_ptr: int, b_ptr: int, n: int, order: int, known_valid: bool, v_ptr: int,
+                         c_ptr: int):
+        """mptg_link_dev_noalloc: link_dev on the library's static device
+        scratch (DMV scenes, at most 2^18 edges, no allocation)."""
+        self._chk(_lib.mptg_link_dev_noalloc(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr), n, order,
+                                             1 if known_valid else 0, C.c_void_p(v_ptr), C.c_void_p(c_ptr)))
+
     def link_phases(self, a: int, b: int, n: int, order: int, known_valid: bool, verdict: int, checks: int) -> dict:
         """Per-phase milliseconds of one arm link() call on device buffers."""
         ms = (C.c_float * 5)()

@@ -90,6 +90,8 @@ _SIGNATURES = [
     ("mptg_link", C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_int64, C.c_int, C.c_int, c_uint8_p, c_uint32_p]),
     ("mptg_link_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p,
                                 C.c_void_p]),
+    ("mptg_link_dev_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int,
+                                        C.c_void_p, C.c_void_p]),
     ("mptg_prrt_init", C.c_int, [C.c_void_p, C.POINTER(PrrtParams), c_double_p, c_double_p, c_double_p]),
     ("mptg_prrt_add_start", C.c_int, [C.c_void_p, c_double_p]),
     ("mptg_prrt_run", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(PrrtStats)]),

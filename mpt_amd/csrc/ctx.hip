@@ -643,4 +643,13 @@ int mptg_link_dev(mptg_ctx* ctx, const double* d_from, const double* d_to, int64
     return launch_link(ctx, d_from, d_to, n, order, endpoints_known_valid, d_verdict, d_checks);
 }
 
+int mptg_link_dev_noalloc(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                          int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks) {
+    int rc = check_device(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!d_from || !d_to || !d_verdict || !d_checks)))
+        return fail(ctx, MPTG_E_ARG, "bad link buffers");
+    return link_dev_noalloc(ctx, d_from, d_to, n, order, endpoints_known_valid, d_verdict, d_checks);
+}
+
 }  // extern "C"

@@ -202,6 +202,10 @@ int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t
 int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                        int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
 int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
+// mptg_link_dev_noalloc's work (validate.hip): the DMV kernels on the
+// library's static device scratch, no allocation
+int link_dev_noalloc(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                     int known_valid, uint8_t* d_verdict, uint32_t* d_checks);
 // mesh scene: one wave per state / edge
 int launch_mesh_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
 int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from, const double* d_to, int64_t n,

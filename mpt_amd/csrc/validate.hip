@@ -25,6 +25,8 @@
 // before the verdict is known, endpoint checks included.
 #include <algorithm>
 #include <array>
+#include <mutex>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -495,30 +497,43 @@ __global__ __launch_bounds__(kBlock) void link_bisect_work(int D, SceneArgs sc, 
 }
 
 // Host replay of the reference ring for n <= 255 (never overflows): the BFS
-// order of the bisection ranges.  Table row n starts at n(n-1)/2.
-std::vector<uint8_t> build_dmv_table() {
-    std::vector<uint8_t> t((size_t)kTableMax * (kTableMax + 1) / 2);
+// order of the bisection ranges.  Table row n starts at n(n-1)/2; t holds
+// kTable8Entries bytes.  No heap: at most n ranges are ever queued.
+constexpr size_t kTable8Entries = (size_t)kTableMax * (kTableMax + 1) / 2;
+void fill_dmv_table(uint8_t* t) {
     for (int m = 1; m <= kTableMax; ++m) {
         size_t off = (size_t)m * (m - 1) / 2, c = 0;
-        std::vector<std::pair<int, int>> q;
-        q.reserve(m);
-        q.emplace_back(1, m);
-        for (size_t h = 0; h < q.size(); ++h) {
-            auto [mn, mx] = q[h];
-            int mid = (mn + mx) / 2;
+        int qmin[kTableMax], qmax[kTableMax];
+        int qe = 0;
+        qmin[qe] = 1;
+        qmax[qe++] = m;
+        for (int h = 0; h < qe; ++h) {
+            const int mn = qmin[h], mx = qmax[h];
+            const int mid = (mn + mx) / 2;
             t[off + c++] = (uint8_t)mid;
-            if (mn < mid) q.emplace_back(mn, mid - 1);
-            if (mid < mx) q.emplace_back(mid + 1, mx);
+            if (mn < mid) {
+                qmin[qe] = mn;
+                qmax[qe++] = mid - 1;
+            }
+            if (mid < mx) {
+                qmin[qe] = mid + 1;
+                qmax[qe++] = mx;
+            }
         }
     }
+}
+
+std::vector<uint8_t> build_dmv_table() {
+    std::vector<uint8_t> t(kTable8Entries);
+    fill_dmv_table(t.data());
     return t;
 }
 
 // Host replay of the reference's 256-slot ring with its sequential fallback
 // (discrete_motion_validator.hpp:102-127) for 256 <= n <= kTable16Max: row n
 // of the 16-bit table lists the waypoint indices 1..n in check order.
-std::vector<uint16_t> build_dmv_table16() {
-    std::vector<uint16_t> t((size_t)table16_row(kTable16Max + 1));
+constexpr size_t kTable16Entries = (size_t)table16_row(kTable16Max + 1);
+void fill_dmv_table16(uint16_t* t) {
     for (int m = kTableMax + 1; m <= kTable16Max; ++m) {
         size_t c = (size_t)table16_row(m);
         int rmin[256], rmax[256];
@@ -548,6 +563,11 @@ std::vector<uint16_t> build_dmv_table16() {
             }
         }
     }
+}
+
+std::vector<uint16_t> build_dmv_table16() {
+    std::vector<uint16_t> t(kTable16Entries);
+    fill_dmv_table16(t.data());
     return t;
 }
 
@@ -599,6 +619,10 @@ int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }
+
+static int launch_dmv(mptg_ctx* ctx, const uint8_t* table8, const uint16_t* table16, int64_t* long_list,
+                      uint32_t* long_count, const double* d_from, const double* d_to, int64_t n, int known_valid,
+                      const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
 
 int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order, int known_valid,
                 uint8_t* d_verdict, uint32_t* d_checks) {
@@ -653,6 +677,16 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     if ((rc = ensure(ctx, ctx->scratch[7], sizeof(int64_t) * n + 256))) return rc;
     uint32_t* long_count = (uint32_t*)ctx->scratch[7].p;
     int64_t* long_list = (int64_t*)((char*)ctx->scratch[7].p + 256);
+    return launch_dmv(ctx, (const uint8_t*)ctx->scratch[6].p, (const uint16_t*)ctx->dmv_table16.p, long_list,
+                      long_count, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks);
+}
+
+// The DMV link kernels over n edges, with the order tables and the long-edge
+// list (n entries) and count given by the caller.
+static int launch_dmv(mptg_ctx* ctx, const uint8_t* table8, const uint16_t* table16, int64_t* long_list,
+                      uint32_t* long_count, const double* d_from, const double* d_to, int64_t n, int known_valid,
+                      const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks) {
+    int rc;
     if ((rc = launch_zero_u32(ctx, long_count, 1))) return rc;
     double inv = 1.0 / ctx->link_param;
     unsigned lg = (unsigned)std::min<int64_t>(n, (int64_t)ctx->cus * ctx->tune.dmv_long_waves_per_cu);
@@ -662,11 +696,10 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     const unsigned sblocks = (unsigned)(((n << lanes_log2) + kBlock - 1) / kBlock);
     auto launch = [&]<int D, int K>() {
         hipLaunchKernelGGL((link_dmv_lanes<D, K>), dim3(sblocks), dim3(kBlock), 0, ctx->stream, space_args(ctx),
-                           scene_args(ctx), inv, lanes_log2, (const uint8_t*)ctx->scratch[6].p, d_from, d_to, n,
-                           known_valid, d_mask, d_verdict, d_checks, long_list, long_count);
+                           scene_args(ctx), inv, lanes_log2, table8, d_from, d_to, n, known_valid, d_mask, d_verdict,
+                           d_checks, long_list, long_count);
         hipLaunchKernelGGL((link_dmv_long<D, K>), dim3(lg), dim3(64), 0, ctx->stream, space_args(ctx),
-                           scene_args(ctx), inv, (const uint16_t*)ctx->dmv_table16.p, d_from, d_to, long_list,
-                           long_count, d_verdict, d_checks);
+                           scene_args(ctx), inv, table16, d_from, d_to, long_list, long_count, d_verdict, d_checks);
     };
     // 2-D grids and boxes get kernels of their own scene kind
     if (ctx->dim == 2 && ctx->scene.kind == MPTG_SCENE_GRID)
@@ -698,6 +731,88 @@ int launch_link_work(mptg_ctx* ctx, const double* d_from, const double* d_to, in
                        ctx->dim, scene_args(ctx), ctx->link_param, d_from, d_to, n, known_valid, d_work + 5);
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
+}
+
+// ---- allocation-free DMV link (mptg_link_dev_noalloc) ---------------------
+// Everything the DMV kernels need beyond the caller's inputs and outputs sits
+// in one static device array per GPU, `scratch`:
+//   [kNaCountOff, +256)            the long-edge count
+//   [kNaTableOff, +kTable8Entries) the 8-bit BFS order table (n <= 255)
+//   [kNaTable16Off, ...)           the 16-bit ring-order table (256..1023)
+//   [kNaListOff, ...)              the long-edge list, kNoAllocMaxEdges int64
+// Host code uses the device address hipGetSymbolAddress returns for it (kept
+// per device), never the array's name.  The tables are written once per
+// device at the first call on it; the count is cleared by a kernel on every
+// call.  Calls that overlap on one GPU share the array and race.
+namespace {
+constexpr int64_t kNoAllocMaxEdges = (int64_t)1 << 18;
+constexpr size_t kNaCountOff = 0;
+constexpr size_t kNaTableOff = 256;
+constexpr size_t kNaTable16Off = kNaTableOff + ((kTable8Entries + 255) / 256) * 256;
+constexpr size_t kNaListOff = kNaTable16Off + ((kTable16Entries * sizeof(uint16_t) + 255) / 256) * 256;
+constexpr size_t kNaBytes = kNaListOff + sizeof(int64_t) * (size_t)kNoAllocMaxEdges;
+
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaMaxDevices = 64;
+struct NoAllocDevice {
+    std::once_flag once;
+    unsigned char* ptr = nullptr;  // device address of `scratch` on this device
+    hipError_t err = hipSuccess;
+};
+NoAllocDevice g_noalloc[kNaMaxDevices];
+// host images of the two tables (static storage: the set-up allocates nothing)
+uint8_t g_table8[kTable8Entries];
+uint16_t g_table16[kTable16Entries];
+std::once_flag g_tables_once;
+
+// Per-device set-up, once, with the context's device current: the array's
+// address and the order tables.
+int noalloc_scratch(mptg_ctx* ctx, unsigned char** out) {
+    if (ctx->device < 0 || ctx->device >= kNaMaxDevices) return fail(ctx, MPTG_E_RANGE, "device id beyond 63");
+    std::call_once(g_tables_once, [] {
+        fill_dmv_table(g_table8);
+        fill_dmv_table16(g_table16);
+    });
+    NoAllocDevice& d = g_noalloc[ctx->device];
+    std::call_once(d.once, [&] {
+        void* p = nullptr;
+        d.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (d.err != hipSuccess) return;
+        unsigned char* base = static_cast<unsigned char*>(p);
+        d.err = hipMemcpy(base + kNaTableOff, g_table8, sizeof(g_table8), hipMemcpyHostToDevice);
+        if (d.err != hipSuccess) return;
+        d.err = hipMemcpy(base + kNaTable16Off, g_table16, sizeof(g_table16), hipMemcpyHostToDevice);
+        if (d.err != hipSuccess) return;
+        d.ptr = base;
+    });
+    if (!d.ptr)
+        return fail(ctx, MPTG_E_HIP, std::string("static scratch set-up failed: ") + hipGetErrorString(d.err));
+    *out = d.ptr;
+    return MPTG_OK;
+}
+}  // namespace
+
+int link_dev_noalloc(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
+                     int known_valid, uint8_t* d_verdict, uint32_t* d_checks) {
+    if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
+    if (ctx->scene.kind == MPTG_SCENE_NONE) return fail(ctx, MPTG_E_STATE, "no scene uploaded");
+    if (ctx->link_mode != MPTG_LINK_DMV || ctx->scene.kind == MPTG_SCENE_MESH)
+        return fail(ctx, MPTG_E_STATE, "the allocation-free link covers discrete-motion-validator scenes "
+                                       "(boxes, grids, arms with DMV links)");
+    if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > ctx->dim)
+        return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
+    if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
+    if (order != MPTG_ORDER_REFERENCE) return fail(ctx, MPTG_E_STATE, "DMV links check in the reference order");
+    if (n > kNoAllocMaxEdges)
+        return fail(ctx, MPTG_E_RANGE, "the allocation-free link takes at most 2^18 edges per call");
+    if (n == 0) return MPTG_OK;
+    unsigned char* base = nullptr;
+    int rc = noalloc_scratch(ctx, &base);
+    if (rc) return rc;
+    return launch_dmv(ctx, base + kNaTableOff, reinterpret_cast<const uint16_t*>(base + kNaTable16Off),
+                      reinterpret_cast<int64_t*>(base + kNaListOff), reinterpret_cast<uint32_t*>(base + kNaCountOff),
+                      d_from, d_to, n, known_valid, nullptr, d_verdict, d_checks);
 }
 
 }  // namespace mptg
