@@ -318,6 +318,8 @@ int mptg_tree_clear(mptg_ctx* ctx) {
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->size = 0;
     ctx->star_ok = true;
+    ctx->valid_upto = 0;
+    ctx->some_invalid = false;
     // the prefilter's coordinate bound: the planner's sampling box and goal
     // (every node a planner adds lies within them), grown again by inserts
     ctx->coord_bound = planner_coord_bound(ctx);
@@ -407,6 +409,8 @@ int mptg_scene_boxes(mptg_ctx* ctx, const double* h_boxes, int n) {
     rc = upload(ctx, (void**)&ctx->scene.boxes, h_boxes, sizeof(double) * 4 * n);
     if (rc) return rc;
     ctx->scene.kind = MPTG_SCENE_BOXES;
+    ctx->valid_upto = 0;  // valid() changed: every node is checked again
+    ctx->some_invalid = false;
     ctx->scene.nboxes = n;
     return MPTG_OK;
 }
@@ -431,6 +435,8 @@ int mptg_scene_grid(mptg_ctx* ctx, const uint8_t* h_cells, int width, int height
     rc = upload(ctx, (void**)&ctx->scene.grid, bits.data(), bits.size() * sizeof(uint64_t));
     if (rc) return rc;
     ctx->scene.kind = MPTG_SCENE_GRID;
+    ctx->valid_upto = 0;  // valid() changed: every node is checked again
+    ctx->some_invalid = false;
     ctx->scene.width = width;
     ctx->scene.height = height;
     ctx->scene.grid_tx = tx;
@@ -539,6 +545,8 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
                                  hipMemcpyHostToDevice, ctx->stream));
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->scene.kind = MPTG_SCENE_ARM;
+    ctx->valid_upto = 0;  // valid() changed: every node is checked again
+    ctx->some_invalid = false;
     ctx->scene.total_length = 0;
     for (int i = 0; i < nlinks; ++i) ctx->scene.total_length += h_lengths[i];
     ctx->scene.nlinks = nlinks;

@@ -51,6 +51,9 @@ struct PrrtState {
     bool counters_valid = false;  // device counters mirror the host state
     mptg_prrt_stats stats_base{};  // host stats when the device counters were reset
     DevBuf q, near, ndist, from, to, status, mask, verdict, checks, counters, goal_buf, blk;
+    // valid() before nearest (no range limit): listed valid samples, their ids,
+    // the list count, and their nearest node and distance
+    DevBuf qv, vidx, vcnt, kn_i, kn_d, vcheck;
     mptg_prrt_params p{};
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
     int64_t iteration = 0;       // next iteration index
@@ -98,6 +101,11 @@ struct mptg_ctx {
     int32_t *child = nullptr, *next = nullptr, *prev = nullptr;
     bool star_ok = true;
     double coord_bound = 0;  // max |coordinate| over inserted nodes and bounds
+    // nodes [0, valid_upto) are known to pass the scene's valid(); some_invalid:
+    // a node was found invalid (both reset when the tree is cleared or the
+    // scene changes)
+    int64_t valid_upto = 0;
+    bool some_invalid = false;
 
     // scene
     mptg::SceneDev scene;
@@ -195,8 +203,11 @@ int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n);
 int launch_zero_pair(mptg_ctx* ctx, uint32_t* a, uint32_t* b);
 // Sets n 32-bit words to v with a kernel.
 int launch_fill_u32(mptg_ctx* ctx, uint32_t* p, int64_t n, uint32_t v);
+// nq_dev (k = 1, L1/L2): only the first *nq_dev of the nq query slots are
+// live (nq stays the layout); the scans run, planned for half the slots.
 int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
-               double* d_dist, int64_t tree_n, const int64_t* n_dev = nullptr, const int32_t* skip = nullptr);
+               double* d_dist, int64_t tree_n, const int64_t* n_dev = nullptr, const int32_t* skip = nullptr,
+               const uint32_t* nq_dev = nullptr);
 int launch_link(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                 int known_valid, uint8_t* d_verdict, uint32_t* d_checks);
 int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,

@@ -40,8 +40,10 @@ constexpr int kBlock = 256;
 // Fewest tree nodes per split of the scan: small trees still spread over
 // many waves, while every split's partial list stays worth its merge.
 constexpr int64_t kMinSplitNodes = 64;
-// Tree nodes every k = 1 query measures first, for a starting bound.
-constexpr int64_t kSeedNodes = 32;
+// Tree nodes every k = 1 query measures first, for a starting bound: every
+// (n / kSeedNodes)-th node, spread over the tree rather than its first nodes
+// (a planner's first nodes crowd around its start).
+constexpr int64_t kSeedNodes = 256;
 
 __device__ __forceinline__ bool better(double a, int ia, double b, int ib) {
     return a < b || (a == b && ia < ib);
@@ -120,6 +122,29 @@ __device__ __forceinline__ float prefilter_threshold(double dk, double S) {
     return (double)f < thr ? float_of(bits_of(f) + 1u) : f;
 }
 
+// An upper bound, in metric units, on the exact distance of a node whose
+// prefilter value (FP32 sum, squared for L2) is `a`: the inverse of the
+// error bound behind prefilter_threshold.  L1: |a - d| <= D eps + g (d +
+// D eps) gives d <= (a + D eps (1 + g)) / (1 - g).  L2: |a - T| <= r + g (T +
+// r) with r = 2 eps sqrt(D T) + D eps^2 gives, for x = sqrt(T), (1 - g) x^2
+// - 2 (1 + g) eps sqrt(D) x - (a + (1 + g) D eps^2) <= 0, bounded by the
+// quadratic's larger root.  A relative 2^-40 covers the FP64 rounding.
+template <int D, int METRIC>
+__device__ __forceinline__ double prefilter_upper(float a, double S) {
+    const double u = 0x1.0p-24;
+    const double eps = 2.01 * u * S;
+    const double g = D * u / (1.0 - D * u);
+    double x;
+    if constexpr (METRIC == MPTG_METRIC_L1) {
+        x = ((double)a + D * eps * (1.0 + g)) / (1.0 - g);
+    } else {
+        const double bq = (1.0 + g) * eps * __builtin_sqrt((double)D);
+        const double c = (double)a + (1.0 + g) * D * eps * eps;
+        x = (bq + __builtin_sqrt(bq * bq + (1.0 - g) * c)) / (1.0 - g);
+    }
+    return x * (1.0 + 0x1.0p-40) + 1e-300;
+}
+
 // Splits actually used for a tree of n nodes when the launch was planned
 // for `planned` (a graph captured for a larger tree): at least kMinSplitNodes
 // per split, as the host plan, and no empty split.
@@ -143,31 +168,41 @@ __device__ __forceinline__ bool knn_extent(const int64_t* n_dev, const int32_t* 
     return true;
 }
 
-// Nearest of the tree's first kSeedNodes nodes to each query (exact FP64
-// distance): a starting bound for the k = 1 scan.  kSeedLanes lanes share a
-// query, each measuring every kSeedLanes-th node; the minimum is exact in
-// any order.
-constexpr int kSeedLanes = 8;
+// Nearest of kSeedNodes tree nodes (every (n / kSeedNodes)-th) to each query
+// (exact FP64 distance): a starting bound for the k = 1 scan.  kSeedLanes
+// lanes share a query, each measuring every kSeedLanes-th of those nodes;
+// the minimum is exact in any order.
+constexpr int kSeedLanes = 32;
 template <int D, int METRIC>
 __global__ __launch_bounds__(kBlock) void knn_seed(const double* __restrict__ x64, int64_t n,
                                                    const double* __restrict__ q, int64_t nq,
                                                    double* __restrict__ seed, const int64_t* n_dev,
-                                                   const int32_t* skip) {
+                                                   const int32_t* skip, const uint32_t* nq_dev) {
     static_assert(64 % kSeedLanes == 0, "lane groups within a wave");
     if (skip && *skip) return;  // uniform: every thread reads the same flag
+    if (nq_dev) nq = *nq_dev;   // live queries: the first *nq_dev slots
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t qi = t / kSeedLanes;
     const int part = (int)(t % kSeedLanes);
     if (n_dev) n = *n_dev;
     const int64_t ns = n < kSeedNodes ? n : kSeedNodes;
+    const int64_t stride = ns > 0 ? n / ns : 1;
     double best = __builtin_inf();
-    if (qi < nq)
+    if (qi < nq) {
+#pragma unroll 8
         for (int64_t j = part; j < ns; j += kSeedLanes)
-            best = __builtin_fmin(best, exact_dist<D, METRIC>(q + qi * D, x64 + j * D));
+            best = __builtin_fmin(best, exact_dist<D, METRIC>(q + qi * D, x64 + j * stride * D));
+    }
 #pragma unroll
     for (int o = 1; o < kSeedLanes; o <<= 1) best = __builtin_fmin(best, __shfl_xor(best, o));
     if (qi < nq && part == 0) seed[qi] = best;
 }
+
+// Tree nodes a block stages in LDS at a time: FP32 rows for the prefilter
+// and FP64 rows for the exact distances, so a lane that passes the prefilter
+// reads its node from LDS instead of waiting on a global load while the
+// rest of its wave idles.
+constexpr int kScanTile = 64;
 
 template <int D, int METRIC, int K>
 __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32, const double* __restrict__ x64,
@@ -175,11 +210,16 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
                                                    int64_t nq, double radius, double pbound,
                                                    double* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                    const int64_t* n_dev, const int32_t* skip,
-                                                   const double* __restrict__ seed) {
+                                                   const double* __restrict__ seed, const uint32_t* nq_dev) {
     constexpr int DP = (D <= 2) ? 2 : (D <= 4 ? 4 : (D <= 8 ? 8 : ((D + 3) / 4) * 4));
-    if (!knn_extent(n_dev, skip, &n, &chunk)) return;
+    __shared__ __align__(16) float t32[kScanTile * DP];
+    __shared__ __align__(16) double t64[kScanTile * D];
+    if (!knn_extent(n_dev, skip, &n, &chunk)) return;  // block-uniform
+    // live queries: all nq, or the first *nq_dev (nq stays the output stride)
+    const int64_t live = nq_dev ? (int64_t)*nq_dev : nq;
+    if ((int64_t)blockIdx.x * kBlock >= live) return;  // block-uniform
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool active = qi < nq;
+    const bool active = qi < live;
     const int64_t qc = active ? qi : 0;
     const int64_t p0 = (int64_t)blockIdx.y * chunk;
     const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
@@ -195,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
     const double S = mq + pbound;
     TopK<K> top;
     top.init();
-    // seed bound (k = 1): the nearest of the tree's first nodes, measured by
+    // seed bound (k = 1): the nearest of a subset of the tree, measured by
     // knn_seed.  The nearest node overall is no farther, so it still passes
     // the prefilter in its own chunk; other chunks skip the nodes beyond the
     // bound (their partial lists may then hold nothing) and the merge of the
@@ -204,58 +244,104 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
     float rthr = prefilter_threshold<D, METRIC>(radius, S);
     float thr = rthr;
     if (!active) thr = -1.0f;  // inactive lanes never take the slow path
-
-    // four tree nodes per step: their rows load together, one branch for all
-    constexpr int U = 4;
-    int64_t p = p0;
-    for (; p + U <= p1; p += U) {
-        float acc[U];
+    // k = 1 candidates awaiting their exact distance.  A node whose prefilter
+    // value exceeds a threshold derived from a listed candidate's upper bound
+    // is farther than that candidate, so it cannot be the nearest; every
+    // listed node gets its exact distance before the result is written.
+    constexpr int kCand = 4;
+    int32_t cand[kCand];
+    int ncand = 0;
+    auto flush = [&]() {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const float* row = x32 + (p + u) * DP;
-            float a = 0.0f;
+        for (int c = 0; c < kCand; ++c)
+            if (c < ncand) {
+                const double dq = exact_dist<D, METRIC>(q + qc * D, x64 + (int64_t)cand[c] * D);
+                if (dq <= radius) top.insert(dq, cand[c]);
+            }
+        ncand = 0;
+    };
+
+    for (int64_t t0 = p0; t0 < p1; t0 += kScanTile) {  // block-uniform bounds
+        const int cnt = (int)(p1 - t0 < kScanTile ? p1 - t0 : kScanTile);
+        __syncthreads();  // every wave is done with the previous tile
+        for (int i = threadIdx.x; i < cnt * DP; i += kBlock) t32[i] = x32[t0 * DP + i];
+        for (int i = threadIdx.x; i < cnt * D; i += kBlock) t64[i] = x64[t0 * D + i];
+        __syncthreads();
+        // four tree nodes per step, one branch for all
+        constexpr int U = 4;
+        int j = 0;
+        for (; j + U <= cnt; j += U) {
+            float acc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float* row = t32 + (j + u) * DP;
+                float a = 0.0f;
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    float t = q32[i] - row[i];
+                    if constexpr (METRIC == MPTG_METRIC_L1) a = a + __builtin_fabsf(t);
+                    else a = __builtin_fmaf(t, t, a);
+                }
+                acc[u] = a;
+            }
+            uint32_t pass = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) pass |= (acc[u] <= thr ? 1u : 0u) << u;
+            if constexpr (K == 1) {
+                // k = 1: list the candidates and tighten the threshold from
+                // their prefilter values; their exact distances are taken
+                // together, when the list is full or at the chunk's end
+                while (pass) {
+                    const int u = __builtin_ctz(pass);
+                    pass &= pass - 1;
+                    if (ncand == kCand) flush();
+                    cand[ncand++] = (int32_t)(t0 + j + u);
+                    float t = prefilter_threshold<D, METRIC>(prefilter_upper<D, METRIC>(acc[u], S), S);
+                    thr = t < thr ? t : thr;
+                }
+            } else {
+                // candidates in node order; a candidate that a lower
+                // threshold would now reject is still safe to evaluate
+                // (insert keeps the exact order)
+                while (pass) {
+                    const int u = __builtin_ctz(pass);
+                    pass &= pass - 1;
+                    double dq = exact_dist<D, METRIC>(q + qc * D, t64 + (j + u) * D);
+                    if (dq <= radius) {
+                        top.insert(dq, (int32_t)(t0 + j + u));
+                        float t = prefilter_threshold<D, METRIC>(top.d[K - 1], S);
+                        thr = t < rthr ? t : rthr;
+                    }
+                }
+            }
+        }
+        for (; j < cnt; ++j) {
+            const float* row = t32 + j * DP;
+            float acc = 0.0f;
 #pragma unroll
             for (int i = 0; i < D; ++i) {
                 float t = q32[i] - row[i];
-                if constexpr (METRIC == MPTG_METRIC_L1) a = a + __builtin_fabsf(t);
-                else a = __builtin_fmaf(t, t, a);
+                if constexpr (METRIC == MPTG_METRIC_L1) acc = acc + __builtin_fabsf(t);
+                else acc = __builtin_fmaf(t, t, acc);
             }
-            acc[u] = a;
-        }
-        uint32_t pass = 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) pass |= (acc[u] <= thr ? 1u : 0u) << u;
-        // candidates in node order; a candidate that a lower threshold would
-        // now reject is still safe to evaluate (insert keeps the exact order)
-        while (pass) {
-            const int u = __builtin_ctz(pass);
-            pass &= pass - 1;
-            double dq = exact_dist<D, METRIC>(q + qc * D, x64 + (p + u) * D);
-            if (dq <= radius) {
-                top.insert(dq, (int32_t)(p + u));
-                float t = prefilter_threshold<D, METRIC>(top.d[K - 1], S);
-                thr = t < rthr ? t : rthr;
-            }
-        }
-    }
-    for (; p < p1; ++p) {
-        const float* row = x32 + p * DP;
-        float acc = 0.0f;
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            float t = q32[i] - row[i];
-            if constexpr (METRIC == MPTG_METRIC_L1) acc = acc + __builtin_fabsf(t);
-            else acc = __builtin_fmaf(t, t, acc);
-        }
-        if (acc <= thr) {
-            double dq = exact_dist<D, METRIC>(q + qc * D, x64 + p * D);
-            if (dq <= radius) {
-                top.insert(dq, (int32_t)p);
-                float t = prefilter_threshold<D, METRIC>(top.d[K - 1], S);
-                thr = t < rthr ? t : rthr;
+            if (acc <= thr) {
+                if constexpr (K == 1) {
+                    if (ncand == kCand) flush();
+                    cand[ncand++] = (int32_t)(t0 + j);
+                    float t = prefilter_threshold<D, METRIC>(prefilter_upper<D, METRIC>(acc, S), S);
+                    thr = t < thr ? t : thr;
+                } else {
+                    double dq = exact_dist<D, METRIC>(q + qc * D, t64 + j * D);
+                    if (dq <= radius) {
+                        top.insert(dq, (int32_t)(t0 + j));
+                        float t = prefilter_threshold<D, METRIC>(top.d[K - 1], S);
+                        thr = t < rthr ? t : rthr;
+                    }
+                }
             }
         }
     }
+    if constexpr (K == 1) flush();
     if (!active) return;
     const int64_t base = ((int64_t)blockIdx.y * nq + qi) * K;
 #pragma unroll
@@ -342,9 +428,10 @@ constexpr int kMergeLanes = 8;
 __global__ __launch_bounds__(kBlock) void knn_merge1(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                      int splits, int64_t nq, double* __restrict__ out_d,
                                                      int32_t* __restrict__ out_i, const int32_t* skip,
-                                                     const int64_t* n_dev, double inf) {
+                                                     const int64_t* n_dev, double inf, const uint32_t* nq_dev) {
     static_assert(64 % kMergeLanes == 0, "lane groups within a wave");
     if (skip && *skip) return;  // uniform: every thread reads the same flag
+    const int64_t live = nq_dev ? (int64_t)*nq_dev : nq;  // nq stays the partial results' stride
     if (n_dev) {
         int64_t chunk;
         splits = (int)knn_used_splits(*n_dev, splits, &chunk);
@@ -354,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void knn_merge1(const double* __restrict__ 
     const int part = (int)(t % kMergeLanes);
     double bd = inf;
     int32_t bi = INT_MAX;
-    if (qi < nq) {
+    if (qi < live) {
 #pragma unroll 4
         for (int s = part; s < splits; s += kMergeLanes) {
             const double d = pd[(int64_t)s * nq + qi];
@@ -374,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void knn_merge1(const double* __restrict__ 
             bi = oi;
         }
     }
-    if (qi < nq && part == 0) {
+    if (qi < live && part == 0) {
         const bool have = bi != INT_MAX;
         out_d[qi] = have ? bd : inf;
         out_i[qi] = have ? bi : -1;
@@ -384,9 +471,10 @@ __global__ __launch_bounds__(kBlock) void knn_merge1(const double* __restrict__ 
 template <int K>
 __global__ __launch_bounds__(kBlock) void knn_finish(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                      int64_t nq, int k, double* __restrict__ out_d,
-                                                     int32_t* __restrict__ out_i, const int32_t* skip) {
+                                                     int32_t* __restrict__ out_i, const int32_t* skip,
+                                                     const uint32_t* nq_dev) {
     const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (qi >= nq || (skip && *skip)) return;
+    if (qi >= nq || (skip && *skip) || (nq_dev && qi >= (int64_t)*nq_dev)) return;
     for (int j = 0; j < k; ++j) {
         int32_t id = pi[qi * K + j];
         bool have = id != INT_MAX;
@@ -583,7 +671,8 @@ struct ScanExtent {
     int64_t n, chunk;
     const int64_t* n_dev;
     const int32_t* skip;
-    double* seed;  // k = 1: per-query starting bound (nullptr: none)
+    double* seed;            // k = 1: per-query starting bound (nullptr: none)
+    const uint32_t* nq_dev;  // live queries on the device (nullptr: all nq)
 };
 
 template <int D, int METRIC, int K>
@@ -591,9 +680,10 @@ void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q,
                  double* pd, int32_t* pi) {
     if (K == 1 && x.seed)
         hipLaunchKernelGGL((knn_seed<D, METRIC>), dim3((unsigned)((nq * kSeedLanes + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, x.seed, x.n_dev, x.skip);
+                           dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, x.seed, x.n_dev, x.skip, x.nq_dev);
     hipLaunchKernelGGL((knn_scan<D, METRIC, K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n, x.chunk,
-                       q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip, (const double*)(K == 1 ? x.seed : nullptr));
+                       q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip, (const double*)(K == 1 ? x.seed : nullptr),
+                       x.nq_dev);
 }
 
 template <int D, int K>
@@ -655,7 +745,9 @@ static int64_t plan_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int64_t* chunk,
 int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
     const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
     int64_t chunk;
-    const int64_t splits = plan_splits(ctx, nq, tree_n, &chunk, K);
+    // k = 1 launches with a device query count plan for half the slots,
+    // which never takes fewer splits
+    const int64_t splits = plan_splits(ctx, K == 1 ? (nq + 1) / 2 : nq, tree_n, &chunk, K);
     int rc;
     if ((rc = ensure(ctx, ctx->scratch[4], sizeof(double) * K * nq * splits))) return rc;
     if (K == 1 && (rc = ensure(ctx, ctx->scratch[3], sizeof(double) * nq))) return rc;
@@ -664,11 +756,14 @@ int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
 
 // The k-list scan: each lane keeps its query's k best in registers.
 static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
-                         double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip) {
+                         double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip,
+                         const uint32_t* nq_dev = nullptr) {
     const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
-    // with n_dev, tree_n is only the size the split count is planned for
+    // with n_dev, tree_n is only the size the split count is planned for;
+    // with nq_dev (live queries on the device, k = 1) the split is planned
+    // for half of the nq slots
     int64_t n = tree_n, chunk;
-    const int64_t splits = plan_splits(ctx, nq, n, &chunk, K);
+    const int64_t splits = plan_splits(ctx, nq_dev ? (nq + 1) / 2 : nq, n, &chunk, K);
     size_t pbytes_d = sizeof(double) * K * nq * splits, pbytes_i = sizeof(int32_t) * K * nq * splits;
     int rc;
     if ((rc = ensure(ctx, ctx->scratch[4], pbytes_d))) return rc;
@@ -680,7 +775,7 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
         if ((rc = ensure(ctx, ctx->scratch[3], sizeof(double) * nq))) return rc;
         seed = (double*)ctx->scratch[3].p;
     }
-    const ScanExtent ext{n, chunk, n_dev, skip, seed};
+    const ScanExtent ext{n, chunk, n_dev, skip, seed, nq_dev};
     double* pd = (double*)ctx->scratch[4].p;
     int32_t* pi = (int32_t*)ctx->scratch[5].p;
     dim3 grid((unsigned)((nq + kBlock - 1) / kBlock), (unsigned)splits);
@@ -692,13 +787,13 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     MPTG_HIP(ctx, hipGetLastError());
     dim3 g1((unsigned)((nq + kBlock - 1) / kBlock));
     if (splits > 1) {
-        if (K == 1) hipLaunchKernelGGL(knn_merge1, dim3((unsigned)((nq * kMergeLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, d_dist, d_idx, skip, n_dev, std::numeric_limits<double>::infinity());
+        if (K == 1) hipLaunchKernelGGL(knn_merge1, dim3((unsigned)((nq * kMergeLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, d_dist, d_idx, skip, n_dev, std::numeric_limits<double>::infinity(), nq_dev);
         else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
         else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
     } else {
-        if (K == 1) hipLaunchKernelGGL((knn_finish<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);
-        else if (K == 20) hipLaunchKernelGGL((knn_finish<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);
-        else hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip);
+        if (K == 1) hipLaunchKernelGGL((knn_finish<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
+        else if (K == 20) hipLaunchKernelGGL((knn_finish<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
+        else hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
     }
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
@@ -823,10 +918,15 @@ static int knn_select_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, 
 constexpr int64_t kSelectMinTree = 1 << 16;  // smaller trees: the k-list scan
 
 int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
-               double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip) {
+               double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip, const uint32_t* nq_dev) {
     if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
     if (nq > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 queries");
     if (!(radius >= 0)) return fail(ctx, MPTG_E_ARG, "radius must be >= 0");
+    if (nq_dev) {  // live queries known only on the device: the k = 1 scans
+        if (k != 1 || ctx->metric == MPTG_METRIC_SE3)
+            return fail(ctx, MPTG_E_STATE, "a device query count needs k = 1 and an L1/L2 space");
+        return knn_scan_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n, n_dev, skip, nq_dev);
+    }
     const bool sel_dim = ctx->dim <= 8 || ctx->dim == 10 || ctx->dim == 20;
     const bool eager = nq > 0 && !n_dev && !skip && !ctx->capturing;
     const bool select = eager && k > 1 && tree_n >= kSelectMinTree && ctx->tune.knn_select &&

@@ -579,5 +579,7 @@ extern "C" int mptg_scene_mesh(mptg_ctx* ctx, const float* h_robot, int nrobot, 
     s.mesh_nnodes = (int)bb.nodes.size();
     s.mesh_nrobot = nrobot;
     s.kind = MPTG_SCENE_MESH;
+    ctx->valid_upto = 0;  // valid() changed: every node is checked again
+    ctx->some_invalid = false;
     return MPTG_OK;
 }

@@ -15,6 +15,7 @@
 // Samples are drawn from the Philox stream (seed, it * B + b).
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "prrt_dev.hpp"
 
@@ -204,6 +205,45 @@ int prrt_reserve(mptg_ctx* ctx, int B) {
     if ((rc = ensure(ctx, pb.counters, sizeof(Counters)))) return rc;
     if ((rc = ensure(ctx, pb.goal_buf, sizeof(int32_t) * 4096))) return rc;
     if ((rc = ensure(ctx, pb.blk, sizeof(int32_t) * 2 * ((B + kAppendBlock - 1) / kAppendBlock)))) return rc;
+    if ((rc = ensure(ctx, pb.qv, sizeof(double) * D * B))) return rc;
+    if ((rc = ensure(ctx, pb.vidx, sizeof(int32_t) * B))) return rc;
+    if ((rc = ensure(ctx, pb.vcnt, 256))) return rc;
+    if ((rc = ensure(ctx, pb.kn_i, sizeof(int32_t) * B))) return rc;
+    if ((rc = ensure(ctx, pb.kn_d, sizeof(double) * B))) return rc;
+    return MPTG_OK;
+}
+
+// valid() before nearest applies to plain PRRT with no range limit in an
+// L1/L2 space with a per-state valid() (not the mesh scene's whole-wave
+// check), in dimensions the nearest-neighbour scans serve.
+static bool prevalid_eligible(const mptg_ctx* ctx) {
+    const PrrtState& s = ctx->prrt;
+    const int D = ctx->dim;
+    const bool scan_dim = (D >= 1 && D <= 8) || D == 10 || D == 12 || D == 16 || D == 20;
+    return s.init && !s.star && std::isinf(s.p.range) && s.p.range > 0 && ctx->metric != MPTG_METRIC_SE3 &&
+           ctx->scene.kind != MPTG_SCENE_NONE && ctx->scene.kind != MPTG_SCENE_MESH && scan_dim &&
+           !(ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > D) &&
+           !(ctx->scene.kind != MPTG_SCENE_ARM && D < 2);
+}
+
+// Checks valid() on the tree nodes not yet known valid ([valid_upto, size)):
+// sets valid_upto = size, or some_invalid.  Synchronous.
+static int tree_validity_refresh(mptg_ctx* ctx) {
+    if (ctx->some_invalid || ctx->valid_upto >= ctx->size) return MPTG_OK;
+    PrrtState& s = ctx->prrt;
+    const int64_t first = ctx->valid_upto, n = ctx->size - first;
+    int rc;
+    if ((rc = ensure(ctx, s.vcheck, (size_t)n))) return rc;
+    if ((rc = launch_valid(ctx, ctx->x64 + first * ctx->dim, n, (uint8_t*)s.vcheck.p))) return rc;
+    std::vector<uint8_t> ok((size_t)n);
+    MPTG_HIP(ctx, hipMemcpyAsync(ok.data(), s.vcheck.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint8_t v : ok)
+        if (!v) {
+            ctx->some_invalid = true;
+            return MPTG_OK;
+        }
+    ctx->valid_upto = ctx->size;
     return MPTG_OK;
 }
 
@@ -220,7 +260,8 @@ void prrt_free(mptg_ctx* ctx) {
                      &pb.verdict, &pb.checks,  &pb.counters, &pb.goal_buf, &pb.dnear, &pb.sidx,   &pb.qs,
                      &pb.nbh_idx, &pb.nbh_d,   &pb.marked, &pb.pc,      &pb.cpar,     &pb.cp_from, &pb.cp_to,
                      &pb.cp_ver,  &pb.cp_chk,  &pb.pair_cp, &pb.rw_from, &pb.rw_to,    &pb.rw_ver,  &pb.rw_chk,
-                     &pb.pair_rw, &pb.front,   &pb.pend,   &pb.star_cnt, &pb.blk};
+                     &pb.pair_rw, &pb.front,   &pb.pend,   &pb.star_cnt, &pb.blk,     &pb.qv,      &pb.vidx,
+                     &pb.vcnt,    &pb.kn_i,    &pb.kn_d,   &pb.vcheck};
     for (DevBuf* b : all) {
         (void)hipFree(b->p);
         b->p = nullptr;
@@ -269,8 +310,10 @@ namespace {
 
 // Enqueues one PRRT iteration.  it >= 0: eager (host-known iteration index
 // and tree size); it < 0: graph body (both read from the device counters,
-// the kNN split planned for plan_n nodes).
-int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan_n) {
+// the kNN split planned for plan_n nodes).  pre: valid() before nearest
+// (no range limit, every tree node valid; see prrt_prevalid) -- the eager
+// grid search keeps its own order, so trees it serves are searched after.
+int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan_n, bool pre) {
     PrrtState& s = ctx->prrt;
     PrrtBufs& pb = bufs(ctx);
     const int B = s.p.batch, D = ctx->dim;
@@ -280,15 +323,50 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
     hipLaunchKernelGGL(prrt_sample, dim3(sample_blocks(ctx->metric, D, B)), dim3(kBlock), 0, ctx->stream, pd,
                        ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)pb.q.p);
     MPTG_HIP(ctx, hipGetLastError());
-    if ((rc = launch_knn(ctx, (const double*)pb.q.p, B, 1, __builtin_inf(), (int32_t*)pb.near.p,
-                         (double*)pb.ndist.p, dev ? plan_n : ctx->size, dev ? &dcnt->size : nullptr,
-                         dev ? &dcnt->done : nullptr)))
-        return rc;
-    dispatch_dim(D, [&]<int DD>() {
-        launch_steer<DD>(ctx, s.p.range, B, (const double*)pb.q.p, (const int32_t*)pb.near.p,
-                         (const double*)pb.ndist.p, (double*)pb.from.p, (double*)pb.to.p, (uint8_t*)pb.status.p,
-                         (uint8_t*)pb.mask.p, (double*)nullptr, (const Counters*)dcnt);
-    });
+    if (pre && !dev && knn_grid_applies(ctx, ctx->size)) pre = false;
+    if (pre) {
+        uint32_t* vcnt = (uint32_t*)pb.vcnt.p;
+        if ((rc = launch_zero_u32(ctx, vcnt, 1))) return rc;
+        const bool arm8 = ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks == D && (D == 5 || D == 10 || D == 20);
+        dispatch_dim(D, [&]<int DD>() {
+            if constexpr (DD == 5 || DD == 10 || DD == 20) {
+                if (arm8) {
+                    hipLaunchKernelGGL((prrt_prevalid<DD, kArmLanes>),
+                                       dim3((unsigned)(((int64_t)B * kArmLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                                       ctx->stream, scene_args(ctx), D, B, (const double*)pb.q.p,
+                                       (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p, (double*)pb.qv.p,
+                                       (int32_t*)pb.vidx.p, vcnt, (const Counters*)dcnt);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((prrt_prevalid<DD, 1>), dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               ctx->stream, scene_args(ctx), D, B, (const double*)pb.q.p, (uint8_t*)pb.status.p,
+                               (uint8_t*)pb.mask.p, (double*)pb.qv.p, (int32_t*)pb.vidx.p, vcnt,
+                               (const Counters*)dcnt);
+        });
+        MPTG_HIP(ctx, hipGetLastError());
+        if ((rc = launch_knn(ctx, (const double*)pb.qv.p, B, 1, __builtin_inf(), (int32_t*)pb.kn_i.p,
+                             (double*)pb.kn_d.p, dev ? plan_n : ctx->size, dev ? &dcnt->size : nullptr,
+                             dev ? &dcnt->done : nullptr, vcnt)))
+            return rc;
+        dispatch_dim(D, [&]<int DD>() {
+            hipLaunchKernelGGL((prrt_steer_listed<DD>), dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               ctx->stream, D, B, (const double*)ctx->x64, (const double*)pb.qv.p,
+                               (const int32_t*)pb.vidx.p, (const uint32_t*)vcnt, (const int32_t*)pb.kn_i.p,
+                               (const double*)pb.kn_d.p, (int32_t*)pb.near.p, (double*)pb.from.p, (double*)pb.to.p,
+                               (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p, (const Counters*)dcnt);
+        });
+    } else {
+        if ((rc = launch_knn(ctx, (const double*)pb.q.p, B, 1, __builtin_inf(), (int32_t*)pb.near.p,
+                             (double*)pb.ndist.p, dev ? plan_n : ctx->size, dev ? &dcnt->size : nullptr,
+                             dev ? &dcnt->done : nullptr)))
+            return rc;
+        dispatch_dim(D, [&]<int DD>() {
+            launch_steer<DD>(ctx, s.p.range, B, (const double*)pb.q.p, (const int32_t*)pb.near.p,
+                             (const double*)pb.ndist.p, (double*)pb.from.p, (double*)pb.to.p, (uint8_t*)pb.status.p,
+                             (uint8_t*)pb.mask.p, (double*)nullptr, (const Counters*)dcnt);
+        });
+    }
     MPTG_HIP(ctx, hipGetLastError());
     if (ctx->scene.kind == MPTG_SCENE_MESH &&
         (rc = launch_mesh_filter(ctx, (const double*)pb.to.p, B, (uint8_t*)pb.mask.p, (uint8_t*)pb.status.p,
@@ -312,7 +390,7 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
 // Captures `iters` iterations into an executable graph.  Everything the
 // body allocates must already be allocated: the capture flag turns any
 // growth into an error instead of a synchronising reallocation.
-int capture_iterations(mptg_ctx* ctx, const PrrtDev& pd, int iters, int64_t plan_n, hipGraphExec_t* out) {
+int capture_iterations(mptg_ctx* ctx, const PrrtDev& pd, int iters, int64_t plan_n, bool pre, hipGraphExec_t* out) {
     *out = nullptr;
     const bool timing = ctx->phase_timing;
     ctx->phase_timing = false;
@@ -324,7 +402,7 @@ int capture_iterations(mptg_ctx* ctx, const PrrtDev& pd, int iters, int64_t plan
         ctx->phase_timing = timing;
         return fail(ctx, MPTG_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
     }
-    for (int k = 0; k < iters && rc == MPTG_OK; ++k) rc = enqueue_iteration(ctx, pd, -1, plan_n);
+    for (int k = 0; k < iters && rc == MPTG_OK; ++k) rc = enqueue_iteration(ctx, pd, -1, plan_n, pre);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(ctx->stream, &g);
     ctx->capturing = false;
@@ -375,6 +453,10 @@ int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo,
     }
     s.init = true;
     ctx->coord_bound = std::max(ctx->coord_bound, planner_coord_bound(ctx));
+    if (prevalid_eligible(ctx)) {  // the check the first run would make
+        int rc2 = tree_validity_refresh(ctx);
+        if (rc2) return rc2;
+    }
     s.iteration = 0;
     s.goals = 0;
     s.goal_ids.clear();
@@ -437,6 +519,12 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
         if (out) *out = s.stats;
         return MPTG_OK;
     }
+    // every node appended is valid (the planner checked it); with the whole
+    // tree known valid and no range limit, samples are checked before the
+    // nearest-neighbour search
+    if (prevalid_eligible(ctx) && (rc = tree_validity_refresh(ctx))) return rc;
+    const bool tree_valid = !ctx->some_invalid && ctx->valid_upto == ctx->size;
+    const bool pre = tree_valid && prevalid_eligible(ctx);
     if ((rc = counters_arm(ctx, dcnt, s.iteration, target_size, max_iters))) return rc;
     if ((rc = ensure_pinned(ctx, sizeof(Counters)))) return rc;
     Counters* hc = (Counters*)ctx->pinned;
@@ -462,7 +550,7 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
         if (replay && (rc = tree_reserve(ctx, ctx->size + span))) break;
         if (!replay) {
             if ((rc = tree_reserve(ctx, ctx->size + B))) break;
-            if ((rc = enqueue_iteration(ctx, pd, s.iteration, 0))) break;
+            if ((rc = enqueue_iteration(ctx, pd, s.iteration, 0, pre))) break;
             warm = G > 0;
         } else {
             if (!s.g_exec || s.g_epoch != ctx->epoch || s.g_iters != G || s.g_cap != ctx->cap ||
@@ -470,7 +558,7 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
                 prrt_graph_drop(ctx);
                 const int64_t plan = std::min<int64_t>(ctx->cap, 2 * (ctx->size + span));
                 if ((rc = knn_reserve(ctx, B, 1, plan))) break;
-                if ((rc = capture_iterations(ctx, pd, G, plan, &s.g_exec))) break;
+                if ((rc = capture_iterations(ctx, pd, G, plan, pre, &s.g_exec))) break;
                 s.g_epoch = ctx->epoch;
                 s.g_iters = G;
                 s.g_cap = ctx->cap;
@@ -521,6 +609,7 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
         prrt_graph_drop(ctx);
         return rc;
     }
+    if (tree_valid) ctx->valid_upto = ctx->size;
     if (out) *out = s.stats;
     return MPTG_OK;
 }

@@ -188,6 +188,94 @@ __global__ __launch_bounds__(kBlock) void prrt_steer_arm(SpaceArgs sp, SceneArgs
     }
 }
 
+// ---- valid() before nearest (PRRT with no range limit) ---------------------
+// With an infinite range the new state is the sample itself
+// (prrt.hpp:430-434 never interpolates), so valid(new) does not depend on
+// the nearest node and an invalid sample's outcome is fixed before the
+// search: rejected as invalid, unless it is at distance 0 from a tree node
+// (:427), which needs a tree node equal to it, hence an invalid tree node.
+// The host takes this path only while every tree node is known valid; then
+// only the valid samples are searched.  prrt_prevalid checks each sample
+// (L lanes per sample: the arm's split disk tests, else one), marks invalid
+// ones and lists the valid ones compactly (queries qv, sample ids vidx,
+// count *vcnt; list order is immaterial: every query is independent).
+template <int D, int L>
+__global__ __launch_bounds__(kBlock) void prrt_prevalid(SceneArgs sc, int dim, int B, const double* __restrict__ q,
+                                                        uint8_t* __restrict__ status, uint8_t* __restrict__ linkmask,
+                                                        double* __restrict__ qv, int32_t* __restrict__ vidx,
+                                                        uint32_t* __restrict__ vcnt,
+                                                        const Counters* __restrict__ cnt) {
+    constexpr int DA = D > 0 ? D : kMaxDim;
+    const int dd = D > 0 ? D : dim;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int b = (int)(t / L);
+    const bool lead = (t % L) == 0;
+    const bool inb = b < B;
+    if (cnt->done) {  // nothing to link in an iteration past the stop
+        if (inb && lead) linkmask[b] = 0;
+        return;
+    }
+    double s[DA];
+#pragma unroll
+    for (int i = 0; i < DA; ++i) s[i] = (inb && i < dd) ? q[(int64_t)b * dd + i] : 0.0;
+    bool ok;
+    if constexpr (L > 1) ok = arm_ok_split<D, L>(sc, [&](int i) { return s[i]; }, inb);
+    else ok = inb && state_valid_t<D>(sc, s);
+    const bool list = inb && lead && ok;
+    if (inb && lead && !ok) {
+        status[b] = ST_INVALID;
+        linkmask[b] = 0;
+    }
+    // one atomic per wave for its listed samples, ranks by ballot
+    const uint64_t m = __ballot(list);
+    const int lane = (int)(threadIdx.x & 63);
+    uint32_t base = 0;
+    if (m) {
+        if (lane == __builtin_ctzll(m)) base = atomicAdd(vcnt, (uint32_t)__popcll(m));
+        base = __shfl(base, __builtin_ctzll(m));
+    }
+    if (list) {
+        const uint32_t k = base + (uint32_t)__popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull));
+        vidx[k] = b;
+#pragma unroll
+        for (int i = 0; i < DA; ++i)
+            if (i < dd) qv[(int64_t)k * dd + i] = s[i];
+    }
+}
+
+// The steer step for the listed samples (k < *vcnt): sample b = vidx[k] with
+// its nearest node kn_i[k] at kn_d[k]; with no range limit the new state is
+// the sample, already valid.  Writes what prrt_steer writes (and near[b]).
+template <int D>
+__global__ __launch_bounds__(kBlock) void prrt_steer_listed(int dim, int B, const double* __restrict__ x64,
+                                                            const double* __restrict__ qv,
+                                                            const int32_t* __restrict__ vidx,
+                                                            const uint32_t* __restrict__ vcnt,
+                                                            const int32_t* __restrict__ kn_i,
+                                                            const double* __restrict__ kn_d, int32_t* __restrict__ near,
+                                                            double* __restrict__ from, double* __restrict__ to,
+                                                            uint8_t* __restrict__ status, uint8_t* __restrict__ linkmask,
+                                                            const Counters* __restrict__ cnt) {
+    const int dd = D > 0 ? D : dim;
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cnt->done || k >= (int64_t)*vcnt || k >= B) return;
+    const int b = vidx[k];
+    const double d = kn_d[k];
+    const int32_t nn = kn_i[k];
+    near[b] = nn;
+    if (d == 0 || nn < 0) {  // prrt.hpp:427-428
+        status[b] = ST_ZERO;
+        linkmask[b] = 0;
+        return;
+    }
+    for (int i = 0; i < dd; ++i) {
+        from[(int64_t)b * dd + i] = x64[(int64_t)nn * dd + i];
+        to[(int64_t)b * dd + i] = qv[k * dd + i];
+    }
+    status[b] = ST_LINK_FAIL;  // until link() says otherwise
+    linkmask[b] = 1;
+}
+
 // Launches the steer kernel for a batch of B samples (arm scenes with
 // D = nlinks in {5, 10, 20}: the split-lane kernel).
 template <int D>

@@ -258,6 +258,34 @@ def test_prrt_tree_parity_arm(ctx_factory):
     _planner_parity(ctx_factory(), scenes.arm_scene(), seed=13, batch=256, target=400)
 
 
+@pytest.mark.parametrize("kind,batch,target", [("boxes", 512, 3000), ("grid", 512, 4000), ("boxes", 1, 400)])
+def test_prrt_unlimited_range(ctx_factory, grid1024, kind, batch, target):
+    """No range limit: the planner checks valid() before the nearest-
+    neighbour search (only valid samples are searched); trees and every
+    statistic equal the oracle's, which searches first."""
+    _say("prrt unlimited range", kind, batch)
+    import dataclasses
+    sc = scenes.boxes_scene() if kind == "boxes" else grid1024
+    _planner_parity(ctx_factory(), dataclasses.replace(sc, range=math.inf), seed=14, batch=batch, target=target)
+
+
+def test_prrt_invalid_start_keeps_search_first(ctx_factory):
+    """A start node inside an obstacle (just inside a box's corner, so that
+    edges leaving the box still pass link(), whose first waypoint is one step
+    away): valid() can no longer be taken before the search (a sample equal
+    to an invalid node counts as distance 0), so the planner searches first;
+    the tree equals the oracle's."""
+    _say("prrt invalid start")
+    import dataclasses
+    sc = scenes.boxes_scene()
+    b = sc.boxes[0]
+    inside = np.array([b[0] + 1e-6, b[1] + 1e-6])
+    sc = dataclasses.replace(sc, start=inside, range=math.inf)
+    assert not oracle.Oracle(sc).valid(inside)[0]
+    st = _planner_parity(ctx_factory(), sc, seed=15, batch=256, target=1500, max_iters=400)
+    assert st["size"] >= 1500
+
+
 def test_prrt_after_init_then_clear(ctx_factory):
     """Planner initialised, then the tree cleared and re-seeded: the nearest-
     neighbour prefilter keeps the planner's sampling box in its coordinate
