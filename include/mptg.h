@@ -103,9 +103,19 @@ int mptg_tree_read(mptg_ctx* ctx, int64_t first, int64_t n, double* h_states, in
  *      (src/mpt/impl/rrg_rewire_neighbors.hpp:66,127).
  *      For each query: the k tree nodes with the smallest distance <= radius,
  *      ascending by (distance, node id); distances are the metric's FP64
- *      value.  Missing slots: idx -1, dist +inf.  k in 1..32; radius +inf for
- *      plain kNN. */
+ *      value.  Missing slots: idx -1, dist +inf.  k in 1..64; radius +inf for
+ *      plain kNN (mptg_rnn below returns every node within a radius). */
 int mptg_knn(mptg_ctx* ctx, const double* h_q, int64_t nq, int k, double radius,
+             int32_t* h_idx, double* h_dist);
+/* Every tree node within `radius` of each query, ascending by (distance,
+ * node id): nigh's nearest(nbh, q, SIZE_MAX, r), the call behind
+ * rewire_r_nearest (src/mpt/impl/rrg_rewire_neighbors.hpp:124-128), with no
+ * limit on the list length.  Query i's neighbours are entries
+ * [h_offsets[i], h_offsets[i + 1]) of h_idx / h_dist; h_offsets holds nq + 1
+ * entries.  When the lists need more than `cap` entries the call fills
+ * h_offsets only and returns MPTG_E_RANGE (call again with h_offsets[nq]
+ * entries).  At most 2^24 queries per call. */
+int mptg_rnn(mptg_ctx* ctx, const double* h_q, int64_t nq, double radius, int64_t cap, int64_t* h_offsets,
              int32_t* h_idx, double* h_dist);
 /* Device buffers, on the context stream.  On trees of 2^16 nodes or more the
  * uniform grid (L1/L2 in 2..8 dimensions, knn_grid on) and the select path
@@ -222,12 +232,22 @@ int mptg_prrt_goals(mptg_ctx* ctx, int32_t* h_ids, int64_t cap, int64_t* n);
  *      mptg_prrt_star_init, mptg_prrt_add_start / _run / _stats_get / _goals
  *      drive the PRRT* loop; the tree must be grown from mptg_prrt_add_start
  *      (nodes from mptg_tree_insert carry no costs: MPTG_E_STATE). */
+enum mptg_rewire_neighbors {
+    MPTG_REWIRE_K_NEAREST = 0, /* rewire_k_nearest  planner_tags.hpp:49, rrg_rewire_neighbors.hpp:46-92 */
+    MPTG_REWIRE_R_NEAREST = 1  /* rewire_r_nearest  planner_tags.hpp:50, rrg_rewire_neighbors.hpp:94-152 */
+};
+
 typedef struct {
-    int32_t k;             /* neighbours per survivor, 1..32; 0 = the reference's
-                              ceil(kRRG * ln(n + 1)), kRRG = factor * e * (1 + 1/dim)
-                              (rrg_rewire_neighbors.hpp:46-67), n = tree size at the
-                              iteration start; MPTG_E_RANGE once that exceeds 32 */
-    int32_t _pad;
+    int32_t k;             /* rewire_k_nearest: neighbours per survivor, 1..64; 0 = the
+                              reference's ceil(kRRG * ln(n + 1)), kRRG = factor * e *
+                              (1 + 1/d) (rrg_rewire_neighbors.hpp:46-67), n = tree size
+                              at the iteration start, d = the space's dimension (6 for
+                              SE(3)); MPTG_E_RANGE once that exceeds 64 (n around 10^8 at
+                              d = 7, 1.6 * 10^6 at d = 2).  Ignored for r-nearest. */
+    int32_t neighbors;     /* mptg_rewire_neighbors; rewire_r_nearest takes every node
+                              within rRRG (ln(n + 1) / (n + 1))^(1/d) (:99-128) and
+                              returns MPTG_E_RANGE at an iteration where a survivor has
+                              more than 63 of them */
     double rewire_factor;  /* setRewireFactor (default 1.1, prrt_star.hpp:176,224) */
 } mptg_prrt_star_params;
 

@@ -201,6 +201,24 @@ class Context:
         self._chk(_lib.mptg_knn(self._h, _p(q, C.c_double), n, k, radius, _p(idx, C.c_int32), _p(dist, C.c_double)))
         return idx, dist
 
+    def rnn(self, q, radius: float, cap: int = 1 << 16):
+        """Every tree node within `radius` of each query, nearest first:
+        (offsets[nq + 1], ids, distances); query i's neighbours are
+        ids[offsets[i]:offsets[i + 1]] (mptg_rnn, count then fill)."""
+        q = _f64(q, self.dim)
+        n = q.shape[0]
+        off = np.zeros(n + 1, dtype=np.int64)
+        while True:
+            idx = np.zeros(max(cap, 1), dtype=np.int32)
+            dist = np.zeros(max(cap, 1))
+            rc = _lib.mptg_rnn(self._h, _p(q, C.c_double), n, radius, cap, _p(off, C.c_int64),
+                               _p(idx, C.c_int32), _p(dist, C.c_double))
+            if rc == -4 and off[n] > cap:  # MPTG_E_RANGE: offsets filled, arrays too small
+                cap = int(off[n])
+                continue
+            self._chk(rc)
+            return off, idx[: off[n]].copy(), dist[: off[n]].copy()
+
     def knn_dev(self, q_ptr: int, nq: int, k: int, radius: float, idx_ptr: int, dist_ptr: int):
         self._chk(_lib.mptg_knn_dev(self._h, C.c_void_p(q_ptr), nq, k, radius, C.c_void_p(idx_ptr),
                                     C.c_void_p(dist_ptr)))
@@ -339,11 +357,13 @@ class Context:
 
     def prrt_star_init(self, sc: Scene, seed: int, batch: int, k: int = 20, rewire_factor: float = 1.1,
                        order: int = ORDER_REFERENCE, range_: Optional[float] = None,
-                       goal_bias: Optional[float] = None):
+                       goal_bias: Optional[float] = None, r_nearest: bool = False):
+        """PRRT*: k neighbours per new node (0: the reference's k formula), or
+        with r_nearest every node within the reference's rewire radius."""
         p = PrrtParams(seed=seed, range=sc.range if range_ is None else range_,
                        goal_bias=sc.goal_bias if goal_bias is None else goal_bias, goal_radius=sc.goal_radius,
                        batch=batch, order=order)
-        sp = PrrtStarParams(k=k, rewire_factor=rewire_factor)
+        sp = PrrtStarParams(k=k, neighbors=1 if r_nearest else 0, rewire_factor=rewire_factor)
         lo, hi, g = _f64(sc.lo), _f64(sc.hi), _f64(sc.goal)
         self._chk(_lib.mptg_prrt_star_init(self._h, C.byref(p), C.byref(sp), _p(lo, C.c_double), _p(hi, C.c_double),
                                            _p(g, C.c_double)))

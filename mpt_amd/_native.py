@@ -46,7 +46,7 @@ class TuneReport(C.Structure):
 
 
 class PrrtStarParams(C.Structure):
-    _fields_ = [("k", C.c_int32), ("_pad", C.c_int32), ("rewire_factor", C.c_double)]
+    _fields_ = [("k", C.c_int32), ("neighbors", C.c_int32), ("rewire_factor", C.c_double)]
 
 
 class PrrtStats(C.Structure):
@@ -81,6 +81,8 @@ _SIGNATURES = [
     ("mptg_tree_read", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, c_double_p, c_int32_p]),
     ("mptg_knn", C.c_int, [C.c_void_p, c_double_p, C.c_int64, C.c_int, C.c_double, c_int32_p, c_double_p]),
     ("mptg_knn_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
+    ("mptg_rnn", C.c_int, [C.c_void_p, c_double_p, C.c_int64, C.c_double, C.c_int64, C.POINTER(C.c_int64),
+                           C.POINTER(C.c_int32), c_double_p]),
     ("mptg_scene_boxes", C.c_int, [C.c_void_p, c_double_p, C.c_int]),
     ("mptg_scene_grid", C.c_int, [C.c_void_p, c_uint8_p, C.c_int, C.c_int]),
     ("mptg_scene_arm", C.c_int, [C.c_void_p, c_double_p, C.c_int, C.c_double, c_double_p, C.c_int]),

@@ -44,7 +44,7 @@ HIP_FLAGS = [
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall"]
 
-LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "prrt_star.hip", "mesh.hip", "diag.hip", "arm.hip", "mlp.hip", "tune.hip", "knn_grid.hip"]
+LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "prrt_star.hip", "mesh.hip", "diag.hip", "arm.hip", "mlp.hip", "tune.hip", "knn_grid.hip", "rnn.hip"]
 ORACLE_SOURCES = ["orc_core.cpp", "orc_mesh.cpp", "orc_prrt.cpp", "orc_prrt_star.cpp", "orc_pool.cpp", "orc_mlp.cpp"]
 
 

@@ -199,6 +199,10 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.mesh_robot.p);
     for (auto& b : ctx->scratch) (void)hipFree(b.p);
     (void)hipFree(ctx->dmv_table16.p);
+    (void)hipFree(ctx->rnn_buf.p);
+    (void)hipFree(ctx->rnn_pairs.p);
+    (void)hipFree(ctx->rnn_q.p);
+    (void)hipFree(ctx->rnn_out.p);
     for (auto& b : ctx->knn_sel) (void)hipFree(b.p);
     for (auto& b : ctx->knn_grid) (void)hipFree(b.p);
     prrt_free(ctx);
@@ -571,7 +575,7 @@ int mptg_knn(mptg_ctx* ctx, const double* h_q, int64_t nq, int k, double radius,
     int rc = check_device(ctx);
     if (rc) return rc;
     if (nq < 0 || (nq > 0 && (!h_q || !h_idx || !h_dist))) return fail(ctx, MPTG_E_ARG, "bad knn buffers");
-    if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..32");
+    if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..64");
     if (nq == 0) return MPTG_OK;
     int d = ctx->dim;
     size_t qb = sizeof(double) * d * nq, ib = sizeof(int32_t) * k * nq, db = sizeof(double) * k * nq;
@@ -593,7 +597,7 @@ int mptg_knn_dev(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double rad
     int rc = check_device(ctx);
     if (rc) return rc;
     if (nq < 0 || (nq > 0 && (!d_q || !d_idx || !d_dist))) return fail(ctx, MPTG_E_ARG, "bad knn buffers");
-    if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..32");
+    if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_ARG, "k must be in 1..64");
     if (nq == 0) return MPTG_OK;
     return launch_knn(ctx, d_q, nq, k, radius, d_idx, d_dist, ctx->size);
 }

@@ -15,7 +15,7 @@ namespace mptg {
 struct Surrogate;
 
 constexpr int kMaxDim = 32;
-constexpr int kMaxK = 32;
+constexpr int kMaxK = 64;
 
 // Device buffer owned by a context.  Grows, never shrinks.
 struct DevBuf {
@@ -72,9 +72,12 @@ struct PrrtState {
     // PRRT*
     bool star = false;
     int32_t star_k = 20;          // 0: the reference's k(n) formula
+    int32_t star_mode = MPTG_REWIRE_K_NEAREST;  // or MPTG_REWIRE_R_NEAREST
     double rewire_factor = 1.1;
-    DevBuf dnear, sidx, qs, nbh_idx, nbh_d, marked, pc, cpar, cp_from, cp_to, cp_ver, cp_chk, pair_cp, rw_from,
-        rw_to, rw_ver, rw_chk, pair_rw, front, pend, star_cnt;
+    // neighbour lists: k-nearest rows of star_k slots, or radius lists in
+    // CSR form (nbh_off per survivor, nbh_own = survivor of each entry)
+    DevBuf dnear, sidx, qs, nbh_idx, nbh_d, nbh_off, nbh_own, marked, pc, cpar, cp_from, cp_to, cp_ver, cp_chk,
+        pair_cp, rw_from, rw_to, rw_ver, rw_chk, pair_rw, front, pend, star_cnt;
 };
 
 }  // namespace mptg
@@ -121,6 +124,9 @@ struct mptg_ctx {
     // results, query cells and order
     mptg::DevBuf knn_grid[6];
     mptg::DevBuf dmv_table16;  // the DMV ring order for 256..1023 interior waypoints (uint16 rows)
+    // radius search: list offsets, scanned offsets, counts, scan scratch;
+    // unsorted pairs and sort scratch; staged host queries; sorted results
+    mptg::DevBuf rnn_buf, rnn_pairs, rnn_q, rnn_out;
     mptg::DevBuf scratch[12];  // 0-2 host staging, 3 kNN seed bounds, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
                                // 9 ranked ordinals, 10 continuation list
     // pinned host staging
@@ -197,6 +203,16 @@ int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n);
 bool knn_grid_applies(const mptg_ctx* ctx, int64_t n);
 int knn_grid_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx, double* d_dist,
                   int64_t n, int64_t** over, uint32_t* m);
+// Radius search on device-resident queries (rnn.hip), over tree nodes
+// [0, ctx->size): rnn_count counts every query's nodes within r and returns
+// the total (one 8-byte readback); rnn_offsets is then the device array of
+// nq + 1 list offsets; rnn_fill writes the lists, each in ascending
+// (distance, id) order, and (if owner is not null) the query of each entry.
+int rnn_count(mptg_ctx* ctx, const double* d_q, int64_t nq, double r, int64_t* total);
+const int64_t* rnn_offsets(const mptg_ctx* ctx);
+int rnn_fill(mptg_ctx* ctx, const double* d_q, int64_t nq, double r, int64_t total, double* d_dist, int32_t* d_idx,
+             int32_t* d_owner);
+bool rnn_dim_ok(int dim);
 // Zeroes n 32-bit words with a kernel (graph-capturable, unlike a memset).
 int launch_zero_u32(mptg_ctx* ctx, uint32_t* p, int64_t n);
 // Zeroes the 32-bit words *a and *b with one small kernel.

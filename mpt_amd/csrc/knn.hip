@@ -45,6 +45,9 @@ constexpr int64_t kMinSplitNodes = 64;
 // (a planner's first nodes crowd around its start).
 constexpr int64_t kSeedNodes = 256;
 
+// Register list length of a k-list scan: k rounded up to 1, 20, 32 or 64.
+inline int k_bucket(int k) { return k <= 1 ? 1 : (k <= 20 ? 20 : (k <= 32 ? 32 : 64)); }
+
 __device__ __forceinline__ bool better(double a, int ia, double b, int ib) {
     return a < b || (a == b && ia < ib);
 }
@@ -743,7 +746,7 @@ static int64_t plan_splits(mptg_ctx* ctx, int64_t nq, int64_t n, int64_t* chunk,
 }
 
 int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
-    const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
+    const int K = k_bucket(k);
     int64_t chunk;
     // k = 1 launches with a device query count plan for half the slots,
     // which never takes fewer splits
@@ -758,7 +761,7 @@ int knn_reserve(mptg_ctx* ctx, int64_t nq, int k, int64_t tree_n) {
 static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
                          double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip,
                          const uint32_t* nq_dev = nullptr) {
-    const int K = k <= 1 ? 1 : (k <= 20 ? 20 : 32);
+    const int K = k_bucket(k);
     // with n_dev, tree_n is only the size the split count is planned for;
     // with nq_dev (live queries on the device, k = 1) the split is planned
     // for half of the nq slots
@@ -782,18 +785,21 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     bool ok;
     if (K == 1) ok = dispatch_dim<1>(ctx, grid, ext, d_q, nq, radius, pd, pi);
     else if (K == 20) ok = dispatch_dim<20>(ctx, grid, ext, d_q, nq, radius, pd, pi);
-    else ok = dispatch_dim<32>(ctx, grid, ext, d_q, nq, radius, pd, pi);
+    else if (K == 32) ok = dispatch_dim<32>(ctx, grid, ext, d_q, nq, radius, pd, pi);
+    else ok = dispatch_dim<64>(ctx, grid, ext, d_q, nq, radius, pd, pi);
     if (!ok) return fail(ctx, MPTG_E_ARG, "unsupported dimension for the nearest-neighbour kernel");
     MPTG_HIP(ctx, hipGetLastError());
     dim3 g1((unsigned)((nq + kBlock - 1) / kBlock));
     if (splits > 1) {
         if (K == 1) hipLaunchKernelGGL(knn_merge1, dim3((unsigned)((nq * kMergeLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, d_dist, d_idx, skip, n_dev, std::numeric_limits<double>::infinity(), nq_dev);
         else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
-        else hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
+        else if (K == 32) hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
+        else hipLaunchKernelGGL((knn_merge<64>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
     } else {
         if (K == 1) hipLaunchKernelGGL((knn_finish<1>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
         else if (K == 20) hipLaunchKernelGGL((knn_finish<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
-        else hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
+        else if (K == 32) hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
+        else hipLaunchKernelGGL((knn_finish<64>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
     }
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
@@ -929,7 +935,8 @@ int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radiu
     }
     const bool sel_dim = ctx->dim <= 8 || ctx->dim == 10 || ctx->dim == 20;
     const bool eager = nq > 0 && !n_dev && !skip && !ctx->capturing;
-    const bool select = eager && k > 1 && tree_n >= kSelectMinTree && ctx->tune.knn_select &&
+    // (the select path keeps lists of up to 32; larger k takes the k-list scan)
+    const bool select = eager && k > 1 && k <= 32 && tree_n >= kSelectMinTree && ctx->tune.knn_select &&
                         ctx->metric != MPTG_METRIC_SE3 && ctx->dim >= 2 && sel_dim;
     if (eager && knn_grid_applies(ctx, tree_n)) {
         // the grid answers most queries; the rest take the scans, gathered

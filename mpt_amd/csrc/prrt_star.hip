@@ -41,7 +41,7 @@ namespace mptg {
 
 namespace {
 
-constexpr int kStarMaxK = 32;
+constexpr int kStarMaxK = 64;
 
 struct StarCounters {
     int64_t m;          // survivors of the current iteration
@@ -131,36 +131,51 @@ __global__ __launch_bounds__(1024) void star_select(int B, int D, const uint8_t*
     }
 }
 
-// Per-lane view of one survivor's neighbour list (lane i = i-th nearest).
-struct NbhLane {
-    bool valid;
-    int32_t nb;
-    double d, key;  // distance to the new state, cost(nb) + d
-    int rank;       // position in ascending (key, lane) order among valid lanes
+// Survivor j's neighbour list: entries [lo, hi) of nbh_idx / nbh_d, in
+// ascending (distance, id) order; rows of k slots (unused slots hold -1) or,
+// with offsets, the CSR lists of a radius search (any length).
+struct NbhList {
+    int64_t lo, hi;
 };
 
-__device__ __forceinline__ NbhLane nbh_lane(int k, int64_t j, int lane, const int32_t* __restrict__ nbh_idx,
-                                            const double* __restrict__ nbh_d, const double* cost) {
-    NbhLane l;
-    l.nb = lane < k ? nbh_idx[j * k + lane] : -1;
-    l.valid = l.nb >= 0;
-    l.d = l.valid ? nbh_d[j * k + lane] : 0.0;
-    l.key = l.valid ? cost[l.nb] + l.d : 0.0;
-    l.rank = 0;
-    for (int o = 0; o < k; ++o) {
-        const double ko = __shfl(l.key, o);
-        const int vo = __shfl((int)l.valid, o);
-        l.rank += (vo && (ko < l.key || (ko == l.key && o < lane))) ? 1 : 0;
-    }
-    return l;
+__device__ __forceinline__ NbhList nbh_list(const int64_t* __restrict__ off, int k, int64_t j) {
+    if (off) return {off[j], off[j + 1]};
+    return {j * k, j * k + k};
 }
 
+__device__ __forceinline__ int64_t nbh_owner(const int32_t* __restrict__ own, int k, int64_t e) {
+    return own ? (int64_t)own[e] : e / k;
+}
+
+// Lexicographic (key, entry) minimum over the wave; entries with no
+// candidate carry (+inf, INT64_MAX).
+__device__ __forceinline__ void wave_min_key(double& key, int64_t& e) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ko = __shfl_xor(key, o);
+        const int64_t eo = __shfl_xor(e, o);
+        if (ko < key || (ko == key && eo < e)) {
+            key = ko;
+            e = eo;
+        }
+    }
+}
+
+__device__ __forceinline__ bool key_before(double k1, int64_t e1, double k2, int64_t e2) {
+    return k1 < k2 || (k1 == k2 && e1 < e2);
+}
+
+// The walk (:580-605) visits the list in ascending (cost(nb) + d, position)
+// order: keys are compared with the list position (here the entry index,
+// which grows with it) breaking ties, exactly as a stable sort on the key.
+
 // ---- choose parent, pass 1: queue the walk's link checks -----------------
-// The sequential walk (:580-605) visits ranks 0, 1, ... and stops at the first
-// neighbour whose key exceeds the parent cost, at the near node, or at the
-// first passing link.  Every neighbour ranked before the first of the first
-// two stops may be reached, so all of their links are queued.
-__global__ __launch_bounds__(256) void star_cp_prepare(int D, int k, int64_t m, const int32_t* __restrict__ sidx,
+// The sequential walk stops at the first neighbour whose key exceeds the
+// parent cost, at the near node, or at the first passing link.  Every
+// neighbour before the first of the first two stops may be reached, so all
+// of their links are queued.  One wave per survivor, 64 entries per page.
+__global__ __launch_bounds__(256) void star_cp_prepare(int D, int k, int64_t m, const int64_t* __restrict__ off,
+                                                       const int32_t* __restrict__ sidx,
                                                        const int32_t* __restrict__ near,
                                                        const double* __restrict__ dnear,
                                                        const double* __restrict__ qs,
@@ -175,27 +190,47 @@ __global__ __launch_bounds__(256) void star_cp_prepare(int D, int k, int64_t m, 
     const int b = sidx[j];
     const int32_t nr = near[b];
     const double pc0 = cost[nr] + dnear[b];
-    const NbhLane l = nbh_lane(k, j, lane, nbh_idx, nbh_d, cost);
-    if (__ballot(l.valid && l.nb == nr && l.key != pc0) && lane == 0) atomicOr(&sc->bad_distance, 1);
-    const int rs = wave_min(l.valid && (l.key > pc0 || l.nb == nr) ? l.rank : 64);
-    const bool cand = l.valid && l.rank < rs;
-    const uint64_t cm = __ballot(cand);
-    unsigned long long base = 0;
-    if (lane == 0 && cm) base = atomicAdd((unsigned long long*)&sc->ncp, (unsigned long long)__popcll(cm));
-    base = __shfl(base, 0);
-    if (lane < k) pair_cp[j * k + lane] = -1;
-    if (cand) {
-        const int64_t e = (int64_t)base + __popcll(cm & lanes_below(lane));
-        pair_cp[j * k + lane] = (int32_t)e;
-        for (int i = 0; i < D; ++i) {
-            cp_from[e * D + i] = x64[(int64_t)l.nb * D + i];
-            cp_to[e * D + i] = qs[j * D + i];
+    const NbhList L = nbh_list(off, k, j);
+    double sk = __builtin_inf();
+    int64_t se = INT64_MAX;
+    bool bad = false;
+    for (int64_t e0 = L.lo; e0 < L.hi; e0 += 64) {
+        const int64_t e = e0 + lane;
+        const int32_t nb = e < L.hi ? nbh_idx[e] : -1;
+        if (nb >= 0) {
+            const double key = cost[nb] + nbh_d[e];
+            bad |= nb == nr && key != pc0;
+            if ((key > pc0 || nb == nr) && key_before(key, e, sk, se)) {
+                sk = key;
+                se = e;
+            }
+        }
+    }
+    wave_min_key(sk, se);
+    if (__ballot(bad) && lane == 0) atomicOr(&sc->bad_distance, 1);
+    for (int64_t e0 = L.lo; e0 < L.hi; e0 += 64) {
+        const int64_t e = e0 + lane;
+        const int32_t nb = e < L.hi ? nbh_idx[e] : -1;
+        const bool cand = nb >= 0 && key_before(cost[nb] + nbh_d[e], e, sk, se);
+        const uint64_t cm = __ballot(cand);
+        unsigned long long base = 0;
+        if (lane == 0 && cm) base = atomicAdd((unsigned long long*)&sc->ncp, (unsigned long long)__popcll(cm));
+        base = __shfl(base, 0);
+        if (e < L.hi) pair_cp[e] = -1;
+        if (cand) {
+            const int64_t q = (int64_t)base + __popcll(cm & lanes_below(lane));
+            pair_cp[e] = (int32_t)q;
+            for (int i = 0; i < D; ++i) {
+                cp_from[q * D + i] = x64[(int64_t)nb * D + i];
+                cp_to[q * D + i] = qs[j * D + i];
+            }
         }
     }
 }
 
 // ---- choose parent, pass 2: the walk's outcome; queue rewire checks --------
-__global__ __launch_bounds__(256) void star_cp_resolve(int D, int k, int64_t m, const int32_t* __restrict__ sidx,
+__global__ __launch_bounds__(256) void star_cp_resolve(int D, int k, int64_t m, const int64_t* __restrict__ off,
+                                                       const int32_t* __restrict__ sidx,
                                                        const int32_t* __restrict__ near,
                                                        const double* __restrict__ dnear,
                                                        const double* __restrict__ qs,
@@ -204,7 +239,7 @@ __global__ __launch_bounds__(256) void star_cp_resolve(int D, int k, int64_t m, 
                                                        const double* __restrict__ x64, const double* cost,
                                                        const int32_t* __restrict__ pair_cp,
                                                        const uint8_t* __restrict__ cp_ver, int32_t* __restrict__ cpar,
-                                                       double* __restrict__ pc_out, uint32_t* __restrict__ marked,
+                                                       double* __restrict__ pc_out, uint8_t* __restrict__ marked,
                                                        double* __restrict__ rw_from, double* __restrict__ rw_to,
                                                        int32_t* __restrict__ pair_rw, StarCounters* sc) {
     const int lane = threadIdx.x & 63;
@@ -213,54 +248,76 @@ __global__ __launch_bounds__(256) void star_cp_resolve(int D, int k, int64_t m, 
     const int b = sidx[j];
     const int32_t nr = near[b];
     const double pc0 = cost[nr] + dnear[b];
-    const NbhLane l = nbh_lane(k, j, lane, nbh_idx, nbh_d, cost);
-    const int32_t e = l.valid ? pair_cp[j * k + lane] : -1;
-    const bool pass = e >= 0 && cp_ver[e] != 0;
-    const bool stop = l.valid && (l.key > pc0 || l.nb == nr || pass);
-    const int rs = wave_min(stop ? l.rank : 64);
-    // the stopping lane (ranks are distinct) and why it stops:
-    // 1 = cost bound (not visited), 2 = near node, 3 = passing link
-    const uint64_t sm = __ballot(l.valid && stop && l.rank == rs);
-    const int src = sm ? __ffsll((unsigned long long)sm) - 1 : 0;
-    const int kind = l.key > pc0 ? 1 : (l.nb == nr ? 2 : 3);
-    const int skind = sm ? __shfl(kind, src) : 0;
-    const int32_t snb = __shfl(l.nb, src);
-    const double skey = __shfl(l.key, src);
-    int32_t par = nr;
-    double pc = pc0;
-    if (skind == 2) {
-        pc = skey;
-    } else if (skind == 3) {
-        par = snb;
-        pc = skey;
-    }
-    const bool mk = l.valid && (l.rank < rs || (l.rank == rs && skind != 1));
-    const uint64_t mm = __ballot(mk);
-    // rewire candidates under the iteration's costs; costs only fall during
-    // the rewire pass, so every pair it rewires while cost(new) = pc is here
-    const bool rw = l.valid && !mk && (pc + l.d < cost[l.nb]);
-    const uint64_t rm = __ballot(rw);
-    unsigned long long base = 0;
-    if (lane == 0 && rm) base = atomicAdd((unsigned long long*)&sc->nrw, (unsigned long long)__popcll(rm));
-    base = __shfl(base, 0);
-    if (lane < k) pair_rw[j * k + lane] = -1;
-    if (rw) {
-        const int64_t r = (int64_t)base + __popcll(rm & lanes_below(lane));
-        pair_rw[j * k + lane] = (int32_t)r;
-        for (int i = 0; i < D; ++i) {
-            rw_from[r * D + i] = qs[j * D + i];
-            rw_to[r * D + i] = x64[(int64_t)l.nb * D + i];
+    const NbhList L = nbh_list(off, k, j);
+    // the first stop in walk order
+    double sk = __builtin_inf();
+    int64_t se = INT64_MAX;
+    for (int64_t e0 = L.lo; e0 < L.hi; e0 += 64) {
+        const int64_t e = e0 + lane;
+        const int32_t nb = e < L.hi ? nbh_idx[e] : -1;
+        if (nb >= 0) {
+            const double key = cost[nb] + nbh_d[e];
+            const int32_t q = pair_cp[e];
+            const bool pass = q >= 0 && cp_ver[q] != 0;
+            if ((key > pc0 || nb == nr || pass) && key_before(key, e, sk, se)) {
+                sk = key;
+                se = e;
+            }
         }
     }
-    const uint64_t vm = __ballot(l.valid);
-    const uint64_t em = __ballot(e >= 0);
+    wave_min_key(sk, se);
+    // why it stops: 1 = cost bound (not visited), 2 = near node, 3 = passing link
+    int skind = 0;
+    int32_t par = nr;
+    double pc = pc0;
+    if (se != INT64_MAX) {
+        const int32_t snb = nbh_idx[se];
+        skind = sk > pc0 ? 1 : (snb == nr ? 2 : 3);
+        if (skind == 2) {
+            pc = sk;
+        } else if (skind == 3) {
+            par = snb;
+            pc = sk;
+        }
+    }
+    unsigned long long nv = 0, ncp = 0, nrw = 0;
+    for (int64_t e0 = L.lo; e0 < L.hi; e0 += 64) {
+        const int64_t e = e0 + lane;
+        const int32_t nb = e < L.hi ? nbh_idx[e] : -1;
+        const bool valid = nb >= 0;
+        const double d = valid ? nbh_d[e] : 0.0;
+        const double cnb = valid ? cost[nb] : 0.0;
+        const bool mk = valid && (key_before(cnb + d, e, sk, se) || (e == se && skind != 1));
+        // rewire candidates under the iteration's costs; costs only fall
+        // during the rewire pass, so every pair it rewires while
+        // cost(new) = pc is here
+        const bool rw = valid && !mk && (pc + d < cnb);
+        const uint64_t rm = __ballot(rw);
+        unsigned long long base = 0;
+        if (lane == 0 && rm) base = atomicAdd((unsigned long long*)&sc->nrw, (unsigned long long)__popcll(rm));
+        base = __shfl(base, 0);
+        if (e < L.hi) {
+            marked[e] = mk ? 1 : 0;
+            pair_rw[e] = -1;
+        }
+        if (rw) {
+            const int64_t r = (int64_t)base + __popcll(rm & lanes_below(lane));
+            pair_rw[e] = (int32_t)r;
+            for (int i = 0; i < D; ++i) {
+                rw_from[r * D + i] = qs[j * D + i];
+                rw_to[r * D + i] = x64[(int64_t)nb * D + i];
+            }
+        }
+        nv += __popcll(__ballot(valid));
+        ncp += __popcll(__ballot(valid && pair_cp[e] >= 0));
+        nrw += __popcll(rm);
+    }
     if (lane == 0) {
         cpar[j] = par;
         pc_out[j] = pc;
-        marked[j] = (uint32_t)mm;
-        atomicAdd((unsigned long long*)&sc->rewire_tests, (unsigned long long)__popcll(vm));
-        atomicAdd((unsigned long long*)&sc->cp_links, (unsigned long long)__popcll(em));
-        atomicAdd((unsigned long long*)&sc->rw_links, (unsigned long long)__popcll(rm));
+        atomicAdd((unsigned long long*)&sc->rewire_tests, nv);
+        atomicAdd((unsigned long long*)&sc->cp_links, ncp);
+        atomicAdd((unsigned long long*)&sc->rw_links, nrw);
         if (par != nr) atomicAdd((unsigned long long*)&sc->parent_changes, 1ull);
     }
 }
@@ -341,13 +398,13 @@ struct RewireShared {
 // The walk is bounded by the tree size `fcap` (also the capacity of each
 // frontier array): a subtree can hold no more nodes, so exceeding it means
 // the child lists are corrupt; the pass then flags the tree and stops.
-__device__ void rewire_apply(RewireShared& s, int64_t pair, int k, int64_t n0, int64_t fcap,
-                             const int32_t* __restrict__ nbh_idx, const double* __restrict__ nbh_d, double* cost,
-                             int32_t* parent, int32_t* child, int32_t* next, int32_t* prev, int32_t* fa, int32_t* fb,
-                             StarCounters* sc) {
+__device__ void rewire_apply(RewireShared& s, int64_t pair, int k, const int32_t* __restrict__ own, int64_t n0,
+                             int64_t fcap, const int32_t* __restrict__ nbh_idx, const double* __restrict__ nbh_d,
+                             double* cost, int32_t* parent, int32_t* child, int32_t* next, int32_t* prev, int32_t* fa,
+                             int32_t* fb, StarCounters* sc) {
     const int t = threadIdx.x;
     if (t == 0) {
-        const int32_t id = (int32_t)(n0 + pair / k);
+        const int32_t id = (int32_t)(n0 + nbh_owner(own, k, pair));
         const int32_t nb = nbh_idx[pair];
         const double nc = cost[id] + nbh_d[pair];
         const double delta = cost[nb] - nc;
@@ -399,10 +456,13 @@ __device__ void rewire_apply(RewireShared& s, int64_t pair, int k, int64_t n0, i
     }
 }
 
-__global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t m, int64_t n0,
+// Pairs are the list entries in order (survivor by survivor, each list in
+// order): `total` of them, survivor of entry e = own[e] (or e / k).
+__global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t total, int64_t n0, int64_t fcap,
+                                                    const int32_t* __restrict__ own,
                                                     const int32_t* __restrict__ nbh_idx,
                                                     const double* __restrict__ nbh_d,
-                                                    const uint32_t* __restrict__ marked,
+                                                    const uint8_t* __restrict__ marked,
                                                     const int32_t* __restrict__ pair_rw,
                                                     const uint8_t* __restrict__ rw_ver,
                                                     const uint8_t* __restrict__ pend_ver,
@@ -411,8 +471,6 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t m, int
                                                     int32_t* fb, double* __restrict__ pend_rows, StarCounters* sc) {
     __shared__ RewireShared s;
     const int t = threadIdx.x;
-    const int64_t total = m * (int64_t)k;
-    const int64_t fcap = n0 + m;
     if (t == 0) {
         s.pos = sc->pos;
         s.pend = sc->pending;
@@ -425,7 +483,7 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t m, int
         // costs are as they were when the pass stopped at this pair, so its
         // condition still holds; the host has checked its link since
         if (pend_ver[0])
-            rewire_apply(s, s.pair, k, n0, fcap, nbh_idx, nbh_d, cost, parent, child, next, prev, fa, fb, sc);
+            rewire_apply(s, s.pair, k, own, n0, fcap, nbh_idx, nbh_d, cost, parent, child, next, prev, fa, fb, sc);
         if (t == 0) {
             sc->pending = 0;
             s.pos = s.pair + 1;
@@ -439,9 +497,8 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t m, int
         bool act = false, unk = false;
         if (pair < total) {
             const int32_t nb = nbh_idx[pair];
-            const int64_t j = pair / k;
-            const int i = (int)(pair - j * k);
-            if (nb >= 0 && !((marked[j] >> i) & 1u)) {
+            const int64_t j = nbh_owner(own, k, pair);
+            if (nb >= 0 && !marked[pair]) {
                 const double nc = cost[n0 + j] + nbh_d[pair];
                 if (nc < cost[nb]) {
                     const int32_t r = pair_rw[pair];
@@ -468,7 +525,7 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t m, int
         if (s.unknown) {
             if (t == 0) {
                 const int64_t pr = p0 + f;
-                const int64_t id = n0 + pr / k;
+                const int64_t id = n0 + nbh_owner(own, k, pr);
                 const int32_t nb = nbh_idx[pr];
                 for (int i = 0; i < D; ++i) {
                     pend_rows[i] = x64[id * D + i];
@@ -482,7 +539,7 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t m, int
             }
             return;
         }
-        rewire_apply(s, p0 + f, k, n0, fcap, nbh_idx, nbh_d, cost, parent, child, next, prev, fa, fb, sc);
+        rewire_apply(s, p0 + f, k, own, n0, fcap, nbh_idx, nbh_d, cost, parent, child, next, prev, fa, fb, sc);
         if (t == 0) {
             s.pos = p0 + f + 1;
             s.first = INT_MAX;
@@ -506,19 +563,65 @@ int star_k(const PrrtState& s, int dim, int64_t n) {
     return (int)std::ceil(kRRG * std::log((double)(n + 1)));
 }
 
+// The space's dimension as the reference's formulas take it
+// (space.dimensions()): the state size for L1/L2, 6 for SE(3) (SO(3) is a
+// 3-manifold, its quaternion has 4 coordinates).
+static int manifold_dim(const mptg_ctx* ctx) { return ctx->metric == MPTG_METRIC_SE3 ? 6 : ctx->dim; }
+
+// rewire_r_nearest's radius at tree size n (rrg_rewire_neighbors.hpp:99-128):
+// r = rRRG (ln(n + 1) / (n + 1))^(1/d), rRRG = f (2 (1 + 1/d) mu / zeta_d)^(1/d),
+// zeta_d = sqrt(pi)^d / Gamma(d/2 + 1) (the unit L2 ball), mu = the sampler's
+// measure (uniform_box_sampler.hpp:76-78, uniform_sampler_scaled.hpp:59-65,
+// uniform_sampler_so3.hpp:79-83, uniform_sampler_cartesian.hpp:93-98): the
+// product of the box extents; SE(3): pi^2 w_so3 times the translation
+// box's product (times w_l2).  Extents multiply left to right.
+double star_radius(const mptg_ctx* ctx, int64_t n) {
+    const PrrtState& s = ctx->prrt;
+    const int d = manifold_dim(ctx);
+    const double pi = 3.14159265358979323846;
+    double mu;
+    if (ctx->metric == MPTG_METRIC_SE3) {
+        double box = 1.0;
+        for (int i = 4; i < 7; ++i) box = box * (s.hi[i] - s.lo[i]);
+        mu = (pi * pi * ctx->w_so3) * (ctx->w_l2 == 1.0 ? box : box * ctx->w_l2);
+    } else {
+        mu = 1.0;
+        for (int i = 0; i < ctx->dim; ++i) mu = mu * (s.hi[i] - s.lo[i]);
+    }
+    const double inv = 1 / (double)d;
+    const double ball = std::pow(std::sqrt(pi), (double)d) / std::tgamma((double)d / 2 + 1);
+    const double rrg = s.rewire_factor * std::pow(2 * (1 + inv) * mu / ball, inv);
+    const double m = (double)(n + 1);
+    return rrg * std::pow(std::log(m) / m, inv);
+}
+
+// Per-survivor buffers for B samples per iteration.
 static int star_reserve(mptg_ctx* ctx, int B) {
     PrrtState& pb = ctx->prrt;
     const int D = ctx->dim;
-    const size_t E = (size_t)B * kStarMaxK;
     int rc;
     if ((rc = ensure(ctx, pb.dnear, sizeof(double) * B))) return rc;
     if ((rc = ensure(ctx, pb.sidx, sizeof(int32_t) * B))) return rc;
     if ((rc = ensure(ctx, pb.qs, sizeof(double) * D * B))) return rc;
-    if ((rc = ensure(ctx, pb.nbh_idx, sizeof(int32_t) * E))) return rc;
-    if ((rc = ensure(ctx, pb.nbh_d, sizeof(double) * E))) return rc;
-    if ((rc = ensure(ctx, pb.marked, sizeof(uint32_t) * B))) return rc;
     if ((rc = ensure(ctx, pb.pc, sizeof(double) * B))) return rc;
     if ((rc = ensure(ctx, pb.cpar, sizeof(int32_t) * B))) return rc;
+    // two pending rows, then verdict and checks of the pending edge
+    if ((rc = ensure(ctx, pb.pend, sizeof(double) * 2 * D + 16))) return rc;
+    if ((rc = ensure(ctx, pb.star_cnt, sizeof(StarCounters)))) return rc;
+    return MPTG_OK;
+}
+
+// Per-entry buffers for E neighbour-list entries (each may queue one
+// choose-parent and one rewire edge); owner: the CSR lists' survivor index.
+static int star_reserve_entries(mptg_ctx* ctx, size_t E, bool owner) {
+    PrrtState& pb = ctx->prrt;
+    const int D = ctx->dim;
+    E = std::max<size_t>(E, 1);
+    int rc;
+    if ((rc = ensure(ctx, pb.nbh_idx, sizeof(int32_t) * E))) return rc;
+    if ((rc = ensure(ctx, pb.nbh_d, sizeof(double) * E))) return rc;
+    if (owner && (rc = ensure(ctx, pb.nbh_own, sizeof(int32_t) * E))) return rc;
+    if ((rc = ensure(ctx, pb.marked, E))) return rc;
     if ((rc = ensure(ctx, pb.cp_from, sizeof(double) * D * E))) return rc;
     if ((rc = ensure(ctx, pb.cp_to, sizeof(double) * D * E))) return rc;
     if ((rc = ensure(ctx, pb.cp_ver, E))) return rc;
@@ -529,9 +632,6 @@ static int star_reserve(mptg_ctx* ctx, int B) {
     if ((rc = ensure(ctx, pb.rw_ver, E))) return rc;
     if ((rc = ensure(ctx, pb.rw_chk, sizeof(uint32_t) * E))) return rc;
     if ((rc = ensure(ctx, pb.pair_rw, sizeof(int32_t) * E))) return rc;
-    // two pending rows, then verdict and checks of the pending edge
-    if ((rc = ensure(ctx, pb.pend, sizeof(double) * 2 * D + 16))) return rc;
-    if ((rc = ensure(ctx, pb.star_cnt, sizeof(StarCounters)))) return rc;
     return MPTG_OK;
 }
 
@@ -575,10 +675,13 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
     const int dp = pitch32(D);
     for (int64_t iter = 0; iter < max_iters && ctx->size < target_size; ++iter) {
         const int64_t n0 = ctx->size;
-        const int k = star_k(s, D, n0);
+        // rewire_k_nearest: rows of the k nearest; rewire_r_nearest: every node
+        // within the radius, in CSR lists
+        const bool by_radius = s.star_mode == MPTG_REWIRE_R_NEAREST;
+        const int k = by_radius ? 1 : star_k(s, manifold_dim(ctx), n0);
         if (k < 1 || k > kStarMaxK)
             return fail(ctx, MPTG_E_RANGE,
-                        "PRRT* neighbour count " + std::to_string(k) + " is outside 1..32 at tree size " +
+                        "PRRT* neighbour count " + std::to_string(k) + " is outside 1..64 at tree size " +
                             std::to_string(n0) + " (set a fixed k)");
         if ((rc = tree_reserve(ctx, n0 + B))) return rc;
         if ((rc = ensure(ctx, s.front, sizeof(int32_t) * 2 * (size_t)(n0 + B)))) return rc;
@@ -613,11 +716,29 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         const int64_t m = hs->m;
         if (m > 0) {
             // ---- neighbours and choose parent
-            if ((rc = launch_knn(ctx, (const double*)s.qs.p, m, k, __builtin_inf(), (int32_t*)s.nbh_idx.p,
-                                 (double*)s.nbh_d.p, n0)))
-                return rc;
+            int64_t E;
+            const int64_t* off = nullptr;
+            const int32_t* own = nullptr;
+            if (by_radius) {
+                const double r = star_radius(ctx, n0);
+                if ((rc = rnn_count(ctx, (const double*)s.qs.p, m, r, &E))) return rc;
+                if (E > (int64_t)INT32_MAX)
+                    return fail(ctx, MPTG_E_RANGE, "more than 2^31 rewire neighbours in one iteration");
+                if ((rc = star_reserve_entries(ctx, (size_t)E, true))) return rc;
+                if ((rc = rnn_fill(ctx, (const double*)s.qs.p, m, r, E, (double*)s.nbh_d.p, (int32_t*)s.nbh_idx.p,
+                                   (int32_t*)s.nbh_own.p)))
+                    return rc;
+                off = rnn_offsets(ctx);
+                own = (const int32_t*)s.nbh_own.p;
+            } else {
+                E = m * (int64_t)k;
+                if ((rc = star_reserve_entries(ctx, (size_t)E, false))) return rc;
+                if ((rc = launch_knn(ctx, (const double*)s.qs.p, m, k, __builtin_inf(), (int32_t*)s.nbh_idx.p,
+                                     (double*)s.nbh_d.p, n0)))
+                    return rc;
+            }
             const unsigned wb = (unsigned)((m + 3) / 4);
-            hipLaunchKernelGGL(star_cp_prepare, dim3(wb), dim3(256), 0, ctx->stream, D, k, m,
+            hipLaunchKernelGGL(star_cp_prepare, dim3(wb), dim3(256), 0, ctx->stream, D, k, m, off,
                                (const int32_t*)s.sidx.p, (const int32_t*)s.near.p, (const double*)s.dnear.p,
                                (const double*)s.qs.p, (const int32_t*)s.nbh_idx.p, (const double*)s.nbh_d.p,
                                (const double*)ctx->x64, (const double*)ctx->cost, (double*)s.cp_from.p,
@@ -629,12 +750,12 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
                 (rc = launch_link(ctx, (const double*)s.cp_from.p, (const double*)s.cp_to.p, hs->ncp, s.p.order,
                                   1, (uint8_t*)s.cp_ver.p, (uint32_t*)s.cp_chk.p)))
                 return rc;
-            hipLaunchKernelGGL(star_cp_resolve, dim3(wb), dim3(256), 0, ctx->stream, D, k, m,
+            hipLaunchKernelGGL(star_cp_resolve, dim3(wb), dim3(256), 0, ctx->stream, D, k, m, off,
                                (const int32_t*)s.sidx.p, (const int32_t*)s.near.p, (const double*)s.dnear.p,
                                (const double*)s.qs.p, (const int32_t*)s.nbh_idx.p, (const double*)s.nbh_d.p,
                                (const double*)ctx->x64, (const double*)ctx->cost, (const int32_t*)s.pair_cp.p,
                                (const uint8_t*)s.cp_ver.p, (int32_t*)s.cpar.p, (double*)s.pc.p,
-                               (uint32_t*)s.marked.p, (double*)s.rw_from.p, (double*)s.rw_to.p,
+                               (uint8_t*)s.marked.p, (double*)s.rw_from.p, (double*)s.rw_to.p,
                                (int32_t*)s.pair_rw.p, dsc);
             MPTG_HIP(ctx, hipGetLastError());
             // ---- append
@@ -654,9 +775,9 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
             uint8_t* pend_ver = (uint8_t*)(pend_rows + 2 * D);
             uint32_t* pend_chk = (uint32_t*)(pend_ver + 8);
             for (;;) {
-                hipLaunchKernelGGL(star_rewire, dim3(1), dim3(1024), 0, ctx->stream, D, k, m, n0,
+                hipLaunchKernelGGL(star_rewire, dim3(1), dim3(1024), 0, ctx->stream, D, k, E, n0, n0 + m, own,
                                    (const int32_t*)s.nbh_idx.p, (const double*)s.nbh_d.p,
-                                   (const uint32_t*)s.marked.p, (const int32_t*)s.pair_rw.p,
+                                   (const uint8_t*)s.marked.p, (const int32_t*)s.pair_rw.p,
                                    (const uint8_t*)s.rw_ver.p, (const uint8_t*)pend_ver, (const double*)ctx->x64,
                                    ctx->cost, ctx->parent, ctx->child, ctx->next, ctx->prev, front,
                                    front + (n0 + B), pend_rows, dsc);
@@ -708,7 +829,9 @@ int mptg_prrt_star_init(mptg_ctx* ctx, const mptg_prrt_params* p, const mptg_prr
                         const double* h_lo, const double* h_hi, const double* h_goal) {
     if (!ctx) return fail(nullptr, MPTG_E_ARG, "null context");
     if (!sp) return fail(ctx, MPTG_E_ARG, "null PRRT* parameters");
-    if (sp->k < 0 || sp->k > kStarMaxK) return fail(ctx, MPTG_E_RANGE, "PRRT* k must be in 0..32");
+    if (sp->k < 0 || sp->k > kStarMaxK) return fail(ctx, MPTG_E_RANGE, "PRRT* k must be in 0..64");
+    if (sp->neighbors != MPTG_REWIRE_K_NEAREST && sp->neighbors != MPTG_REWIRE_R_NEAREST)
+        return fail(ctx, MPTG_E_ARG, "unknown rewire neighbour strategy");
     if (!(sp->rewire_factor > 0) || !std::isfinite(sp->rewire_factor))
         return fail(ctx, MPTG_E_ARG, "rewire factor must be a positive finite number");
     int rc = mptg_prrt_init(ctx, p, h_lo, h_hi, h_goal);
@@ -716,6 +839,7 @@ int mptg_prrt_star_init(mptg_ctx* ctx, const mptg_prrt_params* p, const mptg_prr
     PrrtState& s = ctx->prrt;
     s.star = true;
     s.star_k = sp->k;
+    s.star_mode = sp->neighbors;
     s.rewire_factor = sp->rewire_factor;
     return MPTG_OK;
 }

@@ -107,7 +107,7 @@ int mptg_autotune_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, mptg_
     int rc = check_device_ready(ctx);
     if (rc) return rc;
     if (nq <= 0 || !d_q) return fail(ctx, MPTG_E_ARG, "autotune_knn needs device queries");
-    if (k < 1 || k > 32) return fail(ctx, MPTG_E_RANGE, "k must be in 1..32");
+    if (k < 1 || k > kMaxK) return fail(ctx, MPTG_E_RANGE, "k must be in 1..64");
     if (ctx->size <= 0) return fail(ctx, MPTG_E_EMPTY, "the tree is empty");
     DevBuf& out = ctx->scratch[11];
     if ((rc = ensure(ctx, out, (sizeof(double) + sizeof(int32_t)) * (size_t)k * nq))) return rc;

@@ -63,13 +63,15 @@ struct graph_iterations {
     static constexpr int value = G;
 };
 // PRRT* rewiring (src/mpt/planner_tags.hpp:49-51).  k-nearest is the
-// default, with the reference's k = ceil(kRRG ln(n + 1)); rewire_k<K> fixes
-// k (1..32).  The radius form is not implemented on the device.
+// default, with the reference's k = ceil(kRRG ln(n + 1)) (up to 64);
+// rewire_k<K> fixes k (1..64).  r-nearest takes every node within the
+// reference's radius (rrg_rewire_neighbors.hpp:94-152); solve() throws at an
+// iteration where a new node has more than 63 of them.
 struct rewire_k_nearest {};
 struct rewire_r_nearest {};
 template <int K>
 struct rewire_k {
-    static_assert(K >= 1 && K <= 32, "rewire_k must be in 1..32");
+    static_assert(K >= 1 && K <= 64, "rewire_k must be in 1..64");
     static constexpr int value = K;
 };
 
@@ -122,12 +124,13 @@ using PRRT = PRRTStrategy<PRRTOptions<Opts...>::batch, PRRTOptions<Opts...>::sta
                           PRRTOptions<Opts...>::graph>;
 
 // PRRT* (src/mpt/prrt_star.hpp:55-110): the PRRT options plus rewiring.
-template <int Batch, bool Stats, int Order, int K>
+template <int Batch, bool Stats, int Order, int K, int Neighbors = MPTG_REWIRE_K_NEAREST>
 struct PRRTStarStrategy {
     static constexpr int batch = Batch;
     static constexpr bool stats = Stats;
     static constexpr int order = Order;
     static constexpr int k = K;  // 0: the reference's k-nearest formula
+    static constexpr int neighbors = Neighbors;
 };
 
 template <typename... Opts>
@@ -135,6 +138,7 @@ struct PRRTStarOptions;
 template <>
 struct PRRTStarOptions<> : PRRTOptions<> {
     static constexpr int k = 0;
+    static constexpr int neighbors = MPTG_REWIRE_K_NEAREST;
 };
 template <typename Opt, typename... Rest>
 struct PRRTStarOptions<Opt, Rest...> : PRRTStarOptions<Rest...> {
@@ -151,7 +155,7 @@ struct PRRTStarOptions<graph_iterations<G>, Rest...> : PRRTStarOptions<Rest...> 
 };
 template <typename... Rest>
 struct PRRTStarOptions<rewire_r_nearest, Rest...> : PRRTStarOptions<Rest...> {
-    static_assert(sizeof...(Rest) < 0, "rewire_r_nearest is not implemented by the device planner");
+    static constexpr int neighbors = MPTG_REWIRE_R_NEAREST;
 };
 template <int K, typename... Rest>
 struct PRRTStarOptions<rewire_k<K>, Rest...> : PRRTStarOptions<Rest...> {
@@ -160,7 +164,8 @@ struct PRRTStarOptions<rewire_k<K>, Rest...> : PRRTStarOptions<Rest...> {
 
 template <typename... Opts>
 using PRRTStar = PRRTStarStrategy<PRRTStarOptions<Opts...>::batch, PRRTStarOptions<Opts...>::stats,
-                                  PRRTStarOptions<Opts...>::order, PRRTStarOptions<Opts...>::k>;
+                                  PRRTStarOptions<Opts...>::order, PRRTStarOptions<Opts...>::k,
+                                  PRRTStarOptions<Opts...>::neighbors>;
 
 template <typename Scenario, typename Strategy>
 class GpuPRRT;
@@ -171,9 +176,9 @@ template <typename Scenario, int B, bool S, int O, int G>
 struct PlannerResolver<Scenario, PRRTStrategy<B, S, O, G>> {
     using type = GpuPRRT<Scenario, PRRTStrategy<B, S, O, G>>;
 };
-template <typename Scenario, int B, bool S, int O, int K>
-struct PlannerResolver<Scenario, PRRTStarStrategy<B, S, O, K>> {
-    using type = GpuPRRT<Scenario, PRRTStarStrategy<B, S, O, K>>;
+template <typename Scenario, int B, bool S, int O, int K, int N>
+struct PlannerResolver<Scenario, PRRTStarStrategy<B, S, O, K, N>> {
+    using type = GpuPRRT<Scenario, PRRTStarStrategy<B, S, O, K, N>>;
 };
 
 template <typename Scenario, typename Algorithm>
@@ -182,8 +187,8 @@ using Planner = typename PlannerResolver<Scenario, Algorithm>::type;
 // ---- the planner ---------------------------------------------------------------
 template <typename S>
 struct is_star : std::false_type {};
-template <int B, bool S, int O, int K>
-struct is_star<PRRTStarStrategy<B, S, O, K>> : std::true_type {};
+template <int B, bool S, int O, int K, int N>
+struct is_star<PRRTStarStrategy<B, S, O, K, N>> : std::true_type {};
 
 template <typename Scenario, typename Strategy>
 class GpuPRRT {
@@ -224,6 +229,7 @@ class GpuPRRT {
         if constexpr (kStar) {
             mptg_prrt_star_params sp{};
             sp.k = Strategy::k;
+            sp.neighbors = Strategy::neighbors;
             sp.rewire_factor = rewireFactor_;
             detail::check(ctx_, mptg_prrt_star_init(ctx_, &p, &sp, lo, hi, goal));
         } else {

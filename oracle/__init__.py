@@ -84,6 +84,7 @@ _SIG = [
     ("orc_prrt_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_int64, _dp,
                                  _i32p, C.c_int64, _i32p, C.c_int64, C.POINTER(OrcPrrtStats)]),
     ("orc_star_k", C.c_int32, [C.c_int32, C.c_double, C.c_int32, C.c_int64]),
+    ("orc_star_radius", C.c_double, [C.POINTER(OrcScene), _dp, _dp, C.c_double, C.c_int64]),
     ("orc_prrt_star_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), C.c_int32, C.c_double, _dp,
                                       C.c_int64, C.c_int64, _dp, _i32p, _dp, C.c_int64, _i32p, C.c_int64,
                                       C.POINTER(OrcPrrtStats), C.POINTER(OrcStarStats)]),
@@ -264,9 +265,17 @@ class Oracle:
         stats = {f: int(getattr(st, f)) for f, _ in st._fields_}
         return nodes[:n].copy(), parents[:n].copy(), goals[: min(stats["goals"], 4096)].copy(), stats
 
+    def star_radius(self, n, rewire_factor=1.1) -> float:
+        """rewire_r_nearest's radius at tree size n over the scene's bounds."""
+        lo, hi = _f64(self.scene.lo), _f64(self.scene.hi)
+        return float(self.L.orc_star_radius(C.byref(self.s), _p(lo, C.c_double), _p(hi, C.c_double),
+                                            rewire_factor, n))
+
     def prrt_star(self, seed, batch, target, k=20, rewire_factor=1.1, max_iters=1 << 40, range_=None,
                   goal_bias=None):
-        """Batch-synchronous PRRT*: (nodes, parents, costs, goals, stats, star_stats)."""
+        """Batch-synchronous PRRT*: (nodes, parents, costs, goals, stats,
+        star_stats).  k > 0 nearest neighbours, 0 the reference's k formula,
+        -1 every node within the r-nearest radius."""
         p = self._params(seed, batch, range_, goal_bias)
         cap = target + batch + 8
         nodes = np.zeros((cap, self.dim))

@@ -158,7 +158,12 @@ typedef struct {
 
 /* Neighbour count: k_fixed if > 0, else ceil(factor*e*(1+1/dim)*ln(n+1)). */
 int32_t orc_star_k(int32_t k_fixed, double factor, int32_t dim, int64_t n);
-/* Grows a PRRT* tree; costs[cap] receive the path costs.  Returns the final
+/* rewire_r_nearest's radius at tree size n for the sampling box [lo, hi]
+ * (rrg_rewire_neighbors.hpp:99-128). */
+double orc_star_radius(const orc_scene* sc, const double* lo, const double* hi, double factor, int64_t n);
+/* Grows a PRRT* tree; costs[cap] receive the path costs.  k_fixed > 0: that
+ * many nearest neighbours; 0: the reference's k formula; -1: every node within
+ * the reference's r-nearest radius (rewire_r_nearest).  Returns the final
  * size, -1 if cap is too small, -2 on the reference's "bad distance" abort,
  * -3 if k exceeds 64. */
 int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params* p, int32_t k_fixed, double rewire_factor,

@@ -58,7 +58,7 @@ extern "C" int32_t orc_link(const orc_scene* sc, const double* a, const double* 
 
 namespace {
 
-constexpr int kMaxNbh = 64;
+constexpr int kMaxNbh = 64;  // largest k of rewire_k_nearest
 
 struct StarExt {
     bool accepted = false, goal = false, zero = false, invalid = false, linked = false;
@@ -66,10 +66,10 @@ struct StarExt {
     int32_t near = -1;
     double dnear = 0;
     double state[64];
-    // phase 2
-    int32_t nbr[kMaxNbh];
-    double nd[kMaxNbh];
-    uint64_t marked = 0;
+    // phase 2: the neighbour list (ascending (distance, id); -1 = empty slot)
+    std::vector<int32_t> nbr;
+    std::vector<double> nd;
+    std::vector<uint8_t> marked;
     int32_t parent = -1;
     double pc = 0;
     int64_t cp_links = 0;
@@ -97,6 +97,36 @@ extern "C" int32_t orc_star_k(int32_t k_fixed, double factor, int32_t dim, int64
     if (k_fixed > 0) return k_fixed;
     const double kRRG = factor * 2.718281828459045 * (1.0 + 1.0 / (double)dim);
     return (int32_t)std::ceil(kRRG * std::log((double)(n + 1)));
+}
+
+// The space's dimensions() as the reference's formulas use it: the state
+// size for L_p spaces, 3 + 3 for SE(3) (the SO(3) part is a 3-manifold).
+static int manifold_dim(const orc_scene* sc) { return sc->metric == ORC_METRIC_SE3 ? 6 : sc->dim; }
+
+// RRGRewireNeighbors<rewire_r_nearest>::rewireRadius
+// (src/mpt/impl/rrg_rewire_neighbors.hpp:99-128) with the sampler measures of
+// uniform_box_sampler.hpp:76-78 (box: product of extents),
+// uniform_sampler_so3.hpp:79-83 (pi^2), uniform_sampler_scaled.hpp:59-65
+// (times the weight) and uniform_sampler_cartesian.hpp:93-98 (product over
+// the components, left to right).
+extern "C" double orc_star_radius(const orc_scene* sc, const double* lo, const double* hi, double factor,
+                                  int64_t n) {
+    const int d = manifold_dim(sc);
+    const double pi = 3.14159265358979323846;
+    double mu;
+    if (sc->metric == ORC_METRIC_SE3) {
+        double box = 1.0;
+        for (int i = 4; i < 7; ++i) box = box * (hi[i] - lo[i]);
+        mu = (pi * pi * sc->w_so3) * (sc->w_l2 == 1.0 ? box : box * sc->w_l2);
+    } else {
+        mu = 1.0;
+        for (int i = 0; i < sc->dim; ++i) mu = mu * (hi[i] - lo[i]);
+    }
+    const double invDim = 1 / (double)d;
+    const double unitBall = std::pow(std::sqrt(pi), (double)d) / std::tgamma((double)d / 2 + 1);
+    const double rRRG = factor * std::pow(2 * (1 + invDim) * mu / unitBall, invDim);
+    const double m = (double)(n + 1);
+    return rRRG * std::pow(std::log(m) / m, invDim);
 }
 
 extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params* p, int32_t k_fixed,
@@ -182,21 +212,46 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
             if (e.accepted) surv.push_back(b);
         }
         const int m = (int)surv.size();
-        const int k = orc_star_k(k_fixed, rewire_factor, d, n0);
-        if (k < 1 || k > kMaxNbh) return -3;
+        // rewire_k_nearest: the k nearest (k_fixed > 0, or the formula for 0);
+        // rewire_r_nearest (k_fixed = -1): every node within the radius
+        const bool by_radius = k_fixed == -1;
+        const int k = by_radius ? 0 : orc_star_k(k_fixed, rewire_factor, manifold_dim(sc), n0);
+        const double radius = by_radius ? orc_star_radius(sc, p->lo, p->hi, rewire_factor, n0) : 0.0;
+        if (!by_radius && (k < 1 || k > kMaxNbh)) return -3;
         // ---- phase 2: choose parent against the iteration's tree and costs
         parallel_for(m, hw, [&](int j) {
             StarExt& e = ext[surv[j]];
-            orc_knn(sc, nodes, n0, e.state, 1, k, std::numeric_limits<double>::infinity(), e.nbr, e.nd, 1);
-            int ord[kMaxNbh];
-            double key[kMaxNbh];
+            if (by_radius) {
+                std::vector<std::pair<double, int32_t>> all;
+                for (int64_t i = 0; i < n0; ++i) {
+                    const double di = distance(sc, e.state, nodes + i * d);
+                    if (di <= radius) all.emplace_back(di, (int32_t)i);
+                }
+                std::sort(all.begin(), all.end());
+                e.nbr.resize(all.size());
+                e.nd.resize(all.size());
+                for (size_t i = 0; i < all.size(); ++i) {
+                    e.nd[i] = all[i].first;
+                    e.nbr[i] = all[i].second;
+                }
+            } else {
+                e.nbr.assign(k, -1);
+                e.nd.assign(k, 0.0);
+                orc_knn(sc, nodes, n0, e.state, 1, k, std::numeric_limits<double>::infinity(), e.nbr.data(),
+                        e.nd.data(), 1);
+            }
+            const int nn = (int)e.nbr.size();
+            e.marked.assign(nn, 0);
+            std::vector<int> ord;
+            std::vector<double> key(nn);
             int cnt = 0;
-            for (int i = 0; i < k; ++i)
+            for (int i = 0; i < nn; ++i)
                 if (e.nbr[i] >= 0) {
                     key[i] = costs[e.nbr[i]] + e.nd[i];
-                    ord[cnt++] = i;
+                    ord.push_back(i);
+                    ++cnt;
                 }
-            std::stable_sort(ord, ord + cnt, [&](int a, int b2) { return key[a] < key[b2]; });
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b2) { return key[a] < key[b2]; });
             e.parent = e.near;
             e.pc = costs[e.near] + e.dnear;
             for (int r = 0; r < cnt; ++r)
@@ -209,7 +264,7 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
                 const int i = ord[r];
                 const double newCost = key[i];
                 if (newCost > e.pc) break;
-                e.marked |= 1ull << i;
+                e.marked[i] = 1;
                 if (e.nbr[i] == e.near) {
                     e.pc = newCost;
                     break;
@@ -251,11 +306,11 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
         for (int j = 0; j < m; ++j) {
             const StarExt& e = ext[surv[j]];
             const int32_t id = (int32_t)(n0 + j);
-            for (int i = 0; i < k; ++i) {
+            for (int i = 0; i < (int)e.nbr.size(); ++i) {
                 const int32_t nb = e.nbr[i];
                 if (nb < 0) continue;
                 ss.rewire_tests++;
-                if (e.marked >> i & 1) continue;
+                if (e.marked[i]) continue;
                 const bool queued = e.pc + e.nd[i] < snap[nb];
                 ss.spec_rw += queued;
                 const double newCost = costs[id] + e.nd[i];

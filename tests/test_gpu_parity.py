@@ -82,7 +82,7 @@ def test_knn_radius_and_edge_cases(ctx_factory):
         assert np.array_equal(gi, oi) and np.array_equal(gd, od)
     import mpt_amd
     with pytest.raises(mpt_amd.MptgError):
-        ctx.knn(q, 33)
+        ctx.knn(q, 65)
     i, d = ctx.knn(np.zeros((0, 2)), 3)
     assert i.shape == (0, 3)
 

@@ -86,15 +86,15 @@ def test_prrt_star_k_limit_and_plain_tree(ctx_factory):
     ctx.scene(sc)
     ctx.tree_clear()
     with pytest.raises(mpt_amd.MptgError):
-        ctx.prrt_star_init(sc, 1, 8, k=33)
+        ctx.prrt_star_init(sc, 1, 8, k=65)
     # nodes inserted without costs cannot seed PRRT*
     ctx.tree_insert(np.array([sc.start]))
     ctx.prrt_star_init(sc, 1, 8, k=20)
     with pytest.raises(mpt_amd.MptgError):
         ctx.prrt_run(100, GPU_ITER_CAP)
-    # the reference formula outgrows 32 neighbours for D = 2 past ~1250 nodes
+    # the reference formula needs 39 neighbours for D = 2 at 5000 nodes (the
+    # lists hold up to 64)
     ctx.tree_clear()
     ctx.prrt_star_init(sc, 1, 256, k=0)
     ctx.prrt_add_start(sc.start)
-    with pytest.raises(mpt_amd.MptgError):
-        ctx.prrt_run(5000, GPU_ITER_CAP)
+    assert ctx.prrt_run(5000, GPU_ITER_CAP)["size"] >= 5000

@@ -39,8 +39,25 @@ def _push_update(node, delta):
         _push_update(c, delta)
 
 
+def rewire_radius(sc, n, factor=1.1):
+    """RRGRewireNeighbors<rewire_r_nearest>::rewireRadius
+    (src/mpt/impl/rrg_rewire_neighbors.hpp:99-128) for a box sampler
+    (measure = product of the extents, uniform_box_sampler.hpp:76-78)."""
+    d = sc.dim
+    inv = 1 / d
+    mu = 1.0
+    for lo, hi in zip(sc.lo, sc.hi):
+        mu = mu * (hi - lo)
+    unit_ball = math.pow(math.sqrt(math.pi), d) / math.gamma(d / 2 + 1)
+    rrg = factor * math.pow(2 * (1 + inv) * mu / unit_ball, inv)
+    m = float(n + 1)
+    return rrg * math.pow(math.log(m) / m, inv)
+
+
 def literal_prrt_star(o, sc, seed, target, k):
-    """Sequential PRRT* restated line by line from Worker::addSample; B = 1."""
+    """Sequential PRRT* restated line by line from Worker::addSample; B = 1.
+    k > 0: k nearest; 0: the reference's k formula; -1: every node within
+    the r-nearest radius."""
     nodes = [_Node(np.array(sc.start, dtype=np.float64), 0.0, None, 0)]
     goals = 0
     sidx = 0
@@ -68,8 +85,11 @@ def literal_prrt_star(o, sc, seed, target, k):
             continue
         is_goal = o.distance(new, sc.goal) <= sc.goal_radius
         parent, parent_cost = near, near.cost + d_near                 # :552-553
-        kk = oracle.lib().orc_star_k(k, 1.1, sc.dim, len(nodes))
-        nidx, nd = o.knn(pts, new, kk, threads=1)                      # :559-562
+        if k == -1:                                                    # :559-562, rrg_rewire_neighbors.hpp:124-128
+            nidx, nd = o.knn(pts, new, len(pts), rewire_radius(sc, len(nodes)), threads=1)
+        else:                                                          # rrg_rewire_neighbors.hpp:64-67
+            kk = oracle.lib().orc_star_k(k, 1.1, sc.dim, len(nodes))
+            nidx, nd = o.knn(pts, new, kk, threads=1)
         nbh = [[nodes[i], float(d)] for i, d in zip(nidx[0], nd[0]) if i >= 0]
         order = sorted(range(len(nbh)), key=lambda i: nbh[i][0].cost + nbh[i][1])  # :569-574
         for i in order:                                                # :580-605
@@ -125,7 +145,8 @@ def check_star_tree(o, sc, nodes, parents, costs):
     assert o.valid(nodes).all()
 
 
-@pytest.mark.parametrize("name,target,k", [("boxes", 400, 20), ("boxes", 300, 0), ("arm", 120, 20)])
+@pytest.mark.parametrize("name,target,k", [("boxes", 400, 20), ("boxes", 300, 0), ("arm", 120, 20),
+                                           ("boxes", 300, -1), ("arm", 120, -1)])
 def test_batch1_equals_literal_sequential_restatement(name, target, k):
     sc = scenes.boxes_scene() if name == "boxes" else scenes.arm_scene()
     o = oracle.Oracle(sc)
@@ -134,7 +155,8 @@ def test_batch1_equals_literal_sequential_restatement(name, target, k):
     assert np.array_equal(nodes, ln)
     assert np.array_equal(parents, lp)
     assert np.array_equal(costs, lc)
-    assert ss["rewires"] > 0 and ss["parent_changes"] > 0
+    if name == "boxes" or k != -1:  # the 20-D radius holds few nodes of a 120-node tree
+        assert ss["rewires"] > 0 and ss["parent_changes"] > 0
 
 
 @pytest.mark.parametrize("batch", [1, 5, 64])
@@ -161,6 +183,29 @@ def test_rewiring_lowers_costs_against_prrt():
     ln = np.array([o.distance(sc.start, x) for x in nodes[1:]])
     lp = np.array([o.distance(sc.start, x) for x in pn[1:]])
     assert np.mean(costs[1:] / ln) < np.mean(pc[1:] / lp)
+
+
+def test_rewire_radius_formula():
+    """The oracle's r-nearest radius equals the reference formula evaluated
+    here; SE(3) uses dimension 6 and the measure pi^2 w_so3 times the
+    translation box (uniform_sampler_so3.hpp:79-83, scaled :59-65, cartesian
+    :93-98)."""
+    for sc in (scenes.boxes_scene(), scenes.arm_scene()):
+        o = oracle.Oracle(sc)
+        for n in (1, 10, 1000, 10**6):
+            assert o.star_radius(n) == rewire_radius(sc, n)
+    sc = scenes.se3_scene()
+    o = oracle.Oracle(sc)
+    box = 1.0
+    for i in range(4, 7):
+        box = box * (sc.hi[i] - sc.lo[i])
+    mu = (math.pi * math.pi * sc.w_so3) * box
+    inv = 1 / 6
+    rrg = 1.1 * math.pow(2 * (1 + inv) * mu / (math.pow(math.sqrt(math.pi), 6) / math.gamma(4)), inv)
+    for n in (10, 5000):
+        assert o.star_radius(n) == rrg * math.pow(math.log(n + 1.0) / (n + 1.0), inv)
+    # the radius holds a few dozen nodes of a uniform tree: n f^d 2 (1 + 1/d) ln n / n
+    assert 3 < rewire_radius(scenes.boxes_scene(), 2000) < 10
 
 
 def test_rewire_count_formula():
