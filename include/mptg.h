@@ -245,9 +245,9 @@ typedef struct {
                               SE(3)); MPTG_E_RANGE once that exceeds 64 (n around 10^8 at
                               d = 7, 1.6 * 10^6 at d = 2).  Ignored for r-nearest. */
     int32_t neighbors;     /* mptg_rewire_neighbors; rewire_r_nearest takes every node
-                              within rRRG (ln(n + 1) / (n + 1))^(1/d) (:99-128) and
-                              returns MPTG_E_RANGE at an iteration where a survivor has
-                              more than 63 of them */
+                              within rRRG (ln(n + 1) / (n + 1))^(1/d) (:99-128), in lists
+                              of any length (MPTG_E_RANGE only past 2^31 entries in one
+                              iteration) */
     double rewire_factor;  /* setRewireFactor (default 1.1, prrt_star.hpp:176,224) */
 } mptg_prrt_star_params;
 

@@ -65,8 +65,7 @@ struct graph_iterations {
 // PRRT* rewiring (src/mpt/planner_tags.hpp:49-51).  k-nearest is the
 // default, with the reference's k = ceil(kRRG ln(n + 1)) (up to 64);
 // rewire_k<K> fixes k (1..64).  r-nearest takes every node within the
-// reference's radius (rrg_rewire_neighbors.hpp:94-152); solve() throws at an
-// iteration where a new node has more than 63 of them.
+// reference's radius (rrg_rewire_neighbors.hpp:94-152), however many.
 struct rewire_k_nearest {};
 struct rewire_r_nearest {};
 template <int K>
