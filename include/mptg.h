@@ -49,12 +49,20 @@ enum mptg_scene_kind {
     MPTG_SCENE_BOXES = 1, /* closed 2-D boxes, Rect::pointIsValid   demo/shape_hierarchy.hpp:177-182 */
     MPTG_SCENE_GRID = 2,  /* occupancy grid, PNG2dScenario::valid    demo/png_2d_scenario.hpp:104-110 */
     MPTG_SCENE_ARM = 3,   /* planar arm vs disks                     demo/link_manipulator_scenario.hpp:99-116 */
-    MPTG_SCENE_MESH = 4   /* rigid body vs triangle mesh (SE3 space) demo/se3_rigid_body_scenario.hpp:288-312 */
+    MPTG_SCENE_MESH = 4,  /* rigid body vs triangle mesh (SE3 space) demo/se3_rigid_body_scenario.hpp:288-312 */
+    MPTG_SCENE_SHAPES = 5,/* 2-D point among circles and rectangles   demo/holonomic_2d_point_scenario.hpp:95-113 */
+    MPTG_SCENE_HOST = 6   /* the caller's own valid()/link() on the host (mptg_scene_host) */
 };
 
 enum mptg_link_mode {
-    MPTG_LINK_DMV = 0,    /* DiscreteMotionValidator(space, step)   src/mpt/discrete_motion_validator.hpp:71-130 */
-    MPTG_LINK_BISECT = 1  /* recursive midpoints to L-inf < tol     demo/link_manipulator_scenario.hpp:118-138 */
+    MPTG_LINK_DMV = 0,       /* DiscreteMotionValidator(space, step)   src/mpt/discrete_motion_validator.hpp:71-130 */
+    MPTG_LINK_BISECT = 1,    /* recursive midpoints to L-inf < tol     demo/link_manipulator_scenario.hpp:118-138 */
+    MPTG_LINK_BISECT_L2 = 2  /* recursive midpoints to squared L2 length < tol^2 (tol = 1 in the demos):
+                                Rect::segmentIsValid / bisectSegment  demo/shape_hierarchy.hpp:184-203,
+                                PNG2dScenario::link / validSegment    demo/png_2d_scenario.hpp:112-117,152-165;
+                                for MPTG_SCENE_SHAPES the circles first take the closed form
+                                Circle::segmentIsValid                demo/shape_hierarchy.hpp:228-231,259-270
+                                and the midpoints are tested against the rectangles */
 };
 
 enum mptg_check_order {
@@ -140,6 +148,29 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
  * valid(q) = no (robot triangle, env triangle) pair whose boxes overlap and
  * which intersect (FP32 separating-axis test).  The link form must be DMV. */
 int mptg_scene_mesh(mptg_ctx* ctx, const float* h_robot, int nrobot, const float* h_env, int nenv);
+/* Holonomic2DPointScenario (demo/holonomic_2d_point_scenario.hpp:95-113):
+ * valid(q) = q outside every circle (squared distance to the centre > r^2,
+ * Circle::pointIsValid, shape_hierarchy.hpp:222-226) and every closed
+ * rectangle (Rect::pointIsValid, :177-182).  Circles are [cx, cy, r] (at most
+ * 4096), rectangles [x0, y0, x1, y1] (at most 4096); the space must be 2-D. */
+int mptg_scene_shapes(mptg_ctx* ctx, const double* h_circles, int ncircles, const double* h_rects, int nrects);
+/* The Scenario's own valid()/link() as host callbacks (the reference calls them
+ * per state from its workers, src/mpt/impl/prrt/prrt.hpp:439,454-457;
+ * src/mpt/scenario.hpp:93-130): sampling, nearest neighbours, steering, the
+ * tree and PRRT*'s choose-parent/rewire stay on the device, and every batch of
+ * states or edges the planner (or mptg_valid / mptg_link) needs decided is
+ * copied to the host and handed to one call.  valid_fn sets ok[i] = valid(q_i)
+ * (0/1) for n states of `dim` doubles; link_fn sets ok[i] = link(from_i, to_i)
+ * for n edges; either returns 0, or nonzero to fail the calling entry point
+ * with MPTG_E_ARG.  The callbacks run on the thread that called the entry
+ * point and may split the batch over threads of their own.  link() here is the
+ * Scenario's whole link(): the planner calls valid(new) first, then link(near,
+ * new) (prrt.hpp:439-457).  checks and valid_calls count no host calls (0).
+ * Planner iterations with a host scene run step by step (mptg_prrt_graph's
+ * setting is not used). */
+typedef int (*mptg_valid_fn)(void* user, const double* h_q, int64_t n, uint8_t* h_ok);
+typedef int (*mptg_link_fn)(void* user, const double* h_from, const double* h_to, int64_t n, uint8_t* h_ok);
+int mptg_scene_host(mptg_ctx* ctx, mptg_valid_fn valid_fn, mptg_link_fn link_fn, void* user);
 /* link(): DMV step or bisection tolerance */
 int mptg_set_link(mptg_ctx* ctx, int mode, double param);
 

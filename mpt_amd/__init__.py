@@ -20,8 +20,14 @@ import numpy as np
 
 from . import _native
 from ._native import PrrtParams, PrrtStarParams, PrrtStats
-from .scenes import (LINK_BISECT, LINK_DMV, METRIC_L1, METRIC_L2, METRIC_SE3, ORDER_REFERENCE,
-                     ORDER_SURROGATE, SCENE_ARM, SCENE_BOXES, SCENE_GRID, SCENE_MESH, Scene)
+from .scenes import (LINK_BISECT, LINK_BISECT_L2, LINK_DMV, METRIC_L1, METRIC_L2, METRIC_SE3, ORDER_REFERENCE,
+                     ORDER_SURROGATE, SCENE_ARM, SCENE_BOXES, SCENE_GRID, SCENE_HOST, SCENE_MESH, SCENE_SHAPES,
+                     Scene)
+
+# host scene callbacks (mptg_valid_fn, mptg_link_fn)
+_VALID_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_uint8))
+_LINK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
+                       C.POINTER(C.c_uint8))
 
 __all__ = ["Context", "DeviceArray", "PRRT", "MptgError", "Scene", "METRIC_L1", "METRIC_L2", "METRIC_SE3", "LINK_DMV",
            "LINK_BISECT", "ORDER_REFERENCE", "ORDER_SURROGATE"]
@@ -133,7 +139,9 @@ class Context:
 
     def _chk(self, rc: int):
         if rc != 0:
-            raise MptgError(rc, _lib.mptg_last_error(self._h).decode())
+            exc = getattr(self, "_host_exc", None)
+            self._host_exc = None
+            raise MptgError(rc, _lib.mptg_last_error(self._h).decode()) from exc
 
     @property
     def handle(self):
@@ -244,9 +252,49 @@ class Context:
             rt = np.ascontiguousarray(sc.robot_tris, dtype=np.float32).reshape(-1, 9)
             et = np.ascontiguousarray(sc.env_tris, dtype=np.float32).reshape(-1, 9)
             self._chk(_lib.mptg_scene_mesh(self._h, _p(rt, C.c_float), rt.shape[0], _p(et, C.c_float), et.shape[0]))
+        elif sc.kind == SCENE_SHAPES:
+            ci = _f64(sc.circles if sc.circles is not None else np.zeros((0, 3)), 3)
+            rc = _f64(sc.boxes if sc.boxes is not None else np.zeros((0, 4)), 4)
+            self._chk(_lib.mptg_scene_shapes(self._h, _p(ci, C.c_double), ci.shape[0], _p(rc, C.c_double),
+                                             rc.shape[0]))
         else:
             raise ValueError(f"unknown scene kind {sc.kind}")
         self._chk(_lib.mptg_set_link(self._h, sc.link_mode, sc.link_param))
+
+    def scene_host(self, metric: int, dim: int, valid: Callable, link: Callable, w_so3: float = 1.0,
+                   w_l2: float = 1.0):
+        """A scene whose valid()/link() are the caller's own functions, called
+        on the host for whole batches: valid(q) takes an (n, dim) array and
+        returns n booleans, link(a, b) two (n, dim) arrays and returns n
+        booleans.  Sampling, nearest neighbours, steering and the tree stay
+        on the device.  An exception inside a callback fails the calling
+        method with MptgError and is chained to it."""
+        self.tree_clear()
+        self.space(metric, dim, w_so3, w_l2)
+        self._host_exc = None
+
+        def _valid(_user, q, n, ok):
+            try:
+                arr = np.ctypeslib.as_array(q, shape=(n, dim)).copy()
+                np.ctypeslib.as_array(ok, shape=(n,))[:] = np.asarray(valid(arr), dtype=bool)
+                return 0
+            except Exception as e:  # reported by the failing entry point
+                self._host_exc = e
+                return 1
+
+        def _link(_user, a, b, n, ok):
+            try:
+                aa = np.ctypeslib.as_array(a, shape=(n, dim)).copy()
+                bb = np.ctypeslib.as_array(b, shape=(n, dim)).copy()
+                np.ctypeslib.as_array(ok, shape=(n,))[:] = np.asarray(link(aa, bb), dtype=bool)
+                return 0
+            except Exception as e:
+                self._host_exc = e
+                return 1
+
+        self._host_cbs = (_VALID_FN(_valid), _LINK_FN(_link))  # alive while the scene is
+        self._chk(_lib.mptg_scene_host(self._h, C.cast(self._host_cbs[0], C.c_void_p),
+                                       C.cast(self._host_cbs[1], C.c_void_p), None))
 
     def valid(self, q):
         q = _f64(q, self.dim)

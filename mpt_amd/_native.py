@@ -84,6 +84,8 @@ _SIGNATURES = [
     ("mptg_rnn", C.c_int, [C.c_void_p, c_double_p, C.c_int64, C.c_double, C.c_int64, C.POINTER(C.c_int64),
                            C.POINTER(C.c_int32), c_double_p]),
     ("mptg_scene_boxes", C.c_int, [C.c_void_p, c_double_p, C.c_int]),
+    ("mptg_scene_shapes", C.c_int, [C.c_void_p, c_double_p, C.c_int, c_double_p, C.c_int]),
+    ("mptg_scene_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mptg_scene_grid", C.c_int, [C.c_void_p, c_uint8_p, C.c_int, C.c_int]),
     ("mptg_scene_arm", C.c_int, [C.c_void_p, c_double_p, C.c_int, C.c_double, c_double_p, C.c_int]),
     ("mptg_scene_mesh", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.c_int]),

@@ -44,7 +44,7 @@ HIP_FLAGS = [
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall"]
 
-LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "prrt_star.hip", "mesh.hip", "diag.hip", "arm.hip", "mlp.hip", "tune.hip", "knn_grid.hip", "rnn.hip"]
+LIB_SOURCES = ["ctx.hip", "knn.hip", "validate.hip", "prrt.hip", "prrt_star.hip", "mesh.hip", "diag.hip", "arm.hip", "mlp.hip", "tune.hip", "knn_grid.hip", "rnn.hip", "host_scene.hip"]
 ORACLE_SOURCES = ["orc_core.cpp", "orc_mesh.cpp", "orc_prrt.cpp", "orc_prrt_star.cpp", "orc_pool.cpp", "orc_mlp.cpp"]
 
 
@@ -93,7 +93,8 @@ def build_examples() -> list[Path]:
     outs = []
     for src in sorted((PKG / "examples").glob("*.cpp")):
         out = src.with_suffix(".bin")
-        _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"), "-I", str(PKG / "include"), str(src),
+        _run([cxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-ffp-contract=off", "-I", str(ROOT / "include"), "-I",
+              str(PKG / "include"), str(src),
               "-L", str(PKG), "-lmptg", f"-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-o", str(out)])
         outs.append(out)
     return outs

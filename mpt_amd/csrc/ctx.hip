@@ -191,6 +191,7 @@ void mptg_destroy(mptg_ctx* ctx) {
     (void)hipFree(ctx->scene.grid);
     (void)hipFree(ctx->scene.lengths);
     (void)hipFree(ctx->scene.disks);
+    (void)hipFree(ctx->scene.circles);
     (void)hipFree(ctx->scene.arm_disks.p);
     (void)hipFree(ctx->scene.arm_lim.p);
     (void)hipFree(ctx->scene.arm_index.p);
@@ -419,6 +420,32 @@ int mptg_scene_boxes(mptg_ctx* ctx, const double* h_boxes, int n) {
     return MPTG_OK;
 }
 
+int mptg_scene_shapes(mptg_ctx* ctx, const double* h_circles, int ncircles, const double* h_rects, int nrects) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    if (ncircles < 0 || nrects < 0 || (ncircles > 0 && !h_circles) || (nrects > 0 && !h_rects))
+        return fail(ctx, MPTG_E_ARG, "bad circles or rectangles");
+    if (ncircles > 4096 || nrects > 4096) return fail(ctx, MPTG_E_RANGE, "at most 4096 circles and 4096 rectangles");
+    std::vector<double> c((size_t)4 * ncircles);
+    for (int i = 0; i < ncircles; ++i) {
+        const double r = h_circles[3 * i + 2];
+        if (!std::isfinite(h_circles[3 * i]) || !std::isfinite(h_circles[3 * i + 1]) || !std::isfinite(r))
+            return fail(ctx, MPTG_E_ARG, "circles must be finite");
+        c[4 * i] = h_circles[3 * i];
+        c[4 * i + 1] = h_circles[3 * i + 1];
+        c[4 * i + 2] = r * r;  // radius_ * radius_, as Circle compares
+        c[4 * i + 3] = r;
+    }
+    if ((rc = upload(ctx, (void**)&ctx->scene.circles, c.data(), sizeof(double) * c.size()))) return rc;
+    if ((rc = upload(ctx, (void**)&ctx->scene.boxes, h_rects, sizeof(double) * 4 * nrects))) return rc;
+    ctx->scene.kind = MPTG_SCENE_SHAPES;
+    ctx->scene.ncircles = ncircles;
+    ctx->scene.nboxes = nrects;
+    ctx->valid_upto = 0;  // valid() changed: every node is checked again
+    ctx->some_invalid = false;
+    return MPTG_OK;
+}
+
 int mptg_scene_grid(mptg_ctx* ctx, const uint8_t* h_cells, int width, int height) {
     int rc = check_device_ready(ctx);
     if (rc) return rc;
@@ -562,7 +589,8 @@ int mptg_scene_arm(mptg_ctx* ctx, const double* h_lengths, int nlinks, double ro
 int mptg_set_link(mptg_ctx* ctx, int mode, double param) {
     int rc = check_device_ready(ctx);
     if (rc) return rc;
-    if (mode != MPTG_LINK_DMV && mode != MPTG_LINK_BISECT) return fail(ctx, MPTG_E_ARG, "unknown link mode");
+    if (mode != MPTG_LINK_DMV && mode != MPTG_LINK_BISECT && mode != MPTG_LINK_BISECT_L2)
+        return fail(ctx, MPTG_E_ARG, "unknown link mode");
     if (!(param > 0) || !std::isfinite(param)) return fail(ctx, MPTG_E_ARG, "link parameter must be > 0");
     ctx->link_mode = mode;
     ctx->link_param = param;

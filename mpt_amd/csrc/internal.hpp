@@ -44,6 +44,13 @@ struct SceneDev {
     // mesh (SE(3) rigid body): env BVH nodes, env triangles, robot triangles
     DevBuf mesh_nodes, mesh_tris, mesh_robot;
     int mesh_nnodes = 0, mesh_nrobot = 0;
+    // shapes: circles [cx, cy, r^2, r]; the rectangles are `boxes`
+    double* circles = nullptr;
+    int ncircles = 0;
+    // host scene: the caller's valid()/link() callbacks
+    mptg_valid_fn host_valid = nullptr;
+    mptg_link_fn host_link = nullptr;
+    void* host_user = nullptr;
 };
 
 struct PrrtState {
@@ -235,6 +242,15 @@ int link_dev_noalloc(mptg_ctx* ctx, const double* d_from, const double* d_to, in
                      int known_valid, uint8_t* d_verdict, uint32_t* d_checks);
 // mesh scene: one wave per state / edge
 int launch_mesh_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
+// Host scenes (host_scene.hip): the caller's callbacks over device rows.
+// host_valid_dev: d_ok[i] = valid(q_i).  host_filter: for masked rows, valid()
+// false sets d_status[i] = bad and clears d_mask[i].  host_link_dev: for
+// masked edges (all if d_mask is null), d_verdict[i] = link(from_i, to_i) and
+// d_checks[i] = 0.  All synchronous.
+int host_valid_dev(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok);
+int host_filter(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_mask, uint8_t* d_status, uint8_t bad);
+int host_link_dev(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, const uint8_t* d_mask,
+                  uint8_t* d_verdict, uint32_t* d_checks);
 int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from, const double* d_to, int64_t n,
                      int known_valid, const uint8_t* d_mask, uint8_t* d_verdict, uint32_t* d_checks);
 // planners: states[b] with mask[b] set that collide get status[b] = invalid_code, mask[b] = 0

@@ -221,7 +221,8 @@ static bool prevalid_eligible(const mptg_ctx* ctx) {
     const int D = ctx->dim;
     const bool scan_dim = (D >= 1 && D <= 8) || D == 10 || D == 12 || D == 16 || D == 20;
     return s.init && !s.star && std::isinf(s.p.range) && s.p.range > 0 && ctx->metric != MPTG_METRIC_SE3 &&
-           ctx->scene.kind != MPTG_SCENE_NONE && ctx->scene.kind != MPTG_SCENE_MESH && scan_dim &&
+           ctx->scene.kind != MPTG_SCENE_NONE && ctx->scene.kind != MPTG_SCENE_MESH &&
+           ctx->scene.kind != MPTG_SCENE_HOST && scan_dim &&
            !(ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > D) &&
            !(ctx->scene.kind != MPTG_SCENE_ARM && D < 2);
 }
@@ -371,6 +372,10 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
     if (ctx->scene.kind == MPTG_SCENE_MESH &&
         (rc = launch_mesh_filter(ctx, (const double*)pb.to.p, B, (uint8_t*)pb.mask.p, (uint8_t*)pb.status.p,
                                  ST_INVALID)))
+        return rc;
+    // host scenes: valid(new) from the caller, then link(near, new)
+    if (ctx->scene.kind == MPTG_SCENE_HOST &&
+        (rc = host_filter(ctx, (const double*)pb.to.p, B, (uint8_t*)pb.mask.p, (uint8_t*)pb.status.p, ST_INVALID)))
         return rc;
     if ((rc = launch_link_masked(ctx, (const double*)pb.from.p, (const double*)pb.to.p, B, s.p.order, 1,
                                  (const uint8_t*)pb.mask.p, (uint8_t*)pb.verdict.p, (uint32_t*)pb.checks.p)))
@@ -540,7 +545,8 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
     // the context epoch it was captured at holds (no entry point, no
     // allocation since), for the same tree capacity and iteration count.
     // Before a capture one step-by-step iteration sizes every buffer.
-    const int G = s.graph_iters;
+    // host scenes call back into the host within every iteration
+    const int G = ctx->scene.kind == MPTG_SCENE_HOST ? 0 : s.graph_iters;
     int64_t k = 0;
     bool warm = false;
     while (k < max_iters && ctx->size < target_size) {

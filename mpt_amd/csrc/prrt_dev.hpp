@@ -117,8 +117,9 @@ __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc,
             to[(int64_t)b * dd + i] = s[i];
         }
     // mesh scenes: valid(new) is a whole-wave check, done by the filter
-    // kernel that runs next over the masked samples
-    if (sc.kind != MPTG_SCENE_MESH && !state_valid_t<D>(sc, s)) {
+    // kernel that runs next over the masked samples; host scenes: done by the
+    // caller's valid() over the masked samples
+    if (sc.kind != MPTG_SCENE_MESH && sc.kind != MPTG_SCENE_HOST && !state_valid_t<D>(sc, s)) {
         status[b] = ST_INVALID;
         return;
     }

@@ -705,6 +705,9 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
             (rc = launch_mesh_filter(ctx, (const double*)s.to.p, B, (uint8_t*)s.mask.p, (uint8_t*)s.status.p,
                                      ST_INVALID)))
             return rc;
+        if (ctx->scene.kind == MPTG_SCENE_HOST &&
+            (rc = host_filter(ctx, (const double*)s.to.p, B, (uint8_t*)s.mask.p, (uint8_t*)s.status.p, ST_INVALID)))
+            return rc;
         if ((rc = launch_link_masked(ctx, (const double*)s.from.p, (const double*)s.to.p, B, s.p.order, 1,
                                      (const uint8_t*)s.mask.p, (uint8_t*)s.verdict.p, (uint32_t*)s.checks.p)))
             return rc;

@@ -37,6 +37,8 @@ struct SceneArgs {
     const ArmDisk* arm_disks;  // same disks, kernel layout, ordered by grid cell (row-major)
     const double* arm_lim;     // per-link segment-length bound
     const double* arm_index;   // disk grid, see ArmGrid
+    const double* circles;     // shapes: [cx, cy, r^2, r] per circle
+    int ncircles;
 };
 
 // Uniform grid over the disk centres.  Disks are stored by cell, row-major
@@ -70,7 +72,8 @@ inline SceneArgs scene_args(const mptg_ctx* ctx) {
     const SceneDev& s = ctx->scene;
     return SceneArgs{s.kind,    s.boxes,  s.nboxes, s.grid,   s.width,
                      s.height,  s.grid_tx, s.lengths, s.nlinks, s.disks, s.ndisks,
-                     (const ArmDisk*)s.arm_disks.p, (const double*)s.arm_lim.p, (const double*)s.arm_index.p};
+                     (const ArmDisk*)s.arm_disks.p, (const double*)s.arm_lim.p, (const double*)s.arm_index.p,
+                     s.circles, s.ncircles};
 }
 
 __device__ __forceinline__ double so3_dist(const double* a, const double* b) {
@@ -215,6 +218,53 @@ __device__ __forceinline__ bool grid_free(const SceneArgs& sc, double qx, double
 }
 
 __device__ __forceinline__ bool grid_valid(const SceneArgs& sc, const double* q) { return grid_free(sc, q[0], q[1]); }
+
+// Circle::pointIsValid (demo/shape_hierarchy.hpp:222-226): the squared
+// distance to the centre, x^2 + y^2 in that order, exceeds r^2.
+__device__ __forceinline__ bool circles_valid(const SceneArgs& sc, const double* q) {
+    for (int i = 0; i < sc.ncircles; ++i) {
+        const double* c = sc.circles + 4 * i;
+        const double dx = q[0] - c[0], dy = q[1] - c[1];
+        if (!(dx * dx + dy * dy > c[2])) return false;
+    }
+    return true;
+}
+
+// Holonomic2DPointScenario::valid (demo/holonomic_2d_point_scenario.hpp:95-103):
+// circles, then rectangles (the AND does not depend on the order).
+__device__ __forceinline__ bool shapes_valid(const SceneArgs& sc, const double* q) {
+    return circles_valid(sc, q) && boxes_valid(sc, q);
+}
+
+// Circle::segmentIsValid's closed form (demo/shape_hierarchy.hpp:228-231,
+// distPointSegmentSquared :259-270) for every circle: the squared distance
+// from the centre p to the segment s0 s1, with v = s1 - s0, w = p - s0,
+// c1 = v.w; c1 <= 0: |w|^2; c2 = |v|^2 <= c1: |p - s1|^2; else
+// |(s0 - p) + v (c1 / c2)|^2; it must exceed r^2.
+__device__ __forceinline__ bool circles_segment_clear(const SceneArgs& sc, const double* a, const double* b) {
+    const double vx = b[0] - a[0], vy = b[1] - a[1];
+    for (int i = 0; i < sc.ncircles; ++i) {
+        const double* c = sc.circles + 4 * i;
+        const double wx = c[0] - a[0], wy = c[1] - a[1];
+        const double c1 = vx * wx + vy * wy;
+        double d2;
+        if (c1 <= 0) {
+            d2 = wx * wx + wy * wy;
+        } else {
+            const double c2 = vx * vx + vy * vy;
+            if (c2 <= c1) {
+                const double ex = c[0] - b[0], ey = c[1] - b[1];
+                d2 = ex * ex + ey * ey;
+            } else {
+                const double t = c1 / c2;
+                const double ex = (a[0] - c[0]) + vx * t, ey = (a[1] - c[1]) + vy * t;
+                d2 = ex * ex + ey * ey;
+            }
+        }
+        if (!(d2 > c[2])) return false;
+    }
+    return true;
+}
 
 // distPointSegmentSquared(c, s0, s1) > (r+R)^2, skipping disks whose centre
 // is farther than lim + r + R from the segment start.  lim bounds the FP
@@ -618,6 +668,7 @@ __device__ __forceinline__ bool state_valid_t(const SceneArgs& sc, const double*
         case MPTG_SCENE_BOXES: return boxes_valid(sc, q);
         case MPTG_SCENE_GRID: return grid_valid(sc, q);
         case MPTG_SCENE_ARM: return arm_valid(sc, q);
+        case MPTG_SCENE_SHAPES: return shapes_valid(sc, q);
         default: return true;
     }
 }
@@ -678,6 +729,7 @@ __device__ __forceinline__ bool state_valid(const SceneArgs& sc, const double* q
         case MPTG_SCENE_BOXES: return boxes_valid(sc, q);
         case MPTG_SCENE_GRID: return grid_valid(sc, q);
         case MPTG_SCENE_ARM: return arm_valid(sc, q);
+        case MPTG_SCENE_SHAPES: return shapes_valid(sc, q);
         default: return true;
     }
 }

@@ -124,7 +124,7 @@ int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, 
     if (n <= 0 || !d_from || !d_to) return fail(ctx, MPTG_E_ARG, "autotune_link needs device edges");
     const bool arm_wave = ctx->scene.kind == MPTG_SCENE_ARM && ctx->link_mode == MPTG_LINK_BISECT &&
                           ctx->scene.nlinks == ctx->dim && (ctx->dim == 5 || ctx->dim == 10 || ctx->dim == 20);
-    if (ctx->force_generic || !(ctx->link_mode == MPTG_LINK_DMV || arm_wave))
+    if (ctx->force_generic || ctx->scene.kind == MPTG_SCENE_HOST || !(ctx->link_mode == MPTG_LINK_DMV || arm_wave))
         return fail(ctx, MPTG_E_STATE, "this scene's link kernels have no launch shape to tune");
     DevBuf& out = ctx->scratch[11];
     if ((rc = ensure(ctx, out, (sizeof(uint32_t) + 1) * (size_t)n))) return rc;
@@ -158,6 +158,8 @@ int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep
     PrrtState& s = ctx->prrt;
     if (!s.init) return fail(ctx, MPTG_E_STATE, "mptg_prrt_init must be called first");
     if (ctx->size == 0) return fail(ctx, MPTG_E_EMPTY, "there are no valid initial states");
+    if (ctx->scene.kind == MPTG_SCENE_HOST)
+        return fail(ctx, MPTG_E_STATE, "host scenes run step by step: no launch shape to tune");
     // the knob: plain PRRT, iterations per graph launch; PRRT* (step by step),
     // the scene's per-edge link split (waves per mesh edge or per continued
     // arm edge), whose best value depends on the planner's own edge mix

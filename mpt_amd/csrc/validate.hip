@@ -19,6 +19,18 @@
 //    walk is stackless: the current interval is named by its left/right path
 //    from the root and recomputed from (a, b) when needed, so each midpoint
 //    has exactly the reference's bits whatever the tree's shape.
+//  * BISECT_L2 — the same walk until the squared L2 length (b-a).(b-a) is
+//    below tol^2: Rect::segmentIsValid/bisectSegment
+//    (demo/shape_hierarchy.hpp:184-203) and PNG2dScenario::link/validSegment
+//    (demo/png_2d_scenario.hpp:112-117,152-165).  In the circles-and-
+//    rectangles scene each circle first takes its closed form
+//    (Circle::segmentIsValid, shape_hierarchy.hpp:228-231), as the
+//    scenario's link() tests circles before rectangles
+//    (holonomic_2d_point_scenario.hpp:105-113); the points of the walk are
+//    then tested against the rectangles.  Every rectangle's recursion visits
+//    the same midpoints, so one walk that tests all rectangles per point
+//    gives the same verdict; `checks` counts its points (none when a circle
+//    blocks the segment).
 //
 // verdict is the AND over a fixed set of valid() calls, so it does not depend
 // on the visiting order; `checks` counts the calls the given order makes
@@ -368,12 +380,30 @@ __device__ __forceinline__ void bisect_interval(int D, const double* a, const do
     }
 }
 
+// Whether the interval [lo, hi] still gets a midpoint: L-inf length >= tol
+// (L2 = false) or squared L2 length >= tol^2, summed left to right (L2 = true).
+template <bool L2>
 __device__ __forceinline__ bool bisect_internal(int D, const double* lo, const double* hi, double tol) {
     double m = 0.0;
-    for (int j = 0; j < D; ++j) m = __builtin_fmax(m, __builtin_fabs(lo[j] - hi[j]));
-    return !(m < tol);
+    if constexpr (L2) {
+        for (int j = 0; j < D; ++j) {
+            const double t = hi[j] - lo[j];
+            m = m + t * t;
+        }
+        return !(m < tol * tol);
+    } else {
+        for (int j = 0; j < D; ++j) m = __builtin_fmax(m, __builtin_fabs(lo[j] - hi[j]));
+        return !(m < tol);
+    }
 }
 
+// A point of the walk: the scene's valid(), or for circles and rectangles
+// the rectangles alone (the circles took their closed form first).
+__device__ __forceinline__ bool bisect_point_ok(const SceneArgs& sc, const double* q) {
+    return sc.kind == MPTG_SCENE_SHAPES ? boxes_valid(sc, q) : state_valid(sc, q);
+}
+
+template <bool L2>
 __device__ void bisect_edge(int D, const SceneArgs& sc, double tol, const double* __restrict__ from,
                             const double* __restrict__ to, int64_t e, int known_valid, uint8_t* __restrict__ verdict,
                             uint32_t* __restrict__ checks) {
@@ -382,14 +412,19 @@ __device__ void bisect_edge(int D, const SceneArgs& sc, double tol, const double
         a[j] = from[e * D + j];
         b[j] = to[e * D + j];
     }
+    if (sc.kind == MPTG_SCENE_SHAPES && !circles_segment_clear(sc, a, b)) {
+        verdict[e] = 0;
+        checks[e] = 0;
+        return;
+    }
     uint32_t c = 1;
-    if (!known_valid && !state_valid(sc, a)) {
+    if (!known_valid && !bisect_point_ok(sc, a)) {
         verdict[e] = 0;
         checks[e] = c;
         return;
     }
     ++c;
-    if (!known_valid && !state_valid(sc, b)) {
+    if (!known_valid && !bisect_point_ok(sc, b)) {
         verdict[e] = 0;
         checks[e] = c;
         return;
@@ -398,10 +433,10 @@ __device__ void bisect_edge(int D, const SceneArgs& sc, double tol, const double
     int depth = 0;
     for (;;) {
         bisect_interval(D, a, b, path, depth, lo, hi);
-        if (bisect_internal(D, lo, hi, tol)) {
+        if (bisect_internal<L2>(D, lo, hi, tol)) {
             for (int j = 0; j < D; ++j) lo[j] = (lo[j] + hi[j]) / 2;
             ++c;
-            if (!state_valid(sc, lo)) {
+            if (!bisect_point_ok(sc, lo)) {
                 verdict[e] = 0;
                 checks[e] = c;
                 return;
@@ -425,13 +460,14 @@ __device__ void bisect_edge(int D, const SceneArgs& sc, double tol, const double
     checks[e] = c;
 }
 
+template <bool L2>
 __global__ __launch_bounds__(kBlock) void link_bisect(int D, SceneArgs sc, double tol,
                                                       const double* __restrict__ from, const double* __restrict__ to,
                                                       int64_t n, int known_valid, const uint8_t* __restrict__ mask,
                                                       uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks) {
     int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (e >= n || (mask && !mask[e])) return;
-    bisect_edge(D, sc, tol, from, to, e, known_valid, verdict, checks);
+    bisect_edge<L2>(D, sc, tol, from, to, e, known_valid, verdict, checks);
 }
 
 // Exact walk of the edges the wave kernel listed (grid-stride over the list).
@@ -444,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void link_bisect_list(int D, SceneArgs sc, 
                                                            uint32_t* __restrict__ checks) {
     const int64_t cnt = *list_count;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * kBlock)
-        bisect_edge(D, sc, tol, from, to, list[i], known_valid, verdict, checks);
+        bisect_edge<false>(D, sc, tol, from, to, list[i], known_valid, verdict, checks);
 }
 
 // Instrumented bisection link (not timed): same walk and verdicts, and
@@ -475,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void link_bisect_work(int D, SceneArgs sc, 
     int depth = 0;
     while (ok) {
         bisect_interval(D, a, b, path, depth, lo, hi);
-        if (bisect_internal(D, lo, hi, tol)) {
+        if (bisect_internal<false>(D, lo, hi, tol)) {
             for (int j = 0; j < D; ++j) lo[j] = (lo[j] + hi[j]) / 2;
             ++calls;
             if (!arm_valid_counted(sc, lo, &links, &tests, &near)) break;
@@ -601,6 +637,7 @@ int launch_valid(mptg_ctx* ctx, const double* d_q, int64_t n, uint8_t* d_ok) {
         return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
     if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");
     if (ctx->scene.kind == MPTG_SCENE_MESH) return launch_mesh_valid(ctx, d_q, n, d_ok);
+    if (ctx->scene.kind == MPTG_SCENE_HOST) return host_valid_dev(ctx, d_q, n, d_ok);
     if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks == ctx->dim &&
         (ctx->dim == 5 || ctx->dim == 10 || ctx->dim == 20)) {
         const unsigned g = (unsigned)((n * 8 + kBlock - 1) / kBlock);
@@ -642,6 +679,7 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
         return fail(ctx, MPTG_E_STATE, "surrogate order is implemented for the arm scene with bisection links");
     if (n > ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 edges per call");
     unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    if (ctx->scene.kind == MPTG_SCENE_HOST) return host_link_dev(ctx, d_from, d_to, n, d_mask, d_verdict, d_checks);
     if (ctx->scene.kind == MPTG_SCENE_MESH && ctx->link_mode != MPTG_LINK_DMV)
         return fail(ctx, MPTG_E_STATE, "mesh scenes use the discrete motion validator link");
     if (ctx->link_mode == MPTG_LINK_BISECT) {
@@ -664,7 +702,14 @@ int launch_link_masked(mptg_ctx* ctx, const double* d_from, const double* d_to, 
             }
             if (rc != MPTG_E_STATE) return rc;  // E_STATE: no wave kernel for this arm, use the generic one
         }
-        hipLaunchKernelGGL(link_bisect, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->dim, scene_args(ctx),
+        hipLaunchKernelGGL(link_bisect<false>, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->dim,
+                           scene_args(ctx), ctx->link_param, d_from, d_to, n, known_valid, d_mask, d_verdict,
+                           d_checks);
+        MPTG_HIP(ctx, hipGetLastError());
+        return MPTG_OK;
+    }
+    if (ctx->link_mode == MPTG_LINK_BISECT_L2) {
+        hipLaunchKernelGGL(link_bisect<true>, dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->dim, scene_args(ctx),
                            ctx->link_param, d_from, d_to, n, known_valid, d_mask, d_verdict, d_checks);
         MPTG_HIP(ctx, hipGetLastError());
         return MPTG_OK;
@@ -797,9 +842,9 @@ int link_dev_noalloc(mptg_ctx* ctx, const double* d_from, const double* d_to, in
                      int known_valid, uint8_t* d_verdict, uint32_t* d_checks) {
     if (ctx->dim <= 0) return fail(ctx, MPTG_E_STATE, "mptg_space must be called first");
     if (ctx->scene.kind == MPTG_SCENE_NONE) return fail(ctx, MPTG_E_STATE, "no scene uploaded");
-    if (ctx->link_mode != MPTG_LINK_DMV || ctx->scene.kind == MPTG_SCENE_MESH)
+    if (ctx->link_mode != MPTG_LINK_DMV || ctx->scene.kind == MPTG_SCENE_MESH || ctx->scene.kind == MPTG_SCENE_HOST)
         return fail(ctx, MPTG_E_STATE, "the allocation-free link covers discrete-motion-validator scenes "
-                                       "(boxes, grids, arms with DMV links)");
+                                       "(boxes, grids, arms, circles and rectangles with DMV links)");
     if (ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > ctx->dim)
         return fail(ctx, MPTG_E_ARG, "arm has more links than the state has coordinates");
     if (ctx->scene.kind != MPTG_SCENE_ARM && ctx->dim < 2) return fail(ctx, MPTG_E_ARG, "2-D scenes need dim >= 2");

@@ -11,24 +11,42 @@
 // setRange/getRange, size, addStart, solve(doneFn), solveFor, solveUntil,
 // solved, solution(), solution(fn), printStats, visitGraph.
 //
-// Scenario contract (src/mpt/scenario.hpp:93-130): space(), bounds(),
-// goal(), and — instead of valid()/link() being called per state from N
-// worker threads — a description of valid()/link() as data:
-//     void gpu_scene(mptg_ctx*) const;   // uploads space, obstacles, link form
-// States are any type with operator[] and a compile-time dimension given by
-// Scenario::kDimensions.  Errors from the C ABI are rethrown as
-// std::runtime_error (as the reference throws at prrt.hpp:198).
+// Scenario contract (src/mpt/scenario.hpp:93-130): bounds, goal and
+// distance as below, and valid()/link() in one of two forms:
+//   * as data for the device:
+//         void gpu_scene(mptg_ctx*) const;  // uploads space, obstacles, link form
+//   * as the Scenario's own C++ code, when it has no gpu_scene():
+//         bool valid(const State&) const;
+//         bool link(const State&, const State&) const;
+//         static constexpr int kMetric = MPTG_METRIC_L2;  // or _L1, or _SE3 (then kSO3Weight)
+//     Sampling, nearest neighbours, steering and the tree stay on the GPU;
+//     every batch of valid()/link() calls an iteration needs is run on a
+//     host thread pool of max_threads<N> threads (default: the hardware's
+//     thread count), as the reference's workers call them (prrt.hpp:439-457).
+// Both forms provide State, kDimensions, bounds_to(lo, hi), goal_state(),
+// goal_radius() and distance(a, b).  States are any type with operator[].
+// Errors from the C ABI are rethrown as std::runtime_error (as the reference
+// throws at prrt.hpp:198); an exception thrown by the Scenario's valid() or
+// link() is rethrown as itself.
 #pragma once
 
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
+#include <exception>
+#include <functional>
 #include <iostream>
 #include <limits>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "mptg.h"
@@ -45,9 +63,11 @@ template <bool b>
 struct report_stats {
     static constexpr bool value = b;
 };
-// Accepted for source compatibility; one host thread drives a context.
+// Host threads for a Scenario's own valid()/link() (0: the hardware's
+// thread count).  One host thread drives the GPU context either way.
 template <int N>
 struct max_threads {
+    static_assert(N >= 0, "max_threads must be >= 0");
     static constexpr int value = N;
 };
 // Visiting order of link()'s waypoints: reference, or surrogate-ranked.
@@ -78,16 +98,119 @@ namespace detail {
 inline void check(mptg_ctx* ctx, int rc) {
     if (rc != MPTG_OK) throw std::runtime_error(std::string("mptg: ") + mptg_last_error(ctx));
 }
+
+// Fork-join pool for a host Scenario's valid()/link() batches: the calling
+// thread and n - 1 workers take indices from a shared counter; run() returns
+// when every index is done.  The first exception any task throws is kept.
+class HostPool {
+   public:
+    explicit HostPool(int n) {
+        if (n <= 0) n = static_cast<int>(std::thread::hardware_concurrency());
+        if (n <= 0) n = 1;
+        for (int i = 1; i < n; ++i) workers_.emplace_back([this] { loop(); });
+    }
+    HostPool(const HostPool&) = delete;
+    HostPool& operator=(const HostPool&) = delete;
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    int threads() const { return static_cast<int>(workers_.size()) + 1; }
+
+    // fn(i) for i in [0, n); rethrows the first exception of any call
+    void run(int64_t n, const std::function<void(int64_t)>& fn) {
+        if (n <= 0) return;
+        error_ = nullptr;
+        if (workers_.empty() || n < 2) {
+            for (int64_t i = 0; i < n; ++i) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = &fn;
+            n_ = n;
+            next_.store(0);
+            busy_ = static_cast<int>(workers_.size());
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [this] { return busy_ == 0; });
+        job_ = nullptr;
+        if (error_) std::rethrow_exception(error_);
+    }
+
+   private:
+    void work() {
+        for (int64_t i; (i = next_.fetch_add(1)) < n_;) {
+            try {
+                (*job_)(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(err_m_);
+                if (!error_) error_ = std::current_exception();
+                next_.store(n_);  // the rest is not needed
+            }
+        }
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            {
+                std::lock_guard<std::mutex> g(m_);
+                if (--busy_ == 0) done_.notify_one();
+            }
+        }
+    }
+
+    std::vector<std::thread> workers_;
+    std::mutex m_, err_m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int64_t)>* job_ = nullptr;
+    int64_t n_ = 0;
+    std::atomic<int64_t> next_{0};
+    int busy_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::exception_ptr error_;
+};
+
+template <typename S, typename = void>
+struct has_gpu_scene : std::false_type {};
+template <typename S>
+struct has_gpu_scene<S, std::void_t<decltype(std::declval<const S&>().gpu_scene(static_cast<mptg_ctx*>(nullptr)))>>
+    : std::true_type {};
+
+template <typename S, typename = void>
+struct so3_weight {
+    static constexpr double value = 1.0;
+};
+template <typename S>
+struct so3_weight<S, std::void_t<decltype(S::kSO3Weight)>> {
+    static constexpr double value = S::kSO3Weight;
+};
 }  // namespace detail
 
 // Strategy type produced by PRRT<options...>; option order is free, as in
 // the reference's option parser (src/mpt/impl/packs.hpp:105-125).
-template <int Batch, bool Stats, int Order, int Graph = 0>
+template <int Batch, bool Stats, int Order, int Graph = 0, int Threads = 0>
 struct PRRTStrategy {
     static constexpr int batch = Batch;
     static constexpr bool stats = Stats;
     static constexpr int order = Order;
     static constexpr int graph = Graph;
+    static constexpr int threads = Threads;
 };
 
 template <typename... Opts>
@@ -98,6 +221,7 @@ struct PRRTOptions<> {
     static constexpr bool stats = false;
     static constexpr int order = MPTG_ORDER_REFERENCE;
     static constexpr int graph = 0;
+    static constexpr int threads = 0;
 };
 template <int B, typename... Rest>
 struct PRRTOptions<gpu_batch<B>, Rest...> : PRRTOptions<Rest...> {
@@ -108,7 +232,9 @@ struct PRRTOptions<report_stats<S>, Rest...> : PRRTOptions<Rest...> {
     static constexpr bool stats = S;
 };
 template <int N, typename... Rest>
-struct PRRTOptions<max_threads<N>, Rest...> : PRRTOptions<Rest...> {};
+struct PRRTOptions<max_threads<N>, Rest...> : PRRTOptions<Rest...> {
+    static constexpr int threads = N;
+};
 template <int O, typename... Rest>
 struct PRRTOptions<check_order<O>, Rest...> : PRRTOptions<Rest...> {
     static constexpr int order = O;
@@ -120,16 +246,17 @@ struct PRRTOptions<graph_iterations<G>, Rest...> : PRRTOptions<Rest...> {
 
 template <typename... Opts>
 using PRRT = PRRTStrategy<PRRTOptions<Opts...>::batch, PRRTOptions<Opts...>::stats, PRRTOptions<Opts...>::order,
-                          PRRTOptions<Opts...>::graph>;
+                          PRRTOptions<Opts...>::graph, PRRTOptions<Opts...>::threads>;
 
 // PRRT* (src/mpt/prrt_star.hpp:55-110): the PRRT options plus rewiring.
-template <int Batch, bool Stats, int Order, int K, int Neighbors = MPTG_REWIRE_K_NEAREST>
+template <int Batch, bool Stats, int Order, int K, int Neighbors = MPTG_REWIRE_K_NEAREST, int Threads = 0>
 struct PRRTStarStrategy {
     static constexpr int batch = Batch;
     static constexpr bool stats = Stats;
     static constexpr int order = Order;
     static constexpr int k = K;  // 0: the reference's k-nearest formula
     static constexpr int neighbors = Neighbors;
+    static constexpr int threads = Threads;
 };
 
 template <typename... Opts>
@@ -145,6 +272,8 @@ struct PRRTStarOptions<Opt, Rest...> : PRRTStarOptions<Rest...> {
     static constexpr int batch = PRRTOptions<Opt>::batch != PRRTOptions<>::batch ? PRRTOptions<Opt>::batch : Base::batch;
     static constexpr bool stats = PRRTOptions<Opt>::stats || Base::stats;
     static constexpr int order = PRRTOptions<Opt>::order != PRRTOptions<>::order ? PRRTOptions<Opt>::order : Base::order;
+    static constexpr int threads =
+        PRRTOptions<Opt>::threads != PRRTOptions<>::threads ? PRRTOptions<Opt>::threads : Base::threads;
 };
 template <typename... Rest>
 struct PRRTStarOptions<rewire_k_nearest, Rest...> : PRRTStarOptions<Rest...> {};
@@ -164,20 +293,20 @@ struct PRRTStarOptions<rewire_k<K>, Rest...> : PRRTStarOptions<Rest...> {
 template <typename... Opts>
 using PRRTStar = PRRTStarStrategy<PRRTStarOptions<Opts...>::batch, PRRTStarOptions<Opts...>::stats,
                                   PRRTStarOptions<Opts...>::order, PRRTStarOptions<Opts...>::k,
-                                  PRRTStarOptions<Opts...>::neighbors>;
+                                  PRRTStarOptions<Opts...>::neighbors, PRRTStarOptions<Opts...>::threads>;
 
 template <typename Scenario, typename Strategy>
 class GpuPRRT;
 
 template <typename Scenario, typename Algorithm>
 struct PlannerResolver;
-template <typename Scenario, int B, bool S, int O, int G>
-struct PlannerResolver<Scenario, PRRTStrategy<B, S, O, G>> {
-    using type = GpuPRRT<Scenario, PRRTStrategy<B, S, O, G>>;
+template <typename Scenario, int B, bool S, int O, int G, int T>
+struct PlannerResolver<Scenario, PRRTStrategy<B, S, O, G, T>> {
+    using type = GpuPRRT<Scenario, PRRTStrategy<B, S, O, G, T>>;
 };
-template <typename Scenario, int B, bool S, int O, int K, int N>
-struct PlannerResolver<Scenario, PRRTStarStrategy<B, S, O, K, N>> {
-    using type = GpuPRRT<Scenario, PRRTStarStrategy<B, S, O, K, N>>;
+template <typename Scenario, int B, bool S, int O, int K, int N, int T>
+struct PlannerResolver<Scenario, PRRTStarStrategy<B, S, O, K, N, T>> {
+    using type = GpuPRRT<Scenario, PRRTStarStrategy<B, S, O, K, N, T>>;
 };
 
 template <typename Scenario, typename Algorithm>
@@ -186,8 +315,8 @@ using Planner = typename PlannerResolver<Scenario, Algorithm>::type;
 // ---- the planner ---------------------------------------------------------------
 template <typename S>
 struct is_star : std::false_type {};
-template <int B, bool S, int O, int K, int N>
-struct is_star<PRRTStarStrategy<B, S, O, K, N>> : std::true_type {};
+template <int B, bool S, int O, int K, int N, int T>
+struct is_star<PRRTStarStrategy<B, S, O, K, N, T>> : std::true_type {};
 
 template <typename Scenario, typename Strategy>
 class GpuPRRT {
@@ -195,10 +324,14 @@ class GpuPRRT {
     using State = typename Scenario::State;
     static constexpr int kDim = Scenario::kDimensions;
     static constexpr bool kStar = is_star<Strategy>::value;
+    // the Scenario's valid()/link() run on the host (it has no gpu_scene())
+    static constexpr bool kHostScene = !detail::has_gpu_scene<Scenario>::value;
 
    private:
     Scenario scenario_;
     mptg_ctx* ctx_ = nullptr;
+    std::unique_ptr<detail::HostPool> pool_;
+    std::exception_ptr hostError_;
     uint64_t seed_;
     double maxDistance_ = std::numeric_limits<double>::infinity();
     double goalBias_ = 0.01;
@@ -212,6 +345,53 @@ class GpuPRRT {
         State s{};
         for (int i = 0; i < kDim; ++i) s[i] = in[i];
         return s;
+    }
+
+    // mptg_valid_fn / mptg_link_fn over the Scenario's own valid()/link()
+    static int hostValid(void* user, const double* q, int64_t n, uint8_t* ok) {
+        auto* self = static_cast<GpuPRRT*>(user);
+        try {
+            self->pool_->run(n, [&](int64_t i) { ok[i] = self->scenario_.valid(fromDoubles(q + i * kDim)) ? 1 : 0; });
+            return 0;
+        } catch (...) {
+            self->hostError_ = std::current_exception();
+            return 1;
+        }
+    }
+    static int hostLink(void* user, const double* a, const double* b, int64_t n, uint8_t* ok) {
+        auto* self = static_cast<GpuPRRT*>(user);
+        try {
+            self->pool_->run(n, [&](int64_t i) {
+                ok[i] = self->scenario_.link(fromDoubles(a + i * kDim), fromDoubles(b + i * kDim)) ? 1 : 0;
+            });
+            return 0;
+        } catch (...) {
+            self->hostError_ = std::current_exception();
+            return 1;
+        }
+    }
+
+    // a failing entry point rethrows the Scenario's own exception if it caused it
+    void check(int rc) {
+        if (rc != MPTG_OK && hostError_) {
+            std::exception_ptr e = hostError_;
+            hostError_ = nullptr;
+            std::rethrow_exception(e);
+        }
+        detail::check(ctx_, rc);
+    }
+
+    void uploadScene() {
+        if constexpr (kHostScene) {
+            static_assert(Scenario::kMetric == MPTG_METRIC_L2 || Scenario::kMetric == MPTG_METRIC_L1 ||
+                              Scenario::kMetric == MPTG_METRIC_SE3,
+                          "a Scenario without gpu_scene() names its space by kMetric");
+            pool_ = std::make_unique<detail::HostPool>(Strategy::threads);
+            detail::check(ctx_, mptg_space(ctx_, Scenario::kMetric, kDim, detail::so3_weight<Scenario>::value, 1.0));
+            detail::check(ctx_, mptg_scene_host(ctx_, &GpuPRRT::hostValid, &GpuPRRT::hostLink, this));
+        } else {
+            scenario_.gpu_scene(ctx_);
+        }
     }
 
     void configure() {
@@ -244,7 +424,7 @@ class GpuPRRT {
         if (mptg_create(device, &ctx_) != MPTG_OK)
             throw std::runtime_error(std::string("mptg: ") + mptg_last_error(nullptr));
         try {
-            scenario_.gpu_scene(ctx_);
+            uploadScene();
             configure();
         } catch (...) {
             mptg_destroy(ctx_);
@@ -283,7 +463,7 @@ class GpuPRRT {
     void addStart(const State& q) {
         double d[kDim];
         toDoubles(q, d);
-        detail::check(ctx_, mptg_prrt_add_start(ctx_, d));
+        check(mptg_prrt_add_start(ctx_, d));
     }
 
     // One batched iteration per doneFn() check (prrt.hpp:194-201); with
@@ -293,8 +473,7 @@ class GpuPRRT {
         if (size() == 0) throw std::runtime_error("there are no valid initial states");
         int64_t per_check = 1;
         if constexpr (!kStar) per_check = Strategy::graph > 0 ? Strategy::graph : 1;
-        while (!doneFn())
-            detail::check(ctx_, mptg_prrt_run(ctx_, std::numeric_limits<int32_t>::max(), per_check, nullptr));
+        while (!doneFn()) check(mptg_prrt_run(ctx_, std::numeric_limits<int32_t>::max(), per_check, nullptr));
     }
 
     template <typename Duration>
@@ -311,8 +490,7 @@ class GpuPRRT {
     // Grow until the tree holds at least n nodes (the demos' done condition).
     void solveUntilSize(std::size_t n) {
         if (size() == 0) throw std::runtime_error("there are no valid initial states");
-        detail::check(ctx_, mptg_prrt_run(ctx_, static_cast<int64_t>(n), std::numeric_limits<int64_t>::max(),
-                                          nullptr));
+        check(mptg_prrt_run(ctx_, static_cast<int64_t>(n), std::numeric_limits<int64_t>::max(), nullptr));
     }
 
     bool solved() const {

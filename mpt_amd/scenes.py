@@ -29,8 +29,8 @@ from typing import Optional
 import numpy as np
 
 METRIC_L2, METRIC_L1, METRIC_SE3 = 0, 1, 2
-SCENE_NONE, SCENE_BOXES, SCENE_GRID, SCENE_ARM, SCENE_MESH = 0, 1, 2, 3, 4
-LINK_DMV, LINK_BISECT = 0, 1
+SCENE_NONE, SCENE_BOXES, SCENE_GRID, SCENE_ARM, SCENE_MESH, SCENE_SHAPES, SCENE_HOST = 0, 1, 2, 3, 4, 5, 6
+LINK_DMV, LINK_BISECT, LINK_BISECT_L2 = 0, 1, 2
 ORDER_REFERENCE, ORDER_SURROGATE = 0, 1
 
 
@@ -57,6 +57,7 @@ class Scene:
     disks: Optional[np.ndarray] = None      # (n, 3) cx cy r
     robot_tris: Optional[np.ndarray] = None  # (n, 9) float32, robot frame
     env_tris: Optional[np.ndarray] = None    # (n, 9) float32, world frame
+    circles: Optional[np.ndarray] = None     # (n, 3) cx cy r (shapes scene; rectangles in `boxes`)
     name: str = ""
     extra: dict = field(default_factory=dict)
 
@@ -88,6 +89,28 @@ def boxes_scene(seed: int = 1, nboxes: int = 16, size: float = 100.0) -> Scene:
     return Scene(metric=METRIC_L2, dim=2, kind=SCENE_BOXES, link_mode=LINK_DMV, link_param=0.1,
                  lo=np.zeros(2), hi=np.full(2, size), start=start, goal=goal, range=2.0, boxes=boxes,
                  name=f"boxes{nboxes}_seed{seed}")
+
+
+def shapes_scene(link_mode: int = LINK_BISECT_L2, link_param: float = 1.0) -> Scene:
+    """The holonomic 2-D point demo (demo/holonomic_2d_point_planning.cpp:
+    1024 x 512 canvas, three circles, three rectangles, start (30, 30), goal
+    (994, 482), goal radius 1e-6; demo/holonomic_2d_point_scenario.hpp:88):
+    its link() is the circles' closed form, then per rectangle the endpoints
+    and the midpoint recursion to squared length < 1."""
+    circles = np.array([[170.0, 140.0, 80.0], [800.0, 70.0, 50.0], [900.0, 380.0, 70.0]])
+    rects = np.array([[375.0, 140.0, 520.0, 220.0], [200.0, 320.0, 390.0, 390.0], [600.0, 200.0, 680.0, 450.0]])
+    return Scene(metric=METRIC_L2, dim=2, kind=SCENE_SHAPES, link_mode=link_mode, link_param=link_param,
+                 lo=np.zeros(2), hi=np.array([1024.0, 512.0]), start=np.array([30.0, 30.0]),
+                 goal=np.array([994.0, 482.0]), boxes=rects, circles=circles, name="holonomic_2d_point")
+
+
+def shapes_valid_np(scene: Scene, q: np.ndarray) -> np.ndarray:
+    """Vectorised Holonomic2DPointScenario::valid (circles, then rectangles)."""
+    ok = boxes_valid_np(scene.boxes, q)
+    for cx, cy, r in scene.circles:
+        dx, dy = q[:, 0] - cx, q[:, 1] - cy
+        ok &= dx * dx + dy * dy > r * r
+    return ok
 
 
 def _stamp_disk(grid: np.ndarray, cx: float, cy: float, r: float) -> None:

@@ -33,6 +33,7 @@ class OrcScene(C.Structure):
         ("disks", _dp), ("ndisks", C.c_int32),
         ("robot_tris", C.POINTER(C.c_float)), ("nrobot", C.c_int32),
         ("env_tris", C.POINTER(C.c_float)), ("nenv", C.c_int32), ("mesh_index", C.c_void_p),
+        ("circles", _dp), ("ncircles", C.c_int32),
     ]
 
 
@@ -147,6 +148,10 @@ class Oracle:
             g = np.ascontiguousarray(scene.grid, dtype=np.uint8)
             self._keep.append(g)
             s.grid, s.width, s.height = _p(g, C.c_uint8), g.shape[1], g.shape[0]
+        if getattr(scene, "circles", None) is not None:
+            ci = _f64(scene.circles).reshape(-1, 3)
+            self._keep.append(ci)
+            s.circles, s.ncircles = _p(ci, C.c_double), ci.shape[0]
         if scene.lengths is not None:
             ln = _f64(scene.lengths)
             dk = _f64(scene.disks).reshape(-1, 3)

@@ -22,8 +22,9 @@ extern "C" {
 #endif
 
 enum { ORC_METRIC_L2 = 0, ORC_METRIC_L1 = 1, ORC_METRIC_SE3 = 2 };
-enum { ORC_SCENE_NONE = 0, ORC_SCENE_BOXES = 1, ORC_SCENE_GRID = 2, ORC_SCENE_ARM = 3, ORC_SCENE_MESH = 4 };
-enum { ORC_LINK_DMV = 0, ORC_LINK_BISECT = 1 };
+enum { ORC_SCENE_NONE = 0, ORC_SCENE_BOXES = 1, ORC_SCENE_GRID = 2, ORC_SCENE_ARM = 3, ORC_SCENE_MESH = 4,
+       ORC_SCENE_SHAPES = 5 };
+enum { ORC_LINK_DMV = 0, ORC_LINK_BISECT = 1, ORC_LINK_BISECT_L2 = 2 };
 
 typedef struct {
     int32_t metric;
@@ -46,6 +47,8 @@ typedef struct {
     const float* env_tris;   /* nenv x 9 (world frame) */
     int32_t nenv;
     const void* mesh_index;  /* from orc_mesh_index_build */
+    const double* circles;   /* ncircles x [cx, cy, r] (shapes scene; rectangles are `boxes`) */
+    int32_t ncircles;
 } orc_scene;
 
 /* Uniform-grid index over the environment triangles of a mesh scene (the

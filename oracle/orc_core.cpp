@@ -20,6 +20,13 @@
 //     hit exits) and bisectLink (L-inf < tol, (a+b)/2, pre-order)
 //                                                          demo/link_manipulator_scenario.hpp:99-138
 //     with Circle::segmentIsValid / distPointSegmentSquared demo/shape_hierarchy.hpp:234-237,259-270
+//   * shapes: Holonomic2DPointScenario::valid/link (circles, then rectangles;
+//     Circle::pointIsValid, the closed-form Circle::segmentIsValid, and per
+//     rectangle the endpoints and the midpoint recursion bisectSegment to
+//     squared length < tol^2)                              demo/holonomic_2d_point_scenario.hpp:95-113,
+//                                                          demo/shape_hierarchy.hpp:184-231,259-270
+//   * BISECT_L2 link: valid(a), valid(b), then recursive midpoints until the
+//     squared L2 length is < tol^2                         demo/png_2d_scenario.hpp:112-117,152-165
 //   * mesh: SE3RigidBodyScenario::valid (robot vs environment triangles)
 //                                                          demo/se3_rigid_body_scenario.hpp:288-312
 //     (orc_mesh.cpp)
@@ -203,8 +210,19 @@ bool valid_grid(const orc_scene* sc, const double* q) {
     return sc->grid[(int64_t)sc->width * y + x] == 0;
 }
 
+// Circle::pointIsValid: (q - c).squaredNorm() > r * r, demo/shape_hierarchy.hpp:222-226
+bool valid_circles(const orc_scene* sc, const double* q) {
+    for (int i = 0; i < sc->ncircles; ++i) {
+        const double* c = sc->circles + 3 * i;
+        double dx = q[0] - c[0], dy = q[1] - c[1];
+        if (!(dx * dx + dy * dy > c[2] * c[2])) return false;
+    }
+    return true;
+}
+
 bool valid(const orc_scene* sc, const double* q) {
     switch (sc->kind) {
+        case ORC_SCENE_SHAPES: return valid_circles(sc, q) && valid_boxes(sc, q);
         case ORC_SCENE_BOXES: return valid_boxes(sc, q);
         case ORC_SCENE_GRID: return valid_grid(sc, q);
         case ORC_SCENE_ARM: return valid_arm(sc, q);
@@ -246,12 +264,23 @@ std::size_t dmv_steps(const orc_scene* sc, const double* a, const double* b) {
 
 // Pre-order visit of the recursive-midpoint waypoints (bisectLink).  The
 // visitor receives the midpoint state and returns false to stop.
+// BISECT stops below L-inf length tol (bisectLink); BISECT_L2 below squared L2
+// length tol^2 (bisectSegment / validSegment: (b - a).squaredNorm() < tol * tol).
 template <typename Visit>
 bool bisect_visit(const orc_scene* sc, const double* a, const double* b, Visit& visit, int depth = 0) {
     int d = sc->dim;
-    double m = 0.0;
-    for (int i = 0; i < d; ++i) m = std::max(m, std::fabs(a[i] - b[i]));
-    if (m < sc->link_param) return true;
+    if (sc->link_mode == ORC_LINK_BISECT_L2) {
+        double s2 = 0.0;
+        for (int i = 0; i < d; ++i) {
+            double t = b[i] - a[i];
+            s2 = s2 + t * t;
+        }
+        if (s2 < sc->link_param * sc->link_param) return true;
+    } else {
+        double m = 0.0;
+        for (int i = 0; i < d; ++i) m = std::max(m, std::fabs(a[i] - b[i]));
+        if (m < sc->link_param) return true;
+    }
     double mid[64];
     for (int i = 0; i < d; ++i) mid[i] = (a[i] + b[i]) / 2;
     if (!visit(static_cast<const double*>(mid))) return false;
@@ -266,14 +295,25 @@ struct LinkResult {
 
 LinkResult link(const orc_scene* sc, const double* a, const double* b, bool known_valid) {
     uint32_t checks = 0;
-    if (sc->link_mode == ORC_LINK_BISECT) {
+    if (sc->link_mode == ORC_LINK_BISECT || sc->link_mode == ORC_LINK_BISECT_L2) {
+        // circles and rectangles: every circle's closed form first, then the
+        // endpoints and midpoints against the rectangles (every rectangle's
+        // recursion visits the same midpoints, so testing them all per point
+        // decides the same AND; `checks` counts those points)
+        const bool shapes = sc->kind == ORC_SCENE_SHAPES;
+        if (shapes)
+            for (int i = 0; i < sc->ncircles; ++i) {
+                const double* c = sc->circles + 3 * i;
+                if (!(dist_point_segment_sq(c[0], c[1], a[0], a[1], b[0], b[1]) > c[2] * c[2])) return {false, 0};
+            }
+        auto point_ok = [&](const double* q) { return shapes ? valid_boxes(sc, q) : valid(sc, q); };
         ++checks;
-        if (!known_valid && !valid(sc, a)) return {false, checks};
+        if (!known_valid && !point_ok(a)) return {false, checks};
         ++checks;
-        if (!known_valid && !valid(sc, b)) return {false, checks};
+        if (!known_valid && !point_ok(b)) return {false, checks};
         auto visit = [&](const double* q) {
             ++checks;
-            return valid(sc, q);
+            return point_ok(q);
         };
         bool ok = bisect_visit(sc, a, b, visit);
         return {ok, checks};
@@ -392,7 +432,7 @@ int64_t orc_dmv_order(int64_t steps, int64_t* out, int64_t cap) {
 int64_t orc_link_waypoints(const orc_scene* sc, const double* a, const double* b, double* out, int64_t cap) {
     int d = sc->dim;
     int64_t cnt = 0;
-    if (sc->link_mode == ORC_LINK_BISECT) {
+    if (sc->link_mode == ORC_LINK_BISECT || sc->link_mode == ORC_LINK_BISECT_L2) {
         auto visit = [&](const double* q) {
             if (cnt < cap) std::memcpy(out + cnt * d, q, sizeof(double) * d);
             ++cnt;

@@ -1,14 +1,18 @@
 """The C++ host layer (mpt_amd/include/mpt_amd/*.hpp over include/mptg.h):
 headers compile standalone, and the examples (planar-arm PRRT, SE(3)
-rigid-body PRRT*) plan on the GPU."""
+rigid-body PRRT*, the holonomic 2-D point demo with its Scenario's own C++
+valid()/link()) plan on the GPU; the last one's trees equal the oracle's."""
+import dataclasses
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 EXAMPLE = ROOT / "mpt_amd" / "examples" / "planar_arm.bin"
 EXAMPLE_STAR = ROOT / "mpt_amd" / "examples" / "rigid_body_star.bin"
+EXAMPLE_HOST = ROOT / "mpt_amd" / "examples" / "holonomic_host.bin"
 
 
 def test_headers_compile_standalone(tmp_path):
@@ -28,7 +32,9 @@ def test_headers_compile_standalone(tmp_path):
                    "static_assert(S::k == 20 && S::batch == 512 && S::stats);\n"
                    "static_assert(mpt_amd::PRRTStar<>::k == 0 && mpt_amd::PRRTStar<>::batch == 8192);\n"
                    "using R = mpt_amd::Planner<mpt_amd::RigidBodyMeshes, S>;\n"
-                   "static_assert(R::kStar && R::kDim == 7);\n"
+                   "static_assert(R::kStar && R::kDim == 7 && !R::kHostScene);\n"
+                   "static_assert(mpt_amd::PRRT<mpt_amd::max_threads<3>>::threads == 3 && mpt_amd::PRRT<>::threads == 0);\n"
+                   "static_assert(mpt_amd::PRRTStar<mpt_amd::max_threads<5>, mpt_amd::rewire_k<8>>::threads == 5);\n"
                    "int main() { return 0; }\n")
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", str(ROOT / "include"), "-I",
                         str(ROOT / "mpt_amd" / "include"), str(src)], capture_output=True, text=True)
@@ -36,7 +42,7 @@ def test_headers_compile_standalone(tmp_path):
 
 
 def test_example_is_built():
-    assert EXAMPLE.exists() and EXAMPLE_STAR.exists(), "run __graft_entry__.build()"
+    assert EXAMPLE.exists() and EXAMPLE_STAR.exists() and EXAMPLE_HOST.exists(), "run __graft_entry__.build()"
 
 
 @pytest.mark.gpu
@@ -56,3 +62,33 @@ def test_rigid_body_star_example_plans_on_gpu():
     print(r.stdout[-2000:], flush=True)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "path_valid=1" in r.stdout and "solved=1" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,batch,target", [("prrt1", 1, 600), ("prrt256", 256, 5000), ("star64", 64, 3000)])
+def test_host_scenario_example_equals_oracle(tmp_path, mode, batch, target):
+    """A Scenario with only C++ valid()/link() (no gpu_scene()) plans through
+    mpt_amd::Planner; its tree (and PRRT* path costs) equal the oracle's on
+    the same demo scenario."""
+    import oracle
+    from mpt_amd import scenes
+    print("holonomic host-scenario example", mode, flush=True)
+    out = tmp_path / "tree.bin"
+    r = subprocess.run([str(EXAMPLE_HOST), mode, "7", str(target), "60", "0.05", str(out)], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    raw = out.read_bytes()
+    n = int(np.frombuffer(raw[:8], dtype=np.int64)[0])
+    states = np.frombuffer(raw[8:8 + 16 * n], dtype=np.float64).reshape(n, 2)
+    parents = np.frombuffer(raw[8 + 16 * n:8 + 20 * n], dtype=np.int32)
+    sc = dataclasses.replace(scenes.shapes_scene(), range=60.0, goal_bias=0.05)
+    o = oracle.Oracle(sc)
+    if mode == "star64":
+        costs = np.frombuffer(raw[8 + 20 * n:8 + 28 * n], dtype=np.float64)
+        on, op, oc, og, ost, oss = o.prrt_star(7, batch, target, k=20)
+        assert np.array_equal(costs, oc)
+    else:
+        on, op, og, ost = o.prrt(7, batch, target)
+    assert np.array_equal(states, on) and np.array_equal(parents, op)
+    assert f"link_calls={ost['link_calls']} " in r.stdout
