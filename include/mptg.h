@@ -337,6 +337,8 @@ typedef struct mptg_tune_params {
     int32_t knn_grid;               /* L1/L2 in 2..8 dimensions on trees of 2^16 nodes or more: 1 a
                                        uniform grid built per call, searched cell by cell; 0 the scans */
     int32_t dmv_edge_lanes;         /* DMV link: lanes sharing one edge's checks (1, 2, 4, .., 64) */
+    int32_t knn_q8;                 /* k = 1 under L1 in 8 or more dimensions: 1 a byte prefilter
+                                       (four coordinates per sum-of-absolute-differences), 0 FP32 */
 } mptg_tune_params;
 
 enum {

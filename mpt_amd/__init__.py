@@ -443,14 +443,14 @@ class Context:
         return {"knn_blocks_per_cu": t.knn_blocks_per_cu, "dmv_long_waves_per_cu": t.dmv_long_waves_per_cu,
                 "mesh_waves_per_cu": t.mesh_waves_per_cu, "arm_tail_waves": t.arm_tail_waves,
                 "arm_cull": t.arm_cull, "mesh_edge_waves": t.mesh_edge_waves, "knn_select": t.knn_select,
-                "knn_grid": t.knn_grid, "dmv_edge_lanes": t.dmv_edge_lanes}
+                "knn_grid": t.knn_grid, "dmv_edge_lanes": t.dmv_edge_lanes, "knn_q8": t.knn_q8}
 
     def tune_set(self, **kw):
         cur = self.tune_get()
         cur.update(kw)
         t = _native.TuneParams(cur["knn_blocks_per_cu"], cur["dmv_long_waves_per_cu"], cur["mesh_waves_per_cu"],
                                cur["arm_tail_waves"], cur["arm_cull"], cur["mesh_edge_waves"], cur["knn_select"],
-                               cur["knn_grid"], cur["dmv_edge_lanes"])
+                               cur["knn_grid"], cur["dmv_edge_lanes"], cur["knn_q8"])
         self._chk(_lib.mptg_tune_set(self._h, C.byref(t)))
 
     def autotune_knn(self, d_q: int, nq: int, k: int) -> dict:

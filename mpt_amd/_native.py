@@ -32,7 +32,7 @@ class TuneParams(C.Structure):
     _fields_ = [("knn_blocks_per_cu", C.c_int32), ("dmv_long_waves_per_cu", C.c_int32),
                 ("mesh_waves_per_cu", C.c_int32), ("arm_tail_waves", C.c_int32), ("arm_cull", C.c_int32),
                 ("mesh_edge_waves", C.c_int32), ("knn_select", C.c_int32), ("knn_grid", C.c_int32),
-                ("dmv_edge_lanes", C.c_int32)]
+                ("dmv_edge_lanes", C.c_int32), ("knn_q8", C.c_int32)]
 
 
 class TuneReport(C.Structure):

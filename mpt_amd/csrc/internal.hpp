@@ -155,8 +155,9 @@ struct mptg_ctx {
     // launch shapes (mptg_autotune_*): kNN blocks per CU, waves per CU of the
     // persistent DMV long-edge kernel and of the mesh kernels, waves per edge
     // of the arm continuation, arm link disk culling (0 wave box, 1 per lane),
-    // waves per mesh edge, kNN select and grid paths, lanes per DMV edge
-    mptg_tune_params tune{8, 16, 32, 2, 0, 2, 1, 1, 8};
+    // waves per mesh edge, kNN select and grid paths, lanes per DMV edge, the
+    // byte prefilter of the k = 1 L1 scan
+    mptg_tune_params tune{8, 16, 32, 2, 0, 2, 1, 1, 8, 1};
     hipEvent_t phase_ev[6] = {};
 };
 

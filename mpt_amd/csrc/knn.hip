@@ -268,7 +268,9 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
         const int cnt = (int)(p1 - t0 < kScanTile ? p1 - t0 : kScanTile);
         __syncthreads();  // every wave is done with the previous tile
         for (int i = threadIdx.x; i < cnt * DP; i += kBlock) t32[i] = x32[t0 * DP + i];
-        for (int i = threadIdx.x; i < cnt * D; i += kBlock) t64[i] = x64[t0 * D + i];
+        // k = 1 candidates read their FP64 rows at the flush, from global memory
+        if constexpr (K > 1)
+            for (int i = threadIdx.x; i < cnt * D; i += kBlock) t64[i] = x64[t0 * D + i];
         __syncthreads();
         // four tree nodes per step, one branch for all
         constexpr int U = 4;
@@ -352,6 +354,270 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
         out_d[base + j] = top.d[j];
         out_i[base + j] = top.i[j];
     }
+}
+
+// ---- k = 1 under L1 with a byte prefilter ------------------------------------
+// Every coordinate is quantized to a byte on one grid over [-B, B] (B = the
+// context's bound on every tree and query coordinate): q(x) = round((x + B)
+// / s), s = 2B / 255.  The L1 distance of two byte rows is a sum of absolute
+// byte differences, four coordinates per v_sad_u8 (exact integer arithmetic).
+//
+// Soundness.  For |x| <= B the FP32 quantization (of the FP32 shadow row, or
+// of the FP64 query) satisfies |(x + B) / s - q(x)| <= 1/2 + 6e-5 (rounding to
+// the nearest byte, FP32 arithmetic on values <= 255, the shadow's own
+// rounding 2^-24 B / s).  Per coordinate, | |a - b| / s - |qa - qb| | <= 1 +
+// 1.2e-4, so for the sum SAD over D coordinates
+//     | L1 / s - SAD | <= D (1 + 1.2e-4) < kQ8Slack D,   kQ8Slack = 1.001.
+// A node with SAD > T = floor((d (1 + 2^-40)) / s + kQ8Slack D) therefore has
+// an exact L1 above d (by a relative 2^-40, more than the FP64 rounding of
+// either sum), and cannot beat a listed distance d.  A listed candidate with
+// byte distance a has L1 <= s (a + kQ8Slack D); a node with SAD > a + C2,
+// C2 = ceil(2 kQ8Slack D) + 1, is farther by at least s.  Queries with a
+// coordinate beyond B take no prefilter (every node gets its exact distance).
+//
+// Two passes: knn_q8_min finds every query's smallest byte distance over the
+// whole tree (atomicMin over the splits), then knn_scan_q8 takes the exact
+// FP64 distance of only the nodes within C2 of it -- the nodes that can still
+// be the nearest, typically a handful per query.
+constexpr int kQ8Tile = 256;
+constexpr double kQ8Slack = 1.001;
+
+__device__ __forceinline__ uint32_t q8_byte(float x, float B, float inv_s) {
+    float f = (x + B) * inv_s + 0.5f;
+    f = f < 0.0f ? 0.0f : (f > 255.0f ? 255.0f : f);
+    return (uint32_t)f;
+}
+
+// The query's byte row (NW words); false when a coordinate lies beyond B.
+template <int D>
+__device__ __forceinline__ bool q8_query(const double* __restrict__ q, double bound, float Bf, float inv_sf,
+                                         uint32_t* qw) {
+    constexpr int NW = (D + 3) / 4;
+    bool inside = true;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = 4 * w + b;
+            if (i < D) {
+                const double x = q[i];
+                inside = inside && __builtin_fabs(x) <= bound;
+                v |= q8_byte((float)x, Bf, inv_sf) << (8 * b);
+            }
+        }
+        qw[w] = v;
+    }
+    return inside;
+}
+
+// Stages tree nodes [t0, t0 + cnt) as byte rows, word w of node j at
+// t8[w * kQ8Tile + j].  Whole block.  For D >= 8 the FP32 row pitch is 4 NW
+// floats, so word w is one aligned 16-byte load; all of a thread's loads are
+// issued before any is used.  Bytes of coordinates >= D stay 0, as in the
+// query's row.
+template <int D>
+__device__ __forceinline__ void q8_stage(const float* __restrict__ x32, int64_t t0, int cnt, float Bf, float inv_sf,
+                                         uint32_t* t8) {
+    constexpr int DP = (D <= 2) ? 2 : (D <= 4 ? 4 : (D <= 8 ? 8 : ((D + 3) / 4) * 4));
+    constexpr int NW = (D + 3) / 4;
+    static_assert(D >= 8 && DP == 4 * NW, "byte rows read whole FP32 words");
+    constexpr int WPT = (NW * kQ8Tile + kBlock - 1) / kBlock;  // words per thread, at most
+    constexpr uint32_t kLast = D % 4 == 0 ? 0xffffffffu : ((1u << (8 * (D % 4))) - 1u);
+    const int words = cnt * NW;
+    float4 v[WPT];
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+        const int e = threadIdx.x + k * kBlock;
+        const int j = e / NW, w = e - j * NW;
+        v[k] = e < words ? *reinterpret_cast<const float4*>(x32 + (t0 + j) * DP + 4 * w) : make_float4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+        const int e = threadIdx.x + k * kBlock;
+        if (e < words) {
+            const int j = e / NW, w = e - j * NW;
+            uint32_t b = q8_byte(v[k].x, Bf, inv_sf) | (q8_byte(v[k].y, Bf, inv_sf) << 8) |
+                         (q8_byte(v[k].z, Bf, inv_sf) << 16) | (q8_byte(v[k].w, Bf, inv_sf) << 24);
+            if (w == NW - 1) b &= kLast;
+            t8[w * kQ8Tile + j] = b;
+        }
+    }
+}
+
+// Pass 1: gmin[q] = min over the tree of the byte distance (gmin starts at
+// 0xffffffff; a query beyond B leaves it there).
+template <int D>
+__global__ __launch_bounds__(kBlock) void knn_q8_min(const float* __restrict__ x32, int64_t n, int64_t chunk,
+                                                     const double* __restrict__ q, int64_t nq, double bound,
+                                                     uint32_t* __restrict__ gmin, const int64_t* n_dev,
+                                                     const int32_t* skip, const uint32_t* nq_dev) {
+    constexpr int NW = (D + 3) / 4;
+    __shared__ __align__(16) uint32_t t8[NW * kQ8Tile];
+    if (!knn_extent(n_dev, skip, &n, &chunk)) return;  // block-uniform
+    const int64_t live = nq_dev ? (int64_t)*nq_dev : nq;
+    if ((int64_t)blockIdx.x * kBlock >= live) return;  // block-uniform
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = qi < live;
+    const int64_t qc = active ? qi : 0;
+    const int64_t p0 = (int64_t)blockIdx.y * chunk;
+    const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
+    const float Bf = (float)bound;
+    const float inv_sf = (float)(255.0 / (2.0 * bound));
+    uint32_t qw[NW];
+    const bool inside = q8_query<D>(q + qc * D, bound, Bf, inv_sf, qw);
+    uint32_t m = 0xffffffffu;
+    for (int64_t t0 = p0; t0 < p1; t0 += kQ8Tile) {  // block-uniform bounds
+        const int cnt = (int)(p1 - t0 < kQ8Tile ? p1 - t0 : kQ8Tile);
+        __syncthreads();
+        q8_stage<D>(x32, t0, cnt, Bf, inv_sf, t8);
+        __syncthreads();
+        constexpr int U = 8;
+        int j = 0;
+        for (; j + U <= cnt; j += U) {
+            uint32_t sad[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) sad[u] = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint4 lo = *reinterpret_cast<const uint4*>(t8 + w * kQ8Tile + j);
+                const uint4 hi = *reinterpret_cast<const uint4*>(t8 + w * kQ8Tile + j + 4);
+                sad[0] = __builtin_amdgcn_sad_u8(qw[w], lo.x, sad[0]);
+                sad[1] = __builtin_amdgcn_sad_u8(qw[w], lo.y, sad[1]);
+                sad[2] = __builtin_amdgcn_sad_u8(qw[w], lo.z, sad[2]);
+                sad[3] = __builtin_amdgcn_sad_u8(qw[w], lo.w, sad[3]);
+                sad[4] = __builtin_amdgcn_sad_u8(qw[w], hi.x, sad[4]);
+                sad[5] = __builtin_amdgcn_sad_u8(qw[w], hi.y, sad[5]);
+                sad[6] = __builtin_amdgcn_sad_u8(qw[w], hi.z, sad[6]);
+                sad[7] = __builtin_amdgcn_sad_u8(qw[w], hi.w, sad[7]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) m = sad[u] < m ? sad[u] : m;
+        }
+        for (; j < cnt; ++j) {
+            uint32_t sad = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) sad = __builtin_amdgcn_sad_u8(qw[w], t8[w * kQ8Tile + j], sad);
+            m = sad < m ? sad : m;
+        }
+    }
+    if (active && inside && m != 0xffffffffu) atomicMin(gmin + qi, m);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ x32, const double* __restrict__ x64,
+                                                      int64_t n, int64_t chunk, const double* __restrict__ q,
+                                                      int64_t nq, double radius, double bound,
+                                                      double* __restrict__ out_d, int32_t* __restrict__ out_i,
+                                                      const int64_t* n_dev, const int32_t* skip,
+                                                      const uint32_t* __restrict__ gmin, const uint32_t* nq_dev) {
+    constexpr int NW = (D + 3) / 4;  // words of a byte row
+    // byte rows by word: word w of tile node j at t8[w * kQ8Tile + j]
+    __shared__ __align__(16) uint32_t t8[NW * kQ8Tile];
+    if (!knn_extent(n_dev, skip, &n, &chunk)) return;  // block-uniform
+    const int64_t live = nq_dev ? (int64_t)*nq_dev : nq;
+    if ((int64_t)blockIdx.x * kBlock >= live) return;  // block-uniform
+    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = qi < live;
+    const int64_t qc = active ? qi : 0;
+    const int64_t p0 = (int64_t)blockIdx.y * chunk;
+    const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
+    const float Bf = (float)bound;
+    const float inv_sf = (float)(255.0 / (2.0 * bound));
+    const double inv_s = 255.0 / (2.0 * bound);
+
+    uint32_t qw[NW];
+    const bool inside = q8_query<D>(q + qc * D, bound, Bf, inv_sf, qw);
+    // pass iff SAD <= thr; thr from the current bound d (see above)
+    auto thr_of = [&](double d) -> uint32_t {
+        if (!(d < __builtin_inf())) return 0xffffffffu;
+        const double t = d * (1.0 + 0x1.0p-40) * inv_s + kQ8Slack * D;
+        return t >= 4294967295.0 ? 0xffffffffu : (uint32_t)t;
+    };
+    constexpr uint32_t C2 = (uint32_t)(2.0 * kQ8Slack * D) + 2;
+    // a lane whose query lies outside the grid checks every node exactly;
+    // inactive lanes take no node
+    const bool filter = inside;
+    const bool take_any = active;
+    uint32_t thr = inside ? thr_of(radius) : 0xffffffffu;
+    if (inside) {
+        // within C2 of the tree's smallest byte distance (pass 1)
+        const uint32_t g = gmin[qc];
+        const uint32_t t = g > 0xffffffffu - C2 ? 0xffffffffu : g + C2;
+        thr = t < thr ? t : thr;
+    }
+    TopK<1> top;
+    top.init();
+    constexpr int kCand = 4;
+    int32_t cand[kCand];
+    int ncand = 0;
+    auto flush = [&]() {
+#pragma unroll
+        for (int c = 0; c < kCand; ++c)
+            if (c < ncand) {
+                const double dq = exact_dist<D, MPTG_METRIC_L1>(q + qc * D, x64 + (int64_t)cand[c] * D);
+                if (dq <= radius) top.insert(dq, cand[c]);
+            }
+        ncand = 0;
+    };
+
+    for (int64_t t0 = p0; t0 < p1; t0 += kQ8Tile) {  // block-uniform bounds
+        const int cnt = (int)(p1 - t0 < kQ8Tile ? p1 - t0 : kQ8Tile);
+        __syncthreads();  // every wave is done with the previous tile
+        q8_stage<D>(x32, t0, cnt, Bf, inv_sf, t8);
+        __syncthreads();
+        constexpr int U = 8;
+        int j = 0;
+        for (; j + U <= cnt; j += U) {
+            uint32_t sad[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) sad[u] = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint4 lo = *reinterpret_cast<const uint4*>(t8 + w * kQ8Tile + j);
+                const uint4 hi = *reinterpret_cast<const uint4*>(t8 + w * kQ8Tile + j + 4);
+                sad[0] = __builtin_amdgcn_sad_u8(qw[w], lo.x, sad[0]);
+                sad[1] = __builtin_amdgcn_sad_u8(qw[w], lo.y, sad[1]);
+                sad[2] = __builtin_amdgcn_sad_u8(qw[w], lo.z, sad[2]);
+                sad[3] = __builtin_amdgcn_sad_u8(qw[w], lo.w, sad[3]);
+                sad[4] = __builtin_amdgcn_sad_u8(qw[w], hi.x, sad[4]);
+                sad[5] = __builtin_amdgcn_sad_u8(qw[w], hi.y, sad[5]);
+                sad[6] = __builtin_amdgcn_sad_u8(qw[w], hi.z, sad[6]);
+                sad[7] = __builtin_amdgcn_sad_u8(qw[w], hi.w, sad[7]);
+            }
+            uint32_t pass = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) pass |= (take_any && sad[u] <= thr ? 1u : 0u) << u;
+            while (pass) {
+                const int u = __builtin_ctz(pass);
+                pass &= pass - 1;
+                if (ncand == kCand) flush();
+                cand[ncand++] = (int32_t)(t0 + j + u);
+                if (filter) {
+                    const uint32_t t = sad[u] + C2;
+                    thr = t < thr ? t : thr;
+                }
+            }
+        }
+        for (; j < cnt; ++j) {
+            uint32_t sad = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) sad = __builtin_amdgcn_sad_u8(qw[w], t8[w * kQ8Tile + j], sad);
+            if (take_any && sad <= thr) {
+                if (ncand == kCand) flush();
+                cand[ncand++] = (int32_t)(t0 + j);
+                if (filter) {
+                    const uint32_t t = sad + C2;
+                    thr = t < thr ? t : thr;
+                }
+            }
+        }
+    }
+    flush();
+    if (!active) return;
+    const int64_t base = (int64_t)blockIdx.y * nq + qi;
+    out_d[base] = top.d[0];
+    out_i[base] = top.i[0];
 }
 
 // Direct FP64 scan for metrics without a prefilter (SE(3)) and any dimension.
@@ -676,11 +942,25 @@ struct ScanExtent {
     const int32_t* skip;
     double* seed;            // k = 1: per-query starting bound (nullptr: none)
     const uint32_t* nq_dev;  // live queries on the device (nullptr: all nq)
+    bool q8;                 // k = 1 under L1: the byte prefilter, its minima in `seed`
 };
 
 template <int D, int METRIC, int K>
 void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q, int64_t nq, double radius,
                  double* pd, int32_t* pi) {
+    if constexpr (K == 1 && METRIC == MPTG_METRIC_L1 && D >= 8) {
+        if (x.q8) {
+            // the seed buffer (nq doubles) holds the byte minima (filled with
+            // 0xffffffff by the caller) instead
+            uint32_t* gmin = reinterpret_cast<uint32_t*>(x.seed);
+            hipLaunchKernelGGL((knn_q8_min<D>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, x.n, x.chunk, q, nq,
+                               ctx->coord_bound, gmin, x.n_dev, x.skip, x.nq_dev);
+            hipLaunchKernelGGL((knn_scan_q8<D>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n,
+                               x.chunk, q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip,
+                               (const uint32_t*)gmin, x.nq_dev);
+            return;
+        }
+    }
     if (K == 1 && x.seed)
         hipLaunchKernelGGL((knn_seed<D, METRIC>), dim3((unsigned)((nq * kSeedLanes + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, x.seed, x.n_dev, x.skip, x.nq_dev);
@@ -778,7 +1058,11 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
         if ((rc = ensure(ctx, ctx->scratch[3], sizeof(double) * nq))) return rc;
         seed = (double*)ctx->scratch[3].p;
     }
-    const ScanExtent ext{n, chunk, n_dev, skip, seed, nq_dev};
+    // k = 1 under L1 in 8 or more dimensions: the byte prefilter
+    const bool q8 = seed && ctx->metric == MPTG_METRIC_L1 && ctx->dim >= 8 && ctx->tune.knn_q8 &&
+                    ctx->coord_bound > 0 && ctx->coord_bound < 1e300;
+    if (q8 && (rc = launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(seed), nq, 0xffffffffu))) return rc;
+    const ScanExtent ext{n, chunk, n_dev, skip, seed, nq_dev, q8};
     double* pd = (double*)ctx->scratch[4].p;
     int32_t* pi = (int32_t*)ctx->scratch[5].p;
     dim3 grid((unsigned)((nq + kBlock - 1) / kBlock), (unsigned)splits);

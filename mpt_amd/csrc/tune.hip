@@ -99,6 +99,7 @@ int mptg_tune_set(mptg_ctx* ctx, const mptg_tune_params* p) {
     if (p->knn_grid != 0 && p->knn_grid != 1) return fail(ctx, MPTG_E_RANGE, "knn_grid is 0 or 1");
     if (p->dmv_edge_lanes < 1 || p->dmv_edge_lanes > 64 || (p->dmv_edge_lanes & (p->dmv_edge_lanes - 1)))
         return fail(ctx, MPTG_E_RANGE, "lanes per DMV edge must be a power of two in 1..64");
+    if (p->knn_q8 != 0 && p->knn_q8 != 1) return fail(ctx, MPTG_E_RANGE, "knn_q8 is 0 or 1");
     ctx->tune = *p;
     return MPTG_OK;
 }

@@ -363,3 +363,40 @@ def test_tune_set_rejects_out_of_range(ctx_factory):
         ctx.tune_set(dmv_edge_lanes=3)
     with pytest.raises(Exception):
         ctx.tune_set(dmv_edge_lanes=128)
+
+
+@pytest.mark.parametrize("dim", [8, 10, 16, 20])
+def test_knn_byte_prefilter_equals_fp32_scan_and_oracle(ctx_factory, dim):
+    """k = 1 under L1 in 8 or more dimensions: the byte (sum of absolute
+    differences) prefilter gives the FP32 scan's and the oracle's ids and
+    distances, with duplicates, 1-ulp near ties, nodes on the coordinate
+    bound, queries beyond it, a finite radius and a device query count."""
+    _say("knn byte prefilter", dim)
+    ctx = ctx_factory()
+    rng = np.random.default_rng(300 + dim)
+    n = 20000
+    pts = rng.uniform(-math.pi, math.pi, (n, dim))
+    pts[7] = pts[3]                                   # exact duplicate: the smaller id wins
+    pts[11] = pts[5]
+    pts[11, 0] = np.nextafter(pts[5, 0], np.inf)     # 1-ulp near tie
+    pts[13, :] = math.pi                              # on the bound in every coordinate
+    pts[17, :] = -math.pi
+    q = rng.uniform(-math.pi, math.pi, (3000, dim))
+    q[0] = pts[3]
+    q[1] = pts[5]
+    q[2, :] = math.pi
+    q[3, :] = 5.0 * math.pi                           # beyond the bound: every node exact
+    q[4] = (pts[100] + pts[101]) / 2
+    ctx.space(scenes.METRIC_L1, dim)
+    ctx.tree_insert(pts)
+    o = oracle.Oracle(_plain(scenes.METRIC_L1, dim))
+    oi, od = o.knn(pts, q, 1, threads=16)
+    typical = float(np.median(od))
+    ri, rd = o.knn(pts, q, 1, radius=typical, threads=16)
+    for q8 in (1, 0):
+        ctx.tune_set(knn_q8=q8)
+        gi, gd = ctx.knn(q, 1)
+        assert np.array_equal(gi, oi) and np.array_equal(gd, od), q8
+        gi, gd = ctx.knn(q, 1, radius=typical)
+        assert np.array_equal(gi, ri) and np.array_equal(gd, rd), q8
+    ctx.tune_set(knn_q8=1)
