@@ -248,6 +248,21 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
    when the host has nothing to do between iterations. */
 int mptg_prrt_graph(mptg_ctx* ctx, int iterations_per_launch);
 int mptg_prrt_stats_get(mptg_ctx* ctx, mptg_prrt_stats* out);
+/* PRRT*: the path cost and parent each goal node had when it was found, in discovery order
+   (foundGoal keeps that edge as the solution, src/mpt/impl/prrt_star/prrt_star.hpp:203-216;
+   solution() starts from it, :324-336).  Returns the count via *n (at most 4096 goals kept). */
+int mptg_prrt_goal_edges(mptg_ctx* ctx, double* h_cost, int32_t* h_parent, int64_t cap, int64_t* n);
+/* A Scenario's own sampler (src/mpt/impl/scenario_sampler.hpp:145-187: scenario.sample(rng) or
+   scenario.sampler()) in place of the uniform box sampler, for PRRT and PRRT*.  Every iteration
+   calls fn once, on the thread that called mptg_prrt_run, for the iteration's n = batch sample
+   indices first .. first + n - 1 (first = iteration * batch): it writes n states of `dim` doubles
+   to h_q and ok[i] = 0 for a slot that has no sample (the reference's std::optional<State>
+   without a value: that slot is skipped, as addSample skips it, prrt.hpp:400-403) and returns 0,
+   or nonzero to fail mptg_prrt_run with MPTG_E_ARG.  Goal biasing stays with the planner: when
+   slot 0 takes the bias coin it samples the goal (prrt.hpp:377-384).  fn = NULL restores the box
+   sampler.  Iterations with a custom sampler run step by step (no graph capture). */
+typedef int (*mptg_sample_fn)(void* user, uint64_t seed, int64_t first, int64_t n, double* h_q, uint8_t* h_ok);
+int mptg_prrt_sampler(mptg_ctx* ctx, mptg_sample_fn fn, void* user);
 /* ids of goal nodes in discovery order (foundGoal, prrt.hpp:139-146); returns count via *n */
 int mptg_prrt_goals(mptg_ctx* ctx, int32_t* h_ids, int64_t cap, int64_t* n);
 

@@ -28,6 +28,9 @@ from .scenes import (LINK_BISECT, LINK_BISECT_L2, LINK_DMV, METRIC_L1, METRIC_L2
 _VALID_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_uint8))
 _LINK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
                        C.POINTER(C.c_uint8))
+# custom sampler callback (mptg_sample_fn)
+_SAMPLE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_double),
+                         C.POINTER(C.c_uint8))
 
 __all__ = ["Context", "DeviceArray", "PRRT", "MptgError", "Scene", "METRIC_L1", "METRIC_L2", "METRIC_SE3", "LINK_DMV",
            "LINK_BISECT", "ORDER_REFERENCE", "ORDER_SURROGATE"]
@@ -479,6 +482,40 @@ class Context:
         self._chk(_lib.mptg_prrt_goals(self._h, _p(buf, C.c_int32), buf.shape[0], C.byref(n)))
         return buf[: min(n.value, buf.shape[0])].copy()
 
+    def prrt_sampler(self, fn: Optional[Callable] = None):
+        """A Scenario's own sampler for PRRT / PRRT*: fn(seed, first, n)
+        returns the states of sample indices first .. first + n - 1 as an
+        (n, dim) array, or (states, ok) with ok[i] False for a slot that has
+        no sample.  None restores the box sampler."""
+        if fn is None:
+            self._sampler_cb = None
+            self._chk(_lib.mptg_prrt_sampler(self._h, None, None))
+            return
+        dim = self.dim
+
+        def _sample(_user, seed, first, n, q, ok):
+            try:
+                r = fn(int(seed), int(first), int(n))
+                states, flags = r if isinstance(r, tuple) else (r, None)
+                np.ctypeslib.as_array(q, shape=(n, dim))[:] = np.asarray(states, dtype=np.float64).reshape(n, dim)
+                np.ctypeslib.as_array(ok, shape=(n,))[:] = 1 if flags is None else np.asarray(flags, dtype=bool)
+                return 0
+            except Exception as e:  # reported by the failing entry point
+                self._host_exc = e
+                return 1
+
+        self._sampler_cb = _SAMPLE_FN(_sample)
+        self._chk(_lib.mptg_prrt_sampler(self._h, C.cast(self._sampler_cb, C.c_void_p), None))
+
+    def prrt_goal_edges(self):
+        """PRRT*: (path cost, parent) of every goal node when it was found."""
+        n = C.c_int64()
+        cost = np.zeros(4096)
+        par = np.zeros(4096, dtype=np.int32)
+        self._chk(_lib.mptg_prrt_goal_edges(self._h, _p(cost, C.c_double), _p(par, C.c_int32), 4096, C.byref(n)))
+        k = min(n.value, 4096)
+        return cost[:k].copy(), par[:k].copy()
+
 
 class PRRT:
     """Host mirror of ``mpt::Planner<Scenario, PRRT<...>>`` on the batched GPU
@@ -556,18 +593,22 @@ class PRRT:
         return path[::-1]
 
     def solution(self):
-        """Best (lowest-cost) goal path as a list of states (prrt.hpp:221-274)."""
+        """Best goal path as a list of states, start first (prrt.hpp:208-234,
+        261-273): the goal of least path cost, summed from the goal up, the
+        first found on ties."""
         goals = self.ctx.prrt_goals()
         if goals.size == 0:
             return []
         states, parents = self.ctx.tree_read()
         best, best_cost = None, math.inf
         for g in goals:
-            path = self._path_to(int(g), states, parents)
-            cost = sum(self._dist(path[i], path[i + 1]) for i in range(len(path) - 1))
+            cost, n = 0.0, int(g)
+            while parents[n] >= 0:
+                cost += self._dist(states[n], states[parents[n]])
+                n = int(parents[n])
             if best is None or cost < best_cost:
-                best, best_cost = path, cost
-        return best
+                best, best_cost = int(g), cost
+        return self._path_to(best, states, parents)
 
     def _dist(self, a, b) -> float:
         if self.scenario.metric == METRIC_L1:
@@ -625,20 +666,24 @@ class PRRTStar(PRRT):
         return self.ctx.prrt_costs()
 
     def _best_goal(self):
+        """The goal edge found with the least path cost at the time it was
+        found, the first on ties (foundGoal, prrt_star.hpp:203-216): (goal
+        node, its parent then, that cost)."""
         goals = self.ctx.prrt_goals()
         if goals.size == 0:
-            return None, math.nan
-        c = self.costs()[goals]
-        i = int(np.argmin(c))
-        return int(goals[i]), float(c[i])
+            return None, -1, math.nan
+        cost0, parent0 = self.ctx.prrt_goal_edges()
+        i = int(np.argmin(cost0))  # the first minimum
+        return int(goals[i]), int(parent0[i]), float(cost0[i])
 
     def solutionCost(self) -> float:
-        return self._best_goal()[1]
+        return self._best_goal()[2]
 
     def solution(self):
-        """Path to the lowest-cost goal node (prrt_star.hpp:366-380)."""
-        g, _ = self._best_goal()
+        """The solution edge's path (prrt_star.hpp:324-336): the goal, the
+        parent it was found with, then the current tree up to the start."""
+        g, p0, _ = self._best_goal()
         if g is None:
             return []
         states, parents = self.ctx.tree_read()
-        return self._path_to(g, states, parents)
+        return self._path_to(p0, states, parents) + [states[g]]

@@ -107,6 +107,8 @@ _SIGNATURES = [
     ("mptg_autotune_link", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(TuneReport)]),
     ("mptg_autotune_prrt", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TuneReport)]),
     ("mptg_prrt_goals", C.c_int, [C.c_void_p, c_int32_p, C.c_int64, c_int64_p]),
+    ("mptg_prrt_goal_edges", C.c_int, [C.c_void_p, c_double_p, c_int32_p, C.c_int64, c_int64_p]),
+    ("mptg_prrt_sampler", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mptg_prrt_star_init", C.c_int, [C.c_void_p, C.POINTER(PrrtParams), C.POINTER(PrrtStarParams), c_double_p,
                                       c_double_p, c_double_p]),
     ("mptg_prrt_costs", C.c_int, [C.c_void_p, C.c_int64, C.c_int64, c_double_p]),

@@ -331,6 +331,8 @@ int mptg_tree_clear(mptg_ctx* ctx) {
     ctx->prrt.iteration = 0;
     ctx->prrt.goals = 0;
     ctx->prrt.goal_ids.clear();
+    ctx->prrt.goal_cost0.clear();
+    ctx->prrt.goal_parent0.clear();
     ctx->prrt.stats = mptg_prrt_stats{};
     ctx->prrt.counters_valid = false;
     return MPTG_OK;

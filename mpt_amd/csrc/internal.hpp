@@ -61,6 +61,13 @@ struct PrrtState {
     // valid() before nearest (no range limit): listed valid samples, their ids,
     // the list count, and their nearest node and distance
     DevBuf qv, vidx, vcnt, kn_i, kn_d, vcheck;
+    // a Scenario's own sampler (host callback): staged samples, their flags,
+    // the slots without a sample
+    mptg_sample_fn sampler = nullptr;
+    void* sampler_user = nullptr;
+    DevBuf hs_q, hs_ok, hs_skip;
+    std::vector<double> hs_host_q;
+    std::vector<uint8_t> hs_host_ok;
     mptg_prrt_params p{};
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
     int64_t iteration = 0;       // next iteration index
@@ -76,6 +83,9 @@ struct PrrtState {
     int64_t g_cap = 0, g_plan = 0;
     mptg_prrt_stats stats{};
     std::vector<int32_t> goal_ids;
+    // PRRT*: each goal node's path cost and parent when it was found
+    std::vector<double> goal_cost0;
+    std::vector<int32_t> goal_parent0;
     // PRRT*
     bool star = false;
     int32_t star_k = 20;          // 0: the reference's k(n) formula

@@ -222,7 +222,7 @@ static bool prevalid_eligible(const mptg_ctx* ctx) {
     const bool scan_dim = (D >= 1 && D <= 8) || D == 10 || D == 12 || D == 16 || D == 20;
     return s.init && !s.star && std::isinf(s.p.range) && s.p.range > 0 && ctx->metric != MPTG_METRIC_SE3 &&
            ctx->scene.kind != MPTG_SCENE_NONE && ctx->scene.kind != MPTG_SCENE_MESH &&
-           ctx->scene.kind != MPTG_SCENE_HOST && scan_dim &&
+           ctx->scene.kind != MPTG_SCENE_HOST && !s.sampler && scan_dim &&
            !(ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks > D) &&
            !(ctx->scene.kind != MPTG_SCENE_ARM && D < 2);
 }
@@ -262,7 +262,8 @@ void prrt_free(mptg_ctx* ctx) {
                      &pb.nbh_idx, &pb.nbh_d,   &pb.marked, &pb.pc,      &pb.cpar,     &pb.cp_from, &pb.cp_to,
                      &pb.cp_ver,  &pb.cp_chk,  &pb.pair_cp, &pb.rw_from, &pb.rw_to,    &pb.rw_ver,  &pb.rw_chk,
                      &pb.pair_rw, &pb.front,   &pb.pend,   &pb.star_cnt, &pb.blk,     &pb.qv,      &pb.vidx,
-                     &pb.vcnt,    &pb.kn_i,    &pb.kn_d,   &pb.vcheck, &pb.nbh_off, &pb.nbh_own};
+                     &pb.vcnt,    &pb.kn_i,    &pb.kn_d,   &pb.vcheck, &pb.nbh_off, &pb.nbh_own, &pb.hs_q,
+                     &pb.hs_ok,   &pb.hs_skip};
     for (DevBuf* b : all) {
         (void)hipFree(b->p);
         b->p = nullptr;
@@ -324,6 +325,8 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
     hipLaunchKernelGGL(prrt_sample, dim3(sample_blocks(ctx->metric, D, B)), dim3(kBlock), 0, ctx->stream, pd,
                        ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)pb.q.p);
     MPTG_HIP(ctx, hipGetLastError());
+    uint8_t* skip = nullptr;  // custom sampler: slots without a sample (eager iterations only)
+    if (s.sampler && (rc = take_host_samples(ctx, it, (double*)pb.q.p, dcnt, &skip))) return rc;
     if (pre && !dev && knn_grid_applies(ctx, ctx->size)) pre = false;
     if (pre) {
         uint32_t* vcnt = (uint32_t*)pb.vcnt.p;
@@ -369,6 +372,7 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
         });
     }
     MPTG_HIP(ctx, hipGetLastError());
+    if (skip && (rc = apply_skip(ctx, skip, (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p, dcnt))) return rc;
     if (ctx->scene.kind == MPTG_SCENE_MESH &&
         (rc = launch_mesh_filter(ctx, (const double*)pb.to.p, B, (uint8_t*)pb.mask.p, (uint8_t*)pb.status.p,
                                  ST_INVALID)))
@@ -465,6 +469,8 @@ int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo,
     s.iteration = 0;
     s.goals = 0;
     s.goal_ids.clear();
+    s.goal_cost0.clear();
+    s.goal_parent0.clear();
     s.stats = mptg_prrt_stats{};
     s.counters_valid = false;
     return MPTG_OK;
@@ -545,8 +551,8 @@ int mptg_prrt_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters, mptg_pr
     // the context epoch it was captured at holds (no entry point, no
     // allocation since), for the same tree capacity and iteration count.
     // Before a capture one step-by-step iteration sizes every buffer.
-    // host scenes call back into the host within every iteration
-    const int G = ctx->scene.kind == MPTG_SCENE_HOST ? 0 : s.graph_iters;
+    // host scenes and custom samplers call back into the host within every iteration
+    const int G = (ctx->scene.kind == MPTG_SCENE_HOST || s.sampler) ? 0 : s.graph_iters;
     int64_t k = 0;
     bool warm = false;
     while (k < max_iters && ctx->size < target_size) {
@@ -641,6 +647,26 @@ int mptg_prrt_goals(mptg_ctx* ctx, int32_t* h_ids, int64_t cap, int64_t* n) {
     const auto& g = ctx->prrt.goal_ids;
     *n = ctx->prrt.goals;
     for (int64_t i = 0; i < cap && i < (int64_t)g.size(); ++i) h_ids[i] = g[i];
+    return MPTG_OK;
+}
+
+int mptg_prrt_sampler(mptg_ctx* ctx, mptg_sample_fn fn, void* user) {
+    int rc = check_device_ready(ctx);
+    if (rc) return rc;
+    ctx->prrt.sampler = fn;
+    ctx->prrt.sampler_user = fn ? user : nullptr;
+    return MPTG_OK;
+}
+
+int mptg_prrt_goal_edges(mptg_ctx* ctx, double* h_cost, int32_t* h_parent, int64_t cap, int64_t* n) {
+    if (!ctx || !n) return fail(ctx, MPTG_E_ARG, "null argument");
+    const PrrtState& s = ctx->prrt;
+    if (!s.star) return fail(ctx, MPTG_E_STATE, "goal edges are kept for PRRT* trees");
+    *n = (int64_t)s.goal_cost0.size();
+    for (int64_t i = 0; i < cap && i < *n; ++i) {
+        if (h_cost) h_cost[i] = s.goal_cost0[(size_t)i];
+        if (h_parent) h_parent[i] = s.goal_parent0[(size_t)i];
+    }
     return MPTG_OK;
 }
 

@@ -10,7 +10,8 @@ namespace {
 
 constexpr int kBlock = 256;
 
-enum : uint8_t { ST_ZERO = 0, ST_INVALID = 1, ST_LINK_FAIL = 2, ST_OK = 3 };
+// ST_SKIP: a custom sampler gave no sample for the slot (counted nowhere)
+enum : uint8_t { ST_ZERO = 0, ST_INVALID = 1, ST_LINK_FAIL = 2, ST_OK = 3, ST_SKIP = 4 };
 
 // Planner counters, resident on the device.  `iteration` is the index of the
 // next iteration; `done` turns every planner kernel of an iteration into a
@@ -69,6 +70,42 @@ __global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int metric, in
         sample_state(seed, sidx, metric, 7, pd.lo, pd.hi, v);
     }
     for (int i = 0; i < 7; ++i) q[b * 7 + i] = v[i];
+}
+
+// A Scenario's own sampler (scenario_sampler.hpp:145-187): its staged
+// samples replace the batch's box samples, except slot 0 when it took the
+// goal-bias coin (the same draw as prrt_sample); a slot with no sample keeps
+// its box sample as a placeholder and is marked in `skip`.
+__global__ __launch_bounds__(kBlock) void prrt_take_samples(int D, uint64_t seed, double goal_bias, int64_t it, int B,
+                                                            const Counters* __restrict__ cnt,
+                                                            const double* __restrict__ staged,
+                                                            const uint8_t* __restrict__ ok, double* __restrict__ q,
+                                                            uint8_t* __restrict__ skip) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B || cnt->done) return;
+    const uint64_t sidx = (uint64_t)it * (uint64_t)B + (uint64_t)b;
+    const bool bias = b == 0 && cnt->goals == 0 && goal_bias > 0;
+    if (bias && uniform_at(seed, sidx, RNG_PURPOSE_BIAS, 0) < goal_bias) {
+        skip[b] = 0;
+        return;
+    }
+    if (!ok[b]) {
+        skip[b] = 1;
+        return;
+    }
+    skip[b] = 0;
+    for (int i = 0; i < D; ++i) q[(int64_t)b * D + i] = staged[(int64_t)b * D + i];
+}
+
+// After the steer step: slots without a sample take no part (addSample on
+// an empty std::optional returns at once, prrt.hpp:400-403).
+__global__ __launch_bounds__(kBlock) void prrt_apply_skip(int B, const uint8_t* __restrict__ skip,
+                                                          uint8_t* __restrict__ status, uint8_t* __restrict__ linkmask,
+                                                          const Counters* __restrict__ cnt) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B || cnt->done || !skip[b]) return;
+    status[b] = ST_SKIP;
+    linkmask[b] = 0;
 }
 
 template <int D>
@@ -275,6 +312,39 @@ __global__ __launch_bounds__(kBlock) void prrt_steer_listed(int dim, int B, cons
     }
     status[b] = ST_LINK_FAIL;  // until link() says otherwise
     linkmask[b] = 1;
+}
+
+// A Scenario's own sampler: the host callback fills the iteration's batch,
+// which replaces the box samples in d_q (prrt_take_samples); *skip is the
+// device array of slots without a sample.  Synchronous.
+inline int take_host_samples(mptg_ctx* ctx, int64_t it, double* d_q, const Counters* dcnt, uint8_t** skip) {
+    PrrtState& s = ctx->prrt;
+    const int B = s.p.batch, D = ctx->dim;
+    int rc;
+    if ((rc = ensure(ctx, s.hs_q, sizeof(double) * (size_t)B * D))) return rc;
+    if ((rc = ensure(ctx, s.hs_ok, (size_t)B))) return rc;
+    if ((rc = ensure(ctx, s.hs_skip, (size_t)B))) return rc;
+    s.hs_host_q.assign((size_t)B * D, 0.0);
+    s.hs_host_ok.assign((size_t)B, 1);
+    const int cb = s.sampler(s.sampler_user, s.p.seed, it * (int64_t)B, B, s.hs_host_q.data(), s.hs_host_ok.data());
+    if (cb) return fail(ctx, MPTG_E_ARG, "sampler callback returned " + std::to_string(cb));
+    MPTG_HIP(ctx, hipMemcpy(s.hs_q.p, s.hs_host_q.data(), sizeof(double) * s.hs_host_q.size(), hipMemcpyHostToDevice));
+    MPTG_HIP(ctx, hipMemcpy(s.hs_ok.p, s.hs_host_ok.data(), s.hs_host_ok.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(prrt_take_samples, dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, D,
+                       s.p.seed, s.p.goal_bias, it, B, dcnt, (const double*)s.hs_q.p, (const uint8_t*)s.hs_ok.p, d_q,
+                       (uint8_t*)s.hs_skip.p);
+    MPTG_HIP(ctx, hipGetLastError());
+    *skip = (uint8_t*)s.hs_skip.p;
+    return MPTG_OK;
+}
+
+// Slots without a sample leave the batch after the steer step.
+inline int apply_skip(mptg_ctx* ctx, const uint8_t* skip, uint8_t* status, uint8_t* mask, const Counters* dcnt) {
+    const int B = ctx->prrt.p.batch;
+    hipLaunchKernelGGL(prrt_apply_skip, dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, B,
+                       skip, status, mask, dcnt);
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
 }
 
 // Launches the steer kernel for a batch of B samples (arm scenes with

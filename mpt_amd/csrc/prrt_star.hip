@@ -690,6 +690,8 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         hipLaunchKernelGGL(prrt_sample, dim3(sample_blocks(ctx->metric, D, B)), dim3(kBlock), 0, ctx->stream, pd,
                            ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)s.q.p);
         MPTG_HIP(ctx, hipGetLastError());
+        uint8_t* skip = nullptr;  // custom sampler: slots without a sample
+        if (s.sampler && (rc = take_host_samples(ctx, it, (double*)s.q.p, dcnt, &skip))) return rc;
         if ((rc = launch_knn(ctx, (const double*)s.q.p, B, 1, __builtin_inf(), (int32_t*)s.near.p,
                              (double*)s.ndist.p, n0)))
             return rc;
@@ -701,6 +703,7 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
                              (uint8_t*)s.mask.p, (double*)s.dnear.p, (const Counters*)dcnt);
         });
         MPTG_HIP(ctx, hipGetLastError());
+        if (skip && (rc = apply_skip(ctx, skip, (uint8_t*)s.status.p, (uint8_t*)s.mask.p, dcnt))) return rc;
         if (ctx->scene.kind == MPTG_SCENE_MESH &&
             (rc = launch_mesh_filter(ctx, (const double*)s.to.p, B, (uint8_t*)s.mask.p, (uint8_t*)s.status.p,
                                      ST_INVALID)))
@@ -768,6 +771,21 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
                                ctx->next, ctx->prev, dcnt, (int32_t*)s.goal_buf.p, (int64_t)4096);
             MPTG_HIP(ctx, hipGetLastError());
             if ((rc = read_counters())) return rc;
+            // goals found in this iteration: the path cost and parent they
+            // have now, before the rewire pass (foundGoal keeps the goal edge as
+            // found, prrt_star.hpp:203-216; solution() starts from it, :324-336)
+            const int64_t g0 = (int64_t)s.goal_cost0.size(), g1 = std::min<int64_t>(hc->goals, 4096);
+            for (int64_t g = g0; g < g1; ++g) {
+                int32_t id = -1, par = -1;
+                double c = 0.0;
+                MPTG_HIP(ctx, hipMemcpy(&id, (int32_t*)s.goal_buf.p + g, sizeof(id), hipMemcpyDeviceToHost));
+                const int64_t j = (int64_t)id - n0;
+                if (j < 0 || j >= m) return fail(ctx, MPTG_E_STATE, "goal list out of step with the append");
+                MPTG_HIP(ctx, hipMemcpy(&c, (double*)s.pc.p + j, sizeof(c), hipMemcpyDeviceToHost));
+                MPTG_HIP(ctx, hipMemcpy(&par, (int32_t*)s.cpar.p + j, sizeof(par), hipMemcpyDeviceToHost));
+                s.goal_cost0.push_back(c);
+                s.goal_parent0.push_back(par);
+            }
             // ---- rewire
             if (hs->nrw > 0 &&
                 (rc = launch_link(ctx, (const double*)s.rw_from.p, (const double*)s.rw_to.p, hs->nrw, s.p.order,

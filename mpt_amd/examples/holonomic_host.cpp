@@ -7,16 +7,23 @@
 // sampling, nearest neighbours, steering and the tree on the GPU and calls
 // these functions for each iteration's batch on a host thread pool.
 //
-//   holonomic_host.bin MODE SEED TARGET RANGE GOAL_BIAS OUT
+//   holonomic_host.bin MODE SEED TARGET RANGE GOAL_BIAS OUT [GOAL_RADIUS]
 //     MODE: prrt1 (PRRT, batch 1), prrt256 (PRRT, batch 256),
-//           star64 (PRRT*, batch 64, k = 20)
-//   writes OUT: int64 n, n x 2 doubles (states), n int32 (parents), and for
-//   star64 n doubles (path costs); prints the planner's counters.
+//           star64 (PRRT*, batch 64, k = 20),
+//           sampled256 (PRRT, batch 256, with the Scenario's own sampler:
+//           the left half of the canvas, one draw in ten without a sample)
+//   writes OUT: int64 n, n x 2 doubles (states), n int32 (parents), for star64
+//   n doubles (path costs), then the solution: int64 m, m x 2 doubles
+//   (waypoints, start first) and int64 t (edges the trajectory callback
+//   saw); prints the planner's counters.
 #include <array>
 #include <cmath>
+#include <optional>
+#include <random>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <variant>
 #include <vector>
 
 #include "mpt_amd/planner.hpp"
@@ -79,8 +86,10 @@ class Holonomic2DPoint {
     static constexpr int kMetric = MPTG_METRIC_L2;
     using State = std::array<double, 2>;
 
-    Holonomic2DPoint(int width, int height, std::vector<Circle> circles, std::vector<Rect> rects, const State& goal)
-        : w_(width), h_(height), circles_(std::move(circles)), rects_(std::move(rects)), goal_(goal) {}
+    Holonomic2DPoint(int width, int height, std::vector<Circle> circles, std::vector<Rect> rects, const State& goal,
+                     double goalRadius = 1e-6)
+        : w_(width), h_(height), circles_(std::move(circles)), rects_(std::move(rects)), goal_(goal),
+          goalRadius_(goalRadius) {}
 
     bool valid(const State& q) const {
         for (const Circle& c : circles_)
@@ -104,7 +113,7 @@ class Holonomic2DPoint {
         hi[1] = h_;
     }
     const State& goal_state() const { return goal_; }
-    double goal_radius() const { return 1e-6; }
+    double goal_radius() const { return goalRadius_; }
     double distance(const double* a, const double* b) const {
         const double dx = a[0] - b[0], dy = a[1] - b[1];
         return std::sqrt(dx * dx + dy * dy);
@@ -115,22 +124,38 @@ class Holonomic2DPoint {
     std::vector<Circle> circles_;
     std::vector<Rect> rects_;
     State goal_;
+    double goalRadius_;
 };
 
-template <typename Algorithm>
-int run(const Holonomic2DPoint& scenario, uint64_t seed, size_t target, double range, double bias, const char* out,
+// The same Scenario with a sampler of its own (scenario_sampler.hpp:145-167):
+// uniform over the left half of the canvas, and no sample (std::nullopt) for
+// one draw in ten.
+class Holonomic2DPointSampled : public Holonomic2DPoint {
+   public:
+    using Holonomic2DPoint::Holonomic2DPoint;
+    template <typename RNG>
+    std::optional<State> sample(RNG& rng) const {
+        std::uniform_real_distribution<double> u01(0.0, 1.0);
+        if (u01(rng) < 0.1) return std::nullopt;
+        return State{u01(rng) * 512.0, u01(rng) * 512.0};
+    }
+};
+
+template <typename Algorithm, typename Scenario = Holonomic2DPoint>
+int run(const Scenario& scenario, uint64_t seed, size_t target, double range, double bias, const char* out,
         bool costs) {
-    mpt_amd::Planner<Holonomic2DPoint, Algorithm> planner(scenario, seed);
+    mpt_amd::Planner<Scenario, Algorithm> planner(scenario, seed);
     static_assert(decltype(planner)::kHostScene, "the Scenario has no gpu_scene()");
     planner.setRange(range);
     planner.setGoalBias(bias);
     planner.addStart({30.0, 30.0});
     planner.solveUntilSize(target);
     const mptg_prrt_stats st = planner.stats();
-    std::printf("size=%zu iterations=%lld link_calls=%lld link_ok=%lld invalid=%lld goals=%lld rewires=%lld "
-                "solved=%d\n",
-                planner.size(), (long long)st.iterations, (long long)st.link_calls, (long long)st.link_ok,
-                (long long)st.invalid_states, (long long)st.goals, (long long)st.rewires, planner.solved() ? 1 : 0);
+    std::printf("size=%zu iterations=%lld samples=%lld zero=%lld link_calls=%lld link_ok=%lld invalid=%lld goals=%lld "
+                "rewires=%lld solved=%d\n",
+                planner.size(), (long long)st.iterations, (long long)st.samples, (long long)st.zero_dist,
+                (long long)st.link_calls, (long long)st.link_ok, (long long)st.invalid_states, (long long)st.goals,
+                (long long)st.rewires, planner.solved() ? 1 : 0);
     const int64_t n = mptg_tree_size(planner.context());
     std::vector<double> states((size_t)n * 2), cost((size_t)n);
     std::vector<int32_t> parents((size_t)n);
@@ -142,15 +167,32 @@ int run(const Holonomic2DPoint& scenario, uint64_t seed, size_t target, double r
     std::fwrite(states.data(), sizeof(double), states.size(), f);
     std::fwrite(parents.data(), sizeof(int32_t), parents.size(), f);
     if (costs) std::fwrite(cost.data(), sizeof(double), cost.size(), f);
+    // the solution, as waypoints and through both callback shapes
+    const std::vector<Holonomic2DPoint::State> path = planner.solution();
+    int64_t edges = 0, waypoints = 0;
+    bool same = true;
+    planner.solution([&](const Holonomic2DPoint::State& a, const std::monostate&, const Holonomic2DPoint::State& b,
+                         bool) {
+        same = same && a == path[(size_t)edges] && b == path[(size_t)edges + 1];
+        ++edges;
+    });
+    planner.solution([&](const Holonomic2DPoint::State& q) { same = same && q == path[(size_t)waypoints++]; });
+    const int64_t np = (int64_t)path.size();
+    std::fwrite(&np, sizeof(np), 1, f);
+    for (const auto& q : path) std::fwrite(q.data(), sizeof(double), 2, f);
+    std::fwrite(&edges, sizeof(edges), 1, f);
     std::fclose(f);
+    std::printf("solution waypoints=%lld callback_edges=%lld callbacks_agree=%d\n", (long long)np, (long long)edges,
+                same && waypoints == np && edges == (np > 0 ? np - 1 : 0) ? 1 : 0);
     return 0;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 7) {
-        std::fprintf(stderr, "usage: %s prrt1|prrt256|star64 SEED TARGET RANGE GOAL_BIAS OUT\n", argv[0]);
+    if (argc != 7 && argc != 8) {
+        std::fprintf(stderr, "usage: %s prrt1|prrt256|star64 SEED TARGET RANGE GOAL_BIAS OUT [GOAL_RADIUS]\n",
+                     argv[0]);
         return 1;
     }
     const std::string mode = argv[1];
@@ -159,7 +201,9 @@ int main(int argc, char** argv) {
     const double range = std::strtod(argv[4], nullptr), bias = std::strtod(argv[5], nullptr);
     std::vector<Circle> circles = {{170.0, 140.0, 80.0}, {800.0, 70.0, 50.0}, {900.0, 380.0, 70.0}};
     std::vector<Rect> rects = {{375, 140, 520, 220}, {200, 320, 390, 390}, {600, 200, 680, 450}};
-    const Holonomic2DPoint scenario(1024, 512, circles, rects, {1024 - 30, 512 - 30});
+    // the demo's goal radius is 1e-6 (GoalState(1e-6, goal)); a larger one finds many goals
+    const double goalRadius = argc == 8 ? std::strtod(argv[7], nullptr) : 1e-6;
+    const Holonomic2DPoint scenario(1024, 512, circles, rects, {1024 - 30, 512 - 30}, goalRadius);
     try {
         if (mode == "prrt1")
             return run<mpt_amd::PRRT<mpt_amd::gpu_batch<1>, mpt_amd::max_threads<4>>>(scenario, seed, target, range,
@@ -169,6 +213,10 @@ int main(int argc, char** argv) {
         if (mode == "star64")
             return run<mpt_amd::PRRTStar<mpt_amd::gpu_batch<64>, mpt_amd::rewire_k<20>>>(scenario, seed, target,
                                                                                          range, bias, argv[6], true);
+        if (mode == "sampled256") {
+            const Holonomic2DPointSampled sampled(1024, 512, circles, rects, {1024 - 30, 512 - 30}, goalRadius);
+            return run<mpt_amd::PRRT<mpt_amd::gpu_batch<256>>>(sampled, seed, target, range, bias, argv[6], false);
+        }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 2;

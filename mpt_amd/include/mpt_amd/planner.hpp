@@ -25,6 +25,9 @@
 //     thread count), as the reference's workers call them (prrt.hpp:439-457).
 // Both forms provide State, kDimensions, bounds_to(lo, hi), goal_state(),
 // goal_radius() and distance(a, b).  States are any type with operator[].
+// A Scenario with a sample(RNG&) method (returning a State or a
+// std::optional<State>; scenario_sampler.hpp:145-167) samples in place of the
+// uniform box sampler, in either form (mptg_prrt_sampler).
 // Errors from the C ABI are rethrown as std::runtime_error (as the reference
 // throws at prrt.hpp:198); an exception thrown by the Scenario's valid() or
 // link() is rethrown as itself.
@@ -42,11 +45,14 @@
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <optional>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <thread>
 #include <type_traits>
 #include <utility>
+#include <variant>
 #include <vector>
 
 #include "mptg.h"
@@ -191,6 +197,26 @@ struct has_gpu_scene : std::false_type {};
 template <typename S>
 struct has_gpu_scene<S, std::void_t<decltype(std::declval<const S&>().gpu_scene(static_cast<mptg_ctx*>(nullptr)))>>
     : std::true_type {};
+
+// Scenario::sample(RNG&) (scenario_sampler.hpp:145-167): returns a State or
+// a std::optional<State>
+template <typename S, typename = void>
+struct has_sample : std::false_type {};
+template <typename S>
+struct has_sample<S, std::void_t<decltype(std::declval<S&>().sample(std::declval<std::mt19937_64&>()))>>
+    : std::true_type {};
+template <typename T>
+struct is_optional : std::false_type {};
+template <typename T>
+struct is_optional<std::optional<T>> : std::true_type {};
+
+// splitmix64: one well-mixed 64-bit seed per (planner seed, sample index)
+inline uint64_t sample_seed(uint64_t seed, uint64_t index) {
+    uint64_t z = seed ^ (index * 0x9e3779b97f4a7c15ull + 0x632be59bd9b4e019ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
 
 template <typename S, typename = void>
 struct so3_weight {
@@ -371,6 +397,33 @@ class GpuPRRT {
         }
     }
 
+    // mptg_sample_fn over Scenario::sample(rng): one std::mt19937_64 per sample
+    // index, seeded from (planner seed, index), so a run is reproducible; the
+    // samples are drawn on the calling thread, in index order (the reference
+    // gives each worker a Scenario copy of its own, prrt.hpp:326,343)
+    static int hostSample(void* user, uint64_t seed, int64_t first, int64_t n, double* q, uint8_t* ok) {
+        auto* self = static_cast<GpuPRRT*>(user);
+        try {
+            if constexpr (detail::has_sample<Scenario>::value) {
+                for (int64_t i = 0; i < n; ++i) {
+                    std::mt19937_64 rng(detail::sample_seed(seed, (uint64_t)(first + i)));
+                    auto r = self->scenario_.sample(rng);
+                    if constexpr (detail::is_optional<decltype(r)>::value) {
+                        ok[i] = r ? 1 : 0;
+                        if (r) toDoubles(*r, q + i * kDim);
+                    } else {
+                        ok[i] = 1;
+                        toDoubles(r, q + i * kDim);
+                    }
+                }
+            }
+            return 0;
+        } catch (...) {
+            self->hostError_ = std::current_exception();
+            return 1;
+        }
+    }
+
     // a failing entry point rethrows the Scenario's own exception if it caused it
     void check(int rc) {
         if (rc != MPTG_OK && hostError_) {
@@ -425,6 +478,8 @@ class GpuPRRT {
             throw std::runtime_error(std::string("mptg: ") + mptg_last_error(nullptr));
         try {
             uploadScene();
+            if constexpr (detail::has_sample<Scenario>::value)
+                detail::check(ctx_, mptg_prrt_sampler(ctx_, &GpuPRRT::hostSample, this));
             configure();
         } catch (...) {
             mptg_destroy(ctx_);
@@ -513,26 +568,39 @@ class GpuPRRT {
         if (n) detail::check(ctx_, mptg_tree_read(ctx_, 0, n, states.data(), parents.data()));
     }
 
-    // Best goal by path cost (prrt.hpp:208-234, prrt_star.hpp:336-345); returns
-    // node id or -1.  PRRT* reads the costs the device maintains.
+    // The solution's goal node (-1 if none) and the parent its path leaves
+    // it by.  PRRT (prrt.hpp:208-234): the goal of least path cost, summed
+    // from the goal up, the first found on ties.  PRRT* (foundGoal,
+    // prrt_star.hpp:203-216): the goal edge found with the least path cost
+    // at the time it was found, the first on ties; its path leaves the goal
+    // by the parent it had then and follows the current tree above it
+    // (solution(), :324-336).
     int64_t bestGoal(const std::vector<double>& states, const std::vector<int32_t>& parents,
-                     double* bestCost = nullptr) const {
+                     double* bestCost = nullptr, int64_t* firstParent = nullptr) const {
         int64_t ng = 0;
         detail::check(ctx_, mptg_prrt_goals(ctx_, nullptr, 0, &ng));
+        ng = std::min<int64_t>(ng, 4096);
         std::vector<int32_t> goals(static_cast<size_t>(ng));
         if (ng) detail::check(ctx_, mptg_prrt_goals(ctx_, goals.data(), ng, &ng));
-        std::vector<double> costs;
+        goals.resize(static_cast<size_t>(std::min<int64_t>(ng, 4096)));
+        std::vector<double> cost0;
+        std::vector<int32_t> parent0;
         if constexpr (kStar) {
-            costs.resize(parents.size());
-            if (!costs.empty())
-                detail::check(ctx_, mptg_prrt_costs(ctx_, 0, (int64_t)costs.size(), costs.data()));
+            int64_t ne = 0;
+            cost0.resize(goals.size());
+            parent0.resize(goals.size());
+            if (!goals.empty())
+                detail::check(ctx_, mptg_prrt_goal_edges(ctx_, cost0.data(), parent0.data(), (int64_t)goals.size(), &ne));
         }
         double best = std::numeric_limits<double>::infinity();
-        int64_t bestId = -1;
-        for (int32_t g : goals) {
+        int64_t bestId = -1, bestParent = -1;
+        for (size_t i = 0; i < goals.size(); ++i) {
+            const int32_t g = goals[i];
             double cost = 0;
+            int64_t par = parents[g];
             if constexpr (kStar) {
-                cost = costs[g];
+                cost = cost0[i];
+                par = parent0[i];
             } else {
                 for (int64_t n = g; parents[n] >= 0; n = parents[n])
                     cost += scenario_.distance(&states[n * kDim], &states[parents[n] * kDim]);
@@ -540,19 +608,30 @@ class GpuPRRT {
             if (bestId < 0 || cost < best) {
                 best = cost;
                 bestId = g;
+                bestParent = par;
             }
         }
         if (bestCost) *bestCost = bestId < 0 ? std::numeric_limits<double>::quiet_NaN() : best;
+        if (firstParent) *firstParent = bestParent;
         return bestId;
     }
 
    public:
+    // link()'s trajectory type for Scenarios whose link() returns bool
+    // (link_trajectory<bool>, src/mpt/impl/link_trajectory.hpp:49-56)
+    using Trajectory = std::monostate;
+
     std::vector<State> solution() const {
         std::vector<double> states;
         std::vector<int32_t> parents;
         readTree(states, parents);
         std::vector<State> path;
-        for (int64_t n = bestGoal(states, parents); n >= 0; n = parents[n]) path.push_back(fromDoubles(&states[n * kDim]));
+        int64_t first = -1;
+        const int64_t goal = bestGoal(states, parents, nullptr, &first);
+        if (goal >= 0) {
+            path.push_back(fromDoubles(&states[goal * kDim]));
+            for (int64_t n = first; n >= 0; n = parents[n]) path.push_back(fromDoubles(&states[n * kDim]));
+        }
         return std::vector<State>(path.rbegin(), path.rend());
     }
 
@@ -566,10 +645,19 @@ class GpuPRRT {
         return c;
     }
 
-    // fn(state) once per waypoint of the best path, start first (prrt.hpp:276-285)
+    // The best path through a callback (prrt.hpp:236-285): fn(from, trajectory,
+    // to, true) once per edge, start first, when fn takes that shape
+    // (is_trajectory_callback, link_trajectory.hpp:76-84); otherwise fn(state)
+    // once per waypoint.
     template <typename Fn>
     void solution(Fn fn) const {
-        for (const State& s : solution()) fn(s);
+        const std::vector<State> path = solution();
+        if constexpr (std::is_invocable_v<Fn&, const State&, const Trajectory&, const State&, bool>) {
+            const Trajectory traj{};
+            for (size_t i = 1; i < path.size(); ++i) fn(path[i - 1], traj, path[i], true);
+        } else {
+            for (const State& s : path) fn(s);
+        }
     }
 
     // visitor.vertex(state) / visitor.edge(parentState) per node (prrt.hpp:300-316)

@@ -40,7 +40,8 @@ class OrcScene(C.Structure):
 class OrcPrrtParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("range", C.c_double), ("goal_bias", C.c_double),
                 ("goal_radius", C.c_double), ("batch", C.c_int32), ("_pad", C.c_int32),
-                ("lo", _dp), ("hi", _dp), ("goal", _dp)]
+                ("lo", _dp), ("hi", _dp), ("goal", _dp),
+                ("samples", _dp), ("samples_ok", _u8p), ("nsamples", C.c_int64)]
 
 
 class OrcPrrtStats(C.Structure):
@@ -88,7 +89,7 @@ _SIG = [
     ("orc_star_radius", C.c_double, [C.POINTER(OrcScene), _dp, _dp, C.c_double, C.c_int64]),
     ("orc_prrt_star_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), C.c_int32, C.c_double, _dp,
                                       C.c_int64, C.c_int64, _dp, _i32p, _dp, C.c_int64, _i32p, C.c_int64,
-                                      C.POINTER(OrcPrrtStats), C.POINTER(OrcStarStats)]),
+                                      C.POINTER(OrcPrrtStats), C.POINTER(OrcStarStats), _dp, _i32p]),
     ("orc_pool_run", C.c_int32, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_double,
                                  C.c_int32, C.POINTER(OrcPoolResult)]),
 ]
@@ -245,17 +246,27 @@ class Oracle:
                                                     1 if known_valid else 0, _p(sc, C.c_float), sc.shape[0],
                                                     lead, -1 if top_levels is None else top_levels))
 
-    def _params(self, seed, batch, range_=None, goal_bias=None):
+    def _params(self, seed, batch, range_=None, goal_bias=None, samples=None, samples_ok=None):
+        """samples / samples_ok: a custom sampler's table (row i = sample
+        index i; samples_ok[i] = 0: no sample for that slot)."""
         sc = self.scene
         lo, hi, g = _f64(sc.lo), _f64(sc.hi), _f64(sc.goal)
         self._keep += [lo, hi, g]
-        return OrcPrrtParams(seed=seed, range=sc.range if range_ is None else range_,
-                             goal_bias=sc.goal_bias if goal_bias is None else goal_bias,
-                             goal_radius=sc.goal_radius, batch=batch, lo=_p(lo, C.c_double),
-                             hi=_p(hi, C.c_double), goal=_p(g, C.c_double))
+        p = OrcPrrtParams(seed=seed, range=sc.range if range_ is None else range_,
+                          goal_bias=sc.goal_bias if goal_bias is None else goal_bias,
+                          goal_radius=sc.goal_radius, batch=batch, lo=_p(lo, C.c_double),
+                          hi=_p(hi, C.c_double), goal=_p(g, C.c_double))
+        if samples is not None:
+            tab = _f64(samples).reshape(-1, self.dim)
+            ok = np.ascontiguousarray(np.ones(tab.shape[0], np.uint8) if samples_ok is None else samples_ok,
+                                      dtype=np.uint8)
+            self._keep += [tab, ok]
+            p.samples, p.samples_ok, p.nsamples = _p(tab, C.c_double), _p(ok, C.c_uint8), tab.shape[0]
+        return p
 
-    def prrt(self, seed, batch, target, max_iters=1 << 40, range_=None, goal_bias=None):
-        p = self._params(seed, batch, range_, goal_bias)
+    def prrt(self, seed, batch, target, max_iters=1 << 40, range_=None, goal_bias=None, samples=None,
+             samples_ok=None):
+        p = self._params(seed, batch, range_, goal_bias, samples, samples_ok)
         cap = target + batch + 8
         nodes = np.zeros((cap, self.dim))
         parents = np.zeros(cap, dtype=np.int32)
@@ -266,7 +277,7 @@ class Oracle:
                                 _p(nodes, C.c_double), _p(parents, C.c_int32), cap, _p(goals, C.c_int32), 4096,
                                 C.byref(st))
         if n < 0:
-            raise RuntimeError("oracle capacity exceeded")
+            raise RuntimeError({-1: "oracle capacity exceeded", -4: "sample table too short"}.get(n, str(n)))
         stats = {f: int(getattr(st, f)) for f, _ in st._fields_}
         return nodes[:n].copy(), parents[:n].copy(), goals[: min(stats["goals"], 4096)].copy(), stats
 
@@ -277,28 +288,35 @@ class Oracle:
                                             rewire_factor, n))
 
     def prrt_star(self, seed, batch, target, k=20, rewire_factor=1.1, max_iters=1 << 40, range_=None,
-                  goal_bias=None):
+                  goal_bias=None, samples=None, samples_ok=None):
         """Batch-synchronous PRRT*: (nodes, parents, costs, goals, stats,
         star_stats).  k > 0 nearest neighbours, 0 the reference's k formula,
         -1 every node within the r-nearest radius."""
-        p = self._params(seed, batch, range_, goal_bias)
+        p = self._params(seed, batch, range_, goal_bias, samples, samples_ok)
         cap = target + batch + 8
         nodes = np.zeros((cap, self.dim))
         parents = np.zeros(cap, dtype=np.int32)
         costs = np.zeros(cap)
         goals = np.zeros(4096, dtype=np.int32)
+        gcost = np.zeros(4096)
+        gpar = np.zeros(4096, dtype=np.int32)
         st = OrcPrrtStats()
         ss = OrcStarStats()
         start = _f64(self.scene.start)
         n = self.L.orc_prrt_star_run(C.byref(self.s), C.byref(p), k, rewire_factor, _p(start, C.c_double), target,
                                      max_iters, _p(nodes, C.c_double), _p(parents, C.c_int32), _p(costs, C.c_double),
-                                     cap, _p(goals, C.c_int32), 4096, C.byref(st), C.byref(ss))
+                                     cap, _p(goals, C.c_int32), 4096, C.byref(st), C.byref(ss),
+                                     _p(gcost, C.c_double), _p(gpar, C.c_int32))
         if n < 0:
-            raise RuntimeError({-1: "oracle capacity exceeded", -2: "bad distance", -3: "k out of range"}[n])
+            raise RuntimeError({-1: "oracle capacity exceeded", -2: "bad distance", -3: "k out of range",
+                                -4: "sample table too short"}[n])
         stats = {f: int(getattr(st, f)) for f, _ in st._fields_}
         sstats = {f: int(getattr(ss, f)) for f, _ in ss._fields_}
-        return (nodes[:n].copy(), parents[:n].copy(), costs[:n].copy(), goals[: min(stats["goals"], 4096)].copy(),
-                stats, sstats)
+        ng = min(stats["goals"], 4096)
+        # the goal edges as found: cost and parent of each goal node then
+        sstats["goal_cost0"] = gcost[:ng].copy()
+        sstats["goal_parent0"] = gpar[:ng].copy()
+        return (nodes[:n].copy(), parents[:n].copy(), costs[:n].copy(), goals[:ng].copy(), stats, sstats)
 
     def pool(self, seed, target, seconds, threads, range_=None, goal_bias=None, tree=None):
         """The std::thread worker pool (reference loop restated) from the

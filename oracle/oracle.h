@@ -125,6 +125,12 @@ typedef struct {
     const double* lo;    /* sampling bounds */
     const double* hi;
     const double* goal;
+    /* a Scenario's own sampler (scenario_sampler.hpp:145-187), as a table:
+       row sidx is sample index sidx's state, samples_ok[sidx] = 0 for a
+       slot with no sample (skipped); NULL: the built-in box sampler */
+    const double* samples;
+    const uint8_t* samples_ok;
+    int64_t nsamples;
 } orc_prrt_params;
 
 typedef struct {
@@ -172,7 +178,8 @@ double orc_star_radius(const orc_scene* sc, const double* lo, const double* hi, 
 int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params* p, int32_t k_fixed, double rewire_factor,
                           const double* start, int64_t target, int64_t max_iters, double* nodes,
                           int32_t* parents, double* costs, int64_t cap, int32_t* goal_nodes, int64_t goal_cap,
-                          orc_prrt_stats* stats, orc_star_stats* star_stats);
+                          orc_prrt_stats* stats, orc_star_stats* star_stats, double* goal_cost0,
+                          int32_t* goal_parent0);
 
 /* Reference-style shared-memory worker pool (std::thread, one RNG per
  * worker, mt19937_64 + uniform_real_distribution, shared tree behind a

@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "../mpt_amd/csrc/pmath.hpp"
+#include "../mpt_amd/csrc/sample_hd.hpp"
 
 namespace orc_internal {
 bool valid_mesh(const orc_scene* sc, const double* q);
@@ -356,6 +357,29 @@ void parallel_for(int64_t n, int nthreads, Fn&& fn) {
 // exported API
 // ---------------------------------------------------------------------------
 namespace orc_internal {
+// The sample of slot `sidx`: the goal (the bias coin taken), a row of the
+// custom sampler's table, or the Philox box sample.  false: the custom
+// sampler gave no sample (the slot is skipped, as addSample(std::optional)
+// skips an empty one, prrt.hpp:400-403); -1 via *err when the table is short.
+bool sample_slot(const orc_scene* sc, const orc_prrt_params* p, uint64_t sidx, bool use_goal, double* q, int* err) {
+    const int d = sc->dim;
+    if (use_goal) {
+        std::memcpy(q, p->goal, sizeof(double) * d);
+        return true;
+    }
+    if (p->samples) {
+        if ((int64_t)sidx >= p->nsamples) {
+            *err = -1;
+            return false;
+        }
+        if (p->samples_ok && !p->samples_ok[sidx]) return false;
+        std::memcpy(q, p->samples + (int64_t)sidx * d, sizeof(double) * d);
+        return true;
+    }
+    mptg::sample_state(p->seed, sidx, sc->metric, d, p->lo, p->hi, q);
+    return true;
+}
+
 double distance(const orc_scene* sc, const double* a, const double* b) { return ::distance(sc, a, b); }
 void interpolate(const orc_scene* sc, const double* a, const double* b, double t, double* q) {
     ::interpolate(sc, a, b, t, q);

@@ -29,6 +29,7 @@
 #include "oracle.h"
 
 namespace orc_internal {
+bool sample_slot(const orc_scene* sc, const orc_prrt_params* p, uint64_t sidx, bool use_goal, double* q, int* err);
 double distance(const orc_scene* sc, const double* a, const double* b);
 void interpolate(const orc_scene* sc, const double* a, const double* b, double t, double* q);
 bool valid(const orc_scene* sc, const double* q);
@@ -40,7 +41,7 @@ struct Ext {
     bool accepted = false;
     bool goal = false;
     int32_t parent = -1;
-    bool zero = false, invalid = false, linked = false;
+    bool zero = false, invalid = false, linked = false, skipped = false;
     uint32_t checks = 0;
     double state[64];
 };
@@ -64,7 +65,10 @@ extern "C" int64_t orc_prrt_run(const orc_scene* sc, const orc_prrt_params* p, c
     int64_t goals = 0;
     std::vector<Ext> ext(B);
     unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    int table_err = 0;  // set only if a table row is missing (checked per iteration first)
     for (int64_t it = 0; it < max_iters && n < target; ++it) {
+        // a custom sampler's table must hold the iteration's rows
+        if (p->samples && (it + 1) * (int64_t)B > p->nsamples) return -4;
         const bool bias = goals == 0 && p->goal_bias > 0;
         const int64_t n0 = n;
         auto work = [&](int b) {
@@ -75,10 +79,9 @@ extern "C" int64_t orc_prrt_run(const orc_scene* sc, const orc_prrt_params* p, c
             // slot 0 of the iteration plays the reference's worker 0, the only
             // worker that biases (prrt.hpp:365-388)
             bool useGoal = bias && b == 0 && mptg::uniform_at(p->seed, sidx, mptg::RNG_PURPOSE_BIAS, 0) < p->goal_bias;
-            if (useGoal) {
-                std::memcpy(q, p->goal, sizeof(double) * d);
-            } else {
-                mptg::sample_state(p->seed, sidx, sc->metric, d, p->lo, p->hi, q);
+            if (!sample_slot(sc, p, sidx, useGoal, q, &table_err)) {
+                e.skipped = true;
+                return;
             }
             // nearest: ascending (distance, index) over the iteration's tree
             int32_t best = -1;
@@ -128,7 +131,7 @@ extern "C" int64_t orc_prrt_run(const orc_scene* sc, const orc_prrt_params* p, c
         for (int b = 0; b < B; ++b) {
             const Ext& e = ext[b];
             s.samples++;
-            s.nearest_calls++;
+            s.nearest_calls += !e.skipped;
             s.zero_dist += e.zero;
             s.invalid_states += e.invalid;
             if (e.linked) {

@@ -48,6 +48,7 @@
 #include "oracle.h"
 
 namespace orc_internal {
+bool sample_slot(const orc_scene* sc, const orc_prrt_params* p, uint64_t sidx, bool use_goal, double* q, int* err);
 double distance(const orc_scene* sc, const double* a, const double* b);
 void interpolate(const orc_scene* sc, const double* a, const double* b, double t, double* q);
 bool valid(const orc_scene* sc, const double* q);
@@ -133,7 +134,7 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
                                      double rewire_factor, const double* start, int64_t target, int64_t max_iters,
                                      double* nodes, int32_t* parents, double* costs, int64_t cap,
                                      int32_t* goal_nodes, int64_t goal_cap, orc_prrt_stats* st,
-                                     orc_star_stats* sst) {
+                                     orc_star_stats* sst, double* goal_cost0, int32_t* goal_parent0) {
     using namespace orc_internal;
     const int d = sc->dim;
     const int B = std::max(1, p->batch);
@@ -149,7 +150,10 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
     std::vector<StarExt> ext(B);
     std::vector<int> surv;
     unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    int table_err = 0;  // set only if a table row is missing (checked per iteration first)
     for (int64_t it = 0; it < max_iters && n < target; ++it) {
+        // a custom sampler's table must hold the iteration's rows
+        if (p->samples && (it + 1) * (int64_t)B > p->nsamples) return -4;
         const bool bias = goals == 0 && p->goal_bias > 0;
         const int64_t n0 = n;
         // ---- phase 1: extend against the iteration's tree
@@ -160,11 +164,7 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
             double q[64];
             // slot 0 plays worker 0, the only biasing worker (prrt_star.hpp:457-487)
             bool useGoal = bias && b == 0 && mptg::uniform_at(p->seed, sidx, mptg::RNG_PURPOSE_BIAS, 0) < p->goal_bias;
-            if (useGoal) {
-                std::memcpy(q, p->goal, sizeof(double) * d);
-            } else {
-                mptg::sample_state(p->seed, sidx, sc->metric, d, p->lo, p->hi, q);
-            }
+            if (!sample_slot(sc, p, sidx, useGoal, q, &table_err)) return;  // no sample: skipped
             int32_t best = -1;
             double bd = std::numeric_limits<double>::infinity();
             for (int64_t j = 0; j < n0; ++j) {
@@ -292,7 +292,12 @@ extern "C" int64_t orc_prrt_star_run(const orc_scene* sc, const orc_prrt_params*
             ss.spec_cp += e.spec_cp;
             ss.parent_changes += e.parent != e.near;
             if (e.goal) {
-                if (goals < goal_cap) goal_nodes[goals] = (int32_t)n;
+                // the goal edge as found (foundGoal, prrt_star.hpp:203-216): its cost and parent then
+                if (goals < goal_cap) {
+                    goal_nodes[goals] = (int32_t)n;
+                    if (goal_cost0) goal_cost0[goals] = e.pc;
+                    if (goal_parent0) goal_parent0[goals] = e.parent;
+                }
                 ++goals;
             }
             ++n;

@@ -15,14 +15,16 @@ BIAS = 0x42494153
 SAMPLE = 0x53414D50
 
 
-def literal_prrt(o, sc, seed, batch, target, goal_bias):
+def literal_prrt(o, sc, seed, batch, target, goal_bias, table=None, table_ok=None):
     """Worker::solve + Worker::addSample (src/mpt/impl/prrt/prrt.hpp:358-452)
     restated line by line on the separately pinned primitives (brute-force
     kNN, distance, interpolate, valid, link).  The `batch` samples of one
     iteration all see the tree as it stood at the iteration's start and are
     appended in sample order; slot 0 plays worker 0, the only worker that
     takes the goal-bias coin, and only while no goal is known (:365-388).
-    batch = 1 is the reference's single-worker loop."""
+    batch = 1 is the reference's single-worker loop.  With `table` the
+    Scenario's own sampler gives sample index i as table[i], or no sample when
+    table_ok[i] is 0 (addSample(std::optional) then returns, :400-403)."""
     nodes = [np.array(sc.start, dtype=np.float64)]
     parents = [-1]
     goals = []
@@ -35,6 +37,10 @@ def literal_prrt(o, sc, seed, batch, target, goal_bias):
             sidx = it * batch + b
             if b == 0 and biased and oracle.uniform_at(seed, sidx, BIAS, 0) < goal_bias:  # :381
                 q = np.array(sc.goal, dtype=np.float64)
+            elif table is not None:                                    # sampler(rng_), :385
+                if not table_ok[sidx]:
+                    continue
+                q = np.array(table[sidx], dtype=np.float64)
             else:
                 q = np.array([oracle.uniform_at(seed, sidx, SAMPLE, i) * (sc.hi[i] - sc.lo[i]) + sc.lo[i]
                               for i in range(sc.dim)])
@@ -141,3 +147,27 @@ def test_pool_baseline_grows_a_valid_tree():
     o = oracle.Oracle(sc)
     r = o.pool(seed=3, target=2000, seconds=20.0, threads=4)
     assert r["size"] >= 2000 and r["samples"] >= r["link_calls"] > 0
+
+
+def sampler_table(sc, n, seed):
+    """A custom sampler's output for sample indices 0..n-1: Beta(2, 5) per
+    coordinate over the box (samples crowd towards the low corner), and no
+    sample for one index in eight."""
+    rng = np.random.default_rng(seed)
+    tab = sc.lo + rng.beta(2.0, 5.0, (n, sc.dim)) * (sc.hi - sc.lo)
+    ok = (rng.uniform(size=n) >= 0.125).astype(np.uint8)
+    return tab, ok
+
+
+@pytest.mark.parametrize("batch", [1, 3, 16])
+def test_oracle_custom_sampler_equals_literal_loop(batch):
+    sc = scenes.boxes_scene()
+    o = oracle.Oracle(sc)
+    tab, ok = sampler_table(sc, 1 << 15, 5)
+    nodes, parents, goals, st = o.prrt(seed=23, batch=batch, target=300, goal_bias=0.2, samples=tab, samples_ok=ok)
+    ln, lp, lg = literal_prrt(o, sc, 23, batch, 300, 0.2, table=tab, table_ok=ok)
+    assert np.array_equal(nodes, ln) and np.array_equal(parents, lp) and np.array_equal(goals, lg)
+    assert st["samples"] == st["iterations"] * batch
+    assert st["zero_dist"] + st["invalid_states"] + st["link_calls"] < st["samples"]
+    with pytest.raises(RuntimeError):
+        o.prrt(seed=23, batch=batch, target=100000, samples=tab[:50], samples_ok=ok[:50])
