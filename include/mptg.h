@@ -228,9 +228,11 @@ typedef struct {
     int64_t parent_changes;  /* new nodes whose parent is not the near node */
     int64_t cp_links;        /* choose-parent link() calls evaluated (speculatively, in parallel) */
     int64_t rw_links;        /* rewire link() calls evaluated (speculatively, in parallel) */
-    int64_t deferred;        /* rewire links evaluated one at a time because the new node's
-                                cost dropped within its own iteration */
+    int64_t deferred;        /* rewire pairs whose condition held only because the new node's
+                                cost fell earlier in its own iteration's rewire pass */
     int64_t graph_launches;  /* hipGraph launches (mptg_prrt_graph), not in the reference */
+    int64_t rw_late;         /* of rw_links: queued within the rewire pass, in batches, when
+                                it reached a pair with no verdict (at least `deferred`) */
 } mptg_prrt_stats;
 
 int mptg_prrt_init(mptg_ctx* ctx, const mptg_prrt_params* p, const double* h_lo, const double* h_hi,

@@ -53,7 +53,7 @@ class PrrtStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "iterations", "samples", "zero_dist", "invalid_states", "link_calls", "link_ok",
         "valid_calls", "goals", "size", "rewire_tests", "rewires", "parent_changes", "cp_links", "rw_links",
-        "deferred", "graph_launches")]
+        "deferred", "graph_launches", "rw_late")]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

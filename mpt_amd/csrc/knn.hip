@@ -15,7 +15,9 @@
 //   * FP32 prefilter: d32 is compared against the lane's current k-th FP64
 //     distance plus a proven rounding margin; only lanes that pass evaluate
 //     the FP64 distance and insert into a register-resident sorted list;
-//   * partial lists of the chunks are merged by a second kernel.
+//   * partial lists of the chunks are merged by a second kernel;
+//   * SE(3) queries reverse the roles: a query gets a workgroup, its lanes take
+//     consecutive nodes (knn_wave_se3).
 //
 // Prefilter soundness.  With u = 2^-24, S = max|q| + max|p| and
 // eps = 2.01 u S, each FP32 coordinate difference satisfies
@@ -26,17 +28,21 @@
 // list only if d <= T (T = current k-th distance, squared for L2), hence only
 // if d32 <= T + eta(T).  The kernels use 2 * eta plus a relative 2^-40 for
 // the sqrt rounding of list entries.
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
 #include <limits>
 
 #include "internal.hpp"
+#include "wave_list.hpp"
 
 namespace mptg {
 namespace {
 
 constexpr int kBlock = 256;
+// Longest k-list of the SE(3) wave kernel (one entry per lane).
+constexpr int kWaveMaxK = 64;
 // Fewest tree nodes per split of the scan: small trees still spread over
 // many waves, while every split's partial list stays worth its merge.
 constexpr int64_t kMinSplitNodes = 64;
@@ -621,15 +627,6 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
 }
 
 // Direct FP64 scan for metrics without a prefilter (SE(3)) and any dimension.
-__device__ __forceinline__ double se3_dist(const double* a, const double* b, double w_so3, double w_l2);
-
-template <int K>
-__global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict__ x64, int64_t n, int64_t chunk,
-                                                       const double* __restrict__ q, int64_t nq, double radius,
-                                                       double w_so3, double w_l2, double* __restrict__ out_d,
-                                                       int32_t* __restrict__ out_i, const int64_t* n_dev,
-                                                       const int32_t* skip);
-
 template <int K>
 __global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                     int splits, int64_t nq, int k, double* __restrict__ out_d,
@@ -897,43 +894,91 @@ __global__ __launch_bounds__(kBlock) void knn_scatter(const double* __restrict__
 namespace mptg {
 namespace {
 
-__device__ __forceinline__ double se3_dist(const double* a, const double* b, double w_so3, double w_l2) {
-    double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
-    double ad = __builtin_fabs(dot);
-    double so3 = ad >= 1.0 ? 0.0 : pm::acos(ad);
+// SE(3) distance w_so3 * so3 + w_l2 * |dt| in two terms (the oracle's
+// expression; the sum of the two rounded products is the same either way
+// round).  The translation term is a lower bound of the distance: the
+// rotation term is >= 0 and rounding is monotone.
+__device__ __forceinline__ double se3_trans(const double* a, const double* b, double w_l2) {
     double s = 0.0;
     for (int i = 4; i < 7; ++i) {
         double t = a[i] - b[i];
         s = s + t * t;
     }
-    return w_so3 * so3 + w_l2 * __builtin_sqrt(s);
+    return w_l2 * __builtin_sqrt(s);
 }
 
-template <int K>
-__global__ __launch_bounds__(kBlock) void knn_scan_se3(const double* __restrict__ x64, int64_t n, int64_t chunk,
-                                                       const double* __restrict__ q, int64_t nq, double radius,
-                                                       double w_so3, double w_l2, double* __restrict__ out_d,
-                                                       int32_t* __restrict__ out_i, const int64_t* n_dev,
-                                                       const int32_t* skip) {
-    if (!knn_extent(n_dev, skip, &n, &chunk)) return;
-    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (qi >= nq) return;
-    const int64_t p0 = (int64_t)blockIdx.y * chunk;
-    const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
+__device__ __forceinline__ double se3_rot(const double* a, const double* b, double w_so3) {
+    double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+    double ad = __builtin_fabs(dot);
+    double so3 = ad >= 1.0 ? 0.0 : pm::acos(ad);
+    return w_so3 * so3;
+}
+
+// One workgroup of W waves per query.  Wave w measures tree nodes
+// w * 64 + lane, stepping by 64 W, into its k-list; a node's rotation term
+// (the acos) is evaluated only when its translation term does not exceed the
+// current k-th distance (nor the radius), so once the list has settled most
+// waves skip it.  The W lists meet in LDS, where an entry's rank among all
+// of them -- (distance, id) keys are distinct -- is its output position.
+template <int W>
+__global__ __launch_bounds__(64 * W) void knn_wave_se3(const double* __restrict__ x64, int64_t n,
+                                                       const double* __restrict__ q, int k, double radius,
+                                                       double w_so3, double w_l2, double inf,
+                                                       double* __restrict__ out_d, int32_t* __restrict__ out_i,
+                                                       const int64_t* n_dev, const int32_t* skip) {
+    __shared__ double sd[W * kWaveMaxK];
+    __shared__ int32_t si[W * kWaveMaxK];
+    __shared__ int real_s[W];
+    if (skip && *skip) return;
+    if (n_dev) n = *n_dev;
+    const int64_t qi = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double qq[7];
-    for (int i = 0; i < 7; ++i) qq[i] = q[qi * 7 + i];
-    TopK<K> top;
-    top.init();
-    for (int64_t p = p0; p < p1; ++p) {
-        double dq = se3_dist(qq, x64 + p * 7, w_so3, w_l2);
-        if (dq <= radius) top.insert(dq, (int32_t)p);
-    }
-    const int64_t base = ((int64_t)blockIdx.y * nq + qi) * K;
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-        out_d[base + j] = top.d[j];
-        out_i[base + j] = top.i[j];
+    for (int i = 0; i < 7; ++i) qq[i] = q[qi * 7 + i];
+    WaveList wl;
+    wl.init(k, inf);
+    for (int64_t base = (int64_t)w * 64; base < n; base += 64 * W) {
+        const int64_t p = base + lane;
+        bool pass = false;
+        double dq = 0.0;
+        if (p < n) {
+            const double* x = x64 + p * 7;
+            const double tr = se3_trans(qq, x, w_l2);
+            if (tr <= __builtin_fmin(wl.kd, radius)) {
+                dq = se3_rot(qq, x, w_so3) + tr;
+                pass = dq <= radius && wl_before(dq, (int32_t)p, wl.kd, wl.ki);
+            }
+        }
+        wl.offer(pass, dq, (int32_t)p, lane);
     }
+    const bool mine = lane < k;
+    const unsigned long long real = __ballot(mine && wl.li != INT_MAX);
+    if (mine) {
+        sd[w * k + lane] = wl.ld;
+        si[w * k + lane] = wl.li;
+    }
+    if (lane == 0) real_s[w] = __popcll(real);
+    __syncthreads();
+    int total = 0;
+#pragma unroll
+    for (int v = 0; v < W; ++v) total += real_s[v];
+    for (int c = threadIdx.x; c < W * k; c += 64 * W) {
+        const int32_t ci = si[c];
+        if (ci == INT_MAX) continue;
+        const double cd = sd[c];
+        int rank = 0;
+        for (int e = 0; e < W * k; ++e) rank += (si[e] != INT_MAX && wl_before(sd[e], si[e], cd, ci)) ? 1 : 0;
+        if (rank < k) {
+            out_d[qi * k + rank] = cd;
+            out_i[qi * k + rank] = ci;
+        }
+    }
+    for (int r = threadIdx.x; r < k; r += 64 * W)
+        if (r >= total) {
+            out_d[qi * k + r] = inf;
+            out_i[qi * k + r] = -1;
+        }
 }
 
 struct ScanExtent {
@@ -980,11 +1025,6 @@ bool dispatch_metric(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double
 template <int K>
 bool dispatch_dim(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q, int64_t nq, double radius,
                   double* pd, int32_t* pi) {
-    if (ctx->metric == MPTG_METRIC_SE3) {
-        hipLaunchKernelGGL((knn_scan_se3<K>), grid, dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, x.chunk, q, nq,
-                           radius, ctx->w_so3, ctx->w_l2, pd, pi, x.n_dev, x.skip);
-        return true;
-    }
     switch (ctx->dim) {
         case 1: return dispatch_metric<1, K>(ctx, grid, x, q, nq, radius, pd, pi);
         case 2: return dispatch_metric<2, K>(ctx, grid, x, q, nq, radius, pd, pi);
@@ -1085,6 +1125,33 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
         else if (K == 32) hipLaunchKernelGGL((knn_finish<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
         else hipLaunchKernelGGL((knn_finish<64>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, nq, k, d_dist, d_idx, skip, nq_dev);
     }
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
+}
+
+// SE(3): one workgroup of W waves per query, W a power of two up to 16 for
+// about 4096 waves per launch, and no more waves than 64-node groups.
+static int knn_wave_se3_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radius, int32_t* d_idx,
+                             double* d_dist, int64_t tree_n, const int64_t* n_dev, const int32_t* skip) {
+    if (k < 1 || k > kWaveMaxK) return fail(ctx, MPTG_E_RANGE, "SE(3) nearest neighbours: k must be 1..64");
+    if (nq >= ((int64_t)1 << 31)) return fail(ctx, MPTG_E_RANGE, "at most 2^31 - 1 queries");
+    if (nq == 0) return MPTG_OK;
+    const int64_t want = (4096 + nq - 1) / nq, groups = std::max<int64_t>(1, (tree_n + 63) / 64);
+    int W = 1;
+    while (W < 16 && 2 * W <= want && 2 * W <= groups) W *= 2;
+    const double inf = std::numeric_limits<double>::infinity();
+    const dim3 g((unsigned)nq);
+#define MPTG_WAVE_SE3(WW)                                                                                         \
+    hipLaunchKernelGGL((knn_wave_se3<WW>), g, dim3(64 * WW), 0, ctx->stream, (const double*)ctx->x64, tree_n, d_q, \
+                       k, radius, ctx->w_so3, ctx->w_l2, inf, d_dist, d_idx, n_dev, skip)
+    switch (W) {
+        case 1: MPTG_WAVE_SE3(1); break;
+        case 2: MPTG_WAVE_SE3(2); break;
+        case 4: MPTG_WAVE_SE3(4); break;
+        case 8: MPTG_WAVE_SE3(8); break;
+        default: MPTG_WAVE_SE3(16); break;
+    }
+#undef MPTG_WAVE_SE3
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }
@@ -1217,6 +1284,7 @@ int launch_knn(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, double radiu
             return fail(ctx, MPTG_E_STATE, "a device query count needs k = 1 and an L1/L2 space");
         return knn_scan_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n, n_dev, skip, nq_dev);
     }
+    if (ctx->metric == MPTG_METRIC_SE3) return knn_wave_se3_path(ctx, d_q, nq, k, radius, d_idx, d_dist, tree_n, n_dev, skip);
     const bool sel_dim = ctx->dim <= 8 || ctx->dim == 10 || ctx->dim == 20;
     const bool eager = nq > 0 && !n_dev && !skip && !ctx->capturing;
     // (the select path keeps lists of up to 32; larger k takes the k-list scan)

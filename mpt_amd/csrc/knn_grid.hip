@@ -29,6 +29,7 @@
 
 #include "internal.hpp"
 #include "space_dev.hpp"
+#include "wave_list.hpp"
 
 namespace mptg {
 namespace {
@@ -140,49 +141,6 @@ __global__ __launch_bounds__(kB) void grid_scatter(const double* __restrict__ x6
     sid[pos] = (int32_t)i;
 }
 
-__device__ __forceinline__ bool better(double a, int ia, double b, int ib) {
-    return a < b || (a == b && ia < ib);
-}
-
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
-}
-
-// The wave's k-list: lane j < k holds the j-th best (distance, node id),
-// ascending; (kd, ki) mirror lane k - 1's entry (the k-th best) as
-// wave-uniform values.  +inf comes in as a value (a kernel argument), not a
-// constant: this toolchain encodes a 64-bit scalar move of the constant
-// 0x7ff0000000000000 with a 32-bit literal holding its low half, so a
-// uniform +inf held in scalar registers would read as 0.0.
-struct WaveList {
-    double ld;
-    int32_t li;
-    double kd;
-    int32_t ki;
-    int k;
-    __device__ __forceinline__ void init(int k_, double inf) {
-        ld = inf;
-        li = INT_MAX;
-        kd = inf;
-        ki = INT_MAX;
-        k = k_;
-    }
-    // Inserts a wave-uniform candidate that beats (kd, ki): every entry
-    // after it moves one lane up.
-    __device__ __forceinline__ void insert(double cd, int32_t ci, int lane) {
-        const double pd = __shfl_up(ld, 1);
-        const int32_t pi = __shfl_up(li, 1);
-        const bool shift = lane != 0 && better(cd, ci, pd, pi);
-        const bool here = !shift && better(cd, ci, ld, li);
-        ld = shift ? pd : (here ? cd : ld);
-        li = shift ? pi : (here ? ci : li);
-        kd = readlane_d(ld, k - 1);
-        ki = __builtin_amdgcn_readlane(li, k - 1);
-    }
-};
-
 // Every node of cell c (wave-uniform) within the radius that beats the k-th
 // best, into the list; lanes take consecutive nodes (contiguous rows).  For
 // L2 the square root is taken only for sums that can reach the k-th
@@ -219,15 +177,10 @@ __device__ __forceinline__ void wave_scan_cell(const double* __restrict__ sx, in
             }
             if (pass) {
                 id = sid[j];
-                pass = better(dq, id, wl.kd, wl.ki);
+                pass = wl_before(dq, id, wl.kd, wl.ki);
             }
         }
-        for (unsigned long long m = __ballot(pass); m; m &= m - 1) {
-            const int b = __builtin_ctzll(m);
-            const double cd = readlane_d(dq, b);
-            const int32_t ci = __builtin_amdgcn_readlane(id, b);
-            if (better(cd, ci, wl.kd, wl.ki)) wl.insert(cd, ci, lane);
-        }
+        wl.offer(pass, dq, id, lane);
     }
 }
 
@@ -342,7 +295,7 @@ __global__ __launch_bounds__(kB) void grid_query(const double* __restrict__ sx, 
             unsigned long long scanned = 0;
             for (unsigned long long m = __ballot(want); m; m &= m - 1) {
                 const int b = __builtin_ctzll(m);
-                if (need(readlane_d(lb, b))) {
+                if (need(wl_readlane(lb, b))) {
                     wave_scan_cell<D, METRIC>(sx, n, sid, start, (uint32_t)__builtin_amdgcn_readlane((int)cc, b),
                                               qq, radius, wl, lane);
                     scanned |= 1ull << b;

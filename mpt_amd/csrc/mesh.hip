@@ -489,8 +489,12 @@ int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from
     if (n <= 0) return MPTG_OK;
     int rc;
     if ((rc = launch_fill_u32(ctx, d_checks, n, 0xffffffffu))) return rc;  // first-failure slots
-    const int P = ctx->tune.mesh_edge_waves;  // n <= 2^31 and P <= 32: n * P fits
-    const unsigned gp = (unsigned)std::min<int64_t>(n * P, (int64_t)ctx->cus * ctx->tune.mesh_waves_per_cu);
+    // waves per edge: the tuned count, or more for a batch too small to fill
+    // the device (the rewire pass's late batches hold a few edges), up to 256
+    // (the verdicts and counts do not depend on it)
+    const int64_t fill = (int64_t)ctx->cus * ctx->tune.mesh_waves_per_cu;
+    const int P = (int)std::max<int64_t>(ctx->tune.mesh_edge_waves, std::min<int64_t>(256, fill / n));
+    const unsigned gp = (unsigned)std::min<int64_t>(n * P, fill);
     hipLaunchKernelGGL(mesh_link, dim3(gp), dim3(64), 0, ctx->stream, space_args(ctx), mesh_args(ctx),
                        1.0 / ctx->link_param, d_table, d_from, d_to, n, known_valid, d_mask, P, d_checks);
     hipLaunchKernelGGL(mesh_link_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, space_args(ctx),

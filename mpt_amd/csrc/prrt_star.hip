@@ -27,8 +27,11 @@
 //                                   (:664-688).  A pair whose condition holds
 //                                   only because the new node's own cost fell
 //                                   earlier in the pass has no queued verdict:
-//                                   the pass stops, the host checks that one
-//                                   edge, and the pass resumes.
+//                                   the pass stops there and queues, as one
+//                                   batch, every later pair whose condition
+//                                   holds under the current costs and has no
+//                                   verdict; the host links the batch and the
+//                                   pass resumes at the same pair.
 // Link verdicts do not depend on who asks or in which order, so the
 // speculative batches change the work done, never the tree.
 #include <algorithm>
@@ -47,9 +50,10 @@ struct StarCounters {
     int64_t m;          // survivors of the current iteration
     int64_t ncp, nrw;   // queued choose-parent / rewire edges
     int64_t pos;        // rewire pass: next (survivor, neighbour) pair
-    int64_t pend_pair;  // pair waiting for a verdict
-    int32_t pending, bad_distance;  // bad_distance: 1 near-node cost mismatch, 2 corrupt child lists
-    int64_t rewire_tests, rewires, parent_changes, cp_links, rw_links, deferred;
+    int64_t late_from;  // first row of the rewire pass's latest batch
+    int32_t pending, bad_distance;  // pending: a batch waits for verdicts; bad_distance: 1 near-node
+                                    // cost mismatch, 2 corrupt child lists
+    int64_t rewire_tests, rewires, parent_changes, cp_links, rw_links, deferred, rw_late;
 };
 
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
@@ -387,9 +391,8 @@ __global__ __launch_bounds__(1024) void star_append(SpaceArgs sp, PrrtDev pd, do
 // ---- rewire pass (single block) ---------------------------------------------
 struct RewireShared {
     int64_t pos;
-    int64_t pair;
-    int first, unknown, pend, fn, gn, bad;
-    int64_t seen;
+    int first, unknown, fn, gn, bad, queued;
+    int64_t seen, late;
     double delta;
 };
 
@@ -457,44 +460,40 @@ __device__ void rewire_apply(RewireShared& s, int64_t pair, int k, const int32_t
 }
 
 // Pairs are the list entries in order (survivor by survivor, each list in
-// order): `total` of them, survivor of entry e = own[e] (or e / k).
+// order): `total` of them, survivor of entry e = own[e] (or e / k).  Verdicts
+// queued before the pass have rows below nrw0; a pair the walk reaches with
+// its condition holding and a row at or above nrw0 held only because its new
+// node's cost fell earlier in the pass (`deferred`).  A pair reached with its
+// condition holding and no row stops the pass: every pair from it on whose
+// condition holds under the current costs and has no row gets one (rows
+// [late_from, nrw), pair_rw set), `pending` asks the host to link them, and
+// the next launch resumes at the same pair with the same costs.
 __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t total, int64_t n0, int64_t fcap,
-                                                    const int32_t* __restrict__ own,
+                                                    int64_t nrw0, const int32_t* __restrict__ own,
                                                     const int32_t* __restrict__ nbh_idx,
                                                     const double* __restrict__ nbh_d,
-                                                    const uint8_t* __restrict__ marked,
-                                                    const int32_t* __restrict__ pair_rw,
-                                                    const uint8_t* __restrict__ rw_ver,
-                                                    const uint8_t* __restrict__ pend_ver,
-                                                    const double* __restrict__ x64, double* cost, int32_t* parent,
-                                                    int32_t* child, int32_t* next, int32_t* prev, int32_t* fa,
-                                                    int32_t* fb, double* __restrict__ pend_rows, StarCounters* sc) {
+                                                    const uint8_t* __restrict__ marked, int32_t* pair_rw,
+                                                    const uint8_t* __restrict__ rw_ver, const double* __restrict__ x64,
+                                                    double* cost, int32_t* parent, int32_t* child, int32_t* next,
+                                                    int32_t* prev, int32_t* fa, int32_t* fb,
+                                                    double* __restrict__ rw_from, double* __restrict__ rw_to,
+                                                    StarCounters* sc) {
     __shared__ RewireShared s;
     const int t = threadIdx.x;
     if (t == 0) {
         s.pos = sc->pos;
-        s.pend = sc->pending;
-        s.pair = sc->pend_pair;
         s.first = INT_MAX;
         s.bad = 0;
+        s.late = 0;
+        s.queued = 0;
+        sc->pending = 0;
     }
     __syncthreads();
-    if (s.pend) {
-        // costs are as they were when the pass stopped at this pair, so its
-        // condition still holds; the host has checked its link since
-        if (pend_ver[0])
-            rewire_apply(s, s.pair, k, own, n0, fcap, nbh_idx, nbh_d, cost, parent, child, next, prev, fa, fb, sc);
-        if (t == 0) {
-            sc->pending = 0;
-            s.pos = s.pair + 1;
-        }
-        __syncthreads();
-    }
     for (;;) {
         const int64_t p0 = s.pos;
         if (p0 >= total || s.bad) break;
         const int64_t pair = p0 + t;
-        bool act = false, unk = false;
+        bool act = false, unk = false, late = false;
         if (pair < total) {
             const int32_t nb = nbh_idx[pair];
             const int64_t j = nbh_owner(own, k, pair);
@@ -504,6 +503,7 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t total,
                     const int32_t r = pair_rw[pair];
                     if (r >= 0) {
                         act = rw_ver[r] != 0;
+                        late = r >= nrw0;
                     } else {
                         act = true;
                         unk = true;
@@ -514,6 +514,9 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t total,
         if (act) atomicMin(&s.first, t);
         __syncthreads();
         const int f = s.first;
+        // the walk reaches the pairs up to the first that acts (all of them
+        // when none does) under these costs
+        if (late && t <= f) atomicAdd((unsigned long long*)&s.late, 1ull);
         if (f == INT_MAX) {
             __syncthreads();
             if (t == 0) s.pos = p0 + 1024;
@@ -523,19 +526,29 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t total,
         if (t == f) s.unknown = unk ? 1 : 0;
         __syncthreads();
         if (s.unknown) {
-            if (t == 0) {
-                const int64_t pr = p0 + f;
-                const int64_t id = n0 + nbh_owner(own, k, pr);
-                const int32_t nb = nbh_idx[pr];
+            const int64_t stop = p0 + f;
+            const int64_t base = sc->nrw;
+            for (int64_t e = stop + t; e < total; e += 1024) {
+                const int32_t nb = nbh_idx[e];
+                if (nb < 0 || marked[e] || pair_rw[e] >= 0) continue;
+                const int64_t id = n0 + nbh_owner(own, k, e);
+                if (!(cost[id] + nbh_d[e] < cost[nb])) continue;
+                const int64_t r = base + atomicAdd(&s.queued, 1);
+                pair_rw[e] = (int32_t)r;
                 for (int i = 0; i < D; ++i) {
-                    pend_rows[i] = x64[id * D + i];
-                    pend_rows[D + i] = x64[(int64_t)nb * D + i];
+                    rw_from[r * D + i] = x64[id * D + i];
+                    rw_to[r * D + i] = x64[(int64_t)nb * D + i];
                 }
+            }
+            __syncthreads();
+            if (t == 0) {
+                sc->late_from = base;
+                sc->nrw = base + s.queued;
+                sc->rw_links += s.queued;
+                sc->rw_late += s.queued;
+                sc->deferred += s.late;
                 sc->pending = 1;
-                sc->pend_pair = pr;
-                sc->pos = pr;
-                sc->deferred += 1;
-                sc->rw_links += 1;
+                sc->pos = stop;
             }
             return;
         }
@@ -548,6 +561,7 @@ __global__ __launch_bounds__(1024) void star_rewire(int D, int k, int64_t total,
     }
     if (t == 0) {
         sc->pos = total;
+        sc->deferred += s.late;
         if (s.bad) sc->bad_distance = 2;
     }
 }
@@ -605,8 +619,6 @@ static int star_reserve(mptg_ctx* ctx, int B) {
     if ((rc = ensure(ctx, pb.qs, sizeof(double) * D * B))) return rc;
     if ((rc = ensure(ctx, pb.pc, sizeof(double) * B))) return rc;
     if ((rc = ensure(ctx, pb.cpar, sizeof(int32_t) * B))) return rc;
-    // two pending rows, then verdict and checks of the pending edge
-    if ((rc = ensure(ctx, pb.pend, sizeof(double) * 2 * D + 16))) return rc;
     if ((rc = ensure(ctx, pb.star_cnt, sizeof(StarCounters)))) return rc;
     return MPTG_OK;
 }
@@ -787,26 +799,28 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
                 s.goal_parent0.push_back(par);
             }
             // ---- rewire
-            if (hs->nrw > 0 &&
-                (rc = launch_link(ctx, (const double*)s.rw_from.p, (const double*)s.rw_to.p, hs->nrw, s.p.order,
-                                  1, (uint8_t*)s.rw_ver.p, (uint32_t*)s.rw_chk.p)))
+            const int64_t nrw0 = hs->nrw;
+            if (nrw0 > 0 &&
+                (rc = launch_link(ctx, (const double*)s.rw_from.p, (const double*)s.rw_to.p, nrw0, s.p.order, 1,
+                                  (uint8_t*)s.rw_ver.p, (uint32_t*)s.rw_chk.p)))
                 return rc;
             int32_t* front = (int32_t*)s.front.p;
-            double* pend_rows = (double*)s.pend.p;
-            uint8_t* pend_ver = (uint8_t*)(pend_rows + 2 * D);
-            uint32_t* pend_chk = (uint32_t*)(pend_ver + 8);
             for (;;) {
-                hipLaunchKernelGGL(star_rewire, dim3(1), dim3(1024), 0, ctx->stream, D, k, E, n0, n0 + m, own,
+                hipLaunchKernelGGL(star_rewire, dim3(1), dim3(1024), 0, ctx->stream, D, k, E, n0, n0 + m, nrw0, own,
                                    (const int32_t*)s.nbh_idx.p, (const double*)s.nbh_d.p,
-                                   (const uint8_t*)s.marked.p, (const int32_t*)s.pair_rw.p,
-                                   (const uint8_t*)s.rw_ver.p, (const uint8_t*)pend_ver, (const double*)ctx->x64,
-                                   ctx->cost, ctx->parent, ctx->child, ctx->next, ctx->prev, front,
-                                   front + (n0 + B), pend_rows, dsc);
+                                   (const uint8_t*)s.marked.p, (int32_t*)s.pair_rw.p, (const uint8_t*)s.rw_ver.p,
+                                   (const double*)ctx->x64, ctx->cost, ctx->parent, ctx->child, ctx->next,
+                                   ctx->prev, front, front + (n0 + B), (double*)s.rw_from.p, (double*)s.rw_to.p,
+                                   dsc);
                 MPTG_HIP(ctx, hipGetLastError());
                 if ((rc = read_counters())) return rc;
                 if (hs->bad_distance == 2) return fail(ctx, MPTG_E_STATE, "PRRT* child lists are corrupt");
                 if (!hs->pending) break;
-                if ((rc = launch_link(ctx, pend_rows, pend_rows + D, 1, s.p.order, 1, pend_ver, pend_chk)))
+                // the batch the pass queued when it stopped
+                const int64_t a = hs->late_from, b = hs->nrw;
+                if (b <= a || b > E) return fail(ctx, MPTG_E_STATE, "PRRT* rewire batch out of range");
+                if ((rc = launch_link(ctx, (const double*)s.rw_from.p + a * D, (const double*)s.rw_to.p + a * D,
+                                      b - a, s.p.order, 1, (uint8_t*)s.rw_ver.p + a, (uint32_t*)s.rw_chk.p + a)))
                     return rc;
             }
         } else {
@@ -834,6 +848,7 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         s.stats.cp_links = b0.cp_links + hs->cp_links;
         s.stats.rw_links = b0.rw_links + hs->rw_links;
         s.stats.deferred = b0.deferred + hs->deferred;
+        s.stats.rw_late = b0.rw_late + hs->rw_late;
         s.stats.goals = s.goals;
         s.stats.size = ctx->size;
     }

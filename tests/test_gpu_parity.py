@@ -87,22 +87,36 @@ def test_knn_radius_and_edge_cases(ctx_factory):
     assert i.shape == (0, 3)
 
 
-def test_knn_se3_parity(ctx_factory):
-    _say("knn se3")
+@pytest.mark.parametrize("n,nq,k,radius", [(3000, 300, 20, math.inf), (3000, 300, 1, math.inf),
+                                            (20000, 37, 64, math.inf), (6000, 4096, 1, math.inf),
+                                            (3000, 200, 20, 40.0), (40, 50, 64, math.inf)])
+def test_knn_se3_parity(ctx_factory, n, nq, k, radius):
+    """SE(3) k-nearest (one workgroup of 1..16 waves per query, the rotation
+    term skipped where the translation term already exceeds the k-th
+    distance): ids and distances equal the oracle's, with duplicate nodes,
+    a query on a node, antipodal quaternions (the same rotation), fewer
+    nodes than k and a radius."""
+    _say("knn se3", n, nq, k, radius)
     ctx = ctx_factory()
     rng = np.random.default_rng(8)
 
-    def states(n):
-        qu = rng.normal(size=(n, 4))
+    def states(m):
+        qu = rng.normal(size=(m, 4))
         qu /= np.linalg.norm(qu, axis=1, keepdims=True)
-        return np.concatenate([qu, rng.uniform(0, 100, (n, 3))], axis=1)
+        return np.concatenate([qu, rng.uniform(0, 100, (m, 3))], axis=1)
 
-    pts, q = states(3000), states(300)
+    pts, q = states(n), states(nq)
+    pts[10:20] = pts[3]                                   # duplicates: ties broken by id
+    pts[21] = np.concatenate([-pts[3, :4], pts[3, 4:]])   # the same rotation, negated
+    q[0] = pts[3]
     ctx.space(scenes.METRIC_SE3, 7, 50.0, 1.0)
     ctx.tree_insert(pts)
-    gi, gd = ctx.knn(q, 20)
-    oi, od = oracle.Oracle(_plain(scenes.METRIC_SE3, 7, 50.0, 1.0)).knn(pts, q, 20)
+    gi, gd = ctx.knn(q, k, radius)
+    oi, od = oracle.Oracle(_plain(scenes.METRIC_SE3, 7, 50.0, 1.0)).knn(pts, q, k, radius, threads=16)
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    assert gd[0, 0] == 0.0 and (gi[0, :min(k, 12)] >= 0).all()
+    if radius < math.inf:
+        assert (gi == -1).any() and (gi >= 0).any()
 
 
 @pytest.mark.parametrize("k", [1, 20])

@@ -239,4 +239,5 @@ def test_host_scene_prrt_star_equals_oracle(ctx_factory):
     assert np.array_equal(gn, on) and np.array_equal(gp, op) and np.array_equal(ctx.prrt_costs(), oc)
     for key in ("rewire_tests", "rewires", "parent_changes", "deferred"):
         assert st[key] == oss[key], key
-    assert st["cp_links"] == oss["spec_cp"] and st["rw_links"] == oss["spec_rw"] + oss["deferred"]
+    assert st["cp_links"] == oss["spec_cp"]
+    assert st["rw_links"] == oss["spec_rw"] + st["rw_late"] and st["rw_late"] >= oss["deferred"]
