@@ -167,7 +167,6 @@ def bench_arm(ctx, args, timer):
     da, db = ctx.to_device(a), ctx.to_device(b)
     v = ctx.array(ne, np.uint8)
     c = ctx.array(ne, np.uint32)
-    scores = ctx.array((ne, 512), np.float32)
     out = {}
     for name, order in (("plain", mpt_amd.ORDER_REFERENCE), ("surrogate", mpt_amd.ORDER_SURROGATE)):
         def run(order=order):
@@ -178,7 +177,6 @@ def bench_arm(ctx, args, timer):
         t = timer.time(run, 5)
         out[name] = {"ms": t * 1e3, "edge_checks_per_s": ne / t, "verdicts": v.numpy().copy(),
                      "checks": c.numpy().astype(np.int64)}
-    t_score = timer.time(lambda: ctx.surrogate_score_edges(da.ptr, db.ptr, ne, scores.ptr), 5)
     # per-phase HIP-event times of the link call (score, rank, head, tail, list)
     phases = {}
     for name, order in (("plain", mpt_amd.ORDER_REFERENCE), ("surrogate", mpt_amd.ORDER_SURROGATE)):
@@ -191,8 +189,21 @@ def bench_arm(ctx, args, timer):
     wmax = np.abs(a - b).max(axis=1)
     h = np.ceil(np.log2(np.maximum(wmax / sc.link_param, 1.0)) + 1e-12).astype(int)
     h = np.where(wmax < sc.link_param, 0, h)
-    rows = int(((1 << h) - 1).sum())
+    # the picks kernel scores each edge's top tree (the 15 waypoints of the
+    # top 4 bisection levels, fewer on shallower trees)
+    rows = int(((1 << np.minimum(h, 4)) - 1).sum())
     mlp_flops = rows * 2 * (64 * 128 + 2 * 128 * 128 + 128)
+    t_score = phases["surrogate"]["score"] / 1e3
+    # ceiling of any exact check order: every waypoint of a collision-free
+    # edge is checked whatever the order, so the link on those edges alone
+    # (reference order) bounds the batch's link time from below
+    free = ~coll
+    nf = int(free.sum())
+    fa, fb = ctx.to_device(np.ascontiguousarray(a[free])), ctx.to_device(np.ascontiguousarray(b[free]))
+    vf, cf = ctx.array(nf, np.uint8), ctx.array(nf, np.uint32)
+    t_free = timer.time(lambda: ctx.link_dev(fa.ptr, fb.ptr, nf, mpt_amd.ORDER_REFERENCE, True, vf.ptr, cf.ptr), 5)
+    for x in (fa, fb, vf, cf):
+        x.free()
     flops_p, flops_s = fk_flops(wk_p), fk_flops(wk_s)
     # the dominant kernels: the packed head round + the continuation, surrogate order
     t_kernel = (phases["surrogate"]["head"] + phases["surrogate"]["tail"]) / 1e3
@@ -204,6 +215,8 @@ def bench_arm(ctx, args, timer):
         "surrogate_ms": out["surrogate"]["ms"],
         "score_ms": t_score * 1e3,
         "phases_ms": phases,
+        "collision_free_only_ms": t_free * 1e3,
+        "ceiling_gain": out["plain"]["ms"] / (t_free * 1e3),
         "plain_edge_checks_per_s": out["plain"]["edge_checks_per_s"],
         "surrogate_edge_checks_per_s": out["surrogate"]["edge_checks_per_s"],
         "gain": out["surrogate"]["edge_checks_per_s"] / out["plain"]["edge_checks_per_s"],
@@ -252,7 +265,7 @@ def bench_arm(ctx, args, timer):
            "trees_identical": same_tree, "training": train,
            "edge_batch": edge_batch,
            "planner_gain": sur["edge_checks_per_s"] / plain["edge_checks_per_s"]}
-    for x in (da, db, v, c, scores):
+    for x in (da, db, v, c):
         x.free()
     return res, roof, sc, tree
 
@@ -447,7 +460,8 @@ def kernels_only(ctx, args):
     """Profiling mode (--kernels): the dominant kernels alone, a few launches
     each, so that rocprofv3 kernel traces and counter passes stay short:
     a 256 MiB plain copy (byte-counter reference), then the arm link on 8192
-    planner edges in reference and in surrogate order, and the score kernel."""
+    planner edges in reference and in surrogate order (its picks kernel
+    included)."""
     import mpt_amd
     from mpt_amd import scenes, surrogate
     sc = scenes.arm_scene()
@@ -460,7 +474,6 @@ def kernels_only(ctx, args):
     da, db = ctx.to_device(a), ctx.to_device(b)
     v = ctx.array(ne, np.uint8)
     c = ctx.array(ne, np.uint32)
-    scores = ctx.array((ne, 512), np.float32)
     nbytes = 256 << 20
     src = ctx.array(nbytes // 4, np.float32)
     dst = ctx.array(nbytes // 4, np.float32)
@@ -473,7 +486,6 @@ def kernels_only(ctx, args):
     for name, order in (("plain", mpt_amd.ORDER_REFERENCE), ("surrogate", mpt_amd.ORDER_SURROGATE)):
         res[f"link_{name}_ms"] = timer.time(
             lambda order=order: ctx.link_dev(da.ptr, db.ptr, ne, order, True, v.ptr, c.ptr), reps) * 1e3
-    res["score_ms"] = timer.time(lambda: ctx.surrogate_score_edges(da.ptr, db.ptr, ne, scores.ptr), reps) * 1e3
     print(json.dumps({"mode": "kernels", "edges": ne, "reps": reps, "tail_autotune": tune,
                       "launch_shapes": ctx.tune_get(), **res}))
 

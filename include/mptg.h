@@ -67,8 +67,9 @@ enum mptg_link_mode {
 
 enum mptg_check_order {
     MPTG_ORDER_REFERENCE = 0, /* the reference's waypoint order */
-    MPTG_ORDER_SURROGATE = 1  /* reference ordinals 0-3 first, then 4-63 by descending surrogate
-                                 score (ties: reference order), then the rest in reference order */
+    MPTG_ORDER_SURROGATE = 1  /* the 4 waypoints of highest surrogate score among the 15 of the
+                                 bisection tree's top 4 levels (ties: reference order) first,
+                                 then every other waypoint in reference order */
 };
 
 typedef struct mptg_ctx mptg_ctx;
@@ -405,10 +406,9 @@ int mptg_autotune_link(mptg_ctx* ctx, const double* d_from, const double* d_to, 
 int mptg_autotune_prrt(mptg_ctx* ctx, int64_t target_size, mptg_tune_report* rep);
 
 /* one arm-scene link() call (as mptg_link_dev) timed phase by phase with HIP
- * events on the context stream: h_ms[0] packed head round, [1] surrogate
- * scoring and ranking of the edges it left (surrogate order; one kernel),
- * [2] the gap before the continuation, [3] continuation, [4] irregular-edge
- * walk (ms) */
+ * events on the context stream: h_ms[0] surrogate scoring of every edge's
+ * picks (surrogate order; one kernel), [1] packed head round, [2] the gap
+ * before the continuation, [3] continuation, [4] irregular-edge walk (ms) */
 int mptg_link_phases(mptg_ctx* ctx, const double* d_from, const double* d_to, int64_t n, int order,
                      int endpoints_known_valid, uint8_t* d_verdict, uint32_t* d_checks, float* h_ms);
 /* plain device-to-device copy kernel of `bytes` (multiple of 16, 16-byte

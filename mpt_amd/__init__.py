@@ -331,7 +331,7 @@ class Context:
         ms = (C.c_float * 5)()
         self._chk(_lib.mptg_link_phases(self._h, C.c_void_p(a), C.c_void_p(b), n, order, 1 if known_valid else 0,
                                         C.c_void_p(verdict), C.c_void_p(checks), ms))
-        return dict(zip(("head", "score", "rank", "tail", "list"), (float(x) for x in ms)))
+        return dict(zip(("score", "head", "gap", "tail", "list"), (float(x) for x in ms)))
 
     def diag_copy(self, dst: int, src: int, nbytes: int):
         """Asynchronous plain copy kernel between device buffers (byte-counter reference)."""

@@ -38,39 +38,40 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// The surrogate order of an edge's waypoints (trees of h <= 9 levels):
-// reference ordinals 0 .. kHeadRanks-1 first, then the other waypoints of the
-// top kTopLevels levels (63 spread evenly along the edge) by descending score
-// (ties by reference ordinal), then the deeper waypoints in reference order.
-// The collision region of an edge is usually wide, so the top levels are
-// where a score finds it; ordering the deeper waypoints buys no device time
-// (a colliding edge costs one continuation round whichever rank fails) and
-// would cost up to eight scored tiles per edge instead of one.  The scoring
-// kernel ranks each edge's top tree itself (surrogate_rank_list).
-// (path, depth) of the waypoint at `rank` in the chosen order (tree of h
-// levels): surrogate ranks below the top tree's size come from the ranked
-// list, later ones are the deeper waypoints in reference order.
+// Waypoint at `rank` of the chosen order (tree of h levels): the reference
+// pre-order, or the surrogate order -- the edge's kPicks picks (best-scored
+// nodes of its top tree, from the scoring kernel) at ranks 0..3, then every
+// other waypoint in reference order.  `sorted` holds the picks in ascending
+// ordinal order (ranks >= kPicks only).
 template <int ORD>
-__device__ __forceinline__ void node_at(int h, const uint16_t* __restrict__ ranked, int64_t e, uint32_t rank,
-                                        uint32_t* path, int* depth) {
-    if constexpr (ORD == MPTG_ORDER_SURROGATE) {
-        if (h <= 9) {
-            const uint32_t T = (1u << (h < kTopLevels ? h : kTopLevels)) - 1u;
-            if (rank < T) preorder_path(h, ranked[e * kRankStride + rank], path, depth);
-            else deep_path(h, rank - T, path, depth);
-            return;
-        }
-    }
-    preorder_path(h, rank, path, depth);
+__device__ __forceinline__ uint32_t ordinal_at(uint32_t rank, const uint32_t (&sorted)[kPicks]) {
+    if constexpr (ORD == MPTG_ORDER_SURROGATE) return skip_picks(rank - kPicks, sorted);
+    return rank;
 }
 
-constexpr int kHeadRanks = 4;                     // ranks per edge in the packed round
+// The 4 picks of edge e in ascending ordinal order (wave-uniform loads).
+__device__ __forceinline__ void sorted_picks(const uint16_t* __restrict__ picks, int64_t e, uint32_t (&p)[kPicks]) {
+#pragma unroll
+    for (int i = 0; i < kPicks; ++i) p[i] = picks[e * kPicks + i];
+    // 5-comparator sorting network
+    auto cs = [&](int i, int j) {
+        const uint32_t lo = p[i] < p[j] ? p[i] : p[j], hi = p[i] < p[j] ? p[j] : p[i];
+        p[i] = lo;
+        p[j] = hi;
+    };
+    cs(0, 1);
+    cs(2, 3);
+    cs(0, 2);
+    cs(1, 3);
+    cs(1, 2);
+}
+
+constexpr int kHeadRanks = kPicks;                // ranks per edge in the packed round
 
 // Packed round: 4 ranks per edge, 16 edges per wave (8 when each check takes
-// 2 lanes, head_lanes).  Endpoint checks (when
-// not vouched for), the tree shape, then reference ranks 0..3 (the surrogate
-// order leads with them too, so that no edge this round decides is scored).
-// Edges still undecided go to the continuation list.
+// 2 lanes, head_lanes).  Endpoint checks (when not vouched for), the tree
+// shape, then ranks 0..3 of the chosen order (reference ordinals 0..3, or the
+// surrogate's picks).  Edges still undecided go to the continuation list.
 // Waypoint validity in the link kernels: the wave-box culling (LANE false:
 // all lanes share one disk walk) or each lane's own grid cells (LANE true).
 template <int D, bool COUNT, bool LANE, typename Coord>
@@ -106,7 +107,7 @@ __device__ __forceinline__ uint32_t head_group(uint64_t m, int slot) {
 template <int D, bool COUNT, int ORD, bool LANE>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const double* __restrict__ from, const double* __restrict__ to, int64_t n, int known_valid,
-    const uint8_t* __restrict__ mask, double tol, SceneArgs sc,
+    const uint8_t* __restrict__ mask, double tol, SceneArgs sc, const uint16_t* __restrict__ picks,
     uint8_t* __restrict__ verdict, uint32_t* __restrict__ checks, int64_t* __restrict__ irregular,
     uint32_t* __restrict__ n_irregular, int64_t* __restrict__ cont, uint32_t* __restrict__ n_cont,
     unsigned long long* __restrict__ work) {
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
                 checks[e] = MPTG_CHECKS_OVER_LIMIT;
             }
             live = false;
-        } else if (!regular || (ORD == MPTG_ORDER_SURROGATE && h > 9)) {
+        } else if (!regular || (ORD == MPTG_ORDER_SURROGATE && h > kMaxPickDepth)) {
             if (rk == 0 && first) irregular[atomicAdd(n_irregular, 1u)] = e;  // exact per-edge walk later
             live = false;
         }
@@ -160,7 +161,10 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
     const bool act = live && (uint32_t)rk < W;
     uint32_t path = 0;
     int depth = 0;
-    if (act) preorder_path(h, (uint32_t)rk, &path, &depth);  // reference ranks in both orders
+    if (act) {
+        const uint32_t o = ORD == MPTG_ORDER_SURROGATE ? (uint32_t)picks[e * kPicks + rk] : (uint32_t)rk;
+        preorder_path(h, o, &path, &depth);
+    }
     // every lane calls the check (its wave-wide culling shuffles read every
     // lane), and only lanes with a waypoint count
     const bool okv = check([&](int i) { return act ? coord_mid(a[i], b[i], path, depth) : b[i]; }, act);
@@ -197,9 +201,9 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void arm_link_head(
 // slot ends at exactly the first failing rank of the chosen order.  No wave
 // waits for another: the slot is read and lowered, never awaited.
 template <int D, bool COUNT, int ORD, bool LANE>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(ORD == MPTG_ORDER_REFERENCE ? 5 : 4))) void arm_link_tail(
+__global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(5))) void arm_link_tail(
     const double* __restrict__ from, const double* __restrict__ to, double tol, SceneArgs sc,
-    const uint16_t* __restrict__ ranked, const int64_t* __restrict__ cont, const uint32_t* __restrict__ n_cont,
+    const uint16_t* __restrict__ picks, const int64_t* __restrict__ cont, const uint32_t* __restrict__ n_cont,
     int waves_per_edge, uint32_t* __restrict__ checks, unsigned long long* __restrict__ work) {
     // no block-wide barriers: waves leave independently
     ArmWork wk{0, 0, 0, 0, 0};
@@ -215,6 +219,8 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_
         bool regular = false;
         bisect_shape<D>(a, b, tol, &h, &regular);  // listed edges are regular
         const uint32_t W = (1u << h) - 1u;
+        uint32_t sp[kPicks] = {0, 0, 0, 0};
+        if constexpr (ORD == MPTG_ORDER_SURROGATE) sorted_picks(picks, e, sp);  // W > 4: 4 picks
         for (uint32_t c0 = kHeadRanks + kWave * p; c0 < W; c0 += kWave * P) {
             if (__hip_atomic_load(&checks[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < c0) break;  // uniform
             const uint32_t r = c0 + (uint32_t)lane;
@@ -223,7 +229,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_
             const bool active = r < W;
             uint32_t path;
             int depth;
-            node_at<ORD>(h, ranked, e, active ? r : W - 1u, &path, &depth);
+            preorder_path(h, ordinal_at<ORD>(active ? r : W - 1u, sp), &path, &depth);
             const bool bad = !arm_ok_link<D, COUNT, LANE>(
                 sc, [&](int i) { return coord_mid(a[i], b[i], path, depth); }, active, &wk);
             const uint64_t m = __ballot(bad && active);
@@ -273,27 +279,25 @@ static int launch_arm_phases(mptg_ctx* ctx, const double* from, const double* to
                              int64_t* cont, uint32_t* ncont, unsigned long long* work) {
     const double tol = ctx->link_param;
     const SceneArgs sa = scene_args(ctx);
-    const int64_t per_block = (int64_t)kWavesPerBlock * head_edges<COUNT, LANE>();
-    hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD, LANE>), dim3((unsigned)((n + per_block - 1) / per_block)),
-                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, verdict, checks,
-                       irr, nirr, cont, ncont, work);
-    phase_mark(ctx, 1);
-    const uint16_t* ranked = nullptr;
+    const uint16_t* picks = nullptr;
     if constexpr (ORD == MPTG_ORDER_SURROGATE) {
         int rc;
-        if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kRankStride * n))) return rc;
-        uint16_t* rk = (uint16_t*)ctx->scratch[9].p;
-        if ((rc = surrogate_rank_list(ctx, from, to, n, cont, ncont, (uint32_t)kHeadRanks, rk))) return rc;
-        phase_mark(ctx, 2);
-        ranked = rk;
-    } else {
-        phase_mark(ctx, 2);
+        if ((rc = ensure(ctx, ctx->scratch[9], sizeof(uint16_t) * kPicks * n))) return rc;
+        uint16_t* pk = (uint16_t*)ctx->scratch[9].p;
+        if ((rc = surrogate_picks(ctx, from, to, n, mask, pk))) return rc;
+        picks = pk;
     }
+    phase_mark(ctx, 1);
+    const int64_t per_block = (int64_t)kWavesPerBlock * head_edges<COUNT, LANE>();
+    hipLaunchKernelGGL((arm_link_head<D, COUNT, ORD, LANE>), dim3((unsigned)((n + per_block - 1) / per_block)),
+                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, n, kv, mask, tol, sa, picks, verdict,
+                       checks, irr, nirr, cont, ncont, work);
+    phase_mark(ctx, 2);
     phase_mark(ctx, 3);
     int P = ctx->tune.arm_tail_waves;
     if ((n * P + kWavesPerBlock - 1) / kWavesPerBlock > (int64_t)INT32_MAX) P = 1;  // grid limit
     hipLaunchKernelGGL((arm_link_tail<D, COUNT, ORD, LANE>), dim3((unsigned)((n * P + kWavesPerBlock - 1) / kWavesPerBlock)),
-                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, tol, sa, ranked, cont, ncont, P, checks,
+                       dim3(kWave * kWavesPerBlock), 0, ctx->stream, from, to, tol, sa, picks, cont, ncont, P, checks,
                        work);
     hipLaunchKernelGGL((arm_link_finish<D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, from, to,
                        tol, cont, ncont, verdict, checks);
@@ -322,8 +326,8 @@ static int launch_arm_d(mptg_ctx* ctx, const double* from, const double* to, int
 }
 
 
-// Launches the packed round and the continuation (scoring and ranking the
-// continued edges in between for the surrogate order); edges the kernels
+// Launches the packed round and the continuation (for the surrogate order,
+// the scoring kernel's picks first); edges the kernels
 // list as irregular are left for the caller's exact per-edge kernel
 // (irr/nirr; *nirr is zeroed here).  Returns MPTG_E_STATE when no kernel
 // exists for this arm.

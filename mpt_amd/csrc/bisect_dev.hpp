@@ -43,11 +43,17 @@ __host__ __device__ __forceinline__ uint32_t path_preorder(int h, uint32_t path,
     return o;
 }
 
-// Levels of the bisection tree the surrogate order ranks by score: the
-// 63 waypoints of depths 0..5 spread evenly along the edge.
-constexpr int kTopLevels = 6;
-// Ranked ordinals kept per edge (the top tree, one per lane of a wave).
-constexpr int kRankStride = 64;
+// The surrogate order of an edge's waypoints: the kPicks waypoints with the
+// highest scores among the top tree -- the 2^kPickLevels - 1 nodes of depths
+// 0 .. kPickLevels-1, spread evenly along the edge (every node when the tree
+// is smaller) -- ties by reference ordinal, then every other waypoint in
+// reference order.
+constexpr int kPickLevels = 4;
+constexpr int kPicks = 4;
+// Surrogate-ordered edges have trees of at most this many levels (at most
+// kScoreStride waypoints); the link kernels walk deeper ones in reference
+// order.
+constexpr int kMaxPickDepth = 9;
 
 // Node i of the top tree in breadth-first order (root 0, then each level
 // left to right) -> (path, depth); path bit k is step k (1 = right).
@@ -60,20 +66,13 @@ __host__ __device__ __forceinline__ void bfs_path(uint32_t i, uint32_t* path, in
     *depth = d;
 }
 
-// The k-th node of depth >= kTopLevels in pre-order (tree of h > kTopLevels
-// levels): pre-order visits the 2^kTopLevels subtrees rooted at depth
-// kTopLevels left to right, each one contiguously, so node k is node
-// k mod S of subtree k / S (S = 2^(h - kTopLevels) - 1 nodes each).
-__host__ __device__ __forceinline__ void deep_path(int h, uint32_t k, uint32_t* path, int* depth) {
-    const uint32_t S = (1u << (h - kTopLevels)) - 1u;
-    const uint32_t j = k / S;
-    uint32_t lp;
-    int ld;
-    preorder_path(h - kTopLevels, k % S, &lp, &ld);
-    uint32_t p = 0;
-    for (int t = 0; t < kTopLevels; ++t) p |= ((j >> (kTopLevels - 1 - t)) & 1u) << t;
-    *path = p | (lp << kTopLevels);
-    *depth = kTopLevels + ld;
+// The k-th reference ordinal that is not one of the picks p[0..3] (sorted
+// ascending): each pick at or below the running ordinal moves it up by one.
+__host__ __device__ __forceinline__ uint32_t skip_picks(uint32_t k, const uint32_t (&p)[kPicks]) {
+    uint32_t o = k;
+#pragma unroll
+    for (int i = 0; i < kPicks; ++i) o += p[i] <= o ? 1u : 0u;
+    return o;
 }
 
 // One coordinate of the midpoint of the node at `path`: the reference's
@@ -126,40 +125,6 @@ __device__ __forceinline__ uint64_t score_key(float s, uint32_t ordinal) {
         hi = ~mono;
     }
     return ((uint64_t)hi << 32) | ordinal;
-}
-
-// Ascending bitonic sort of the wave's 64 * NE keys; element i = e * 64 +
-// lane lives in key[e] of that lane, so after the sort rank r = c * 64 + lane
-// is key[c] of that lane.  Partners within a lane are registers, partners in
-// other lanes come by __shfl_xor.
-template <int NE>
-__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&key)[NE], int lane) {
-    constexpr int N = 64 * NE;
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            uint64_t nk[NE];
-#pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                const int i = e * 64 + lane;
-                uint64_t other;
-                if (j >= 64) {
-                    other = key[e ^ (j >> 6)];
-                } else {
-                    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)key[e], j);
-                    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(key[e] >> 32), j);
-                    other = ((uint64_t)hi << 32) | lo;
-                }
-                const bool lower = (i & j) == 0, asc = (i & k) == 0;
-                const uint64_t mn = key[e] < other ? key[e] : other;
-                const uint64_t mx = key[e] < other ? other : key[e];
-                nk[e] = (lower == asc) ? mn : mx;
-            }
-#pragma unroll
-            for (int e = 0; e < NE; ++e) key[e] = nk[e];
-        }
-    }
 }
 
 

@@ -267,8 +267,8 @@ int launch_mesh_link(mptg_ctx* ctx, const uint8_t* d_table, const double* d_from
 // planners: states[b] with mask[b] set that collide get status[b] = invalid_code, mask[b] = 0
 int launch_mesh_filter(mptg_ctx* ctx, const double* d_states, int64_t n, uint8_t* d_mask, uint8_t* d_status,
                        int invalid_code);
-// Arm link kernels in `order`; with the surrogate order the edges left after
-// the packed round (reference ranks 0..3) are scored and ranked first.
+// Arm link kernels in `order`; with the surrogate order every edge's picks
+// are scored first, and the packed round checks them.
 int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t n, int known_valid,
                     const uint8_t* mask, uint8_t* verdict, uint32_t* checks, int64_t* irr, uint32_t* nirr,
                     unsigned long long* work, int order);
@@ -276,11 +276,10 @@ int launch_arm_link(mptg_ctx* ctx, const double* from, const double* to, int64_t
 // scores[e * 512 + reference ordinal]
 int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
                           float* scores);
-// Surrogate order of the edges list[0 .. *n_list) (device count; at most n
-// of them): scores the waypoints of each edge's top bisection levels and
-// ranks them in the same kernel, writing the reference ordinal of ranks
-// 0 .. 62 to ranked[e * 64 + rank] (ordinals below `lead` first).
-int surrogate_rank_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const int64_t* list,
-                        const uint32_t* n_list, uint32_t lead, uint16_t* ranked);
+// The surrogate's picks of every (masked) regular arm edge of at most 9
+// levels: the reference ordinals of the 4 best-scored nodes of its top
+// tree, best first, at picks[e * 4 + r].
+int surrogate_picks(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                    uint16_t* picks);
 
 }  // namespace mptg

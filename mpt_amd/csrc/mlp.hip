@@ -311,26 +311,41 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
 }
 
-// buf[64 waypoints][K] (BF16, row stride kLdsRow) -> buf[64][128] = relu(X W^T + b)
-template <int K>
+// Orders a wave's LDS accesses: a block barrier in one-wave blocks, a wave
+// barrier with a workgroup fence where waves of a block run apart.
+template <bool WAVE>
+__device__ __forceinline__ void buf_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
+// buf[16 NST waypoints][K] (BF16, row stride kLdsRow) -> buf[16 NST][128] =
+// relu(X W^T + b); W^T rows WS elements apart (K in global memory, padded in
+// LDS).  A row's result does not depend on NST or on the other rows.
+template <int K, int WS = K, bool WAVE = false, int NST = 4>
 __device__ __forceinline__ void wave_layer_t(uint16_t* __restrict__ buf, const uint16_t* __restrict__ wt,
                                              const float* __restrict__ bias, int lane) {
     constexpr int KS = K / 32;
     const int r = lane & 15, kq = lane >> 4;
-    bf16x8 xf[4][KS];
+    bf16x8 xf[NST][KS];
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+    for (int st = 0; st < NST; ++st)
 #pragma unroll
         for (int kk = 0; kk < KS; ++kk) xf[st][kk] = load8(buf + (st * 16 + r) * kLdsRow + kk * 32 + 8 * kq);
-    __syncthreads();  // every lane holds its fragments before the buffer is overwritten
+    buf_sync<WAVE>();  // every lane holds its fragments before the buffer is overwritten
 #pragma unroll 2
     for (int ot = 0; ot < HID / 16; ++ot) {
         bf16x8 wf[KS];
 #pragma unroll
-        for (int kk = 0; kk < KS; ++kk) wf[kk] = load8(wt + (ot * 16 + r) * K + kk * 32 + 8 * kq);
+        for (int kk = 0; kk < KS; ++kk) wf[kk] = load8(wt + (ot * 16 + r) * WS + kk * 32 + 8 * kq);
         const float4 bb = *reinterpret_cast<const float4*>(bias + ot * 16 + 4 * kq);
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
+        for (int st = 0; st < NST; ++st) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kk = 0; kk < KS; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[kk], xf[st][kk], acc, 0, 0, 0);
@@ -342,33 +357,36 @@ __device__ __forceinline__ void wave_layer_t(uint16_t* __restrict__ buf, const u
             *reinterpret_cast<uint2*>(buf + (st * 16 + r) * kLdsRow + ot * 16 + 4 * kq) = pk;
         }
     }
-    __syncthreads();
+    buf_sync<WAVE>();
 }
 
 // Last hidden layer and the output unit: logit of waypoint st * 16 + (lane & 15)
 // in out[st] (all lanes), H3 rounded to BF16 as the training forward does.
+template <int WS = HID, int NST = 4>
 __device__ __forceinline__ void wave_layer_logit(const uint16_t* __restrict__ buf, const uint16_t* __restrict__ wt,
                                                  const float* __restrict__ bias, const uint16_t* __restrict__ w4,
-                                                 int lane, float out[4]) {
+                                                 int lane, float out[NST]) {
     constexpr int KS = HID / 32;
     const int r = lane & 15, kq = lane >> 4;
-    bf16x8 xf[4][KS];
+    bf16x8 xf[NST][KS];
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+    for (int st = 0; st < NST; ++st)
 #pragma unroll
         for (int kk = 0; kk < KS; ++kk) xf[st][kk] = load8(buf + (st * 16 + r) * kLdsRow + kk * 32 + 8 * kq);
-    float part[4] = {0.f, 0.f, 0.f, 0.f};
+    float part[NST];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) part[st] = 0.f;
 #pragma unroll 2
     for (int ot = 0; ot < HID / 16; ++ot) {
         bf16x8 wf[KS];
 #pragma unroll
-        for (int kk = 0; kk < KS; ++kk) wf[kk] = load8(wt + (ot * 16 + r) * HID + kk * 32 + 8 * kq);
+        for (int kk = 0; kk < KS; ++kk) wf[kk] = load8(wt + (ot * 16 + r) * WS + kk * 32 + 8 * kq);
         const float4 bb = *reinterpret_cast<const float4*>(bias + ot * 16 + 4 * kq);
         const uint2 wq = *reinterpret_cast<const uint2*>(w4 + ot * 16 + 4 * kq);
         const float w0 = bf16_to_float((uint16_t)(wq.x & 0xffffu)), w1 = bf16_to_float((uint16_t)(wq.x >> 16));
         const float w2 = bf16_to_float((uint16_t)(wq.y & 0xffffu)), w3 = bf16_to_float((uint16_t)(wq.y >> 16));
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
+        for (int st = 0; st < NST; ++st) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kk = 0; kk < KS; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[kk], xf[st][kk], acc, 0, 0, 0);
@@ -378,7 +396,7 @@ __device__ __forceinline__ void wave_layer_logit(const uint16_t* __restrict__ bu
         }
     }
 #pragma unroll
-    for (int st = 0; st < 4; ++st) {
+    for (int st = 0; st < NST; ++st) {
         float v = part[st];
         v += __shfl_xor(v, 16);
         v += __shfl_xor(v, 32);
@@ -413,69 +431,29 @@ __device__ __forceinline__ void arm_features_f(const float* q, const double* len
     for (int j = 40 + 2 * (D < 12 ? D : 12); j < 64; ++j) out[j] = 0;
 }
 
-// Edges: e = item (all n, or those with mask[e] set), or e = list[item] for
-// item < *n_list when a list is given.  Without `ranked`: the scores of every
-// waypoint (tile = 64 reference ordinals from c0) at scores[e * 512 +
-// ordinal].  With `ranked` (one tile per edge): row i is node i of the top
-// tree (breadth-first), and the wave sorts its rows' keys -- reference
-// ordinals below `lead` first, then descending score, ties by ordinal -- and
-// writes the ordinal of each rank to ranked[e * kRankStride + rank].
+// FP32 features of the waypoint at (path, depth) of link(a, b): the dyadic
+// position t of the node's midpoint along the edge, the FP32 interpolation
+// a + t (b - a), then arm_features_f.  Every scoring kernel computes a
+// waypoint's features here, so a waypoint gets the same score in each.
 template <int D>
-__global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__ from, const double* __restrict__ to,
-                                                      int64_t n, const uint8_t* __restrict__ mask,
-                                                      const int64_t* __restrict__ list,
-                                                      const uint32_t* __restrict__ n_list, double tol,
-                                                      const double* __restrict__ lengths, float inv_total,
-                                                      const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
-                                                      const float* __restrict__ p, float* __restrict__ scores,
-                                                      int tiles, uint16_t* __restrict__ ranked, uint32_t lead) {
-    const bool top = ranked != nullptr;
-    __shared__ __attribute__((aligned(16))) uint16_t buf[64 * kLdsRow];
-    const int lane = threadIdx.x;
-    const int64_t item = blockIdx.x / tiles;
-    const uint32_t c0 = (uint32_t)(blockIdx.x % tiles) * 64;
-    int64_t e = item;
-    if (list) {
-        if (item >= (int64_t)*n_list) return;
-        e = list[item];
-    } else if (e >= n || (mask && !mask[e])) {
-        return;
-    }
-    const double* a = from + e * D;
-    const double* b = to + e * D;
-    int h = 0;
-    bool regular = false;
-    if (!bisect_shape<D>(a, b, tol, &h, &regular)) return;
-    uint32_t W = (1u << h) - 1u;
-    if (c0 >= W || W > (uint32_t)kScoreStride) return;
-    // top: row i is node i of the top tree (breadth-first), else reference
-    // ordinal c0 + i
-    if (top) W = (1u << (h < kTopLevels ? h : kTopLevels)) - 1u;
-    // features of this lane's waypoint (rows beyond W are zero)
-    const uint32_t o = c0 + (uint32_t)lane;
-    uint16_t f[IN];
-    if (o < W) {
-        uint32_t path;
-        int depth;
-        if (top) bfs_path(o, &path, &depth);
-        else preorder_path(h, o, &path, &depth);
-        // dyadic position of the node's midpoint along the edge
-        float t = 0.f, step = 0.5f;
+__device__ __forceinline__ void node_features(const double* a, const double* b, uint32_t path, int depth,
+                                              const double* lengths, float inv_total, uint16_t* f) {
+    float t = 0.f, step = 0.5f;
 #pragma unroll 1
-        for (int k = 0; k < depth; ++k, step *= 0.5f)
-            if ((path >> k) & 1u) t += step;
-        t += step;
-        float q[D];
+    for (int k = 0; k < depth; ++k, step *= 0.5f)
+        if ((path >> k) & 1u) t += step;
+    t += step;
+    float q[D];
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-            const float ai = (float)a[i], bi = (float)b[i];
-            q[i] = ai + t * (bi - ai);
-        }
-        arm_features_f<D>(q, lengths, inv_total, f);
-    } else {
-#pragma unroll
-        for (int j = 0; j < IN; ++j) f[j] = 0;
+    for (int i = 0; i < D; ++i) {
+        const float ai = (float)a[i], bi = (float)b[i];
+        q[i] = ai + t * (bi - ai);
     }
+    arm_features_f<D>(q, lengths, inv_total, f);
+}
+
+// Lane feature row -> the wave's LDS activation buffer (row `lane`).
+__device__ __forceinline__ void store_row(uint16_t* buf, int lane, const uint16_t (&f)[IN]) {
 #pragma unroll
     for (int j = 0; j < IN; j += 8) {
         uint4 v;
@@ -485,36 +463,204 @@ __global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__
         v.w = (uint32_t)f[j + 6] | ((uint32_t)f[j + 7] << 16);
         *reinterpret_cast<uint4*>(buf + lane * kLdsRow + j) = v;
     }
+}
+
+// Scores of every waypoint of edges e < n with mask[e] set (or all): one
+// wave per tile of 64 reference ordinals from c0, written to
+// scores[e * kScoreStride + ordinal].
+template <int D>
+__global__ __launch_bounds__(64) void score_waypoints(const double* __restrict__ from, const double* __restrict__ to,
+                                                      int64_t n, const uint8_t* __restrict__ mask, double tol,
+                                                      const double* __restrict__ lengths, float inv_total,
+                                                      const uint16_t* __restrict__ wt, const uint16_t* __restrict__ wb,
+                                                      const float* __restrict__ p, float* __restrict__ scores,
+                                                      int tiles) {
+    __shared__ __attribute__((aligned(16))) uint16_t buf[64 * kLdsRow];
+    const int lane = threadIdx.x;
+    const int64_t e = blockIdx.x / tiles;
+    const uint32_t c0 = (uint32_t)(blockIdx.x % tiles) * 64;
+    if (e >= n || (mask && !mask[e])) return;
+    const double* a = from + e * D;
+    const double* b = to + e * D;
+    int h = 0;
+    bool regular = false;
+    if (!bisect_shape<D>(a, b, tol, &h, &regular)) return;
+    const uint32_t W = (1u << h) - 1u;
+    if (c0 >= W || W > (uint32_t)kScoreStride) return;
+    const uint32_t o = c0 + (uint32_t)lane;
+    uint16_t f[IN];
+    if (o < W) {
+        uint32_t path;
+        int depth;
+        preorder_path(h, o, &path, &depth);
+        node_features<D>(a, b, path, depth, lengths, inv_total, f);
+    } else {
+#pragma unroll
+        for (int j = 0; j < IN; ++j) f[j] = 0;
+    }
+    store_row(buf, lane, f);
     __syncthreads();
     wave_layer_t<IN>(buf, wt, p + OFF_B1, lane);
     wave_layer_t<HID>(buf, wt + IN * HID, p + OFF_B2, lane);
     float logit[4];
     wave_layer_logit(buf, wt + IN * HID + HID * HID, p + OFF_B3, wb + IN * HID + 2 * HID * HID, lane, logit);
     const float b4 = p[OFF_B4];
-    if (top) {
-        // every lane holds the logits of waypoints st * 16 + (lane & 15):
-        // lane i takes its own row, i = (i >> 4) * 16 + (i & 15)
-        const int q = lane >> 4;
-        const float s = (q == 0 ? logit[0] : q == 1 ? logit[1] : q == 2 ? logit[2] : logit[3]) + b4;
-        uint64_t key[1] = {~0ull};
-        if (o < W) {
-            uint32_t path;
-            int depth;
-            bfs_path(o, &path, &depth);
-            const uint32_t ord = path_preorder(h, path, depth);
-            // led ordinals sort first: every score key has a nonzero high word
-            key[0] = ord < lead ? (uint64_t)ord : score_key(s, ord);
-        }
-        wave_bitonic_sort<1>(key, lane);
-        if (o < W) ranked[e * kRankStride + lane] = (uint16_t)key[0];
-        return;
-    }
     // lanes 0..15 write waypoints st * 16 + lane
     if (lane < 16) {
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
             const uint32_t oo = c0 + (uint32_t)(st * 16 + lane);
             if (oo < W) scores[e * kScoreStride + oo] = logit[st] + b4;
+        }
+    }
+}
+
+// The surrogate's picks of edges e < n with mask[e] set (or all) whose tree
+// is regular with at most kMaxPickDepth levels: the reference ordinals of
+// each edge's kPicks best-scored top-tree nodes -- descending score, ties by
+// ordinal -- at picks[e * kPicks + r], in that order (fewer on trees of fewer
+// nodes).
+//
+// Persistent blocks of kPickWaves waves (two per SIMD) with the weights
+// staged once into LDS (W^T rows padded by 8 elements against bank
+// conflicts).  Block b takes the edge slots [b S, (b + 1) S), kPickChunk at a
+// time: the block lists the chunk's live edges in LDS, then its waves take
+// tiles of kPickEdges listed edges in turn.  A tile has 16 rows per edge
+// (edge slot j = lane / 16, node lane % 16 of the top tree in breadth-first
+// order, 15 at most).  A row's arithmetic is that of score_waypoints, so a
+// waypoint gets the same score.
+constexpr int kPickWaves = 8, kPickEdges = 2, kPickChunk = 64 * kPickWaves;
+constexpr int kW1S = IN + 8, kWS = HID + 8;
+struct PickLds {
+    uint16_t w1[HID * kW1S], w2[HID * kWS], w3[HID * kWS];
+    float b1[HID], b2[HID], b3[HID];
+    uint16_t w4[HID];
+    uint16_t act[kPickWaves][16 * kPickEdges * kLdsRow];
+    uint32_t list[kPickChunk];  // (slot offset in the chunk << 6) | h
+    uint32_t wsum[kPickWaves];
+};
+
+template <int D>
+__global__ __launch_bounds__(64 * kPickWaves) void score_picks(const double* __restrict__ from,
+                                                               const double* __restrict__ to, int64_t n,
+                                                               const uint8_t* __restrict__ mask, double tol,
+                                                               int64_t per_block, const double* __restrict__ lengths,
+                                                               float inv_total, const uint16_t* __restrict__ wt,
+                                                               const uint16_t* __restrict__ wb,
+                                                               const float* __restrict__ p,
+                                                               uint16_t* __restrict__ picks) {
+    __shared__ __attribute__((aligned(16))) PickLds L;
+    const int t = threadIdx.x;
+    // stage W1^T [128][64], W2^T, W3^T [128][128] in 16-byte chunks: every
+    // load of a thread issued before its stores
+    {
+        constexpr int T = 64 * kPickWaves, N1 = HID * IN / 8 / T, N2 = HID * HID / 8 / T;
+        uint4 v1[N1], v2[N2], v3[N2];
+#pragma unroll
+        for (int k = 0; k < N1; ++k) v1[k] = reinterpret_cast<const uint4*>(wt)[t + k * T];
+#pragma unroll
+        for (int k = 0; k < N2; ++k) {
+            v2[k] = reinterpret_cast<const uint4*>(wt + IN * HID)[t + k * T];
+            v3[k] = reinterpret_cast<const uint4*>(wt + IN * HID + HID * HID)[t + k * T];
+        }
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            const int i = t + k * T, row = i / (IN / 8), c = i % (IN / 8);
+            *reinterpret_cast<uint4*>(L.w1 + row * kW1S + c * 8) = v1[k];
+        }
+#pragma unroll
+        for (int k = 0; k < N2; ++k) {
+            const int i = t + k * T, row = i / (HID / 8), c = i % (HID / 8);
+            *reinterpret_cast<uint4*>(L.w2 + row * kWS + c * 8) = v2[k];
+            *reinterpret_cast<uint4*>(L.w3 + row * kWS + c * 8) = v3[k];
+        }
+    }
+    for (int i = t; i < HID; i += 64 * kPickWaves) {
+        L.b1[i] = p[OFF_B1 + i];
+        L.b2[i] = p[OFF_B2 + i];
+        L.b3[i] = p[OFF_B3 + i];
+        L.w4[i] = wb[IN * HID + 2 * HID * HID + i];
+    }
+    const float b4 = p[OFF_B4];
+    const int wave = t >> 6, lane = t & 63;
+    const int slot = lane >> 4, node = lane & 15;
+    uint16_t* buf = L.act[wave];
+    const int64_t s0 = (int64_t)blockIdx.x * per_block;
+    const int64_t s1 = s0 + per_block < n ? s0 + per_block : n;
+    // block-uniform chunk loop: every wave reaches each barrier
+    for (int64_t c0 = s0; c0 < s1; c0 += kPickChunk) {
+        // the chunk's live edges, listed in slot order
+        const int64_t ei = c0 + t;
+        bool live = ei < s1 && (!mask || mask[ei]);
+        int hh = 0;
+        if (live) {
+            bool regular = false;
+            live = bisect_shape<D>(from + ei * D, to + ei * D, tol, &hh, &regular) && regular && hh <= kMaxPickDepth;
+        }
+        const uint64_t lm = __ballot(live);
+        __syncthreads();  // the previous chunk's tiles are done with the list (and, once, the weights are staged)
+        if (lane == 0) L.wsum[wave] = (uint32_t)__popcll(lm);
+        __syncthreads();
+        uint32_t before = 0, cnt = 0;
+#pragma unroll
+        for (int w = 0; w < kPickWaves; ++w) {
+            before += w < wave ? L.wsum[w] : 0u;
+            cnt += L.wsum[w];
+        }
+        if (live) L.list[before + __popcll(lm & (lane ? (~0ull >> (64 - lane)) : 0ull))] = ((uint32_t)t << 6) | (uint32_t)hh;
+        __syncthreads();
+        const int tiles = (int)((cnt + kPickEdges - 1) / kPickEdges);
+        for (int tile = wave; tile < tiles; tile += kPickWaves) {
+            // lanes of slots >= kPickEdges hold no row (they load and
+            // multiply fragments like the others)
+            const int j = tile * kPickEdges + slot;
+            const bool have = slot < kPickEdges && (uint32_t)j < cnt;
+            const uint32_t ent = have ? L.list[j] : 0u;
+            const int64_t e = c0 + (int64_t)(ent >> 6);
+            const int h = (int)(ent & 63u);
+            const uint32_t T = have ? (1u << (h < kPickLevels ? h : kPickLevels)) - 1u : 0u;
+            const bool row = (uint32_t)node < T;
+            uint16_t f[IN];
+            uint32_t ord = 0;
+            if (row) {
+                uint32_t path;
+                int depth;
+                bfs_path((uint32_t)node, &path, &depth);
+                ord = path_preorder(h, path, depth);
+                node_features<D>(from + e * D, to + e * D, path, depth, lengths, inv_total, f);
+            } else {
+#pragma unroll
+                for (int q = 0; q < IN; ++q) f[q] = 0;
+            }
+            if (slot < kPickEdges) store_row(buf, lane, f);
+            buf_sync<true>();
+            wave_layer_t<IN, kW1S, true, kPickEdges>(buf, L.w1, L.b1, lane);
+            wave_layer_t<HID, kWS, true, kPickEdges>(buf, L.w2, L.b2, lane);
+            float logit[kPickEdges];
+            wave_layer_logit<kWS, kPickEdges>(buf, L.w3, L.b3, L.w4, lane, logit);
+            // every lane holds the logits of rows st * 16 + (lane & 15): its
+            // own row is st = slot
+            const float sc = (slot == 0 ? logit[0] : logit[kPickEdges - 1]) + b4;
+            uint64_t key = row ? score_key(sc, ord) : ~0ull;
+            // kPicks rounds of a minimum over the edge's 16 lanes; the lane
+            // holding it (keys are distinct: ordinals differ) writes its
+            // ordinal and drops out
+#pragma unroll
+            for (int r = 0; r < kPicks; ++r) {
+                uint64_t m = key;
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) {
+                    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)m, o);
+                    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(m >> 32), o);
+                    const uint64_t x = ((uint64_t)hi << 32) | lo;
+                    m = x < m ? x : m;
+                }
+                if (m != ~0ull && key == m) {
+                    picks[e * kPicks + r] = (uint16_t)ord;
+                    key = ~0ull;
+                }
+            }
+            buf_sync<true>();  // the next tile's rows overwrite the buffer
         }
     }
 }
@@ -674,41 +820,62 @@ static int sur_check(mptg_ctx* ctx, bool need_init) {
     return MPTG_OK;
 }
 
-// One launch of the fused scoring kernel (ranked: the top-tree ranking mode).
-static int score_launch(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
-                        const int64_t* list, const uint32_t* n_list, float* scores, uint16_t* ranked,
-                        uint32_t lead) {
+static int score_check(mptg_ctx* ctx) {
     Surrogate& S = surrogate(ctx);
     if (!S.init) return fail(ctx, MPTG_E_STATE, "surrogate order needs an initialised surrogate");
     if (ctx->scene.kind != MPTG_SCENE_ARM || ctx->dim != ctx->scene.nlinks)
         return fail(ctx, MPTG_E_STATE, "surrogate order is defined for the arm scene");
-    const float inv_total = (float)(1.0 / ctx->scene.total_length);
-    // 64-waypoint tiles per edge (the top tree is one)
-    const int tiles = ranked ? 1 : kScoreStride / 64;
+    if (ctx->dim != 5 && ctx->dim != 10 && ctx->dim != 20)
+        return fail(ctx, MPTG_E_STATE, "surrogate order: arms of 5, 10 or 20 links");
+    return MPTG_OK;
+}
+
+#define MPTG_SCORE_ARGS()                                                                                        \
+    const Surrogate& S = surrogate(ctx);                                                                         \
+    const float inv_total = (float)(1.0 / ctx->scene.total_length);                                              \
+    const uint16_t* wt = (const uint16_t*)S.wtb.p;                                                               \
+    const uint16_t* wb = (const uint16_t*)S.wb.p;                                                                \
+    const float* p = (const float*)S.params.p;                                                                   \
+    const double* L = (const double*)ctx->scene.lengths
+
+int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                          float* scores) {
+    int rc;
+    if ((rc = score_check(ctx))) return rc;
+    if (n <= 0) return MPTG_OK;
+    MPTG_SCORE_ARGS();
+    const int tiles = kScoreStride / 64;  // 64-waypoint tiles per edge
     const unsigned blocks = (unsigned)(n * tiles);
-    const uint16_t* wt = (const uint16_t*)S.wtb.p;
-    const uint16_t* wb = (const uint16_t*)S.wb.p;
-    const float* p = (const float*)S.params.p;
-    const double* L = (const double*)ctx->scene.lengths;
     switch (ctx->dim) {
-        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, ranked, lead); break;
-        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, ranked, lead); break;
-        case 20: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, list, n_list, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles, ranked, lead); break;
-        default: return fail(ctx, MPTG_E_STATE, "surrogate order: arms of 5, 10 or 20 links");
+        case 5: hipLaunchKernelGGL(score_waypoints<5>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles); break;
+        case 10: hipLaunchKernelGGL(score_waypoints<10>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles); break;
+        default: hipLaunchKernelGGL(score_waypoints<20>, dim3(blocks), dim3(64), 0, ctx->stream, from, to, n, mask, ctx->link_param, L, inv_total, wt, wb, p, scores, tiles); break;
     }
     MPTG_HIP(ctx, hipGetLastError());
     return MPTG_OK;
 }
 
-int surrogate_score_edges(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
-                          float* scores) {
-    return score_launch(ctx, from, to, n, mask, nullptr, nullptr, scores, nullptr, 0);
+int surrogate_picks(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const uint8_t* mask,
+                    uint16_t* picks) {
+    int rc;
+    if ((rc = score_check(ctx))) return rc;
+    if (n <= 0) return MPTG_OK;
+    MPTG_SCORE_ARGS();
+    // one block per CU at most (the LDS holds one), a few tiles per wave at
+    // the least
+    const int64_t blocks = std::max<int64_t>(
+        1, std::min<int64_t>(ctx->cus, (n + 2 * kPickEdges * kPickWaves - 1) / (2 * kPickEdges * kPickWaves)));
+    const int64_t per_block = (n + blocks - 1) / blocks;
+    const dim3 g((unsigned)blocks), blk(64 * kPickWaves);
+    switch (ctx->dim) {
+        case 5: hipLaunchKernelGGL(score_picks<5>, g, blk, 0, ctx->stream, from, to, n, mask, ctx->link_param, per_block, L, inv_total, wt, wb, p, picks); break;
+        case 10: hipLaunchKernelGGL(score_picks<10>, g, blk, 0, ctx->stream, from, to, n, mask, ctx->link_param, per_block, L, inv_total, wt, wb, p, picks); break;
+        default: hipLaunchKernelGGL(score_picks<20>, g, blk, 0, ctx->stream, from, to, n, mask, ctx->link_param, per_block, L, inv_total, wt, wb, p, picks); break;
+    }
+    MPTG_HIP(ctx, hipGetLastError());
+    return MPTG_OK;
 }
-
-int surrogate_rank_list(mptg_ctx* ctx, const double* from, const double* to, int64_t n, const int64_t* list,
-                        const uint32_t* n_list, uint32_t lead, uint16_t* ranked) {
-    return score_launch(ctx, from, to, n, nullptr, list, n_list, nullptr, ranked, lead);
-}
+#undef MPTG_SCORE_ARGS
 
 }  // namespace mptg
 

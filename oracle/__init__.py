@@ -82,7 +82,7 @@ _SIG = [
     ("orc_pool_run_tree", C.c_int32, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, _i32p, C.c_int64,
                                       C.c_int64, C.c_double, C.c_int32, C.POINTER(OrcPoolResult)]),
     ("orc_checks_in_score_order", C.c_uint32, [C.POINTER(OrcScene), _dp, _dp, C.c_int32,
-                                               C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32]),
+                                               C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
     ("orc_prrt_run", C.c_int64, [C.POINTER(OrcScene), C.POINTER(OrcPrrtParams), _dp, C.c_int64, C.c_int64, _dp,
                                  _i32p, C.c_int64, _i32p, C.c_int64, C.POINTER(OrcPrrtStats)]),
     ("orc_star_k", C.c_int32, [C.c_int32, C.c_double, C.c_int32, C.c_int64]),
@@ -235,16 +235,18 @@ class Oracle:
             raise ValueError("arm link work needs an arm scene with bisection links")
         return {"calls": int(out[0]), "fk": int(out[1]), "tests": int(out[2]), "near_tests": int(out[3])}
 
-    def checks_in_score_order(self, a, b, scores, known_valid=False, lead=0, top_levels=None) -> int:
+    def checks_in_score_order(self, a, b, scores, known_valid=False, lead=0, top_levels=None, picks=None) -> int:
         """valid() calls of link(a, b) with reference ordinals 0..lead-1 first,
         then the other waypoints of bisection depth < top_levels by descending
         score, then the deeper ones in reference order (top_levels None: all
-        the rest by score)."""
+        the rest by score).  picks: only that many of the scored waypoints
+        lead, and every other waypoint follows in reference order."""
         a, b = _f64(a), _f64(b)
         sc = np.ascontiguousarray(scores, dtype=np.float32)
         return int(self.L.orc_checks_in_score_order(C.byref(self.s), _p(a, C.c_double), _p(b, C.c_double),
                                                     1 if known_valid else 0, _p(sc, C.c_float), sc.shape[0],
-                                                    lead, -1 if top_levels is None else top_levels))
+                                                    lead, -1 if top_levels is None else top_levels,
+                                                    -1 if picks is None else picks))
 
     def _params(self, seed, batch, range_=None, goal_bias=None, samples=None, samples_ok=None):
         """samples / samples_ok: a custom sampler's table (row i = sample

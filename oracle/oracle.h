@@ -108,11 +108,13 @@ int32_t orc_arm_link_work(const orc_scene* sc, const double* from, const double*
  * order, after the same endpoint checks: reference ordinals 0 .. lead-1 first,
  * then the other waypoints of bisection depth < top_levels by descending
  * score (ties: ascending reference ordinal), then the deeper ones in reference
- * order (top_levels < 0: every waypoint after the lead by score).  scores[] is
+ * order (top_levels < 0: every waypoint after the lead by score).  With
+ * picks >= 0 only the first `picks` of the scored waypoints come before the
+ * rest, and every other waypoint follows in reference order.  scores[] is
  * indexed by reference ordinal (pre-order of the bisection tree). */
 uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const double* b,
                                    int32_t known_valid, const float* scores, int64_t nscores, int32_t lead,
-                                   int32_t top_levels);
+                                   int32_t top_levels, int32_t picks);
 
 /* --- batch-synchronous PRRT ------------------------------------------------ */
 typedef struct {

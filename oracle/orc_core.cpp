@@ -518,7 +518,7 @@ int32_t orc_arm_link_work(const orc_scene* sc, const double* from, const double*
 
 uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const double* b,
                                    int32_t known_valid, const float* scores, int64_t nscores, int32_t lead,
-                                   int32_t top_levels) {
+                                   int32_t top_levels, int32_t picks) {
     int d = sc->dim;
     uint32_t checks = 0;
     int nend = sc->link_mode == ORC_LINK_BISECT ? 2 : 1;
@@ -562,6 +562,13 @@ uint32_t orc_checks_in_score_order(const orc_scene* sc, const double* a, const d
     for (int64_t i = l; i < n; ++i) (top_levels < 0 || depth[(size_t)i] < top_levels ? order : deep).push_back(i);
     std::stable_sort(order.begin() + (ptrdiff_t)top0, order.end(),
                      [&](int64_t x, int64_t y) { return scores[x] > scores[y]; });
+    if (picks >= 0 && (size_t)picks < order.size() - top0) {
+        // only the first `picks` of the scored waypoints lead; every other
+        // waypoint follows in reference order
+        deep.insert(deep.end(), order.begin() + (ptrdiff_t)(top0 + (size_t)picks), order.end());
+        order.resize(top0 + (size_t)picks);
+        std::sort(deep.begin(), deep.end());
+    }
     order.insert(order.end(), deep.begin(), deep.end());
     for (int64_t o : order) {
         ++checks;

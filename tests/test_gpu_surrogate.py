@@ -192,13 +192,13 @@ def test_surrogate_order_parity_and_savings(trained):
     ctx.surrogate_score_edges(da.ptr, db.ptr, m, scores.ptr)
     sc_np = scores.numpy()
     o = oracle.Oracle(sc)
-    # the surrogate order: reference ordinals 0..3 (the packed round), then
-    # the other waypoints of the top 6 bisection levels by descending score,
-    # then the deeper ones in reference order
-    for e in range(0, m, 4):
+    # the surrogate order: the 4 best-scored waypoints of the top 4
+    # bisection levels (the packed round), then every other waypoint in
+    # reference order
+    for e in range(0, m, 2):
         W = len(o.waypoints(a[e], b[e]))
-        assert c_sur[e] == o.checks_in_score_order(a[e], b[e], sc_np[e, :W], known_valid=True, lead=4,
-                                                   top_levels=6), e
+        assert c_sur[e] == o.checks_in_score_order(a[e], b[e], sc_np[e, :W], known_valid=True, lead=0,
+                                                   top_levels=4, picks=4), e
     # fused scores == generic forward on the same waypoints, up to the BF16
     # rounding of features built from FP32-interpolated waypoints
     zs, ss = [], []

@@ -373,3 +373,13 @@ def test_score_order_counts_checks():
                     want = 2 + (bad[0] + 1 if bad else len(wp))
                     got = o.checks_in_score_order(a, b, s, known_valid=True, lead=lead, top_levels=top)
                     assert got == want, (lead, top)
+                    # picks: only the first few of the scored ones lead, every
+                    # other waypoint follows in reference order
+                    for npk in (0, 1, 4, 100):
+                        lead_ = head + mid[:npk]
+                        order = lead_ + [i for i in range(len(wp)) if i not in set(lead_)]
+                        bad = [pos for pos, i in enumerate(order) if not ok[i]]
+                        want = 2 + (bad[0] + 1 if bad else len(wp))
+                        got = o.checks_in_score_order(a, b, s, known_valid=True, lead=lead, top_levels=top,
+                                                      picks=npk)
+                        assert got == want, (lead, top, npk)
