@@ -145,7 +145,7 @@ struct mptg_ctx {
     // unsorted pairs and sort scratch; staged host queries; sorted results
     mptg::DevBuf rnn_buf, rnn_pairs, rnn_q, rnn_out;
     mptg::DevBuf scratch[12];  // 0-2 host staging, 3 kNN seed bounds, 4-5 kNN partials, 6 DMV table, 7 edge lists, 8 scores,
-                               // 9 ranked ordinals, 10 continuation list
+                               // 9 surrogate picks, 10 continuation list
     // pinned host staging
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

@@ -381,10 +381,22 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
 // C2 = ceil(2 kQ8Slack D) + 1, is farther by at least s.  Queries with a
 // coordinate beyond B take no prefilter (every node gets its exact distance).
 //
-// Two passes: knn_q8_min finds every query's smallest byte distance over the
-// whole tree (atomicMin over the splits), then knn_scan_q8 takes the exact
-// FP64 distance of only the nodes within C2 of it -- the nodes that can still
-// be the nearest, typically a handful per query.
+// One pass with a bound shared by the query's chunks.  knn_q8_seed measures
+// kSeedNodes spread tree nodes exactly and starts the query's byte bound
+// gthr[q] at T(their nearest).  Every byte bound taken below is at least the
+// nearest node's SAD: T(d) for the distance d of a real node (the nearest is
+// no farther), and a + C2 for any node of byte distance a (the nearest has
+// L1 <= that node's L1 <= s (a + kQ8Slack D), so SAD <= a + C2).  knn_scan_q8
+// keeps a lane bound thr = min(T(radius), gthr[q], a + C2 over its
+// candidates so far), lowers gthr[q] to it with atomicMin at the end of each
+// tile and reads it back at the start of the next, so each chunk's scan
+// tightens the others'; a node gets its exact FP64 distance only when its
+// SAD is within thr.  Nodes are tested 8 at a time against the minimum of
+// their SADs first.  (Measured alternatives at a 10^5-node tree, 2 725
+// queries: a separate pass for each query's exact byte minimum first, 160 +
+// 223 us; byte rows in scalar registers, 4 queries per lane, 137 + 175 us;
+// that layout in one pass, slower again from its divergent candidate
+// checks; this one, 16 + 223 us.)
 constexpr int kQ8Tile = 256;
 constexpr double kQ8Slack = 1.001;
 
@@ -451,63 +463,38 @@ __device__ __forceinline__ void q8_stage(const float* __restrict__ x32, int64_t 
     }
 }
 
-// Pass 1: gmin[q] = min over the tree of the byte distance (gmin starts at
-// 0xffffffff; a query beyond B leaves it there).
+// Byte bound of a distance d (see above): T(d) = floor(d (1 + 2^-40) / s +
+// kQ8Slack D); ~0 for d = inf or beyond the word.
+__device__ __forceinline__ uint32_t q8_thr(double d, double inv_s, int D) {
+    if (!(d < __builtin_inf())) return 0xffffffffu;
+    const double t = d * (1.0 + 0x1.0p-40) * inv_s + kQ8Slack * D;
+    return t >= 4294967295.0 ? 0xffffffffu : (uint32_t)t;
+}
+
+// gthr[q] = T(exact distance to the nearest of kSeedNodes spread tree nodes)
+// for every live query (knn_seed's sample and lane layout).
 template <int D>
-__global__ __launch_bounds__(kBlock) void knn_q8_min(const float* __restrict__ x32, int64_t n, int64_t chunk,
-                                                     const double* __restrict__ q, int64_t nq, double bound,
-                                                     uint32_t* __restrict__ gmin, const int64_t* n_dev,
-                                                     const int32_t* skip, const uint32_t* nq_dev) {
-    constexpr int NW = (D + 3) / 4;
-    __shared__ __align__(16) uint32_t t8[NW * kQ8Tile];
-    if (!knn_extent(n_dev, skip, &n, &chunk)) return;  // block-uniform
-    const int64_t live = nq_dev ? (int64_t)*nq_dev : nq;
-    if ((int64_t)blockIdx.x * kBlock >= live) return;  // block-uniform
-    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool active = qi < live;
-    const int64_t qc = active ? qi : 0;
-    const int64_t p0 = (int64_t)blockIdx.y * chunk;
-    const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
-    const float Bf = (float)bound;
-    const float inv_sf = (float)(255.0 / (2.0 * bound));
-    uint32_t qw[NW];
-    const bool inside = q8_query<D>(q + qc * D, bound, Bf, inv_sf, qw);
-    uint32_t m = 0xffffffffu;
-    for (int64_t t0 = p0; t0 < p1; t0 += kQ8Tile) {  // block-uniform bounds
-        const int cnt = (int)(p1 - t0 < kQ8Tile ? p1 - t0 : kQ8Tile);
-        __syncthreads();
-        q8_stage<D>(x32, t0, cnt, Bf, inv_sf, t8);
-        __syncthreads();
-        constexpr int U = 8;
-        int j = 0;
-        for (; j + U <= cnt; j += U) {
-            uint32_t sad[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) sad[u] = 0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const uint4 lo = *reinterpret_cast<const uint4*>(t8 + w * kQ8Tile + j);
-                const uint4 hi = *reinterpret_cast<const uint4*>(t8 + w * kQ8Tile + j + 4);
-                sad[0] = __builtin_amdgcn_sad_u8(qw[w], lo.x, sad[0]);
-                sad[1] = __builtin_amdgcn_sad_u8(qw[w], lo.y, sad[1]);
-                sad[2] = __builtin_amdgcn_sad_u8(qw[w], lo.z, sad[2]);
-                sad[3] = __builtin_amdgcn_sad_u8(qw[w], lo.w, sad[3]);
-                sad[4] = __builtin_amdgcn_sad_u8(qw[w], hi.x, sad[4]);
-                sad[5] = __builtin_amdgcn_sad_u8(qw[w], hi.y, sad[5]);
-                sad[6] = __builtin_amdgcn_sad_u8(qw[w], hi.z, sad[6]);
-                sad[7] = __builtin_amdgcn_sad_u8(qw[w], hi.w, sad[7]);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) m = sad[u] < m ? sad[u] : m;
-        }
-        for (; j < cnt; ++j) {
-            uint32_t sad = 0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) sad = __builtin_amdgcn_sad_u8(qw[w], t8[w * kQ8Tile + j], sad);
-            m = sad < m ? sad : m;
-        }
+__global__ __launch_bounds__(kBlock) void knn_q8_seed(const double* __restrict__ x64, int64_t n,
+                                                      const double* __restrict__ q, int64_t nq, double bound,
+                                                      uint32_t* __restrict__ gthr, const int64_t* n_dev,
+                                                      const int32_t* skip, const uint32_t* nq_dev) {
+    if (skip && *skip) return;
+    if (nq_dev) nq = *nq_dev;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t qi = t / kSeedLanes;
+    const int part = (int)(t % kSeedLanes);
+    if (n_dev) n = *n_dev;
+    const int64_t ns = n < kSeedNodes ? n : kSeedNodes;
+    const int64_t stride = ns > 0 ? n / ns : 1;
+    double best = __builtin_inf();
+    if (qi < nq) {
+#pragma unroll 8
+        for (int64_t j = part; j < ns; j += kSeedLanes)
+            best = __builtin_fmin(best, exact_dist<D, MPTG_METRIC_L1>(q + qi * D, x64 + j * stride * D));
     }
-    if (active && inside && m != 0xffffffffu) atomicMin(gmin + qi, m);
+#pragma unroll
+    for (int o = 1; o < kSeedLanes; o <<= 1) best = __builtin_fmin(best, __shfl_xor(best, o));
+    if (qi < nq && part == 0) gthr[qi] = q8_thr(best, 255.0 / (2.0 * bound), D);
 }
 
 template <int D>
@@ -516,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
                                                       int64_t nq, double radius, double bound,
                                                       double* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                       const int64_t* n_dev, const int32_t* skip,
-                                                      const uint32_t* __restrict__ gmin, const uint32_t* nq_dev) {
+                                                      uint32_t* __restrict__ gthr, const uint32_t* nq_dev) {
     constexpr int NW = (D + 3) / 4;  // words of a byte row
     // byte rows by word: word w of tile node j at t8[w * kQ8Tile + j]
     __shared__ __align__(16) uint32_t t8[NW * kQ8Tile];
@@ -534,24 +521,13 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
 
     uint32_t qw[NW];
     const bool inside = q8_query<D>(q + qc * D, bound, Bf, inv_sf, qw);
-    // pass iff SAD <= thr; thr from the current bound d (see above)
-    auto thr_of = [&](double d) -> uint32_t {
-        if (!(d < __builtin_inf())) return 0xffffffffu;
-        const double t = d * (1.0 + 0x1.0p-40) * inv_s + kQ8Slack * D;
-        return t >= 4294967295.0 ? 0xffffffffu : (uint32_t)t;
-    };
     constexpr uint32_t C2 = (uint32_t)(2.0 * kQ8Slack * D) + 2;
     // a lane whose query lies outside the grid checks every node exactly;
     // inactive lanes take no node
-    const bool filter = inside;
+    const bool filter = active && inside;
     const bool take_any = active;
-    uint32_t thr = inside ? thr_of(radius) : 0xffffffffu;
-    if (inside) {
-        // within C2 of the tree's smallest byte distance (pass 1)
-        const uint32_t g = gmin[qc];
-        const uint32_t t = g > 0xffffffffu - C2 ? 0xffffffffu : g + C2;
-        thr = t < thr ? t : thr;
-    }
+    uint32_t thr = filter ? q8_thr(radius, inv_s, D) : 0xffffffffu;
+    uint32_t shared = 0xffffffffu;  // gthr[q] as last read or published
     TopK<1> top;
     top.init();
     constexpr int kCand = 4;
@@ -566,9 +542,21 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
             }
         ncand = 0;
     };
+    auto take = [&](uint32_t sad, int64_t node) {
+        if (ncand == kCand) flush();
+        cand[ncand++] = (int32_t)node;
+        if (filter) {
+            const uint32_t t = sad > 0xffffffffu - C2 ? 0xffffffffu : sad + C2;
+            thr = t < thr ? t : thr;
+        }
+    };
 
     for (int64_t t0 = p0; t0 < p1; t0 += kQ8Tile) {  // block-uniform bounds
         const int cnt = (int)(p1 - t0 < kQ8Tile ? p1 - t0 : kQ8Tile);
+        if (filter) {  // the other chunks' progress
+            shared = __hip_atomic_load(gthr + qc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            thr = shared < thr ? shared : thr;
+        }
         __syncthreads();  // every wave is done with the previous tile
         q8_stage<D>(x32, t0, cnt, Bf, inv_sf, t8);
         __syncthreads();
@@ -591,32 +579,24 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
                 sad[6] = __builtin_amdgcn_sad_u8(qw[w], hi.z, sad[6]);
                 sad[7] = __builtin_amdgcn_sad_u8(qw[w], hi.w, sad[7]);
             }
-            uint32_t pass = 0;
+            uint32_t m8 = sad[0];
 #pragma unroll
-            for (int u = 0; u < U; ++u) pass |= (take_any && sad[u] <= thr ? 1u : 0u) << u;
-            while (pass) {
-                const int u = __builtin_ctz(pass);
-                pass &= pass - 1;
-                if (ncand == kCand) flush();
-                cand[ncand++] = (int32_t)(t0 + j + u);
-                if (filter) {
-                    const uint32_t t = sad[u] + C2;
-                    thr = t < thr ? t : thr;
-                }
+            for (int u = 1; u < U; ++u) m8 = sad[u] < m8 ? sad[u] : m8;
+            if (take_any && m8 <= thr) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (sad[u] <= thr) take(sad[u], t0 + j + u);
             }
         }
         for (; j < cnt; ++j) {
             uint32_t sad = 0;
 #pragma unroll
             for (int w = 0; w < NW; ++w) sad = __builtin_amdgcn_sad_u8(qw[w], t8[w * kQ8Tile + j], sad);
-            if (take_any && sad <= thr) {
-                if (ncand == kCand) flush();
-                cand[ncand++] = (int32_t)(t0 + j);
-                if (filter) {
-                    const uint32_t t = sad + C2;
-                    thr = t < thr ? t : thr;
-                }
-            }
+            if (take_any && sad <= thr) take(sad, t0 + j);
+        }
+        if (filter && thr < shared) {  // share this chunk's progress
+            atomicMin(gthr + qc, thr);
+            shared = thr;
         }
     }
     flush();
@@ -626,7 +606,8 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
     out_i[base] = top.i[0];
 }
 
-// Direct FP64 scan for metrics without a prefilter (SE(3)) and any dimension.
+// Merge of the chunks' partial k-lists: each lane keeps its query's k best
+// over the splits' lists.
 template <int K>
 __global__ __launch_bounds__(kBlock) void knn_merge(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                     int splits, int64_t nq, int k, double* __restrict__ out_d,
@@ -987,7 +968,7 @@ struct ScanExtent {
     const int32_t* skip;
     double* seed;            // k = 1: per-query starting bound (nullptr: none)
     const uint32_t* nq_dev;  // live queries on the device (nullptr: all nq)
-    bool q8;                 // k = 1 under L1: the byte prefilter, its minima in `seed`
+    bool q8;                 // k = 1 under L1: the byte prefilter, its shared bounds in `seed`
 };
 
 template <int D, int METRIC, int K>
@@ -995,14 +976,14 @@ void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q,
                  double* pd, int32_t* pi) {
     if constexpr (K == 1 && METRIC == MPTG_METRIC_L1 && D >= 8) {
         if (x.q8) {
-            // the seed buffer (nq doubles) holds the byte minima (filled with
-            // 0xffffffff by the caller) instead
-            uint32_t* gmin = reinterpret_cast<uint32_t*>(x.seed);
-            hipLaunchKernelGGL((knn_q8_min<D>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, x.n, x.chunk, q, nq,
-                               ctx->coord_bound, gmin, x.n_dev, x.skip, x.nq_dev);
+            // the seed buffer (nq doubles) holds the queries' shared byte
+            // bounds instead
+            uint32_t* gthr = reinterpret_cast<uint32_t*>(x.seed);
+            hipLaunchKernelGGL((knn_q8_seed<D>), dim3((unsigned)((nq * kSeedLanes + kBlock - 1) / kBlock)),
+                               dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, ctx->coord_bound, gthr, x.n_dev,
+                               x.skip, x.nq_dev);
             hipLaunchKernelGGL((knn_scan_q8<D>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n,
-                               x.chunk, q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip,
-                               (const uint32_t*)gmin, x.nq_dev);
+                               x.chunk, q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip, gthr, x.nq_dev);
             return;
         }
     }
@@ -1101,7 +1082,6 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     // k = 1 under L1 in 8 or more dimensions: the byte prefilter
     const bool q8 = seed && ctx->metric == MPTG_METRIC_L1 && ctx->dim >= 8 && ctx->tune.knn_q8 &&
                     ctx->coord_bound > 0 && ctx->coord_bound < 1e300;
-    if (q8 && (rc = launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(seed), nq, 0xffffffffu))) return rc;
     const ScanExtent ext{n, chunk, n_dev, skip, seed, nq_dev, q8};
     double* pd = (double*)ctx->scratch[4].p;
     int32_t* pi = (int32_t*)ctx->scratch[5].p;
