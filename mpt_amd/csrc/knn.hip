@@ -396,7 +396,8 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
 // queries: a separate pass for each query's exact byte minimum first, 160 +
 // 223 us; byte rows in scalar registers, 4 queries per lane, 137 + 175 us;
 // that layout in one pass, slower again from its divergent candidate
-// checks; this one, 16 + 223 us.)
+// checks; this one, 16 + 223 us.  Tiles of 128 or 512 nodes instead of 256
+// cost occupancy: 3 and 2 waves per SIMD against 4.)
 constexpr int kQ8Tile = 256;
 constexpr double kQ8Slack = 1.001;
 
@@ -472,7 +473,9 @@ __device__ __forceinline__ uint32_t q8_thr(double d, double inv_s, int D) {
 }
 
 // gthr[q] = T(exact distance to the nearest of kSeedNodes spread tree nodes)
-// for every live query (knn_seed's sample and lane layout).
+// for every live query (knn_seed's sample; a wave per query, each lane
+// measuring kSeedNodes / 64 nodes).
+constexpr int kQ8SeedLanes = 64;
 template <int D>
 __global__ __launch_bounds__(kBlock) void knn_q8_seed(const double* __restrict__ x64, int64_t n,
                                                       const double* __restrict__ q, int64_t nq, double bound,
@@ -481,19 +484,19 @@ __global__ __launch_bounds__(kBlock) void knn_q8_seed(const double* __restrict__
     if (skip && *skip) return;
     if (nq_dev) nq = *nq_dev;
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t qi = t / kSeedLanes;
-    const int part = (int)(t % kSeedLanes);
+    const int64_t qi = t / kQ8SeedLanes;
+    const int part = (int)(t % kQ8SeedLanes);
     if (n_dev) n = *n_dev;
     const int64_t ns = n < kSeedNodes ? n : kSeedNodes;
     const int64_t stride = ns > 0 ? n / ns : 1;
     double best = __builtin_inf();
     if (qi < nq) {
-#pragma unroll 8
-        for (int64_t j = part; j < ns; j += kSeedLanes)
+#pragma unroll 4
+        for (int64_t j = part; j < ns; j += kQ8SeedLanes)
             best = __builtin_fmin(best, exact_dist<D, MPTG_METRIC_L1>(q + qi * D, x64 + j * stride * D));
     }
 #pragma unroll
-    for (int o = 1; o < kSeedLanes; o <<= 1) best = __builtin_fmin(best, __shfl_xor(best, o));
+    for (int o = 1; o < kQ8SeedLanes; o <<= 1) best = __builtin_fmin(best, __shfl_xor(best, o));
     if (qi < nq && part == 0) gthr[qi] = q8_thr(best, 255.0 / (2.0 * bound), D);
 }
 
@@ -979,7 +982,7 @@ void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q,
             // the seed buffer (nq doubles) holds the queries' shared byte
             // bounds instead
             uint32_t* gthr = reinterpret_cast<uint32_t*>(x.seed);
-            hipLaunchKernelGGL((knn_q8_seed<D>), dim3((unsigned)((nq * kSeedLanes + kBlock - 1) / kBlock)),
+            hipLaunchKernelGGL((knn_q8_seed<D>), dim3((unsigned)((nq * kQ8SeedLanes + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, ctx->coord_bound, gthr, x.n_dev,
                                x.skip, x.nq_dev);
             hipLaunchKernelGGL((knn_scan_q8<D>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n,
