@@ -335,8 +335,8 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
         dispatch_dim(D, [&]<int DD>() {
             if constexpr (DD == 5 || DD == 10 || DD == 20) {
                 if (arm8) {
-                    hipLaunchKernelGGL((prrt_prevalid<DD, kArmLanes>),
-                                       dim3((unsigned)(((int64_t)B * kArmLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                    hipLaunchKernelGGL((prrt_prevalid<DD, kPrevalidLanes>),
+                                       dim3((unsigned)(((int64_t)B * kPrevalidLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                                        ctx->stream, scene_args(ctx), D, B, (const double*)pb.q.p,
                                        (uint8_t*)pb.status.p, (uint8_t*)pb.mask.p, (double*)pb.qv.p,
                                        (int32_t*)pb.vidx.p, vcnt, (const Counters*)dcnt);

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc,
 // share its disk tests (arm_ok_split); lane 0 of each group writes.  Same
 // outputs as prrt_steer.
 constexpr int kArmLanes = 8;
+constexpr int kPrevalidLanes = 4;  // lanes per sample in prrt_prevalid on arm scenes
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void prrt_steer_arm(SpaceArgs sp, SceneArgs sc, double range, int B,
