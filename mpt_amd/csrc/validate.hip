@@ -117,7 +117,7 @@ __device__ __forceinline__ bool valid_k(const SceneArgs& sc, const double* q) {
 // up to 3 coordinates, whose waypoints stay in registers.
 template <int D>
 constexpr int dmv_unroll() {
-    return D > 0 && D <= 3 ? 4 : 1;
+    return D > 0 && D <= 3 ? 2 : 1;
 }
 
 // G = 2^lg lanes per edge (1..64; lanes of one edge sit in one wave).  An
