@@ -5,9 +5,13 @@
 //     level by level and left to right (pre-order ordinals rise within a
 //     level);
 //   * skip_picks(k, picks) is the k-th reference ordinal outside the picks,
-//     against a brute-force walk, for pick sets drawn over the whole tree.
+//     against a brute-force walk, for pick sets drawn over the whole tree;
+//   * coord_mid equals the reference's midpoint recursion (lo + hi) / 2
+//     along the path (demo/link_manipulator_scenario.hpp:118-138) bit for
+//     bit, for random endpoints, paths and depths 0..20.
 // Prints "bad=N" and exits non-zero when a check fails.
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "bisect_dev.hpp"
@@ -82,6 +86,29 @@ int main() {
             }
             if (k != W - kPicks) ++bad;
         }
+    }
+    // coord_mid against the recursion as the reference writes it
+    auto recursion = [](double lo, double hi, uint32_t path, int depth) {
+        for (int k = 0; k < depth; ++k) {
+            const double m = (lo + hi) / 2;
+            if ((path >> k) & 1u) lo = m;
+            else hi = m;
+        }
+        return (lo + hi) / 2;
+    };
+    for (int t = 0; t < 200000; ++t) {
+        auto unit = [&]() { return (double)(next() >> 11) * 0x1.0p-53; };
+        const double a = (unit() * 2.0 - 1.0) * 3.2;
+        double b = (unit() * 2.0 - 1.0) * 3.2;
+        if (t % 7 == 0) b = a + (unit() - 0.5) * 1e-9;  // nearly equal ends
+        if (t % 11 == 0) b = -a;
+        const int depth = (int)(next() % 21);
+        const uint32_t path = depth ? (uint32_t)next() & ((1u << depth) - 1u) : 0u;
+        const double x = coord_mid(a, b, path, depth), y = recursion(a, b, path, depth);
+        uint64_t xb, yb;
+        std::memcpy(&xb, &x, 8);
+        std::memcpy(&yb, &y, 8);
+        if (xb != yb) ++bad;
     }
     std::printf("bad=%d\n", bad);
     return bad != 0;
