@@ -395,6 +395,9 @@ def bench_se3(ctx, args):
     ctx.prrt_run(1 << 30, args.warmup)
     s0 = ctx.prrt_stats()
     ctx.sync()
+    # the tree grows during the timed iterations (PRRT* trees carry path costs and
+    # child lists, so a step cannot start again from a reloaded tree): the start
+    # and end sizes are reported, both fixed by the seed, --warmup and --steps
     t0 = time.perf_counter()
     ctx.prrt_run(1 << 30, args.steps)
     ctx.sync()
@@ -406,7 +409,8 @@ def bench_se3(ctx, args):
             "extensions_per_s": d["samples"] / dt, "accepted_per_s": d["link_ok"] / dt,
             "edge_checks_per_s": links / dt, "extend_links": d["link_calls"], "choose_parent_links": d["cp_links"],
             "rewire_links": d["rw_links"], "rewires": d["rewires"], "deferred": d["deferred"],
-            "tree_size": s1["size"], "goals": s1["goals"], "autotune_mesh_edge_waves": tune,
+            "tree_size_start": s0["size"], "tree_size": s1["size"], "goals": s1["goals"],
+            "autotune_mesh_edge_waves": tune,
             "launch_shapes": ctx.tune_get()}
 
 
