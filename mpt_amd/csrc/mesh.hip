@@ -18,8 +18,9 @@
 //   mesh_link   one edge per wave; waypoints in DMV order one at a time
 //               (reference checks_done counts), early exit on the first hit
 // Per state, lane l poses robot triangles [8l, 8l + 8) of each 512-triangle
-// chunk, takes their box, and walks the env BVH depth-first with a private
-// stack in LDS; a per-wave LDS flag stops every lane at the first hit.
+// chunk and takes their box; the wave walks the env BVH depth-first once with
+// one LDS stack, visiting a node when any lane's box overlaps it, and stops
+// at the first leaf where some lane finds a hit.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -35,8 +36,8 @@ namespace {
 
 constexpr int kLaneTris = 8;                    // robot triangles per lane
 constexpr int kChunkTris = 64 * kLaneTris;      // robot triangles per wave pass
-constexpr int kStackStride = 41;                // LDS stack slots per lane (odd: no bank conflicts)
-constexpr int kBuildDepth = 38;                 // BVH depth bound (stack needs depth + 1)
+constexpr int kBuildDepth = 38;                 // BVH depth bound
+constexpr int kStackSlots = kBuildDepth + 1;    // LDS stack of a wave's walk
 constexpr int kMaxDmvWaypointsMesh = 1 << 24;   // same limit as the other DMV kernels
 constexpr int kTableMaxMesh = 255;
 
@@ -188,16 +189,20 @@ __device__ __forceinline__ void load_env_tri(const float4* __restrict__ t, int i
 }
 
 // Does the robot at state q collide with the environment?  Whole wave; q is
-// the same on every lane.  stack: this wave's 64 * kStackStride LDS slots;
-// flag: this wave's LDS word.
-__device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack, volatile int* flag) {
+// the same on every lane.  stack: this wave's kStackSlots LDS slots.
+//
+// Per 512-triangle chunk, lane l poses robot triangles [8l, 8l + 8) and takes
+// their box; the wave walks the environment BVH depth-first once, visiting a
+// node when any lane's box overlaps it (a ballot), so node and triangle loads
+// are wave-uniform and no lane diverges in the walk.  At a leaf each lane
+// whose box overlaps it tests its triangles against the leaf's: a pair is
+// tested only when the boxes overlap, as the predicate requires, and a lane
+// whose box overlaps a leaf overlaps every ancestor of it, so the pairs
+// tested are those a per-lane walk would test.
+__device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack) {
     const int lane = threadIdx.x & 63;
     if (m.nnodes == 0 || m.nrobot == 0) return false;
     const Pose32 pose = pose_from_state(q);
-    if (lane == 0) *flag = 0;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    int32_t* stk = stack + lane * kStackStride;
     for (int c0 = 0; c0 < m.nrobot; c0 += kChunkTris) {
         const int first = c0 + lane * kLaneTris;
         const int cnt = max(0, min(kLaneTris, m.nrobot - first));
@@ -218,22 +223,24 @@ __device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack
                 lhi[i] = thi[i] > lhi[i] ? thi[i] : lhi[i];
             }
         }
-        int sp = 0;
-        if (cnt > 0) stk[sp++] = 0;
-        while (sp > 0 && *flag == 0) {
-            const MeshNode n = m.nodes[stk[--sp]];
-            if (!boxes_overlap(n.lo, n.hi, llo, lhi)) continue;
+        int sp = 0;  // wave-uniform; a pop precedes each pair of pushes, so sp <= depth + 1
+        stack[sp++] = 0;
+        while (sp > 0) {
+            const int ni = __builtin_amdgcn_readfirstlane(stack[--sp]);
+            const MeshNode n = m.nodes[ni];
+            const bool ov = cnt > 0 && boxes_overlap(n.lo, n.hi, llo, lhi);
+            if (__ballot(ov) == 0) continue;
             if (n.b > 0) {
-                stk[sp++] = n.b;
-                stk[sp++] = n.a;
+                stack[sp++] = n.b;
+                stack[sp++] = n.a;
                 continue;
             }
             bool hit = false;
 #pragma unroll 1
-            for (int i = n.a; i < n.a - n.b && !hit; ++i) {
+            for (int i = n.a; i < n.a - n.b; ++i) {  // wave-uniform bounds
                 float ev[9], elo[3], ehi[3];
                 load_env_tri(m.tris, i, ev, elo, ehi);
-                if (!boxes_overlap(elo, ehi, llo, lhi)) continue;
+                if (!ov || hit || !boxes_overlap(elo, ehi, llo, lhi)) continue;
 #pragma unroll 1
                 for (int t = 0; t < cnt && !hit; ++t) {
                     const float* src = m.robot + (size_t)(first + t) * 9;
@@ -244,24 +251,20 @@ __device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack
                     hit = pair_collides(tri, tlo, thi, ev, elo, ehi);
                 }
             }
-            if (hit) *flag = 1;
+            if (__ballot(hit)) return true;
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        if (*flag) return true;
     }
     return false;
 }
 
 __global__ __launch_bounds__(64) void mesh_valid(MeshArgs m, const double* __restrict__ q, int64_t n,
                                                  uint8_t* __restrict__ ok) {
-    __shared__ int32_t stack[64 * kStackStride];
-    __shared__ int flag;
+    __shared__ int32_t stack[kStackSlots];
     for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
         double s[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) s[j] = q[i * 7 + j];
-        const bool hit = mesh_collides(m, s, stack, &flag);
+        const bool hit = mesh_collides(m, s, stack);
         if (threadIdx.x == 0) ok[i] = hit ? 0 : 1;
     }
 }
@@ -269,14 +272,13 @@ __global__ __launch_bounds__(64) void mesh_valid(MeshArgs m, const double* __res
 __global__ __launch_bounds__(64) void mesh_filter(MeshArgs m, const double* __restrict__ q, int64_t n,
                                                   uint8_t* __restrict__ mask, uint8_t* __restrict__ status,
                                                   int invalid_code) {
-    __shared__ int32_t stack[64 * kStackStride];
-    __shared__ int flag;
+    __shared__ int32_t stack[kStackSlots];
     for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
         if (!mask[i]) continue;
         double s[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) s[j] = q[i * 7 + j];
-        const bool hit = mesh_collides(m, s, stack, &flag);
+        const bool hit = mesh_collides(m, s, stack);
         if (threadIdx.x == 0 && hit) {
             mask[i] = 0;
             status[i] = (uint8_t)invalid_code;
@@ -304,8 +306,7 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
                                                 const double* __restrict__ to, int64_t n, int known_valid,
                                                 const uint8_t* __restrict__ mask, int waves_per_edge,
                                                 uint32_t* __restrict__ slot) {
-    __shared__ int32_t stack[64 * kStackStride];
-    __shared__ int flag;
+    __shared__ int32_t stack[kStackSlots];
     __shared__ uint64_t rmin[256], rmax[256];
     __shared__ uint64_t emitted[64];
     __shared__ int cnt_s;
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
             }
         };
         load_ab();
-        if (p == 0 && !known_valid && mesh_collides(m, b, stack, &flag)) {
+        if (p == 0 && !known_valid && mesh_collides(m, b, stack)) {
             if (lane == 0) atomicMin(&slot[e], 0u);
             __syncthreads();
             continue;
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
                 if (current(e) < pos) break;  // an earlier position already failed
                 load_ab();
                 space_interp(sp, a, b, (double)ord[o] * delta, q);
-                if (mesh_collides(m, q, stack, &flag)) {
+                if (mesh_collides(m, q, stack)) {
                     if (lane == 0) atomicMin(&slot[e], pos);
                     break;
                 }
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(64) void mesh_link(SpaceArgs sp, MeshArgs m, double
                     load_ab();
                     space_interp(sp, a, b, (double)emitted[o] * delta, q);
                     ++done;
-                    if (mesh_collides(m, q, stack, &flag)) {
+                    if (mesh_collides(m, q, stack)) {
                         failed = true;
                         break;
                     }
