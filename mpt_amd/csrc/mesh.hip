@@ -37,7 +37,7 @@ namespace {
 constexpr int kLaneTris = 8;                    // robot triangles per lane
 constexpr int kChunkTris = 64 * kLaneTris;      // robot triangles per wave pass
 constexpr int kBuildDepth = 38;                 // BVH depth bound
-constexpr int kStackSlots = kBuildDepth + 1;    // LDS stack of a wave's walk
+constexpr int kStackSlots = 2 * kBuildDepth;    // LDS stack of a wave's walk: one (a, b) per level
 constexpr int kMaxDmvWaypointsMesh = 1 << 24;   // same limit as the other DMV kernels
 constexpr int kTableMaxMesh = 255;
 
@@ -223,35 +223,61 @@ __device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack
                 lhi[i] = thi[i] > lhi[i] ? thi[i] : lhi[i];
             }
         }
-        int sp = 0;  // wave-uniform; a pop precedes each pair of pushes, so sp <= depth + 1
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int ni = __builtin_amdgcn_readfirstlane(stack[--sp]);
-            const MeshNode n = m.nodes[ni];
-            const bool ov = cnt > 0 && boxes_overlap(n.lo, n.hi, llo, lhi);
-            if (__ballot(ov) == 0) continue;
-            if (n.b > 0) {
-                stack[sp++] = n.b;
-                stack[sp++] = n.a;
-                continue;
-            }
-            bool hit = false;
-#pragma unroll 1
-            for (int i = n.a; i < n.a - n.b; ++i) {  // wave-uniform bounds
-                float ev[9], elo[3], ehi[3];
-                load_env_tri(m.tris, i, ev, elo, ehi);
-                if (!ov || hit || !boxes_overlap(elo, ehi, llo, lhi)) continue;
-#pragma unroll 1
-                for (int t = 0; t < cnt && !hit; ++t) {
-                    const float* src = m.robot + (size_t)(first + t) * 9;
-                    float tri[9], tlo[3], thi[3];
-#pragma unroll
-                    for (int v = 0; v < 3; ++v) xform(pose, src + 3 * v, tri + 3 * v);
-                    tri_box(tri, tlo, thi);
-                    hit = pair_collides(tri, tlo, thi, ev, elo, ehi);
+        const bool live = cnt > 0;
+        // Current node (ca, cb) = its children, or its leaf range [ca, ca - cb)
+        // when cb <= 0; a node is entered only when some lane's box overlaps
+        // it.  Both children of an internal node are loaded and tested
+        // together; the wave descends into the left one that overlaps and
+        // keeps the right one on the stack (at most one entry per level).
+        const MeshNode root = m.nodes[0];
+        if (__ballot(live && boxes_overlap(root.lo, root.hi, llo, lhi)) == 0) continue;
+        int ca = root.a, cb = root.b;
+        int sp = 0;  // wave-uniform
+        for (;;) {
+            if (cb > 0) {
+                const MeshNode L = m.nodes[ca], R = m.nodes[cb];
+                const bool gl = __ballot(live && boxes_overlap(L.lo, L.hi, llo, lhi)) != 0;
+                const bool gr = __ballot(live && boxes_overlap(R.lo, R.hi, llo, lhi)) != 0;
+                if (gl) {
+                    if (gr) {
+                        stack[2 * sp] = R.a;
+                        stack[2 * sp + 1] = R.b;
+                        ++sp;
+                    }
+                    ca = L.a;
+                    cb = L.b;
+                    continue;
                 }
+                if (gr) {
+                    ca = R.a;
+                    cb = R.b;
+                    continue;
+                }
+            } else {
+                // a lane meets a triangle's box only if it meets the leaf's
+                // and every ancestor's, so these pairs are the per-lane walk's
+                bool hit = false;
+#pragma unroll 1
+                for (int i = ca; i < ca - cb; ++i) {  // wave-uniform bounds
+                    float ev[9], elo[3], ehi[3];
+                    load_env_tri(m.tris, i, ev, elo, ehi);
+                    if (!live || hit || !boxes_overlap(elo, ehi, llo, lhi)) continue;
+#pragma unroll 1
+                    for (int t = 0; t < cnt && !hit; ++t) {
+                        const float* src = m.robot + (size_t)(first + t) * 9;
+                        float tri[9], tlo[3], thi[3];
+#pragma unroll
+                        for (int v = 0; v < 3; ++v) xform(pose, src + 3 * v, tri + 3 * v);
+                        tri_box(tri, tlo, thi);
+                        hit = pair_collides(tri, tlo, thi, ev, elo, ehi);
+                    }
+                }
+                if (__ballot(hit)) return true;
             }
-            if (__ballot(hit)) return true;
+            if (sp == 0) break;
+            --sp;
+            ca = __builtin_amdgcn_readfirstlane(stack[2 * sp]);
+            cb = __builtin_amdgcn_readfirstlane(stack[2 * sp + 1]);
         }
     }
     return false;
