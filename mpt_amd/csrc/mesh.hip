@@ -8,8 +8,10 @@
 // apart, so the verdict is the predicate's exactly.
 //
 // Layout in HBM (built on the host at upload):
-//   env BVH   MeshNode[nnodes] (32 B: box + children or leaf range), depth-first
-//             order, binned-SAH splits, <= 4 triangles per leaf, depth <= 38
+//   env BVH   binned-SAH binary tree (<= 4 triangles per leaf, depth <= 38)
+//             collapsed to MeshNode4[nnodes]: 128 B per node holding the
+//             boxes (by axis) and references of up to four nodes two levels
+//             below it, so one uniform load serves two levels of the walk
 //   env tris  16 floats (64 B) per triangle in leaf order: v0 v1 v2, box lo, hi
 //   robot     9 floats per triangle, Morton-sorted by centroid so that the 8
 //             consecutive triangles a lane owns are close together
@@ -37,7 +39,8 @@ namespace {
 constexpr int kLaneTris = 8;                    // robot triangles per lane
 constexpr int kChunkTris = 64 * kLaneTris;      // robot triangles per wave pass
 constexpr int kBuildDepth = 38;                 // BVH depth bound
-constexpr int kStackSlots = 2 * kBuildDepth;    // LDS stack of a wave's walk: one (a, b) per level
+// LDS stack of a wave's walk: up to three (a, b) entries per four-wide level
+constexpr int kStackSlots = 2 * 3 * (kBuildDepth / 2 + 1);
 constexpr int kMaxDmvWaypointsMesh = 1 << 24;   // same limit as the other DMV kernels
 constexpr int kTableMaxMesh = 255;
 
@@ -160,6 +163,78 @@ int bvh_depth(const std::vector<MeshNode>& nodes, int n, int d) {
     return std::max(bvh_depth(nodes, nodes[n].a, d + 1), bvh_depth(nodes, nodes[n].b, d + 1));
 }
 
+// Four-wide node: slot k holds the box and reference of a binary node one or
+// two levels below (the children, with an internal child replaced by its two
+// children).  b[k] > 0: a[k] is a MeshNode4 index; b[k] <= 0: triangles
+// [a[k], a[k] - b[k]); an unused slot has an empty box (lo > hi).
+struct MeshNode4 {
+    float lo[3][4], hi[3][4];
+    int32_t a[4], b[4];
+};
+static_assert(sizeof(MeshNode4) == 128, "four-wide node is 128 bytes");
+
+// Collapse the binary subtree under internal node id into out (pre-order).
+int collapse4(const std::vector<MeshNode>& n2, int id, std::vector<MeshNode4>& out) {
+    const int me = (int)out.size();
+    out.push_back(MeshNode4{});
+    int slot[4], ns = 0;
+    for (const int c : {n2[id].a, n2[id].b}) {
+        if (n2[c].b > 0) {
+            slot[ns++] = n2[c].a;
+            slot[ns++] = n2[c].b;
+        } else {
+            slot[ns++] = c;
+        }
+    }
+    MeshNode4 node;
+    for (int k = 0; k < 4; ++k) {
+        for (int i = 0; i < 3; ++i) {
+            node.lo[i][k] = INFINITY;
+            node.hi[i][k] = -INFINITY;
+        }
+        node.a[k] = 0;
+        node.b[k] = 0;
+    }
+    for (int k = 0; k < ns; ++k) {
+        const MeshNode& g = n2[slot[k]];
+        for (int i = 0; i < 3; ++i) {
+            node.lo[i][k] = g.lo[i];
+            node.hi[i][k] = g.hi[i];
+        }
+        if (g.b > 0) {
+            node.a[k] = collapse4(n2, slot[k], out);
+            node.b[k] = 1;
+        } else {
+            node.a[k] = g.a;
+            node.b[k] = g.b;
+        }
+    }
+    out[(size_t)me] = node;
+    return me;
+}
+
+// The four-wide tree of a binary BVH (root first); a leaf root becomes one
+// node with a single slot.
+std::vector<MeshNode4> collapse_bvh(const std::vector<MeshNode>& n2) {
+    std::vector<MeshNode4> out;
+    if (n2.empty()) return out;
+    if (n2[0].b > 0) {
+        collapse4(n2, 0, out);
+        return out;
+    }
+    MeshNode4 node;
+    for (int k = 0; k < 4; ++k) {
+        for (int i = 0; i < 3; ++i) {
+            node.lo[i][k] = k == 0 ? n2[0].lo[i] : INFINITY;
+            node.hi[i][k] = k == 0 ? n2[0].hi[i] : -INFINITY;
+        }
+        node.a[k] = k == 0 ? n2[0].a : 0;
+        node.b[k] = k == 0 ? n2[0].b : 0;
+    }
+    out.push_back(node);
+    return out;
+}
+
 uint32_t spread10(uint32_t v) {
     v &= 1023u;
     v = (v | (v << 16)) & 0x030000FFu;
@@ -173,7 +248,7 @@ uint32_t spread10(uint32_t v) {
 // device
 // ---------------------------------------------------------------------------
 struct MeshArgs {
-    const MeshNode* nodes;
+    const MeshNode4* nodes;
     const float4* tris;   // 4 float4 per triangle
     const float* robot;   // 9 floats per triangle
     int nnodes, nrobot;
@@ -224,33 +299,36 @@ __device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack
             }
         }
         const bool live = cnt > 0;
-        // Current node (ca, cb) = its children, or its leaf range [ca, ca - cb)
-        // when cb <= 0; a node is entered only when some lane's box overlaps
-        // it.  Both children of an internal node are loaded and tested
-        // together; the wave descends into the left one that overlaps and
-        // keeps the right one on the stack (at most one entry per level).
-        const MeshNode root = m.nodes[0];
-        if (__ballot(live && boxes_overlap(root.lo, root.hi, llo, lhi)) == 0) continue;
-        int ca = root.a, cb = root.b;
-        int sp = 0;  // wave-uniform
+        // Current entry (ca, cb): a four-wide node (cb > 0) or a leaf's
+        // triangles [ca, ca - cb); an entry is entered only when some lane's
+        // box overlaps its box.  The four slot boxes of a node arrive in one
+        // uniform load; the wave descends into the first slot that some lane
+        // overlaps and keeps the other such slots on the stack.
+        int ca = 0, cb = 1;
+        int sp = 0;  // wave-uniform; at most 3 entries per four-wide level
         for (;;) {
             if (cb > 0) {
-                const MeshNode L = m.nodes[ca], R = m.nodes[cb];
-                const bool gl = __ballot(live && boxes_overlap(L.lo, L.hi, llo, lhi)) != 0;
-                const bool gr = __ballot(live && boxes_overlap(R.lo, R.hi, llo, lhi)) != 0;
-                if (gl) {
-                    if (gr) {
-                        stack[2 * sp] = R.a;
-                        stack[2 * sp + 1] = R.b;
-                        ++sp;
-                    }
-                    ca = L.a;
-                    cb = L.b;
-                    continue;
+                const MeshNode4 N = m.nodes[ca];
+                uint32_t take = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float blo[3] = {N.lo[0][k], N.lo[1][k], N.lo[2][k]};
+                    const float bhi[3] = {N.hi[0][k], N.hi[1][k], N.hi[2][k]};
+                    take |= __ballot(live && boxes_overlap(blo, bhi, llo, lhi)) != 0 ? 1u << k : 0u;
                 }
-                if (gr) {
-                    ca = R.a;
-                    cb = R.b;
+                if (take != 0) {
+#pragma unroll
+                    for (int k = 3; k >= 1; --k) {
+                        // every slot taken but the first goes on the stack
+                        if (((take >> k) & 1u) && (take & ((1u << k) - 1u))) {
+                            stack[2 * sp] = N.a[k];
+                            stack[2 * sp + 1] = N.b[k];
+                            ++sp;
+                        }
+                    }
+                    const int f = __builtin_ctz(take);
+                    ca = f == 0 ? N.a[0] : f == 1 ? N.a[1] : f == 2 ? N.a[2] : N.a[3];
+                    cb = f == 0 ? N.b[0] : f == 1 ? N.b[1] : f == 2 ? N.b[2] : N.b[3];
                     continue;
                 }
             } else {
@@ -489,7 +567,7 @@ __global__ __launch_bounds__(256) void mesh_link_finish(SpaceArgs sp, double inv
 
 MeshArgs mesh_args(const mptg_ctx* ctx) {
     const SceneDev& s = ctx->scene;
-    return MeshArgs{(const MeshNode*)s.mesh_nodes.p, (const float4*)s.mesh_tris.p, (const float*)s.mesh_robot.p,
+    return MeshArgs{(const MeshNode4*)s.mesh_nodes.p, (const float4*)s.mesh_tris.p, (const float*)s.mesh_robot.p,
                     s.mesh_nnodes, s.mesh_nrobot};
 }
 
@@ -597,17 +675,18 @@ extern "C" int mptg_scene_mesh(mptg_ctx* ctx, const float* h_robot, int nrobot, 
     }
     SceneDev& s = ctx->scene;
     MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if ((rc = ensure(ctx, s.mesh_nodes, sizeof(MeshNode) * std::max<size_t>(1, bb.nodes.size())))) return rc;
+    const std::vector<MeshNode4> nodes4 = collapse_bvh(bb.nodes);
+    if ((rc = ensure(ctx, s.mesh_nodes, sizeof(MeshNode4) * std::max<size_t>(1, nodes4.size())))) return rc;
     if ((rc = ensure(ctx, s.mesh_tris, sizeof(float) * std::max<size_t>(16, tris.size())))) return rc;
     if ((rc = ensure(ctx, s.mesh_robot, sizeof(float) * std::max<size_t>(9, robot.size())))) return rc;
-    if (!bb.nodes.empty())
-        MPTG_HIP(ctx, hipMemcpy(s.mesh_nodes.p, bb.nodes.data(), sizeof(MeshNode) * bb.nodes.size(),
+    if (!nodes4.empty())
+        MPTG_HIP(ctx, hipMemcpy(s.mesh_nodes.p, nodes4.data(), sizeof(MeshNode4) * nodes4.size(),
                                 hipMemcpyHostToDevice));
     if (!tris.empty())
         MPTG_HIP(ctx, hipMemcpy(s.mesh_tris.p, tris.data(), sizeof(float) * tris.size(), hipMemcpyHostToDevice));
     if (!robot.empty())
         MPTG_HIP(ctx, hipMemcpy(s.mesh_robot.p, robot.data(), sizeof(float) * robot.size(), hipMemcpyHostToDevice));
-    s.mesh_nnodes = (int)bb.nodes.size();
+    s.mesh_nnodes = (int)nodes4.size();
     s.mesh_nrobot = nrobot;
     s.kind = MPTG_SCENE_MESH;
     ctx->valid_upto = 0;  // valid() changed: every node is checked again
