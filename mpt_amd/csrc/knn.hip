@@ -177,6 +177,24 @@ __device__ __forceinline__ bool knn_extent(const int64_t* n_dev, const int32_t* 
     return true;
 }
 
+// Splits of a byte-prefilter scan (k = 1) over n nodes for `live` queries,
+// chosen on the device because the live count is only known there: enough
+// chunks that the live query blocks times the splits fill `target` blocks,
+// no more than `planned` (the partial results' splits, the grid's y extent)
+// and at least kMinSplitNodes nodes per chunk.  Returns the splits used
+// (every chunk non-empty) and sets the chunk length.
+__device__ __forceinline__ int64_t q8_splits(int64_t n, int64_t live, int64_t planned, int64_t target,
+                                             int64_t* chunk) {
+    const int64_t qb = live > 0 ? (live + kBlock - 1) / kBlock : 1;
+    int64_t s = (target + qb - 1) / qb;
+    const int64_t max_s = (n + kMinSplitNodes - 1) / kMinSplitNodes;
+    s = s < max_s ? s : max_s;
+    s = s < planned ? s : planned;
+    s = s > 1 ? s : 1;
+    *chunk = n > 0 ? (n + s - 1) / s : 0;
+    return n > 0 ? (n + *chunk - 1) / *chunk : 1;
+}
+
 // Nearest of kSeedNodes tree nodes (every (n / kSeedNodes)-th) to each query
 // (exact FP64 distance): a starting bound for the k = 1 scan.  kSeedLanes
 // lanes share a query, each measuring every kSeedLanes-th of those nodes;
@@ -400,6 +418,7 @@ __global__ __launch_bounds__(kBlock) void knn_scan(const float* __restrict__ x32
 // cost occupancy: 3 and 2 waves per SIMD against 4.)
 constexpr int kQ8Tile = 256;
 constexpr double kQ8Slack = 1.001;
+constexpr int kQ8BlocksPerCu = 4;
 
 __device__ __forceinline__ uint32_t q8_byte(float x, float B, float inv_s) {
     float f = (x + B) * inv_s + 0.5f;
@@ -506,17 +525,28 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
                                                       int64_t nq, double radius, double bound,
                                                       double* __restrict__ out_d, int32_t* __restrict__ out_i,
                                                       const int64_t* n_dev, const int32_t* skip,
-                                                      uint32_t* __restrict__ gthr, const uint32_t* nq_dev) {
+                                                      uint32_t* __restrict__ gthr, const uint32_t* nq_dev,
+                                                      int64_t target) {
     constexpr int NW = (D + 3) / 4;  // words of a byte row
     // byte rows by word: word w of tile node j at t8[w * kQ8Tile + j]
     __shared__ __align__(16) uint32_t t8[NW * kQ8Tile];
-    if (!knn_extent(n_dev, skip, &n, &chunk)) return;  // block-uniform
+    (void)chunk;  // re-planned below from the live query count
+    if (skip && *skip) return;  // block-uniform
+    if (n_dev) n = *n_dev;
     const int64_t live = nq_dev ? (int64_t)*nq_dev : nq;
-    if ((int64_t)blockIdx.x * kBlock >= live) return;  // block-uniform
-    const int64_t qi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (live <= 0) return;
+    // the grid's blocks, taken in order, are (query block, split) pairs of
+    // the live queries: qblocks <= gridDim.x and splits <= gridDim.y, so
+    // every pair has a block
+    const int64_t qblocks = (live + kBlock - 1) / kBlock;
+    const int64_t splits = q8_splits(n, live, gridDim.y, target, &chunk);
+    const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    if (b >= qblocks * splits) return;  // block-uniform
+    const int64_t split = b / qblocks;
+    const int64_t qi = (b - split * qblocks) * kBlock + threadIdx.x;
     const bool active = qi < live;
     const int64_t qc = active ? qi : 0;
-    const int64_t p0 = (int64_t)blockIdx.y * chunk;
+    const int64_t p0 = split * chunk;
     const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
     const float Bf = (float)bound;
     const float inv_sf = (float)(255.0 / (2.0 * bound));
@@ -604,7 +634,7 @@ __global__ __launch_bounds__(kBlock) void knn_scan_q8(const float* __restrict__ 
     }
     flush();
     if (!active) return;
-    const int64_t base = (int64_t)blockIdx.y * nq + qi;
+    const int64_t base = split * nq + qi;
     out_d[base] = top.d[0];
     out_i[base] = top.i[0];
 }
@@ -678,14 +708,16 @@ constexpr int kMergeLanes = 8;
 __global__ __launch_bounds__(kBlock) void knn_merge1(const double* __restrict__ pd, const int32_t* __restrict__ pi,
                                                      int splits, int64_t nq, double* __restrict__ out_d,
                                                      int32_t* __restrict__ out_i, const int32_t* skip,
-                                                     const int64_t* n_dev, double inf, const uint32_t* nq_dev) {
+                                                     const int64_t* n_dev, double inf, const uint32_t* nq_dev,
+                                                     int64_t n, int64_t q8_target) {
     static_assert(64 % kMergeLanes == 0, "lane groups within a wave");
     if (skip && *skip) return;  // uniform: every thread reads the same flag
     const int64_t live = nq_dev ? (int64_t)*nq_dev : nq;  // nq stays the partial results' stride
-    if (n_dev) {
-        int64_t chunk;
+    int64_t chunk;
+    if (q8_target > 0)  // the byte-prefilter scan's own split of the live queries
+        splits = (int)q8_splits(n_dev ? *n_dev : n, live, splits, q8_target, &chunk);
+    else if (n_dev)
         splits = (int)knn_used_splits(*n_dev, splits, &chunk);
-    }
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t qi = t / kMergeLanes;
     const int part = (int)(t % kMergeLanes);
@@ -972,6 +1004,7 @@ struct ScanExtent {
     double* seed;            // k = 1: per-query starting bound (nullptr: none)
     const uint32_t* nq_dev;  // live queries on the device (nullptr: all nq)
     bool q8;                 // k = 1 under L1: the byte prefilter, its shared bounds in `seed`
+    int64_t q8_target;       // byte prefilter: blocks its device-side split aims at
 };
 
 template <int D, int METRIC, int K>
@@ -986,7 +1019,8 @@ void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q,
                                dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, ctx->coord_bound, gthr, x.n_dev,
                                x.skip, x.nq_dev);
             hipLaunchKernelGGL((knn_scan_q8<D>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n,
-                               x.chunk, q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip, gthr, x.nq_dev);
+                               x.chunk, q, nq, radius, ctx->coord_bound, pd, pi, x.n_dev, x.skip, gthr, x.nq_dev,
+                               x.q8_target);
             return;
         }
     }
@@ -1085,7 +1119,10 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     // k = 1 under L1 in 8 or more dimensions: the byte prefilter
     const bool q8 = seed && ctx->metric == MPTG_METRIC_L1 && ctx->dim >= 8 && ctx->tune.knn_q8 &&
                     ctx->coord_bound > 0 && ctx->coord_bound < 1e300;
-    const ScanExtent ext{n, chunk, n_dev, skip, seed, nq_dev, q8};
+    // the byte prefilter's blocks (4 waves, 4 resident per CU) re-split on
+    // the device to one full round for the live queries
+    const int64_t q8_target = q8 ? (int64_t)ctx->cus * kQ8BlocksPerCu : 0;
+    const ScanExtent ext{n, chunk, n_dev, skip, seed, nq_dev, q8, q8_target};
     double* pd = (double*)ctx->scratch[4].p;
     int32_t* pi = (int32_t*)ctx->scratch[5].p;
     dim3 grid((unsigned)((nq + kBlock - 1) / kBlock), (unsigned)splits);
@@ -1098,7 +1135,7 @@ static int knn_scan_path(mptg_ctx* ctx, const double* d_q, int64_t nq, int k, do
     MPTG_HIP(ctx, hipGetLastError());
     dim3 g1((unsigned)((nq + kBlock - 1) / kBlock));
     if (splits > 1) {
-        if (K == 1) hipLaunchKernelGGL(knn_merge1, dim3((unsigned)((nq * kMergeLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, d_dist, d_idx, skip, n_dev, std::numeric_limits<double>::infinity(), nq_dev);
+        if (K == 1) hipLaunchKernelGGL(knn_merge1, dim3((unsigned)((nq * kMergeLanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, d_dist, d_idx, skip, n_dev, std::numeric_limits<double>::infinity(), nq_dev, n, q8_target);
         else if (K == 20) hipLaunchKernelGGL((knn_merge<20>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
         else if (K == 32) hipLaunchKernelGGL((knn_merge<32>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
         else hipLaunchKernelGGL((knn_merge<64>), g1, dim3(kBlock), 0, ctx->stream, pd, pi, (int)splits, nq, k, d_dist, d_idx, skip, n_dev);
