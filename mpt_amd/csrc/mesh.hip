@@ -254,6 +254,35 @@ struct MeshArgs {
     int nnodes, nrobot;
 };
 
+// The scene's BVH nodes are written only at upload, so the walk reads them
+// through the constant address space: its wave-uniform node loads then become
+// scalar loads also in kernels that store elsewhere (mesh_link).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* scene_const(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
+// Four-wide node i, word by word (lo[3][4], hi[3][4], a[4], b[4]).
+__device__ __forceinline__ MeshNode4 load_node4(const MeshNode4* nodes, int i) {
+    const __attribute__((address_space(4))) uint32_t* w = scene_const((const uint32_t*)nodes) + 32 * (size_t)i;
+    MeshNode4 n;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            n.lo[k][j] = __uint_as_float(w[4 * k + j]);
+            n.hi[k][j] = __uint_as_float(w[12 + 4 * k + j]);
+        }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        n.a[j] = (int32_t)w[24 + j];
+        n.b[j] = (int32_t)w[28 + j];
+    }
+    return n;
+}
+
+// Triangle record i (16 floats: v0 v1 v2, box lo, box hi, one unused),
+// vector loads: as scalar loads the leaf tests ran slower in mesh_filter.
 __device__ __forceinline__ void load_env_tri(const float4* __restrict__ t, int i, float* v, float* lo, float* hi) {
     const float4 a = t[4 * i], b = t[4 * i + 1], c = t[4 * i + 2], d = t[4 * i + 3];
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
@@ -308,7 +337,7 @@ __device__ bool mesh_collides(const MeshArgs& m, const double* q, int32_t* stack
         int sp = 0;  // wave-uniform; at most 3 entries per four-wide level
         for (;;) {
             if (cb > 0) {
-                const MeshNode4 N = m.nodes[ca];
+                const MeshNode4 N = load_node4(m.nodes, ca);
                 uint32_t take = 0;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
