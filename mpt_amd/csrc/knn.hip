@@ -492,27 +492,46 @@ __device__ __forceinline__ uint32_t q8_thr(double d, double inv_s, int D) {
 }
 
 // gthr[q] = T(exact distance to the nearest of kSeedNodes spread tree nodes)
-// for every live query (knn_seed's sample; a wave per query, each lane
-// measuring kSeedNodes / 64 nodes).
-constexpr int kQ8SeedLanes = 64;
+// for every live query (knn_seed's sample).  A block serves kQ8SeedQueries
+// queries: it stages the sample's coordinates in LDS once (coordinate-major,
+// so the lanes of a query read consecutive words), and kQ8SeedLanes lanes
+// per query measure every kQ8SeedLanes-th sample node.  The minimum is exact
+// in any order, and each distance is exact_dist's left-to-right sum.  (A
+// wave per query reading the sample from L2 took 17 us for ~2 700 queries.)
+constexpr int kQ8SeedQueries = 16;
+constexpr int kQ8SeedLanes = kBlock / kQ8SeedQueries;
 template <int D>
 __global__ __launch_bounds__(kBlock) void knn_q8_seed(const double* __restrict__ x64, int64_t n,
                                                       const double* __restrict__ q, int64_t nq, double bound,
                                                       uint32_t* __restrict__ gthr, const int64_t* n_dev,
                                                       const int32_t* skip, const uint32_t* nq_dev) {
+    static_assert(64 % kQ8SeedLanes == 0, "lane groups within a wave");
+    __shared__ double xs[D * kSeedNodes];  // coordinate i of sample node j at xs[i * kSeedNodes + j]
     if (skip && *skip) return;
     if (nq_dev) nq = *nq_dev;
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t qi = t / kQ8SeedLanes;
-    const int part = (int)(t % kQ8SeedLanes);
+    if ((int64_t)blockIdx.x * kQ8SeedQueries >= nq) return;  // block-uniform
     if (n_dev) n = *n_dev;
-    const int64_t ns = n < kSeedNodes ? n : kSeedNodes;
+    const int ns = (int)(n < kSeedNodes ? n : kSeedNodes);
     const int64_t stride = ns > 0 ? n / ns : 1;
+    for (int e = threadIdx.x; e < ns * D; e += kBlock) {
+        const int j = e / D, i = e - j * D;
+        xs[i * kSeedNodes + j] = x64[(int64_t)j * stride * D + i];
+    }
+    __syncthreads();
+    const int64_t qi = (int64_t)blockIdx.x * kQ8SeedQueries + threadIdx.x / kQ8SeedLanes;
+    const int part = (int)(threadIdx.x % kQ8SeedLanes);
     double best = __builtin_inf();
     if (qi < nq) {
-#pragma unroll 4
-        for (int64_t j = part; j < ns; j += kQ8SeedLanes)
-            best = __builtin_fmin(best, exact_dist<D, MPTG_METRIC_L1>(q + qi * D, x64 + j * stride * D));
+        double qv[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) qv[i] = q[qi * D + i];
+#pragma unroll 2
+        for (int j = part; j < ns; j += kQ8SeedLanes) {
+            double x[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) x[i] = xs[i * kSeedNodes + j];
+            best = __builtin_fmin(best, exact_dist<D, MPTG_METRIC_L1>(qv, x));
+        }
     }
 #pragma unroll
     for (int o = 1; o < kQ8SeedLanes; o <<= 1) best = __builtin_fmin(best, __shfl_xor(best, o));
@@ -1015,7 +1034,7 @@ void launch_scan(mptg_ctx* ctx, dim3 grid, const ScanExtent& x, const double* q,
             // the seed buffer (nq doubles) holds the queries' shared byte
             // bounds instead
             uint32_t* gthr = reinterpret_cast<uint32_t*>(x.seed);
-            hipLaunchKernelGGL((knn_q8_seed<D>), dim3((unsigned)((nq * kQ8SeedLanes + kBlock - 1) / kBlock)),
+            hipLaunchKernelGGL((knn_q8_seed<D>), dim3((unsigned)((nq + kQ8SeedQueries - 1) / kQ8SeedQueries)),
                                dim3(kBlock), 0, ctx->stream, ctx->x64, x.n, q, nq, ctx->coord_bound, gthr, x.n_dev,
                                x.skip, x.nq_dev);
             hipLaunchKernelGGL((knn_scan_q8<D>), grid, dim3(kBlock), 0, ctx->stream, ctx->x32, ctx->x64, x.n,
