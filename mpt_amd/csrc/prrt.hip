@@ -322,15 +322,15 @@ int enqueue_iteration(mptg_ctx* ctx, const PrrtDev& pd, int64_t it, int64_t plan
     Counters* dcnt = (Counters*)pb.counters.p;
     const bool dev = it < 0;
     int rc;
+    if (pre && !dev && knn_grid_applies(ctx, ctx->size)) pre = false;
+    uint32_t* vcnt = (uint32_t*)pb.vcnt.p;  // cleared by prrt_sample when the pre path runs
     hipLaunchKernelGGL(prrt_sample, dim3(sample_blocks(ctx->metric, D, B)), dim3(kBlock), 0, ctx->stream, pd,
-                       ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)pb.q.p);
+                       ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)pb.q.p,
+                       pre ? vcnt : nullptr);
     MPTG_HIP(ctx, hipGetLastError());
     uint8_t* skip = nullptr;  // custom sampler: slots without a sample (eager iterations only)
     if (s.sampler && (rc = take_host_samples(ctx, it, (double*)pb.q.p, dcnt, &skip))) return rc;
-    if (pre && !dev && knn_grid_applies(ctx, ctx->size)) pre = false;
     if (pre) {
-        uint32_t* vcnt = (uint32_t*)pb.vcnt.p;
-        if ((rc = launch_zero_u32(ctx, vcnt, 1))) return rc;
         const bool arm8 = ctx->scene.kind == MPTG_SCENE_ARM && ctx->scene.nlinks == D && (D == 5 || D == 10 || D == 20);
         dispatch_dim(D, [&]<int DD>() {
             if constexpr (DD == 5 || DD == 10 || DD == 20) {

@@ -43,12 +43,15 @@ inline unsigned sample_blocks(int metric, int D, int B) {
 
 // it < 0: the iteration index is the device counter's (graph replays).
 // Thread t draws coordinate t % D of sample t / D (boxes; coalesced
-// stores), or the whole state for SO(3) x R^3.
+// stores), or the whole state for SO(3) x R^3.  zero_word (may be null): a
+// counter of the iteration's later kernels, cleared here so that it needs no
+// launch of its own.
 __global__ __launch_bounds__(kBlock) void prrt_sample(PrrtDev pd, int metric, int D, uint64_t seed, double goal_bias,
                                                       int64_t it, int B, const Counters* __restrict__ cnt,
-                                                      double* __restrict__ q) {
+                                                      double* __restrict__ q, uint32_t* __restrict__ zero_word) {
     const int per = sample_threads(metric, D);
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (zero_word && t == 0) *zero_word = 0u;
     if (t >= (int64_t)B * per || cnt->done) return;
     const int64_t b = t / per;
     if (it < 0) it = cnt->iteration;

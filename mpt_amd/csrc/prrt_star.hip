@@ -700,7 +700,8 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         const int64_t it = s.iteration;
         // ---- extend
         hipLaunchKernelGGL(prrt_sample, dim3(sample_blocks(ctx->metric, D, B)), dim3(kBlock), 0, ctx->stream, pd,
-                           ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)s.q.p);
+                           ctx->metric, D, s.p.seed, s.p.goal_bias, it, B, (const Counters*)dcnt, (double*)s.q.p,
+                           nullptr);
         MPTG_HIP(ctx, hipGetLastError());
         uint8_t* skip = nullptr;  // custom sampler: slots without a sample
         if (s.sampler && (rc = take_host_samples(ctx, it, (double*)s.q.p, dcnt, &skip))) return rc;
