@@ -13,13 +13,13 @@
 //             boxes (by axis) and references of up to four nodes two levels
 //             below it, so one uniform load serves two levels of the walk
 //   env tris  16 floats (64 B) per triangle in leaf order: v0 v1 v2, box lo, hi
-//   robot     9 floats per triangle, Morton-sorted by centroid so that the 8
+//   robot     9 floats per triangle, Morton-sorted by centroid so that the 4
 //             consecutive triangles a lane owns are close together
 // Kernels (one 64-lane wave per block, one item per wave at a time):
 //   mesh_valid  one state per wave
 //   mesh_link   one edge per wave; waypoints in DMV order one at a time
 //               (reference checks_done counts), early exit on the first hit
-// Per state, lane l poses robot triangles [8l, 8l + 8) of each 512-triangle
+// Per state, lane l poses robot triangles [4l, 4l + 4) of each 256-triangle
 // chunk and takes their box; the wave walks the env BVH depth-first once with
 // one LDS stack, visiting a node when any lane's box overlaps it, and stops
 // at the first leaf where some lane finds a hit.
@@ -36,7 +36,7 @@ namespace mptg {
 
 namespace {
 
-constexpr int kLaneTris = 8;                    // robot triangles per lane
+constexpr int kLaneTris = 4;                    // robot triangles per lane
 constexpr int kChunkTris = 64 * kLaneTris;      // robot triangles per wave pass
 constexpr int kBuildDepth = 38;                 // BVH depth bound
 // LDS stack of a wave's walk: up to three (a, b) entries per four-wide level
@@ -295,7 +295,7 @@ __device__ __forceinline__ void load_env_tri(const float4* __restrict__ t, int i
 // Does the robot at state q collide with the environment?  Whole wave; q is
 // the same on every lane.  stack: this wave's kStackSlots LDS slots.
 //
-// Per 512-triangle chunk, lane l poses robot triangles [8l, 8l + 8) and takes
+// Per 256-triangle chunk, lane l poses robot triangles [4l, 4l + 4) and takes
 // their box; the wave walks the environment BVH depth-first once, visiting a
 // node when any lane's box overlaps it (a ballot), so node and triangle loads
 // are wave-uniform and no lane diverges in the walk.  At a leaf each lane
