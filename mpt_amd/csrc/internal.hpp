@@ -94,7 +94,7 @@ struct PrrtState {
     // neighbour lists: k-nearest rows of star_k slots, or radius lists in
     // CSR form (nbh_off per survivor, nbh_own = survivor of each entry)
     DevBuf dnear, sidx, qs, nbh_idx, nbh_d, nbh_off, nbh_own, marked, pc, cpar, cp_from, cp_to, cp_ver, cp_chk,
-        pair_cp, rw_from, rw_to, rw_ver, rw_chk, pair_rw, front, star_cnt;
+        pair_cp, rw_from, rw_to, rw_ver, rw_chk, pair_rw, front;
 };
 
 }  // namespace mptg

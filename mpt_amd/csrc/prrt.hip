@@ -202,7 +202,7 @@ int prrt_reserve(mptg_ctx* ctx, int B) {
     if ((rc = ensure(ctx, pb.mask, B))) return rc;
     if ((rc = ensure(ctx, pb.verdict, B))) return rc;
     if ((rc = ensure(ctx, pb.checks, sizeof(uint32_t) * B))) return rc;
-    if ((rc = ensure(ctx, pb.counters, sizeof(Counters)))) return rc;
+    if ((rc = ensure(ctx, pb.counters, kCounterBlock))) return rc;
     if ((rc = ensure(ctx, pb.goal_buf, sizeof(int32_t) * 4096))) return rc;
     if ((rc = ensure(ctx, pb.blk, sizeof(int32_t) * 2 * ((B + kAppendBlock - 1) / kAppendBlock)))) return rc;
     if ((rc = ensure(ctx, pb.qv, sizeof(double) * D * B))) return rc;
@@ -261,7 +261,7 @@ void prrt_free(mptg_ctx* ctx) {
                      &pb.verdict, &pb.checks,  &pb.counters, &pb.goal_buf, &pb.dnear, &pb.sidx,   &pb.qs,
                      &pb.nbh_idx, &pb.nbh_d,   &pb.marked, &pb.pc,      &pb.cpar,     &pb.cp_from, &pb.cp_to,
                      &pb.cp_ver,  &pb.cp_chk,  &pb.pair_cp, &pb.rw_from, &pb.rw_to,    &pb.rw_ver,  &pb.rw_chk,
-                     &pb.pair_rw, &pb.front,   &pb.star_cnt, &pb.blk,     &pb.qv,      &pb.vidx,
+                     &pb.pair_rw, &pb.front,   &pb.blk,     &pb.qv,      &pb.vidx,
                      &pb.vcnt,    &pb.kn_i,    &pb.kn_d,   &pb.vcheck, &pb.nbh_off, &pb.nbh_own, &pb.hs_q,
                      &pb.hs_ok,   &pb.hs_skip};
     for (DevBuf* b : all) {

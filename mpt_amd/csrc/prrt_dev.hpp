@@ -30,6 +30,12 @@ struct Counters {
     int64_t size_seen, goals_seen;
 };
 
+// The counters' device block: Counters at 0, PRRT*'s StarCounters at
+// kStarCounterOffset, so that one copy brings both to the host.
+constexpr size_t kStarCounterOffset = 256;
+constexpr size_t kCounterBlock = 512;
+static_assert(sizeof(Counters) <= kStarCounterOffset, "Counters fit below the PRRT* counters");
+
 struct PrrtDev {
     double lo[kMaxDim], hi[kMaxDim], goal[kMaxDim];
 };

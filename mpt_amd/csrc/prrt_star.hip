@@ -55,6 +55,7 @@ struct StarCounters {
                                     // cost mismatch, 2 corrupt child lists
     int64_t rewire_tests, rewires, parent_changes, cp_links, rw_links, deferred, rw_late;
 };
+static_assert(kStarCounterOffset + sizeof(StarCounters) <= kCounterBlock, "PRRT* counters fit the block");
 
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
 
@@ -619,7 +620,6 @@ static int star_reserve(mptg_ctx* ctx, int B) {
     if ((rc = ensure(ctx, pb.qs, sizeof(double) * D * B))) return rc;
     if ((rc = ensure(ctx, pb.pc, sizeof(double) * B))) return rc;
     if ((rc = ensure(ctx, pb.cpar, sizeof(int32_t) * B))) return rc;
-    if ((rc = ensure(ctx, pb.star_cnt, sizeof(StarCounters)))) return rc;
     return MPTG_OK;
 }
 
@@ -657,7 +657,7 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
     if ((rc = prrt_reserve(ctx, B))) return rc;
     if ((rc = star_reserve(ctx, B))) return rc;
     Counters* dcnt = (Counters*)s.counters.p;
-    StarCounters* dsc = (StarCounters*)s.star_cnt.p;
+    StarCounters* dsc = (StarCounters*)((char*)s.counters.p + kStarCounterOffset);
     if (!s.counters_valid) {
         Counters c{};
         c.size = ctx->size;
@@ -669,12 +669,12 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         s.counters_valid = true;
         s.stats_base = s.stats;
     }
-    if ((rc = ensure_pinned(ctx, sizeof(Counters) + sizeof(StarCounters)))) return rc;
+    if ((rc = ensure_pinned(ctx, kStarCounterOffset + sizeof(StarCounters)))) return rc;
     Counters* hc = (Counters*)ctx->pinned;
-    StarCounters* hs = (StarCounters*)((char*)ctx->pinned + sizeof(Counters));
-    auto read_counters = [&]() -> int {
-        MPTG_HIP(ctx, hipMemcpyAsync(hc, dcnt, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
-        MPTG_HIP(ctx, hipMemcpyAsync(hs, dsc, sizeof(StarCounters), hipMemcpyDeviceToHost, ctx->stream));
+    StarCounters* hs = (StarCounters*)((char*)ctx->pinned + kStarCounterOffset);
+    auto read_counters = [&]() -> int {  // both counter sets in one copy
+        MPTG_HIP(ctx, hipMemcpyAsync(hc, dcnt, kStarCounterOffset + sizeof(StarCounters), hipMemcpyDeviceToHost,
+                                     ctx->stream));
         MPTG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         return MPTG_OK;
     };
