@@ -131,10 +131,12 @@ __global__ __launch_bounds__(kBlock) void prrt_steer(SpaceArgs sp, SceneArgs sc,
     if (b >= B) return;
     if (cnt->done) {  // nothing to link in an iteration past the stop
         linkmask[b] = 0;
+        if (dnear) dnear[b] = ndist[b];
         return;
     }
     double d = ndist[b];
     int32_t nn = near[b];
+    if (dnear) dnear[b] = d;  // replaced below when the step is cut to range
     linkmask[b] = 0;
     if (d == 0 || nn < 0) {
         status[b] = ST_ZERO;
@@ -196,10 +198,14 @@ __global__ __launch_bounds__(kBlock) void prrt_steer_arm(SpaceArgs sp, SceneArgs
 #pragma unroll
     for (int i = 0; i < D; ++i) s[i] = 0.0;
     if (inb && cnt->done) {
-        if (lead) linkmask[b] = 0;
+        if (lead) {
+            linkmask[b] = 0;
+            if (dnear) dnear[b] = ndist[b];
+        }
     } else if (inb) {
         const double d = ndist[b];
         const int32_t nn = near[b];
+        if (dnear && lead) dnear[b] = d;  // replaced below when the step is cut to range
         if (d == 0 || nn < 0) {
             if (lead) {
                 linkmask[b] = 0;

@@ -708,8 +708,8 @@ int prrt_star_run(mptg_ctx* ctx, int64_t target_size, int64_t max_iters) {
         if ((rc = launch_knn(ctx, (const double*)s.q.p, B, 1, __builtin_inf(), (int32_t*)s.near.p,
                              (double*)s.ndist.p, n0)))
             return rc;
-        MPTG_HIP(ctx, hipMemcpyAsync(s.dnear.p, s.ndist.p, sizeof(double) * B, hipMemcpyDeviceToDevice,
-                                     ctx->stream));
+        // the steer kernel writes dnear: d(near, new), the kNN distance unless
+        // the step was cut to the range
         dispatch_dim(D, [&]<int DD>() {
             launch_steer<DD>(ctx, s.p.range, B, (const double*)s.q.p, (const int32_t*)s.near.p,
                              (const double*)s.ndist.p, (double*)s.from.p, (double*)s.to.p, (uint8_t*)s.status.p,
